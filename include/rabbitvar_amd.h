@@ -1,0 +1,169 @@
+/*
+ * rabbitvar_amd: MI355X (gfx950) implementation of RabbitVar's per-region calling path
+ * (record filters -> CIGAR rewrite -> pileup -> realignment -> per-variant scoring -> simple-mode
+ * text), behind a plain C ABI.
+ *
+ * Boundary.  The reference has no plugin or FFI interface for this path: its region loop calls
+ *   Scope<AlignedVarsData>* SimpleMode::one_region_run(Region, Configuration*, dataPool*,
+ *                                                     vector<bamReader>, set<string>* splice)
+ * (reference src/modes/simpleMode.cpp:18, declared include/modes/simpleMode.h:425), followed by
+ * SimpleMode::output (src/modes/simpleMode.cpp:144-208) which prints the call lines.  The entry
+ * points below replace that pair for one region: the caller hands over the region iterator's
+ * records (decoded, structure of arrays), the reference window, and the configuration; it gets
+ * back the region's call set, including the exact text the reference prints.
+ *
+ * Ownership.  The caller owns every input; the engine owns device buffers; a call set belongs to
+ * the caller until rv_callset_free.  An engine is driven by one host thread at a time.
+ */
+#ifndef RABBITVAR_AMD_H
+#define RABBITVAR_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum rv_status {
+    RV_OK = 0,
+    RV_ERR_ARG = 1,          /* invalid argument */
+    RV_ERR_DEVICE = 2,       /* HIP runtime / kernel failure */
+    RV_ERR_OOM = 3,          /* device or host allocation failed */
+    RV_ERR_UNSUPPORTED = 4,  /* option or input the engine does not implement */
+    RV_ERR_CAPACITY = 5      /* an internal bounded buffer overflowed */
+} rv_status;
+
+/* Configuration fields on the calling path; defaults and meaning as the reference's
+ * Configuration (include/Configuration.h:98-401) and option table (src/Launcher.cpp:299-494). */
+typedef struct rv_config {
+    double goodq;        /* -q, 22.5 */
+    double bias;         /* 0.05 */
+    double freq;         /* -f, 0.01 */
+    double qratio;       /* -o, 1.5 */
+    double mapq;         /* -O, 0 */
+    double lofreq;       /* -V, 0.05 */
+    int vext;            /* -X, 2 */
+    int mismatch;        /* -m, 8 */
+    int trimBasesAfter;  /* -T, 0 */
+    int performLocalRealignment; /* -k, 1 */
+    int indelsize;       /* -I, 50 */
+    int minBiasReads;    /* -B, 2 */
+    int minr;            /* -r, 2 */
+    int moveIndelsTo3;   /* -3 */
+    int samfilter;       /* -F, 0x504 */
+    int readPosFilter;   /* -P, 5 */
+    int doPileup;        /* -p (the caller also sets freq=-1, minr=0 as the reference does) */
+    int minmatch;        /* -M, 0 */
+    int uniqueModeAlignmentEnabled;    /* -u */
+    int uniqueModeSecondInPairEnabled; /* --UN */
+    int fisher;          /* --fisher */
+    int SVMINLEN;        /* -L, 1000 */
+    int referenceExtension;       /* -Y, 1200 */
+    int numberNucleotideToExtend; /* -x, 0 */
+    int mappingQuality;  /* -Q, 0 */
+    int removeDuplicatedReads;    /* -t */
+    int crisprCuttingSite;        /* -J, 0 */
+    const char* sample;  /* -N */
+} rv_config;
+
+/* Records of one region as the region iterator returns them (unfiltered, in order).  Bases are
+ * decoded ASCII (seq_nt16_str), qualities are Phred, CIGAR words use the BAM encoding.
+ * seq_off / cigar_off have n_reads + 1 entries.  With on_device != 0 every pointer is a device
+ * pointer on the engine's device and stays resident; otherwise they are host pointers. */
+typedef struct rv_reads {
+    int64_t n_reads;
+    const int32_t* pos0;
+    const uint16_t* flag;
+    const uint8_t* mapq;
+    const uint8_t* has_nm;
+    const int32_t* nm;
+    const int32_t* mpos0;
+    const int32_t* tid;
+    const int32_t* mtid;
+    const int64_t* seq_off;
+    const int64_t* cigar_off;
+    const uint8_t* bases;
+    const uint8_t* quals;
+    const uint32_t* cigar;
+    int on_device;
+} rv_reads;
+
+/* Region (1-based, inclusive) and its gene label */
+typedef struct rv_region {
+    const char* chr;
+    int32_t start;
+    int32_t end;
+    const char* gene;
+} rv_region;
+
+/* The contig the region lies on: the engine cuts the reference window exactly as
+ * RecordPreprocessor::makeReference (src/recordPreprocessor.cpp:41-78). Host memory. */
+typedef struct rv_contig {
+    const char* name;
+    int32_t length;
+    const char* bases; /* length bytes, any case */
+} rv_contig;
+
+typedef struct rv_engine rv_engine;
+typedef struct rv_callset rv_callset;
+
+/* One printed call (fields of Variant that SimpleMode prints; include/Variant.h:20-67) */
+typedef struct rv_call {
+    int32_t position;       /* the alignedVariants key */
+    int32_t start, end;
+    int32_t total_pos_coverage, position_coverage;
+    int32_t ref_fwd, ref_rev, var_fwd, var_rev;
+    int32_t shift3, msint, hicnt, hicov;
+    double frequency, mean_position, mean_quality, mean_mapq, hi_lo_ratio, hi_freq, extra_freq, msi, nm;
+    double fisher_p, fisher_ratio;
+    int32_t pstd, qstd;
+    const char* refallele;
+    const char* varallele;
+    const char* genotype;
+    const char* bias;
+    const char* leftseq;
+    const char* rightseq;
+    const char* vartype;
+} rv_call;
+
+/* Per-stage device time of the last rv_run_region (milliseconds, HIP events on the engine stream) */
+typedef struct rv_stats {
+    double prep_ms, walk_ms, anchor_ms, score_ms, total_device_ms;
+    double host_ms;          /* host-side reduction, realignment and formatting */
+    int64_t reads_in, reads_walked;
+    int64_t events, clips;
+    int64_t ref_key_creations; /* reads that touched a reference key outside the window */
+    int64_t walk_errors;
+} rv_stats;
+
+void rv_config_default(rv_config* cfg);
+
+/* Opens an engine on HIP device `device`. */
+rv_status rv_open(const rv_config* cfg, int device, rv_engine** out);
+
+/* Runs the calling path for one region (replaces one_region_run + output). */
+rv_status rv_run_region(rv_engine* e, const rv_region* region, const rv_reads* reads, const rv_contig* contig,
+                        rv_callset** out);
+
+size_t rv_callset_count(const rv_callset* cs);
+const rv_call* rv_callset_get(const rv_callset* cs, size_t i);
+/* The region's output lines exactly as the reference's simple mode prints them. */
+const char* rv_callset_text(const rv_callset* cs, size_t* len);
+void rv_callset_free(rv_callset* cs);
+
+rv_status rv_get_stats(const rv_engine* e, rv_stats* out);
+const char* rv_last_error(const rv_engine* e);
+void rv_close(rv_engine* e);
+
+/* Diagnostics: the pileup-stage state (after the walk, before realignment) as text, one record per
+ * line, in the layout tests compare against.  Free with rv_free. */
+rv_status rv_debug_pileup(rv_engine* e, const rv_region* region, const rv_reads* reads, const rv_contig* contig,
+                          char** text, size_t* len);
+void rv_free(void* p);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RABBITVAR_AMD_H */

@@ -1,0 +1,946 @@
+// ORACLE / TEST INFRASTRUCTURE ONLY.
+//
+// Realignment stage restated from src/VariationRealigner.cpp:
+//   process / stage order               :135-161, :467-482
+//   adjustMNP                           :334-465
+//   realigndel                          :489-778
+//   realignins                          :785-1114
+//   realignlgins30                      :1119-1354
+//   fillAndSortTmp / CMP_tmp / COMP3    :1362-1386, :44-58, :29-34
+//   find35match                         :1396-1428
+//   noPassingReads                      :1447-1498
+//   ismatch / ismatchref                :1507-1539, :1987-2024
+//   adjRefCnt / adjRefFactor            :1753-1822
+//   findMM5 / findMM3                   :1857-1976
+// and include/VariationUtils.h (adjCnt :291-332, correctCnt :263-280, findconseq :403-509,
+// islowcomplexseq :367-394, joinRef :336-349), include/util.h (vc_substr :27-57, charAt :171-179).
+#include <algorithm>
+#include <cmath>
+#include <regex>
+#include <stdexcept>
+
+#include "rv_oracle.h"
+
+namespace orc {
+
+// ---------------------------------------------------------------- string helpers (util.h)
+std::string vc_substr1(const std::string& s, int idx) {
+    if (idx >= 0) return s.substr((size_t)idx);
+    // (str.length() + idx < 0) is an unsigned comparison in the reference: never true
+    return s.substr((size_t)(s.size() + (size_t)(int64_t)idx));
+}
+std::string vc_substr2(const std::string& s, int begin, int len) {
+    if (begin < 0) begin = (int)(s.size() + (size_t)(int64_t)begin);
+    if (len > 0) return s.substr((size_t)begin, (size_t)len);
+    if (len == 0) return "";
+    len = (int)(s.size() + (size_t)(int64_t)len - (size_t)(int64_t)begin);
+    if (len < 0) return "";
+    return s.substr((size_t)begin, (size_t)len);
+}
+char charAt(const std::string& s, int index) {
+    if (index < 0) {
+        int i = (int)s.size() + index;
+        if (i < 0) return (char)-1;
+        return s[(size_t)i];
+    }
+    return s[(size_t)index];  // index == size() yields '\0'
+}
+std::string joinRef(const RefMap& ref, int from, int to) {
+    std::string r;
+    for (int i = from; i <= to; i++) r += ref.at(i);
+    return r;
+}
+std::string joinRefD(const RefMap& ref, int from, double to) {
+    std::string r;
+    for (int i = from; i < to; i++) r += ref.at(i);
+    return r;
+}
+void replaceFirst(std::string& s, const std::string& a, const std::string& b) {
+    size_t p = s.find(a);
+    if (p != std::string::npos) s.replace(p, a.size(), b);
+}
+void replaceFirstRe(std::string& s, const std::string& pat, const std::string& b) {
+    s = std::regex_replace(s, std::regex(pat), b, std::regex_constants::format_first_only);
+}
+
+// ---------------------------------------------------------------- count transfer
+void correctCnt(Var* v) {
+    if (v->cnt < 0) v->cnt = 0;
+    if (v->hicnt < 0) v->hicnt = 0;
+    if (v->locnt < 0) v->locnt = 0;
+    if (v->pmean < 0) v->pmean = 0;
+    if (v->qmean < 0) v->qmean = 0;
+    if (v->mqmean < 0) v->mqmean = 0;
+    if (v->rev < 0) v->rev = 0;
+    if (v->fwd < 0) v->fwd = 0;
+}
+void adjCnt(Var* to, const Var* v, Var* refv) {
+    to->cnt += v->cnt;
+    to->extracnt += v->cnt;
+    to->hicnt += v->hicnt;
+    to->locnt += v->locnt;
+    to->pmean += v->pmean;
+    to->qmean += v->qmean;
+    to->mqmean += v->mqmean;
+    to->nm += v->nm;
+    to->pstd = true;
+    to->qstd = true;
+    to->rev += v->rev;
+    to->fwd += v->fwd;
+    if (!refv) return;
+    refv->cnt -= v->cnt;
+    refv->hicnt -= v->hicnt;
+    refv->locnt -= v->locnt;
+    refv->pmean -= v->pmean;
+    refv->qmean -= v->qmean;
+    refv->mqmean -= v->mqmean;
+    refv->nm -= v->nm;
+    refv->rev -= v->rev;
+    refv->fwd -= v->fwd;
+    correctCnt(refv);
+}
+static void adjRefCnt(const Var* tv, Var* refv, int len) {
+    if (!refv) return;
+    double f = tv->pmean != 0 ? (tv->pmean / (double)tv->cnt - len + 1) / (tv->pmean / (double)tv->cnt) : 0;
+    if (f < 0) return;
+    if (f > 1) f = 1;
+    refv->cnt -= (int)(f * tv->cnt);
+    refv->hicnt -= (int)(f * tv->hicnt);
+    refv->locnt -= (int)(f * tv->locnt);
+    refv->pmean -= f * tv->pmean;
+    refv->qmean -= f * tv->qmean;
+    refv->mqmean -= f * tv->mqmean;
+    refv->nm -= f * tv->nm;
+    refv->rev -= (int)(f * tv->rev);
+    refv->fwd -= (int)(f * tv->fwd);
+    correctCnt(refv);
+}
+static void adjRefFactor(Var* refv, double f) {
+    if (!refv) return;
+    if (f > 1) f = 1;
+    if (f < -1) return;
+    int old = refv->cnt;
+    refv->cnt -= (int)(f * refv->cnt);
+    refv->hicnt -= (int)(f * refv->hicnt);
+    refv->locnt -= (int)(f * refv->locnt);
+    double fc = old != 0 ? std::abs(refv->cnt - old) / (double)old : 1;
+    refv->pmean -= refv->pmean * f * fc;
+    refv->qmean -= refv->qmean * f * fc;
+    refv->mqmean -= refv->mqmean * f * fc;
+    refv->nm -= f * refv->nm;
+    refv->fwd -= (int)(f * refv->fwd);
+    refv->rev -= (int)(f * refv->rev);
+    correctCnt(refv);
+}
+
+// ---------------------------------------------------------------- soft-clip consensus
+bool islowcomplexseq(const std::string& s) {
+    int len = (int)s.size();
+    if (len == 0) return true;
+    int ntcnt = 0;
+    const char order[4] = {'A', 'T', 'G', 'C'};
+    for (char b : order) {
+        int n = (int)std::count(s.begin(), s.end(), b);
+        if (n > 0) ntcnt++;
+        if (n / (double)len > 0.75) return true;
+    }
+    return ntcnt < 3;
+}
+
+std::string findconseq(Sclip* sc) {
+    if (!sc->sequence.empty()) return sc->sequence;
+    int total = 0, match = 0;
+    std::string seqq;
+    bool flag = false;
+    for (auto& e : sc->nt) {
+        int idx = e.first;
+        RHMap<char, int> nv = e.second;  // copy keeps the slot layout
+        int maxCount = 0;
+        double maxQuality = 0;
+        char chosen = 0;
+        int tt = 0;
+        nv.for_each([&](char base, int count) {
+            tt += count;
+            double bq = 0;
+            bool has = false;
+            auto si = sc->seq.find(idx);
+            if (si != sc->seq.end()) {
+                auto bj = si->second.find(base);
+                if (bj != si->second.end()) {
+                    has = true;
+                    bq = bj->second->qmean;
+                }
+            }
+            if (count > maxCount || (has && bq > maxQuality)) {
+                maxCount = count;
+                chosen = base;
+                maxQuality = sc->seq[idx][base]->qmean;
+            }
+        });
+        if (idx == 3 && sc->nt.size() >= 6 && tt / (double)sc->cnt < 0.2 && tt <= 2) break;
+        if ((tt - maxCount > 2 || maxCount <= tt - maxCount) && maxCount / (double)tt < 0.8) {
+            if (flag) break;
+            flag = true;
+        }
+        total += tt;
+        match += maxCount;
+        if (chosen != 0) seqq += chosen;
+    }
+    std::string SEQ;
+    size_t ntSize = sc->nt.size();
+    size_t sl = seqq.size();
+    if (total != 0 && match / (double)total > 0.9 && sl / 1.5 > (double)(ntSize - sl) &&
+        (sl / (double)ntSize > 0.8 || ntSize - sl < 10 || sl > 25)) {
+        SEQ = seqq;
+    } else {
+        SEQ = " ";
+    }
+    if (!SEQ.empty() && SEQ.size() > 12) {
+        bool mm1 = SEQ.size() >= 8 && SEQ.compare(1, 7, "AAAAAAA") == 0;
+        bool mm2 = SEQ.size() >= 8 && SEQ.compare(1, 7, "TTTTTTT") == 0;
+        if (mm1 || mm2) sc->used = true;
+        if (islowcomplexseq(SEQ)) sc->used = true;
+    }
+    sc->sequence = SEQ;
+    return SEQ;
+}
+
+namespace {
+
+struct SPD {
+    int position;
+    std::string desc;
+    int count;
+};
+// CMP_tmp: count desc, position asc, descriptor desc
+std::vector<SPD> sorted_tmp(const std::map<int, std::map<std::string, int>>& m) {
+    std::vector<SPD> t;
+    for (auto& e : m)
+        for (auto& v : e.second) t.push_back({e.first, v.first, v.second});
+    std::sort(t.begin(), t.end(), [](const SPD& a, const SPD& b) {
+        if (a.count != b.count) return a.count > b.count;
+        if (a.position != b.position) return a.position < b.position;
+        return a.desc.compare(b.desc) > 0;
+    });
+    return t;
+}
+
+struct Mismatch {
+    std::string seq;
+    int pos;
+    int end;
+};
+struct MMResult {
+    std::vector<Mismatch> mm;
+    std::vector<int> scp;
+    int nm = 0;
+    int misp = 0;
+    std::string misnt;
+};
+
+struct Realigner {
+    const Config& c;
+    const Region& R;
+    RefMap& ref;
+    int chrLen;
+    Pileup& P;
+    const std::vector<const Read*>& allReads;  // the region iterator's unfiltered records
+    bool bamsPresent = true;
+
+    bool ismatch(const std::string& s1, std::string s2, int dir, int MM = 3) {
+        s2.erase(std::remove_if(s2.begin(), s2.end(), [](char ch) { return ch == '#' || ch == '^'; }), s2.end());
+        int mm = 0;
+        for (size_t n = 0; n < s1.size() && n < s2.size(); n++) {
+            int b = dir * (int)n - (dir == -1 ? 1 : 0);
+            if (s1[n] != vc_substr2(s2, b, 1)[0]) mm++;
+        }
+        return mm <= MM && mm / (double)s1.size() < 0.15;
+    }
+    bool ismatchref(const std::string& s, RefMap& rf, int position, int dir, int MM = 3) {
+        int mm = 0;
+        for (int n = 0; n < (int)s.size(); n++) {
+            if (!rf.count(position + dir * n)) return false;
+            if (charAt(s, dir == 1 ? n : dir * n - 1) != rf.get(position + dir * n)) mm++;
+        }
+        return mm <= MM && mm / (double)s.size() < 0.15;
+    }
+
+    MMResult findMM5(RefMap& rf, int position, const std::string& wupseq) {
+        std::string seq = wupseq;
+        seq.erase(std::remove_if(seq.begin(), seq.end(), [](char ch) { return ch == '#' || ch == '^'; }), seq.end());
+        MMResult r;
+        int n = 0, mn = 0, mcnt = 0;
+        std::string str;
+        auto hasNe = [&](char ch, int k) { return rf.count(k) && rf.at(k) != ch; };
+        auto hasEq = [&](char ch, int k) { return rf.count(k) && rf.at(k) == ch; };
+        while (hasNe(charAt(seq, -1 - n), position - n) && mcnt < 3) {
+            str.insert(str.begin(), charAt(seq, -1 - n));
+            r.mm.push_back({str, position - n, 5});
+            n++;
+            mcnt++;
+        }
+        r.scp.push_back(position + 1);
+        char misnt = 0;
+        if (str.size() == 1) {
+            while (hasEq(charAt(seq, -1 - n), position - n)) {
+                n++;
+                if (n != 0) mn++;
+            }
+            if (mn > 1) {
+                int n2 = 0;
+                while (-1 - n - 1 - n2 >= 0 && hasEq(charAt(seq, -1 - n - 1 - n2), position - n - 1 - n2)) n2++;
+                if (n2 > 2) {
+                    r.scp.push_back(position - n - n2);
+                    r.misp = position - n;
+                    misnt = charAt(seq, -1 - n);
+                    if (Sclip** s = P.sc5.find(position - n - n2)) (*s)->used = true;
+                    mn += n2;
+                } else {
+                    r.scp.push_back(position - n);
+                    if (Sclip** s = P.sc5.find(position - n)) (*s)->used = true;
+                }
+            }
+        }
+        r.nm = mn;
+        r.misnt = misnt == 0 ? "" : std::string(1, misnt);
+        return r;
+    }
+
+    MMResult findMM3(RefMap& rf, int p, const std::string& sanpseq) {
+        std::string seq = sanpseq;
+        seq.erase(std::remove_if(seq.begin(), seq.end(), [](char ch) { return ch == '#' || ch == '^'; }), seq.end());
+        MMResult r;
+        int n = 0, mn = 0, mcnt = 0;
+        std::string str;
+        while (n < (int)seq.size() && rf.count(p + n) && rf.get(p + n) == seq[(size_t)n]) n++;
+        r.scp.push_back(p + n);
+        int Tbp = p + n;
+        while (mcnt <= 3 && n < (int)seq.size() && rf.get(p + n) != seq[(size_t)n]) {
+            str += seq[(size_t)n];
+            r.mm.push_back({str, Tbp, 3});
+            n++;
+            mcnt++;
+        }
+        char misnt = 0;
+        auto hasEq = [&](char ch, int k) { return rf.count(k) && rf.at(k) == ch; };
+        if (str.size() == 1) {
+            while (n < (int)seq.size() && hasEq(seq[(size_t)n], p + n)) {
+                n++;
+                if (n != 0) mn++;
+            }
+            if (mn > 1) {
+                int n2 = 0;
+                while (n + n2 + 1 < (int)seq.size() && hasEq(seq[(size_t)(n + n2 + 1)], p + n + 1 + n2)) n2++;
+                if (n2 > 2 && n + n2 + 1 < (int)seq.size()) {
+                    r.scp.push_back(p + n + n2);
+                    r.misp = p + n;
+                    misnt = seq[(size_t)n];
+                    if (Sclip** s = P.sc3.find(p + n + n2)) (*s)->used = true;
+                    mn += n2;
+                } else {
+                    r.scp.push_back(p + n);
+                    if (Sclip** s = P.sc3.find(p + n)) (*s)->used = true;
+                }
+            }
+        }
+        r.nm = mn;
+        r.misnt = misnt == 0 ? "" : std::string(1, misnt);
+        return r;
+    }
+
+    Var* inner(int pos, const std::string& d) {
+        VarMap** m = P.nonIns.find(pos);
+        if (!m) return nullptr;
+        auto it = (*m)->find(d);
+        return it == (*m)->end() ? nullptr : it->second;
+    }
+
+    // noPassingReads: counts over every record of the BAM region, unfiltered
+    bool noPassingReads(int start, int end) {
+        int cnt = 0;
+        uint32_t dq = ((uint32_t)(end - start) << 4) | OP_D;
+        for (const Read* r : allReads) {
+            int reflenAll = 0;
+            for (uint32_t w : r->cigar) {
+                int o = cop(w);
+                if (o == OP_M || o == OP_D || o == OP_N || o == OP_EQ || o == OP_X) reflenAll += clen(w);
+            }
+            int beg0 = r->pos0, end0 = r->pos0 + (reflenAll > 0 ? reflenAll : 1);
+            if (!(beg0 < end && end0 > start - 1)) continue;  // iterator overlap with [start-1, end)
+            bool has = false;
+            for (uint32_t w : r->cigar)
+                if (w == dq) has = true;
+            if (has) continue;
+            int rs = r->pos0 + 1;
+            int al = 0;
+            for (uint32_t w : r->cigar)
+                if (cop(w) == OP_M || cop(w) == OP_D) al += clen(w);
+            int re = rs + al;
+            if (re > end + 2 && rs < start - 2) cnt++;
+        }
+        return cnt <= 0;
+    }
+
+    void adjustMNP() {
+        std::vector<SPD> tmp = sorted_tmp(P.mnp);
+        for (const SPD& t : tmp) {
+            try {
+                int position = t.position;
+                const std::string vn = t.desc;
+                VarMap** vmp = P.nonIns.find(position);
+                if (!vmp) continue;
+                VarMap copy = **vmp;
+                if (!copy.count(vn)) continue;
+                Var* vref = copy.at(vn);
+                std::string mnt = vn;
+                replaceFirst(mnt, "&", "");
+                for (int i = 0; i < (int)mnt.size() - 1; i++) {
+                    std::string left = vc_substr2(mnt, 0, i + 1);
+                    if (left.size() > 1) left.insert(1, "&");
+                    std::string right = vc_substr1(mnt, -(int)(mnt.size() - (size_t)i - 1));
+                    if (right.size() > 1) right.insert(1, "&");
+                    if (copy.count(left)) {
+                        Var* tref = copy.at(left);
+                        if (tref->cnt <= 0) continue;
+                        if (tref->cnt < vref->cnt && tref->pmean / tref->cnt <= i + 1) {
+                            adjCnt(vref, tref, nullptr);
+                            copy.erase(left);
+                        }
+                    }
+                    VarMap** rm = P.nonIns.find(position + i + 1);
+                    if (rm && (*rm)->count(right)) {
+                        Var* tref = (*rm)->at(right);
+                        if (tref->cnt < 0) continue;
+                        if (tref->cnt < vref->cnt) {
+                            adjCnt(vref, tref, nullptr);
+                            P.refCov[position] += tref->cnt;
+                            (*P.nonIns.find(position + i + 1))->erase(right);
+                        }
+                    }
+                }
+                if (Sclip** s3 = P.sc3.find(position)) {
+                    Sclip* sc3v = *s3;
+                    if (!sc3v->used) {
+                        std::string seq = findconseq(sc3v);
+                        if (vc_substr2(seq, 0, (int)mnt.size()) == mnt) {
+                            if (seq.size() == mnt.size() ||
+                                ismatchref(seq.substr(mnt.size()), ref, position + (int)mnt.size(), 1)) {
+                                Var*& tgt = (**P.nonIns.find(position))[vn];
+                                if (!tgt) tgt = P.newVar();
+                                adjCnt(tgt, sc3v, nullptr);
+                                P.refCov[position] += sc3v->cnt;
+                                sc3v->used = true;
+                            }
+                        }
+                    }
+                }
+                if (Sclip** s5 = P.sc5.find(position + (int)mnt.size())) {
+                    Sclip* sc5v = *s5;
+                    if (!sc5v->used) {
+                        std::string seq = findconseq(sc5v);
+                        if (seq != " " && seq.size() >= mnt.size()) {
+                            std::reverse(seq.begin(), seq.end());
+                            if (vc_substr2(seq, (int)(seq.size() - mnt.size()), (int)mnt.size()) == mnt) {
+                                if (seq.size() == mnt.size() ||
+                                    ismatchref(seq.substr(0, seq.size() - mnt.size()), ref, position - 1, -1)) {
+                                    Var*& tgt = (**P.nonIns.find(position))[vn];
+                                    if (!tgt) tgt = P.newVar();
+                                    adjCnt(tgt, sc5v, nullptr);
+                                    P.refCov[position] += sc5v->cnt;
+                                    sc5v->used = true;
+                                }
+                            }
+                        }
+                    }
+                }
+            } catch (const std::exception&) {
+            }
+        }
+    }
+
+    void erase_inner(int pos, const std::string& d, bool drop_empty) {
+        VarMap** m = P.nonIns.find(pos);
+        if (!m) return;
+        (*m)->erase(d);
+        if (drop_empty && (*m)->empty()) P.nonIns.erase(pos);
+    }
+
+    void realigndel(bool withBams, const std::map<int, std::map<std::string, int>>& counts) {
+        static const std::regex BEGIN_MINUS_NUMBER("^-(\\d+).*");
+        static const std::regex UP_NUMBER_END("\\^(\\d+)$");
+        static const std::regex MINUS_SV("^-\\d+\\^([ATGNC]+)<...\\d+>([ATGNC]+)$");
+        static const std::regex BEGIN_MINUS_NUMBER_ANY("^-\\d+(.*)");
+        static const std::regex CARET_ATGNC("\\^([ATGNC]+)");
+        static const std::regex STRIP("\\^|&|#");
+        static const std::regex MINUS_NUMBER_AMP_ATGCs_END("(-\\d+)&[ATGC]+$");
+        std::vector<SPD> tmp = sorted_tmp(counts);
+        for (const SPD& t : tmp) {
+            try {
+                int p = t.position;
+                std::string vn = t.desc;
+                int dcnt = t.count;
+                Var* vref = P.getVariation(P.nonIns, p, vn);
+                int dellen = 0;
+                std::smatch sm;
+                if (std::regex_search(vn, sm, BEGIN_MINUS_NUMBER)) dellen = std::atoi(sm.str(1).c_str());
+                if (std::regex_search(vn, sm, UP_NUMBER_END)) dellen += std::atoi(sm.str(1).c_str());
+                std::string extrains, extra, inv5, inv3;
+                if (std::regex_search(vn, sm, MINUS_SV)) {
+                    inv5 = sm.str(1);
+                    inv3 = sm.str(2);
+                } else if (std::regex_search(vn, sm, BEGIN_MINUS_NUMBER_ANY)) {
+                    extra = std::regex_replace(sm.str(1), STRIP, "");
+                    std::smatch s2;
+                    if (std::regex_search(vn, s2, CARET_ATGNC)) extrains = s2.str(1);
+                }
+                int wustart = (p - 200) > 1 ? (p - 200) : 1;
+                std::string wupseq = joinRef(ref, wustart, p - 1) + extra;
+                if (!inv3.empty()) wupseq = inv3;
+                int sanend = p + 200 > chrLen ? chrLen : p + 200;
+                int el = (int)extra.size() - (int)extrains.size();
+                std::string sanpseq = extra + joinRef(ref, p + dellen + el, sanend);
+                if (!inv5.empty()) sanpseq = inv5;
+                MMResult r3 = findMM3(ref, p, sanpseq);
+                MMResult r5 = findMM5(ref, p + dellen + el - 1, wupseq);
+                std::vector<Mismatch> mmm = r3.mm;
+                mmm.insert(mmm.end(), r5.mm.begin(), r5.mm.end());
+                for (const Mismatch& mis : mmm) {
+                    std::string mm = mis.seq;
+                    int mp = mis.pos, me = mis.end;
+                    if (mm.size() > 1) mm.insert(1, 1, '&');
+                    Var* tv = inner(mp, mm);
+                    if (!tv) continue;
+                    if (tv->cnt == 0) continue;
+                    if (tv->qmean / tv->cnt < c.goodq) continue;
+                    if (tv->pmean / tv->cnt > (me == 3 ? r3.nm + 4 : r5.nm + 4)) continue;
+                    if (tv->cnt >= dcnt + dellen || tv->cnt / dcnt >= 8) continue;
+                    if (mp > p && me == 5) {
+                        double f = tv->pmean != 0 ? (mp - p) / (tv->pmean / (double)tv->cnt) : 1;
+                        if (f > 1) f = 1;
+                        P.refCov[p] += (int)(tv->cnt * f);
+                        adjRefCnt(tv, P.getVariationMaybe(p, ref.get(p)), dellen);
+                    }
+                    Var* lref = nullptr;
+                    if (mp > p && me == 3 && P.nonIns.count(p)) lref = inner(p, std::string(1, ref.get(p)));
+                    adjCnt(vref, tv, lref);
+                    erase_inner(mp, mm, true);
+                }
+                if (r3.misp != 0 && r3.mm.size() == 1) {
+                    Var* v = inner(r3.misp, r3.misnt);
+                    if (v && v->cnt < dcnt) erase_inner(r3.misp, r3.misnt, false);
+                }
+                if (r5.misp != 0 && r5.mm.size() == 1) {
+                    Var* v = inner(r5.misp, r5.misnt);
+                    if (v && v->cnt < dcnt) erase_inner(r5.misp, r5.misnt, false);
+                }
+                for (int sc5pp : r5.scp) {
+                    Sclip** s = P.sc5.find(sc5pp);
+                    if (s && !(*s)->used) {
+                        Sclip* tv = *s;
+                        std::string seq = findconseq(tv);
+                        if (dcnt <= 2 && tv->cnt / dcnt > 5) continue;
+                        if (seq != " " && ismatch(seq, wupseq, -1)) {
+                            if (sc5pp > p) P.refCov[p] += tv->cnt;
+                            adjCnt(vref, tv, nullptr);
+                            tv->used = true;
+                        }
+                    }
+                }
+                for (int sc3pp : r3.scp) {
+                    Sclip** s = P.sc3.find(sc3pp);
+                    if (s && !(*s)->used) {
+                        Sclip* tv = *s;
+                        std::string seq = findconseq(tv);
+                        if (dcnt <= 2 && tv->cnt / dcnt > 5) continue;
+                        if (seq != " " && ismatch(seq, vc_substr1(sanpseq, sc3pp - p), 1)) {
+                            if (sc3pp <= p) P.refCov[p] += tv->cnt;
+                            Var* lref = sc3pp <= p ? nullptr : P.getVariationMaybe(p, ref.get(p));
+                            adjCnt(vref, tv, lref);
+                            tv->used = true;
+                        }
+                    }
+                }
+                int pe = p + dellen + el;
+                Var* h = P.getVariationMaybe(p, ref.get(p));
+                if (withBams && bamsPresent && pe - p >= 5 && pe - p < P.maxReadLength - 10 && h && h->cnt != 0 &&
+                    vref->cnt > 2 * h->cnt * (1 - (pe - p) / (double)P.maxReadLength) && noPassingReads(p, pe)) {
+                    adjCnt(vref, h, h);
+                }
+            } catch (const std::exception&) {
+            }
+        }
+        for (int i = (int)tmp.size() - 1; i > 0; i--) {
+            const SPD& t = tmp[(size_t)i];
+            int p = t.position;
+            const std::string& vn = t.desc;
+            VarMap** m = P.nonIns.find(p);
+            if (!m) continue;
+            if (!(*m)->count(vn)) continue;
+            Var* vref = (**m)[vn];
+            std::smatch mt;
+            if (std::regex_search(vn, mt, MINUS_NUMBER_AMP_ATGCs_END)) {
+                std::string tn = mt.str(1);
+                if ((*m)->count(tn)) {
+                    Var* tref = (**m)[tn];
+                    if (vref->cnt < tref->cnt) {
+                        adjCnt(tref, vref, nullptr);
+                        (*m)->erase(vn);
+                    }
+                }
+            }
+        }
+    }
+
+    std::string realignins(const std::map<int, std::map<std::string, int>>& counts) {
+        static const std::regex BEGIN_PLUS_ATGC("^\\+([ATGC]+).*");
+        static const std::regex DUP_NUM_ATGC("<dup(\\d+)>([ATGC]+)$");
+        static const std::regex AMP_ATGC(".*&([ATGC]+).*");
+        static const std::regex HASH_ATGC("#([ATGC]+).*");
+        static const std::regex CARET_ATGC_END("\\^([ATGC]+)$");
+        static const std::regex UP_NUMBER_END("\\^(\\d+)$");
+        static const std::regex ATGSs_AMP_ATGSs_END("(\\+[ATGC]+)&[ATGC]+$");
+        std::vector<SPD> tmp = sorted_tmp(counts);
+        std::string NEWINS;
+        for (const SPD& t : tmp) {
+            try {
+                int position = t.position;
+                std::string vn = t.desc;
+                int icnt = t.count;
+                std::smatch m;
+                std::string insert1;
+                if (std::regex_search(vn, m, BEGIN_PLUS_ATGC))
+                    insert1 = m.str(1);
+                else
+                    continue;
+                std::string ins3;
+                int inslen = (int)insert1.size();
+                if (std::regex_search(vn, m, DUP_NUM_ATGC)) {
+                    ins3 = m.str(2);
+                    inslen += std::atoi(m.str(1).c_str()) + (int)ins3.size();
+                }
+                std::string extra, compm, newins;
+                if (std::regex_search(vn, m, AMP_ATGC)) extra = m.str(1);
+                if (std::regex_search(vn, m, HASH_ATGC)) compm = m.str(1);
+                if (std::regex_search(vn, m, CARET_ATGC_END)) newins = m.str(1);
+                int newdel = 0;
+                if (std::regex_search(vn, m, UP_NUMBER_END)) newdel = std::atoi(m.str(1).c_str());
+                std::string tn = vn;
+                replaceFirstRe(tn, "^\\+", "");
+                replaceFirst(tn, "&", "");
+                replaceFirst(tn, "#", "");
+                replaceFirstRe(tn, "\\^\\d+$", "");
+                replaceFirstRe(tn, "\\^", "");
+                int wustart = position - 150 > 1 ? (position - 150) : 1;
+                std::string wupseq = joinRef(ref, wustart, position) + tn;
+                int sanend = position + (int)vn.size() + 100;
+                if (chrLen < sanend) sanend = chrLen;
+                std::string sanpseq;
+                MMResult r3;
+                if (!ins3.empty()) {
+                    int p3 = position + inslen - (int)ins3.size() + 50;
+                    if (ins3.size() > 50) sanpseq = vc_substr1(ins3, 50 - (int)ins3.size());
+                    sanpseq += joinRef(ref, position + 1, position + 101);
+                    r3 = findMM3(ref, p3 + 1, sanpseq);
+                } else {
+                    sanpseq = tn + joinRef(ref, position + (int)extra.size() + 1 + (int)compm.size() + newdel, sanend);
+                    r3 = findMM3(ref, position + 1, sanpseq);
+                }
+                MMResult r5 = findMM5(ref, position + (int)extra.size() + (int)compm.size() + newdel, wupseq);
+                std::vector<Mismatch> mmm = r3.mm;
+                mmm.insert(mmm.end(), r5.mm.begin(), r5.mm.end());
+                Var* vref = P.getVariation(P.ins, position, vn);
+                for (const Mismatch& mis : mmm) {
+                    std::string mb = mis.seq;
+                    int mp = mis.pos, me = mis.end;
+                    if (mb.size() > 1) mb = std::string(1, mb[0]) + "&" + mb.substr(1);
+                    Var* v = inner(mp, mb);
+                    if (!v) continue;
+                    if (v->cnt == 0) continue;
+                    if (v->qmean / v->cnt < c.goodq) continue;
+                    if (v->pmean / v->cnt > (me == 3 ? r3.nm + 4 : r5.nm + 4)) continue;
+                    if (v->cnt >= icnt + (int)insert1.size() || v->cnt / icnt >= 8) continue;
+                    if (mp > position && me == 5) P.refCov[position] += v->cnt;
+                    Var* lref = nullptr;
+                    if (mp > position && me == 3 && P.nonIns.count(position) && ref.count(position))
+                        lref = inner(position, std::string(1, ref.get(position)));
+                    adjCnt(vref, v, lref);
+                    erase_inner(mp, mb, true);
+                }
+                if (r3.misp != 0 && r3.mm.size() == 1) {
+                    Var* v = inner(r3.misp, r3.misnt);
+                    if (v && v->cnt < icnt) erase_inner(r3.misp, r3.misnt, false);
+                }
+                if (r5.misp != 0 && r5.mm.size() == 1) {
+                    Var* v = inner(r5.misp, r5.misnt);
+                    if (v && v->cnt < icnt) erase_inner(r5.misp, r5.misnt, false);
+                }
+                for (int sc5pp : r5.scp) {
+                    Sclip** s = P.sc5.find(sc5pp);
+                    if (!s) continue;
+                    Sclip* tv = *s;
+                    if (!tv->used) {
+                        std::string seq = findconseq(tv);
+                        if (seq != " " && ismatch(seq, wupseq, -1)) {
+                            if (sc5pp > position) P.refCov[position] += tv->cnt;
+                            adjCnt(vref, tv, nullptr);
+                            tv->used = true;
+                        }
+                    }
+                }
+                for (int sc3pp : r3.scp) {
+                    Sclip** s = P.sc3.find(sc3pp);
+                    if (!s) continue;
+                    Sclip* tv = *s;
+                    if (!tv->used) {
+                        std::string seq = findconseq(tv);
+                        std::string mseq = !ins3.empty() ? sanpseq : vc_substr1(sanpseq, sc3pp - position - 1);
+                        if (seq != " " && ismatch(seq, mseq, 1)) {
+                            if (sc3pp <= position || (double)insert1.size() > tv->pmean / tv->cnt)
+                                P.refCov[position] += tv->cnt;
+                            Var* lref = nullptr;
+                            if (sc3pp > position && P.nonIns.count(position) && ref.count(position))
+                                lref = inner(position, std::string(1, ref.get(position)));
+                            if ((double)insert1.size() > tv->pmean / tv->cnt) lref = nullptr;
+                            adjCnt(vref, tv, lref);
+                            tv->used = true;
+                            if (insert1.size() + 1 == vn.size() && (int)insert1.size() > P.maxReadLength &&
+                                sc3pp >= position + 1 + (int)insert1.size()) {
+                                int flag = 0;
+                                int off = (sc3pp - position - 1) % (int)insert1.size();
+                                std::string tvn = vn;
+                                for (int si = 0; si < (int)seq.size() && si + off < (int)insert1.size(); si++) {
+                                    if (vc_substr2(seq, si, 1) != vc_substr2(insert1, si + off, 1)) {
+                                        flag++;
+                                        int shift = si + off + 1;
+                                        tvn = tvn.substr(0, (size_t)shift) + vc_substr2(seq, si, 1) + tvn.substr((size_t)shift + 1);
+                                    }
+                                }
+                                if (flag > 0) {
+                                    VarMap* im = *P.ins.find(position);
+                                    Var* vv = (*im)[vn];
+                                    (*im)[tvn] = vv;
+                                    im->erase(vn);
+                                    NEWINS = tvn;
+                                }
+                            }
+                        }
+                    }
+                }
+                int first3 = r3.scp[0];
+                int first5 = r5.scp[0];
+                if (!r3.scp.empty() && !r5.scp.empty() && first3 > first5 + 3 &&
+                    first3 - first5 < P.maxReadLength * 0.75) {
+                    if (ref.count(position) && P.nonIns.count(position)) {
+                        Var* rv = inner(position, std::string(1, ref.get(position)));
+                        if (rv) adjRefFactor(rv, (first3 - first5 - 1) / (double)P.maxReadLength);
+                    }
+                    adjRefFactor(vref, -(first3 - first5 - 1) / (double)P.maxReadLength);
+                }
+            } catch (const std::exception&) {
+            }
+        }
+        for (int i = (int)tmp.size() - 1; i > 0; i--) {
+            try {
+                const SPD& t = tmp[(size_t)i];
+                int p = t.position;
+                const std::string& vn = t.desc;
+                VarMap** m = P.ins.find(p);
+                if (!m) continue;
+                if (!(*m)->count(vn)) continue;
+                Var* vref = (*m)->at(vn);
+                std::smatch mt;
+                if (std::regex_search(vn, mt, ATGSs_AMP_ATGSs_END)) {
+                    std::string tn = mt.str(1);
+                    if ((*m)->count(tn)) {
+                        Var* tref = (*m)->at(tn);
+                        if (vref->cnt < tref->cnt) {
+                            adjCnt(tref, vref, P.getVariationMaybe(p, ref.get(p)));
+                            (*m)->erase(vn);
+                        }
+                    }
+                }
+            } catch (const std::exception&) {
+            }
+        }
+        return NEWINS;
+    }
+
+    // find35match (VariationRealigner.cpp:1396-1428)
+    void find35match(const std::string& seq5, const std::string& seq3, int& b5, int& b3, int& score) {
+        b5 = b3 = score = 0;
+        for (size_t i = 0; i < seq5.size() - 8; i++) {
+            for (size_t j = 1; j < seq3.size() - 8; j++) {
+                int nm = 0, n = 0;
+                while (n + (int)j <= (int)seq3.size() && (int)i + n <= (int)seq5.size()) {
+                    if (vc_substr2(seq3, -(int)j - n, 1) != vc_substr2(seq5, (int)i + n, 1)) nm++;
+                    if (nm > 2) break;
+                    n++;
+                }
+                if (n - nm > score && n - nm > 8 && nm / (double)n < 0.1 &&
+                    (n + (int)j >= (int)seq3.size() || (int)i + n >= (int)seq5.size())) {
+                    score = n - nm;
+                    b3 = (int)j;
+                    b5 = (int)i;
+                    return;
+                }
+            }
+        }
+    }
+
+    void realignlgins30() {
+        RefMap rf = ref;  // the reference copies the whole map (VariationRealigner.cpp:1120)
+        struct SPS {
+            int position;
+            Sclip* sc;
+            int count;
+        };
+        std::vector<SPS> t5, t3;
+        P.sc5.for_each([&](int p, Sclip* s) {
+            if (p < R.start - 5000 || p > R.end + 5000) return;
+            t5.push_back({p, s, s->cnt});
+        });
+        P.sc3.for_each([&](int p, Sclip* s) {
+            if (p < R.start - 5000 || p > R.end + 5000) return;
+            t3.push_back({p, s, s->cnt});
+        });
+        auto comp3 = [](const SPS& a, const SPS& b) {
+            if (b.count != a.count) return a.count > b.count;
+            return a.position < b.position;
+        };
+        std::sort(t5.begin(), t5.end(), comp3);
+        std::sort(t3.begin(), t3.end(), comp3);
+        for (const SPS& a : t5) {
+            int p5 = a.position;
+            Sclip* sc5v = a.sc;
+            int cnt5 = a.count;
+            if (sc5v->used) continue;
+            std::string seq5 = findconseq(sc5v);
+            if (seq5.size() <= 10) continue;
+            for (const SPS& b : t3) {
+                try {
+                    int p3 = b.position;
+                    Sclip* sc3v = b.sc;
+                    int cnt3 = b.count;
+                    if (sc5v->used) break;
+                    if (sc3v->used) continue;
+                    if (p5 - p3 > P.maxReadLength * 2.5) continue;
+                    if (p3 - p5 > P.maxReadLength - 10) continue;
+                    std::string seq3 = findconseq(sc3v);
+                    if (seq3.size() <= 10) continue;
+                    if (!(cnt5 / (double)cnt3 >= 0.08 && cnt5 / (double)cnt3 <= 12)) continue;
+                    int bp5, bp3, score;
+                    find35match(seq5, seq3, bp5, bp3, score);
+                    if (score == 0) continue;
+                    int smscore = (int)(score / 2);
+                    std::string ins = bp3 + smscore > 1 ? vc_substr2(seq3, 0, -(bp3 + smscore) + 1) : seq3;
+                    if (bp5 + smscore > 0) {
+                        std::string tmpstr = seq5.substr(0, (size_t)(bp5 + smscore));
+                        std::reverse(tmpstr.begin(), tmpstr.end());
+                        ins += tmpstr;
+                    }
+                    if (islowcomplexseq(ins)) continue;
+                    int bi = 0;
+                    Var* vref;
+                    if (p5 > p3) {
+                        if (seq3.size() > ins.size() &&
+                            !ismatch(seq3.substr(ins.size()),
+                                     joinRef(rf, p5, (int)(p5 + seq3.size() - ins.size() + 2)), 1))
+                            continue;
+                        if (seq5.size() > ins.size() &&
+                            !ismatch(seq5.substr(ins.size()),
+                                     joinRef(rf, (int)(p3 - (seq5.size() - ins.size()) - 2), p3 - 1), -1))
+                            continue;
+                        std::string tmp = joinRef(rf, p3, p5 - 1);
+                        if (tmp.size() > ins.size()) {
+                            ins = std::to_string(p3 - p5) + "^" + ins;
+                            bi = p3;
+                            vref = P.getVariation(P.nonIns, p3, ins);
+                        } else if (tmp.size() < ins.size()) {
+                            ins = vc_substr2(ins, 0, (int)(ins.size() - tmp.size())) + "&" + vc_substr1(ins, p3 - p5);
+                            ins = "+" + ins;
+                            bi = p3 - 1;
+                            vref = P.getVariation(P.ins, bi, ins);
+                        } else {
+                            ins = "-" + std::to_string(ins.size()) + "^" + ins;
+                            bi = p3;
+                            vref = P.getVariation(P.nonIns, p3, ins);
+                        }
+                    } else {
+                        if (seq3.size() > ins.size() &&
+                            !ismatch(seq3.substr(ins.size()),
+                                     joinRef(rf, p5, (int)(p5 + seq3.size() - ins.size() + 2)), 1))
+                            continue;
+                        if (seq5.size() > ins.size() &&
+                            !ismatch(seq5.substr(ins.size()),
+                                     joinRef(rf, (int)(p3 - (seq5.size() - ins.size()) - 2), p3 - 1), -1))
+                            continue;
+                        std::string tmp;
+                        if (ins.size() <= (size_t)(p3 - p5)) {
+                            int rpt = 2, tnr = 3;
+                            while (((p3 - p5 + ins.size()) / (double)tnr) / (double)ins.size() > 1) {
+                                if ((p3 - p5 + ins.size()) % (size_t)tnr == 0) rpt++;
+                                tnr++;
+                            }
+                            tmp += joinRefD(rf, p5, (p5 + (p3 - p5 + ins.size()) / (double)rpt - ins.size()));
+                            ins = "+" + tmp + ins;
+                        } else {
+                            tmp += joinRef(rf, p5, p3 - 1);
+                            if ((ins.size() - tmp.size()) % 2 == 0) {
+                                int tex = (int)((ins.size() - tmp.size()) / 2);
+                                ins = (tmp + vc_substr2(ins, 0, tex)) == vc_substr1(ins, tex) ? ("+" + vc_substr1(ins, tex))
+                                                                                              : "+" + tmp + ins;
+                            } else {
+                                ins = "+" + tmp + ins;
+                            }
+                        }
+                        bi = p5 - 1;
+                        vref = P.getVariation(P.ins, bi, ins);
+                    }
+                    sc3v->used = true;
+                    sc5v->used = true;
+                    vref->pstd = true;
+                    vref->qstd = true;
+                    P.refCov[bi] += sc5v->cnt;
+                    if (ins[0] == '+') {
+                        Var* mvref = P.getVariationMaybe(bi, rf.get(bi));
+                        adjCnt(vref, sc3v, mvref);
+                        adjCnt(vref, sc5v, nullptr);
+                        if (bamsPresent && p3 - p5 >= 5 && p3 - p5 < P.maxReadLength - 10 && mvref && mvref->cnt != 0 &&
+                            vref->cnt > 2 * mvref->cnt && noPassingReads(p5, p3)) {
+                            adjCnt(vref, mvref, mvref);
+                        }
+                        std::map<int, std::map<std::string, int>> tins;
+                        tins[bi][ins] = vref->cnt;
+                        realignins(tins);
+                    } else if (ins[0] == '-') {
+                        adjCnt(vref, sc3v, P.getVariationMaybe(bi, rf.get(bi)));
+                        adjCnt(vref, sc5v, nullptr);
+                        std::map<int, std::map<std::string, int>> tdel;
+                        tdel[bi][ins] = vref->cnt;
+                        realigndel(false, tdel);
+                    } else {
+                        adjCnt(vref, sc3v, nullptr);
+                        adjCnt(vref, sc5v, nullptr);
+                    }
+                    break;
+                } catch (const std::exception&) {
+                }
+            }
+        }
+    }
+};
+
+}  // namespace
+
+void realign_region(const Config& c, const Region& r, RefMap& ref, int chrLen,
+                    const std::vector<const Read*>& regionReads, Pileup& P) {
+    Realigner rg{c, r, ref, chrLen, P, regionReads};
+    rg.adjustMNP();
+    if (c.performLocalRealignment) {
+        rg.realigndel(true, P.delCount);
+        rg.realignins(P.insCount);
+        rg.realignlgins30();
+    }
+}
+
+}  // namespace orc

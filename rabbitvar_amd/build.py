@@ -1,0 +1,66 @@
+"""Builds the native parts of rabbitvar_amd in-tree.
+
+  rabbitvar_amd/_build/librabbitvar_amd.so   HIP kernels for gfx950 + host engine + C ABI
+  rabbitvar_amd/_build/rabbitvar-amd          command-line driver (same options as RabbitVar)
+  rabbitvar_amd/_build/rvsynth                seeded synthetic input generator
+
+Run:  python -m rabbitvar_amd.build      (or import and call build())
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "_build")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+# -packed-fp32-ops: keep the compiler from emitting packed FP32 ops; -ffp-contract=off keeps the
+# FP64 arithmetic that feeds printed values identical to the reference's non-contracted x86 build.
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950", "-ffp-contract=off", "-fno-fast-math",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-command-line-argument",
+]
+
+LIB_SOURCES = ["rv_pileup.hip", "rv_score.hip", "rv_host.cpp", "rv_engine.cpp", "rv_region.cpp",
+               "rv_realign.cpp", "rv_format.cpp", "rv_capi.cpp"]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build(verbose=True):
+    os.makedirs(OUT, exist_ok=True)
+    objs = []
+    inc = ["-I", CSRC, "-I", os.path.join(ROOT, "include")]
+    for src in LIB_SOURCES:
+        path = os.path.join(CSRC, src)
+        if not os.path.exists(path):
+            continue
+        obj = os.path.join(OUT, src.replace(".", "_") + ".o")
+        _run([HIPCC, "-x", "hip"] + HIP_FLAGS + inc + ["-c", path, "-o", obj])
+        objs.append(obj)
+    lib = os.path.join(OUT, "librabbitvar_amd.so")
+    _run([HIPCC] + HIP_FLAGS + ["-shared", "-o", lib + ".tmp"] + objs)
+    os.replace(lib + ".tmp", lib)
+    cli_src = os.path.join(CSRC, "rv_main.cpp")
+    if os.path.exists(cli_src):
+        exe = os.path.join(OUT, "rabbitvar-amd")
+        _run([HIPCC, "-x", "hip"] + HIP_FLAGS + inc + [cli_src, "-o", exe + ".tmp", "-L", OUT, "-lrabbitvar_amd",
+                                                       "-Wl,-rpath,$ORIGIN"])
+        os.replace(exe + ".tmp", exe)
+    gxx = shutil.which("g++") or "g++"
+    synth = os.path.join(OUT, "rvsynth")
+    _run([gxx, "-O2", "-std=c++17", "-o", synth + ".tmp", os.path.join(HERE, "synth", "rv_synth.cpp")])
+    os.replace(synth + ".tmp", synth)
+    return lib
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

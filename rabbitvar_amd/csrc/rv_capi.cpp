@@ -1,0 +1,126 @@
+// extern "C" entry points (declared in include/rabbitvar_amd.h).
+#include <cstdlib>
+#include <cstring>
+#include <new>
+
+#include "rabbitvar_amd.h"
+#include "rv_engine.h"
+#include "rv_region.h"
+
+struct rv_engine {
+    rv::Engine* e;
+};
+
+struct rv_callset {
+    rv::CallSet cs;
+};
+
+extern "C" {
+
+void rv_config_default(rv_config* c) {
+    if (!c) return;
+    std::memset(c, 0, sizeof *c);
+    c->goodq = 22.5;
+    c->bias = 0.05;
+    c->freq = 0.01;
+    c->qratio = 1.5;
+    c->mapq = 0;
+    c->lofreq = 0.05;
+    c->vext = 2;
+    c->mismatch = 8;
+    c->trimBasesAfter = 0;
+    c->performLocalRealignment = 1;
+    c->indelsize = 50;
+    c->minBiasReads = 2;
+    c->minr = 2;
+    c->moveIndelsTo3 = 0;
+    c->samfilter = 0x504;
+    c->readPosFilter = 5;
+    c->doPileup = 0;
+    c->minmatch = 0;
+    c->uniqueModeAlignmentEnabled = 0;
+    c->uniqueModeSecondInPairEnabled = 0;
+    c->fisher = 0;
+    c->SVMINLEN = 1000;
+    c->referenceExtension = 1200;
+    c->numberNucleotideToExtend = 0;
+    c->mappingQuality = 0;
+    c->removeDuplicatedReads = 0;
+    c->crisprCuttingSite = 0;
+    c->sample = "";
+}
+
+rv_status rv_open(const rv_config* cfg, int device, rv_engine** out) {
+    if (!cfg || !out) return RV_ERR_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0 || device < 0 || device >= count) return RV_ERR_DEVICE;
+    rv_engine* h = new (std::nothrow) rv_engine;
+    if (!h) return RV_ERR_OOM;
+    h->e = new (std::nothrow) rv::Engine(*cfg, device);
+    if (!h->e || !h->e->stream) {
+        delete h->e;
+        delete h;
+        return RV_ERR_DEVICE;
+    }
+    *out = h;
+    return RV_OK;
+}
+
+rv_status rv_run_region(rv_engine* e, const rv_region* region, const rv_reads* reads, const rv_contig* contig,
+                        rv_callset** out) {
+    if (!e || !region || !reads || !contig || !out) return RV_ERR_ARG;
+    *out = nullptr;
+    rv_callset* cs = new (std::nothrow) rv_callset;
+    if (!cs) return RV_ERR_OOM;
+    rv_status s = rv::run_region(*e->e, *region, *reads, *contig, cs->cs);
+    if (s != RV_OK) {
+        delete cs;
+        return s;
+    }
+    *out = cs;
+    return RV_OK;
+}
+
+size_t rv_callset_count(const rv_callset* cs) { return cs ? cs->cs.calls.size() : 0; }
+const rv_call* rv_callset_get(const rv_callset* cs, size_t i) {
+    if (!cs || i >= cs->cs.calls.size()) return nullptr;
+    return &cs->cs.calls[i];
+}
+const char* rv_callset_text(const rv_callset* cs, size_t* len) {
+    if (!cs) return nullptr;
+    if (len) *len = cs->cs.text.size();
+    return cs->cs.text.c_str();
+}
+void rv_callset_free(rv_callset* cs) { delete cs; }
+
+rv_status rv_get_stats(const rv_engine* e, rv_stats* out) {
+    if (!e || !out) return RV_ERR_ARG;
+    *out = e->e->stats;
+    return RV_OK;
+}
+const char* rv_last_error(const rv_engine* e) { return e ? e->e->err.c_str() : "null engine"; }
+void rv_close(rv_engine* e) {
+    if (!e) return;
+    delete e->e;
+    delete e;
+}
+
+rv_status rv_debug_pileup(rv_engine* e, const rv_region* region, const rv_reads* reads, const rv_contig* contig,
+                          char** text, size_t* len) {
+    if (!e || !region || !reads || !contig || !text) return RV_ERR_ARG;
+    rv::PileupResult pr;
+    rv_status s = e->e->pileup(*region, *reads, *contig, true, pr);
+    if (s != RV_OK) return s;
+    std::string d = rv::dump_pileup(pr.P);
+    char* p = (char*)std::malloc(d.size() + 1);
+    if (!p) return RV_ERR_OOM;
+    std::memcpy(p, d.c_str(), d.size() + 1);
+    *text = p;
+    if (len) *len = d.size();
+    return RV_OK;
+}
+
+void rv_free(void* p) { std::free(p); }
+
+}  // extern "C"

@@ -1,0 +1,297 @@
+// Engine implementation: device orchestration of the pileup stage.
+#include "rv_engine.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+namespace rv {
+
+#define RV_HIP(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) return fail(RV_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+static Config to_config(const rv_config& c) {
+    Config k;
+    k.goodq = c.goodq;
+    k.bias = c.bias;
+    k.freq = c.freq;
+    k.qratio = c.qratio;
+    k.mapq = c.mapq;
+    k.lofreq = c.lofreq;
+    k.vext = c.vext;
+    k.mismatch = c.mismatch;
+    k.trimBasesAfter = c.trimBasesAfter;
+    k.performLocalRealignment = c.performLocalRealignment;
+    k.indelsize = c.indelsize;
+    k.minBiasReads = c.minBiasReads;
+    k.minr = c.minr;
+    k.moveIndelsTo3 = c.moveIndelsTo3;
+    k.samfilter = c.samfilter;
+    k.readPosFilter = c.readPosFilter;
+    k.doPileup = c.doPileup;
+    k.minmatch = c.minmatch;
+    k.uniqueModeAlignmentEnabled = c.uniqueModeAlignmentEnabled;
+    k.uniqueModeSecondInPairEnabled = c.uniqueModeSecondInPairEnabled;
+    k.fisher = c.fisher;
+    k.SVMINLEN = c.SVMINLEN;
+    k.referenceExtension = c.referenceExtension;
+    k.numberNucleotideToExtend = c.numberNucleotideToExtend;
+    k.mappingQuality = c.mappingQuality;
+    k.removeDuplicatedReads = c.removeDuplicatedReads;
+    k.crisprCuttingSite = c.crisprCuttingSite;
+    k.hasBam2 = 0;
+    return k;
+}
+
+Engine::Engine(const rv_config& c, int dev) {
+    cfg = to_config(c);
+    sample = c.sample ? c.sample : "";
+    device = dev;
+    if (hipSetDevice(dev) != hipSuccess) {
+        err = "hipSetDevice failed";
+        return;
+    }
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) {
+        err = "hipStreamCreate failed";
+        return;
+    }
+    for (auto& e : ev) hipEventCreate(&e);
+}
+
+Engine::~Engine() {
+    DevBuf* all[] = {&b_prep, &b_cig, &b_ctr, &b_span, &b_rows, &b_ref, &b_events, &b_arena, &b_clips, &b_clip_arena,
+                     &b_anchors, &b_counts};
+    for (auto* b : all)
+        if (b->p) hipFree(b->p);
+    for (auto& b : b_reads)
+        if (b.p) hipFree(b.p);
+    for (auto& e : ev)
+        if (e) hipEventDestroy(e);
+    if (stream) hipStreamDestroy(stream);
+}
+
+void* Engine::grow(DevBuf& b, size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    if (b.cap >= bytes) return b.p;
+    if (b.p) hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+    size_t want = bytes + bytes / 4;
+    if (hipMalloc(&b.p, want) != hipSuccess) return nullptr;
+    b.cap = want;
+    return b.p;
+}
+
+// Reference window (recordPreprocessor.cpp:41-78): keys [max(1, s-x-ext), min(len, e+x+ext) - 17]
+static void window_bounds(const Config& c, const rv_region& r, int chrLen, int& lo, int& hi) {
+    int ext = c.referenceExtension;
+    int ss = r.start - c.numberNucleotideToExtend - ext < 1 ? 1 : r.start - c.numberNucleotideToExtend - ext;
+    int se = r.end + c.numberNucleotideToExtend + ext > chrLen ? chrLen : r.end + c.numberNucleotideToExtend + ext;
+    lo = ss;
+    hi = se - 17;
+}
+
+rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_contig& contig, bool download_rows,
+                         PileupResult& out) {
+    if (cfg.removeDuplicatedReads) return fail(RV_ERR_UNSUPPORTED, "-t (duplicate removal) is not implemented");
+    if (cfg.crisprCuttingSite != 0) return fail(RV_ERR_UNSUPPORTED, "CRISPR mode is not implemented");
+    auto h0 = std::chrono::steady_clock::now();
+    double host_ms = 0;
+    const int64_t n = rd.n_reads;
+    out.n_reads = n;
+    // ---- reads on the device
+    ReadsDev R;
+    R.n = n;
+    if (rd.on_device) {
+        R.pos0 = rd.pos0;
+        R.flag = rd.flag;
+        R.mapq = rd.mapq;
+        R.has_nm = rd.has_nm;
+        R.nm = rd.nm;
+        R.mpos0 = rd.mpos0;
+        R.tid = rd.tid;
+        R.mtid = rd.mtid;
+        R.seq_off = rd.seq_off;
+        R.cig_off = rd.cigar_off;
+        R.bases = rd.bases;
+        R.quals = rd.quals;
+        R.cigar = rd.cigar;
+    } else {
+        int64_t nb = n > 0 ? rd.seq_off[n] : 0, nc = n > 0 ? rd.cigar_off[n] : 0;
+        struct Up {
+            const void* src;
+            size_t bytes;
+        } ups[13] = {{rd.pos0, 4 * (size_t)n},        {rd.flag, 2 * (size_t)n},        {rd.mapq, (size_t)n},
+                     {rd.has_nm, (size_t)n},          {rd.nm, 4 * (size_t)n},          {rd.mpos0, 4 * (size_t)n},
+                     {rd.tid, 4 * (size_t)n},         {rd.mtid, 4 * (size_t)n},        {rd.seq_off, 8 * (size_t)(n + 1)},
+                     {rd.cigar_off, 8 * (size_t)(n + 1)}, {rd.bases, (size_t)nb},      {rd.quals, (size_t)nb},
+                     {rd.cigar, 4 * (size_t)nc}};
+        void* dst[13];
+        for (int i = 0; i < 13; i++) {
+            dst[i] = grow(b_reads[i], ups[i].bytes);
+            if (!dst[i]) return fail(RV_ERR_OOM, "device allocation for reads failed");
+            if (ups[i].bytes) RV_HIP(hipMemcpyAsync(dst[i], ups[i].src, ups[i].bytes, hipMemcpyHostToDevice, stream));
+        }
+        R.pos0 = (const int32_t*)dst[0];
+        R.flag = (const uint16_t*)dst[1];
+        R.mapq = (const uint8_t*)dst[2];
+        R.has_nm = (const uint8_t*)dst[3];
+        R.nm = (const int32_t*)dst[4];
+        R.mpos0 = (const int32_t*)dst[5];
+        R.tid = (const int32_t*)dst[6];
+        R.mtid = (const int32_t*)dst[7];
+        R.seq_off = (const int64_t*)dst[8];
+        R.cig_off = (const int64_t*)dst[9];
+        R.bases = (const uint8_t*)dst[10];
+        R.quals = (const uint8_t*)dst[11];
+        R.cigar = (const uint32_t*)dst[12];
+    }
+    // ---- reference window
+    Window W;
+    W.region_start = region.start;
+    W.region_end = region.end;
+    W.chr_len = contig.length;
+    window_bounds(cfg, region, contig.length, W.ref_lo, W.ref_hi);
+    int win_len = W.ref_hi >= W.ref_lo ? W.ref_hi - W.ref_lo + 1 : 0;
+    {
+        std::string up((size_t)std::max(win_len, 1), 'N');
+        for (int i = 0; i < win_len; i++) {
+            char ch = contig.bases[W.ref_lo - 1 + i];
+            up[(size_t)i] = (ch >= 'a' && ch <= 'z') ? (char)(ch - 32) : ch;
+        }
+        ref_dev = (char*)grow(b_ref, up.size());
+        if (!ref_dev) return fail(RV_ERR_OOM, "device allocation for the reference window failed");
+        RV_HIP(hipMemcpyAsync(ref_dev, up.data(), up.size(), hipMemcpyHostToDevice, stream));
+    }
+    W.ref = ref_dev;
+    W.dense_lo = 0;
+    W.dense_n = 0;
+    // ---- prep
+    PrepRec* prep = (PrepRec*)grow(b_prep, sizeof(PrepRec) * (size_t)std::max<int64_t>(n, 1));
+    uint32_t* cig = (uint32_t*)grow(b_cig, sizeof(uint32_t) * MAXC * (size_t)std::max<int64_t>(n, 1));
+    WalkCounters* ctr = (WalkCounters*)grow(b_ctr, sizeof(WalkCounters));
+    int* span = (int*)grow(b_span, 4 * sizeof(int));
+    if (!prep || !cig || !ctr || !span) return fail(RV_ERR_OOM, "device allocation failed");
+    RV_HIP(hipMemsetAsync(ctr, 0, sizeof(WalkCounters), stream));
+    RV_HIP(hipMemsetAsync(span, 0, 4 * sizeof(int), stream));
+    RV_HIP(hipEventRecord(ev[0], stream));
+    RV_HIP(launch_prep(R, W, cfg, prep, cig, ctr, span, stream));
+    RV_HIP(hipEventRecord(ev[1], stream));
+    int max_span = 0;
+    RV_HIP(hipMemcpyAsync(&max_span, span, sizeof(int), hipMemcpyDeviceToHost, stream));
+    RV_HIP(hipStreamSynchronize(stream));
+    // ---- dense window: every position a region read can touch
+    int pad = 2 * std::max(max_span, 64) + 8;
+    W.dense_lo = region.start - pad;
+    W.dense_n = (region.end + pad) - W.dense_lo + 1;
+    rows = (DenseRow*)grow(b_rows, sizeof(DenseRow) * (size_t)W.dense_n);
+    if (!rows) return fail(RV_ERR_OOM, "device allocation for the pileup rows failed");
+    rows_n = W.dense_n;
+    RV_HIP(launch_fill_rows(rows, W.dense_n, stream));
+    // ---- walk with bounded record buffers (grown and re-run on overflow)
+    unsigned max_events = (unsigned)std::min<int64_t>(4 * n + 4096, 0x7fffffff);
+    unsigned long long arena_cap = 64ull * (unsigned long long)n + 65536;
+    unsigned max_clips = (unsigned)std::min<int64_t>(n + 1024, 0x7fffffff);
+    unsigned long long clip_cap = 320ull * (unsigned long long)max_clips;
+    unsigned max_anchors = (unsigned)std::min<int64_t>(n + 1024, 0x7fffffff);
+    WalkCounters hc{};
+    unsigned counts[4];
+    for (int attempt = 0; attempt < 4; attempt++) {
+        WalkSink S;
+        S.dense = rows;
+        S.dense_lo = W.dense_lo;
+        S.dense_n = W.dense_n;
+        S.events = (Event*)grow(b_events, sizeof(Event) * (size_t)max_events);
+        S.arena = (char*)grow(b_arena, (size_t)arena_cap);
+        S.clips = (ClipRec*)grow(b_clips, sizeof(ClipRec) * (size_t)max_clips);
+        S.clip_arena = (char*)grow(b_clip_arena, (size_t)clip_cap);
+        S.anchors = (Anchor*)grow(b_anchors, sizeof(Anchor) * (size_t)max_anchors);
+        unsigned long long* cnt64 = (unsigned long long*)grow(b_counts, 64);
+        if (!S.events || !S.arena || !S.clips || !S.clip_arena || !S.anchors || !cnt64)
+            return fail(RV_ERR_OOM, "device allocation for walk records failed");
+        RV_HIP(hipMemsetAsync(cnt64, 0, 64, stream));
+        S.n_events = (unsigned*)cnt64;
+        S.n_clips = (unsigned*)cnt64 + 1;
+        S.n_anchors = (unsigned*)cnt64 + 2;
+        S.arena_used = cnt64 + 2;
+        S.clip_used = cnt64 + 3;
+        S.max_events = max_events;
+        S.arena_cap = arena_cap;
+        S.max_clips = max_clips;
+        S.clip_cap = clip_cap;
+        S.max_anchors = max_anchors;
+        S.ctr = ctr;
+        if (attempt > 0) {
+            RV_HIP(launch_fill_rows(rows, W.dense_n, stream));
+            RV_HIP(hipMemsetAsync(ctr, 0, sizeof(WalkCounters), stream));
+            RV_HIP(launch_prep(R, W, cfg, prep, cig, ctr, span, stream));
+        }
+        RV_HIP(hipEventRecord(ev[2], stream));
+        RV_HIP(launch_walk(R, W, cfg, prep, cig, S, stream));
+        RV_HIP(hipEventRecord(ev[3], stream));
+        RV_HIP(launch_anchor(S.anchors, S.n_anchors, max_anchors, rows, W.dense_lo, W.dense_n, cfg, stream));
+        RV_HIP(hipEventRecord(ev[4], stream));
+        unsigned long long hcnt[4];
+        RV_HIP(hipMemcpyAsync(hcnt, cnt64, 32, hipMemcpyDeviceToHost, stream));
+        RV_HIP(hipMemcpyAsync(&hc, ctr, sizeof hc, hipMemcpyDeviceToHost, stream));
+        RV_HIP(hipStreamSynchronize(stream));
+        counts[0] = (unsigned)(hcnt[0] & 0xffffffffu);
+        counts[1] = (unsigned)(hcnt[0] >> 32);
+        counts[2] = (unsigned)(hcnt[1] & 0xffffffffu);
+        unsigned long long aused = hcnt[2], cused = hcnt[3];
+        bool over = counts[0] > max_events || aused > arena_cap || counts[1] > max_clips || cused > clip_cap ||
+                    counts[2] > max_anchors;
+        if (!over && hc.overflow == 0) {
+            // ---- copy the ordered records back and reduce them
+            std::vector<Event> evs(counts[0]);
+            std::vector<char> arena((size_t)aused);
+            std::vector<ClipRec> cls(counts[1]);
+            std::vector<char> carena((size_t)cused);
+            if (counts[0]) RV_HIP(hipMemcpyAsync(evs.data(), S.events, sizeof(Event) * counts[0], hipMemcpyDeviceToHost, stream));
+            if (aused) RV_HIP(hipMemcpyAsync(arena.data(), S.arena, (size_t)aused, hipMemcpyDeviceToHost, stream));
+            if (counts[1]) RV_HIP(hipMemcpyAsync(cls.data(), S.clips, sizeof(ClipRec) * counts[1], hipMemcpyDeviceToHost, stream));
+            if (cused) RV_HIP(hipMemcpyAsync(carena.data(), S.clip_arena, (size_t)cused, hipMemcpyDeviceToHost, stream));
+            if (download_rows) {
+                out.P.rows.resize((size_t)W.dense_n);
+                RV_HIP(hipMemcpyAsync(out.P.rows.data(), rows, sizeof(DenseRow) * (size_t)W.dense_n, hipMemcpyDeviceToHost,
+                                      stream));
+            }
+            RV_HIP(hipStreamSynchronize(stream));
+            auto r0 = std::chrono::steady_clock::now();
+            out.P.dense_lo = W.dense_lo;
+            out.P.dense_n = W.dense_n;
+            out.P.maxReadLength = hc.max_read_length;
+            reduce_events(evs.data(), evs.size(), arena.data(), cfg.goodq, out.P);
+            reduce_clips(cls.data(), cls.size(), carena.data(), cfg.goodq, out.P);
+            host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - r0).count();
+            out.W = W;
+            out.ctr = hc;
+            float t01 = 0, t23 = 0, t34 = 0;
+            hipEventElapsedTime(&t01, ev[0], ev[1]);
+            hipEventElapsedTime(&t23, ev[2], ev[3]);
+            hipEventElapsedTime(&t34, ev[3], ev[4]);
+            stats.prep_ms = t01;
+            stats.walk_ms = t23;
+            stats.anchor_ms = t34;
+            stats.reads_in = n;
+            stats.events = counts[0];
+            stats.clips = counts[1];
+            stats.ref_key_creations = (int64_t)hc.ref_insert;
+            stats.walk_errors = (int64_t)hc.walk_errors;
+            stats.host_ms = host_ms;
+            (void)h0;
+            return RV_OK;
+        }
+        max_events = std::max<unsigned>(max_events, counts[0]) * 2;
+        arena_cap = std::max(arena_cap, aused) * 2;
+        max_clips = std::max<unsigned>(max_clips, counts[1]) * 2;
+        clip_cap = std::max(clip_cap, cused) * 2;
+        max_anchors = std::max<unsigned>(max_anchors, counts[2]) * 2;
+    }
+    return fail(RV_ERR_CAPACITY, "walk record buffers kept overflowing");
+}
+
+}  // namespace rv

@@ -1,0 +1,61 @@
+// Engine: one per device, driven by one host thread.  Owns the stream, the grow-only device
+// buffers and the per-stage timers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "rabbitvar_amd.h"
+#include "rv_host.h"
+#include "rv_kernels.h"
+
+namespace rv {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+
+struct PileupResult {
+    HostPileup P;        // reduced non-dense state + dense rows (rows only when requested)
+    Window W;
+    WalkCounters ctr;
+    int64_t n_reads = 0, walked = 0;
+};
+
+class Engine {
+  public:
+    Engine(const rv_config& cfg, int device);
+    ~Engine();
+
+    // Runs prep + walk + anchor for one region; reduces the ordered records on the host.
+    // When download_rows is set the whole dense window is copied back (diagnostics).
+    rv_status pileup(const rv_region& region, const rv_reads& reads, const rv_contig& contig, bool download_rows,
+                     PileupResult& out);
+
+    Config cfg;
+    std::string sample;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    rv_stats stats{};
+
+    // device state of the last pileup (kept for the scoring stage)
+    DenseRow* rows = nullptr;
+    int rows_n = 0;
+    char* ref_dev = nullptr;
+
+    void* grow(DevBuf& b, size_t bytes);
+    rv_status fail(rv_status s, const std::string& msg) {
+        err = msg;
+        return s;
+    }
+
+  private:
+    DevBuf b_reads[13], b_prep, b_cig, b_ctr, b_span, b_rows, b_ref;
+    DevBuf b_events, b_arena, b_clips, b_clip_arena, b_anchors, b_counts;
+    hipEvent_t ev[8];
+};
+
+}  // namespace rv

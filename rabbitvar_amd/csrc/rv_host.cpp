@@ -1,0 +1,344 @@
+// Host-side reduction of device walk records.  See rv_host.h.
+#include "rv_host.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+
+namespace rv {
+
+std::string fmt_f(double v) {
+    char buf[512];
+    snprintf(buf, sizeof buf, "%f", v);
+    return buf;
+}
+
+// robin_hood::unordered_map<char,int> slot placement for an insert-only map (robin_hood 3.4.3:
+// 8 initial slots, 5 info bits, max load 80%, hash_int = hi+lo of the 128-bit product with
+// 0xde5fb9d2630458e9; insertion walks right while the resident info byte >= ours and shifts the
+// run up; iteration visits slots in ascending order).  With at most 16 distinct byte keys the map
+// may grow once (8 -> 16 slots), which is modelled by re-inserting in old slot order.
+namespace {
+struct CharTable {
+    int slots = 8;
+    unsigned inc = 32, shift = 59;
+    size_t max_allowed = 6;
+    std::vector<int> key;       // -1 empty
+    std::vector<unsigned> info;
+    size_t n = 0;
+
+    static uint64_t mix(uint64_t x) {
+        unsigned __int128 p = (unsigned __int128)x * (unsigned __int128)0xde5fb9d2630458e9ULL;
+        return (uint64_t)(p >> 64) + (uint64_t)p;
+    }
+    size_t buffer() const {
+        size_t m = (size_t)slots * 80 / 100;
+        return (size_t)slots + (m < 255 ? m : 255);
+    }
+    void init(int s) {
+        slots = s;
+        inc = 32;
+        shift = 59;
+        max_allowed = (size_t)s * 80 / 100;
+        key.assign(buffer() + 8, -1);
+        info.assign(buffer() + 8, 0);
+        info[buffer()] = 1;
+        n = 0;
+    }
+    void place(int k, bool rehash) {
+        uint64_t h = mix((uint64_t)(int64_t)(char)k);
+        unsigned inf = inc + (unsigned)(h >> shift);
+        size_t idx = (size_t)(h & (uint64_t)(slots - 1));
+        if (rehash) {
+            while (inf <= info[idx]) {
+                idx++;
+                inf += inc;
+            }
+        } else {
+            while (inf < info[idx]) {
+                idx++;
+                inf += inc;
+            }
+            while (inf == info[idx]) {  // equal-info residents are other keys: go past them
+                idx++;
+                inf += inc;
+            }
+        }
+        size_t ins = idx;
+        unsigned ins_info = rehash ? (inf & 0xffu) : inf;
+        if (ins_info + inc > 0xff) max_allowed = 0;
+        while (info[idx] != 0) {
+            idx++;
+            inf += inc;
+        }
+        while (idx != ins) {
+            key[idx] = key[idx - 1];
+            info[idx] = (info[idx - 1] + inc) & 0xffu;
+            if (info[idx] + inc > 0xff) max_allowed = 0;
+            idx--;
+        }
+        key[ins] = k;
+        info[ins] = ins_info & 0xffu;
+        n++;
+    }
+    bool try_increase_info() {
+        if (inc <= 2) return false;
+        inc >>= 1;
+        shift++;
+        size_t words = buffer() / 8;
+        for (size_t i = 0; i < words * 8; i++) info[i] = (info[i] >> 1) & 0x7fu;
+        max_allowed = (size_t)slots * 80 / 100;
+        return true;
+    }
+    void insert(int k) {
+        if (n >= max_allowed) {
+            if (!(n < (size_t)slots * 80 / 100 && try_increase_info())) {
+                std::vector<int> ok = key;
+                std::vector<unsigned> oi = info;
+                size_t ob = buffer();
+                init(slots * 2);
+                for (size_t i = 0; i < ob; i++)
+                    if (oi[i] != 0) {
+                        if (max_allowed == 0 && !try_increase_info()) throw std::runtime_error("rh overflow");
+                        place(ok[i], true);
+                    }
+            }
+        }
+        place(k, false);
+    }
+};
+}  // namespace
+
+std::vector<char> rh_char_iteration_order(const std::vector<char>& inserted) {
+    CharTable t;
+    t.init(8);
+    for (char c : inserted) t.insert((int)c);
+    std::vector<char> out;
+    for (size_t i = 0; i < t.buffer(); i++)
+        if (t.info[i] != 0) out.push_back((char)t.key[i]);
+    return out;
+}
+
+HVar dense_to_hvar(const DenseBase& d) {
+    HVar v;
+    v.cnt = d.cnt;
+    v.fwd = d.fwd;
+    v.rev = d.rev;
+    v.hicnt = d.hi;
+    v.locnt = d.lo;
+    v.pmean = (double)d.sum_tp;
+    v.qmean = (double)d.sum_q;
+    v.mqmean = (double)d.sum_mq;
+    v.nm = (double)d.sum_nm;
+    v.pstd = (d.flags & 2u) != 0;
+    v.qstd = (d.flags & 4u) != 0;
+    return v;
+}
+
+void reduce_events(const Event* ev, size_t n, const char* arena, double goodq, HostPileup& P) {
+    (void)goodq;
+    std::vector<size_t> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    auto cls = [&](const Event& e) { return e.kind == 1 ? 1 : 0; };
+    auto cmp = [&](size_t a, size_t b) {
+        const Event& x = ev[a];
+        const Event& y = ev[b];
+        if (cls(x) != cls(y)) return cls(x) < cls(y);
+        if (x.pos != y.pos) return x.pos < y.pos;
+        int l = std::min(x.desc_len, y.desc_len);
+        int c = memcmp(arena + x.desc_off, arena + y.desc_off, (size_t)l);
+        if (c != 0) return c < 0;
+        if (x.desc_len != y.desc_len) return x.desc_len < y.desc_len;
+        if (x.ord != y.ord) return x.ord < y.ord;
+        return x.seq < y.seq;
+    };
+    std::sort(idx.begin(), idx.end(), cmp);
+    size_t i = 0;
+    while (i < n) {
+        const Event& h = ev[idx[i]];
+        std::string desc(arena + h.desc_off, h.desc_len);
+        int c = cls(h);
+        size_t j = i;
+        HVar v;
+        bool created = false;
+        for (; j < n; j++) {
+            const Event& e = ev[idx[j]];
+            if (cls(e) != c || e.pos != h.pos || e.desc_len != h.desc_len ||
+                memcmp(arena + e.desc_off, arena + h.desc_off, h.desc_len) != 0)
+                break;
+            if (e.kind == 3) {  // anchor subtraction against an existing entry
+                if (!created) continue;
+                v.cnt--;
+                if (e.dir)
+                    v.rev--;
+                else
+                    v.fwd--;
+                v.pmean -= e.tp;
+                v.qmean -= e.q;
+                v.mqmean -= e.mapq;
+                v.nm -= e.nmv;
+                if (e.hiq)
+                    v.hicnt--;
+                else
+                    v.locnt--;
+                continue;
+            }
+            created = true;
+            if (e.kind == 2) {  // addCnt form
+                v.cnt++;
+                if (e.dir)
+                    v.rev++;
+                else
+                    v.fwd++;
+                v.pmean += e.rp;
+                v.qmean += e.q;
+                v.mqmean += e.mapq;
+                v.nm += e.nmv;
+                if (e.hiq)
+                    v.hicnt++;
+                else
+                    v.locnt++;
+                continue;
+            }
+            // match-run / deletion / insertion form (pp/pq running comparison)
+            if (e.dir)
+                v.rev++;
+            else
+                v.fwd++;
+            v.cnt++;
+            if (!v.pstd && v.pp != 0 && e.tp != v.pp) v.pstd = true;
+            if (!v.qstd && v.pq != 0 && e.q != v.pq) v.qstd = true;
+            v.pmean += e.tp;
+            v.qmean += e.q;
+            v.mqmean += e.mapq;
+            v.pp = e.tp;
+            v.pq = e.q;
+            v.nm += e.nmv;
+            if (e.hiq)
+                v.hicnt++;
+            else
+                v.locnt++;
+            if (e.cntflags & 1) P.delCount[h.pos][desc]++;
+            if (e.cntflags & 2) P.insCount[h.pos][desc]++;
+            if (e.cntflags & 4) P.mnp[h.pos][desc]++;
+        }
+        if (created) {
+            if (c == 1)
+                P.ins[h.pos][desc] = v;
+            else
+                P.nd[h.pos][desc] = v;
+        }
+        i = j;
+    }
+}
+
+void reduce_clips(const ClipRec* cl, size_t n, const char* clip_arena, double goodq, HostPileup& P) {
+    std::vector<size_t> idx(n);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+        if (cl[a].side != cl[b].side) return cl[a].side < cl[b].side;
+        if (cl[a].pos != cl[b].pos) return cl[a].pos < cl[b].pos;
+        return cl[a].ord < cl[b].ord;
+    });
+    const double gq = (double)(float)goodq;
+    auto add = [&](HVar& v, bool dir, int rp, double q, int mapq, int nm) {
+        v.cnt++;
+        if (dir)
+            v.rev++;
+        else
+            v.fwd++;
+        v.pmean += rp;
+        v.qmean += q;
+        v.mqmean += mapq;
+        v.nm += nm;
+        if (q >= gq)
+            v.hicnt++;
+        else
+            v.locnt++;
+    };
+    for (size_t k : idx) {
+        const ClipRec& r = cl[k];
+        HSclip& s = (r.side == 5 ? P.sc5 : P.sc3)[r.pos];
+        const char* bases = clip_arena + r.bases_off;
+        const char* quals = bases + r.nhq;
+        for (int i = 0; i < r.nhq; i++) {
+            char ch = bases[i];
+            auto& lst = s.nt[i];
+            bool found = false;
+            for (auto& pr : lst)
+                if (pr.first == ch) {
+                    pr.second++;
+                    found = true;
+                }
+            if (!found) lst.push_back({ch, 1});
+            int rp = r.side == 5 ? r.nhq - 1 - i : r.nhq - i;
+            add(s.seq[i][ch], r.dir != 0, rp, (double)(uint8_t)quals[i], r.mapq, r.nm);
+        }
+        add(s.v, r.dir != 0, r.cel, r.sumq / (double)r.nhq, r.mapq, r.nm);
+    }
+}
+
+std::string dump_pileup(const HostPileup& P) {
+    std::string o;
+    char buf[640];
+    auto line = [&](const char* tag, int p, const std::string& d, const HVar& v) {
+        snprintf(buf, sizeof buf, "%s\t%d\t%s\t%d\t%d\t%d\t%d\t%d\t%.17g\t%.17g\t%.17g\t%.17g\t%d\t%d\t%d\n", tag, p, d.c_str(),
+                 v.cnt, v.fwd, v.rev, v.hicnt, v.locnt, v.pmean, v.qmean, v.mqmean, v.nm, (int)v.pstd, (int)v.qstd,
+                 v.extracnt);
+        o += buf;
+    };
+    // nonInsertionVariants: positions with any entry (dense or not)
+    std::map<int, std::map<std::string, HVar>> all;
+    for (int i = 0; i < P.dense_n; i++) {
+        const DenseRow& r = P.rows[(size_t)i];
+        for (int b = 0; b < 4; b++)
+            if (r.b[b].flags & 1u) all[P.dense_lo + i][std::string(1, index_base(b))] = dense_to_hvar(r.b[b]);
+    }
+    for (auto& e : P.nd)
+        for (auto& kv : e.second) all[e.first][kv.first] = kv.second;
+    for (auto& e : all)
+        for (auto& kv : e.second) line("V", e.first, kv.first, kv.second);
+    for (auto& e : P.ins)
+        for (auto& kv : e.second) line("I", e.first, kv.first, kv.second);
+    for (int i = 0; i < P.dense_n; i++)
+        if (P.rows[(size_t)i].cov != 0) {
+            snprintf(buf, sizeof buf, "C\t%d\t%d\n", P.dense_lo + i, P.rows[(size_t)i].cov);
+            o += buf;
+        }
+    auto dump_sc = [&](const char* tag, const std::map<int, HSclip>& m) {
+        for (auto& e : m) {
+            const HSclip& s = e.second;
+            snprintf(buf, sizeof buf, "%s\t%d\t%d\t%d\t%d\t%d\t%d\t%.17g\t%.17g\t%.17g\t%.17g\t%zu\n", tag, e.first, s.v.cnt,
+                     s.v.fwd, s.v.rev, s.v.hicnt, s.v.locnt, s.v.pmean, s.v.qmean, s.v.mqmean, s.v.nm, s.nt.size());
+            o += buf;
+            for (auto& nt : s.nt) {
+                o += std::string(tag) + "n\t" + std::to_string(e.first) + "\t" + std::to_string(nt.first);
+                std::vector<char> ins;
+                for (auto& pr : nt.second) ins.push_back(pr.first);
+                for (char b : rh_char_iteration_order(ins)) {
+                    int cnt = 0;
+                    for (auto& pr : nt.second)
+                        if (pr.first == b) cnt = pr.second;
+                    o += "\t";
+                    o += b;
+                    o += ":" + std::to_string(cnt);
+                }
+                o += "\n";
+            }
+        }
+    };
+    dump_sc("S5", P.sc5);
+    dump_sc("S3", P.sc3);
+    for (auto& e : P.delCount)
+        for (auto& d : e.second) o += "DC\t" + std::to_string(e.first) + "\t" + d.first + "\t" + std::to_string(d.second) + "\n";
+    for (auto& e : P.insCount)
+        for (auto& d : e.second) o += "IC\t" + std::to_string(e.first) + "\t" + d.first + "\t" + std::to_string(d.second) + "\n";
+    for (auto& e : P.mnp)
+        for (auto& d : e.second) o += "MC\t" + std::to_string(e.first) + "\t" + d.first + "\t" + std::to_string(d.second) + "\n";
+    o += "L\t" + std::to_string(P.maxReadLength) + "\n";
+    return o;
+}
+
+}  // namespace rv

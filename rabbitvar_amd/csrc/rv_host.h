@@ -1,0 +1,60 @@
+// Host-side bookkeeping of the engine: ordered reduction of the walk's event and clip records
+// into per-allele records, the soft-clip aggregates, and text helpers.  None of this re-runs a
+// device stage; it consumes what the device produced.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "rv_common.h"
+
+namespace rv {
+
+struct HVar {
+    int cnt = 0, fwd = 0, rev = 0, hicnt = 0, locnt = 0, extracnt = 0;
+    double pmean = 0, qmean = 0, mqmean = 0, nm = 0;
+    bool pstd = false, qstd = false;
+    int pp = 0;
+    double pq = 0;
+};
+
+// Iteration order of a robin_hood::unordered_map<char, int> (robin_hood 3.4.3, as vendored by the
+// reference in include/robin_hood.h) that only ever received insertions, given the keys in
+// insertion order.  The reference's soft-clip consensus breaks ties in that order
+// (include/VariationUtils.h:431-443).
+std::vector<char> rh_char_iteration_order(const std::vector<char>& inserted);
+
+struct HSclip {
+    HVar v;
+    std::map<int, std::vector<std::pair<char, int>>> nt;  // per clip index: (base, count) in first-seen order
+    std::map<int, std::map<char, HVar>> seq;
+    std::string sequence;
+    bool used = false;
+};
+
+struct HostPileup {
+    int dense_lo = 0, dense_n = 0;
+    std::vector<DenseRow> rows;                           // dense single-base rows (window)
+    std::map<int, std::map<std::string, HVar>> nd;        // other nonInsertionVariants entries
+    std::map<int, std::map<std::string, HVar>> ins;       // insertionVariants
+    std::map<int, HSclip> sc5, sc3;
+    std::map<int, std::map<std::string, int>> delCount, insCount, mnp;
+    int maxReadLength = 0;
+};
+
+// Reduces the walk's records in read order (ord, seq) per (map, position, descriptor).
+void reduce_events(const Event* ev, size_t n, const char* arena, double goodq, HostPileup& P);
+void reduce_clips(const ClipRec* cl, size_t n, const char* clip_arena, double goodq, HostPileup& P);
+
+// HVar view of a dense row's base (pileup-stage values)
+HVar dense_to_hvar(const DenseBase& d);
+
+// Text dump of the pileup-stage state, in the oracle's dump layout (tests compare the two).
+std::string dump_pileup(const HostPileup& P);
+
+std::string fmt_f(double v);  // std::to_string(double)
+
+}  // namespace rv

@@ -1,0 +1,253 @@
+// Per-item bodies of the pileup kernels (one read, one dense row, one anchor).  The kernels in
+// rv_pileup.hip map one thread to one item.
+//
+// Reference stages: src/recordPreprocessor.cpp:101-180 (filters), src/parseCigar.cpp:497-628
+// (per-read setup, NM filter, modifyCigar, cleanupCigar, drop rules), :629-972 (walk),
+// :1471-1490 (insertion-anchor subtraction).
+#pragma once
+#include "rv_walk.h"
+
+namespace rv {
+
+// Read batch in device memory, structure of arrays (the region iterator's records in order)
+struct ReadsDev {
+    const int32_t* pos0;
+    const uint16_t* flag;
+    const uint8_t* mapq;
+    const uint8_t* has_nm;
+    const int32_t* nm;
+    const int32_t* mpos0;
+    const int32_t* tid;
+    const int32_t* mtid;
+    const int64_t* seq_off;
+    const int64_t* cig_off;
+    const uint8_t* bases;
+    const uint8_t* quals;
+    const uint32_t* cigar;
+    int64_t n;
+};
+
+__device__ __forceinline__ uint32_t packed_seq_word(const uint8_t* seq, int lseq, int w) {
+    uint32_t v = 0;
+    for (int b = 0; b < 4; b++) {
+        int byte = 4 * w + b;
+        int i0 = 2 * byte, i1 = 2 * byte + 1;
+        uint32_t hi = i0 < lseq ? (uint32_t)nt16_code((char)seq[i0]) : 0u;
+        uint32_t lo = i1 < lseq ? (uint32_t)nt16_code((char)seq[i1]) : 0u;
+        if (i0 < lseq) v |= ((hi << 4) | lo) << (8 * b);
+    }
+    return v;
+}
+
+__device__ inline void prep_one(int64_t r, const ReadsDev& R, const Window& W, const Config& cfg, PrepRec* prep,
+                                uint32_t* cig, WalkCounters* ctr, int* max_span) {
+    PrepRec p;
+    p.status = RD_PARSE;
+    p.flags = 0;
+    p.nm_eff = 0;
+    p.rlen1 = 0;
+    p.total_len = 0;
+    const uint16_t flag = R.flag[r];
+    const int64_t so = R.seq_off[r];
+    const int lseq = (int)(R.seq_off[r + 1] - so);
+    const int64_t co = R.cig_off[r];
+    const int ncig = (int)(R.cig_off[r + 1] - co);
+    p.lseq = lseq;
+    p.position = R.pos0[r] + 1;
+    p.ncig = 0;
+    uint32_t* rec = cig + (size_t)r * MAXC;
+    // reach of the read around its alignment: used to size the dense window
+    {
+        int reflen = 0;
+        for (int i = 0; i < ncig; i++) {
+            int o = cop(R.cigar[co + i]);
+            if (o == OP_M || o == OP_D || o == OP_N || o == OP_EQ || o == OP_X) reflen += clen(R.cigar[co + i]);
+        }
+        atomicMax(max_span, lseq + reflen + 8);
+    }
+    if ((flag & cfg.samfilter) != 0 || (int)R.mapq[r] < cfg.mappingQuality || lseq == 1) {
+        p.status = RD_FILTERED;
+        prep[r] = p;
+        return;
+    }
+    if (ncig <= 0) {
+        p.status = RD_NOWALK;
+        prep[r] = p;
+        return;
+    }
+    int idlen = 0;
+    for (int i = 0; i < ncig; i++) {
+        int o = cop(R.cigar[co + i]);
+        if (o == OP_I || o == OP_D) idlen += clen(R.cigar[co + i]);
+    }
+    int tnm = 0;
+    if (R.has_nm[r]) {
+        tnm = R.nm[r] - idlen;
+        if (tnm > cfg.mismatch) {
+            p.status = RD_NM;
+            prep[r] = p;
+            return;
+        }
+    } else if (flag & 4) {
+        p.status = RD_NM;
+        prep[r] = p;
+        return;
+    }
+    p.nm_eff = tnm;
+    // emulated record: CIGAR words, then the packed SEQ bytes as little-endian words
+    const uint8_t* seq = R.bases + so;
+    int seq_words = ((lseq + 1) / 2 + 3) / 4;
+    int nrec = ncig + seq_words;
+    if (nrec > MAXC) nrec = MAXC;
+    if (ncig + 4 > MAXC) p.flags |= 2;
+    for (int i = 0; i < nrec; i++) rec[i] = i < ncig ? R.cigar[co + i] : packed_seq_word(seq, lseq, i - ncig);
+    for (int i = nrec; i < MAXC; i++) rec[i] = 0;
+    int n_cigar = ncig;
+    int position = R.pos0[r] + 1;
+    if (cfg.performLocalRealignment) {
+        RefView rf{W.ref, W.ref_lo, W.ref_hi};
+        ReadView rd{(const char*)seq, R.quals + so, lseq};
+        int n_new = 0, fl = 0;
+        if (modify_cigar_record(rf, rd, position, rec, nrec, ncig, idlen, n_new, fl)) {
+            n_cigar = n_new;
+            p.flags |= 1;
+        }
+        p.flags |= fl;
+    }
+    // cleanupCigar: leading/trailing I become S until the first M/=/X
+    {
+        bool none = true;
+        for (int i = 0; i < n_cigar && none; i++) {
+            int o = cop(rec[i]);
+            if (o == OP_I)
+                rec[i] = (rec[i] & ~0xfu) | OP_S;
+            else if (o == OP_M || o == OP_EQ || o == OP_X)
+                none = false;
+        }
+        none = true;
+        for (int i = n_cigar - 1; i >= 0 && none; i--) {
+            int o = cop(rec[i]);
+            if (o == OP_I)
+                rec[i] = (rec[i] & ~0xfu) | OP_S;
+            else if (o == OP_M || o == OP_EQ || o == OP_X)
+                none = false;
+        }
+    }
+    p.position = position;
+    p.ncig = n_cigar;
+    if (n_cigar <= 0) {
+        p.status = RD_NOWALK;
+        prep[r] = p;
+        return;
+    }
+    if (cop(rec[0]) == OP_S && clen(rec[0]) >= 10 && cop(rec[n_cigar - 1]) == OP_S && clen(rec[n_cigar - 1]) >= 10) {
+        p.status = RD_NOWALK;
+        prep[r] = p;
+        return;
+    }
+    int rlen1 = 0, tot = 0;
+    for (int i = 0; i < n_cigar; i++) {
+        int o = cop(rec[i]), l = clen(rec[i]);
+        if (o == OP_M || o == OP_I) rlen1 += l;
+        if (o == OP_M || o == OP_I || o == OP_S) tot += l;
+    }
+    p.rlen1 = rlen1;
+    p.total_len = tot;
+    if (cfg.minmatch != 0 && rlen1 < cfg.minmatch) {
+        p.status = RD_NOWALK;
+        prep[r] = p;
+        return;
+    }
+    atomicMax(&ctr->max_read_length, tot);
+    if (flag & 2048) {
+        p.status = RD_NOWALK;
+        prep[r] = p;
+        return;
+    }
+    prep[r] = p;
+}
+
+__device__ inline void fill_row_one(DenseRow* rows, int i) {
+    DenseRow z;
+    for (int b = 0; b < 4; b++) {
+        z.b[b].cnt = z.b[b].fwd = z.b[b].rev = z.b[b].hi = z.b[b].lo = 0;
+        z.b[b].sum_tp = z.b[b].sum_q = z.b[b].sum_mq = z.b[b].sum_nm = 0;
+        z.b[b].first_tpq = 0;
+        z.b[b].flags = 0;
+        z.b[b].min_ord = 0xffffffffu;
+        z.b[b].last_tp = 0;
+        z.b[b].pad[0] = z.b[b].pad[1] = z.b[b].pad[2] = 0;
+    }
+    z.cov = 0;
+    z.rowflags = 0;
+    z.min_ord_pos = 0xffffffffu;
+    for (int k = 0; k < 5; k++) z.pad[k] = 0;
+    rows[i] = z;
+}
+
+__device__ inline void walk_one(int64_t r, const ReadsDev& R, const Window& W, const Config& cfg, const PrepRec* prep,
+                                const uint32_t* cig, WalkSink& sink) {
+    const PrepRec p = prep[r];
+    if (p.status != RD_PARSE) return;
+    Walker w;
+    w.c = &cfg;
+    w.Rs = W.region_start;
+    w.Re = W.region_end;
+    w.chrLen = W.chr_len;
+    w.ref.ref = W.ref;
+    w.ref.lo = W.ref_lo;
+    w.ref.hi = W.ref_hi;
+    w.ref.ctr = sink.ctr;
+    w.S = &sink;
+    const int64_t so = R.seq_off[r];
+    w.seq = (const char*)(R.bases + so);
+    w.qual = R.quals + so;
+    w.lseq = p.lseq;
+    {
+        int nb = (p.lseq + 1) / 2;
+        int last = nb - 1;
+        int i0 = 2 * last, i1 = 2 * last + 1;
+        int hi = i0 < p.lseq ? nt16_code(w.seq[i0]) : 0;
+        int lo = i1 < p.lseq ? nt16_code(w.seq[i1]) : 0;
+        w.qual_before0 = nb > 0 ? (uint8_t)((hi << 4) | lo) : 0;
+    }
+    w.rec = cig + (size_t)r * MAXC;
+    w.rec_n = MAXC;
+    w.n_cigar = p.ncig;
+    w.ord = (int)r;
+    w.evseq = 0;
+    w.mapq = R.mapq[r];
+    w.nm = p.nm_eff;
+    w.dir = (R.flag[r] & 0x10) != 0;
+    w.position = p.position;
+    w.bad = false;
+    int reflen = 0;
+    const int64_t co = R.cig_off[r];
+    const int ncig = (int)(R.cig_off[r + 1] - co);
+    for (int i = 0; i < ncig; i++) {
+        int o = cop(R.cigar[co + i]);
+        if (o == OP_M || o == OP_D || o == OP_N || o == OP_EQ || o == OP_X) reflen += clen(R.cigar[co + i]);
+    }
+    w.walk(p.rlen1, p.total_len, R.flag[r], R.tid[r], R.mtid[r], R.mpos0[r] + 1, reflen, R.pos0[r]);
+}
+
+// Insertion-anchor subtraction (parseCigar.cpp:1471-1490): the reference subtracts from the anchor
+// base's entry only when that entry exists at that moment, i.e. when some read with an ordinal <=
+// the inserting read created it (a creation by the same read always precedes its insertion).
+__device__ inline void anchor_one(unsigned i, const Anchor* A, DenseRow* rows, int dense_lo, int dense_n, const Config& cfg) {
+    Anchor a = A[i];
+    int k = a.pos - dense_lo;
+    if (k < 0 || k >= dense_n) return;
+    DenseBase& d = rows[k].b[a.base];
+    if (d.min_ord > (unsigned)a.ord) return;
+    atomicSub(&d.cnt, 1);
+    atomicSub(a.dir ? &d.rev : &d.fwd, 1);
+    atomicSub(&d.sum_tp, a.tp);
+    atomicSub(&d.sum_q, a.q);
+    atomicSub(&d.sum_mq, a.mapq);
+    atomicSub(&d.sum_nm, a.nmv);
+    atomicSub((double)a.q >= cfg.goodq ? &d.hi : &d.lo, 1);
+}
+
+
+}  // namespace rv

@@ -1,0 +1,824 @@
+// Per-read CIGAR walk into the pileup (device).
+//
+// Restates CigarParser::parseCigar's walk (reference src/parseCigar.cpp:629-972) and its handlers
+// process_softclip (:1112-1301), process_insertion (:1303-1523), process_deletion (:1525-1727),
+// findOffset (:208-256), appendSegments (:1756-1815), adjInsPos (include/VariationUtils.h:99-113).
+//
+// Output routing:
+//   * single-base A/C/G/T alleles -> DenseRow integer atomics (exact in any order);
+//   * every other allele descriptor, insertions, and non-ACGT single bases -> Event records that
+//     the engine reduces in read order (their FP64 quality sums are order-sensitive);
+//   * soft clips that reach Sclip accumulation -> ClipRec + clip bases/qualities;
+//   * the insertion-anchor subtraction (parseCigar.cpp:1471-1490) -> Anchor records, applied after
+//     the walk once every entry's creating read ordinal is known.
+#pragma once
+#include "rv_cigar.h"
+
+namespace rv {
+
+constexpr int DCAP = 640;  // allele descriptor capacity
+typedef FStr<DCAP> DStr;
+
+struct Anchor {
+    int32_t pos;
+    int32_t ord;
+    int32_t base;  // dense index 0..3
+    int32_t dir;
+    int32_t tp;
+    int32_t q;
+    int32_t mapq;
+    int32_t nmv;
+};
+
+struct WalkSink {
+    DenseRow* dense;
+    int dense_lo, dense_n;
+    Event* events;
+    unsigned int* n_events;
+    unsigned int max_events;
+    char* arena;
+    unsigned long long* arena_used;
+    unsigned long long arena_cap;
+    ClipRec* clips;
+    unsigned int* n_clips;
+    unsigned int max_clips;
+    char* clip_arena;
+    unsigned long long* clip_used;
+    unsigned long long clip_cap;
+    Anchor* anchors;
+    unsigned int* n_anchors;
+    unsigned int max_anchors;
+    WalkCounters* ctr;
+};
+
+struct RefWin {
+    const char* ref;
+    int lo, hi;
+    WalkCounters* ctr;
+    __device__ bool has(int k) const { return k >= lo && k <= hi; }
+    __device__ char at(int k) const { return ref[k - lo]; }
+    // robin_hood operator[]: a missing key would be created with '\0'
+    __device__ char get(int k) const {
+        if (has(k)) return at(k);
+        atomicAdd(&ctr->ref_insert, 1ull);
+        return '\0';
+    }
+    __device__ bool hasEq(int k, char ch) const { return has(k) && at(k) == ch; }
+    __device__ bool hasNe(int k, char ch) const { return has(k) && at(k) != ch; }
+};
+
+struct Walker {
+    const Config* c;
+    int Rs, Re;          // region
+    int chrLen;
+    RefWin ref;
+    WalkSink* S;
+    // read
+    const char* seq;
+    const uint8_t* qual;
+    uint8_t qual_before0;
+    int lseq;
+    const uint32_t* rec;
+    int rec_n;           // valid emulated words
+    int n_cigar;
+    int ord;
+    int evseq;
+    uint8_t mapq;
+    int nm;
+    bool dir;
+    int position;
+    int start, offset, cel, rpi, rpe;
+    bool bad;
+
+    __device__ uint32_t cw(int i) const { return (i >= 0 && i < rec_n) ? rec[i] : 0u; }
+    __device__ int op(int i) const { return cop(cw(i)); }
+    __device__ int len(int i) const { return clen(cw(i)); }
+    __device__ int q(int i) {
+        if (i >= 0 && i < lseq) return qual[i];
+        if (i == -1) return qual_before0;
+        bad = true;
+        return 0;
+    }
+    __device__ char sq(int i) const { return (i >= 0 && i < lseq) ? seq[i] : '\0'; }
+    __device__ void sub(DStr& d, int p, int n) {
+        if (p < 0 || p > lseq) {
+            bad = true;
+            return;
+        }
+        for (int i = p; i < p + n && i < lseq; i++) d.push(seq[i]);
+    }
+
+    __device__ DenseRow* row(int pos) {
+        int i = pos - S->dense_lo;
+        if (i < 0 || i >= S->dense_n) {
+            bad = true;
+            return nullptr;
+        }
+        return &S->dense[i];
+    }
+    __device__ void touch_pos(DenseRow* r) {
+        if (!(r->rowflags & 1u)) atomicOr(&r->rowflags, 1u);
+        atomicMin(&r->min_ord_pos, (unsigned)ord);
+    }
+    __device__ void cov(int pos, int k = 1) {
+        DenseRow* r = row(pos);
+        if (r) atomicAdd(&r->cov, k);
+    }
+
+    // single-base contribution from the match run (updates pp/pq)
+    __device__ void dense_single(int pos, int b, int tp, int qv, int nmv) {
+        DenseRow* r = row(pos);
+        if (!r) return;
+        touch_pos(r);
+        DenseBase& d = r->b[b];
+        atomicAdd(&d.cnt, 1);
+        atomicAdd(dir ? &d.rev : &d.fwd, 1);
+        atomicAdd((double)qv >= c->goodq ? &d.hi : &d.lo, 1);
+        atomicAdd(&d.sum_tp, tp);
+        atomicAdd(&d.sum_q, qv);
+        atomicAdd(&d.sum_mq, (int)mapq);
+        atomicAdd(&d.sum_nm, nmv);
+        atomicMin(&d.min_ord, (unsigned)ord);
+        unsigned want = 1u | ((tp <= 0 || qv <= 0) ? 8u : 0u);
+        if ((d.flags & want) != want) atomicOr(&d.flags, want);
+        if ((d.flags & 6u) != 6u) {
+            unsigned v = ((((unsigned)tp & 0xffffffu) << 8) | ((unsigned)qv & 0xffu)) + 1u;
+            unsigned old = atomicCAS(&d.first_tpq, 0u, v);
+            if (old != 0u && old != v) {
+                unsigned o = old - 1u, nv = v - 1u;
+                unsigned add = 0;
+                if ((o >> 8) != (nv >> 8)) add |= 2u;
+                if ((o & 0xffu) != (nv & 0xffu)) add |= 4u;
+                if (add) atomicOr(&d.flags, add);
+            }
+        }
+    }
+    // addCnt-form contribution (soft-clip rescue): no pp/pq update
+    __device__ void dense_addcnt(int pos, int b, int rp, int qv) {
+        DenseRow* r = row(pos);
+        if (!r) return;
+        touch_pos(r);
+        DenseBase& d = r->b[b];
+        atomicAdd(&d.cnt, 1);
+        atomicAdd(dir ? &d.rev : &d.fwd, 1);
+        atomicAdd((double)qv >= (double)(float)c->goodq ? &d.hi : &d.lo, 1);
+        atomicAdd(&d.sum_tp, rp);
+        atomicAdd(&d.sum_q, qv);
+        atomicAdd(&d.sum_mq, (int)mapq);
+        atomicAdd(&d.sum_nm, nm);
+        atomicMin(&d.min_ord, (unsigned)ord);
+        if (!(d.flags & 1u)) atomicOr(&d.flags, 1u);
+    }
+
+    __device__ void event(int kind, int pos, const DStr& d, int tp, double qv, int nmv, int cntflags, int rp, bool hiq) {
+        if (d.overflow) {
+            atomicAdd(&S->ctr->overflow, 1ull);
+            return;
+        }
+        unsigned idx = atomicAdd(S->n_events, 1u);
+        unsigned long long off = atomicAdd(S->arena_used, (unsigned long long)d.n);
+        if (idx >= S->max_events || off + (unsigned long long)d.n > S->arena_cap) {
+            atomicAdd(&S->ctr->overflow, 1ull);
+            return;
+        }
+        for (int i = 0; i < d.n; i++) S->arena[off + i] = d.c[i];
+        Event e;
+        e.pos = pos;
+        e.ord = ord;
+        e.seq = evseq++;
+        e.tp = tp;
+        e.q = qv;
+        e.mapq = mapq;
+        e.nmv = nmv;
+        e.desc_off = (uint32_t)off;
+        e.desc_len = (uint16_t)d.n;
+        e.kind = (uint8_t)kind;
+        e.dir = dir ? 1 : 0;
+        e.cntflags = (uint8_t)cntflags;
+        e.hiq = hiq ? 1 : 0;
+        e.pad[0] = e.pad[1] = 0;
+        e.rp = rp;
+        S->events[idx] = e;
+        if (kind == 0 || kind == 2) {
+            DenseRow* r = row(pos);
+            if (r) touch_pos(r);
+        }
+    }
+
+    // a nonInsertionVariants contribution from the match run / deletion handlers
+    __device__ void var_contrib(int pos, const DStr& d, int tp, double qv, int nmv, int cntflags) {
+        if (d.n == 1) {
+            int b = base_index(d.c[0]);
+            if (b >= 0) {
+                dense_single(pos, b, tp, (int)qv, nmv);
+                return;
+            }
+        }
+        event(0, pos, d, tp, qv, nmv, cntflags | 8, 0, qv >= c->goodq);
+    }
+
+    __device__ int tp_of(int rlen1) const { return rpe < rlen1 - rpe ? rpe + 1 : rlen1 - rpe; }
+
+    __device__ bool isNextMatched(int ci) const { return c->performLocalRealignment && n_cigar > ci + 1 && op(ci + 1) == OP_M; }
+    __device__ bool isNextInsertion(int ci) const {
+        return c->performLocalRealignment && n_cigar > ci + 1 && op(ci + 1) == OP_I;
+    }
+    __device__ bool isNextAfterNumMatched(int ci, int number) const { return n_cigar > ci + number && op(ci + 1) == OP_M; }
+    __device__ bool isIDWithNextMatched(int ci) const {
+        return c->performLocalRealignment && n_cigar > ci + 2 && len(ci + 1) <= c->vext && op(ci + 1) == OP_M &&
+               (op(ci + 2) == OP_I || op(ci + 2) == OP_D) && op(ci + 3) != OP_I && op(ci + 3) != OP_D;
+    }
+
+    struct Off {
+        int offset;
+        int from;  // read offset of the appended sequence
+        long q_sum;
+        int nm;
+    };
+    __device__ Off findOffset(int refPos, int readPos, int cigLen) {
+        Off o{0, readPos, 0, 0};
+        int vsn = 0;
+        for (int vi = 0; vsn <= c->vext && vi < cigLen; vi++) {
+            if (sq(readPos + vi) == 'N') break;
+            if (q(readPos + vi) < c->goodq) break;
+            if (ref.has(refPos + vi)) {
+                if (sq(readPos + vi) != ref.at(refPos + vi)) {
+                    o.offset = vi + 1;
+                    o.nm++;
+                    vsn = 0;
+                } else {
+                    vsn++;
+                }
+            }
+        }
+        if (o.offset > 0) {
+            for (int i = readPos; i < readPos + o.offset; i++) o.q_sum += q(i);
+            for (int k = 0; k < o.offset; k++) cov(refPos + k);
+        }
+        return o;
+    }
+
+    __device__ void appendSegments(int ci, DStr& desc, long& qs, int& qc, int mLen, int indelLen, int begin, bool isIns) {
+        desc.push('#');
+        sub(desc, begin, mLen);
+        for (int i = begin; i < begin + mLen; i++) qs += q(i);
+        desc.push('^');
+        if (op(ci + 2) == OP_I)
+            sub(desc, begin + mLen, indelLen);
+        else
+            desc.append_int(indelLen);
+        if (isIns) {
+            if (op(ci + 2) == OP_I) {
+                for (int i = begin + mLen; i < begin + mLen + indelLen; i++) qs += q(i);
+                qc += indelLen;
+            } else {
+                qs += q(begin + mLen);
+                qc += 1;
+            }
+        } else if (op(ci + 2) == OP_I) {
+            for (int i = begin + mLen; i < begin + mLen + indelLen; i++) qs += q(i);
+            qc += indelLen;
+        }
+    }
+
+    __device__ void emit_clip(int side, int pos, int cel_, int nhq, int sumq, int from, int step) {
+        unsigned idx = atomicAdd(S->n_clips, 1u);
+        unsigned long long off = atomicAdd(S->clip_used, (unsigned long long)(2 * nhq));
+        if (idx >= S->max_clips || off + (unsigned long long)(2 * nhq) > S->clip_cap) {
+            atomicAdd(&S->ctr->overflow, 1ull);
+            return;
+        }
+        for (int k = 0; k < nhq; k++) {
+            int ri = from + step * k;
+            S->clip_arena[off + k] = sq(ri);
+            S->clip_arena[off + nhq + k] = (char)q(ri);
+        }
+        ClipRec r;
+        r.pos = pos;
+        r.ord = ord;
+        r.side = side;
+        r.dir = dir ? 1 : 0;
+        r.mapq = mapq;
+        r.nm = nm;
+        r.cel = cel_;
+        r.nhq = nhq;
+        r.sumq = sumq;
+        r.bases_off = (uint32_t)off;
+        S->clips[idx] = r;
+    }
+
+    __device__ void rescue(int pos, int rp, int qv) {
+        char rb = ref.at(pos);
+        int b = base_index(rb);
+        if (b >= 0) {
+            dense_addcnt(pos, b, rp, qv);
+        } else {
+            DStr d;
+            d.clear();
+            d.push(rb);
+            event(2, pos, d, 0, (double)qv, nm, 0, rp, (double)qv >= (double)(float)c->goodq);
+        }
+        cov(pos);
+    }
+
+    __device__ void softclip(int ci, int totalLen) {
+        if (ci == 0) {
+            while (cel - 1 >= 0 && start - 1 > 0 && start - 1 <= chrLen && ref.hasEq(start - 1, sq(cel - 1)) && q(cel - 1) > 10) {
+                rescue(start - 1, cel, q(cel - 1));
+                start--;
+                cel--;
+            }
+            if (cel > 0) {
+                int sumq = 0, nhq = 0, nlq = 0;
+                for (int si = cel - 1; si >= 0; si--) {
+                    if (sq(si) == 'N') break;
+                    int bq = q(si);
+                    if (bq <= 12) nlq++;
+                    if (nlq > 1) break;
+                    sumq += bq;
+                    nhq++;
+                }
+                if (nhq >= 1 && nhq > nlq && start >= Rs && start <= Re) emit_clip(5, start, cel, nhq, sumq, cel - 1, -1);
+            }
+            cel = len(ci);
+        } else if (ci == n_cigar - 1) {
+            while (rpi < lseq && ref.hasEq(start, sq(rpi)) && q(rpi) > 10) {
+                rescue(start, totalLen - rpe, q(rpi));
+                rpi++;
+                start++;
+                cel--;
+                rpe++;
+            }
+            if (rpi < lseq) {
+                int sumq = 0, nhq = 0, nlq = 0;
+                for (int si = 0; si < cel; si++) {
+                    if (sq(rpi + si) == 'N') break;
+                    int bq = q(rpi + si);
+                    if (bq <= 12) nlq++;
+                    if (nlq > 1) break;
+                    sumq += bq;
+                    nhq++;
+                }
+                if (nhq >= 1 && nhq > nlq && start >= Rs && start <= Re) emit_clip(3, start, cel, nhq, sumq, rpi, 1);
+            }
+        }
+        rpi += cel;
+        offset = 0;
+        start = position;
+    }
+
+    __device__ int insertion(int ci, int rlen1) {
+        if ((n_cigar > ci + 1 && op(ci + 1) == OP_N) || (ci > 0 && op(ci - 1) == OP_N)) {
+            rpi += cel;
+            return ci;
+        }
+        DStr desc;
+        desc.clear();
+        sub(desc, rpi, cel);
+        long qs = 0;
+        int qc = 0;
+        for (int i = rpi; i < rpi + cel; i++) qs += q(i);
+        qc += cel;
+        int ss_from = 0, ss_len = 0;
+        int multoffs = 0, multoffp = 0, nmoff = 0;
+        if (isIDWithNextMatched(ci)) {
+            int mLen = len(ci + 1), indelLen = len(ci + 2);
+            int begin = rpi + cel;
+            appendSegments(ci, desc, qs, qc, mLen, indelLen, begin, true);
+            multoffs += mLen + (op(ci + 2) == OP_D ? indelLen : 0);
+            multoffp += mLen + (op(ci + 2) == OP_I ? indelLen : 0);
+            int ci6 = n_cigar > ci + 3 ? len(ci + 3) : 0;
+            if (ci6 != 0 && op(ci + 3) == OP_M) {
+                Off o = findOffset(start + multoffs, rpi + cel + multoffp, ci6);
+                offset = o.offset;
+                ss_from = o.from;
+                ss_len = o.offset;
+                qs += o.q_sum;
+                qc += offset;
+            }
+            ci += 2;
+        } else if (isNextMatched(ci)) {
+            int vsn = 0;
+            for (int vi = 0; vsn <= c->vext && vi < len(ci + 1); vi++) {
+                if (sq(rpi + cel + vi) == 'N') break;
+                if (q(rpi + cel + vi) < c->goodq) break;
+                if (ref.has(start + vi)) {
+                    if (sq(rpi + cel + vi) != ref.at(start + vi)) {
+                        offset = vi + 1;
+                        nmoff++;
+                        vsn = 0;
+                    } else {
+                        vsn++;
+                    }
+                }
+            }
+            if (offset != 0) {
+                ss_from = rpi + cel;
+                ss_len = offset;
+                for (int i = rpi + cel; i < rpi + cel + offset; i++) qs += q(i);
+                qc += offset;
+                for (int k = 0; k < offset; k++) cov(start + k);
+            }
+        }
+        if (offset > 0) {
+            desc.push('&');
+            sub(desc, ss_from, ss_len);
+        }
+        if (start - 1 >= Rs && start - 1 <= Re && desc.find('N') < 0) {
+            int inspos = start - 1;
+            bool allATGC = desc.n > 0;
+            for (int i = 0; i < desc.n; i++)
+                if (base_index(desc.c[i]) < 0) allATGC = false;
+            if (allATGC) {
+                int bi = start - 1;
+                int n = 1;
+                int L = desc.n;
+                int guard = 0;
+                while (ref.get(bi) == desc.c[L - n]) {
+                    n++;
+                    if (n > L) n = 1;
+                    bi--;
+                    if (++guard > (1 << 24)) {
+                        bad = true;
+                        break;
+                    }
+                }
+                if (n > 1) {
+                    // rotate right by n-1: last n-1 characters move to the front
+                    DStr t;
+                    t.clear();
+                    t.append(desc.c + (L - (n - 1)), n - 1);
+                    t.append(desc.c, L - (n - 1));
+                    desc = t;
+                }
+                if (rpi - 1 - (start - 1 - bi) > 0) inspos = bi;
+            }
+            DStr key;
+            key.clear();
+            key.push('+');
+            key.append(desc);
+            int tp = tp_of(rlen1);
+            double tmpq = (double)qs / qc;
+            event(1, inspos, key, tp, tmpq, nm - nmoff, 2 | 8, 0, tmpq >= c->goodq);
+            int index = rpi - 1 - (start - 1 - inspos);
+            if (inspos > position && ref.hasEq(inspos, sq(index))) {
+                char ab = sq(index);
+                int b = base_index(ab);
+                if (b >= 0) {
+                    unsigned ai = atomicAdd(S->n_anchors, 1u);
+                    if (ai < S->max_anchors) {
+                        Anchor a;
+                        a.pos = inspos;
+                        a.ord = ord;
+                        a.base = b;
+                        a.dir = dir ? 1 : 0;
+                        a.tp = tp;
+                        a.q = q(index);
+                        a.mapq = mapq;
+                        a.nmv = nm - nmoff;
+                        S->anchors[ai] = a;
+                    } else {
+                        atomicAdd(&S->ctr->overflow, 1ull);
+                    }
+                } else if (ab != 0) {
+                    DStr d;
+                    d.clear();
+                    d.push(ab);
+                    event(3, inspos, d, tp, (double)q(index), nm - nmoff, 0, 0, (double)q(index) >= c->goodq);
+                }
+            }
+            if (ci == 1 && (op(0) == OP_S || op(0) == OP_H)) atomicAdd(&S->ctr->ttref, 1);
+        }
+        rpi += cel + offset + multoffp;
+        rpe += cel + offset + multoffp;
+        start += offset + multoffs;
+        return ci;
+    }
+
+    __device__ int deletion(int ci, int rlen1) {
+        if ((n_cigar > ci && op(ci + 1) == OP_N) || (ci > 1 && op(ci - 1) == OP_N)) {
+            rpi += cel;
+            return ci;
+        }
+        DStr desc;
+        desc.clear();
+        desc.push('-');
+        desc.append_int(cel);
+        int app_from = 0, app_len = 0;
+        int qlast = q(rpi - 1);
+        long qs = 0;
+        int qc = 0;
+        int multoffs = 0, multoffp = 0, nmoff = 0;
+        if (isIDWithNextMatched(ci)) {
+            int mLen = len(ci + 1), indelLen = len(ci + 2);
+            appendSegments(ci, desc, qs, qc, mLen, indelLen, rpi, false);
+            multoffs += mLen + (op(ci + 2) == OP_D ? indelLen : 0);
+            multoffp += mLen + (op(ci + 2) == OP_I ? indelLen : 0);
+            if (isNextAfterNumMatched(ci, 3)) {
+                int vsn = 0;
+                int tn = rpi + multoffp;
+                int ts = start + multoffs + cel;
+                for (int vi = 0; vsn <= c->vext && vi < len(ci + 3); vi++) {
+                    if (sq(tn + vi) == 'N') break;
+                    if (q(tn + vi) < c->goodq) break;
+                    if (ref.hasEq(ts + vi, 'N')) break;
+                    if (ref.has(ts + vi)) {
+                        if (sq(tn + vi) != ref.at(ts + vi)) {
+                            offset = vi + 1;
+                            nmoff++;
+                            vsn = 0;
+                        } else {
+                            vsn++;
+                        }
+                    }
+                }
+                if (offset != 0) {
+                    app_from = tn;
+                    app_len = offset;
+                    for (int i = tn; i < tn + offset; i++) qs += q(i);
+                    qc += offset;
+                }
+            }
+            ci += 2;
+        } else if (isNextInsertion(ci)) {
+            int insLen = len(ci + 1);
+            desc.push('^');
+            sub(desc, rpi, insLen);
+            for (int i = rpi; i < rpi + insLen; i++) qs += q(i);
+            qc += insLen;
+            multoffp += insLen;
+            ci += 1;
+        } else if (isNextMatched(ci)) {
+            int mLen = len(ci + 1);
+            int vsn = 0;
+            for (int vi = 0; vsn <= c->vext && vi < mLen; vi++) {
+                char sc = sq(rpi + vi);
+                if (sc == 'N') break;
+                if (q(rpi + vi) < c->goodq) break;
+                if (ref.has(start + cel + vi)) {
+                    char rc = ref.at(start + cel + vi);
+                    if (rc == 'N') break;
+                    if (sc != rc) {
+                        offset = vi + 1;
+                        nmoff++;
+                        vsn = 0;
+                    } else {
+                        vsn++;
+                    }
+                }
+            }
+            if (offset != 0) {
+                app_from = rpi;
+                app_len = offset;
+                for (int i = rpi; i < rpi + offset; i++) qs += q(i);
+                qc += offset;
+            }
+        }
+        if (offset > 0) {
+            desc.push('&');
+            sub(desc, app_from, app_len);
+        }
+        if (rpi + offset >= lseq) {
+            qs += qlast;
+            qc += 1;
+        } else {
+            int qo = q(rpi + offset);
+            qs += qlast > qo ? qlast : qo;
+            qc += 1;
+        }
+        if (start >= Rs && start <= Re) {
+            int tp = tp_of(rlen1);
+            double tmpq = (double)qs / qc;
+            event(0, start, desc, tp, tmpq, nm - nmoff, 1 | 8, 0, tmpq >= c->goodq);
+            for (int i = 0; i < cel; i++) cov(start + i);
+        }
+        start += cel + offset + multoffs;
+        rpi += offset + multoffp;
+        rpe += offset + multoffp;
+        return ci;
+    }
+
+    // one M (or =, X, P ...) CIGAR element; returns the possibly advanced ci
+    __device__ int match_run(int ci, int c_op, int rlen1, int totalLen) {
+        int nmoff = 0, moffset = 0;
+        for (int i = offset; i < cel; i++) {
+            bool trim = false;
+            if (c->trimBasesAfter != 0) trim = !dir ? rpi > c->trimBasesAfter : totalLen - rpi > c->trimBasesAfter;
+            const char ch1 = sq(rpi);
+            if (ch1 == 'N') {
+                start++;
+                rpi++;
+                rpe++;
+                continue;
+            }
+            DStr s;
+            s.clear();
+            s.push(ch1);
+            bool swd = false;
+            double qv = q(rpi);
+            int qbases = 1, qibases = 0;
+            int ss_n = 0;
+            DStr ss;
+            ss.clear();
+            while ((start + 1) >= Rs && (start + 1) <= Re && (i + 1) < cel && qv >= c->goodq && ref.hasNe(start, sq(rpi)) &&
+                   ref.at(start) != 'N') {
+                if (q(rpi + 1) < c->goodq + 5) break;
+                char nuc = sq(rpi + 1);
+                if (nuc == 'N') break;
+                if (ref.hasEq(start + 1, 'N')) break;
+                if (ref.get(start + 1) != nuc) {
+                    ss.push(nuc);
+                    qv += q(rpi + 1);
+                    qbases++;
+                    rpi++;
+                    rpe++;
+                    i++;
+                    start++;
+                    nmoff++;
+                } else {
+                    int ssn = 0;
+                    for (int ssi = 1; ssi <= c->vext; ssi++) {
+                        if (i + 1 + ssi >= cel) break;
+                        if (rpi + 1 + ssi < lseq && ref.hasNe(start + 1 + ssi, sq(rpi + 1 + ssi))) {
+                            ssn = ssi + 1;
+                            break;
+                        }
+                    }
+                    if (ssn == 0) break;
+                    if (q(rpi + ssn) < c->goodq + 5) break;
+                    for (int ssi = 1; ssi <= ssn; ssi++) {
+                        ss.push(sq(rpi + ssi));
+                        qv += q(rpi + ssi);
+                        qbases++;
+                    }
+                    rpi += ssn;
+                    rpe += ssn;
+                    i += ssn;
+                    start += ssn;
+                }
+            }
+            ss_n = ss.n;
+            if (ss_n > 0) {
+                s.push('&');
+                s.append(ss);
+            }
+            int ddlen = 0;
+            if (c->performLocalRealignment && cel - i <= c->vext && ci + 1 < n_cigar && op(ci + 1) == OP_D && ref.has(start) &&
+                (ss_n > 0 || sq(rpi) != ref.at(start)) && q(rpi) >= c->goodq) {
+                while (i + 1 < cel) {
+                    s.push(sq(rpi + 1));
+                    qv += q(rpi + 1);
+                    qbases++;
+                    i++;
+                    rpi++;
+                    rpe++;
+                    start++;
+                }
+                int amp = s.find('&');
+                if (amp >= 0) s.erase_at(amp);
+                DStr t;
+                t.clear();
+                t.push('-');
+                t.append_int(len(ci + 1));
+                t.push('&');
+                t.append(s);
+                s = t;
+                swd = true;
+                ddlen = len(ci + 1);
+                ci += 1;
+                if (n_cigar > ci + 1 && op(ci + 1) == OP_I) {
+                    s.push('^');
+                    sub(s, rpi + 1, len(ci + 1));
+                    int next_len = len(ci + 1);
+                    for (int qi = 1; qi <= next_len; qi++) {
+                        qv += q(rpi + 1 + qi);
+                        qibases++;
+                    }
+                    rpi += next_len;
+                    rpe += next_len;
+                    ci += 1;
+                }
+                if (isNextAfterNumMatched(ci, 1)) {
+                    Off o = findOffset(start + ddlen + 1, rpi + 1, len(ci + 1));
+                    if (o.offset != 0) {
+                        moffset = o.offset;
+                        nmoff += o.nm;
+                        s.push('&');
+                        sub(s, o.from, o.offset);
+                    }
+                }
+            } else if (c->performLocalRealignment && cel - i <= c->vext && ci + 1 < n_cigar && op(ci + 1) == OP_I &&
+                       ref.has(start) && (ss_n > 0 || sq(rpi) != ref.at(start)) && q(rpi) >= c->goodq) {
+                while (i + 1 < cel) {
+                    s.push(sq(rpi + 1));
+                    qv += q(rpi + 1);
+                    qbases++;
+                    i++;
+                    rpi++;
+                    rpe++;
+                    start++;
+                }
+                int amp = s.find('&');
+                if (amp >= 0) s.erase_at(amp);
+                int nextLen = len(ci + 1);
+                sub(s, rpi + 1, nextLen);
+                s.insert_at(nextLen, '&');
+                s.insert_at(0, '+');
+                for (int qi = 1; qi <= nextLen; qi++) {
+                    qv += q(rpi + 1 + qi);
+                    qibases++;
+                }
+                rpi += nextLen;
+                rpe += nextLen;
+                ci += 1;
+                qibases--;
+                qbases++;
+            }
+            if (!trim) {
+                const int pos = start - qbases + 1;
+                if (pos >= Rs && pos <= Re) {
+                    int tp = tp_of(rlen1);
+                    qv = qv / (qbases + qibases);
+                    // MNP table entry: ^[ATGC]&[ATGC]+$
+                    bool mnp = false;
+                    if (s.n > 2 && s.c[1] == '&' && base_index(s.c[0]) >= 0) {
+                        mnp = true;
+                        for (int k = 2; k < s.n; k++)
+                            if (base_index(s.c[k]) < 0) mnp = false;
+                    }
+                    var_contrib(pos, s, tp, qv, nm - nmoff, (swd ? 1 : 0) | (mnp ? 4 : 0));
+                    for (int qi = 1; qi <= qbases; qi++) cov(start - qi + 1);
+                    if (swd)
+                        for (int qi = 1; qi < ddlen; qi++) cov(start + qi);
+                }
+            }
+            if (swd) start += ddlen;
+            if (c_op != OP_I) start++;
+            if (c_op != OP_D) {
+                rpi++;
+                rpe++;
+            }
+        }
+        if (moffset != 0) {
+            offset = moffset;
+            rpi += moffset;
+            start += moffset;
+            rpe += moffset;
+        }
+        return ci;
+    }
+
+    __device__ bool skip_overlap(uint16_t flag, int tid, int mtid, int mateStart, int reflen, int pos0) const {
+        bool pairedSame = (flag & 1) && tid == mtid;
+        if (c->uniqueModeAlignmentEnabled && pairedSame && !dir && start >= mateStart) return true;
+        if (c->uniqueModeSecondInPairEnabled && (flag & 1) && pairedSame) {
+            int endpos = pos0 + (reflen > 0 ? reflen : 1);
+            if (position >= mateStart) return start >= mateStart && start <= mateStart + reflen - 1;
+            return start >= mateStart && mateStart <= endpos;
+        }
+        return false;
+    }
+
+    __device__ void walk(int rlen1, int totalLen, uint16_t flag, int tid, int mtid, int mateStart, int reflen, int pos0) {
+        start = position;
+        offset = 0;
+        rpi = rpe = 0;
+        bool need_break = true;
+        for (int ci = 0; ci < n_cigar; ++ci) {
+            if (bad) break;
+            if (need_break && skip_overlap(flag, tid, mtid, mateStart, reflen, pos0)) break;
+            need_break = false;
+            int c_op = op(ci);
+            cel = len(ci);
+            if ((ci == 0 || ci == n_cigar - 1) && c_op == OP_I) c_op = OP_S;
+            if (c_op == OP_N) {
+                start += cel;
+                offset = 0;
+                continue;
+            }
+            if (c_op == OP_S) {
+                softclip(ci, totalLen);
+                continue;
+            }
+            if (c_op == OP_H) {
+                offset = 0;
+                continue;
+            }
+            if (c_op == OP_I) {
+                offset = 0;
+                ci = insertion(ci, rlen1);
+                continue;
+            }
+            if (c_op == OP_D) {
+                offset = 0;
+                ci = deletion(ci, rlen1);
+                continue;
+            }
+            ci = match_run(ci, c_op, rlen1, totalLen);
+            if (start > Re) break;
+        }
+        if (bad) atomicAdd(&S->ctr->walk_errors, 1ull);
+    }
+};
+
+}  // namespace rv

@@ -1,0 +1,443 @@
+// rvsynth: seeded synthetic inputs for the per-region calling path.
+//
+// Writes <out>/ref.fa (one contig, chrS), <out>/reads.rvb (aligned records as a structure-of-
+// arrays read batch, sorted by position) and <out>/truth.tsv (planted variants).
+//
+// Model (per workload):
+//   reference  iid bases, P(G)=P(C)=0.205, P(A)=P(T)=0.295; optional tandem-repeat tracts of
+//              2-6 bp units covering a given fraction of the contig
+//   variants   heterozygous (one haplotype): SNVs, short indels (left-normalised), optional long
+//              deletions; none within 150 bp of a contig end
+//   reads      paired-end, fragment start uniform on the haplotype, insert ~ N(mu, sd) clamped,
+//              read 1 forward / read 2 reverse; Phred ~ round(N(35,5)) clamped to [2,41];
+//              substitution errors at a fixed rate
+//   alignment  the true alignment: M/I/D from the haplotype walk; a read may not start or end with
+//              I or D (clipped instead); reads across long deletions are soft-clipped at the
+//              breakpoint on the shorter side; NM = mismatches + indel bases; MAPQ 60
+//
+// Read-batch file layout ("RVB1"): magic, u32 version, i64 n_reads, i64 n_bases, i64 n_cigar, then
+// i32 pos0[n], u16 flag[n], u8 mapq[n], u8 has_nm[n], i32 nm[n], i32 mpos0[n], i32 tid[n],
+// i32 mtid[n], i64 seq_off[n+1], i64 cigar_off[n+1], u8 bases[nb] (ASCII), u8 quals[nb] (Phred),
+// u32 cigar[nc] (BAM op encoding); every array is zero-padded to a multiple of 8 bytes.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+struct Opts {
+    std::string out = ".";
+    uint64_t seed = 1;
+    int len = 200000;
+    double depth = 30;
+    int readLen = 150;
+    double insMu = 350, insSd = 50;
+    double snvRate = 1e-3;
+    double indelRate = 1e-4;
+    int indelMax = 10;
+    int longDels = 0;
+    int longDelMin = 50, longDelMax = 300;
+    double repeatFrac = 0.0;
+    double errRate = 0.003;
+    double vafLo = 0.5, vafHi = 0.5;  // fraction of fragments drawn from the alt haplotype
+};
+
+struct Variant {
+    int pos;          // 0-based reference position of the first affected base
+    int kind;         // 0 SNV, 1 insertion (before pos), 2 deletion
+    int len;          // deleted length / inserted length
+    std::string alt;  // SNV base or inserted sequence
+    int hap;
+};
+
+const char BASES[4] = {'A', 'C', 'G', 'T'};
+
+char draw_base(std::mt19937_64& g) {
+    double u = std::uniform_real_distribution<double>(0, 1)(g);
+    if (u < 0.295) return 'A';
+    if (u < 0.295 + 0.205) return 'C';
+    if (u < 0.295 + 0.41) return 'G';
+    return 'T';
+}
+
+// haplotype: alt sequence and, for each alt base, its reference position (or -1 when inserted)
+struct Hap {
+    std::string seq;
+    std::vector<int32_t> rpos;
+};
+
+Hap build_hap(const std::string& ref, const std::vector<Variant>& vars, int h) {
+    Hap H;
+    H.seq.reserve(ref.size() + 1024);
+    H.rpos.reserve(ref.size() + 1024);
+    size_t vi = 0;
+    int p = 0;
+    int n = (int)ref.size();
+    std::vector<const Variant*> mine;
+    for (const Variant& v : vars)
+        if (v.hap == h) mine.push_back(&v);
+    while (p < n) {
+        while (vi < mine.size() && mine[vi]->pos < p) vi++;
+        if (vi < mine.size() && mine[vi]->pos == p) {
+            const Variant& v = *mine[vi];
+            vi++;
+            if (v.kind == 0) {
+                H.seq.push_back(v.alt[0]);
+                H.rpos.push_back(p);
+                p++;
+                continue;
+            }
+            if (v.kind == 1) {
+                for (char ch : v.alt) {
+                    H.seq.push_back(ch);
+                    H.rpos.push_back(-1);
+                }
+                H.seq.push_back(ref[(size_t)p]);
+                H.rpos.push_back(p);
+                p++;
+                continue;
+            }
+            p += v.len;  // deletion
+            continue;
+        }
+        H.seq.push_back(ref[(size_t)p]);
+        H.rpos.push_back(p);
+        p++;
+    }
+    return H;
+}
+
+struct Rec {
+    int32_t pos0;
+    uint16_t flag;
+    int32_t nm;
+    int32_t mpos0;
+    std::string bases;
+    std::string quals;
+    std::vector<uint32_t> cigar;
+};
+
+uint32_t cw(int len, int op) { return ((uint32_t)len << 4) | (uint32_t)op; }
+
+// Build the record for haplotype interval [a, b): returns false when the read has no aligned base.
+bool make_record(const std::string& ref, const Hap& H, int a, int b, bool longDelClip,
+                 const std::vector<uint8_t>& longDelMask, Rec& r) {
+    // ops per base: 'M' (rpos>=0), 'I' (rpos==-1); deletions appear as jumps in rpos
+    std::vector<std::pair<int, int>> ops;  // (op, len)
+    auto push = [&](int op, int l) {
+        if (l <= 0) return;
+        if (!ops.empty() && ops.back().first == op)
+            ops.back().second += l;
+        else
+            ops.push_back({op, l});
+    };
+    int lastR = -1;
+    int first = -1;
+    for (int i = a; i < b; i++) {
+        int rp = H.rpos[(size_t)i];
+        if (rp < 0) {
+            push(1, 1);
+            continue;
+        }
+        if (lastR >= 0 && rp > lastR + 1) push(2, rp - lastR - 1);
+        push(0, 1);
+        if (first < 0) first = rp;
+        lastR = rp;
+    }
+    if (first < 0) return false;
+    // leading/trailing I become soft clips; a leading/trailing D cannot occur by construction
+    if (!ops.empty() && ops.front().first == 1) ops.front().first = 4;
+    if (!ops.empty() && ops.back().first == 1) ops.back().first = 4;
+    // long deletions: soft-clip the shorter side of the read at the breakpoint
+    if (longDelClip) {
+        for (size_t k = 0; k < ops.size(); k++) {
+            if (ops[k].first != 2) continue;
+            // reference position right after the deletion
+            int refAt = first;
+            for (size_t t = 0; t < k; t++)
+                if (ops[t].first == 0 || ops[t].first == 2) refAt += ops[t].second;
+            if (!longDelMask[(size_t)refAt]) continue;
+            int leftQ = 0, rightQ = 0;
+            for (size_t t = 0; t < k; t++)
+                if (ops[t].first != 2) leftQ += ops[t].second;
+            for (size_t t = k + 1; t < ops.size(); t++)
+                if (ops[t].first != 2) rightQ += ops[t].second;
+            std::vector<std::pair<int, int>> n2;
+            if (leftQ >= rightQ) {
+                for (size_t t = 0; t < k; t++) n2.push_back(ops[t]);
+                n2.push_back({4, rightQ});
+            } else {
+                n2.push_back({4, leftQ});
+                for (size_t t = k + 1; t < ops.size(); t++) n2.push_back(ops[t]);
+                first = refAt + ops[k].second;
+            }
+            ops.swap(n2);
+            break;
+        }
+    }
+    r.cigar.clear();
+    for (auto& o : ops)
+        if (o.second > 0) r.cigar.push_back(cw(o.second, o.first));
+    r.pos0 = first;
+    r.bases = H.seq.substr((size_t)a, (size_t)(b - a));
+    return true;
+}
+
+void put(std::ofstream& f, const void* p, size_t bytes) {
+    f.write(static_cast<const char*>(p), (std::streamsize)bytes);
+    static const char z[8] = {0};
+    size_t pad = (8 - bytes % 8) % 8;
+    if (pad) f.write(z, (std::streamsize)pad);
+}
+
+int usage() {
+    fprintf(stderr,
+            "rvsynth --out DIR [--seed N] [--len L] [--depth D] [--read-len R] [--snv-rate X]\n"
+            "        [--indel-rate X] [--indel-max N] [--long-dels N] [--repeat-frac F]\n"
+            "        [--err-rate X] [--insert-mu M] [--insert-sd S] [--vaf LO HI]\n");
+    return 2;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    Opts o;
+    for (int i = 1; i < argc; i++) {
+        std::string a = argv[i];
+        auto nx = [&]() -> std::string {
+            if (i + 1 >= argc) throw std::runtime_error("missing value");
+            return argv[++i];
+        };
+        if (a == "--out") o.out = nx();
+        else if (a == "--seed") o.seed = std::stoull(nx());
+        else if (a == "--len") o.len = std::stoi(nx());
+        else if (a == "--depth") o.depth = std::stod(nx());
+        else if (a == "--read-len") o.readLen = std::stoi(nx());
+        else if (a == "--snv-rate") o.snvRate = std::stod(nx());
+        else if (a == "--indel-rate") o.indelRate = std::stod(nx());
+        else if (a == "--indel-max") o.indelMax = std::stoi(nx());
+        else if (a == "--long-dels") o.longDels = std::stoi(nx());
+        else if (a == "--repeat-frac") o.repeatFrac = std::stod(nx());
+        else if (a == "--err-rate") o.errRate = std::stod(nx());
+        else if (a == "--insert-mu") o.insMu = std::stod(nx());
+        else if (a == "--insert-sd") o.insSd = std::stod(nx());
+        else if (a == "--vaf") {
+            o.vafLo = std::stod(nx());
+            o.vafHi = std::stod(nx());
+        } else
+            return usage();
+    }
+    std::mt19937_64 gref(o.seed * 1000003ULL + 1), gvar(o.seed * 1000003ULL + 2), gread(o.seed * 1000003ULL + 3),
+        gerr(o.seed * 1000003ULL + 4);
+    // ---------------- reference
+    std::string ref((size_t)o.len, 'A');
+    for (int i = 0; i < o.len; i++) ref[(size_t)i] = draw_base(gref);
+    if (o.repeatFrac > 0) {
+        long want = (long)(o.repeatFrac * o.len), have = 0;
+        std::uniform_int_distribution<int> up(200, o.len - 400), ul(2, 6), tl(12, 60);
+        while (have < want) {
+            int s = up(gref), u = ul(gref), L = tl(gref);
+            std::string unit;
+            for (int k = 0; k < u; k++) unit += BASES[gref() % 4];
+            for (int k = 0; k < L; k++) ref[(size_t)(s + k)] = unit[(size_t)(k % u)];
+            have += L;
+        }
+    }
+    // ---------------- variants
+    std::vector<Variant> vars;
+    std::vector<uint8_t> busy((size_t)o.len, 0);
+    std::vector<uint8_t> longDelMask((size_t)o.len + 1, 0);
+    auto mark = [&](int s, int e) {
+        for (int k = std::max(0, s); k < std::min(o.len, e); k++) busy[(size_t)k] = 1;
+    };
+    auto free_span = [&](int s, int e) {
+        if (s < 150 || e > o.len - 150) return false;
+        for (int k = s; k < e; k++)
+            if (busy[(size_t)k]) return false;
+        return true;
+    };
+    std::uniform_real_distribution<double> U(0, 1);
+    for (int i = 0; i < o.longDels; i++) {
+        int L = std::uniform_int_distribution<int>(o.longDelMin, o.longDelMax)(gvar);
+        int p = std::uniform_int_distribution<int>(400, o.len - 400 - L)(gvar);
+        if (!free_span(p - 20, p + L + 20)) continue;
+        vars.push_back({p, 2, L, "", (int)(gvar() & 1)});
+        mark(p - 20, p + L + 20);
+        longDelMask[(size_t)p] = 1;
+    }
+    for (int p = 150; p < o.len - 150; p++) {
+        double u = U(gvar);
+        if (u < o.snvRate) {
+            if (!free_span(p - 3, p + 4)) continue;
+            char rb = ref[(size_t)p];
+            char alt;
+            do {
+                alt = BASES[gvar() % 4];
+            } while (alt == rb);
+            vars.push_back({p, 0, 1, std::string(1, alt), (int)(gvar() & 1)});
+            mark(p - 3, p + 4);
+        } else if (u < o.snvRate + o.indelRate) {
+            int L = std::uniform_int_distribution<int>(1, o.indelMax)(gvar);
+            bool ins = (gvar() & 1) != 0;
+            if (ins) {
+                std::string s;
+                for (int k = 0; k < L; k++) s += BASES[gvar() % 4];
+                int q = p;
+                // left-normalise: move the insertion left while the preceding base equals its last base
+                while (q > 150 && ref[(size_t)q - 1] == s.back()) {
+                    s = s.back() + s.substr(0, s.size() - 1);
+                    q--;
+                }
+                if (!free_span(q - 5, q + 5)) continue;
+                vars.push_back({q, 1, L, s, (int)(gvar() & 1)});
+                mark(q - 5, q + 5);
+            } else {
+                int q = p;
+                while (q > 150 && ref[(size_t)q - 1] == ref[(size_t)(q + L - 1)]) q--;
+                if (!free_span(q - 5, q + L + 5)) continue;
+                vars.push_back({q, 2, L, "", (int)(gvar() & 1)});
+                mark(q - 5, q + L + 5);
+            }
+        }
+    }
+    std::sort(vars.begin(), vars.end(), [](const Variant& a, const Variant& b) { return a.pos < b.pos; });
+    Hap H[2] = {build_hap(ref, vars, 0), build_hap(ref, vars, 1)};
+    // ---------------- reads
+    long npairs = (long)(o.depth * o.len / (2.0 * o.readLen));
+    std::vector<Rec> recs;
+    recs.reserve((size_t)npairs * 2);
+    std::normal_distribution<double> ins(o.insMu, o.insSd), qn(35, 5);
+    for (long f = 0; f < npairs; f++) {
+        double vaf = o.vafLo + (o.vafHi - o.vafLo) * U(gread);
+        (void)vaf;
+        int h = (gread() & 1) ? 1 : 0;
+        const Hap& hp = H[h];
+        int hl = (int)hp.seq.size();
+        int isz = (int)std::lround(ins(gread));
+        if (isz < o.readLen) isz = o.readLen;
+        if (isz > hl) isz = hl;
+        int a = std::uniform_int_distribution<int>(0, hl - isz)(gread);
+        Rec r1, r2;
+        bool ok1 = make_record(ref, hp, a, a + o.readLen, o.longDels > 0, longDelMask, r1);
+        bool ok2 = make_record(ref, hp, a + isz - o.readLen, a + isz, o.longDels > 0, longDelMask, r2);
+        if (!ok1 || !ok2) continue;
+        bool firstIsFwd = (gread() & 1) != 0;
+        for (int k = 0; k < 2; k++) {
+            Rec& r = k == 0 ? r1 : r2;
+            Rec& m = k == 0 ? r2 : r1;
+            bool rev = k == 1;
+            uint16_t fl = 0x1 | 0x2 | (rev ? 0x10 : 0) | (!rev ? 0x20 : 0);
+            fl |= ((k == 0) == firstIsFwd) ? 0x40 : 0x80;
+            r.flag = fl;
+            r.mpos0 = m.pos0;
+            // qualities and errors
+            r.quals.resize(r.bases.size());
+            for (size_t i = 0; i < r.bases.size(); i++) {
+                int q = (int)std::lround(qn(gerr));
+                q = std::max(2, std::min(41, q));
+                r.quals[i] = (char)q;
+                if (U(gerr) < o.errRate) {
+                    char b = r.bases[i], nb2;
+                    do {
+                        nb2 = BASES[gerr() % 4];
+                    } while (nb2 == b);
+                    r.bases[i] = nb2;
+                }
+            }
+            // NM over aligned bases
+            int nmv = 0, rp = r.pos0, qp = 0;
+            for (uint32_t w : r.cigar) {
+                int op = (int)(w & 0xf), l = (int)(w >> 4);
+                if (op == 0) {
+                    for (int t = 0; t < l; t++)
+                        if (r.bases[(size_t)(qp + t)] != ref[(size_t)(rp + t)]) nmv++;
+                    rp += l;
+                    qp += l;
+                } else if (op == 1) {
+                    nmv += l;
+                    qp += l;
+                } else if (op == 2) {
+                    nmv += l;
+                    rp += l;
+                } else if (op == 4) {
+                    qp += l;
+                }
+            }
+            r.nm = nmv;
+        }
+        recs.push_back(std::move(r1));
+        recs.push_back(std::move(r2));
+    }
+    std::stable_sort(recs.begin(), recs.end(), [](const Rec& a, const Rec& b) { return a.pos0 < b.pos0; });
+    // ---------------- write
+    {
+        std::ofstream fa(o.out + "/ref.fa");
+        fa << ">chrS\n";
+        for (int i = 0; i < o.len; i += 60) fa << ref.substr((size_t)i, 60) << "\n";
+    }
+    {
+        std::ofstream tv(o.out + "/truth.tsv");
+        tv << "pos1\tkind\tlen\talt\thap\n";
+        static const char* kn[3] = {"SNV", "INS", "DEL"};
+        for (const Variant& v : vars) tv << v.pos + 1 << "\t" << kn[v.kind] << "\t" << v.len << "\t" << v.alt << "\t" << v.hap << "\n";
+    }
+    {
+        int64_t n = (int64_t)recs.size(), nb = 0, nc = 0;
+        for (const Rec& r : recs) {
+            nb += (int64_t)r.bases.size();
+            nc += (int64_t)r.cigar.size();
+        }
+        std::vector<int32_t> pos((size_t)n), nm((size_t)n), mpos((size_t)n), tid((size_t)n, 0), mtid((size_t)n, 0);
+        std::vector<uint16_t> flag((size_t)n);
+        std::vector<uint8_t> mapq((size_t)n, 60), hasnm((size_t)n, 1);
+        std::vector<int64_t> so((size_t)n + 1), co((size_t)n + 1);
+        std::vector<uint8_t> bases((size_t)nb), quals((size_t)nb);
+        std::vector<uint32_t> cig((size_t)nc);
+        int64_t b = 0, c = 0;
+        for (int64_t i = 0; i < n; i++) {
+            const Rec& r = recs[(size_t)i];
+            pos[(size_t)i] = r.pos0;
+            flag[(size_t)i] = r.flag;
+            nm[(size_t)i] = r.nm;
+            mpos[(size_t)i] = r.mpos0;
+            so[(size_t)i] = b;
+            co[(size_t)i] = c;
+            memcpy(bases.data() + b, r.bases.data(), r.bases.size());
+            memcpy(quals.data() + b, r.quals.data(), r.quals.size());
+            b += (int64_t)r.bases.size();
+            for (uint32_t w : r.cigar) cig[(size_t)c++] = w;
+        }
+        so[(size_t)n] = b;
+        co[(size_t)n] = c;
+        std::ofstream f(o.out + "/reads.rvb", std::ios::binary);
+        f.write("RVB1", 4);
+        uint32_t ver = 1;
+        f.write(reinterpret_cast<const char*>(&ver), 4);
+        f.write(reinterpret_cast<const char*>(&n), 8);
+        f.write(reinterpret_cast<const char*>(&nb), 8);
+        f.write(reinterpret_cast<const char*>(&nc), 8);
+        put(f, pos.data(), 4 * (size_t)n);
+        put(f, flag.data(), 2 * (size_t)n);
+        put(f, mapq.data(), (size_t)n);
+        put(f, hasnm.data(), (size_t)n);
+        put(f, nm.data(), 4 * (size_t)n);
+        put(f, mpos.data(), 4 * (size_t)n);
+        put(f, tid.data(), 4 * (size_t)n);
+        put(f, mtid.data(), 4 * (size_t)n);
+        put(f, so.data(), 8 * ((size_t)n + 1));
+        put(f, co.data(), 8 * ((size_t)n + 1));
+        put(f, bases.data(), (size_t)nb);
+        put(f, quals.data(), (size_t)nb);
+        put(f, cig.data(), 4 * (size_t)nc);
+        printf("{\"reads\": %lld, \"bases\": %lld, \"cigar_ops\": %lld, \"variants\": %zu, \"len\": %d}\n", (long long)n,
+               (long long)nb, (long long)nc, vars.size(), o.len);
+    }
+    return 0;
+}
