@@ -1,0 +1,7 @@
+"""rabbitvar_amd: RabbitVar's per-region calling path on AMD MI355X (gfx950).
+
+The compute path is the native library rabbitvar_amd/_build/librabbitvar_amd.so (HIP kernels + C++
+host engine + C ABI, include/rabbitvar_amd.h).  This package holds the Python host mirror of that
+interface (engine.Engine), the input readers (io), and the build script (build).
+"""
+from .io import read_fasta, read_rvb  # noqa: F401

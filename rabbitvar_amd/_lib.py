@@ -1,0 +1,97 @@
+"""ctypes bindings for librabbitvar_amd.so (declarations in include/rabbitvar_amd.h).
+
+The library is loaded from rabbitvar_amd/_build/.  There is no fallback: when the native library is
+missing the import of the engine raises, naming the build command.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "librabbitvar_amd.so")
+
+c_int = ctypes.c_int
+c_int32 = ctypes.c_int32
+c_int64 = ctypes.c_int64
+c_double = ctypes.c_double
+c_char_p = ctypes.c_char_p
+c_void_p = ctypes.c_void_p
+c_size_t = ctypes.c_size_t
+
+
+class RvConfig(ctypes.Structure):
+    _fields_ = [
+        ("goodq", c_double), ("bias", c_double), ("freq", c_double), ("qratio", c_double),
+        ("mapq", c_double), ("lofreq", c_double),
+        ("vext", c_int), ("mismatch", c_int), ("trimBasesAfter", c_int),
+        ("performLocalRealignment", c_int), ("indelsize", c_int), ("minBiasReads", c_int),
+        ("minr", c_int), ("moveIndelsTo3", c_int), ("samfilter", c_int), ("readPosFilter", c_int),
+        ("doPileup", c_int), ("minmatch", c_int), ("uniqueModeAlignmentEnabled", c_int),
+        ("uniqueModeSecondInPairEnabled", c_int), ("fisher", c_int), ("SVMINLEN", c_int),
+        ("referenceExtension", c_int), ("numberNucleotideToExtend", c_int),
+        ("mappingQuality", c_int), ("removeDuplicatedReads", c_int), ("crisprCuttingSite", c_int),
+        ("sample", c_char_p),
+    ]
+
+
+class RvReads(ctypes.Structure):
+    _fields_ = [
+        ("n_reads", c_int64), ("pos0", c_void_p), ("flag", c_void_p), ("mapq", c_void_p),
+        ("has_nm", c_void_p), ("nm", c_void_p), ("mpos0", c_void_p), ("tid", c_void_p),
+        ("mtid", c_void_p), ("seq_off", c_void_p), ("cigar_off", c_void_p), ("bases", c_void_p),
+        ("quals", c_void_p), ("cigar", c_void_p), ("on_device", c_int),
+    ]
+
+
+class RvRegion(ctypes.Structure):
+    _fields_ = [("chr", c_char_p), ("start", c_int32), ("end", c_int32), ("gene", c_char_p)]
+
+
+class RvContig(ctypes.Structure):
+    _fields_ = [("name", c_char_p), ("length", c_int32), ("bases", c_char_p)]
+
+
+class RvStats(ctypes.Structure):
+    _fields_ = [
+        ("prep_ms", c_double), ("walk_ms", c_double), ("anchor_ms", c_double), ("score_ms", c_double),
+        ("total_device_ms", c_double), ("host_ms", c_double),
+        ("reads_in", c_int64), ("reads_walked", c_int64), ("events", c_int64), ("clips", c_int64),
+        ("ref_key_creations", c_int64), ("walk_errors", c_int64),
+    ]
+
+
+# exported symbols and their signatures: name -> (restype, argtypes)
+SIGNATURES = {
+    "rv_config_default": (None, [ctypes.POINTER(RvConfig)]),
+    "rv_open": (c_int, [ctypes.POINTER(RvConfig), c_int, ctypes.POINTER(c_void_p)]),
+    "rv_run_region": (c_int, [c_void_p, ctypes.POINTER(RvRegion), ctypes.POINTER(RvReads),
+                              ctypes.POINTER(RvContig), ctypes.POINTER(c_void_p)]),
+    "rv_callset_count": (c_size_t, [c_void_p]),
+    "rv_callset_get": (c_void_p, [c_void_p, c_size_t]),
+    "rv_callset_text": (ctypes.POINTER(ctypes.c_char), [c_void_p, ctypes.POINTER(c_size_t)]),
+    "rv_callset_free": (None, [c_void_p]),
+    "rv_get_stats": (c_int, [c_void_p, ctypes.POINTER(RvStats)]),
+    "rv_last_error": (c_char_p, [c_void_p]),
+    "rv_close": (None, [c_void_p]),
+    "rv_debug_pileup": (c_int, [c_void_p, ctypes.POINTER(RvRegion), ctypes.POINTER(RvReads),
+                                ctypes.POINTER(RvContig), ctypes.POINTER(ctypes.POINTER(ctypes.c_char)),
+                                ctypes.POINTER(c_size_t)]),
+    "rv_free": (None, [c_void_p]),
+}
+
+_lib = None
+
+
+def load():
+    """Loads the native library (raises OSError when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OSError(f"{LIB_PATH} is missing: build it with `python -m rabbitvar_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib

@@ -1,0 +1,98 @@
+"""Readers for the inputs of the calling path: single-contig FASTA and read-batch files (RVB1, the
+structure-of-arrays layout written by rvsynth; see rabbitvar_amd/synth/rv_synth.cpp), plus the
+region iterator's record selection."""
+import numpy as np
+
+
+def read_fasta(path):
+    """Returns (name, sequence bytes) of the first contig."""
+    name = None
+    parts = []
+    with open(path, "rb") as f:
+        for line in f:
+            line = line.rstrip(b"\r\n")
+            if line.startswith(b">"):
+                if name is not None:
+                    break
+                name = line[1:].split()[0].decode()
+                continue
+            parts.append(line)
+    return name, b"".join(parts)
+
+
+_FIELDS = [("pos0", np.int32, 0), ("flag", np.uint16, 0), ("mapq", np.uint8, 0), ("has_nm", np.uint8, 0),
+           ("nm", np.int32, 0), ("mpos0", np.int32, 0), ("tid", np.int32, 0), ("mtid", np.int32, 0),
+           ("seq_off", np.int64, 1), ("cigar_off", np.int64, 1)]
+
+
+def _ranges(starts, lens):
+    """Concatenation of arange(starts[i], starts[i] + lens[i]) for all i."""
+    lens = np.asarray(lens, dtype=np.int64)
+    total = int(lens.sum())
+    if total == 0:
+        return np.zeros(0, np.int64)
+    excl = np.concatenate([[0], np.cumsum(lens)[:-1]])
+    return np.repeat(np.asarray(starts, np.int64) - excl, lens) + np.arange(total, dtype=np.int64)
+
+
+class ReadBatch:
+    """Aligned records as structure-of-arrays numpy arrays (sorted by pos0)."""
+
+    def __init__(self, arrays):
+        self.__dict__.update(arrays)
+        self.n = int(len(self.pos0))
+
+    def ref_len(self):
+        """Reference span of each record (M/D/N/=/X lengths; 1 when empty, as bam_endpos)."""
+        ops = self.cigar & 0xF
+        lens = (self.cigar >> 4).astype(np.int64)
+        on_ref = np.isin(ops, (0, 2, 3, 7, 8))
+        csum = np.concatenate([[0], np.cumsum(np.where(on_ref, lens, 0))])
+        span = csum[self.cigar_off[1:]] - csum[self.cigar_off[:-1]]
+        return np.where(span > 0, span, 1)
+
+    def select(self, start, end):
+        """Records the region iterator returns for chr:start-end (1-based, inclusive): those whose
+        [pos0, pos0 + span) overlaps [start - 1, end), in file order."""
+        span = self.ref_len()
+        keep = (self.pos0 < end) & (self.pos0 + span > start - 1)
+        idx = np.nonzero(keep)[0]
+        return self.take(idx)
+
+    def take(self, idx):
+        idx = np.asarray(idx, dtype=np.int64)
+        so, co = self.seq_off, self.cigar_off
+        slen = so[idx + 1] - so[idx]
+        clen = co[idx + 1] - co[idx]
+        new_so = np.concatenate([[0], np.cumsum(slen)]).astype(np.int64)
+        new_co = np.concatenate([[0], np.cumsum(clen)]).astype(np.int64)
+        bidx = _ranges(so[idx], slen)
+        cidx = _ranges(co[idx], clen)
+        arrays = {name: getattr(self, name)[idx] for name, _, extra in _FIELDS if not extra}
+        arrays.update(seq_off=new_so, cigar_off=new_co, bases=self.bases[bidx], quals=self.quals[bidx],
+                      cigar=self.cigar[cidx])
+        return ReadBatch(arrays)
+
+
+def read_rvb(path):
+    with open(path, "rb") as f:
+        data = f.read()
+    if data[:4] != b"RVB1":
+        raise ValueError(f"{path}: not a read-batch file")
+    n, nb, nc = np.frombuffer(data, dtype=np.int64, count=3, offset=8)
+    off = 32
+    arrays = {}
+
+    def take(dtype, count):
+        nonlocal off
+        a = np.frombuffer(data, dtype=dtype, count=int(count), offset=off).copy()
+        nbytes = a.nbytes
+        off += nbytes + (-nbytes) % 8
+        return a
+
+    for name, dtype, extra in _FIELDS:
+        arrays[name] = take(dtype, n + extra)
+    arrays["bases"] = take(np.uint8, nb)
+    arrays["quals"] = take(np.uint8, nb)
+    arrays["cigar"] = take(np.uint32, nc)
+    return ReadBatch(arrays)

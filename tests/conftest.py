@@ -1,0 +1,68 @@
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SYNTH = os.path.join(ROOT, "rabbitvar_amd", "_build", "rvsynth")
+ORACLE = os.path.join(ROOT, "oracle", "_build", "rvcpu")
+
+# Synthetic workloads used by the parity tests (small enough for the oracle to finish in seconds).
+CASES = {
+    # 200 kb contig, 30x, SNVs 1e-3, indels 1-10 bp at 1e-4
+    "c1": ["--seed", "1", "--len", "200000", "--depth", "30"],
+    # 300 kb, 60x, 2% tandem-repeat tracts, indels 1-30 bp at 1/8000, 12 long deletions (clipped reads)
+    "c2s": ["--seed", "2", "--len", "300000", "--depth", "60", "--indel-rate", "0.000125", "--indel-max", "30",
+            "--long-dels", "12", "--repeat-frac", "0.02"],
+    # dense variant edge cases: high indel rate, long indels, low quality tail errors
+    "edge": ["--seed", "5", "--len", "60000", "--depth", "40", "--indel-rate", "0.002", "--indel-max", "40",
+             "--snv-rate", "0.004", "--long-dels", "6", "--repeat-frac", "0.05", "--err-rate", "0.01"],
+}
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
+
+
+def _ensure_tools():
+    if not os.path.exists(ORACLE):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    if not os.path.exists(SYNTH):
+        os.makedirs(os.path.dirname(SYNTH), exist_ok=True)
+        subprocess.run(["g++", "-O2", "-std=c++17", "-o", SYNTH,
+                        os.path.join(ROOT, "rabbitvar_amd", "synth", "rv_synth.cpp")], check=True)
+
+
+@pytest.fixture(scope="session")
+def tools():
+    _ensure_tools()
+    return {"oracle": ORACLE, "synth": SYNTH}
+
+
+@pytest.fixture(scope="session")
+def dataset(tmp_path_factory, tools):
+    cache = {}
+
+    def get(name):
+        if name not in cache:
+            d = tmp_path_factory.mktemp(name)
+            subprocess.run([tools["synth"], "--out", str(d)] + CASES[name], check=True, capture_output=True)
+            cache[name] = str(d)
+        return cache[name]
+
+    return get
+
+
+def run_oracle(d, region, extra=(), dump=None, out="out.txt"):
+    cmd = [ORACLE, "-G", os.path.join(d, "ref.fa"), "-b", os.path.join(d, "reads.rvb"), "-N", "S1", "-R", region,
+           "--out", os.path.join(d, out)] + list(extra)
+    if dump:
+        cmd += ["--dump-pileup", os.path.join(d, dump)]
+    subprocess.run(cmd, check=True, capture_output=True)
+    with open(os.path.join(d, out)) as f:
+        text = f.read()
+    dump_text = None
+    if dump:
+        with open(os.path.join(d, dump)) as f:
+            dump_text = f.read()
+    return text, dump_text
