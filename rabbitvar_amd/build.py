@@ -34,17 +34,20 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
-def build(verbose=True):
+def build(verbose=True, jobs=None):
+    from concurrent.futures import ThreadPoolExecutor
+
     os.makedirs(OUT, exist_ok=True)
-    objs = []
     inc = ["-I", CSRC, "-I", os.path.join(ROOT, "include")]
+    cmds, objs = [], []
     for src in LIB_SOURCES:
         path = os.path.join(CSRC, src)
-        if not os.path.exists(path):
-            continue
         obj = os.path.join(OUT, src.replace(".", "_") + ".o")
-        _run([HIPCC, "-x", "hip"] + HIP_FLAGS + inc + ["-c", path, "-o", obj])
+        cmds.append([HIPCC, "-x", "hip"] + HIP_FLAGS + inc + ["-c", path, "-o", obj])
         objs.append(obj)
+    jobs = jobs or min(len(cmds), max(1, min(8, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(_run, cmds))
     lib = os.path.join(OUT, "librabbitvar_amd.so")
     _run([HIPCC] + HIP_FLAGS + ["-shared", "-o", lib + ".tmp"] + objs)
     os.replace(lib + ".tmp", lib)

@@ -59,16 +59,21 @@ Engine::Engine(const rv_config& c, int dev) {
         return;
     }
     for (auto& e : ev) hipEventCreate(&e);
+    for (auto& e : sev) hipEventCreate(&e);
 }
 
 Engine::~Engine() {
-    DevBuf* all[] = {&b_prep, &b_cig, &b_ctr, &b_span, &b_rows, &b_ref, &b_events, &b_arena, &b_clips, &b_clip_arena,
-                     &b_anchors, &b_counts};
+    DevBuf* all[] = {&b_prep,      &b_cig,     &b_ctr,    &b_span,       &b_rows,     &b_ref,    &b_events,
+                     &b_arena,     &b_clips,   &b_clip_arena, &b_anchors, &b_counts,   &b_gather_ids, &b_gather_out,
+                     &b_cx_pos,    &b_cx_off,  &b_cx,     &b_cx_arena,   &b_cx_cov,   &b_cx_key, &b_out,
+                     &b_out_arena, &b_score_ctr};
     for (auto* b : all)
         if (b->p) hipFree(b->p);
     for (auto& b : b_reads)
         if (b.p) hipFree(b.p);
     for (auto& e : ev)
+        if (e) hipEventDestroy(e);
+    for (auto& e : sev)
         if (e) hipEventDestroy(e);
     if (stream) hipStreamDestroy(stream);
 }
@@ -165,6 +170,8 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
         ref_dev = (char*)grow(b_ref, up.size());
         if (!ref_dev) return fail(RV_ERR_OOM, "device allocation for the reference window failed");
         RV_HIP(hipMemcpyAsync(ref_dev, up.data(), up.size(), hipMemcpyHostToDevice, stream));
+        RV_HIP(hipStreamSynchronize(stream));  // `up` is pageable and about to go out of scope
+        out.ref_win = up.substr(0, (size_t)win_len);
     }
     W.ref = ref_dev;
     W.dense_lo = 0;
@@ -292,6 +299,21 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
         max_anchors = std::max<unsigned>(max_anchors, counts[2]) * 2;
     }
     return fail(RV_ERR_CAPACITY, "walk record buffers kept overflowing");
+}
+
+bool Engine::fetch_blocks(const std::vector<int>& blocks, std::vector<DenseRow>& out) {
+    constexpr int ROWS = 64;
+    out.assign(blocks.size() * ROWS, DenseRow{});
+    if (blocks.empty()) return true;
+    int32_t* ids = (int32_t*)grow(b_gather_ids, sizeof(int32_t) * blocks.size());
+    DenseRow* dst = (DenseRow*)grow(b_gather_out, sizeof(DenseRow) * ROWS * blocks.size());
+    if (!ids || !dst) return false;
+    std::vector<int32_t> h(blocks.begin(), blocks.end());
+    if (hipMemcpyAsync(ids, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice, stream) != hipSuccess) return false;
+    if (launch_gather_blocks(rows, rows_n, ids, (int)h.size(), dst, stream) != hipSuccess) return false;
+    if (hipMemcpyAsync(out.data(), dst, sizeof(DenseRow) * out.size(), hipMemcpyDeviceToHost, stream) != hipSuccess)
+        return false;
+    return hipStreamSynchronize(stream) == hipSuccess;
 }
 
 }  // namespace rv

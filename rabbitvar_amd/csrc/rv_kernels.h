@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include "rv_pileup_body.h"
+#include "rv_score.h"
 
 namespace rv {
 
@@ -13,5 +14,12 @@ hipError_t launch_walk(const ReadsDev& R, const Window& W, const Config& cfg, co
                        const WalkSink& sink, hipStream_t s);
 hipError_t launch_anchor(const Anchor* a, const unsigned* n_anchors, unsigned cap, DenseRow* rows, int dense_lo,
                          int dense_n, const Config& cfg, hipStream_t s);
+
+static_assert(sizeof(DenseRow) % 16 == 0, "dense rows are copied in 16-byte words");
+hipError_t launch_gather_blocks(const DenseRow* rows, int dense_n, const int32_t* blocks, int nb, DenseRow* out,
+                                hipStream_t s);
+
+hipError_t launch_score(const ScoreIn& S, hipStream_t s);
+hipError_t launch_fisher_table(const int* tab, int n, double* out, hipStream_t s);
 
 }  // namespace rv

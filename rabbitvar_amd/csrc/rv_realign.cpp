@@ -1,0 +1,1125 @@
+// Local realignment (host side), restating src/VariationRealigner.cpp over the engine's position
+// store:
+//   process / stage order               :135-161, :467-482
+//   adjustMNP                           :334-465
+//   realigndel                          :489-778
+//   realignins                          :785-1114
+//   realignlgins30                      :1119-1354
+//   fillAndSortTmp / CMP_tmp / COMP3    :1362-1386, :44-58, :29-34
+//   find35match                         :1396-1428
+//   noPassingReads                      :1447-1498
+//   ismatch / ismatchref                :1507-1539, :1987-2024
+//   adjRefCnt / adjRefFactor            :1753-1822
+//   findMM5 / findMM3                   :1857-1976
+// with include/VariationUtils.h (adjCnt :291-332, correctCnt :263-280, findconseq :403-509,
+// islowcomplexseq :367-394, joinRef :336-349) and include/util.h (vc_substr :27-57, charAt :171-179).
+// A candidate whose processing throws (a reference key outside the loaded window, an out-of-range
+// substring) is abandoned with the changes made so far, as the reference's per-candidate catch does.
+#include "rv_realign.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <regex>
+#include <stdexcept>
+
+namespace rv {
+
+// ------------------------------------------------------------------ rows
+bool HostRowSource::fetch(const std::vector<int>& blocks, std::vector<DenseRow>& out) {
+    out.assign(blocks.size() * (size_t)BLOCK, DenseRow{});
+    for (size_t k = 0; k < blocks.size(); k++)
+        for (int i = 0; i < BLOCK; i++) {
+            long long r = (long long)blocks[k] * BLOCK + i;
+            if (r >= 0 && r < P_.dense_n && (size_t)r < P_.rows.size()) out[k * BLOCK + (size_t)i] = P_.rows[(size_t)r];
+        }
+    return true;
+}
+
+bool RowCache::load(const std::vector<int>& blocks) {
+    if (blocks.empty()) return true;
+    std::vector<DenseRow> buf;
+    if (!src_->fetch(blocks, buf)) return false;
+    fetches++;
+    blocks_fetched += (int64_t)blocks.size();
+    for (size_t k = 0; k < blocks.size(); k++) {
+        std::unique_ptr<DenseRow[]> b(new DenseRow[RowSource::BLOCK]);
+        std::memcpy(b.get(), buf.data() + k * RowSource::BLOCK, sizeof(DenseRow) * RowSource::BLOCK);
+        blocks_[blocks[k]] = std::move(b);
+    }
+    return true;
+}
+
+bool RowCache::prefetch(const std::vector<int>& positions, int radius) {
+    std::set<int> want;
+    int nb = (n_ + RowSource::BLOCK - 1) / RowSource::BLOCK;
+    for (int p : positions) {
+        int a = std::max(0, p - radius - lo_), b = std::min(n_ - 1, p + radius - lo_);
+        if (a > b) continue;
+        for (int k = a / RowSource::BLOCK; k <= b / RowSource::BLOCK && k < nb; k++)
+            if (!blocks_.count(k)) want.insert(k);
+    }
+    return load(std::vector<int>(want.begin(), want.end()));
+}
+
+const DenseRow* RowCache::row(int pos) {
+    int i = pos - lo_;
+    if (i < 0 || i >= n_) return nullptr;
+    int k = i / RowSource::BLOCK;
+    auto it = blocks_.find(k);
+    if (it == blocks_.end()) {
+        if (!load({k})) throw std::runtime_error("dense row fetch failed");
+        it = blocks_.find(k);
+    }
+    return &it->second[(size_t)(i % RowSource::BLOCK)];
+}
+
+// ------------------------------------------------------------------ reference
+char HostRef::at(int k) const {
+    if (k >= lo && k <= hi) return seq[k - lo];
+    if (created.count(k)) return '\0';
+    throw std::out_of_range("reference key");
+}
+char HostRef::get(int k) {
+    if (k >= lo && k <= hi) return seq[k - lo];
+    created.insert(k);
+    return '\0';
+}
+
+// ------------------------------------------------------------------ position store
+RState::RState(HostPileup& P_, RowCache& rc) : P(P_), rows(rc) {
+    for (auto& e : P.ins)
+        for (auto& kv : e.second) {
+            HVar* v = new_var();
+            *v = kv.second;
+            ins[e.first][kv.first] = v;
+        }
+}
+
+HVar* RState::new_var() {
+    pool.emplace_back();
+    return &pool.back();
+}
+
+RPos& RState::at(int p) {
+    RPos& r = pos[p];
+    if (r.loaded) return r;
+    r.loaded = true;
+    if (const DenseRow* d = rows.row(p)) {
+        r.key = (d->rowflags & 1u) != 0;
+        r.cov = d->cov;
+        for (int b = 0; b < 4; b++)
+            if (d->b[b].flags & 1u) {
+                HVar* v = new_var();
+                *v = dense_to_hvar(d->b[b]);
+                r.v[std::string(1, index_base(b))] = v;
+            }
+    }
+    auto it = P.nd.find(p);
+    if (it != P.nd.end()) {
+        r.key = true;
+        for (auto& kv : it->second) {
+            HVar* v = new_var();
+            *v = kv.second;
+            r.v[kv.first] = v;
+        }
+    }
+    return r;
+}
+
+HVar* RState::inner(int p, const std::string& d) {
+    RPos& r = at(p);
+    if (!r.key) return nullptr;
+    auto it = r.v.find(d);
+    if (it == r.v.end()) return nullptr;
+    r.dirty = true;  // the caller may change it
+    return it->second;
+}
+
+HVar* RState::get_variation(int p, const std::string& d) {
+    RPos& r = at(p);
+    r.dirty = true;
+    if (!r.key) {
+        r.key = true;
+        r.v.clear();
+    }
+    HVar*& v = r.v[d];
+    if (!v) v = new_var();
+    return v;
+}
+
+HVar* RState::get_ins(int p, const std::string& d) {
+    HVar*& v = ins[p][d];
+    if (!v) v = new_var();
+    return v;
+}
+
+HVar* RState::maybe(int p, char refBase) {
+    if (refBase == 0) return nullptr;
+    return inner(p, std::string(1, refBase));
+}
+
+void RState::erase_inner(int p, const std::string& d, bool drop_empty) {
+    RPos& r = at(p);
+    if (!r.key) return;
+    r.dirty = true;
+    r.v.erase(d);
+    if (drop_empty && r.v.empty()) r.key = false;
+}
+
+void RState::add_cov(int p, int k) {
+    RPos& r = at(p);
+    r.dirty = true;
+    r.cov += k;
+}
+
+// ------------------------------------------------------------------ string helpers (util.h)
+namespace {
+
+std::string sub1(const std::string& s, int idx) {
+    if (idx >= 0) return s.substr((size_t)idx);
+    return s.substr((size_t)(s.size() + (size_t)(int64_t)idx));
+}
+std::string sub2(const std::string& s, int begin, int len) {
+    if (begin < 0) begin = (int)(s.size() + (size_t)(int64_t)begin);
+    if (len > 0) return s.substr((size_t)begin, (size_t)len);
+    if (len == 0) return "";
+    len = (int)(s.size() + (size_t)(int64_t)len - (size_t)(int64_t)begin);
+    if (len < 0) return "";
+    return s.substr((size_t)begin, (size_t)len);
+}
+char char_at(const std::string& s, int index) {
+    if (index < 0) {
+        int i = (int)s.size() + index;
+        if (i < 0) return (char)-1;
+        return s[(size_t)i];
+    }
+    return s[(size_t)index];
+}
+std::string join_ref(const HostRef& ref, int from, int to) {
+    std::string r;
+    for (int i = from; i <= to; i++) r += ref.at(i);
+    return r;
+}
+std::string join_ref_d(const HostRef& ref, int from, double to) {
+    std::string r;
+    for (int i = from; i < to; i++) r += ref.at(i);
+    return r;
+}
+void replace_first(std::string& s, const std::string& a, const std::string& b) {
+    size_t p = s.find(a);
+    if (p != std::string::npos) s.replace(p, a.size(), b);
+}
+void replace_first_re(std::string& s, const std::regex& re, const std::string& b) {
+    s = std::regex_replace(s, re, b, std::regex_constants::format_first_only);
+}
+bool starts(const std::string& s, char c) { return !s.empty() && s[0] == c; }
+
+// ------------------------------------------------------------------ count transfer
+void correct_cnt(HVar* v) {
+    if (v->cnt < 0) v->cnt = 0;
+    if (v->hicnt < 0) v->hicnt = 0;
+    if (v->locnt < 0) v->locnt = 0;
+    if (v->pmean < 0) v->pmean = 0;
+    if (v->qmean < 0) v->qmean = 0;
+    if (v->mqmean < 0) v->mqmean = 0;
+    if (v->rev < 0) v->rev = 0;
+    if (v->fwd < 0) v->fwd = 0;
+}
+void adj_cnt(HVar* to, const HVar* v, HVar* refv) {
+    to->cnt += v->cnt;
+    to->extracnt += v->cnt;
+    to->hicnt += v->hicnt;
+    to->locnt += v->locnt;
+    to->pmean += v->pmean;
+    to->qmean += v->qmean;
+    to->mqmean += v->mqmean;
+    to->nm += v->nm;
+    to->pstd = true;
+    to->qstd = true;
+    to->rev += v->rev;
+    to->fwd += v->fwd;
+    if (!refv) return;
+    refv->cnt -= v->cnt;
+    refv->hicnt -= v->hicnt;
+    refv->locnt -= v->locnt;
+    refv->pmean -= v->pmean;
+    refv->qmean -= v->qmean;
+    refv->mqmean -= v->mqmean;
+    refv->nm -= v->nm;
+    refv->rev -= v->rev;
+    refv->fwd -= v->fwd;
+    correct_cnt(refv);
+}
+void adj_ref_cnt(const HVar* tv, HVar* refv, int len) {
+    if (!refv) return;
+    double f = tv->pmean != 0 ? (tv->pmean / (double)tv->cnt - len + 1) / (tv->pmean / (double)tv->cnt) : 0;
+    if (f < 0) return;
+    if (f > 1) f = 1;
+    refv->cnt -= (int)(f * tv->cnt);
+    refv->hicnt -= (int)(f * tv->hicnt);
+    refv->locnt -= (int)(f * tv->locnt);
+    refv->pmean -= f * tv->pmean;
+    refv->qmean -= f * tv->qmean;
+    refv->mqmean -= f * tv->mqmean;
+    refv->nm -= f * tv->nm;
+    refv->rev -= (int)(f * tv->rev);
+    refv->fwd -= (int)(f * tv->fwd);
+    correct_cnt(refv);
+}
+void adj_ref_factor(HVar* refv, double f) {
+    if (!refv) return;
+    if (f > 1) f = 1;
+    if (f < -1) return;
+    int old = refv->cnt;
+    refv->cnt -= (int)(f * refv->cnt);
+    refv->hicnt -= (int)(f * refv->hicnt);
+    refv->locnt -= (int)(f * refv->locnt);
+    double fc = old != 0 ? std::abs(refv->cnt - old) / (double)old : 1;
+    refv->pmean -= refv->pmean * f * fc;
+    refv->qmean -= refv->qmean * f * fc;
+    refv->mqmean -= refv->mqmean * f * fc;
+    refv->nm -= f * refv->nm;
+    refv->fwd -= (int)(f * refv->fwd);
+    refv->rev -= (int)(f * refv->rev);
+    correct_cnt(refv);
+}
+
+bool low_complexity(const std::string& s) {
+    int len = (int)s.size();
+    if (len == 0) return true;
+    int ntcnt = 0;
+    const char order[4] = {'A', 'T', 'G', 'C'};
+    for (char b : order) {
+        int n = (int)std::count(s.begin(), s.end(), b);
+        if (n > 0) ntcnt++;
+        if (n / (double)len > 0.75) return true;
+    }
+    return ntcnt < 3;
+}
+
+}  // namespace
+
+std::string findconseq(HSclip& sc) {
+    if (!sc.sequence.empty()) return sc.sequence;
+    int total = 0, match = 0;
+    std::string seqq;
+    bool flag = false;
+    for (auto& e : sc.nt) {
+        int idx = e.first;
+        std::vector<char> firsts;
+        for (auto& pr : e.second) firsts.push_back(pr.first);
+        int maxCount = 0;
+        double maxQuality = 0;
+        char chosen = 0;
+        int tt = 0;
+        auto si = sc.seq.find(idx);
+        for (char base : rh_char_iteration_order(firsts)) {
+            int count = 0;
+            for (auto& pr : e.second)
+                if (pr.first == base) count = pr.second;
+            tt += count;
+            double bq = 0;
+            bool has = false;
+            if (si != sc.seq.end()) {
+                auto bj = si->second.find(base);
+                if (bj != si->second.end()) {
+                    has = true;
+                    bq = bj->second.qmean;
+                }
+            }
+            if (count > maxCount || (has && bq > maxQuality)) {
+                maxCount = count;
+                chosen = base;
+                maxQuality = sc.seq[idx][base].qmean;
+            }
+        }
+        if (idx == 3 && sc.nt.size() >= 6 && tt / (double)sc.v.cnt < 0.2 && tt <= 2) break;
+        if ((tt - maxCount > 2 || maxCount <= tt - maxCount) && maxCount / (double)tt < 0.8) {
+            if (flag) break;
+            flag = true;
+        }
+        total += tt;
+        match += maxCount;
+        if (chosen != 0) seqq += chosen;
+    }
+    std::string SEQ;
+    size_t ntSize = sc.nt.size();
+    size_t sl = seqq.size();
+    if (total != 0 && match / (double)total > 0.9 && sl / 1.5 > (double)(ntSize - sl) &&
+        (sl / (double)ntSize > 0.8 || ntSize - sl < 10 || sl > 25)) {
+        SEQ = seqq;
+    } else {
+        SEQ = " ";
+    }
+    if (!SEQ.empty() && SEQ.size() > 12) {
+        bool mm1 = SEQ.size() >= 8 && SEQ.compare(1, 7, "AAAAAAA") == 0;
+        bool mm2 = SEQ.size() >= 8 && SEQ.compare(1, 7, "TTTTTTT") == 0;
+        if (mm1 || mm2) sc.used = true;
+        if (low_complexity(SEQ)) sc.used = true;
+    }
+    sc.sequence = SEQ;
+    return SEQ;
+}
+
+namespace {
+
+struct Cand {
+    int position;
+    std::string desc;
+    int count;
+};
+// CMP_tmp: count descending, position ascending, descriptor descending
+std::vector<Cand> sorted_candidates(const std::map<int, std::map<std::string, int>>& m) {
+    std::vector<Cand> t;
+    for (auto& e : m)
+        for (auto& v : e.second) t.push_back({e.first, v.first, v.second});
+    std::sort(t.begin(), t.end(), [](const Cand& a, const Cand& b) {
+        if (a.count != b.count) return a.count > b.count;
+        if (a.position != b.position) return a.position < b.position;
+        return a.desc.compare(b.desc) > 0;
+    });
+    return t;
+}
+
+struct Mismatch {
+    std::string seq;
+    int pos;
+    int end;
+};
+struct MM {
+    std::vector<Mismatch> mm;
+    std::vector<int> scp;
+    int nm = 0;
+    int misp = 0;
+    std::string misnt;
+};
+
+std::string strip_marks(std::string s) {
+    s.erase(std::remove_if(s.begin(), s.end(), [](char ch) { return ch == '#' || ch == '^'; }), s.end());
+    return s;
+}
+
+struct Re {
+    std::regex begin_minus_number{"^-(\\d+).*"};
+    std::regex up_number_end{"\\^(\\d+)$"};
+    std::regex minus_sv{"^-\\d+\\^([ATGNC]+)<...\\d+>([ATGNC]+)$"};
+    std::regex begin_minus_number_any{"^-\\d+(.*)"};
+    std::regex caret_atgnc{"\\^([ATGNC]+)"};
+    std::regex strip{"\\^|&|#"};
+    std::regex minus_number_amp_atgcs_end{"(-\\d+)&[ATGC]+$"};
+    std::regex begin_plus_atgc{"^\\+([ATGC]+).*"};
+    std::regex dup_num_atgc{"<dup(\\d+)>([ATGC]+)$"};
+    std::regex amp_atgc{".*&([ATGC]+).*"};
+    std::regex hash_atgc{"#([ATGC]+).*"};
+    std::regex caret_atgc_end{"\\^([ATGC]+)$"};
+    std::regex plus_atgcs_amp_atgcs_end{"(\\+[ATGC]+)&[ATGC]+$"};
+    std::regex lead_plus{"^\\+"};
+    std::regex caret_digits_end{"\\^\\d+$"};
+    std::regex caret{"\\^"};
+};
+
+const Re& res() {
+    static const Re r;
+    return r;
+}
+
+struct Realigner {
+    const Config& c;
+    int rstart, rend;
+    HostRef& ref;
+    int chrLen;
+    const HostReads* reads;
+    RState& S;
+    RealignStats& st;
+
+    int maxlen() const { return S.P.maxReadLength; }
+
+    bool ismatch(const std::string& s1, std::string s2, int dir, int MMx = 3) {
+        s2 = strip_marks(s2);
+        int mm = 0;
+        for (size_t n = 0; n < s1.size() && n < s2.size(); n++) {
+            int b = dir * (int)n - (dir == -1 ? 1 : 0);
+            if (s1[n] != sub2(s2, b, 1)[0]) mm++;
+        }
+        return mm <= MMx && mm / (double)s1.size() < 0.15;
+    }
+    bool ismatchref(const std::string& s, HostRef& rf, int position, int dir, int MMx = 3) {
+        int mm = 0;
+        for (int n = 0; n < (int)s.size(); n++) {
+            if (!rf.count(position + dir * n)) return false;
+            if (char_at(s, dir == 1 ? n : dir * n - 1) != rf.get(position + dir * n)) mm++;
+        }
+        return mm <= MMx && mm / (double)s.size() < 0.15;
+    }
+
+    MM findMM5(HostRef& rf, int position, const std::string& wupseq) {
+        std::string seq = strip_marks(wupseq);
+        MM r;
+        int n = 0, mn = 0, mcnt = 0;
+        std::string str;
+        auto ne = [&](char ch, int k) { return rf.count(k) && rf.at(k) != ch; };
+        auto eq = [&](char ch, int k) { return rf.count(k) && rf.at(k) == ch; };
+        while (ne(char_at(seq, -1 - n), position - n) && mcnt < 3) {
+            str.insert(str.begin(), char_at(seq, -1 - n));
+            r.mm.push_back({str, position - n, 5});
+            n++;
+            mcnt++;
+        }
+        r.scp.push_back(position + 1);
+        char misnt = 0;
+        if (str.size() == 1) {
+            while (eq(char_at(seq, -1 - n), position - n)) {
+                n++;
+                if (n != 0) mn++;
+            }
+            if (mn > 1) {
+                int n2 = 0;
+                while (-1 - n - 1 - n2 >= 0 && eq(char_at(seq, -1 - n - 1 - n2), position - n - 1 - n2)) n2++;
+                if (n2 > 2) {
+                    r.scp.push_back(position - n - n2);
+                    r.misp = position - n;
+                    misnt = char_at(seq, -1 - n);
+                    auto s = S.P.sc5.find(position - n - n2);
+                    if (s != S.P.sc5.end()) s->second.used = true;
+                    mn += n2;
+                } else {
+                    r.scp.push_back(position - n);
+                    auto s = S.P.sc5.find(position - n);
+                    if (s != S.P.sc5.end()) s->second.used = true;
+                }
+            }
+        }
+        r.nm = mn;
+        r.misnt = misnt == 0 ? "" : std::string(1, misnt);
+        return r;
+    }
+
+    MM findMM3(HostRef& rf, int p, const std::string& sanpseq) {
+        std::string seq = strip_marks(sanpseq);
+        MM r;
+        int n = 0, mn = 0, mcnt = 0;
+        std::string str;
+        while (n < (int)seq.size() && rf.count(p + n) && rf.get(p + n) == seq[(size_t)n]) n++;
+        r.scp.push_back(p + n);
+        int Tbp = p + n;
+        while (mcnt <= 3 && n < (int)seq.size() && rf.get(p + n) != seq[(size_t)n]) {
+            str += seq[(size_t)n];
+            r.mm.push_back({str, Tbp, 3});
+            n++;
+            mcnt++;
+        }
+        char misnt = 0;
+        auto eq = [&](char ch, int k) { return rf.count(k) && rf.at(k) == ch; };
+        if (str.size() == 1) {
+            while (n < (int)seq.size() && eq(seq[(size_t)n], p + n)) {
+                n++;
+                if (n != 0) mn++;
+            }
+            if (mn > 1) {
+                int n2 = 0;
+                while (n + n2 + 1 < (int)seq.size() && eq(seq[(size_t)(n + n2 + 1)], p + n + 1 + n2)) n2++;
+                if (n2 > 2 && n + n2 + 1 < (int)seq.size()) {
+                    r.scp.push_back(p + n + n2);
+                    r.misp = p + n;
+                    misnt = seq[(size_t)n];
+                    auto s = S.P.sc3.find(p + n + n2);
+                    if (s != S.P.sc3.end()) s->second.used = true;
+                    mn += n2;
+                } else {
+                    r.scp.push_back(p + n);
+                    auto s = S.P.sc3.find(p + n);
+                    if (s != S.P.sc3.end()) s->second.used = true;
+                }
+            }
+        }
+        r.nm = mn;
+        r.misnt = misnt == 0 ? "" : std::string(1, misnt);
+        return r;
+    }
+
+    HSclip* sc5(int p) {
+        auto it = S.P.sc5.find(p);
+        return it == S.P.sc5.end() ? nullptr : &it->second;
+    }
+    HSclip* sc3(int p) {
+        auto it = S.P.sc3.find(p);
+        return it == S.P.sc3.end() ? nullptr : &it->second;
+    }
+
+    // noPassingReads (VariationRealigner.cpp:1447-1498): no record of the region spans [start, end]
+    // without carrying exactly this deletion
+    bool noPassingReads(int start, int end) {
+        if (!reads) throw std::runtime_error("region reads unavailable");
+        int cnt = 0;
+        uint32_t dq = ((uint32_t)(end - start) << 4) | OP_D;
+        for (int64_t i = 0; i < reads->n; i++) {
+            const uint32_t* cg = reads->cigar + reads->cigar_off[i];
+            int nc = (int)(reads->cigar_off[i + 1] - reads->cigar_off[i]);
+            int reflenAll = 0, al = 0;
+            bool has = false;
+            for (int k = 0; k < nc; k++) {
+                uint32_t w = cg[k];
+                int o = (int)(w & 0xF), l = (int)(w >> 4);
+                if (o == OP_M || o == OP_D || o == OP_N || o == OP_EQ || o == OP_X) reflenAll += l;
+                if (o == OP_M || o == OP_D) al += l;
+                if (w == dq) has = true;
+            }
+            int beg0 = reads->pos0[i], end0 = reads->pos0[i] + (reflenAll > 0 ? reflenAll : 1);
+            if (!(beg0 < end && end0 > start - 1)) continue;
+            if (has) continue;
+            int rs = reads->pos0[i] + 1;
+            int re = rs + al;
+            if (re > end + 2 && rs < start - 2) cnt++;
+        }
+        return cnt <= 0;
+    }
+
+    void adjustMNP() {
+        std::vector<Cand> tmp = sorted_candidates(S.P.mnp);
+        st.candidates += (int64_t)tmp.size();
+        for (const Cand& t : tmp) {
+            try {
+                int position = t.position;
+                const std::string vn = t.desc;
+                RPos& rp = S.at(position);
+                if (!rp.key) continue;
+                std::map<std::string, HVar*> copy = rp.v;
+                if (!copy.count(vn)) continue;
+                rp.dirty = true;
+                HVar* vref = copy.at(vn);
+                std::string mnt = vn;
+                replace_first(mnt, "&", "");
+                for (int i = 0; i < (int)mnt.size() - 1; i++) {
+                    std::string left = sub2(mnt, 0, i + 1);
+                    if (left.size() > 1) left.insert(1, "&");
+                    std::string right = sub1(mnt, -(int)(mnt.size() - (size_t)i - 1));
+                    if (right.size() > 1) right.insert(1, "&");
+                    if (copy.count(left)) {
+                        HVar* tref = copy.at(left);
+                        if (tref->cnt <= 0) continue;
+                        if (tref->cnt < vref->cnt && tref->pmean / tref->cnt <= i + 1) {
+                            adj_cnt(vref, tref, nullptr);
+                            copy.erase(left);
+                        }
+                    }
+                    RPos& rr = S.at(position + i + 1);
+                    if (rr.key && rr.v.count(right)) {
+                        HVar* tref = rr.v.at(right);
+                        if (tref->cnt < 0) continue;
+                        if (tref->cnt < vref->cnt) {
+                            adj_cnt(vref, tref, nullptr);
+                            S.add_cov(position, tref->cnt);
+                            rr.dirty = true;
+                            rr.v.erase(right);
+                        }
+                    }
+                }
+                if (HSclip* s3 = sc3(position)) {
+                    if (!s3->used) {
+                        std::string seq = findconseq(*s3);
+                        if (sub2(seq, 0, (int)mnt.size()) == mnt) {
+                            if (seq.size() == mnt.size() || ismatchref(seq.substr(mnt.size()), ref, position + (int)mnt.size(), 1)) {
+                                RPos& r0 = S.at(position);
+                                HVar*& tgt = r0.v[vn];
+                                if (!tgt) tgt = S.new_var();
+                                r0.dirty = true;
+                                adj_cnt(tgt, &s3->v, nullptr);
+                                S.add_cov(position, s3->v.cnt);
+                                s3->used = true;
+                            }
+                        }
+                    }
+                }
+                if (HSclip* s5 = sc5(position + (int)mnt.size())) {
+                    if (!s5->used) {
+                        std::string seq = findconseq(*s5);
+                        if (seq != " " && seq.size() >= mnt.size()) {
+                            std::reverse(seq.begin(), seq.end());
+                            if (sub2(seq, (int)(seq.size() - mnt.size()), (int)mnt.size()) == mnt) {
+                                if (seq.size() == mnt.size() ||
+                                    ismatchref(seq.substr(0, seq.size() - mnt.size()), ref, position - 1, -1)) {
+                                    RPos& r0 = S.at(position);
+                                    HVar*& tgt = r0.v[vn];
+                                    if (!tgt) tgt = S.new_var();
+                                    r0.dirty = true;
+                                    adj_cnt(tgt, &s5->v, nullptr);
+                                    S.add_cov(position, s5->v.cnt);
+                                    s5->used = true;
+                                }
+                            }
+                        }
+                    }
+                }
+            } catch (const std::exception&) {
+            }
+        }
+    }
+
+    void realigndel(bool withBams, const std::map<int, std::map<std::string, int>>& counts) {
+        const Re& R = res();
+        std::vector<Cand> tmp = sorted_candidates(counts);
+        st.candidates += (int64_t)tmp.size();
+        for (const Cand& t : tmp) {
+            try {
+                int p = t.position;
+                std::string vn = t.desc;
+                int dcnt = t.count;
+                HVar* vref = S.get_variation(p, vn);
+                int dellen = 0;
+                std::smatch sm;
+                if (std::regex_search(vn, sm, R.begin_minus_number)) dellen = std::atoi(sm.str(1).c_str());
+                if (std::regex_search(vn, sm, R.up_number_end)) dellen += std::atoi(sm.str(1).c_str());
+                std::string extrains, extra, inv5, inv3;
+                if (std::regex_search(vn, sm, R.minus_sv)) {
+                    inv5 = sm.str(1);
+                    inv3 = sm.str(2);
+                } else if (std::regex_search(vn, sm, R.begin_minus_number_any)) {
+                    extra = std::regex_replace(sm.str(1), R.strip, "");
+                    std::smatch s2;
+                    if (std::regex_search(vn, s2, R.caret_atgnc)) extrains = s2.str(1);
+                }
+                int wustart = (p - 200) > 1 ? (p - 200) : 1;
+                std::string wupseq = join_ref(ref, wustart, p - 1) + extra;
+                if (!inv3.empty()) wupseq = inv3;
+                int sanend = p + 200 > chrLen ? chrLen : p + 200;
+                int el = (int)extra.size() - (int)extrains.size();
+                std::string sanpseq = extra + join_ref(ref, p + dellen + el, sanend);
+                if (!inv5.empty()) sanpseq = inv5;
+                MM r3 = findMM3(ref, p, sanpseq);
+                MM r5 = findMM5(ref, p + dellen + el - 1, wupseq);
+                std::vector<Mismatch> mmm = r3.mm;
+                mmm.insert(mmm.end(), r5.mm.begin(), r5.mm.end());
+                for (const Mismatch& mis : mmm) {
+                    std::string mm = mis.seq;
+                    int mp = mis.pos, me = mis.end;
+                    if (mm.size() > 1) mm.insert(1, 1, '&');
+                    HVar* tv = S.inner(mp, mm);
+                    if (!tv) continue;
+                    if (tv->cnt == 0) continue;
+                    if (tv->qmean / tv->cnt < c.goodq) continue;
+                    if (tv->pmean / tv->cnt > (me == 3 ? r3.nm + 4 : r5.nm + 4)) continue;
+                    if (tv->cnt >= dcnt + dellen || tv->cnt / dcnt >= 8) continue;
+                    if (mp > p && me == 5) {
+                        double f = tv->pmean != 0 ? (mp - p) / (tv->pmean / (double)tv->cnt) : 1;
+                        if (f > 1) f = 1;
+                        S.add_cov(p, (int)(tv->cnt * f));
+                        adj_ref_cnt(tv, S.maybe(p, ref.get(p)), dellen);
+                    }
+                    HVar* lref = nullptr;
+                    if (mp > p && me == 3 && S.has_key(p)) lref = S.inner(p, std::string(1, ref.get(p)));
+                    adj_cnt(vref, tv, lref);
+                    S.erase_inner(mp, mm, true);
+                }
+                if (r3.misp != 0 && r3.mm.size() == 1) {
+                    HVar* v = S.inner(r3.misp, r3.misnt);
+                    if (v && v->cnt < dcnt) S.erase_inner(r3.misp, r3.misnt, false);
+                }
+                if (r5.misp != 0 && r5.mm.size() == 1) {
+                    HVar* v = S.inner(r5.misp, r5.misnt);
+                    if (v && v->cnt < dcnt) S.erase_inner(r5.misp, r5.misnt, false);
+                }
+                for (int sc5pp : r5.scp) {
+                    HSclip* tv = sc5(sc5pp);
+                    if (tv && !tv->used) {
+                        std::string seq = findconseq(*tv);
+                        if (dcnt <= 2 && tv->v.cnt / dcnt > 5) continue;
+                        if (seq != " " && ismatch(seq, wupseq, -1)) {
+                            if (sc5pp > p) S.add_cov(p, tv->v.cnt);
+                            adj_cnt(vref, &tv->v, nullptr);
+                            tv->used = true;
+                        }
+                    }
+                }
+                for (int sc3pp : r3.scp) {
+                    HSclip* tv = sc3(sc3pp);
+                    if (tv && !tv->used) {
+                        std::string seq = findconseq(*tv);
+                        if (dcnt <= 2 && tv->v.cnt / dcnt > 5) continue;
+                        if (seq != " " && ismatch(seq, sub1(sanpseq, sc3pp - p), 1)) {
+                            if (sc3pp <= p) S.add_cov(p, tv->v.cnt);
+                            HVar* lref = sc3pp <= p ? nullptr : S.maybe(p, ref.get(p));
+                            adj_cnt(vref, &tv->v, lref);
+                            tv->used = true;
+                        }
+                    }
+                }
+                int pe = p + dellen + el;
+                HVar* h = S.maybe(p, ref.get(p));
+                if (withBams && pe - p >= 5 && pe - p < maxlen() - 10 && h && h->cnt != 0 &&
+                    vref->cnt > 2 * h->cnt * (1 - (pe - p) / (double)maxlen()) && noPassingReads(p, pe)) {
+                    adj_cnt(vref, h, h);
+                }
+            } catch (const std::exception&) {
+            }
+        }
+        for (int i = (int)tmp.size() - 1; i > 0; i--) {
+            const Cand& t = tmp[(size_t)i];
+            int p = t.position;
+            const std::string& vn = t.desc;
+            RPos& rp = S.at(p);
+            if (!rp.key) continue;
+            if (!rp.v.count(vn)) continue;
+            HVar* vref = rp.v[vn];
+            std::smatch mt;
+            if (std::regex_search(vn, mt, R.minus_number_amp_atgcs_end)) {
+                std::string tn = mt.str(1);
+                if (rp.v.count(tn)) {
+                    HVar* tref = rp.v[tn];
+                    if (vref->cnt < tref->cnt) {
+                        rp.dirty = true;
+                        adj_cnt(tref, vref, nullptr);
+                        rp.v.erase(vn);
+                    }
+                }
+            }
+        }
+    }
+
+    void realignins(const std::map<int, std::map<std::string, int>>& counts) {
+        const Re& R = res();
+        std::vector<Cand> tmp = sorted_candidates(counts);
+        st.candidates += (int64_t)tmp.size();
+        for (const Cand& t : tmp) {
+            try {
+                int position = t.position;
+                std::string vn = t.desc;
+                int icnt = t.count;
+                std::smatch m;
+                std::string insert1;
+                if (std::regex_search(vn, m, R.begin_plus_atgc))
+                    insert1 = m.str(1);
+                else
+                    continue;
+                std::string ins3;
+                int inslen = (int)insert1.size();
+                if (std::regex_search(vn, m, R.dup_num_atgc)) {
+                    ins3 = m.str(2);
+                    inslen += std::atoi(m.str(1).c_str()) + (int)ins3.size();
+                }
+                std::string extra, compm, newins;
+                if (std::regex_search(vn, m, R.amp_atgc)) extra = m.str(1);
+                if (std::regex_search(vn, m, R.hash_atgc)) compm = m.str(1);
+                if (std::regex_search(vn, m, R.caret_atgc_end)) newins = m.str(1);
+                int newdel = 0;
+                if (std::regex_search(vn, m, R.up_number_end)) newdel = std::atoi(m.str(1).c_str());
+                std::string tn = vn;
+                replace_first_re(tn, R.lead_plus, "");
+                replace_first(tn, "&", "");
+                replace_first(tn, "#", "");
+                replace_first_re(tn, R.caret_digits_end, "");
+                replace_first_re(tn, R.caret, "");
+                int wustart = position - 150 > 1 ? (position - 150) : 1;
+                std::string wupseq = join_ref(ref, wustart, position) + tn;
+                int sanend = position + (int)vn.size() + 100;
+                if (chrLen < sanend) sanend = chrLen;
+                std::string sanpseq;
+                MM r3;
+                if (!ins3.empty()) {
+                    int p3 = position + inslen - (int)ins3.size() + 50;
+                    if (ins3.size() > 50) sanpseq = sub1(ins3, 50 - (int)ins3.size());
+                    sanpseq += join_ref(ref, position + 1, position + 101);
+                    r3 = findMM3(ref, p3 + 1, sanpseq);
+                } else {
+                    sanpseq = tn + join_ref(ref, position + (int)extra.size() + 1 + (int)compm.size() + newdel, sanend);
+                    r3 = findMM3(ref, position + 1, sanpseq);
+                }
+                MM r5 = findMM5(ref, position + (int)extra.size() + (int)compm.size() + newdel, wupseq);
+                std::vector<Mismatch> mmm = r3.mm;
+                mmm.insert(mmm.end(), r5.mm.begin(), r5.mm.end());
+                HVar* vref = S.get_ins(position, vn);
+                for (const Mismatch& mis : mmm) {
+                    std::string mb = mis.seq;
+                    int mp = mis.pos, me = mis.end;
+                    if (mb.size() > 1) mb = std::string(1, mb[0]) + "&" + mb.substr(1);
+                    HVar* v = S.inner(mp, mb);
+                    if (!v) continue;
+                    if (v->cnt == 0) continue;
+                    if (v->qmean / v->cnt < c.goodq) continue;
+                    if (v->pmean / v->cnt > (me == 3 ? r3.nm + 4 : r5.nm + 4)) continue;
+                    if (v->cnt >= icnt + (int)insert1.size() || v->cnt / icnt >= 8) continue;
+                    if (mp > position && me == 5) S.add_cov(position, v->cnt);
+                    HVar* lref = nullptr;
+                    if (mp > position && me == 3 && S.has_key(position) && ref.count(position))
+                        lref = S.inner(position, std::string(1, ref.get(position)));
+                    adj_cnt(vref, v, lref);
+                    S.erase_inner(mp, mb, true);
+                }
+                if (r3.misp != 0 && r3.mm.size() == 1) {
+                    HVar* v = S.inner(r3.misp, r3.misnt);
+                    if (v && v->cnt < icnt) S.erase_inner(r3.misp, r3.misnt, false);
+                }
+                if (r5.misp != 0 && r5.mm.size() == 1) {
+                    HVar* v = S.inner(r5.misp, r5.misnt);
+                    if (v && v->cnt < icnt) S.erase_inner(r5.misp, r5.misnt, false);
+                }
+                for (int sc5pp : r5.scp) {
+                    HSclip* tv = sc5(sc5pp);
+                    if (!tv) continue;
+                    if (!tv->used) {
+                        std::string seq = findconseq(*tv);
+                        if (seq != " " && ismatch(seq, wupseq, -1)) {
+                            if (sc5pp > position) S.add_cov(position, tv->v.cnt);
+                            adj_cnt(vref, &tv->v, nullptr);
+                            tv->used = true;
+                        }
+                    }
+                }
+                for (int sc3pp : r3.scp) {
+                    HSclip* tv = sc3(sc3pp);
+                    if (!tv) continue;
+                    if (!tv->used) {
+                        std::string seq = findconseq(*tv);
+                        std::string mseq = !ins3.empty() ? sanpseq : sub1(sanpseq, sc3pp - position - 1);
+                        if (seq != " " && ismatch(seq, mseq, 1)) {
+                            if (sc3pp <= position || (double)insert1.size() > tv->v.pmean / tv->v.cnt)
+                                S.add_cov(position, tv->v.cnt);
+                            HVar* lref = nullptr;
+                            if (sc3pp > position && S.has_key(position) && ref.count(position))
+                                lref = S.inner(position, std::string(1, ref.get(position)));
+                            if ((double)insert1.size() > tv->v.pmean / tv->v.cnt) lref = nullptr;
+                            adj_cnt(vref, &tv->v, lref);
+                            tv->used = true;
+                            if (insert1.size() + 1 == vn.size() && (int)insert1.size() > maxlen() &&
+                                sc3pp >= position + 1 + (int)insert1.size()) {
+                                int flag = 0;
+                                int off = (sc3pp - position - 1) % (int)insert1.size();
+                                std::string tvn = vn;
+                                for (int si = 0; si < (int)seq.size() && si + off < (int)insert1.size(); si++) {
+                                    if (sub2(seq, si, 1) != sub2(insert1, si + off, 1)) {
+                                        flag++;
+                                        int shift = si + off + 1;
+                                        tvn = tvn.substr(0, (size_t)shift) + sub2(seq, si, 1) + tvn.substr((size_t)shift + 1);
+                                    }
+                                }
+                                if (flag > 0) {
+                                    auto& im = S.ins[position];
+                                    HVar* vv = im[vn];
+                                    im[tvn] = vv;
+                                    im.erase(vn);
+                                }
+                            }
+                        }
+                    }
+                }
+                int first3 = r3.scp[0];
+                int first5 = r5.scp[0];
+                if (!r3.scp.empty() && !r5.scp.empty() && first3 > first5 + 3 && first3 - first5 < maxlen() * 0.75) {
+                    if (ref.count(position) && S.has_key(position)) {
+                        HVar* rv = S.inner(position, std::string(1, ref.get(position)));
+                        if (rv) adj_ref_factor(rv, (first3 - first5 - 1) / (double)maxlen());
+                    }
+                    adj_ref_factor(vref, -(first3 - first5 - 1) / (double)maxlen());
+                }
+            } catch (const std::exception&) {
+            }
+        }
+        for (int i = (int)tmp.size() - 1; i > 0; i--) {
+            try {
+                const Cand& t = tmp[(size_t)i];
+                int p = t.position;
+                const std::string& vn = t.desc;
+                auto mi = S.ins.find(p);
+                if (mi == S.ins.end()) continue;
+                auto& m = mi->second;
+                if (!m.count(vn)) continue;
+                HVar* vref = m.at(vn);
+                std::smatch mt;
+                if (std::regex_search(vn, mt, R.plus_atgcs_amp_atgcs_end)) {
+                    std::string tn = mt.str(1);
+                    if (m.count(tn)) {
+                        HVar* tref = m.at(tn);
+                        if (vref->cnt < tref->cnt) {
+                            adj_cnt(tref, vref, S.maybe(p, ref.get(p)));
+                            m.erase(vn);
+                        }
+                    }
+                }
+            } catch (const std::exception&) {
+            }
+        }
+    }
+
+    // find35match (VariationRealigner.cpp:1396-1428)
+    void find35match(const std::string& seq5, const std::string& seq3, int& b5, int& b3, int& score) {
+        b5 = b3 = score = 0;
+        for (size_t i = 0; i < seq5.size() - 8; i++) {
+            for (size_t j = 1; j < seq3.size() - 8; j++) {
+                int nm = 0, n = 0;
+                while (n + (int)j <= (int)seq3.size() && (int)i + n <= (int)seq5.size()) {
+                    if (sub2(seq3, -(int)j - n, 1) != sub2(seq5, (int)i + n, 1)) nm++;
+                    if (nm > 2) break;
+                    n++;
+                }
+                if (n - nm > score && n - nm > 8 && nm / (double)n < 0.1 &&
+                    (n + (int)j >= (int)seq3.size() || (int)i + n >= (int)seq5.size())) {
+                    score = n - nm;
+                    b3 = (int)j;
+                    b5 = (int)i;
+                    return;
+                }
+            }
+        }
+    }
+
+    void realignlgins30() {
+        HostRef rf = ref;  // the reference copies the whole map (VariationRealigner.cpp:1120)
+        struct SP {
+            int position;
+            HSclip* sc;
+            int count;
+        };
+        std::vector<SP> t5, t3;
+        for (auto& e : S.P.sc5)
+            if (!(e.first < rstart - 5000 || e.first > rend + 5000)) t5.push_back({e.first, &e.second, e.second.v.cnt});
+        for (auto& e : S.P.sc3)
+            if (!(e.first < rstart - 5000 || e.first > rend + 5000)) t3.push_back({e.first, &e.second, e.second.v.cnt});
+        auto comp3 = [](const SP& a, const SP& b) {
+            if (b.count != a.count) return a.count > b.count;
+            return a.position < b.position;
+        };
+        std::sort(t5.begin(), t5.end(), comp3);
+        std::sort(t3.begin(), t3.end(), comp3);
+        st.candidates += (int64_t)(t5.size() + t3.size());
+        for (const SP& a : t5) {
+            int p5 = a.position;
+            HSclip* sc5v = a.sc;
+            int cnt5 = a.count;
+            if (sc5v->used) continue;
+            std::string seq5 = findconseq(*sc5v);
+            if (seq5.size() <= 10) continue;
+            for (const SP& b : t3) {
+                try {
+                    int p3 = b.position;
+                    HSclip* sc3v = b.sc;
+                    int cnt3 = b.count;
+                    if (sc5v->used) break;
+                    if (sc3v->used) continue;
+                    if (p5 - p3 > maxlen() * 2.5) continue;
+                    if (p3 - p5 > maxlen() - 10) continue;
+                    std::string seq3 = findconseq(*sc3v);
+                    if (seq3.size() <= 10) continue;
+                    if (!(cnt5 / (double)cnt3 >= 0.08 && cnt5 / (double)cnt3 <= 12)) continue;
+                    int bp5, bp3, score;
+                    find35match(seq5, seq3, bp5, bp3, score);
+                    if (score == 0) continue;
+                    int smscore = (int)(score / 2);
+                    std::string ins = bp3 + smscore > 1 ? sub2(seq3, 0, -(bp3 + smscore) + 1) : seq3;
+                    if (bp5 + smscore > 0) {
+                        std::string tmpstr = seq5.substr(0, (size_t)(bp5 + smscore));
+                        std::reverse(tmpstr.begin(), tmpstr.end());
+                        ins += tmpstr;
+                    }
+                    if (low_complexity(ins)) continue;
+                    int bi = 0;
+                    HVar* vref;
+                    if (p5 > p3) {
+                        if (seq3.size() > ins.size() &&
+                            !ismatch(seq3.substr(ins.size()), join_ref(rf, p5, (int)(p5 + seq3.size() - ins.size() + 2)), 1))
+                            continue;
+                        if (seq5.size() > ins.size() &&
+                            !ismatch(seq5.substr(ins.size()), join_ref(rf, (int)(p3 - (seq5.size() - ins.size()) - 2), p3 - 1), -1))
+                            continue;
+                        std::string tmp = join_ref(rf, p3, p5 - 1);
+                        if (tmp.size() > ins.size()) {
+                            ins = std::to_string(p3 - p5) + "^" + ins;
+                            bi = p3;
+                            vref = S.get_variation(p3, ins);
+                        } else if (tmp.size() < ins.size()) {
+                            ins = sub2(ins, 0, (int)(ins.size() - tmp.size())) + "&" + sub1(ins, p3 - p5);
+                            ins = "+" + ins;
+                            bi = p3 - 1;
+                            vref = S.get_ins(bi, ins);
+                        } else {
+                            ins = "-" + std::to_string(ins.size()) + "^" + ins;
+                            bi = p3;
+                            vref = S.get_variation(p3, ins);
+                        }
+                    } else {
+                        if (seq3.size() > ins.size() &&
+                            !ismatch(seq3.substr(ins.size()), join_ref(rf, p5, (int)(p5 + seq3.size() - ins.size() + 2)), 1))
+                            continue;
+                        if (seq5.size() > ins.size() &&
+                            !ismatch(seq5.substr(ins.size()), join_ref(rf, (int)(p3 - (seq5.size() - ins.size()) - 2), p3 - 1), -1))
+                            continue;
+                        std::string tmp;
+                        if (ins.size() <= (size_t)(p3 - p5)) {
+                            int rpt = 2, tnr = 3;
+                            while (((p3 - p5 + ins.size()) / (double)tnr) / (double)ins.size() > 1) {
+                                if ((p3 - p5 + ins.size()) % (size_t)tnr == 0) rpt++;
+                                tnr++;
+                            }
+                            tmp += join_ref_d(rf, p5, (p5 + (p3 - p5 + ins.size()) / (double)rpt - ins.size()));
+                            ins = "+" + tmp + ins;
+                        } else {
+                            tmp += join_ref(rf, p5, p3 - 1);
+                            if ((ins.size() - tmp.size()) % 2 == 0) {
+                                int tex = (int)((ins.size() - tmp.size()) / 2);
+                                ins = (tmp + sub2(ins, 0, tex)) == sub1(ins, tex) ? ("+" + sub1(ins, tex)) : "+" + tmp + ins;
+                            } else {
+                                ins = "+" + tmp + ins;
+                            }
+                        }
+                        bi = p5 - 1;
+                        vref = S.get_ins(bi, ins);
+                    }
+                    sc3v->used = true;
+                    sc5v->used = true;
+                    vref->pstd = true;
+                    vref->qstd = true;
+                    S.add_cov(bi, sc5v->v.cnt);
+                    if (ins[0] == '+') {
+                        HVar* mvref = S.maybe(bi, rf.get(bi));
+                        adj_cnt(vref, &sc3v->v, mvref);
+                        adj_cnt(vref, &sc5v->v, nullptr);
+                        if (p3 - p5 >= 5 && p3 - p5 < maxlen() - 10 && mvref && mvref->cnt != 0 && vref->cnt > 2 * mvref->cnt &&
+                            noPassingReads(p5, p3)) {
+                            adj_cnt(vref, mvref, mvref);
+                        }
+                        std::map<int, std::map<std::string, int>> tins;
+                        tins[bi][ins] = vref->cnt;
+                        realignins(tins);
+                    } else if (ins[0] == '-') {
+                        adj_cnt(vref, &sc3v->v, S.maybe(bi, rf.get(bi)));
+                        adj_cnt(vref, &sc5v->v, nullptr);
+                        std::map<int, std::map<std::string, int>> tdel;
+                        tdel[bi][ins] = vref->cnt;
+                        realigndel(false, tdel);
+                    } else {
+                        adj_cnt(vref, &sc3v->v, nullptr);
+                        adj_cnt(vref, &sc5v->v, nullptr);
+                    }
+                    break;
+                } catch (const std::exception&) {
+                }
+            }
+        }
+    }
+};
+
+}  // namespace
+
+void realign(const Config& c, int region_start, int region_end, HostRef& ref, int chrLen, const HostReads* reads,
+             RState& S, RealignStats& st) {
+    // prefetch the rows around every candidate in one transfer
+    std::vector<int> cand;
+    for (auto* m : {&S.P.mnp, &S.P.delCount, &S.P.insCount})
+        for (auto& e : *m) cand.push_back(e.first);
+    for (auto& e : S.P.sc5) cand.push_back(e.first);
+    for (auto& e : S.P.sc3) cand.push_back(e.first);
+    for (auto& e : S.P.nd) cand.push_back(e.first);
+    for (auto& e : S.P.ins) cand.push_back(e.first);
+    S.rows.prefetch(cand, 256);
+    Realigner rg{c, region_start, region_end, ref, chrLen, reads, S, st};
+    rg.adjustMNP();
+    if (c.performLocalRealignment) {
+        rg.realigndel(true, S.P.delCount);
+        rg.realignins(S.P.insCount);
+        rg.realignlgins30();
+    }
+    st.touched = (int64_t)S.pos.size();
+    for (auto& e : S.pos)
+        if (e.second.dirty) st.changed++;
+}
+
+}  // namespace rv

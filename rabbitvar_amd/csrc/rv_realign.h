@@ -1,0 +1,118 @@
+// Local realignment of one region (host side of the engine).
+//
+// The realignment stage of the reference (src/VariationRealigner.cpp:135-161) rewrites the pileup
+// around candidate indels, MNVs and soft-clip clusters: a few thousand positions of a region whose
+// pileup has millions.  It runs here on the host over a position store that materializes the
+// device's dense rows lazily (block fetches through RowSource) and records which positions it
+// changed; the scoring kernels then read changed positions from a record list and every other
+// position straight from the dense rows still resident in HBM.
+#pragma once
+#include <cstdint>
+#include <deque>
+#include <map>
+#include <memory>
+#include <set>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "rv_host.h"
+
+namespace rv {
+
+// Source of dense rows (positions dense_lo + i).  fetch() fills rows for the given block ids
+// (block b covers positions dense_lo + b*BLOCK .. +BLOCK-1, clipped to the window).
+class RowSource {
+  public:
+    static constexpr int BLOCK = 64;
+    virtual ~RowSource() {}
+    virtual bool fetch(const std::vector<int>& blocks, std::vector<DenseRow>& out) = 0;
+};
+
+// Host rows already downloaded (tests, diagnostics).
+class HostRowSource : public RowSource {
+  public:
+    explicit HostRowSource(const HostPileup& P) : P_(P) {}
+    bool fetch(const std::vector<int>& blocks, std::vector<DenseRow>& out) override;
+
+  private:
+    const HostPileup& P_;
+};
+
+class RowCache {
+  public:
+    RowCache(RowSource* src, int dense_lo, int dense_n) : src_(src), lo_(dense_lo), n_(dense_n) {}
+    // fetches, in one transfer, every block within `radius` of the given positions
+    bool prefetch(const std::vector<int>& positions, int radius);
+    const DenseRow* row(int pos);  // nullptr outside the dense window
+    int64_t fetches = 0, blocks_fetched = 0;
+
+  private:
+    bool load(const std::vector<int>& blocks);
+    RowSource* src_;
+    int lo_, n_;
+    std::unordered_map<int, std::unique_ptr<DenseRow[]>> blocks_;
+};
+
+// The reference window as the reference's map: keys [lo, hi]; get() of a missing key creates it
+// holding '\0' (robin_hood operator[]), at() of a missing key throws.
+struct HostRef {
+    const char* seq = nullptr;
+    int lo = 1, hi = 0;
+    std::set<int> created;
+    bool count(int k) const { return (k >= lo && k <= hi) || created.count(k) != 0; }
+    char at(int k) const;
+    char get(int k);
+};
+
+// Region reads for noPassingReads (host arrays).
+struct HostReads {
+    int64_t n = 0;
+    const int32_t* pos0 = nullptr;
+    const int64_t* cigar_off = nullptr;
+    const uint32_t* cigar = nullptr;
+};
+
+struct RPos {
+    bool loaded = false;
+    bool key = false;   // nonInsertionVariants has the position
+    int cov = 0;        // refCoverage
+    bool dirty = false;
+    std::map<std::string, HVar*> v;  // nonInsertionVariants[pos]
+};
+
+// Pileup state the realigner reads and writes.  Records live in a pool so that pointers stay valid
+// across renames and erasures, as the reference's heap-allocated Variation objects do.
+class RState {
+  public:
+    RState(HostPileup& P, RowCache& rows);
+    RPos& at(int pos);
+    bool has_key(int pos) { return at(pos).key; }
+    HVar* inner(int pos, const std::string& d);
+    HVar* get_variation(int pos, const std::string& d);  // getVariation on nonInsertionVariants
+    HVar* get_ins(int pos, const std::string& d);        // getVariation on insertionVariants
+    HVar* maybe(int pos, char refBase);                  // getVariationMaybe
+    void erase_inner(int pos, const std::string& d, bool drop_empty);
+    void add_cov(int pos, int k);
+    HVar* new_var();
+
+    HostPileup& P;
+    RowCache& rows;
+    std::deque<HVar> pool;
+    std::unordered_map<int, RPos> pos;
+    std::map<int, std::map<std::string, HVar*>> ins;
+    std::vector<std::map<std::string, HVar*>> graveyard;  // maps of erased positions
+};
+
+struct RealignStats {
+    int64_t candidates = 0, touched = 0, changed = 0;
+};
+
+// adjustMNP, realigndel, realignins, realignlgins30 in the reference's order.
+void realign(const Config& c, int region_start, int region_end, HostRef& ref, int chrLen, const HostReads* reads,
+             RState& S, RealignStats& st);
+
+// soft-clip consensus (include/VariationUtils.h:403-509)
+std::string findconseq(HSclip& sc);
+
+}  // namespace rv

@@ -1,14 +1,286 @@
-// Region driver.
+// Region driver: pileup (device) -> realignment (host, on the changed positions only) -> scoring
+// (device) -> simple-mode text.  Replaces the body of one_region_run and the printer that follows
+// it (src/modes/simpleMode.cpp:18-64, :144-208; src/ToVarsBuilder.cpp:97-205).
 #include "rv_region.h"
 
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+#include "rv_format.h"
+#include "rv_realign.h"
+
 namespace rv {
+
+namespace {
+
+class DeviceRowSource : public RowSource {
+  public:
+    explicit DeviceRowSource(Engine& e) : e_(e) {}
+    bool fetch(const std::vector<int>& blocks, std::vector<DenseRow>& out) override { return e_.fetch_blocks(blocks, out); }
+
+  private:
+    Engine& e_;
+};
+
+HVarD to_dev(const HVar& v) {
+    HVarD d;
+    d.cnt = v.cnt;
+    d.fwd = v.fwd;
+    d.rev = v.rev;
+    d.hicnt = v.hicnt;
+    d.locnt = v.locnt;
+    d.extracnt = v.extracnt;
+    d.pmean = v.pmean;
+    d.qmean = v.qmean;
+    d.mqmean = v.mqmean;
+    d.nm = v.nm;
+    d.pstd = v.pstd ? 1 : 0;
+    d.qstd = v.qstd ? 1 : 0;
+    return d;
+}
+
+// Positions the scoring stage reads from records instead of the dense rows.
+struct CxHost {
+    std::vector<int32_t> pos, off, cov;
+    std::vector<uint8_t> key;
+    std::vector<CxEntry> ent;
+    std::string arena;
+};
+
+void build_cx(RState& S, CxHost& cx) {
+    std::vector<int> ps;
+    for (auto& e : S.P.nd) ps.push_back(e.first);
+    for (auto& e : S.ins) ps.push_back(e.first);
+    for (auto& e : S.pos)
+        if (e.second.dirty) ps.push_back(e.first);
+    std::sort(ps.begin(), ps.end());
+    ps.erase(std::unique(ps.begin(), ps.end()), ps.end());
+    S.rows.prefetch(ps, 0);
+    cx.off.push_back(0);
+    auto add = [&](int kind, const std::string& d, const HVar& v) {
+        CxEntry c;
+        c.kind = kind;
+        c.desc_off = (uint32_t)cx.arena.size();
+        c.desc_len = (int32_t)d.size();
+        c.pad = 0;
+        c.v = to_dev(v);
+        cx.arena += d;
+        cx.ent.push_back(c);
+    };
+    for (int p : ps) {
+        RPos& r = S.at(p);
+        cx.pos.push_back(p);
+        cx.cov.push_back(r.cov);
+        cx.key.push_back(r.key ? 1 : 0);
+        if (r.key)
+            for (auto& kv : r.v) add(0, kv.first, *kv.second);
+        auto it = S.ins.find(p);
+        if (it != S.ins.end())
+            for (auto& kv : it->second) add(1, kv.first, *kv.second);
+        cx.off.push_back((int32_t)cx.ent.size());
+    }
+}
+
+}  // namespace
+
+#define RV_HIPS(x)                                                                                   \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess) return e.fail(RV_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
 
 rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, const rv_contig& contig, CallSet& out) {
     PileupResult pr;
     rv_status s = e.pileup(region, reads, contig, false, pr);
     if (s != RV_OK) return s;
+    auto h0 = std::chrono::steady_clock::now();
+    // ---- realignment on the host
+    DeviceRowSource src(e);
+    RowCache rc(&src, pr.P.dense_lo, pr.P.dense_n);
+    RState S(pr.P, rc);
+    HostRef ref;
+    ref.seq = pr.ref_win.data();
+    ref.lo = pr.W.ref_lo;
+    ref.hi = pr.W.ref_lo + (int)pr.ref_win.size() - 1;
+    std::vector<int32_t> h_pos0;
+    std::vector<int64_t> h_coff;
+    std::vector<uint32_t> h_cig;
+    HostReads hr;
+    hr.n = reads.n_reads;
+    if (!reads.on_device) {
+        hr.pos0 = reads.pos0;
+        hr.cigar_off = reads.cigar_off;
+        hr.cigar = reads.cigar;
+    } else if (reads.n_reads > 0) {
+        h_pos0.resize((size_t)reads.n_reads);
+        h_coff.resize((size_t)reads.n_reads + 1);
+        RV_HIPS(hipMemcpy(h_pos0.data(), reads.pos0, 4 * h_pos0.size(), hipMemcpyDeviceToHost));
+        RV_HIPS(hipMemcpy(h_coff.data(), reads.cigar_off, 8 * h_coff.size(), hipMemcpyDeviceToHost));
+        h_cig.resize((size_t)h_coff.back());
+        if (!h_cig.empty()) RV_HIPS(hipMemcpy(h_cig.data(), reads.cigar, 4 * h_cig.size(), hipMemcpyDeviceToHost));
+        hr.pos0 = h_pos0.data();
+        hr.cigar_off = h_coff.data();
+        hr.cigar = h_cig.data();
+    }
+    RealignStats rst;
+    try {
+        realign(e.cfg, region.start, region.end, ref, contig.length, &hr, S, rst);
+    } catch (const std::exception& ex) {
+        return e.fail(RV_ERR_DEVICE, std::string("realignment: ") + ex.what());
+    }
+    CxHost cx;
+    try {
+        build_cx(S, cx);
+    } catch (const std::exception& ex) {
+        return e.fail(RV_ERR_DEVICE, std::string("record list: ") + ex.what());
+    }
+    double host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+    // ---- scoring on the device
+    const int ncx = (int)cx.pos.size();
+    ScoreIn in{};
+    in.rows = e.rows;
+    in.dense_lo = pr.P.dense_lo;
+    in.dense_n = pr.P.dense_n;
+    in.n_cx = ncx;
+    in.ref = e.ref_dev;
+    in.ref_lo = ref.lo;
+    in.ref_hi = ref.hi;
+    in.chr_len = contig.length;
+    in.region_start = region.start;
+    in.region_end = region.end;
+    in.cfg = e.cfg;
+    auto up = [&](DevBuf& b, const void* src, size_t bytes) -> void* {
+        void* d = e.grow(b, bytes);
+        if (d && bytes) {
+            if (hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, e.stream) != hipSuccess) return nullptr;
+        }
+        return d;
+    };
+    in.cx_pos = (const int32_t*)up(e.b_cx_pos, cx.pos.data(), 4 * cx.pos.size());
+    in.cx_off = (const int32_t*)up(e.b_cx_off, cx.off.data(), 4 * cx.off.size());
+    in.cx = (const CxEntry*)up(e.b_cx, cx.ent.data(), sizeof(CxEntry) * cx.ent.size());
+    in.cx_arena = (const char*)up(e.b_cx_arena, cx.arena.data(), cx.arena.size());
+    in.cx_cov = (const int32_t*)up(e.b_cx_cov, cx.cov.data(), 4 * cx.cov.size());
+    in.cx_key = (const uint8_t*)up(e.b_cx_key, cx.key.data(), cx.key.size());
+    if (!in.cx_pos || !in.cx_off || !in.cx || !in.cx_arena || !in.cx_cov || !in.cx_key)
+        return e.fail(RV_ERR_OOM, "device buffers for the record list failed");
+    unsigned max_out = (unsigned)std::min<int64_t>(65536 + 8 * (int64_t)ncx + pr.P.dense_n / 16, 0x7fffffff);
+    unsigned long long arena_cap = 160ull * max_out;
+    std::vector<OutRec> recs;
+    std::vector<char> arena;
+    unsigned long long hc[8];
+    float score_ms = 0;
+    for (int attempt = 0; attempt < 3; attempt++) {
+        in.out = (OutRec*)e.grow(e.b_out, sizeof(OutRec) * max_out);
+        in.arena = (char*)e.grow(e.b_out_arena, (size_t)arena_cap);
+        unsigned long long* ctr = (unsigned long long*)e.grow(e.b_score_ctr, 64);
+        if (!in.out || !in.arena || !ctr) return e.fail(RV_ERR_OOM, "device buffers for the scored records failed");
+        RV_HIPS(hipMemsetAsync(ctr, 0, 64, e.stream));
+        in.n_out = (unsigned*)ctr;
+        in.arena_used = ctr + 1;
+        in.counters = ctr + 2;
+        in.max_out = max_out;
+        in.arena_cap = arena_cap;
+        RV_HIPS(hipEventRecord(e.sev[0], e.stream));
+        RV_HIPS(launch_score(in, e.stream));
+        RV_HIPS(hipEventRecord(e.sev[1], e.stream));
+        RV_HIPS(hipMemcpyAsync(hc, ctr, 64, hipMemcpyDeviceToHost, e.stream));
+        RV_HIPS(hipStreamSynchronize(e.stream));
+        RV_HIPS(hipEventElapsedTime(&score_ms, e.sev[0], e.sev[1]));
+        unsigned n_out = (unsigned)(hc[0] & 0xffffffffu);
+        unsigned long long used = hc[1];
+        if (n_out <= max_out && used <= arena_cap) {
+            if (hc[2] != 0) {
+                // a position needed more alleles or string space than a scoring thread holds
+                unsigned long long aused = used;
+                (void)aused;
+                return e.fail(RV_ERR_CAPACITY, "scoring: " + std::to_string(hc[2]) +
+                                                   " position(s) exceeded the per-thread allele/string capacity");
+            }
+            recs.resize(n_out);
+            arena.resize((size_t)used + 1);
+            if (n_out) RV_HIPS(hipMemcpyAsync(recs.data(), in.out, sizeof(OutRec) * n_out, hipMemcpyDeviceToHost, e.stream));
+            if (used) RV_HIPS(hipMemcpyAsync(arena.data(), in.arena, (size_t)used, hipMemcpyDeviceToHost, e.stream));
+            RV_HIPS(hipStreamSynchronize(e.stream));
+            break;
+        }
+        if (attempt == 2) return e.fail(RV_ERR_CAPACITY, "scored record buffers kept overflowing");
+        max_out = std::max<unsigned>(max_out, n_out) * 2;
+        arena_cap = std::max(arena_cap, used) * 2;
+    }
+    // ---- text, in position order
+    auto t0 = std::chrono::steady_clock::now();
+    std::sort(recs.begin(), recs.end(), [](const OutRec& a, const OutRec& b) {
+        if (a.position != b.position) return a.position < b.position;
+        return a.order < b.order;
+    });
+    LineContext ctx{e.sample.c_str(), region.gene ? region.gene : "", region.chr ? region.chr : "", region.start, region.end,
+                    e.cfg.fisher != 0};
     out.text.clear();
-    return e.fail(RV_ERR_UNSUPPORTED, "scoring stage not built yet");
+    out.calls.clear();
+    out.strings.clear();
+    out.strings.reserve(recs.size());
+    out.calls.reserve(recs.size());
+    for (const OutRec& o : recs) {
+        format_line(out.text, o, arena.data(), ctx);
+        CallStrings cs;
+        const char* p = arena.data() + o.str_off;
+        std::string* dst[6] = {&cs.refallele, &cs.varallele, &cs.genotype, &cs.bias, &cs.leftseq, &cs.rightseq};
+        for (int k = 0; k < 6; k++) {
+            dst[k]->assign(p, o.len[k]);
+            p += o.len[k];
+        }
+        cs.vartype = vartype_name(o, arena.data());
+        out.strings.push_back(std::move(cs));
+    }
+    for (size_t i = 0; i < recs.size(); i++) {
+        const OutRec& o = recs[i];
+        const CallStrings& cs = out.strings[i];
+        rv_call c;
+        c.position = o.position;
+        c.start = o.start;
+        c.end = o.end;
+        c.total_pos_coverage = o.tpc;
+        c.position_coverage = o.poscov;
+        c.ref_fwd = o.rfc;
+        c.ref_rev = o.rrc;
+        c.var_fwd = o.fwd;
+        c.var_rev = o.rev;
+        c.shift3 = o.shift3;
+        c.msint = o.msint;
+        c.hicnt = o.hicnt;
+        c.hicov = o.hicov;
+        c.frequency = o.freq;
+        c.mean_position = o.pmean;
+        c.mean_quality = o.qmean;
+        c.mean_mapq = o.mq;
+        c.hi_lo_ratio = o.hlr;
+        c.hi_freq = o.hifreq;
+        c.extra_freq = o.extrafreq;
+        c.msi = o.msi;
+        c.nm = o.nm;
+        c.fisher_p = o.fisher_p;
+        c.fisher_ratio = o.fisher_ratio;
+        c.pstd = o.pstd;
+        c.qstd = o.qstd;
+        c.refallele = cs.refallele.c_str();
+        c.varallele = cs.varallele.c_str();
+        c.genotype = cs.genotype.c_str();
+        c.bias = cs.bias.c_str();
+        c.leftseq = cs.leftseq.c_str();
+        c.rightseq = cs.rightseq.c_str();
+        c.vartype = cs.vartype.c_str();
+        out.calls.push_back(c);
+    }
+    host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    e.stats.score_ms = score_ms;
+    e.stats.host_ms += host_ms;
+    e.stats.total_device_ms = e.stats.prep_ms + e.stats.walk_ms + e.stats.anchor_ms + score_ms;
+    e.last_cx_positions = ncx;
+    e.last_row_blocks = rc.blocks_fetched;
+    e.last_insertion_side_effects = (int64_t)hc[3];
+    return RV_OK;
 }
 
 }  // namespace rv

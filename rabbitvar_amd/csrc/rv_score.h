@@ -1,0 +1,69 @@
+// Per-position scoring (device): restates ToVarsBuilder::process for one position
+// (reference src/ToVarsBuilder.cpp:97-205 and the helpers it calls, :213-1116), the simple-mode
+// output filter (src/modes/simpleMode.cpp:144-208 with include/Variant.h:99-248) and the two-sided
+// Fisher exact test the printer adds with --fisher (src/modes/simpleMode.cpp:96-108; htslib's
+// kt_fisher_exact, whose published algorithm is restated in fisher_two_sided below).
+//
+// One thread scores one position.  Positions whose alleles are all single-base A/C/G/T rows read
+// the dense pileup table; positions the host touched (non-dense descriptors, insertions,
+// realignment results) read a CSR record list.  Printed variants are appended to an output
+// buffer with their strings in an arena; the host only formats numbers.
+//
+// This header holds the layouts shared with the host; the device code is in rv_score_body.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "rv_common.h"
+
+namespace rv {
+
+struct HVarD {
+    int32_t cnt, fwd, rev, hicnt, locnt, extracnt;
+    double pmean, qmean, mqmean, nm;
+    int32_t pstd, qstd;
+};
+
+struct CxEntry {
+    int32_t kind;  // 0 nonInsertionVariants, 1 insertionVariants
+    uint32_t desc_off;
+    int32_t desc_len;
+    int32_t pad;
+    HVarD v;
+};
+
+struct OutRec {
+    int32_t position, order, start, end;
+    int32_t tpc, poscov, rfc, rrc, fwd, rev, shift3, msint, hicnt, hicov, pstd, qstd, vartype;
+    int32_t nfields;  // 5 for a pileup line without a reference variant
+    double freq, pmean, qmean, mq, hlr, hifreq, extrafreq, msi, nm, fisher_p, fisher_ratio;
+    uint32_t str_off;
+    uint16_t len[6];  // refallele, varallele, genotype, bias, leftseq, rightseq
+};
+
+enum : int { VT_SNV = 0, VT_INS = 1, VT_DEL = 2, VT_COMPLEX = 3, VT_EMPTY = 4, VT_SV = 5 };
+
+struct ScoreIn {
+    const DenseRow* rows;
+    int dense_lo, dense_n;
+    const int32_t* cx_pos;
+    int n_cx;
+    const int32_t* cx_off;
+    const CxEntry* cx;
+    const char* cx_arena;
+    const int32_t* cx_cov;
+    const uint8_t* cx_key;
+    const char* ref;
+    int ref_lo, ref_hi, chr_len;
+    int region_start, region_end;
+    Config cfg;
+    OutRec* out;
+    unsigned* n_out;
+    unsigned max_out;
+    char* arena;
+    unsigned long long* arena_used;
+    unsigned long long arena_cap;
+    unsigned long long* counters;  // [0] capacity overflows, [1] insertion side effects on p+1,
+                                   // [2] sorts above 16 elements, [3] qstd ambiguous rows
+};
+
+}  // namespace rv
