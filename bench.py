@@ -1,0 +1,154 @@
+"""Throughput of the calling path (pileup -> realignment -> calls) on one MI355X.
+
+Workload: a 5 Mb synthetic contig with 2% tandem-repeat tracts, 60x of 150 bp paired-end reads,
+SNVs at 1/1000 bp, 1-30 bp indels at 1/8000 bp, 200 deletions of 50-300 bp with the reads across
+their breakpoints soft-clipped; one region covering the whole contig, --fisher.  The reads are made
+by rabbitvar_amd/_build/rvsynth (seeded) and uploaded to HBM before timing; one step is one full
+region call (device pileup, host realignment of the candidates, device scoring, text).
+
+Prints one JSON line (see the keys below).  `cpu_baseline` times the CPU restatement of the same
+path (oracle/_build/rvcpu, single thread) on a 2 Mb slice of the same data.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+SYNTH = os.path.join(ROOT, "rabbitvar_amd", "_build", "rvsynth")
+ORACLE = os.path.join(ROOT, "oracle", "_build", "rvcpu")
+
+METRIC = "reads/s pileup→realign→calls, 1 GPU; screened-mode speedup at ≥99.9% recall"
+WORKLOAD = ["--seed", "7", "--len", "5000000", "--depth", "60", "--indel-rate", "0.000125", "--indel-max", "30",
+            "--long-dels", "200", "--repeat-frac", "0.02"]
+CPU_SAMPLE = (1, 2000000)
+
+# sizes of the walk kernel's fixed per-read inputs (bytes): read fields (pos0 4, flag 2, mapq 1,
+# has_nm 1, nm 4, mpos0 4, tid 4, mtid 4, seq_off 8, cigar_off 8) and the prep record (32)
+READ_FIELDS = 40
+PREP_REC = 32
+DENSE_ROW = 288
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def ensure_data(d):
+    if os.path.exists(os.path.join(d, "reads.rvb")) and os.path.exists(os.path.join(d, "ref.fa")):
+        return
+    os.makedirs(d, exist_ok=True)
+    log(f"generating the workload into {d}")
+    subprocess.run([SYNTH, "--out", d] + WORKLOAD, check=True, stdout=subprocess.DEVNULL)
+
+
+def cpu_baseline(d):
+    if not os.path.exists(ORACLE):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    s, e = CPU_SAMPLE
+    out = os.path.join(d, "cpu_sample.txt")
+    log(f"timing the CPU path on chrS:{s}-{e}")
+    r = subprocess.run([ORACLE, "-G", os.path.join(d, "ref.fa"), "-b", os.path.join(d, "reads.rvb"), "-N", "S1",
+                        "-R", f"chrS:{s}-{e}", "--fisher", "--time", "--out", out],
+                       check=True, capture_output=True, text=True)
+    t = json.loads(r.stdout.strip().splitlines()[-1])
+    return {"value": round(t["reads_per_s"], 1), "unit": "reads/s", "cores": 1, "kind": "port",
+            "sample": f"chrS:{s}-{e} of the same workload ({t['reads']} reads, {t['seconds']:.1f} s, "
+                      f"single-threaded CPU restatement of the path)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--data", default=os.environ.get("RV_BENCH_DATA", "/tmp/rv_bench_5mb_60x"))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU: the calling path has no multi-GPU mode here (asked for {a.gpus}); "
+              "regions would be sharded across GPUs by the caller.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    sys.path.insert(0, ROOT)
+    from rabbitvar_amd import read_fasta, read_rvb
+    from rabbitvar_amd.engine import DeviceReads, Engine
+
+    ensure_data(a.data)
+    name, seq = read_fasta(os.path.join(a.data, "ref.fa"))
+    batch = read_rvb(os.path.join(a.data, "reads.rvb"))
+    L = len(seq)
+    log(f"{batch.n} reads over {L} bp; uploading to HBM")
+    dev = DeviceReads(batch)
+    eng = Engine(sample="S1", fisher=1)
+
+    def step():
+        return eng.run_region(name, 1, L, dev, name, seq)
+
+    for i in range(a.warmup):
+        t0 = time.perf_counter()
+        text = step()
+        log(f"warmup {i}: {time.perf_counter() - t0:.3f} s, {len(text.splitlines())} calls")
+    walk_ms = []
+    stage = {k: 0.0 for k in ("prep_ms", "walk_ms", "anchor_ms", "score_ms", "host_ms")}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        step()
+        st = eng.stats()
+        walk_ms.append(st["walk_ms"])
+        for k in stage:
+            stage[k] += st[k]
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    log(f"timed {a.steps} steps in {dt:.3f} s")
+    reads_per_s = batch.n * a.steps / dt
+
+    # walk kernel: algorithmic bytes per launch
+    lseq = np.diff(batch.seq_off).astype(np.int64)
+    ncig = np.diff(batch.cigar_off).astype(np.int64)
+    read_bytes = int((READ_FIELDS + PREP_REC + 2 * lseq + 4 * ncig).sum())
+    st = eng.stats()
+    alg_bytes = read_bytes + 2 * DENSE_ROW * st["dense_rows"] + st["event_bytes"]
+    avg_walk_s = sum(walk_ms) / len(walk_ms) / 1e3
+    achieved = alg_bytes / avg_walk_s / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "walk_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    res = {
+        "metric": METRIC,
+        "value": round(reads_per_s, 1),
+        "unit": "reads/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(dt / a.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32+f64",
+        "data": "synthetic",
+        "config": {"workload": "5 Mb contig, 60x 150 bp PE, SNV 1e-3, indels 1-30 bp at 1/8000, 200 deletions "
+                               "50-300 bp, 2% tandem repeats; one region chrS:1-5000000, --fisher",
+                   "reads": batch.n, "region_bp": L},
+        "stages_ms_per_step": {k: round(v / a.steps, 3) for k, v in stage.items()},
+        "roofline": {"kernel": "k_walk", "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": round(avg_walk_s * 1e3, 4)},
+    }
+    if not a.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline(a.data)
+    eng.close()
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -288,6 +288,10 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
             stats.clips = counts[1];
             stats.ref_key_creations = (int64_t)hc.ref_insert;
             stats.walk_errors = (int64_t)hc.walk_errors;
+            stats.dense_rows = W.dense_n;
+            stats.event_bytes = (int64_t)(sizeof(Event) * counts[0] + aused + sizeof(ClipRec) * counts[1] + cused);
+            stats.cx_positions = 0;
+            stats.row_blocks = 0;
             stats.host_ms = host_ms;
             (void)h0;
             return RV_OK;

@@ -551,6 +551,10 @@ struct Realigner {
     // without carrying exactly this deletion
     bool noPassingReads(int start, int end) {
         if (!reads) throw std::runtime_error("region reads unavailable");
+        if (reads->n > 0 && !reads->pos0) {
+            HostReads& hr = const_cast<HostReads&>(*reads);
+            if (!hr.load || !hr.load(hr)) throw std::runtime_error("region reads could not be copied to the host");
+        }
         int cnt = 0;
         uint32_t dq = ((uint32_t)(end - start) << 4) | OP_D;
         for (int64_t i = 0; i < reads->n; i++) {

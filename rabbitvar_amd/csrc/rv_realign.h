@@ -9,6 +9,7 @@
 #pragma once
 #include <cstdint>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <set>
@@ -65,12 +66,14 @@ struct HostRef {
     char get(int k);
 };
 
-// Region reads for noPassingReads (host arrays).
+// Region reads for noPassingReads (host arrays).  When the batch is device-resident the arrays
+// are copied on first use through `load`.
 struct HostReads {
     int64_t n = 0;
     const int32_t* pos0 = nullptr;
     const int64_t* cigar_off = nullptr;
     const uint32_t* cigar = nullptr;
+    std::function<bool(HostReads&)> load;
 };
 
 struct RPos {

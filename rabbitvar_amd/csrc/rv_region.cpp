@@ -113,15 +113,21 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
         hr.cigar_off = reads.cigar_off;
         hr.cigar = reads.cigar;
     } else if (reads.n_reads > 0) {
-        h_pos0.resize((size_t)reads.n_reads);
-        h_coff.resize((size_t)reads.n_reads + 1);
-        RV_HIPS(hipMemcpy(h_pos0.data(), reads.pos0, 4 * h_pos0.size(), hipMemcpyDeviceToHost));
-        RV_HIPS(hipMemcpy(h_coff.data(), reads.cigar_off, 8 * h_coff.size(), hipMemcpyDeviceToHost));
-        h_cig.resize((size_t)h_coff.back());
-        if (!h_cig.empty()) RV_HIPS(hipMemcpy(h_cig.data(), reads.cigar, 4 * h_cig.size(), hipMemcpyDeviceToHost));
-        hr.pos0 = h_pos0.data();
-        hr.cigar_off = h_coff.data();
-        hr.cigar = h_cig.data();
+        hr.load = [&](HostReads& h) {
+            h_pos0.resize((size_t)reads.n_reads);
+            h_coff.resize((size_t)reads.n_reads + 1);
+            if (hipMemcpy(h_pos0.data(), reads.pos0, 4 * h_pos0.size(), hipMemcpyDeviceToHost) != hipSuccess) return false;
+            if (hipMemcpy(h_coff.data(), reads.cigar_off, 8 * h_coff.size(), hipMemcpyDeviceToHost) != hipSuccess)
+                return false;
+            h_cig.resize((size_t)h_coff.back());
+            if (!h_cig.empty() &&
+                hipMemcpy(h_cig.data(), reads.cigar, 4 * h_cig.size(), hipMemcpyDeviceToHost) != hipSuccess)
+                return false;
+            h.pos0 = h_pos0.data();
+            h.cigar_off = h_coff.data();
+            h.cigar = h_cig.data();
+            return true;
+        };
     }
     RealignStats rst;
     try {
@@ -277,6 +283,8 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     e.stats.score_ms = score_ms;
     e.stats.host_ms += host_ms;
     e.stats.total_device_ms = e.stats.prep_ms + e.stats.walk_ms + e.stats.anchor_ms + score_ms;
+    e.stats.cx_positions = ncx;
+    e.stats.row_blocks = rc.blocks_fetched;
     e.last_cx_positions = ncx;
     e.last_row_blocks = rc.blocks_fetched;
     e.last_insertion_side_effects = (int64_t)hc[3];

@@ -16,6 +16,37 @@ class EngineError(RuntimeError):
     pass
 
 
+_ARRAYS = ["pos0", "flag", "mapq", "has_nm", "nm", "mpos0", "tid", "mtid", "seq_off", "cigar_off", "bases", "quals",
+           "cigar"]
+
+
+class DeviceReads:
+    """A read batch resident in device memory (torch tensors on a HIP device hold the buffers; the
+    engine reads them in place).  Built from a host ReadBatch."""
+
+    def __init__(self, batch, device="cuda:0"):
+        import torch
+
+        self.n = batch.n
+        self.t = {}
+        for name in _ARRAYS:
+            a = np.ascontiguousarray(getattr(batch, name))
+            if a.dtype == np.uint16:
+                a = a.view(np.int16)
+            elif a.dtype == np.uint32:
+                a = a.view(np.int32)
+            if a.size == 0:
+                a = np.zeros(1, a.dtype)
+            self.t[name] = torch.from_numpy(a).to(device)
+        torch.cuda.synchronize()
+
+    def nbytes(self):
+        return sum(int(t.numel() * t.element_size()) for t in self.t.values())
+
+    def rv_reads(self):
+        return _lib.RvReads(self.n, *[ctypes.c_void_p(self.t[k].data_ptr()) for k in _ARRAYS], 1)
+
+
 def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data) if a is not None and a.size else ctypes.c_void_p(0)
 
@@ -51,11 +82,12 @@ class Engine:
 
     def _args(self, chrom, start, end, reads, contig_name, contig_bases, gene=None):
         self._keep = []
-        arrays = [reads.pos0, reads.flag, reads.mapq, reads.has_nm, reads.nm, reads.mpos0, reads.tid, reads.mtid,
-                  reads.seq_off, reads.cigar_off, reads.bases, reads.quals, reads.cigar]
-        arrays = [np.ascontiguousarray(a) for a in arrays]
-        self._keep.append(arrays)
-        r = _lib.RvReads(reads.n, *[_ptr(a) for a in arrays], 0)
+        if isinstance(reads, DeviceReads):
+            r = reads.rv_reads()
+        else:
+            arrays = [np.ascontiguousarray(getattr(reads, k)) for k in _ARRAYS]
+            self._keep.append(arrays)
+            r = _lib.RvReads(reads.n, *[_ptr(a) for a in arrays], 0)
         g = (gene if gene is not None else chrom).encode()
         c = chrom.encode()
         self._keep += [g, c, contig_bases]
