@@ -138,7 +138,7 @@ typedef struct rv_stats {
     int64_t dense_rows;        /* positions in the dense pileup window */
     int64_t event_bytes;       /* bytes of ordered walk records written by the walk kernel */
     int64_t cx_positions;      /* positions scored from the record list (non-dense or realigned) */
-    int64_t row_blocks;        /* 64-row blocks copied to the host for realignment */
+    int64_t rows_fetched;      /* dense rows copied to the host for realignment */
 } rv_stats;
 
 void rv_config_default(rv_config* cfg);

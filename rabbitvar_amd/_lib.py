@@ -56,7 +56,7 @@ class RvStats(ctypes.Structure):
         ("total_device_ms", c_double), ("host_ms", c_double),
         ("reads_in", c_int64), ("reads_walked", c_int64), ("events", c_int64), ("clips", c_int64),
         ("ref_key_creations", c_int64), ("walk_errors", c_int64), ("dense_rows", c_int64),
-        ("event_bytes", c_int64), ("cx_positions", c_int64), ("row_blocks", c_int64),
+        ("event_bytes", c_int64), ("cx_positions", c_int64), ("rows_fetched", c_int64),
     ]
 
 

@@ -291,7 +291,7 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
             stats.dense_rows = W.dense_n;
             stats.event_bytes = (int64_t)(sizeof(Event) * counts[0] + aused + sizeof(ClipRec) * counts[1] + cused);
             stats.cx_positions = 0;
-            stats.row_blocks = 0;
+            stats.rows_fetched = 0;
             stats.host_ms = host_ms;
             (void)h0;
             return RV_OK;
@@ -305,16 +305,17 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
     return fail(RV_ERR_CAPACITY, "walk record buffers kept overflowing");
 }
 
-bool Engine::fetch_blocks(const std::vector<int>& blocks, std::vector<DenseRow>& out) {
-    constexpr int ROWS = 64;
-    out.assign(blocks.size() * ROWS, DenseRow{});
-    if (blocks.empty()) return true;
-    int32_t* ids = (int32_t*)grow(b_gather_ids, sizeof(int32_t) * blocks.size());
-    DenseRow* dst = (DenseRow*)grow(b_gather_out, sizeof(DenseRow) * ROWS * blocks.size());
+bool Engine::fetch_rows(const std::vector<int>& idx, std::vector<DenseRow>& out) {
+    out.assign(idx.size(), DenseRow{});
+    if (idx.empty()) return true;
+    for (int i : idx)
+        if (i < 0 || i >= rows_n) return false;
+    int32_t* ids = (int32_t*)grow(b_gather_ids, sizeof(int32_t) * idx.size());
+    DenseRow* dst = (DenseRow*)grow(b_gather_out, sizeof(DenseRow) * idx.size());
     if (!ids || !dst) return false;
-    std::vector<int32_t> h(blocks.begin(), blocks.end());
-    if (hipMemcpyAsync(ids, h.data(), sizeof(int32_t) * h.size(), hipMemcpyHostToDevice, stream) != hipSuccess) return false;
-    if (launch_gather_blocks(rows, rows_n, ids, (int)h.size(), dst, stream) != hipSuccess) return false;
+    if (hipMemcpyAsync(ids, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice, stream) != hipSuccess)
+        return false;
+    if (launch_gather_rows(rows, rows_n, ids, (int)idx.size(), dst, stream) != hipSuccess) return false;
     if (hipMemcpyAsync(out.data(), dst, sizeof(DenseRow) * out.size(), hipMemcpyDeviceToHost, stream) != hipSuccess)
         return false;
     return hipStreamSynchronize(stream) == hipSuccess;

@@ -48,16 +48,16 @@ class Engine {
     char* ref_dev = nullptr;
 
     void* grow(DevBuf& b, size_t bytes);
-    // copies dense-row blocks of the last pileup (RowSource::BLOCK rows each) to the host
-    bool fetch_blocks(const std::vector<int>& blocks, std::vector<DenseRow>& out);
+    // copies the listed dense rows (indices into the window) of the last pileup to the host
+    bool fetch_rows(const std::vector<int>& rows, std::vector<DenseRow>& out);
 
     // buffers of the realignment / scoring stages
     DevBuf b_gather_ids, b_gather_out, b_cx_pos, b_cx_off, b_cx, b_cx_arena, b_cx_cov, b_cx_key, b_out, b_out_arena,
         b_score_ctr;
     hipEvent_t sev[2] = {nullptr, nullptr};
-    // figures of the last region: record-list positions, dense-row blocks fetched for the
+    // figures of the last region: record-list positions, dense rows fetched for the
     // realigner, insertions whose coverage came from the next position
-    int64_t last_cx_positions = 0, last_row_blocks = 0, last_insertion_side_effects = 0;
+    int64_t last_cx_positions = 0, last_rows_fetched = 0, last_insertion_side_effects = 0;
     rv_status fail(rv_status s, const std::string& msg) {
         err = msg;
         return s;

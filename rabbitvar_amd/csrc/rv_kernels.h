@@ -16,8 +16,8 @@ hipError_t launch_anchor(const Anchor* a, const unsigned* n_anchors, unsigned ca
                          int dense_n, const Config& cfg, hipStream_t s);
 
 static_assert(sizeof(DenseRow) % 16 == 0, "dense rows are copied in 16-byte words");
-hipError_t launch_gather_blocks(const DenseRow* rows, int dense_n, const int32_t* blocks, int nb, DenseRow* out,
-                                hipStream_t s);
+hipError_t launch_gather_rows(const DenseRow* rows, int dense_n, const int32_t* idx, int n, DenseRow* out,
+                              hipStream_t s);
 
 hipError_t launch_score(const ScoreIn& S, hipStream_t s);
 hipError_t launch_fisher_table(const int* tab, int n, double* out, hipStream_t s);

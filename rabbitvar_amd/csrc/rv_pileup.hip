@@ -32,24 +32,19 @@ __global__ void __launch_bounds__(256) k_anchor(const Anchor* A, const unsigned*
     if (i < n) anchor_one(i, A, rows, dense_lo, dense_n, cfg);
 }
 
-// Copies whole 64-row blocks of the dense window into a packed buffer (rows past the window are
-// zero-filled).  One workgroup per block; each lane moves 16-byte words.
-__global__ void __launch_bounds__(256) k_gather_blocks(const DenseRow* rows, int dense_n, const int32_t* blocks, int nb,
-                                                         DenseRow* out) {
+// Copies the listed dense rows into a packed buffer (indices outside the window yield zero rows).
+// One lane per 16-byte word.
+__global__ void __launch_bounds__(256) k_gather_rows(const DenseRow* rows, int dense_n, const int32_t* idx, int n,
+                                                       DenseRow* out) {
     constexpr int WORDS = (int)(sizeof(DenseRow) / 16);
-    constexpr int ROWS = 64;
-    const int b = blockIdx.x;
-    if (b >= nb) return;
-    const int64_t first = (int64_t)blocks[b] * ROWS;
-    const uint4* src = reinterpret_cast<const uint4*>(rows);
-    uint4* dst = reinterpret_cast<uint4*>(out + (int64_t)b * ROWS);
-    for (int w = threadIdx.x; w < ROWS * WORDS; w += blockDim.x) {
-        const int r = w / WORDS;
-        const int64_t g = first + r;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (g >= 0 && g < dense_n) v = src[g * WORDS + (w - r * WORDS)];
-        dst[w] = v;
-    }
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= (int64_t)n * WORDS) return;
+    const int64_t k = w / WORDS;
+    const int j = (int)(w - k * WORDS);
+    const int r = idx[k];
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (r >= 0 && r < dense_n) v = reinterpret_cast<const uint4*>(rows)[(int64_t)r * WORDS + j];
+    reinterpret_cast<uint4*>(out)[w] = v;
 }
 
 // ---------------------------------------------------------------- launch wrappers
@@ -80,10 +75,12 @@ hipError_t launch_anchor(const Anchor* a, const unsigned* n_anchors, unsigned ca
     return hipGetLastError();
 }
 
-hipError_t launch_gather_blocks(const DenseRow* rows, int dense_n, const int32_t* blocks, int nb, DenseRow* out,
-                                hipStream_t s) {
-    if (nb <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gather_blocks, dim3((unsigned)nb), dim3(256), 0, s, rows, dense_n, blocks, nb, out);
+hipError_t launch_gather_rows(const DenseRow* rows, int dense_n, const int32_t* idx, int n, DenseRow* out,
+                              hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    constexpr int WORDS = (int)(sizeof(DenseRow) / 16);
+    hipLaunchKernelGGL(k_gather_rows, dim3(grid_for((int64_t)n * WORDS, 256)), dim3(256), 0, s, rows, dense_n, idx, n,
+                       out);
     return hipGetLastError();
 }
 

@@ -18,6 +18,7 @@
 #include "rv_realign.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <regex>
@@ -26,52 +27,50 @@
 namespace rv {
 
 // ------------------------------------------------------------------ rows
-bool HostRowSource::fetch(const std::vector<int>& blocks, std::vector<DenseRow>& out) {
-    out.assign(blocks.size() * (size_t)BLOCK, DenseRow{});
-    for (size_t k = 0; k < blocks.size(); k++)
-        for (int i = 0; i < BLOCK; i++) {
-            long long r = (long long)blocks[k] * BLOCK + i;
-            if (r >= 0 && r < P_.dense_n && (size_t)r < P_.rows.size()) out[k * BLOCK + (size_t)i] = P_.rows[(size_t)r];
-        }
+bool HostRowSource::fetch(const std::vector<int>& rows, std::vector<DenseRow>& out) {
+    out.assign(rows.size(), DenseRow{});
+    for (size_t k = 0; k < rows.size(); k++) {
+        int r = rows[k];
+        if (r >= 0 && r < P_.dense_n && (size_t)r < P_.rows.size()) out[k] = P_.rows[(size_t)r];
+    }
     return true;
 }
 
-bool RowCache::load(const std::vector<int>& blocks) {
-    if (blocks.empty()) return true;
+bool RowCache::load(std::vector<int>& rows) {
+    std::sort(rows.begin(), rows.end());
+    rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
+    if (rows.empty()) return true;
     std::vector<DenseRow> buf;
-    if (!src_->fetch(blocks, buf)) return false;
+    if (!src_->fetch(rows, buf)) return false;
     fetches++;
-    blocks_fetched += (int64_t)blocks.size();
-    for (size_t k = 0; k < blocks.size(); k++) {
-        std::unique_ptr<DenseRow[]> b(new DenseRow[RowSource::BLOCK]);
-        std::memcpy(b.get(), buf.data() + k * RowSource::BLOCK, sizeof(DenseRow) * RowSource::BLOCK);
-        blocks_[blocks[k]] = std::move(b);
-    }
+    rows_fetched += (int64_t)rows.size();
+    rows_.reserve(rows_.size() + rows.size());
+    for (size_t k = 0; k < rows.size(); k++) rows_.emplace(rows[k], buf[k]);
     return true;
 }
 
 bool RowCache::prefetch(const std::vector<int>& positions, int radius) {
-    std::set<int> want;
-    int nb = (n_ + RowSource::BLOCK - 1) / RowSource::BLOCK;
+    std::vector<int> want;
     for (int p : positions) {
         int a = std::max(0, p - radius - lo_), b = std::min(n_ - 1, p + radius - lo_);
-        if (a > b) continue;
-        for (int k = a / RowSource::BLOCK; k <= b / RowSource::BLOCK && k < nb; k++)
-            if (!blocks_.count(k)) want.insert(k);
+        for (int i = a; i <= b; i++)
+            if (!rows_.count(i)) want.push_back(i);
     }
-    return load(std::vector<int>(want.begin(), want.end()));
+    return load(want);
 }
 
 const DenseRow* RowCache::row(int pos) {
     int i = pos - lo_;
     if (i < 0 || i >= n_) return nullptr;
-    int k = i / RowSource::BLOCK;
-    auto it = blocks_.find(k);
-    if (it == blocks_.end()) {
-        if (!load({k})) throw std::runtime_error("dense row fetch failed");
-        it = blocks_.find(k);
+    auto it = rows_.find(i);
+    if (it == rows_.end()) {
+        std::vector<int> want;
+        for (int k = std::max(0, i - MISS_RADIUS); k <= std::min(n_ - 1, i + MISS_RADIUS); k++)
+            if (!rows_.count(k)) want.push_back(k);
+        if (!load(want)) throw std::runtime_error("dense row fetch failed");
+        it = rows_.find(i);
     }
-    return &it->second[(size_t)(i % RowSource::BLOCK)];
+    return &it->second;
 }
 
 // ------------------------------------------------------------------ reference
@@ -985,6 +984,14 @@ struct Realigner {
         std::sort(t5.begin(), t5.end(), comp3);
         std::sort(t3.begin(), t3.end(), comp3);
         st.candidates += (int64_t)(t5.size() + t3.size());
+        // The reference scans every 3' cluster for every 5' cluster and skips those outside
+        // [p5 - 2.5 * maxReadLength, p5 + maxReadLength - 10] before anything else; visiting only
+        // the clusters inside that window, in the same (COMP3) order, is the same computation.
+        std::vector<std::pair<int, int>> by_pos;  // (position, rank in t3)
+        by_pos.reserve(t3.size());
+        for (size_t i = 0; i < t3.size(); i++) by_pos.push_back({t3[i].position, (int)i});
+        std::sort(by_pos.begin(), by_pos.end());
+        std::vector<int> ranks;
         for (const SP& a : t5) {
             int p5 = a.position;
             HSclip* sc5v = a.sc;
@@ -992,7 +999,16 @@ struct Realigner {
             if (sc5v->used) continue;
             std::string seq5 = findconseq(*sc5v);
             if (seq5.size() <= 10) continue;
-            for (const SP& b : t3) {
+            ranks.clear();
+            {
+                const double lo = p5 - maxlen() * 2.5;
+                auto it = std::lower_bound(by_pos.begin(), by_pos.end(), std::make_pair((int)std::floor(lo) - 1, -1));
+                for (; it != by_pos.end() && it->first - p5 <= maxlen() - 10; ++it)
+                    if (!(p5 - it->first > maxlen() * 2.5)) ranks.push_back(it->second);
+                std::sort(ranks.begin(), ranks.end());
+            }
+            for (int ri : ranks) {
+                const SP& b = t3[(size_t)ri];
                 try {
                     int p3 = b.position;
                     HSclip* sc3v = b.sc;
@@ -1105,21 +1121,37 @@ struct Realigner {
 
 void realign(const Config& c, int region_start, int region_end, HostRef& ref, int chrLen, const HostReads* reads,
              RState& S, RealignStats& st) {
-    // prefetch the rows around every candidate in one transfer
-    std::vector<int> cand;
+    // prefetch, in one transfer, the rows the candidates' own handlers read first: the candidate
+    // positions, the few bases after them, and every non-dense / insertion position (read again
+    // when the record list is built)
+    std::vector<int> cand, exact;
     for (auto* m : {&S.P.mnp, &S.P.delCount, &S.P.insCount})
         for (auto& e : *m) cand.push_back(e.first);
-    for (auto& e : S.P.sc5) cand.push_back(e.first);
-    for (auto& e : S.P.sc3) cand.push_back(e.first);
-    for (auto& e : S.P.nd) cand.push_back(e.first);
-    for (auto& e : S.P.ins) cand.push_back(e.first);
-    S.rows.prefetch(cand, 256);
+    for (auto& e : S.P.nd) exact.push_back(e.first);
+    for (auto& e : S.P.ins) exact.push_back(e.first);
+    auto now = [] { return std::chrono::steady_clock::now(); };
+    auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+        return std::chrono::duration<double, std::milli>(b - a).count();
+    };
+    auto t0 = now();
+    for (int p : exact) cand.push_back(p);
+    S.rows.prefetch(cand, 4);
+    auto t1 = now();
     Realigner rg{c, region_start, region_end, ref, chrLen, reads, S, st};
     rg.adjustMNP();
+    auto t2 = now();
+    st.ms_prefetch = ms(t0, t1);
+    st.ms_mnp = ms(t1, t2);
     if (c.performLocalRealignment) {
         rg.realigndel(true, S.P.delCount);
+        auto t3 = now();
         rg.realignins(S.P.insCount);
+        auto t4 = now();
         rg.realignlgins30();
+        auto t5 = now();
+        st.ms_del = ms(t2, t3);
+        st.ms_ins = ms(t3, t4);
+        st.ms_lgins = ms(t4, t5);
     }
     st.touched = (int64_t)S.pos.size();
     for (auto& e : S.pos)

@@ -21,38 +21,38 @@
 
 namespace rv {
 
-// Source of dense rows (positions dense_lo + i).  fetch() fills rows for the given block ids
-// (block b covers positions dense_lo + b*BLOCK .. +BLOCK-1, clipped to the window).
+// Source of dense rows (row i holds position dense_lo + i).  fetch() copies the listed rows.
 class RowSource {
   public:
-    static constexpr int BLOCK = 64;
     virtual ~RowSource() {}
-    virtual bool fetch(const std::vector<int>& blocks, std::vector<DenseRow>& out) = 0;
+    virtual bool fetch(const std::vector<int>& rows, std::vector<DenseRow>& out) = 0;
 };
 
 // Host rows already downloaded (tests, diagnostics).
 class HostRowSource : public RowSource {
   public:
     explicit HostRowSource(const HostPileup& P) : P_(P) {}
-    bool fetch(const std::vector<int>& blocks, std::vector<DenseRow>& out) override;
+    bool fetch(const std::vector<int>& rows, std::vector<DenseRow>& out) override;
 
   private:
     const HostPileup& P_;
 };
 
+// Rows the realigner has read, by position.  A miss copies a small neighbourhood of rows.
 class RowCache {
   public:
+    static constexpr int MISS_RADIUS = 16;
     RowCache(RowSource* src, int dense_lo, int dense_n) : src_(src), lo_(dense_lo), n_(dense_n) {}
-    // fetches, in one transfer, every block within `radius` of the given positions
+    // fetches, in one transfer, every row within `radius` of the given positions
     bool prefetch(const std::vector<int>& positions, int radius);
     const DenseRow* row(int pos);  // nullptr outside the dense window
-    int64_t fetches = 0, blocks_fetched = 0;
+    int64_t fetches = 0, rows_fetched = 0;
 
   private:
-    bool load(const std::vector<int>& blocks);
+    bool load(std::vector<int>& rows);
     RowSource* src_;
     int lo_, n_;
-    std::unordered_map<int, std::unique_ptr<DenseRow[]>> blocks_;
+    std::unordered_map<int, DenseRow> rows_;  // keyed by row index
 };
 
 // The reference window as the reference's map: keys [lo, hi]; get() of a missing key creates it
@@ -109,6 +109,7 @@ class RState {
 
 struct RealignStats {
     int64_t candidates = 0, touched = 0, changed = 0;
+    double ms_prefetch = 0, ms_mnp = 0, ms_del = 0, ms_ins = 0, ms_lgins = 0;
 };
 
 // adjustMNP, realigndel, realignins, realignlgins30 in the reference's order.

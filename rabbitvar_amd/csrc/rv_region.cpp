@@ -17,7 +17,7 @@ namespace {
 class DeviceRowSource : public RowSource {
   public:
     explicit DeviceRowSource(Engine& e) : e_(e) {}
-    bool fetch(const std::vector<int>& blocks, std::vector<DenseRow>& out) override { return e_.fetch_blocks(blocks, out); }
+    bool fetch(const std::vector<int>& rows, std::vector<DenseRow>& out) override { return e_.fetch_rows(rows, out); }
 
   private:
     Engine& e_;
@@ -284,9 +284,9 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     e.stats.host_ms += host_ms;
     e.stats.total_device_ms = e.stats.prep_ms + e.stats.walk_ms + e.stats.anchor_ms + score_ms;
     e.stats.cx_positions = ncx;
-    e.stats.row_blocks = rc.blocks_fetched;
+    e.stats.rows_fetched = rc.rows_fetched;
     e.last_cx_positions = ncx;
-    e.last_row_blocks = rc.blocks_fetched;
+    e.last_rows_fetched = rc.rows_fetched;
     e.last_insertion_side_effects = (int64_t)hc[3];
     return RV_OK;
 }
