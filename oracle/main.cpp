@@ -170,7 +170,23 @@ void dump_pileup(std::ostream& o, Pileup& P) {
 
 }  // namespace
 
+// --fisher-table IN OUT: one "n11 n12 n21 n22" table per input line -> "two left right" (%.17g)
+static int fisher_table(const std::string& in, const std::string& out) {
+    std::ifstream f(in);
+    std::ofstream o(out);
+    long long a, b, c, d;
+    char buf[128];
+    while (f >> a >> b >> c >> d) {
+        double l = 0, r = 0;
+        double two = kt_fisher_two_sided((int)a, (int)b, (int)c, (int)d, &l, &r);
+        snprintf(buf, sizeof buf, "%.17g %.17g %.17g\n", two, l, r);
+        o << buf;
+    }
+    return 0;
+}
+
 int main(int argc, char** argv) {
+    if (argc == 4 && std::string(argv[1]) == "--fisher-table") return fisher_table(argv[2], argv[3]);
     Config c;
     std::string fasta, bam, regionArg, bed, out = "./out.txt", dumpP, dumpR;
     int colC = -1, colS = -1, colE = -1, colG = -1;

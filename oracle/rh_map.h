@@ -118,8 +118,12 @@ class RHMap {
             if (info_[i] != 0) out.push_back(keys_[i]);
         return out;
     }
-    // slot of a key (tests)
+    // slot of a key and table state (tests)
     size_t slot_of(K key) const { return find_slot(key); }
+    unsigned info_at(size_t slot) const { return info_[slot]; }
+    size_t mask() const { return mask_; }
+    unsigned info_inc() const { return inc_; }
+    unsigned info_hash_shift() const { return hash_shift_; }
 
   private:
     std::vector<K> keys_;

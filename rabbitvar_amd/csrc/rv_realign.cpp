@@ -1119,6 +1119,62 @@ struct Realigner {
 
 }  // namespace
 
+namespace {
+
+HVarD to_dev(const HVar& v) {
+    HVarD d;
+    d.cnt = v.cnt;
+    d.fwd = v.fwd;
+    d.rev = v.rev;
+    d.hicnt = v.hicnt;
+    d.locnt = v.locnt;
+    d.extracnt = v.extracnt;
+    d.pmean = v.pmean;
+    d.qmean = v.qmean;
+    d.mqmean = v.mqmean;
+    d.nm = v.nm;
+    d.pstd = v.pstd ? 1 : 0;
+    d.qstd = v.qstd ? 1 : 0;
+    return d;
+}
+
+}  // namespace
+
+void build_record_list(RState& S, CxHost& cx) {
+    std::vector<int> ps;
+    for (auto& e : S.P.nd) ps.push_back(e.first);
+    for (auto& e : S.ins) ps.push_back(e.first);
+    for (auto& e : S.pos)
+        if (e.second.dirty) ps.push_back(e.first);
+    std::sort(ps.begin(), ps.end());
+    ps.erase(std::unique(ps.begin(), ps.end()), ps.end());
+    S.rows.prefetch(ps, 0);
+    cx.off.push_back(0);
+    auto add = [&](int kind, const std::string& d, const HVar& v) {
+        CxEntry c;
+        c.kind = kind;
+        c.desc_off = (uint32_t)cx.arena.size();
+        c.desc_len = (int32_t)d.size();
+        c.pad = 0;
+        c.v = to_dev(v);
+        cx.arena += d;
+        cx.ent.push_back(c);
+    };
+    for (int p : ps) {
+        RPos& r = S.at(p);
+        cx.pos.push_back(p);
+        cx.cov.push_back(r.cov);
+        cx.key.push_back(r.key ? 1 : 0);
+        if (r.key)
+            for (auto& kv : r.v) add(0, kv.first, *kv.second);
+        auto it = S.ins.find(p);
+        if (it != S.ins.end())
+            for (auto& kv : it->second) add(1, kv.first, *kv.second);
+        cx.off.push_back((int32_t)cx.ent.size());
+    }
+}
+
+
 void realign(const Config& c, int region_start, int region_end, HostRef& ref, int chrLen, const HostReads* reads,
              RState& S, RealignStats& st) {
     // prefetch, in one transfer, the rows the candidates' own handlers read first: the candidate

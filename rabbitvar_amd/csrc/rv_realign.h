@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "rv_host.h"
+#include "rv_score.h"
 
 namespace rv {
 
@@ -115,6 +116,17 @@ struct RealignStats {
 // adjustMNP, realigndel, realignins, realignlgins30 in the reference's order.
 void realign(const Config& c, int region_start, int region_end, HostRef& ref, int chrLen, const HostReads* reads,
              RState& S, RealignStats& st);
+
+// Positions the scoring stage reads from a record list instead of the dense rows: every
+// non-dense / insertion position and every position the realigner changed, ascending, with the
+// nonInsertionVariants entries (key order) followed by the insertionVariants entries (key order).
+struct CxHost {
+    std::vector<int32_t> pos, off, cov;
+    std::vector<uint8_t> key;
+    std::vector<CxEntry> ent;
+    std::string arena;
+};
+void build_record_list(RState& S, CxHost& cx);
 
 // soft-clip consensus (include/VariationUtils.h:403-509)
 std::string findconseq(HSclip& sc);

@@ -23,65 +23,6 @@ class DeviceRowSource : public RowSource {
     Engine& e_;
 };
 
-HVarD to_dev(const HVar& v) {
-    HVarD d;
-    d.cnt = v.cnt;
-    d.fwd = v.fwd;
-    d.rev = v.rev;
-    d.hicnt = v.hicnt;
-    d.locnt = v.locnt;
-    d.extracnt = v.extracnt;
-    d.pmean = v.pmean;
-    d.qmean = v.qmean;
-    d.mqmean = v.mqmean;
-    d.nm = v.nm;
-    d.pstd = v.pstd ? 1 : 0;
-    d.qstd = v.qstd ? 1 : 0;
-    return d;
-}
-
-// Positions the scoring stage reads from records instead of the dense rows.
-struct CxHost {
-    std::vector<int32_t> pos, off, cov;
-    std::vector<uint8_t> key;
-    std::vector<CxEntry> ent;
-    std::string arena;
-};
-
-void build_cx(RState& S, CxHost& cx) {
-    std::vector<int> ps;
-    for (auto& e : S.P.nd) ps.push_back(e.first);
-    for (auto& e : S.ins) ps.push_back(e.first);
-    for (auto& e : S.pos)
-        if (e.second.dirty) ps.push_back(e.first);
-    std::sort(ps.begin(), ps.end());
-    ps.erase(std::unique(ps.begin(), ps.end()), ps.end());
-    S.rows.prefetch(ps, 0);
-    cx.off.push_back(0);
-    auto add = [&](int kind, const std::string& d, const HVar& v) {
-        CxEntry c;
-        c.kind = kind;
-        c.desc_off = (uint32_t)cx.arena.size();
-        c.desc_len = (int32_t)d.size();
-        c.pad = 0;
-        c.v = to_dev(v);
-        cx.arena += d;
-        cx.ent.push_back(c);
-    };
-    for (int p : ps) {
-        RPos& r = S.at(p);
-        cx.pos.push_back(p);
-        cx.cov.push_back(r.cov);
-        cx.key.push_back(r.key ? 1 : 0);
-        if (r.key)
-            for (auto& kv : r.v) add(0, kv.first, *kv.second);
-        auto it = S.ins.find(p);
-        if (it != S.ins.end())
-            for (auto& kv : it->second) add(1, kv.first, *kv.second);
-        cx.off.push_back((int32_t)cx.ent.size());
-    }
-}
-
 }  // namespace
 
 #define RV_HIPS(x)                                                                                   \
@@ -137,7 +78,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     }
     CxHost cx;
     try {
-        build_cx(S, cx);
+        build_record_list(S, cx);
     } catch (const std::exception& ex) {
         return e.fail(RV_ERR_DEVICE, std::string("record list: ") + ex.what());
     }

@@ -74,6 +74,50 @@ class ReadBatch:
         return ReadBatch(arrays)
 
 
+def write_rvb(path, batch):
+    """Writes a ReadBatch in the RVB1 layout read_rvb() reads."""
+    n = batch.n
+    nb = int(batch.seq_off[-1]) if n else 0
+    nc = int(batch.cigar_off[-1]) if n else 0
+    with open(path, "wb") as f:
+        f.write(b"RVB1")
+        f.write(np.array([1], np.uint32).tobytes())
+        f.write(np.array([n, nb, nc], np.int64).tobytes())
+        arrays = [(getattr(batch, name), dtype) for name, dtype, _ in _FIELDS]
+        arrays += [(batch.bases, np.uint8), (batch.quals, np.uint8), (batch.cigar, np.uint32)]
+        for a, dtype in arrays:
+            b = np.ascontiguousarray(a, dtype=dtype).tobytes()
+            f.write(b)
+            f.write(b"\0" * ((-len(b)) % 8))
+
+
+def batch_from_records(records):
+    """ReadBatch from a list of dicts with keys pos0, flag, mapq, nm (optional), mpos0, tid, mtid,
+    seq (str), qual (list of Phred values) and cigar (list of (op, length), ops as BAM codes)."""
+    n = len(records)
+    arrays = {name: np.zeros(n + extra, dtype) for name, dtype, extra in _FIELDS}
+    bases, quals, cigar = [], [], []
+    for i, r in enumerate(records):
+        arrays["pos0"][i] = r["pos0"]
+        arrays["flag"][i] = r.get("flag", 0)
+        arrays["mapq"][i] = r.get("mapq", 60)
+        arrays["has_nm"][i] = 1 if "nm" in r else 0
+        arrays["nm"][i] = r.get("nm", 0)
+        arrays["mpos0"][i] = r.get("mpos0", -1)
+        arrays["tid"][i] = r.get("tid", 0)
+        arrays["mtid"][i] = r.get("mtid", -1)
+        assert len(r["seq"]) == len(r["qual"])
+        bases.extend(r["seq"].encode())
+        quals.extend(r["qual"])
+        cigar.extend((length << 4) | op for op, length in r["cigar"])
+        arrays["seq_off"][i + 1] = len(bases)
+        arrays["cigar_off"][i + 1] = len(cigar)
+    arrays["bases"] = np.array(bases, np.uint8)
+    arrays["quals"] = np.array(quals, np.uint8)
+    arrays["cigar"] = np.array(cigar, np.uint32)
+    return ReadBatch(arrays)
+
+
 def read_rvb(path):
     with open(path, "rb") as f:
         data = f.read()

@@ -1,0 +1,195 @@
+// Test harness only: runs the engine's device stages sequentially on the CPU (prep, walk, anchor,
+// then the host reduction and realigner, then one score_position call per position exactly as the
+// two scoring kernels map threads to positions) and prints the simple-mode lines.
+//
+//   rv_cpu_sim REF.fa READS.rvb START END K FISHER OUT      whole path for chrS:START-END
+//   rv_cpu_sim --rh-chars B1 B2 ...                         robin_hood <char,int> order of bytes
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+#include "hip/hip_runtime.h"
+#include "rv_format.h"
+#include "rv_host.h"
+#include "rv_pileup_body.h"
+#include "rv_realign.h"
+#include "rv_score_body.h"
+
+using namespace rv;
+
+template <class T>
+static void rd(std::ifstream& f, std::vector<T>& v, size_t n) {
+    v.resize(n);
+    f.read((char*)v.data(), (std::streamsize)(sizeof(T) * n));
+    size_t pad = (8 - (sizeof(T) * n) % 8) % 8;
+    char t[8];
+    if (pad) f.read(t, (std::streamsize)pad);
+}
+
+int main(int argc, char** argv) {
+    if (argc >= 2 && std::string(argv[1]) == "--rh-chars") {
+        std::vector<char> ins;
+        for (int i = 2; i < argc; i++) ins.push_back((char)atoi(argv[i]));
+        for (char c : rh_char_iteration_order(ins)) printf("%d ", (int)(unsigned char)c);
+        printf("\n");
+        return 0;
+    }
+    if (argc < 8) return 2;
+    std::string fa = argv[1], rvb = argv[2];
+    int rs = atoi(argv[3]), re = atoi(argv[4]), k = atoi(argv[5]), fisher = atoi(argv[6]);
+    const char* outp = argv[7];
+    std::ifstream ff(fa);
+    std::string line, seq;
+    while (std::getline(ff, line))
+        if (!line.empty() && line[0] != '>') seq += line;
+    for (char& c : seq) c = (char)toupper((unsigned char)c);
+    std::ifstream f(rvb, std::ios::binary);
+    char mg[4];
+    uint32_t ver;
+    int64_t n, nb, nc;
+    f.read(mg, 4);
+    f.read((char*)&ver, 4);
+    f.read((char*)&n, 8);
+    f.read((char*)&nb, 8);
+    f.read((char*)&nc, 8);
+    std::vector<int32_t> pos, nm, mpos, tid, mtid;
+    std::vector<uint16_t> flag;
+    std::vector<uint8_t> mapq, hasnm, bases, quals;
+    std::vector<int64_t> so, co;
+    std::vector<uint32_t> cig;
+    rd(f, pos, n); rd(f, flag, n); rd(f, mapq, n); rd(f, hasnm, n); rd(f, nm, n); rd(f, mpos, n); rd(f, tid, n);
+    rd(f, mtid, n); rd(f, so, n + 1); rd(f, co, n + 1); rd(f, bases, nb); rd(f, quals, nb); rd(f, cig, nc);
+    // the region iterator's records
+    std::vector<int32_t> P0, NM, MP, TI, MT;
+    std::vector<uint16_t> FL;
+    std::vector<uint8_t> MQ, HN, B, Q;
+    std::vector<int64_t> SO{0}, CO{0};
+    std::vector<uint32_t> CG;
+    for (int64_t i = 0; i < n; i++) {
+        int rl = 0;
+        for (int64_t j = co[i]; j < co[i + 1]; j++) {
+            int o = cig[j] & 15;
+            if (o == 0 || o == 2 || o == 3 || o == 7 || o == 8) rl += (int)(cig[j] >> 4);
+        }
+        if (rl == 0) rl = 1;
+        if (!(pos[i] < re && pos[i] + rl > rs - 1)) continue;
+        P0.push_back(pos[i]); FL.push_back(flag[i]); MQ.push_back(mapq[i]); HN.push_back(hasnm[i]); NM.push_back(nm[i]);
+        MP.push_back(mpos[i]); TI.push_back(tid[i]); MT.push_back(mtid[i]);
+        for (int64_t j = so[i]; j < so[i + 1]; j++) {
+            B.push_back(bases[j]);
+            Q.push_back(quals[j]);
+        }
+        SO.push_back((int64_t)B.size());
+        for (int64_t j = co[i]; j < co[i + 1]; j++) CG.push_back(cig[j]);
+        CO.push_back((int64_t)CG.size());
+    }
+    const int64_t m = (int64_t)P0.size();
+    ReadsDev R{P0.data(), FL.data(), MQ.data(), HN.data(), NM.data(), MP.data(), TI.data(), MT.data(),
+               SO.data(), CO.data(), B.data(), Q.data(), CG.data(), m};
+    Config cfg{};
+    cfg.goodq = 22.5; cfg.bias = 0.05; cfg.freq = 0.01; cfg.qratio = 1.5; cfg.mapq = 0; cfg.lofreq = 0.05;
+    cfg.vext = 2; cfg.mismatch = 8; cfg.performLocalRealignment = k; cfg.indelsize = 50; cfg.minBiasReads = 2;
+    cfg.minr = 2; cfg.samfilter = 0x504; cfg.readPosFilter = 5; cfg.SVMINLEN = 1000; cfg.referenceExtension = 1200;
+    cfg.fisher = fisher;
+    const int chrLen = (int)seq.size();
+    Window W;
+    W.region_start = rs;
+    W.region_end = re;
+    W.chr_len = chrLen;
+    int ss = rs - 1200 < 1 ? 1 : rs - 1200, se = re + 1200 > chrLen ? chrLen : re + 1200;
+    W.ref_lo = ss;
+    W.ref_hi = se - 17;
+    std::string win = seq.substr((size_t)(ss - 1), (size_t)(W.ref_hi - W.ref_lo + 1));
+    W.ref = win.c_str();
+    std::vector<PrepRec> prep((size_t)std::max<int64_t>(m, 1));
+    std::vector<uint32_t> cg((size_t)std::max<int64_t>(m, 1) * MAXC);
+    WalkCounters ctr{};
+    int span = 0;
+    for (int64_t r = 0; r < m; r++) prep_one(r, R, W, cfg, prep.data(), cg.data(), &ctr, &span);
+    const int pad = 2 * std::max(span, 64) + 8;
+    W.dense_lo = rs - pad;
+    W.dense_n = (re + pad) - W.dense_lo + 1;
+    std::vector<DenseRow> rows((size_t)W.dense_n);
+    for (int i = 0; i < W.dense_n; i++) fill_row_one(rows.data(), i);
+    std::vector<Event> ev((size_t)(8 * m + 4096));
+    std::vector<char> arena((size_t)(128 * m + 65536));
+    std::vector<ClipRec> cl((size_t)(m + 1024));
+    std::vector<char> carena((size_t)(320 * (m + 1024)));
+    std::vector<Anchor> an((size_t)(m + 1024));
+    unsigned ne = 0, ncl = 0, na = 0;
+    unsigned long long au = 0, cu = 0;
+    WalkSink S{rows.data(), W.dense_lo, W.dense_n, ev.data(), &ne, (unsigned)ev.size(), arena.data(), &au,
+               arena.size(), cl.data(), &ncl, (unsigned)cl.size(), carena.data(), &cu, carena.size(), an.data(), &na,
+               (unsigned)an.size(), &ctr};
+    for (int64_t r = 0; r < m; r++) walk_one(r, R, W, cfg, prep.data(), cg.data(), S);
+    for (unsigned i = 0; i < na; i++) anchor_one(i, an.data(), rows.data(), W.dense_lo, W.dense_n, cfg);
+    HostPileup HP;
+    HP.dense_lo = W.dense_lo;
+    HP.dense_n = W.dense_n;
+    HP.rows = rows;
+    HP.maxReadLength = ctr.max_read_length;
+    reduce_events(ev.data(), ne, arena.data(), cfg.goodq, HP);
+    reduce_clips(cl.data(), ncl, carena.data(), cfg.goodq, HP);
+    HostRowSource hsrc(HP);
+    RowCache rc(&hsrc, HP.dense_lo, HP.dense_n);
+    RState RS(HP, rc);
+    HostRef ref;
+    ref.seq = win.data();
+    ref.lo = W.ref_lo;
+    ref.hi = W.ref_hi;
+    HostReads hr;
+    hr.n = m;
+    hr.pos0 = P0.data();
+    hr.cigar_off = CO.data();
+    hr.cigar = CG.data();
+    RealignStats rst;
+    realign(cfg, rs, re, ref, chrLen, &hr, RS, rst);
+    CxHost cx;
+    build_record_list(RS, cx);
+    ScoreIn in{};
+    in.rows = rows.data();
+    in.dense_lo = W.dense_lo;
+    in.dense_n = W.dense_n;
+    in.cx_pos = cx.pos.data();
+    in.n_cx = (int)cx.pos.size();
+    in.cx_off = cx.off.data();
+    in.cx = cx.ent.data();
+    in.cx_arena = cx.arena.data();
+    in.cx_cov = cx.cov.data();
+    in.cx_key = cx.key.data();
+    in.ref = win.data();
+    in.ref_lo = W.ref_lo;
+    in.ref_hi = W.ref_hi;
+    in.chr_len = chrLen;
+    in.region_start = rs;
+    in.region_end = re;
+    in.cfg = cfg;
+    std::vector<OutRec> out(200000);
+    std::vector<char> oarena((size_t)64 << 20);
+    unsigned nout = 0;
+    unsigned long long aused = 0, sctr[4] = {0, 0, 0, 0};
+    in.out = out.data();
+    in.n_out = &nout;
+    in.max_out = (unsigned)out.size();
+    in.arena = oarena.data();
+    in.arena_used = &aused;
+    in.arena_cap = oarena.size();
+    in.counters = sctr;
+    for (int i = 0; i < W.dense_n; i++) score_position<128, 4>(in, W.dense_lo + i, true);
+    for (int i = 0; i < in.n_cx; i++) score_position<1536, MAXV>(in, cx.pos[(size_t)i], false);
+    out.resize(std::min<unsigned>(nout, (unsigned)out.size()));
+    std::sort(out.begin(), out.end(), [](const OutRec& a, const OutRec& b) {
+        return a.position != b.position ? a.position < b.position : a.order < b.order;
+    });
+    LineContext ctx{"S1", "chrS", "chrS", rs, re, fisher != 0};
+    std::string text;
+    for (auto& r : out) format_line(text, r, oarena.data(), ctx);
+    std::ofstream o(outp);
+    o << text;
+    fprintf(stderr, "reads %lld events %u clips %u walk_errors %llu overflow %llu capacity %llu\n", (long long)m, ne, ncl,
+            (unsigned long long)ctr.walk_errors, (unsigned long long)ctr.overflow, sctr[0]);
+    return (ctr.walk_errors || ctr.overflow || sctr[0] || nout > 200000) ? 3 : 0;
+}

@@ -1,0 +1,115 @@
+"""Hand-built inputs whose pileup and calls follow from first principles (used by the oracle tests
+and the matching GPU tests).
+
+A 600 bp contig chrT; ten 100 bp reads aligned at 201-300 (alternately forward and reverse, Phred
+30, MAPQ 60); four of them carry one event at 251:
+  snv  - a substitution at 251                       (CIGAR 100M, NM 1)
+  del  - a 3 bp deletion of 251-253                  (50M3D47M, NM 3)
+  ins  - a 2 bp insertion after 250                  (50M2I50M, NM 2)
+"""
+import os
+import random
+
+from rabbitvar_amd import batch_from_records, write_rvb
+
+L = 600
+START = 200  # pos0 of every read (1-based 201)
+RLEN = 100
+EV = 251
+
+
+def reference():
+    rnd = random.Random(42)
+    s = [rnd.choice("ACGT") for _ in range(L)]
+    # no short repeats around the event so that nothing shifts the alleles
+    s[EV - 2:EV + 4] = list("ACGTCA")  # positions 250..255 = A C G T C A
+    return "".join(s)
+
+
+def tp(i, rlen1):
+    """The reference's read-position statistic for read index i (parseCigar: min(i + 1, rlen1 - i))."""
+    return i + 1 if i < rlen1 - i else rlen1 - i
+
+
+def build(kind, d):
+    s = reference()
+    recs = []
+    for r in range(10):
+        flag = 16 if r % 2 else 0
+        seq = s[START:START + RLEN]
+        cigar = [(0, RLEN)]
+        nm = 0
+        if r < 4:
+            if kind == "snv":
+                alt = "T" if s[EV - 1] != "T" else "A"
+                seq = seq[:EV - 1 - START] + alt + seq[EV - START:]
+                nm = 1
+            elif kind == "del":
+                seq = s[START:EV - 1] + s[EV + 2:START + RLEN]
+                cigar = [(0, EV - 1 - START), (2, 3), (0, RLEN - (EV - 1 - START) - 3)]
+                nm = 3
+            elif kind == "ins":
+                seq = s[START:EV - 1] + "GG" + s[EV - 1:START + RLEN]
+                cigar = [(0, EV - 1 - START), (1, 2), (0, RLEN - (EV - 1 - START))]
+                nm = 2
+        recs.append({"pos0": START, "flag": flag, "mapq": 60, "nm": nm, "seq": seq, "qual": [30] * len(seq),
+                     "cigar": cigar})
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, "ref.fa"), "w") as f:
+        f.write(">chrT\n")
+        for i in range(0, L, 60):
+            f.write(s[i:i + 60] + "\n")
+    write_rvb(os.path.join(d, "reads.rvb"), batch_from_records(recs))
+    return s
+
+
+def read_layout(kind):
+    """Per read: (rlen1 = M + I length, {ref position: read index} over its M bases, NM - indel length)."""
+    out = []
+    for r in range(10):
+        carrier = r < 4
+        m = {}
+        if not carrier or kind == "snv":
+            for i in range(RLEN):
+                m[START + 1 + i] = i
+            out.append((RLEN, m, 1 if (carrier and kind == "snv") else 0))
+        elif kind == "del":
+            a = EV - 1 - START
+            for i in range(a):
+                m[START + 1 + i] = i
+            for i in range(a, RLEN - 3):
+                m[START + 1 + i + 3] = i
+            out.append((RLEN - 3, m, 0))
+        else:  # ins
+            a = EV - 1 - START
+            for i in range(a):
+                m[START + 1 + i] = i
+            for i in range(a, RLEN):
+                m[START + 1 + i] = i + 2
+            out.append((RLEN + 2, m, 0))
+    return out
+
+
+def expected_ref_rows(kind, s):
+    """position -> (cnt, fwd, rev, hicnt, locnt, pmean, qmean, mqmean, nm, pstd, qstd) of the reference allele
+    at positions where every covering read matches the reference.  Each read contributes its
+    read-position statistic tp, quality 30, MAPQ 60 and its NM minus its inserted and deleted bases
+    (parseCigar's nm - idlen)."""
+    lay = read_layout(kind)
+    rows = {}
+    for p in range(START + 1, START + RLEN + 1):
+        if kind == "snv" and p == EV:
+            continue
+        if kind == "del" and EV <= p <= EV + 2:
+            continue
+        if kind == "ins" and p == EV - 1:
+            continue  # the insertion's anchor base carries the insertion bookkeeping
+        cov = [(r, rlen1, m[p], nm) for r, (rlen1, m, nm) in enumerate(lay) if p in m]
+        if len(cov) != 10:
+            continue
+        fwd = sum(1 for r, *_ in cov if r % 2 == 0)
+        tps = [tp(i, rl) for _, rl, i, _ in cov]
+        # pstd: contributions at two different read positions; qstd: two different qualities
+        rows[p] = (10, fwd, 10 - fwd, 10, 0, sum(tps), 300, 600, sum(nm for *_, nm in cov),
+                   int(len(set(tps)) > 1), 0)
+    return rows

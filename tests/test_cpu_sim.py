@@ -1,0 +1,69 @@
+"""The engine's own stage code, compiled for the host (tests/cpu_sim: device bodies run one item at a
+time with host stand-ins for the atomics), against the oracle.  Catches logic regressions in the
+HIP path without a GPU; the GPU tests run the same comparisons on the device."""
+import os
+import random
+import subprocess
+
+import pytest
+
+from conftest import ROOT, run_oracle
+
+SIM_DIR = os.path.join(ROOT, "tests", "cpu_sim")
+SIM = os.path.join(SIM_DIR, "_build", "rv_cpu_sim")
+CSRC = os.path.join(ROOT, "rabbitvar_amd", "csrc")
+
+
+@pytest.fixture(scope="session")
+def sim():
+    os.makedirs(os.path.dirname(SIM), exist_ok=True)
+    srcs = [os.path.join(SIM_DIR, "rv_cpu_sim.cpp")] + [os.path.join(CSRC, f) for f in
+                                                        ("rv_host.cpp", "rv_realign.cpp", "rv_format.cpp")]
+    subprocess.run(["g++", "-O2", "-std=c++17", "-I", SIM_DIR, "-I", CSRC, "-I", os.path.join(ROOT, "include"), "-o",
+                    SIM] + srcs, check=True)
+    return SIM
+
+
+@pytest.mark.parametrize("case,k,fisher", [("c1", 1, 1), ("edge", 1, 1), ("edge", 0, 0)])
+def test_engine_stages_match_oracle(sim, dataset, case, k, fisher):
+    d = dataset(case)
+    with open(os.path.join(d, "ref.fa")) as f:
+        L = sum(len(x.strip()) for x in f if not x.startswith(">"))
+    extra = ["-k", str(k)] + (["--fisher"] if fisher else [])
+    want, _ = run_oracle(d, f"chrS:1-{L}", extra, out=f"sim_{k}_{fisher}.txt")
+    out = os.path.join(d, f"cpu_sim_{k}_{fisher}.txt")
+    subprocess.run([sim, os.path.join(d, "ref.fa"), os.path.join(d, "reads.rvb"), "1", str(L), str(k), str(fisher), out],
+                   check=True)
+    with open(out) as f:
+        got = f.read()
+    assert len(want.splitlines()) > 100
+    assert sorted(got.splitlines()) == sorted(want.splitlines())
+
+
+@pytest.mark.parametrize("start,end", [(1, 1), (5000, 5150), (59000, 60000)])
+def test_engine_stages_subregions(sim, dataset, start, end):
+    d = dataset("edge")
+    want, _ = run_oracle(d, f"chrS:{start}-{end}", ["--fisher"], out=f"simsub_{start}.txt")
+    out = os.path.join(d, f"cpu_sim_sub_{start}.txt")
+    subprocess.run([sim, os.path.join(d, "ref.fa"), os.path.join(d, "reads.rvb"), str(start), str(end), "1", "1", out],
+                   check=True)
+    with open(out) as f:
+        assert sorted(f.read().splitlines()) == sorted(want.splitlines())
+
+
+def test_char_map_order_matches_oracle_clone(sim, tools):
+    """The engine's replay of robin_hood<char,int> iteration order (soft-clip consensus ties) against
+    the oracle's full clone, for random insertion sequences of up to 16 distinct bytes."""
+    probe = os.path.join(ROOT, "oracle", "_build", "rhmap_probe")
+    if not os.path.exists(probe):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    rnd = random.Random(3)
+    alphabet = [ord(c) for c in "ACGTN"] + list(range(0, 256, 17))
+    for _ in range(200):
+        k = rnd.randint(1, 16)
+        seq = rnd.sample(sorted(set(alphabet)), min(k, len(set(alphabet))))
+        r = subprocess.run([probe], input="".join(f"c {b}\n" for b in seq), capture_output=True, text=True, check=True)
+        want = [int(x) for x in r.stdout.splitlines()[-1].split()[1:]]
+        g = subprocess.run([sim, "--rh-chars"] + [str(b) for b in seq], capture_output=True, text=True, check=True)
+        got = [int(x) for x in g.stdout.split()]
+        assert got == want, seq
