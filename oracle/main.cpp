@@ -9,7 +9,10 @@
 //   rvcpu -G ref.fa -b reads.rvb -N sample (-R chr:s-e | -i regions.bed [-c 1 -S 2 -E 3 -g 4])
 //         [-f F] [-k 0|1] [--fisher] [-q Q] [-m M] [-X X] [-P P] [-r R] [-o O] [-O MQ] [-B B]
 //         [-Q MAPQ] [-F hex] [-T T] [-M M] [-p] [-3] [-t] [-Y ext] [-x ext] [--out FILE]
-//         [--dump-pileup FILE] [--dump-realigned FILE] [--time]
+//         [--dump-pileup FILE] [--dump-realigned FILE] [--dump-order FILE] [--time]
+//   rvcpu --fisher-table IN OUT
+// --dump-order writes the iteration order of nonInsertionVariants after the pileup and after
+// realignment, and of alignedVariants, one line each per region.
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -188,7 +191,7 @@ static int fisher_table(const std::string& in, const std::string& out) {
 int main(int argc, char** argv) {
     if (argc == 4 && std::string(argv[1]) == "--fisher-table") return fisher_table(argv[2], argv[3]);
     Config c;
-    std::string fasta, bam, regionArg, bed, out = "./out.txt", dumpP, dumpR;
+    std::string fasta, bam, regionArg, bed, out = "./out.txt", dumpP, dumpR, dumpO;
     int colC = -1, colS = -1, colE = -1, colG = -1;
     bool timing = false;
     c.sample = "";
@@ -231,6 +234,7 @@ int main(int argc, char** argv) {
         else if (a == "--out") out = nxt();
         else if (a == "--dump-pileup") dumpP = nxt();
         else if (a == "--dump-realigned") dumpR = nxt();
+        else if (a == "--dump-order") dumpO = nxt();
         else if (a == "--time") timing = true;
         else if (a == "--th") nxt();
         else {
@@ -290,6 +294,13 @@ int main(int argc, char** argv) {
     std::ofstream dp, dr;
     if (!dumpP.empty()) dp.open(dumpP);
     if (!dumpR.empty()) dr.open(dumpR);
+    std::ofstream dord;
+    if (!dumpO.empty()) dord.open(dumpO);
+    auto dump_keys = [&](const char* tag, const std::vector<int>& keys) {
+        dord << tag;
+        for (int k : keys) dord << ' ' << k;
+        dord << '\n';
+    };
     auto t0 = std::chrono::steady_clock::now();
     long nreads = 0;
     double st[5] = {0, 0, 0, 0, 0};
@@ -317,15 +328,18 @@ int main(int argc, char** argv) {
         pileup_region(c, R, ref, chrLen, filtered, P, ctr);
         auto a1 = now();
         if (dp.is_open()) dump_pileup(dp, P);
+        if (dord.is_open()) dump_keys("pileup", P.nonIns.keys_in_order());
         auto a2 = now();
         realign_region(c, R, ref, chrLen, regionReads, P);
         auto a3 = now();
         if (dr.is_open()) dump_pileup(dr, P);
+        if (dord.is_open()) dump_keys("realigned", P.nonIns.keys_in_order());
         Aligned A;
         auto a4 = now();
         score_region(c, R, ref, chrLen, P, A);
         auto a5 = now();
         std::string text = format_simple(c, R, A);
+        if (dord.is_open()) dump_keys("vars", A.vars.keys_in_order());
         auto a6 = now();
         st[0] += std::chrono::duration<double>(a1 - a0).count();
         st[1] += std::chrono::duration<double>(a3 - a2).count();

@@ -25,8 +25,8 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-LIB_SOURCES = ["rv_pileup.hip", "rv_score.hip", "rv_host.cpp", "rv_engine.cpp", "rv_region.cpp",
-               "rv_realign.cpp", "rv_format.cpp", "rv_capi.cpp"]
+LIB_SOURCES = ["rv_pileup.hip", "rv_score.hip", "rv_order.hip", "rv_host.cpp", "rv_engine.cpp", "rv_region.cpp",
+               "rv_realign.cpp", "rv_format.cpp", "rv_order.cpp", "rv_capi.cpp"]
 
 
 def _run(cmd):

@@ -201,8 +201,8 @@ struct DenseRow {
     DenseBase b[4];
     int32_t cov;       // refCoverage
     uint32_t rowflags; // bit0: row touched by any getVariation on the position (nonIns key exists)
-    uint32_t min_ord_pos;  // first-touch read ordinal of the nonIns position key
-    uint32_t pad[5];
+    unsigned long long first_touch;  // (read ordinal << 24 | touch index within the read) of the key creation
+    uint32_t pad[4];
 };
 static_assert(sizeof(DenseRow) == 288, "DenseRow layout");
 

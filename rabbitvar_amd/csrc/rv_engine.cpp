@@ -66,7 +66,8 @@ Engine::~Engine() {
     DevBuf* all[] = {&b_prep,      &b_cig,     &b_ctr,    &b_span,       &b_rows,     &b_ref,    &b_events,
                      &b_arena,     &b_clips,   &b_clip_arena, &b_anchors, &b_counts,   &b_gather_ids, &b_gather_out,
                      &b_cx_pos,    &b_cx_off,  &b_cx,     &b_cx_arena,   &b_cx_cov,   &b_cx_key, &b_out,
-                     &b_out_arena, &b_score_ctr};
+                     &b_out_arena, &b_score_ctr, &b_status, &b_keys, &b_keys2, &b_kpos, &b_kpos2, &b_sort_tmp,
+                     &b_kcount};
     for (auto* b : all)
         if (b->p) hipFree(b->p);
     for (auto& b : b_reads)
@@ -197,6 +198,7 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
     rows = (DenseRow*)grow(b_rows, sizeof(DenseRow) * (size_t)W.dense_n);
     if (!rows) return fail(RV_ERR_OOM, "device allocation for the pileup rows failed");
     rows_n = W.dense_n;
+    rows_lo = W.dense_lo;
     RV_HIP(launch_fill_rows(rows, W.dense_n, stream));
     // ---- walk with bounded record buffers (grown and re-run on overflow)
     unsigned max_events = (unsigned)std::min<int64_t>(4 * n + 4096, 0x7fffffff);
@@ -303,6 +305,34 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
         max_anchors = std::max<unsigned>(max_anchors, counts[2]) * 2;
     }
     return fail(RV_ERR_CAPACITY, "walk record buffers kept overflowing");
+}
+
+rv_status Engine::key_order(int n_reads, std::vector<int32_t>& created) {
+    created.clear();
+    if (rows_n <= 0) return RV_OK;
+    auto* keys = (unsigned long long*)grow(b_keys, sizeof(unsigned long long) * (size_t)rows_n);
+    auto* keys2 = (unsigned long long*)grow(b_keys2, sizeof(unsigned long long) * (size_t)rows_n);
+    auto* pos = (int32_t*)grow(b_kpos, sizeof(int32_t) * (size_t)rows_n);
+    auto* pos2 = (int32_t*)grow(b_kpos2, sizeof(int32_t) * (size_t)rows_n);
+    auto* cnt = (unsigned*)grow(b_kcount, sizeof(unsigned));
+    if (!keys || !keys2 || !pos || !pos2 || !cnt) return fail(RV_ERR_OOM, "device buffers for the key order failed");
+    RV_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned), stream));
+    RV_HIP(launch_key_list(rows, rows_n, rows_lo, keys, pos, cnt, stream));
+    unsigned n = 0;
+    RV_HIP(hipMemcpyAsync(&n, cnt, sizeof n, hipMemcpyDeviceToHost, stream));
+    RV_HIP(hipStreamSynchronize(stream));
+    if (n == 0) return RV_OK;
+    int end_bit = 25;  // touch index (24 bits) + read ordinal
+    while (end_bit < 64 && (1ll << (end_bit - 24)) <= (long long)n_reads) end_bit++;
+    size_t tb = 0;
+    RV_HIP(sort_key_pairs(nullptr, tb, keys, keys2, pos, pos2, (int)n, end_bit, stream));
+    void* tmp = grow(b_sort_tmp, tb);
+    if (!tmp) return fail(RV_ERR_OOM, "device buffer for the key sort failed");
+    RV_HIP(sort_key_pairs(tmp, tb, keys, keys2, pos, pos2, (int)n, end_bit, stream));
+    created.resize(n);
+    RV_HIP(hipMemcpyAsync(created.data(), pos2, sizeof(int32_t) * n, hipMemcpyDeviceToHost, stream));
+    RV_HIP(hipStreamSynchronize(stream));
+    return RV_OK;
 }
 
 bool Engine::fetch_rows(const std::vector<int>& idx, std::vector<DenseRow>& out) {

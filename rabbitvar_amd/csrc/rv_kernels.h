@@ -19,6 +19,12 @@ static_assert(sizeof(DenseRow) % 16 == 0, "dense rows are copied in 16-byte word
 hipError_t launch_gather_rows(const DenseRow* rows, int dense_n, const int32_t* idx, int n, DenseRow* out,
                               hipStream_t s);
 
+hipError_t launch_key_list(const DenseRow* rows, int dense_n, int dense_lo, unsigned long long* keys, int32_t* pos,
+                           unsigned* count, hipStream_t s);
+// temp == nullptr: sets temp_bytes only
+hipError_t sort_key_pairs(void* temp, size_t& temp_bytes, const unsigned long long* kin, unsigned long long* kout,
+                          const int32_t* vin, int32_t* vout, int n, int end_bit, hipStream_t s);
+
 hipError_t launch_score(const ScoreIn& S, hipStream_t s);
 hipError_t launch_fisher_table(const int* tab, int n, double* out, hipStream_t s);
 

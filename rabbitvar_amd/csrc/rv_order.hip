@@ -1,0 +1,41 @@
+// nonInsertionVariants keys in creation order: compact the rows whose key exists into
+// (first touch, position) pairs, then sort the pairs by first touch (hipCUB radix sort).
+#include <hip/hip_runtime.h>
+
+#include <hipcub/hipcub.hpp>
+
+#include "rv_kernels.h"
+
+namespace rv {
+
+// one lane per dense row; one counter update per wavefront
+__global__ void __launch_bounds__(256) k_key_list(const DenseRow* rows, int dense_n, int dense_lo,
+                                                    unsigned long long* keys, int32_t* pos, unsigned* count) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool has = i < dense_n && (rows[i].rowflags & 1u);
+    const unsigned long long mask = __ballot(has);
+    const int lane = threadIdx.x & 63;
+    unsigned base = 0;
+    if (lane == 0 && mask) base = atomicAdd(count, (unsigned)__popcll(mask));
+    base = __shfl(base, 0);
+    if (has) {
+        const unsigned off = base + (unsigned)__popcll(mask & ((1ull << lane) - 1ull));
+        keys[off] = rows[i].first_touch;
+        pos[off] = dense_lo + i;
+    }
+}
+
+hipError_t launch_key_list(const DenseRow* rows, int dense_n, int dense_lo, unsigned long long* keys, int32_t* pos,
+                           unsigned* count, hipStream_t s) {
+    if (dense_n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_key_list, dim3((unsigned)((dense_n + 255) / 256)), dim3(256), 0, s, rows, dense_n, dense_lo,
+                       keys, pos, count);
+    return hipGetLastError();
+}
+
+hipError_t sort_key_pairs(void* temp, size_t& temp_bytes, const unsigned long long* kin, unsigned long long* kout,
+                          const int32_t* vin, int32_t* vout, int n, int end_bit, hipStream_t s) {
+    return hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, kin, kout, vin, vout, n, 0, end_bit, s);
+}
+
+}  // namespace rv

@@ -180,8 +180,8 @@ __device__ inline void fill_row_one(DenseRow* rows, int i) {
     }
     z.cov = 0;
     z.rowflags = 0;
-    z.min_ord_pos = 0xffffffffu;
-    for (int k = 0; k < 5; k++) z.pad[k] = 0;
+    z.first_touch = ~0ull;
+    for (int k = 0; k < 4; k++) z.pad[k] = 0;
     rows[i] = z;
 }
 
@@ -216,6 +216,7 @@ __device__ inline void walk_one(int64_t r, const ReadsDev& R, const Window& W, c
     w.n_cigar = p.ncig;
     w.ord = (int)r;
     w.evseq = 0;
+    w.tseq = 0;
     w.mapq = R.mapq[r];
     w.nm = p.nm_eff;
     w.dir = (R.flag[r] & 0x10) != 0;

@@ -141,6 +141,7 @@ HVar* RState::get_variation(int p, const std::string& d) {
     if (!r.key) {
         r.key = true;
         r.v.clear();
+        key_log.push_back({p, true});
     }
     HVar*& v = r.v[d];
     if (!v) v = new_var();
@@ -163,7 +164,10 @@ void RState::erase_inner(int p, const std::string& d, bool drop_empty) {
     if (!r.key) return;
     r.dirty = true;
     r.v.erase(d);
-    if (drop_empty && r.v.empty()) r.key = false;
+    if (drop_empty && r.v.empty()) {
+        r.key = false;
+        key_log.push_back({p, false});
+    }
 }
 
 void RState::add_cov(int p, int k) {

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "rv_host.h"
+#include "rv_order.h"
 #include "rv_score.h"
 
 namespace rv {
@@ -105,7 +106,7 @@ class RState {
     std::deque<HVar> pool;
     std::unordered_map<int, RPos> pos;
     std::map<int, std::map<std::string, HVar*>> ins;
-    std::vector<std::map<std::string, HVar*>> graveyard;  // maps of erased positions
+    std::vector<KeyOp> key_log;  // nonInsertionVariants key insertions / erasures, in order
 };
 
 struct RealignStats {

@@ -5,9 +5,13 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cstring>
+#include <thread>
+#include <unordered_map>
 
 #include "rv_format.h"
+#include "rv_order.h"
 #include "rv_realign.h"
 
 namespace rv {
@@ -35,7 +39,27 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     PileupResult pr;
     rv_status s = e.pileup(region, reads, contig, false, pr);
     if (s != RV_OK) return s;
+    // ---- nonInsertionVariants keys in creation order; their slot-order replay runs on a second
+    // host thread while this one realigns
+    std::vector<int32_t> created;
+    s = e.key_order((int)std::min<int64_t>(reads.n_reads, INT32_MAX), created);
+    if (s != RV_OK) return s;
     auto h0 = std::chrono::steady_clock::now();
+    RHOrder nonins;
+    std::exception_ptr replay_err;
+    std::thread replay([&] {
+        try {
+            replay_created(nonins, created);
+        } catch (...) {
+            replay_err = std::current_exception();
+        }
+    });
+    struct Joiner {
+        std::thread& t;
+        ~Joiner() {
+            if (t.joinable()) t.join();
+        }
+    } joiner{replay};
     // ---- realignment on the host
     DeviceRowSource src(e);
     RowCache rc(&src, pr.P.dense_lo, pr.P.dense_n);
@@ -112,10 +136,14 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     in.cx_key = (const uint8_t*)up(e.b_cx_key, cx.key.data(), cx.key.size());
     if (!in.cx_pos || !in.cx_off || !in.cx || !in.cx_arena || !in.cx_cov || !in.cx_key)
         return e.fail(RV_ERR_OOM, "device buffers for the record list failed");
+    const size_t n_status = (size_t)pr.P.dense_n + (size_t)ncx;
+    in.status = (uint8_t*)e.grow(e.b_status, n_status);
+    if (!in.status) return e.fail(RV_ERR_OOM, "device buffer for position status failed");
     unsigned max_out = (unsigned)std::min<int64_t>(65536 + 8 * (int64_t)ncx + pr.P.dense_n / 16, 0x7fffffff);
     unsigned long long arena_cap = 160ull * max_out;
     std::vector<OutRec> recs;
     std::vector<char> arena;
+    std::vector<uint8_t> status;
     unsigned long long hc[8];
     float score_ms = 0;
     for (int attempt = 0; attempt < 3; attempt++) {
@@ -124,6 +152,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
         unsigned long long* ctr = (unsigned long long*)e.grow(e.b_score_ctr, 64);
         if (!in.out || !in.arena || !ctr) return e.fail(RV_ERR_OOM, "device buffers for the scored records failed");
         RV_HIPS(hipMemsetAsync(ctr, 0, 64, e.stream));
+        RV_HIPS(hipMemsetAsync(in.status, 0, n_status, e.stream));
         in.n_out = (unsigned*)ctr;
         in.arena_used = ctr + 1;
         in.counters = ctr + 2;
@@ -149,6 +178,8 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
             arena.resize((size_t)used + 1);
             if (n_out) RV_HIPS(hipMemcpyAsync(recs.data(), in.out, sizeof(OutRec) * n_out, hipMemcpyDeviceToHost, e.stream));
             if (used) RV_HIPS(hipMemcpyAsync(arena.data(), in.arena, (size_t)used, hipMemcpyDeviceToHost, e.stream));
+            status.resize(n_status);
+            RV_HIPS(hipMemcpyAsync(status.data(), in.status, n_status, hipMemcpyDeviceToHost, e.stream));
             RV_HIPS(hipStreamSynchronize(e.stream));
             break;
         }
@@ -156,9 +187,29 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
         max_out = std::max<unsigned>(max_out, n_out) * 2;
         arena_cap = std::max(arena_cap, used) * 2;
     }
-    // ---- text, in position order
+    // ---- text, in the printer's order: alignedVariants slot order, then the variants' order
     auto t0 = std::chrono::steady_clock::now();
+    replay.join();
+    if (replay_err) return e.fail(RV_ERR_DEVICE, "key order replay failed");
+    std::vector<int32_t> proc = replay_finish(nonins, S.key_log);
+    std::vector<int32_t> aligned = aligned_order(proc, [&](int32_t p) -> int {
+        auto it = std::lower_bound(cx.pos.begin(), cx.pos.end(), p);
+        if (it != cx.pos.end() && *it == p) return status[(size_t)pr.P.dense_n + (size_t)(it - cx.pos.begin())];
+        const int i = p - pr.P.dense_lo;
+        return (i >= 0 && i < pr.P.dense_n) ? status[(size_t)i] : 0;
+    });
+    {
+        std::unordered_map<int32_t, int32_t> rank;
+        rank.reserve(recs.size() * 2 + 1);
+        for (const OutRec& o : recs) rank.emplace(o.position, INT32_MAX);
+        for (size_t i = 0; i < aligned.size(); i++) {
+            auto it = rank.find(aligned[i]);
+            if (it != rank.end()) it->second = (int32_t)i;
+        }
+        for (OutRec& o : recs) o.start_rank = rank[o.position];
+    }
     std::sort(recs.begin(), recs.end(), [](const OutRec& a, const OutRec& b) {
+        if (a.start_rank != b.start_rank) return a.start_rank < b.start_rank;
         if (a.position != b.position) return a.position < b.position;
         return a.order < b.order;
     });

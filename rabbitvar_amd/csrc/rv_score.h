@@ -38,6 +38,7 @@ struct OutRec {
     double freq, pmean, qmean, mq, hlr, hifreq, extrafreq, msi, nm, fisher_p, fisher_ratio;
     uint32_t str_off;
     uint16_t len[6];  // refallele, varallele, genotype, bias, leftseq, rightseq
+    int32_t start_rank;  // host: the position's rank in the printer's order
 };
 
 enum : int { VT_SNV = 0, VT_INS = 1, VT_DEL = 2, VT_COMPLEX = 3, VT_EMPTY = 4, VT_SV = 5 };
@@ -62,6 +63,8 @@ struct ScoreIn {
     char* arena;
     unsigned long long* arena_used;
     unsigned long long arena_cap;
+    uint8_t* status;  // per dense row, then per record-list position: 1 = alignedVariants entry kept,
+                      // 2 = added and erased again (nothing above -f), 0 = not added
     unsigned long long* counters;  // [0] capacity overflows, [1] insertion side effects on p+1,
                                    // [2] sorts above 16 elements, [3] qstd ambiguous rows
 };

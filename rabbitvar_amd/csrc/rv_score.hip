@@ -11,12 +11,15 @@ namespace rv {
 
 __global__ void __launch_bounds__(128) k_score_dense(ScoreIn S, const int32_t* pos_list, int n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) score_position<128, 4>(S, pos_list ? pos_list[i] : S.dense_lo + i, true);
+    if (i < n) {
+        const int p = pos_list ? pos_list[i] : S.dense_lo + i;
+        score_position<128, 4>(S, p, true, p - S.dense_lo);
+    }
 }
 
 __global__ void __launch_bounds__(64) k_score_cx(ScoreIn S) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < S.n_cx) score_position<1536, MAXV>(S, S.cx_pos[i], false);
+    if (i < S.n_cx) score_position<1536, MAXV>(S, S.cx_pos[i], false, S.dense_n + i);
 }
 
 __global__ void __launch_bounds__(256) k_fisher_table(const int* tab, int n, double* out) {

@@ -459,7 +459,7 @@ struct Strs {
 }  // namespace
 
 template <int CAP, int NA>
-__device__ void score_position(const ScoreIn& S, int p, bool dense_only) {
+__device__ void score_position(const ScoreIn& S, int p, bool dense_only, int slot) {
     const Config& c = S.cfg;
     PosView<NA> V;
     load_pos<NA>(S, p, V);
@@ -534,7 +534,12 @@ __device__ void score_position(const ScoreIn& S, int p, bool dense_only) {
         }
     }
     if (nv == 0) return;  // no Vars entry was created for the position
-    if (!c.doPileup && maxfreq <= c.freq && !c.hasBam2) return;
+    // alignedVariants: the position was added, and is erased again when nothing passes -f
+    if (!c.doPileup && maxfreq <= c.freq && !c.hasBam2) {
+        S.status[slot] = 2;
+        return;
+    }
+    S.status[slot] = 1;
     bool in_region = !(p < S.region_start || p > S.region_end);
     // the output filter skips positions outside the region; nothing else reads them
     if (!in_region) return;

@@ -83,6 +83,7 @@ struct Walker {
     int n_cigar;
     int ord;
     int evseq;
+    uint32_t tseq;       // nonInsertionVariants key touches so far in this read
     uint8_t mapq;
     int nm;
     bool dir;
@@ -116,9 +117,13 @@ struct Walker {
         }
         return &S->dense[i];
     }
+    // a getVariation on nonInsertionVariants at this row: the key exists from here on, and the
+    // smallest (read, touch) pair records when the reference created it (its hash-map insertion
+    // order, replayed on the host)
     __device__ void touch_pos(DenseRow* r) {
         if (!(r->rowflags & 1u)) atomicOr(&r->rowflags, 1u);
-        atomicMin(&r->min_ord_pos, (unsigned)ord);
+        const unsigned long long key = ((unsigned long long)(unsigned)ord << 24) | (tseq++ & 0xffffffu);
+        if (key < r->first_touch) atomicMin(&r->first_touch, key);
     }
     __device__ void cov(int pos, int k = 1) {
         DenseRow* r = row(pos);

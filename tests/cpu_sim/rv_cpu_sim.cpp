@@ -4,7 +4,11 @@
 //
 //   rv_cpu_sim REF.fa READS.rvb START END K FISHER OUT      whole path for chrS:START-END
 //   rv_cpu_sim --rh-chars B1 B2 ...                         robin_hood <char,int> order of bytes
+//   rv_cpu_sim --rh-ints < OPS                              robin_hood <int,V> order after "i K" /
+//                                                           "e K" operations (one per line)
 #include <algorithm>
+#include <climits>
+#include <unordered_map>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -15,7 +19,9 @@
 #include "rv_format.h"
 #include "rv_host.h"
 #include "rv_pileup_body.h"
+#include "rv_order.h"
 #include "rv_realign.h"
+#include "rv_rhorder.h"
 #include "rv_score_body.h"
 
 using namespace rv;
@@ -35,6 +41,17 @@ int main(int argc, char** argv) {
         for (int i = 2; i < argc; i++) ins.push_back((char)atoi(argv[i]));
         for (char c : rh_char_iteration_order(ins)) printf("%d ", (int)(unsigned char)c);
         printf("\n");
+        return 0;
+    }
+    if (argc >= 2 && std::string(argv[1]) == "--rh-ints") {
+        RHOrder m;
+        char op[8];
+        long long key;
+        while (scanf("%7s %lld", op, &key) == 2) {
+            if (op[0] == 'i') m.insert((int32_t)key);
+            if (op[0] == 'e') m.erase((int32_t)key);
+        }
+        m.for_each([](int32_t k) { printf("%d\n", k); });
         return 0;
     }
     if (argc < 8) return 2;
@@ -126,6 +143,13 @@ int main(int argc, char** argv) {
                (unsigned)an.size(), &ctr};
     for (int64_t r = 0; r < m; r++) walk_one(r, R, W, cfg, prep.data(), cg.data(), S);
     for (unsigned i = 0; i < na; i++) anchor_one(i, an.data(), rows.data(), W.dense_lo, W.dense_n, cfg);
+    // nonInsertionVariants keys in creation order
+    std::vector<std::pair<uint64_t, int32_t>> ft;
+    for (int i = 0; i < W.dense_n; i++)
+        if (rows[(size_t)i].rowflags & 1u) ft.push_back({rows[(size_t)i].first_touch, W.dense_lo + i});
+    std::sort(ft.begin(), ft.end());
+    std::vector<int32_t> created;
+    for (auto& e : ft) created.push_back(e.second);
     HostPileup HP;
     HP.dense_lo = W.dense_lo;
     HP.dense_n = W.dense_n;
@@ -178,11 +202,25 @@ int main(int argc, char** argv) {
     in.arena_used = &aused;
     in.arena_cap = oarena.size();
     in.counters = sctr;
-    for (int i = 0; i < W.dense_n; i++) score_position<128, 4>(in, W.dense_lo + i, true);
-    for (int i = 0; i < in.n_cx; i++) score_position<1536, MAXV>(in, cx.pos[(size_t)i], false);
+    std::vector<uint8_t> status((size_t)W.dense_n + cx.pos.size(), 0);
+    in.status = status.data();
+    for (int i = 0; i < W.dense_n; i++) score_position<128, 4>(in, W.dense_lo + i, true, i);
+    for (int i = 0; i < in.n_cx; i++) score_position<1536, MAXV>(in, cx.pos[(size_t)i], false, W.dense_n + i);
     out.resize(std::min<unsigned>(nout, (unsigned)out.size()));
-    std::sort(out.begin(), out.end(), [](const OutRec& a, const OutRec& b) {
-        return a.position != b.position ? a.position < b.position : a.order < b.order;
+    // the printer's order: alignedVariants slot order, then the variants' order within a position
+    std::vector<int32_t> proc = noninsertion_order(created, RS.key_log);
+    std::vector<int32_t> aligned = aligned_order(proc, [&](int32_t p) -> int {
+        auto it = std::lower_bound(cx.pos.begin(), cx.pos.end(), p);
+        if (it != cx.pos.end() && *it == p) return status[(size_t)W.dense_n + (size_t)(it - cx.pos.begin())];
+        const int i = p - W.dense_lo;
+        return (i >= 0 && i < W.dense_n) ? status[(size_t)i] : 0;
+    });
+    std::unordered_map<int32_t, int32_t> rank;
+    for (size_t i = 0; i < aligned.size(); i++) rank[aligned[i]] = (int32_t)i;
+    std::sort(out.begin(), out.end(), [&](const OutRec& a, const OutRec& b) {
+        const int32_t ra = rank.count(a.position) ? rank[a.position] : INT32_MAX;
+        const int32_t rb = rank.count(b.position) ? rank[b.position] : INT32_MAX;
+        return ra != rb ? ra < rb : (a.position != b.position ? a.position < b.position : a.order < b.order);
     });
     LineContext ctx{"S1", "chrS", "chrS", rs, re, fisher != 0};
     std::string text;

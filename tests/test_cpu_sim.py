@@ -18,7 +18,8 @@ CSRC = os.path.join(ROOT, "rabbitvar_amd", "csrc")
 def sim():
     os.makedirs(os.path.dirname(SIM), exist_ok=True)
     srcs = [os.path.join(SIM_DIR, "rv_cpu_sim.cpp")] + [os.path.join(CSRC, f) for f in
-                                                        ("rv_host.cpp", "rv_realign.cpp", "rv_format.cpp")]
+                                                        ("rv_host.cpp", "rv_realign.cpp", "rv_format.cpp",
+                                                         "rv_order.cpp")]
     subprocess.run(["g++", "-O2", "-std=c++17", "-I", SIM_DIR, "-I", CSRC, "-I", os.path.join(ROOT, "include"), "-o",
                     SIM] + srcs, check=True)
     return SIM
@@ -38,6 +39,7 @@ def test_engine_stages_match_oracle(sim, dataset, case, k, fisher):
         got = f.read()
     assert len(want.splitlines()) > 100
     assert sorted(got.splitlines()) == sorted(want.splitlines())
+    assert got == want  # same lines in the same (robin_hood slot) order
 
 
 @pytest.mark.parametrize("start,end", [(1, 1), (5000, 5150), (59000, 60000)])
@@ -48,7 +50,9 @@ def test_engine_stages_subregions(sim, dataset, start, end):
     subprocess.run([sim, os.path.join(d, "ref.fa"), os.path.join(d, "reads.rvb"), str(start), str(end), "1", "1", out],
                    check=True)
     with open(out) as f:
-        assert sorted(f.read().splitlines()) == sorted(want.splitlines())
+        got = f.read()
+    assert sorted(got.splitlines()) == sorted(want.splitlines())
+    assert got == want
 
 
 def test_char_map_order_matches_oracle_clone(sim, tools):
@@ -67,3 +71,29 @@ def test_char_map_order_matches_oracle_clone(sim, tools):
         g = subprocess.run([sim, "--rh-chars"] + [str(b) for b in seq], capture_output=True, text=True, check=True)
         got = [int(x) for x in g.stdout.split()]
         assert got == want, seq
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_int_map_order_matches_oracle_clone(sim, tools, seed):
+    """The engine's robin_hood slot-order model (rv_rhorder.h) against the oracle's clone, on insert /
+    erase sequences large enough to grow the table many times and to halve the info increment."""
+    probe = os.path.join(ROOT, "oracle", "_build", "rhmap_probe")
+    if not os.path.exists(probe):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    rnd = random.Random(seed)
+    n = [100, 5000, 60000, 200000, 20000, 1000][seed]
+    ops, live = [], []
+    for i in range(n):
+        if live and rnd.random() < 0.1:
+            k = live.pop(rnd.randrange(len(live)))
+            ops.append(f"e {k}\n")
+        else:
+            k = rnd.randrange(-1000, 10**7) if seed % 2 else i * 3 + rnd.randrange(3)
+            live.append(k)
+            ops.append(f"i {k}\n")
+    text = "".join(ops)
+    r = subprocess.run([probe], input=text, capture_output=True, text=True, check=True)
+    want = [int(ln.split()[1]) for ln in r.stdout.splitlines()[1:-1]]
+    g = subprocess.run([sim, "--rh-ints"], input=text, capture_output=True, text=True, check=True)
+    got = [int(x) for x in g.stdout.split()]
+    assert got == want
