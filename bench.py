@@ -25,12 +25,18 @@ WORKLOAD = ["--seed", "7", "--len", "5000000", "--depth", "60", "--indel-rate", 
             "--long-dels", "200", "--repeat-frac", "0.02"]
 CPU_SAMPLE = (1, 2000000)
 
-# sizes of the walk kernel's fixed per-read inputs (bytes): read fields (pos0 4, flag 2, mapq 1,
-# has_nm 1, nm 4, mpos0 4, tid 4, mtid 4, seq_off 8, cigar_off 8) and the prep record (32)
-READ_FIELDS = 40
-PREP_REC = 32
-DENSE_ROW = 288
 HBM_PEAK_GBS = 8000.0
+
+
+def walk_alg_bytes(batch, region_bp):
+    """Algorithmic HBM bytes of one pileup launch: every read's bases, qualities, CIGAR and 16 B of
+    fixed fields in, the reference window in (region + 2 x 1200 bp), and the minimum per-position
+    output out (coverage + 4 bases x 2 strands as int32 = 36 B)."""
+    import numpy as np
+
+    lseq = np.diff(batch.seq_off).astype(np.int64)
+    ncig = np.diff(batch.cigar_off).astype(np.int64)
+    return int((2 * lseq + 4 * ncig + 16).sum()) + (region_bp + 2400) + 36 * region_bp
 
 
 def log(msg):
@@ -73,7 +79,6 @@ def main():
               "regions would be sharded across GPUs by the caller.", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import torch
 
     sys.path.insert(0, ROOT)
@@ -110,12 +115,7 @@ def main():
     log(f"timed {a.steps} steps in {dt:.3f} s")
     reads_per_s = batch.n * a.steps / dt
 
-    # walk kernel: algorithmic bytes per launch
-    lseq = np.diff(batch.seq_off).astype(np.int64)
-    ncig = np.diff(batch.cigar_off).astype(np.int64)
-    read_bytes = int((READ_FIELDS + PREP_REC + 2 * lseq + 4 * ncig).sum())
-    st = eng.stats()
-    alg_bytes = read_bytes + 2 * DENSE_ROW * st["dense_rows"] + st["event_bytes"]
+    alg_bytes = walk_alg_bytes(batch, L)
     avg_walk_s = sum(walk_ms) / len(walk_ms) / 1e3
     achieved = alg_bytes / avg_walk_s / 1e9
     traffic = None

@@ -12,6 +12,11 @@
 //   src/parseCigar.cpp:1756-1815         (appendSegments)
 //   src/parseCigar.cpp:1858-1892         (cleanupCigar)
 //   include/VariationUtils.h:99-113      (adjInsPos)
+//
+// Pinned by tests/test_oracle_cases.py (hand-built reads: match runs, substitutions, deletions,
+// insertions, coverage, strand and quality sums, read-position statistic, pstd/qstd).
+// Parity unpinned: the soft-clip rescue and consensus tables, findOffset / appendSegments (complex
+// M-run descriptors) and the insertion-anchor bookkeeping have no independent check.
 #include <algorithm>
 #include <regex>
 #include <stdexcept>

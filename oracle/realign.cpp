@@ -14,6 +14,10 @@
 //   findMM5 / findMM3                   :1857-1976
 // and include/VariationUtils.h (adjCnt :291-332, correctCnt :263-280, findconseq :403-509,
 // islowcomplexseq :367-394, joinRef :336-349), include/util.h (vc_substr :27-57, charAt :171-179).
+//
+// Parity unpinned: the realigner's decisions have no golden vector or closed-form check (the
+// reference ships no tests and cannot be built here); agreement with the engine's separate host
+// restatement on synthetic data is the only check.
 #include <algorithm>
 #include <cmath>
 #include <regex>

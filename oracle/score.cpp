@@ -18,6 +18,11 @@
 // the published algorithm (kf_lgamma: Lanczos/AS245 form; hypergeo_acc: incremental recurrence
 // re-anchored every 11 steps; tails walked inward while p < 0.99999999*q, boundary included when
 // p < 1.00000001*q).
+//
+// Pinned: kt_fisher_two_sided by tests/test_oracle_fisher.py (exact rational enumeration, symmetry,
+// scipy); the printed fields of SNV / deletion / insertion calls by tests/test_oracle_cases.py.
+// Parity unpinned: complex-allele construction in collectReferenceVariants (&, #, ^ descriptors)
+// and findMSI have no independent check.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
