@@ -101,7 +101,8 @@ def main():
         text = step()
         log(f"warmup {i}: {time.perf_counter() - t0:.3f} s, {len(text.splitlines())} calls")
     walk_ms = []
-    stage = {k: 0.0 for k in ("prep_ms", "walk_ms", "anchor_ms", "score_ms", "host_ms")}
+    stage = {k: 0.0 for k in ("prep_ms", "walk_ms", "anchor_ms", "score_ms", "reduce_ms", "realign_ms", "order_ms",
+                              "format_ms", "wall_ms")}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(a.steps):

@@ -139,6 +139,12 @@ typedef struct rv_stats {
     int64_t event_bytes;       /* bytes of ordered walk records written by the walk kernel */
     int64_t cx_positions;      /* positions scored from the record list (non-dense or realigned) */
     int64_t rows_fetched;      /* dense rows copied to the host for realignment */
+    double reduce_ms;          /* host: ordered reduction of the walk records */
+    double realign_ms;         /* host: realignment (and the record list for scoring) */
+    double order_ms;           /* key-order sort and the map-order replay left on the critical path */
+    double format_ms;          /* host: text */
+    double wall_ms;            /* rv_run_region, end to end */
+    int64_t order_sorted;      /* 1: output order by the slot sort, 0: by the full map replay */
 } rv_stats;
 
 void rv_config_default(rv_config* cfg);

@@ -1,6 +1,8 @@
 // Engine implementation: device orchestration of the pileup stage.
 #include "rv_engine.h"
 
+#include "rv_order.h"
+
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -295,6 +297,7 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
             stats.cx_positions = 0;
             stats.rows_fetched = 0;
             stats.host_ms = host_ms;
+            stats.reduce_ms = host_ms;
             (void)h0;
             return RV_OK;
         }
@@ -307,8 +310,9 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
     return fail(RV_ERR_CAPACITY, "walk record buffers kept overflowing");
 }
 
-rv_status Engine::key_order(int n_reads, std::vector<int32_t>& created) {
+rv_status Engine::key_order(int n_reads, std::vector<int32_t>& created, std::vector<int32_t>& slot_sorted) {
     created.clear();
+    slot_sorted.clear();
     if (rows_n <= 0) return RV_OK;
     auto* keys = (unsigned long long*)grow(b_keys, sizeof(unsigned long long) * (size_t)rows_n);
     auto* keys2 = (unsigned long long*)grow(b_keys2, sizeof(unsigned long long) * (size_t)rows_n);
@@ -329,6 +333,23 @@ rv_status Engine::key_order(int n_reads, std::vector<int32_t>& created) {
     void* tmp = grow(b_sort_tmp, tb);
     if (!tmp) return fail(RV_ERR_OOM, "device buffer for the key sort failed");
     RV_HIP(sort_key_pairs(tmp, tb, keys, keys2, pos, pos2, (int)n, end_bit, stream));
+    // the same keys in sorted slot order: pos2 holds them in creation order
+    auto* pos3 = (int32_t*)grow(b_kpos3, sizeof(int32_t) * (size_t)rows_n);
+    if (!pos3) return fail(RV_ERR_OOM, "device buffers for the key order failed");
+    const int cap_log2 = rh_log2(rh_capacity(n));
+    int idx_bits = 1;
+    while (idx_bits < 40 && (1ll << idx_bits) <= (long long)n) idx_bits++;
+    const int slot_bits = cap_log2 + 5 + idx_bits;
+    if (slot_bits <= 64) {
+        RV_HIP(launch_slot_keys(pos2, (int)n, cap_log2, idx_bits, keys, pos, stream));
+        size_t tb2 = 0;
+        RV_HIP(sort_key_pairs(nullptr, tb2, keys, keys2, pos, pos3, (int)n, slot_bits, stream));
+        void* tmp2 = grow(b_sort_tmp, std::max(tb, tb2));
+        if (!tmp2) return fail(RV_ERR_OOM, "device buffer for the key sort failed");
+        RV_HIP(sort_key_pairs(tmp2, tb2, keys, keys2, pos, pos3, (int)n, slot_bits, stream));
+        slot_sorted.resize(n);
+        RV_HIP(hipMemcpyAsync(slot_sorted.data(), pos3, sizeof(int32_t) * n, hipMemcpyDeviceToHost, stream));
+    }
     created.resize(n);
     RV_HIP(hipMemcpyAsync(created.data(), pos2, sizeof(int32_t) * n, hipMemcpyDeviceToHost, stream));
     RV_HIP(hipStreamSynchronize(stream));

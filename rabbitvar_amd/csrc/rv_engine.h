@@ -51,12 +51,13 @@ class Engine {
     // copies the listed dense rows (indices into the window) of the last pileup to the host
     bool fetch_rows(const std::vector<int>& rows, std::vector<DenseRow>& out);
 
-    // nonInsertionVariants keys of the last pileup in creation order
-    rv_status key_order(int n_reads, std::vector<int32_t>& created);
+    // nonInsertionVariants keys of the last pileup in creation order, and the same keys in the
+    // slot order they take when sorted (see rv_order.h)
+    rv_status key_order(int n_reads, std::vector<int32_t>& created, std::vector<int32_t>& slot_sorted);
 
     // buffers of the realignment / scoring stages
     DevBuf b_gather_ids, b_gather_out, b_cx_pos, b_cx_off, b_cx, b_cx_arena, b_cx_cov, b_cx_key, b_out, b_out_arena,
-        b_score_ctr, b_status, b_keys, b_keys2, b_kpos, b_kpos2, b_sort_tmp, b_kcount;
+        b_score_ctr, b_status, b_keys, b_keys2, b_kpos, b_kpos2, b_sort_tmp, b_kcount, b_kpos3;
     hipEvent_t sev[2] = {nullptr, nullptr};
     // figures of the last region: record-list positions, dense rows fetched for the
     // realigner, insertions whose coverage came from the next position

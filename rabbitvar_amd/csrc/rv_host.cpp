@@ -6,6 +6,7 @@
 #include <cstring>
 #include <numeric>
 #include <stdexcept>
+#include <unordered_map>
 
 namespace rv {
 
@@ -111,7 +112,7 @@ struct CharTable {
 };
 }  // namespace
 
-std::vector<char> rh_char_iteration_order(const std::vector<char>& inserted) {
+static std::vector<char> char_order_uncached(const std::vector<char>& inserted) {
     CharTable t;
     t.init(8);
     for (char c : inserted) t.insert((int)c);
@@ -119,6 +120,35 @@ std::vector<char> rh_char_iteration_order(const std::vector<char>& inserted) {
     for (size_t i = 0; i < t.buffer(); i++)
         if (t.info[i] != 0) out.push_back((char)t.key[i]);
     return out;
+}
+
+std::vector<char> rh_char_iteration_order(const std::vector<char>& inserted) {
+    if (inserted.size() > 7) return char_order_uncached(inserted);
+    char buf[8];
+    int n = rh_char_order_small(inserted.data(), (int)inserted.size(), buf);
+    return std::vector<char>(buf, buf + n);
+}
+
+int rh_char_order_small(const char* inserted, int n, char* out) {
+    // soft-clip tables see a handful of distinct insertion sequences (first-seen orders of a few
+    // bases), so the order is memoised per sequence
+    uint64_t key = (uint64_t)n;
+    for (int i = 0; i < n; i++) key |= (uint64_t)(uint8_t)inserted[i] << (8 * (i + 1));
+    struct Order {
+        char c[8];
+        int n;
+    };
+    thread_local std::unordered_map<uint64_t, Order> memo;
+    auto it = memo.find(key);
+    if (it == memo.end()) {
+        std::vector<char> o = char_order_uncached(std::vector<char>(inserted, inserted + n));
+        Order v{};
+        v.n = (int)o.size();
+        for (int i = 0; i < v.n; i++) v.c[i] = o[(size_t)i];
+        it = memo.emplace(key, v).first;
+    }
+    for (int i = 0; i < it->second.n; i++) out[i] = it->second.c[i];
+    return it->second.n;
 }
 
 HVar dense_to_hvar(const DenseBase& d) {

@@ -26,6 +26,8 @@ struct HVar {
 // insertion order.  The reference's soft-clip consensus breaks ties in that order
 // (include/VariationUtils.h:431-443).
 std::vector<char> rh_char_iteration_order(const std::vector<char>& inserted);
+// the same for at most 7 distinct keys, without allocating (out holds n entries)
+int rh_char_order_small(const char* inserted, int n, char* out);
 
 struct HSclip {
     HVar v;

@@ -21,6 +21,8 @@ hipError_t launch_gather_rows(const DenseRow* rows, int dense_n, const int32_t* 
 
 hipError_t launch_key_list(const DenseRow* rows, int dense_n, int dense_lo, unsigned long long* keys, int32_t* pos,
                            unsigned* count, hipStream_t s);
+hipError_t launch_slot_keys(const int32_t* keys, int n, int cap_log2, int idx_bits, unsigned long long* comp,
+                            int32_t* val, hipStream_t s);
 // temp == nullptr: sets temp_bytes only
 hipError_t sort_key_pairs(void* temp, size_t& temp_bytes, const unsigned long long* kin, unsigned long long* kout,
                           const int32_t* vin, int32_t* vout, int n, int end_bit, hipStream_t s);

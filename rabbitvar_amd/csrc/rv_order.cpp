@@ -1,10 +1,94 @@
-// Slot-order replay of nonInsertionVariants (see rv_order.h).
+// Slot-order replay and sort of the reference's robin_hood maps (see rv_order.h).
 #include "rv_order.h"
+
+#include <algorithm>
+#include <unordered_set>
 
 namespace rv {
 
+namespace {
+
+uint64_t hash_int(int32_t k) {
+    const uint64_t x = (uint64_t)(int64_t)k;
+    const unsigned __int128 p = (unsigned __int128)x * (unsigned __int128)0xde5fb9d2630458e9ULL;
+    return (uint64_t)(p >> 64) + (uint64_t)p;
+}
+
+int bits_for(size_t n) {
+    int b = 1;
+    while (b < 63 && ((size_t)1 << b) <= n) b++;
+    return b;
+}
+
+}  // namespace
+
+size_t rh_capacity(size_t max_count) {
+    size_t cap = 8;
+    while (cap * 80 / 100 < max_count) cap *= 2;
+    return cap;
+}
+
+int rh_log2(size_t cap) {
+    int l = 0;
+    while (((size_t)1 << l) < cap) l++;
+    return l;
+}
+
+int rh_max_displacement(const int32_t* keys, size_t n, size_t cap) {
+    const uint64_t mask = (uint64_t)cap - 1;
+    int64_t prev = -1;
+    int maxd = 0;
+    for (size_t i = 0; i < n; i++) {
+        const int64_t home = (int64_t)(hash_int(keys[i]) & mask);
+        const int64_t slot = home > prev + 1 ? home : prev + 1;
+        if (slot - home > maxd) maxd = (int)(slot - home);
+        prev = slot;
+    }
+    return maxd;
+}
+
+std::vector<int32_t> rh_sort_keys(const std::vector<int32_t>& keys, size_t cap) {
+    const size_t n = keys.size();
+    const int ib = bits_for(n);
+    const int cl = rh_log2(cap);
+    const int total = cl + 5 + ib;
+    std::vector<uint64_t> a(n), b(n);
+    std::vector<int32_t> va(n), vb(n);
+    for (size_t i = 0; i < n; i++) {
+        const uint64_t h = hash_int(keys[i]);
+        a[i] = ((((h & ((uint64_t)cap - 1)) << 5) | (31 - (h >> 59))) << ib) | (uint64_t)i;
+        va[i] = keys[i];
+    }
+    // LSD radix sort, 11-bit digits
+    constexpr int D = 11;
+    std::vector<size_t> cnt((size_t)1 << D);
+    for (int sh = 0; sh < total; sh += D) {
+        std::fill(cnt.begin(), cnt.end(), 0);
+        for (size_t i = 0; i < n; i++) cnt[(a[i] >> sh) & ((1u << D) - 1)]++;
+        size_t acc = 0;
+        for (auto& c : cnt) {
+            size_t t = c;
+            c = acc;
+            acc += t;
+        }
+        for (size_t i = 0; i < n; i++) {
+            const size_t d = (a[i] >> sh) & ((1u << D) - 1);
+            b[cnt[d]] = a[i];
+            vb[cnt[d]++] = va[i];
+        }
+        a.swap(b);
+        va.swap(vb);
+    }
+    return va;
+}
+
 void replay_created(RHOrder& m, const std::vector<int32_t>& created) {
-    for (int32_t p : created) m.insert(p);
+    constexpr size_t AHEAD = 24;
+    const size_t n = created.size();
+    for (size_t i = 0; i < n; i++) {
+        if (i + AHEAD < n) m.prefetch(created[i + AHEAD]);
+        m.insert(created[i]);
+    }
 }
 
 std::vector<int32_t> noninsertion_order(const std::vector<int32_t>& created, const std::vector<KeyOp>& log) {
@@ -24,6 +108,74 @@ std::vector<int32_t> replay_finish(RHOrder& m, const std::vector<KeyOp>& log) {
     out.reserve(m.size());
     m.for_each([&](int32_t k) { out.push_back(k); });
     return out;
+}
+
+bool noninsertion_order_sorted(const std::vector<int32_t>& walk_sorted, const std::vector<KeyOp>& log,
+                               std::vector<int32_t>& out) {
+    const int maxd = rh_max_displacement(walk_sorted.data(), walk_sorted.size(), rh_capacity(walk_sorted.size()));
+    return noninsertion_order_sorted(walk_sorted, maxd, log, out);
+}
+
+bool noninsertion_order_sorted(const std::vector<int32_t>& walk_sorted, int maxd, const std::vector<KeyOp>& log,
+                               std::vector<int32_t>& out) {
+    const size_t n_walk = walk_sorted.size();
+    const size_t cap = rh_capacity(n_walk);
+    if (log.empty()) {
+        if (maxd >= RH_OVERFLOW_DISP) return false;
+        out = walk_sorted;
+        return true;
+    }
+    // the realigner's operations: erased walk keys leave the order; inserted keys join it after
+    // every earlier key of equal (home, hash bits)
+    std::unordered_set<int32_t> walk_keys(walk_sorted.begin(), walk_sorted.end());
+    std::unordered_set<int32_t> erased, added_set;
+    std::vector<int32_t> added;  // in insertion order
+    size_t count = n_walk, max_count = n_walk;
+    for (const KeyOp& op : log) {
+        const bool present = (walk_keys.count(op.pos) && !erased.count(op.pos)) || added_set.count(op.pos);
+        if (op.insert) {
+            if (present) continue;
+            if (walk_keys.count(op.pos)) erased.insert(op.pos);  // re-inserted: now the latest key
+            added_set.insert(op.pos);
+            added.erase(std::remove(added.begin(), added.end(), op.pos), added.end());
+            added.push_back(op.pos);
+            ++count;
+            if (count > max_count) max_count = count;
+        } else {
+            if (!present) continue;
+            if (added_set.count(op.pos)) {
+                added_set.erase(op.pos);
+                added.erase(std::remove(added.begin(), added.end(), op.pos), added.end());
+            } else {
+                erased.insert(op.pos);
+            }
+            --count;
+        }
+    }
+    if (rh_capacity(max_count) != cap) return false;
+    // the walk's packing already holds every walk key; each added key lengthens a cluster by at
+    // most one slot
+    if (maxd + (int)added.size() >= RH_OVERFLOW_DISP) return false;
+    const uint64_t mask = (uint64_t)cap - 1;
+    auto rank = [&](int32_t k) {  // (home, hash bits desc)
+        const uint64_t h = hash_int(k);
+        return ((h & mask) << 5) | (31 - (h >> 59));
+    };
+    std::vector<std::pair<uint64_t, int32_t>> extra;
+    for (int32_t k : added) extra.push_back({rank(k), k});
+    std::stable_sort(extra.begin(), extra.end(),
+                     [](const std::pair<uint64_t, int32_t>& x, const std::pair<uint64_t, int32_t>& y) { return x.first < y.first; });
+    out.clear();
+    out.reserve(n_walk + extra.size());
+    size_t e = 0;
+    for (size_t i = 0; i < n_walk; i++) {
+        const int32_t k = walk_sorted[i];
+        const uint64_t r = rank(k);
+        while (e < extra.size() && extra[e].first < r) out.push_back(extra[e++].second);
+        if (!erased.count(k)) out.push_back(k);
+    }
+    while (e < extra.size()) out.push_back(extra[e++].second);
+    return true;
 }
 
 }  // namespace rv

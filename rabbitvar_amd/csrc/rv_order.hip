@@ -33,6 +33,31 @@ hipError_t launch_key_list(const DenseRow* rows, int dense_n, int dense_lo, unsi
     return hipGetLastError();
 }
 
+// robin_hood slot-order sort key of each key, given its insertion index i: home slot (low hash
+// bits), then the info byte's hash bits descending, then insertion index
+__global__ void __launch_bounds__(256) k_slot_keys(const int32_t* keys, int n, int cap_log2, int idx_bits,
+                                                     unsigned long long* comp, int32_t* val) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t k = keys[i];
+    const unsigned long long x = (unsigned long long)(long long)k;
+    const unsigned long long hi = __umul64hi(x, 0xde5fb9d2630458e9ull);
+    const unsigned long long lo = x * 0xde5fb9d2630458e9ull;
+    const unsigned long long h = hi + lo;
+    const unsigned long long home = h & ((1ull << cap_log2) - 1ull);
+    const unsigned long long hb = h >> 59;
+    comp[i] = (((home << 5) | (31ull - hb)) << idx_bits) | (unsigned long long)i;
+    val[i] = k;
+}
+
+hipError_t launch_slot_keys(const int32_t* keys, int n, int cap_log2, int idx_bits, unsigned long long* comp,
+                            int32_t* val, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_slot_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, keys, n, cap_log2, idx_bits, comp,
+                       val);
+    return hipGetLastError();
+}
+
 hipError_t sort_key_pairs(void* temp, size_t& temp_bytes, const unsigned long long* kin, unsigned long long* kout,
                           const int32_t* vin, int32_t* vout, int n, int end_bit, hipStream_t s) {
     return hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, kin, kout, vin, vout, n, 0, end_bit, s);

@@ -44,8 +44,11 @@ bool RowCache::load(std::vector<int>& rows) {
     if (!src_->fetch(rows, buf)) return false;
     fetches++;
     rows_fetched += (int64_t)rows.size();
-    rows_.reserve(rows_.size() + rows.size());
-    for (size_t k = 0; k < rows.size(); k++) rows_.emplace(rows[k], buf[k]);
+    if (slot_.empty()) slot_.assign((size_t)n_, -1);
+    for (size_t k = 0; k < rows.size(); k++) {
+        slot_[(size_t)rows[k]] = (int32_t)store_.size();
+        store_.push_back(buf[k]);
+    }
     return true;
 }
 
@@ -54,7 +57,7 @@ bool RowCache::prefetch(const std::vector<int>& positions, int radius) {
     for (int p : positions) {
         int a = std::max(0, p - radius - lo_), b = std::min(n_ - 1, p + radius - lo_);
         for (int i = a; i <= b; i++)
-            if (!rows_.count(i)) want.push_back(i);
+            if (!have(i)) want.push_back(i);
     }
     return load(want);
 }
@@ -62,15 +65,13 @@ bool RowCache::prefetch(const std::vector<int>& positions, int radius) {
 const DenseRow* RowCache::row(int pos) {
     int i = pos - lo_;
     if (i < 0 || i >= n_) return nullptr;
-    auto it = rows_.find(i);
-    if (it == rows_.end()) {
+    if (!have(i)) {
         std::vector<int> want;
         for (int k = std::max(0, i - MISS_RADIUS); k <= std::min(n_ - 1, i + MISS_RADIUS); k++)
-            if (!rows_.count(k)) want.push_back(k);
+            if (!have(k)) want.push_back(k);
         if (!load(want)) throw std::runtime_error("dense row fetch failed");
-        it = rows_.find(i);
     }
-    return &it->second;
+    return &store_[(size_t)slot_[(size_t)i]];
 }
 
 // ------------------------------------------------------------------ reference
@@ -308,23 +309,38 @@ std::string findconseq(HSclip& sc) {
     int total = 0, match = 0;
     std::string seqq;
     bool flag = false;
+    auto si = sc.seq.begin();
     for (auto& e : sc.nt) {
         int idx = e.first;
-        std::vector<char> firsts;
-        for (auto& pr : e.second) firsts.push_back(pr.first);
+        char firsts[8], order[8];
+        std::vector<char> big;
+        int nk = (int)e.second.size(), no;
+        const char* ord;
+        if (nk <= 7) {
+            for (int k = 0; k < nk; k++) firsts[k] = e.second[(size_t)k].first;
+            no = rh_char_order_small(firsts, nk, order);
+            ord = order;
+        } else {
+            for (auto& pr : e.second) big.push_back(pr.first);
+            big = rh_char_iteration_order(big);
+            no = (int)big.size();
+            ord = big.data();
+        }
         int maxCount = 0;
         double maxQuality = 0;
         char chosen = 0;
         int tt = 0;
-        auto si = sc.seq.find(idx);
-        for (char base : rh_char_iteration_order(firsts)) {
+        while (si != sc.seq.end() && si->first < idx) ++si;
+        const bool seq_here = si != sc.seq.end() && si->first == idx;
+        for (int oi = 0; oi < no; oi++) {
+            const char base = ord[oi];
             int count = 0;
             for (auto& pr : e.second)
                 if (pr.first == base) count = pr.second;
             tt += count;
             double bq = 0;
             bool has = false;
-            if (si != sc.seq.end()) {
+            if (seq_here) {
                 auto bj = si->second.find(base);
                 if (bj != si->second.end()) {
                     has = true;
@@ -334,7 +350,7 @@ std::string findconseq(HSclip& sc) {
             if (count > maxCount || (has && bq > maxQuality)) {
                 maxCount = count;
                 chosen = base;
-                maxQuality = sc.seq[idx][base].qmean;
+                maxQuality = has ? bq : sc.seq[idx][base].qmean;
             }
         }
         if (idx == 3 && sc.nt.size() >= 6 && tt / (double)sc.v.cnt < 0.2 && tt <= 2) break;
@@ -438,12 +454,18 @@ struct Realigner {
 
     int maxlen() const { return S.P.maxReadLength; }
 
-    bool ismatch(const std::string& s1, std::string s2, int dir, int MMx = 3) {
-        s2 = strip_marks(s2);
+    bool ismatch(const std::string& s1, const std::string& s2_in, int dir, int MMx = 3) {
+        const bool marked = s2_in.find_first_of("#^") != std::string::npos;
+        std::string stripped;
+        if (marked) stripped = strip_marks(s2_in);
+        const std::string& s2 = marked ? stripped : s2_in;
         int mm = 0;
-        for (size_t n = 0; n < s1.size() && n < s2.size(); n++) {
-            int b = dir * (int)n - (dir == -1 ? 1 : 0);
-            if (s1[n] != sub2(s2, b, 1)[0]) mm++;
+        const int l2 = (int)s2.size();
+        for (int n = 0; n < (int)s1.size() && n < l2; n++) {
+            // vc_substr(s2, b, 1)[0] with b = n (forward) or -n-1 (from the end)
+            const int b = dir * n - (dir == -1 ? 1 : 0);
+            const int at = b < 0 ? l2 + b : b;
+            if (s1[(size_t)n] != s2[(size_t)at]) mm++;
         }
         return mm <= MMx && mm / (double)s1.size() < 0.15;
     }
@@ -953,8 +975,10 @@ struct Realigner {
         for (size_t i = 0; i < seq5.size() - 8; i++) {
             for (size_t j = 1; j < seq3.size() - 8; j++) {
                 int nm = 0, n = 0;
-                while (n + (int)j <= (int)seq3.size() && (int)i + n <= (int)seq5.size()) {
-                    if (sub2(seq3, -(int)j - n, 1) != sub2(seq5, (int)i + n, 1)) nm++;
+                const int l3 = (int)seq3.size(), l5 = (int)seq5.size();
+                while (n + (int)j <= l3 && (int)i + n <= l5) {
+                    // vc_substr(seq3, -j-n, 1) vs vc_substr(seq5, i+n, 1); the latter is empty at the end
+                    if ((int)i + n >= l5 || seq3[(size_t)(l3 - (int)j - n)] != seq5[(size_t)((int)i + n)]) nm++;
                     if (nm > 2) break;
                     n++;
                 }

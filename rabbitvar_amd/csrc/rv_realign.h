@@ -52,9 +52,11 @@ class RowCache {
 
   private:
     bool load(std::vector<int>& rows);
+    bool have(int i) const { return !slot_.empty() && slot_[(size_t)i] >= 0; }
     RowSource* src_;
     int lo_, n_;
-    std::unordered_map<int, DenseRow> rows_;  // keyed by row index
+    std::vector<int32_t> slot_;  // row index -> position in store_, -1 = not fetched
+    std::deque<DenseRow> store_;  // stable addresses
 };
 
 // The reference window as the reference's map: keys [lo, hi]; get() of a missing key creates it

@@ -36,20 +36,35 @@ class DeviceRowSource : public RowSource {
     } while (0)
 
 rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, const rv_contig& contig, CallSet& out) {
+    const auto w0 = std::chrono::steady_clock::now();
+    auto ms_since = [](std::chrono::steady_clock::time_point t) {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+    };
     PileupResult pr;
     rv_status s = e.pileup(region, reads, contig, false, pr);
     if (s != RV_OK) return s;
     // ---- nonInsertionVariants keys in creation order; their slot-order replay runs on a second
     // host thread while this one realigns
-    std::vector<int32_t> created;
-    s = e.key_order((int)std::min<int64_t>(reads.n_reads, INT32_MAX), created);
+    std::vector<int32_t> created, walk_sorted;
+    const auto k0 = std::chrono::steady_clock::now();
+    s = e.key_order((int)std::min<int64_t>(reads.n_reads, INT32_MAX), created, walk_sorted);
     if (s != RV_OK) return s;
+    double order_ms = ms_since(k0);
     auto h0 = std::chrono::steady_clock::now();
+    // a second host thread checks whether the sorted slot order provably holds and otherwise
+    // replays the walk's insertions, while this one realigns
     RHOrder nonins;
+    int walk_maxd = RH_OVERFLOW_DISP;
+    bool replayed = false;
     std::exception_ptr replay_err;
     std::thread replay([&] {
         try {
-            replay_created(nonins, created);
+            if (walk_sorted.size() == created.size())
+                walk_maxd = rh_max_displacement(walk_sorted.data(), walk_sorted.size(), rh_capacity(walk_sorted.size()));
+            if (walk_maxd >= RH_OVERFLOW_DISP) {
+                replay_created(nonins, created);
+                replayed = true;
+            }
         } catch (...) {
             replay_err = std::current_exception();
         }
@@ -107,6 +122,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
         return e.fail(RV_ERR_DEVICE, std::string("record list: ") + ex.what());
     }
     double host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
+    const double realign_ms = host_ms;
     // ---- scoring on the device
     const int ncx = (int)cx.pos.size();
     ScoreIn in{};
@@ -190,8 +206,14 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     // ---- text, in the printer's order: alignedVariants slot order, then the variants' order
     auto t0 = std::chrono::steady_clock::now();
     replay.join();
+    const auto o1 = std::chrono::steady_clock::now();
     if (replay_err) return e.fail(RV_ERR_DEVICE, "key order replay failed");
-    std::vector<int32_t> proc = replay_finish(nonins, S.key_log);
+    std::vector<int32_t> proc;
+    if (replayed || !noninsertion_order_sorted(walk_sorted, walk_maxd, S.key_log, proc)) {
+        if (!replayed) replay_created(nonins, created);
+        proc = replay_finish(nonins, S.key_log);
+    }
+    e.stats.order_sorted = replayed ? 0 : 1;
     std::vector<int32_t> aligned = aligned_order(proc, [&](int32_t p) -> int {
         auto it = std::lower_bound(cx.pos.begin(), cx.pos.end(), p);
         if (it != cx.pos.end() && *it == p) return status[(size_t)pr.P.dense_n + (size_t)(it - cx.pos.begin())];
@@ -213,24 +235,46 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
         if (a.position != b.position) return a.position < b.position;
         return a.order < b.order;
     });
+    order_ms += ms_since(t0);
+    const auto f0 = std::chrono::steady_clock::now();
     LineContext ctx{e.sample.c_str(), region.gene ? region.gene : "", region.chr ? region.chr : "", region.start, region.end,
                     e.cfg.fisher != 0};
     out.text.clear();
     out.calls.clear();
     out.strings.clear();
-    out.strings.reserve(recs.size());
+    out.strings.resize(recs.size());
     out.calls.reserve(recs.size());
-    for (const OutRec& o : recs) {
-        format_line(out.text, o, arena.data(), ctx);
-        CallStrings cs;
-        const char* p = arena.data() + o.str_off;
-        std::string* dst[6] = {&cs.refallele, &cs.varallele, &cs.genotype, &cs.bias, &cs.leftseq, &cs.rightseq};
-        for (int k = 0; k < 6; k++) {
-            dst[k]->assign(p, o.len[k]);
-            p += o.len[k];
-        }
-        cs.vartype = vartype_name(o, arena.data());
-        out.strings.push_back(std::move(cs));
+    {
+        // lines are independent: format contiguous chunks on several threads, join in order
+        const size_t n = recs.size();
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const size_t nt = n < 4096 ? 1 : std::min<size_t>(std::min(hw, 16u), n / 2048);
+        std::vector<std::string> parts(nt);
+        auto work = [&](size_t t) {
+            const size_t a = n * t / nt, b = n * (t + 1) / nt;
+            std::string& txt = parts[t];
+            txt.reserve((b - a) * 200);
+            for (size_t i = a; i < b; i++) {
+                const OutRec& o = recs[i];
+                format_line(txt, o, arena.data(), ctx);
+                CallStrings& cs = out.strings[i];
+                const char* p = arena.data() + o.str_off;
+                std::string* dst[6] = {&cs.refallele, &cs.varallele, &cs.genotype, &cs.bias, &cs.leftseq, &cs.rightseq};
+                for (int k = 0; k < 6; k++) {
+                    dst[k]->assign(p, o.len[k]);
+                    p += o.len[k];
+                }
+                cs.vartype = vartype_name(o, arena.data());
+            }
+        };
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < nt; t++) pool.emplace_back(work, t);
+        if (nt > 0) work(0);
+        for (auto& th : pool) th.join();
+        size_t total = 0;
+        for (auto& p : parts) total += p.size();
+        out.text.reserve(total);
+        for (auto& p : parts) out.text += p;
     }
     for (size_t i = 0; i < recs.size(); i++) {
         const OutRec& o = recs[i];
@@ -272,6 +316,11 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
         out.calls.push_back(c);
     }
     host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    e.stats.realign_ms = realign_ms;
+    e.stats.order_ms = order_ms;
+    e.stats.format_ms = ms_since(f0);
+    e.stats.wall_ms = ms_since(w0);
+    (void)o1;
     e.stats.score_ms = score_ms;
     e.stats.host_ms += host_ms;
     e.stats.total_device_ms = e.stats.prep_ms + e.stats.walk_ms + e.stats.anchor_ms + score_ms;

@@ -54,6 +54,35 @@ int main(int argc, char** argv) {
         m.for_each([](int32_t k) { printf("%d\n", k); });
         return 0;
     }
+    if (argc >= 4 && std::string(argv[1]) == "--order-check") {
+        // random walk keys + a random realigner log: the sorted form, when it claims to apply,
+        // must equal the replay.  Prints "applied equal" counts.
+        const int n = atoi(argv[2]);
+        unsigned seed = (unsigned)atoi(argv[3]);
+        auto rnd = [&]() {
+            seed = seed * 1664525u + 1013904223u;
+            return seed >> 8;
+        };
+        int applied = 0, equal = 0;
+        for (int trial = 0; trial < 20; trial++) {
+            std::vector<int32_t> created;
+            for (int i = 0; i < n; i++) created.push_back((int32_t)(i * 3 + (int)(rnd() % 3)));
+            std::vector<KeyOp> log;
+            const int nl = (int)(rnd() % 6);
+            for (int i = 0; i < nl; i++) {
+                const bool ins = rnd() % 2;
+                const int32_t k = rnd() % 3 ? created[rnd() % created.size()] : (int32_t)(n * 3 + (int)(rnd() % 1000));
+                log.push_back({k, ins});
+            }
+            std::vector<int32_t> want = noninsertion_order(created, log), got;
+            if (noninsertion_order_sorted(rh_sort_keys(created, rh_capacity(created.size())), log, got)) {
+                applied++;
+                if (got == want) equal++;
+            }
+        }
+        printf("%d %d\n", applied, equal);
+        return 0;
+    }
     if (argc < 8) return 2;
     std::string fa = argv[1], rvb = argv[2];
     int rs = atoi(argv[3]), re = atoi(argv[4]), k = atoi(argv[5]), fisher = atoi(argv[6]);
@@ -209,12 +238,36 @@ int main(int argc, char** argv) {
     out.resize(std::min<unsigned>(nout, (unsigned)out.size()));
     // the printer's order: alignedVariants slot order, then the variants' order within a position
     std::vector<int32_t> proc = noninsertion_order(created, RS.key_log);
-    std::vector<int32_t> aligned = aligned_order(proc, [&](int32_t p) -> int {
+    {
+        // the sorted form must agree with the replay whenever it claims to apply
+        std::vector<int32_t> sorted_walk = rh_sort_keys(created, rh_capacity(created.size())), proc2;
+        const bool ok = noninsertion_order_sorted(sorted_walk, RS.key_log, proc2);
+        fprintf(stderr, "order: keys %zu log %zu sorted-form %s\n", created.size(), RS.key_log.size(),
+                ok ? (proc2 == proc ? "applies, equal" : "applies, DIFFERS") : "not provable");
+        if (ok && proc2 != proc) return 4;
+    }
+    auto status_of = [&](int32_t p) -> int {
         auto it = std::lower_bound(cx.pos.begin(), cx.pos.end(), p);
         if (it != cx.pos.end() && *it == p) return status[(size_t)W.dense_n + (size_t)(it - cx.pos.begin())];
         const int i = p - W.dense_lo;
         return (i >= 0 && i < W.dense_n) ? status[(size_t)i] : 0;
-    });
+    };
+    std::vector<int32_t> aligned = aligned_order(proc, status_of);
+    {
+        RHOrder m;
+        for (int32_t p : proc) {
+            const int st = status_of(p);
+            if (st == 0) continue;
+            m.insert(p);
+            if (st == 2) m.erase(p);
+        }
+        std::vector<int32_t> ref;
+        m.for_each([&](int32_t k) { ref.push_back(k); });
+        if (ref != aligned) {
+            fprintf(stderr, "order: alignedVariants sorted form DIFFERS from the replay\n");
+            return 4;
+        }
+    }
     std::unordered_map<int32_t, int32_t> rank;
     for (size_t i = 0; i < aligned.size(); i++) rank[aligned[i]] = (int32_t)i;
     std::sort(out.begin(), out.end(), [&](const OutRec& a, const OutRec& b) {
@@ -227,6 +280,8 @@ int main(int argc, char** argv) {
     for (auto& r : out) format_line(text, r, oarena.data(), ctx);
     std::ofstream o(outp);
     o << text;
+    fprintf(stderr, "realign ms: prefetch %.1f mnp %.1f del %.1f ins %.1f lgins %.1f\n", rst.ms_prefetch, rst.ms_mnp,
+            rst.ms_del, rst.ms_ins, rst.ms_lgins);
     fprintf(stderr, "reads %lld events %u clips %u walk_errors %llu overflow %llu capacity %llu\n", (long long)m, ne, ncl,
             (unsigned long long)ctr.walk_errors, (unsigned long long)ctr.overflow, sctr[0]);
     return (ctr.walk_errors || ctr.overflow || sctr[0] || nout > 200000) ? 3 : 0;

@@ -97,3 +97,12 @@ def test_int_map_order_matches_oracle_clone(sim, tools, seed):
     g = subprocess.run([sim, "--rh-ints"], input=text, capture_output=True, text=True, check=True)
     got = [int(x) for x in g.stdout.split()]
     assert got == want
+
+
+@pytest.mark.parametrize("n,seed", [(3400, 1), (27000, 2), (1700000, 5)])
+def test_sorted_slot_order_with_realigner_log(sim, n, seed):
+    """The sorted form of nonInsertionVariants' slot order, with random realigner insertions and
+    erasures, against the full replay: whenever it claims to apply it must be identical."""
+    r = subprocess.run([sim, "--order-check", str(n), str(seed)], capture_output=True, text=True, check=True)
+    applied, equal = (int(x) for x in r.stdout.split())
+    assert applied == equal and applied > 0
