@@ -214,21 +214,25 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
         proc = replay_finish(nonins, S.key_log);
     }
     e.stats.order_sorted = replayed ? 0 : 1;
+    // one status per window position: record-list positions override their dense rows
+    for (size_t k = 0; k < cx.pos.size(); k++) {
+        const int i = cx.pos[k] - pr.P.dense_lo;
+        if (i >= 0 && i < pr.P.dense_n) status[(size_t)i] = status[(size_t)pr.P.dense_n + k];
+    }
     std::vector<int32_t> aligned = aligned_order(proc, [&](int32_t p) -> int {
-        auto it = std::lower_bound(cx.pos.begin(), cx.pos.end(), p);
-        if (it != cx.pos.end() && *it == p) return status[(size_t)pr.P.dense_n + (size_t)(it - cx.pos.begin())];
         const int i = p - pr.P.dense_lo;
         return (i >= 0 && i < pr.P.dense_n) ? status[(size_t)i] : 0;
     });
     {
-        std::unordered_map<int32_t, int32_t> rank;
-        rank.reserve(recs.size() * 2 + 1);
-        for (const OutRec& o : recs) rank.emplace(o.position, INT32_MAX);
+        std::vector<int32_t> rank((size_t)std::max(pr.P.dense_n, 1), INT32_MAX);
         for (size_t i = 0; i < aligned.size(); i++) {
-            auto it = rank.find(aligned[i]);
-            if (it != rank.end()) it->second = (int32_t)i;
+            const int k = aligned[i] - pr.P.dense_lo;
+            if (k >= 0 && k < pr.P.dense_n) rank[(size_t)k] = (int32_t)i;
         }
-        for (OutRec& o : recs) o.start_rank = rank[o.position];
+        for (OutRec& o : recs) {
+            const int k = o.position - pr.P.dense_lo;
+            o.start_rank = (k >= 0 && k < pr.P.dense_n) ? rank[(size_t)k] : INT32_MAX;
+        }
     }
     std::sort(recs.begin(), recs.end(), [](const OutRec& a, const OutRec& b) {
         if (a.start_rank != b.start_rank) return a.start_rank < b.start_rank;
