@@ -18,10 +18,28 @@ __global__ void __launch_bounds__(256) k_fill_rows(DenseRow* rows, int n) {
     if (i < n) fill_row_one(rows, i);
 }
 
-__global__ void __launch_bounds__(256) k_walk(ReadsDev R, Window W, Config cfg, const PrepRec* prep, const uint32_t* cig,
-                                                WalkSink sink) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < R.n) walk_one(r, R, W, cfg, prep, cig, sink);
+// One workgroup walks WALK_READS consecutive reads (sorted by position), one wavefront per read at a
+// time, into an LDS tile that starts TILE_MARGIN bases before the first read; the tile is added to
+// the dense rows once at the end.
+constexpr int WALK_THREADS = 1024;
+constexpr int WALK_READS = 64;
+constexpr int TILE_MARGIN = 64;
+
+__global__ void __launch_bounds__(WALK_THREADS) k_walk(ReadsDev R, Window W, Config cfg, const PrepRec* prep,
+                                                         const uint32_t* cig, WalkSink sink) {
+    __shared__ TileAcc tile;
+    const int waves = (int)(blockDim.x / warpSize);
+    const int wave = (int)(threadIdx.x / warpSize), lane = (int)(threadIdx.x % warpSize);
+    const int64_t r0 = (int64_t)blockIdx.x * WALK_READS;
+    const int tile_lo = (r0 < R.n ? prep[r0].position : 0) - TILE_MARGIN;
+    tile_init(&tile, (int)threadIdx.x, (int)blockDim.x);
+    __syncthreads();
+    for (int k = wave; k < WALK_READS; k += waves) {
+        const int64_t r = r0 + k;
+        if (r < R.n) walk_one(r, R, W, cfg, prep, cig, sink, lane, &tile, tile_lo);
+    }
+    __syncthreads();
+    tile_flush(&tile, tile_lo, sink.dense, sink.dense_lo, sink.dense_n, (int)threadIdx.x, (int)blockDim.x);
 }
 
 __global__ void __launch_bounds__(256) k_anchor(const Anchor* A, const unsigned* n_anchors, unsigned cap, DenseRow* rows,
@@ -64,7 +82,7 @@ hipError_t launch_fill_rows(DenseRow* rows, int n, hipStream_t s) {
 hipError_t launch_walk(const ReadsDev& R, const Window& W, const Config& cfg, const PrepRec* prep, const uint32_t* cig,
                        const WalkSink& sink, hipStream_t s) {
     if (R.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_walk, dim3(grid_for(R.n, 256)), dim3(256), 0, s, R, W, cfg, prep, cig, sink);
+    hipLaunchKernelGGL(k_walk, dim3(grid_for(R.n, WALK_READS)), dim3(WALK_THREADS), 0, s, R, W, cfg, prep, cig, sink);
     return hipGetLastError();
 }
 hipError_t launch_anchor(const Anchor* a, const unsigned* n_anchors, unsigned cap, DenseRow* rows, int dense_lo,

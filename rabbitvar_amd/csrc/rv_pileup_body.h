@@ -185,11 +185,18 @@ __device__ inline void fill_row_one(DenseRow* rows, int i) {
     rows[i] = z;
 }
 
+// One read, walked by a whole wavefront (lane = this thread's lane; see rv_walk.h).  Contributions
+// to positions [tile_lo, tile_lo + TILE_POS) go to the workgroup's tile.
 __device__ inline void walk_one(int64_t r, const ReadsDev& R, const Window& W, const Config& cfg, const PrepRec* prep,
-                                const uint32_t* cig, WalkSink& sink) {
+                                const uint32_t* cig, WalkSink& sink, int lane, TileAcc* tile, int tile_lo) {
     const PrepRec p = prep[r];
     if (p.status != RD_PARSE) return;
     Walker w;
+    w.lane = lane;
+    w.leader = lane == 0;
+    w.tile = tile;
+    w.tile_lo = tile_lo;
+    w.ref.leader = lane == 0;
     w.c = &cfg;
     w.Rs = W.region_start;
     w.Re = W.region_end;
@@ -230,6 +237,57 @@ __device__ inline void walk_one(int64_t r, const ReadsDev& R, const Window& W, c
         if (o == OP_M || o == OP_D || o == OP_N || o == OP_EQ || o == OP_X) reflen += clen(R.cigar[co + i]);
     }
     w.walk(p.rlen1, p.total_len, R.flag[r], R.tid[r], R.mtid[r], R.mpos0[r] + 1, reflen, R.pos0[r]);
+}
+
+// Tile set-up and flush, strided over the workgroup's threads.
+__device__ inline void tile_init(TileAcc* t, int tid, int nthreads) {
+    int32_t* f = &t->f[0][0][0];
+    for (int k = tid; k < TF_N * 4 * TILE_POS; k += nthreads) f[k] = (k / (4 * TILE_POS) == TF_MINORD) ? -1 : 0;
+    for (int k = tid; k < TILE_POS; k += nthreads) {
+        t->cov[k] = 0;
+        t->rowflags[k] = 0;
+        t->first_touch[k] = ~0ull;
+    }
+}
+
+// Adds the tile into the dense rows.  For the pstd/qstd state: the tile's own "differs" flags
+// cover pairs inside the workgroup, and comparing the tile's first (position, quality) value with
+// the row's covers pairs across workgroups (all values equal within each side, so the two sides
+// differ exactly when their first values do).
+__device__ inline void tile_flush(const TileAcc* t, int tile_lo, DenseRow* rows, int dense_lo, int dense_n, int tid,
+                                  int nthreads) {
+    for (int k = tid; k < 4 * TILE_POS; k += nthreads) {
+        const int b = k / TILE_POS, i = k - b * TILE_POS;
+        const int ri = tile_lo + i - dense_lo;
+        if (ri < 0 || ri >= dense_n) continue;
+        DenseRow& r = rows[ri];
+        if (b == 0) {
+            if (t->cov[i]) atomicAdd(&r.cov, t->cov[i]);
+            if (t->rowflags[i]) atomicOr(&r.rowflags, t->rowflags[i]);
+            if (t->first_touch[i] != ~0ull) atomicMin(&r.first_touch, t->first_touch[i]);
+        }
+        const unsigned fl = (unsigned)t->f[TF_FLAGS][b][i];
+        if (fl == 0) continue;  // no contribution to this base
+        DenseBase& d = r.b[b];
+        const int32_t* v[9] = {&t->f[TF_CNT][b][i], &t->f[TF_FWD][b][i], &t->f[TF_REV][b][i], &t->f[TF_HI][b][i],
+                               &t->f[TF_LO][b][i],  &t->f[TF_TP][b][i],  &t->f[TF_Q][b][i],   &t->f[TF_MQ][b][i],
+                               &t->f[TF_NM][b][i]};
+        int32_t* g[9] = {&d.cnt, &d.fwd, &d.rev, &d.hi, &d.lo, &d.sum_tp, &d.sum_q, &d.sum_mq, &d.sum_nm};
+        for (int f = 0; f < 9; f++)
+            if (*v[f]) atomicAdd(g[f], *v[f]);
+        atomicMin(&d.min_ord, (unsigned)t->f[TF_MINORD][b][i]);
+        unsigned add = fl;
+        const unsigned first = (unsigned)t->f[TF_FIRST][b][i];
+        if (first != 0u) {
+            const unsigned old = atomicCAS(&d.first_tpq, 0u, first);
+            if (old != 0u && old != first) {
+                const unsigned o = old - 1u, nv = first - 1u;
+                if ((o >> 8) != (nv >> 8)) add |= 2u;
+                if ((o & 0xffu) != (nv & 0xffu)) add |= 4u;
+            }
+        }
+        if ((d.flags & add) != add) atomicOr(&d.flags, add);
+    }
 }
 
 // Insertion-anchor subtraction (parseCigar.cpp:1471-1490): the reference subtracts from the anchor

@@ -11,6 +11,13 @@
 //   * soft clips that reach Sclip accumulation -> ClipRec + clip bases/qualities;
 //   * the insertion-anchor subtraction (parseCigar.cpp:1471-1490) -> Anchor records, applied after
 //     the walk once every entry's creating read ordinal is known.
+//
+// One wavefront walks one read.  The read's control flow is lane-uniform: every lane keeps the
+// same walker state and runs the same steps, and only lane 0 (the leader) issues the side effects.
+// Inside an M run the plain bases (neither the start of a multi-base allele, nor the last bases
+// before an adjacent indel, nor non-ACGT) are spread over the lanes, one base per lane, so a
+// wavefront updates 64 consecutive positions at once; every other base goes through the
+// sequential steps.
 #pragma once
 #include "rv_cigar.h"
 
@@ -18,6 +25,20 @@ namespace rv {
 
 constexpr int DCAP = 640;  // allele descriptor capacity
 typedef FStr<DCAP> DStr;
+
+// Workgroup-local accumulator for a window of TILE_POS consecutive positions (LDS).  The reads of
+// one workgroup add their single-base contributions here with LDS atomics; the tile is then added
+// to the dense rows once per touched (position, base).  Every field merges exactly: counts and sums
+// add, creation ordinals take the minimum, flags OR, and the first (position, quality) value is
+// compared with the rows' own first value (see tile_flush).
+constexpr int TILE_POS = 512;
+enum : int { TF_CNT, TF_FWD, TF_REV, TF_HI, TF_LO, TF_TP, TF_Q, TF_MQ, TF_NM, TF_MINORD, TF_FLAGS, TF_FIRST, TF_N };
+struct TileAcc {
+    int32_t f[TF_N][4][TILE_POS];  // field, base, position
+    int32_t cov[TILE_POS];
+    uint32_t rowflags[TILE_POS];
+    unsigned long long first_touch[TILE_POS];
+};
 
 struct Anchor {
     int32_t pos;
@@ -55,12 +76,13 @@ struct RefWin {
     const char* ref;
     int lo, hi;
     WalkCounters* ctr;
+    bool leader;
     __device__ bool has(int k) const { return k >= lo && k <= hi; }
     __device__ char at(int k) const { return ref[k - lo]; }
     // robin_hood operator[]: a missing key would be created with '\0'
     __device__ char get(int k) const {
         if (has(k)) return at(k);
-        atomicAdd(&ctr->ref_insert, 1ull);
+        if (leader) atomicAdd(&ctr->ref_insert, 1ull);
         return '\0';
     }
     __device__ bool hasEq(int k, char ch) const { return has(k) && at(k) == ch; }
@@ -90,6 +112,10 @@ struct Walker {
     int position;
     int start, offset, cel, rpi, rpe;
     bool bad;
+    int lane;      // lane within the wavefront walking this read
+    bool leader;   // lane 0: issues the side effects of the lane-uniform steps
+    TileAcc* tile; // workgroup accumulator (nullptr: dense rows directly)
+    int tile_lo;   // position of tile slot 0
 
     __device__ uint32_t cw(int i) const { return (i >= 0 && i < rec_n) ? rec[i] : 0u; }
     __device__ int op(int i) const { return cop(cw(i)); }
@@ -120,21 +146,74 @@ struct Walker {
     // a getVariation on nonInsertionVariants at this row: the key exists from here on, and the
     // smallest (read, touch) pair records when the reference created it (its hash-map insertion
     // order, replayed on the host)
-    __device__ void touch_pos(DenseRow* r) {
+    __device__ int tile_slot(int pos) const {
+        const int i = pos - tile_lo;
+        return (tile && i >= 0 && i < TILE_POS) ? i : -1;
+    }
+    __device__ void touch_key(DenseRow* r, int pos, unsigned t) {
+        const unsigned long long key = ((unsigned long long)(unsigned)ord << 24) | (t & 0xffffffu);
+        const int ti = tile_slot(pos);
+        if (ti >= 0) {
+            if (!(tile->rowflags[ti] & 1u)) atomicOr(&tile->rowflags[ti], 1u);
+            if (key < tile->first_touch[ti]) atomicMin(&tile->first_touch[ti], key);
+            return;
+        }
         if (!(r->rowflags & 1u)) atomicOr(&r->rowflags, 1u);
-        const unsigned long long key = ((unsigned long long)(unsigned)ord << 24) | (tseq++ & 0xffffffu);
         if (key < r->first_touch) atomicMin(&r->first_touch, key);
+    }
+    __device__ void touch_pos(DenseRow* r, int pos) {
+        const unsigned t = tseq++;
+        if (leader) touch_key(r, pos, t);
+    }
+    __device__ void add_cov(DenseRow* r, int pos, int k) {
+        const int ti = tile_slot(pos);
+        if (ti >= 0)
+            atomicAdd(&tile->cov[ti], k);
+        else
+            atomicAdd(&r->cov, k);
     }
     __device__ void cov(int pos, int k = 1) {
         DenseRow* r = row(pos);
-        if (r) atomicAdd(&r->cov, k);
+        if (r && leader) add_cov(r, pos, k);
     }
 
     // single-base contribution from the match run (updates pp/pq)
     __device__ void dense_single(int pos, int b, int tp, int qv, int nmv) {
         DenseRow* r = row(pos);
         if (!r) return;
-        touch_pos(r);
+        touch_pos(r, pos);
+        if (leader) dense_add(r, pos, b, tp, qv, nmv);
+    }
+    // the pp/pq-updating contribution (first (tp, q) value + differs flags)
+    __device__ void dense_add(DenseRow* r, int pos, int b, int tp, int qv, int nmv) {
+        const int ti = tile_slot(pos);
+        if (ti >= 0) {
+            auto F = [&](int f) -> int32_t* { return &tile->f[f][b][ti]; };
+            atomicAdd(F(TF_CNT), 1);
+            atomicAdd(F(dir ? TF_REV : TF_FWD), 1);
+            atomicAdd(F((double)qv >= c->goodq ? TF_HI : TF_LO), 1);
+            atomicAdd(F(TF_TP), tp);
+            atomicAdd(F(TF_Q), qv);
+            atomicAdd(F(TF_MQ), (int)mapq);
+            atomicAdd(F(TF_NM), nmv);
+            unsigned* mo = (unsigned*)F(TF_MINORD);
+            if ((unsigned)ord < *mo) atomicMin(mo, (unsigned)ord);
+            unsigned* fl = (unsigned*)F(TF_FLAGS);
+            const unsigned want = 1u | ((tp <= 0 || qv <= 0) ? 8u : 0u);
+            if ((*fl & want) != want) atomicOr(fl, want);
+            if ((*fl & 6u) != 6u) {
+                const unsigned v = ((((unsigned)tp & 0xffffffu) << 8) | ((unsigned)qv & 0xffu)) + 1u;
+                const unsigned old = atomicCAS((unsigned*)F(TF_FIRST), 0u, v);
+                if (old != 0u && old != v) {
+                    const unsigned o = old - 1u, nv = v - 1u;
+                    unsigned add = 0;
+                    if ((o >> 8) != (nv >> 8)) add |= 2u;
+                    if ((o & 0xffu) != (nv & 0xffu)) add |= 4u;
+                    if (add) atomicOr(fl, add);
+                }
+            }
+            return;
+        }
         DenseBase& d = r->b[b];
         atomicAdd(&d.cnt, 1);
         atomicAdd(dir ? &d.rev : &d.fwd, 1);
@@ -162,7 +241,23 @@ struct Walker {
     __device__ void dense_addcnt(int pos, int b, int rp, int qv) {
         DenseRow* r = row(pos);
         if (!r) return;
-        touch_pos(r);
+        touch_pos(r, pos);
+        if (!leader) return;
+        const int ti = tile_slot(pos);
+        if (ti >= 0) {
+            auto F = [&](int f) -> int32_t* { return &tile->f[f][b][ti]; };
+            atomicAdd(F(TF_CNT), 1);
+            atomicAdd(F(dir ? TF_REV : TF_FWD), 1);
+            atomicAdd(F((double)qv >= (double)(float)c->goodq ? TF_HI : TF_LO), 1);
+            atomicAdd(F(TF_TP), rp);
+            atomicAdd(F(TF_Q), qv);
+            atomicAdd(F(TF_MQ), (int)mapq);
+            atomicAdd(F(TF_NM), nm);
+            atomicMin((unsigned*)F(TF_MINORD), (unsigned)ord);
+            unsigned* fl = (unsigned*)F(TF_FLAGS);
+            if (!(*fl & 1u)) atomicOr(fl, 1u);
+            return;
+        }
         DenseBase& d = r->b[b];
         atomicAdd(&d.cnt, 1);
         atomicAdd(dir ? &d.rev : &d.fwd, 1);
@@ -177,9 +272,15 @@ struct Walker {
 
     __device__ void event(int kind, int pos, const DStr& d, int tp, double qv, int nmv, int cntflags, int rp, bool hiq) {
         if (d.overflow) {
-            atomicAdd(&S->ctr->overflow, 1ull);
+            if (leader) atomicAdd(&S->ctr->overflow, 1ull);
             return;
         }
+        const int myseq = evseq++;
+        if (kind == 0 || kind == 2) {
+            DenseRow* r = row(pos);
+            if (r) touch_pos(r, pos);
+        }
+        if (!leader) return;
         unsigned idx = atomicAdd(S->n_events, 1u);
         unsigned long long off = atomicAdd(S->arena_used, (unsigned long long)d.n);
         if (idx >= S->max_events || off + (unsigned long long)d.n > S->arena_cap) {
@@ -190,7 +291,7 @@ struct Walker {
         Event e;
         e.pos = pos;
         e.ord = ord;
-        e.seq = evseq++;
+        e.seq = myseq;
         e.tp = tp;
         e.q = qv;
         e.mapq = mapq;
@@ -204,10 +305,6 @@ struct Walker {
         e.pad[0] = e.pad[1] = 0;
         e.rp = rp;
         S->events[idx] = e;
-        if (kind == 0 || kind == 2) {
-            DenseRow* r = row(pos);
-            if (r) touch_pos(r);
-        }
     }
 
     // a nonInsertionVariants contribution from the match run / deletion handlers
@@ -287,6 +384,8 @@ struct Walker {
     }
 
     __device__ void emit_clip(int side, int pos, int cel_, int nhq, int sumq, int from, int step) {
+        for (int k = 0; k < nhq; k++) (void)q(from + step * k);  // range checks on every lane
+        if (!leader) return;
         unsigned idx = atomicAdd(S->n_clips, 1u);
         unsigned long long off = atomicAdd(S->clip_used, (unsigned long long)(2 * nhq));
         if (idx >= S->max_clips || off + (unsigned long long)(2 * nhq) > S->clip_cap) {
@@ -469,7 +568,9 @@ struct Walker {
             if (inspos > position && ref.hasEq(inspos, sq(index))) {
                 char ab = sq(index);
                 int b = base_index(ab);
-                if (b >= 0) {
+                const int qa = q(index);
+                if (b >= 0 && !leader) {
+                } else if (b >= 0) {
                     unsigned ai = atomicAdd(S->n_anchors, 1u);
                     if (ai < S->max_anchors) {
                         Anchor a;
@@ -478,7 +579,7 @@ struct Walker {
                         a.base = b;
                         a.dir = dir ? 1 : 0;
                         a.tp = tp;
-                        a.q = q(index);
+                        a.q = qa;
                         a.mapq = mapq;
                         a.nmv = nm - nmoff;
                         S->anchors[ai] = a;
@@ -492,7 +593,7 @@ struct Walker {
                     event(3, inspos, d, tp, (double)q(index), nm - nmoff, 0, 0, (double)q(index) >= c->goodq);
                 }
             }
-            if (ci == 1 && (op(0) == OP_S || op(0) == OP_H)) atomicAdd(&S->ctr->ttref, 1);
+            if (ci == 1 && (op(0) == OP_S || op(0) == OP_H) && leader) atomicAdd(&S->ctr->ttref, 1);
         }
         rpi += cel + offset + multoffp;
         rpe += cel + offset + multoffp;
@@ -603,10 +704,73 @@ struct Walker {
         return ci;
     }
 
+    // Bases i, i+1, ... of the current M run that the sequential steps below would each record as
+    // a single-base allele (or skip, for N), one base per lane; returns how many were handled.
+    // A base stops the run when it could start a multi-base allele (a high-quality mismatch with a
+    // next base in the run and the region), when it is a high-quality mismatch within the last
+    // `vext` bases before an adjacent D/I, or when it is not A/C/G/T/N.
+    __device__ int plain_chunk(int i, int ci, int totalLen, int rlen1, int nmoff) {
+        const int span = min((int)warpSize, cel - i);
+        const int k = lane;
+        const bool act = k < span;
+        const int rp = rpi + k, st = start + k, j = i + k, re1 = rpe + k;
+        bool stop = false, isN = false;
+        int qv = 0, b = -1;
+        if (act) {
+            if (rp < 0 || rp >= lseq) {
+                stop = true;
+            } else {
+                const char ch = seq[rp];
+                if (ch == 'N') {
+                    isN = true;
+                } else {
+                    qv = qual[rp];
+                    b = base_index(ch);
+                    if (b < 0) {
+                        stop = true;
+                    } else if (ref.hasNe(st, ch) && (double)qv >= c->goodq) {
+                        if ((st + 1) >= Rs && (st + 1) <= Re && j + 1 < cel && ref.at(st) != 'N') stop = true;
+                        if (c->performLocalRealignment && cel - j <= c->vext && ci + 1 < n_cigar &&
+                            (op(ci + 1) == OP_D || op(ci + 1) == OP_I))
+                            stop = true;
+                    }
+                }
+            }
+        }
+        const unsigned long long stops = __ballot(act && stop);
+        const int f = stops ? (__ffsll(stops) - 1) : span;
+        bool doit = act && k < f && !isN;
+        if (doit && c->trimBasesAfter != 0) doit = !(!dir ? rp > c->trimBasesAfter : totalLen - rp > c->trimBasesAfter);
+        doit = doit && st >= Rs && st <= Re;
+        const int ri = st - S->dense_lo;
+        const bool inwin = ri >= 0 && ri < S->dense_n;
+        if (__ballot(doit && !inwin)) bad = true;  // as row() would flag it
+        const unsigned long long md = __ballot(doit && inwin);
+        if (doit && inwin) {
+            DenseRow* r = &S->dense[ri];
+            const unsigned t = tseq + (unsigned)__popcll(md & ((1ull << k) - 1ull));
+            const int tp = re1 < rlen1 - re1 ? re1 + 1 : rlen1 - re1;
+            touch_key(r, st, t);
+            dense_add(r, st, b, tp, qv, nm - nmoff);
+            add_cov(r, st, 1);
+        }
+        tseq += (unsigned)__popcll(md);
+        return f;
+    }
+
     // one M (or =, X, P ...) CIGAR element; returns the possibly advanced ci
     __device__ int match_run(int ci, int c_op, int rlen1, int totalLen) {
         int nmoff = 0, moffset = 0;
-        for (int i = offset; i < cel; i++) {
+        int i = offset;
+        while (i < cel) {
+            {
+                const int f = plain_chunk(i, ci, totalLen, rlen1, nmoff);
+                i += f;
+                start += f;
+                rpi += f;
+                rpe += f;
+                if (i >= cel) break;
+            }
             bool trim = false;
             if (c->trimBasesAfter != 0) trim = !dir ? rpi > c->trimBasesAfter : totalLen - rpi > c->trimBasesAfter;
             const char ch1 = sq(rpi);
@@ -614,6 +778,7 @@ struct Walker {
                 start++;
                 rpi++;
                 rpe++;
+                i++;
                 continue;
             }
             DStr s;
@@ -763,6 +928,7 @@ struct Walker {
                 rpi++;
                 rpe++;
             }
+            i++;
         }
         if (moffset != 0) {
             offset = moffset;
@@ -822,7 +988,7 @@ struct Walker {
             ci = match_run(ci, c_op, rlen1, totalLen);
             if (start > Re) break;
         }
-        if (bad) atomicAdd(&S->ctr->walk_errors, 1ull);
+        if (bad && leader) atomicAdd(&S->ctr->walk_errors, 1ull);
     }
 };
 

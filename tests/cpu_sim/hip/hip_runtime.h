@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
+#include <algorithm>
 using std::exp;
 using std::log;
 #define __host__
@@ -21,3 +22,9 @@ inline unsigned long long atomicMin(unsigned long long* p, unsigned long long v)
 inline unsigned atomicOr(unsigned* p, unsigned v) { unsigned o = *p; *p |= v; return o; }
 inline unsigned atomicCAS(unsigned* p, unsigned c, unsigned v) { unsigned o = *p; if (o == c) *p = v; return o; }
 inline int atomicMax(int* p, int v) { int o = *p; if (v > o) *p = v; return o; }
+// a wavefront of one lane
+constexpr int warpSize = 1;
+inline unsigned long long __ballot(int pred) { return pred ? 1ull : 0ull; }
+inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
+inline int __ffsll(unsigned long long x) { return __builtin_ffsll((long long)x); }
+using std::min;
