@@ -141,6 +141,9 @@ def main():
                                "50-300 bp, 2% tandem repeats; one region chrS:1-5000000, --fisher",
                    "reads": batch.n, "region_bp": L},
         "stages_ms_per_step": {k: round(v / a.steps, 3) for k, v in stage.items()},
+        "host_detail_ms_last_step": dict(zip(["rl_prefetch", "rl_mnp", "rl_del", "rl_ins", "rl_lgins", "record_list",
+                                              "key_sort", "order_wait", "aligned_order", "rank_sort", "reduce_events",
+                                              "reduce_clips"], [round(x, 2) for x in st["detail_ms"]])),
         "roofline": {"kernel": "k_walk", "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": round(avg_walk_s * 1e3, 4)},

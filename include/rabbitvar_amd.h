@@ -145,6 +145,9 @@ typedef struct rv_stats {
     double format_ms;          /* host: text */
     double wall_ms;            /* rv_run_region, end to end */
     int64_t order_sorted;      /* 1: output order by the slot sort, 0: by the full map replay */
+    double detail_ms[12];      /* finer host split: realign prefetch, mnp, del, ins, lgins, record list,
+                                  key sort, order-thread wait, alignedVariants order, rank+sort,
+                                  event reduction, clip reduction */
 } rv_stats;
 
 void rv_config_default(rv_config* cfg);

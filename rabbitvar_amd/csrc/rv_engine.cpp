@@ -69,7 +69,7 @@ Engine::~Engine() {
                      &b_arena,     &b_clips,   &b_clip_arena, &b_anchors, &b_counts,   &b_gather_ids, &b_gather_out,
                      &b_cx_pos,    &b_cx_off,  &b_cx,     &b_cx_arena,   &b_cx_cov,   &b_cx_key, &b_out,
                      &b_out_arena, &b_score_ctr, &b_status, &b_keys, &b_keys2, &b_kpos, &b_kpos2, &b_sort_tmp,
-                     &b_kcount};
+                     &b_kcount, &b_args, &b_kpos3};
     for (auto* b : all)
         if (b->p) hipFree(b->p);
     for (auto& b : b_reads)
@@ -187,8 +187,17 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
     if (!prep || !cig || !ctr || !span) return fail(RV_ERR_OOM, "device allocation failed");
     RV_HIP(hipMemsetAsync(ctr, 0, sizeof(WalkCounters), stream));
     RV_HIP(hipMemsetAsync(span, 0, 4 * sizeof(int), stream));
+    WalkArgs* dargs = (WalkArgs*)grow(b_args, sizeof(WalkArgs));
+    if (!dargs) return fail(RV_ERR_OOM, "device allocation failed");
+    // kernel arguments live in device memory (synchronous copies: the stream is idle here)
+    WalkArgs hargs{};
+    hargs.R = R;
+    hargs.W = W;
+    hargs.cfg = cfg;
+    RV_HIP(hipStreamSynchronize(stream));
+    RV_HIP(hipMemcpy(dargs, &hargs, sizeof hargs, hipMemcpyHostToDevice));
     RV_HIP(hipEventRecord(ev[0], stream));
-    RV_HIP(launch_prep(R, W, cfg, prep, cig, ctr, span, stream));
+    RV_HIP(launch_prep(dargs, n, prep, cig, ctr, span, stream));
     RV_HIP(hipEventRecord(ev[1], stream));
     int max_span = 0;
     RV_HIP(hipMemcpyAsync(&max_span, span, sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -235,13 +244,17 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
         S.clip_cap = clip_cap;
         S.max_anchors = max_anchors;
         S.ctr = ctr;
+        hargs.W = W;
+        hargs.S = S;
+        RV_HIP(hipStreamSynchronize(stream));
+        RV_HIP(hipMemcpy(dargs, &hargs, sizeof hargs, hipMemcpyHostToDevice));
         if (attempt > 0) {
             RV_HIP(launch_fill_rows(rows, W.dense_n, stream));
             RV_HIP(hipMemsetAsync(ctr, 0, sizeof(WalkCounters), stream));
-            RV_HIP(launch_prep(R, W, cfg, prep, cig, ctr, span, stream));
+            RV_HIP(launch_prep(dargs, n, prep, cig, ctr, span, stream));
         }
         RV_HIP(hipEventRecord(ev[2], stream));
-        RV_HIP(launch_walk(R, W, cfg, prep, cig, S, stream));
+        RV_HIP(launch_walk(dargs, n, prep, cig, stream));
         RV_HIP(hipEventRecord(ev[3], stream));
         RV_HIP(launch_anchor(S.anchors, S.n_anchors, max_anchors, rows, W.dense_lo, W.dense_n, cfg, stream));
         RV_HIP(hipEventRecord(ev[4], stream));
@@ -276,7 +289,10 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
             out.P.dense_n = W.dense_n;
             out.P.maxReadLength = hc.max_read_length;
             reduce_events(evs.data(), evs.size(), arena.data(), cfg.goodq, out.P);
+            const auto r1 = std::chrono::steady_clock::now();
             reduce_clips(cls.data(), cls.size(), carena.data(), cfg.goodq, out.P);
+            stats.detail_ms[10] = std::chrono::duration<double, std::milli>(r1 - r0).count();
+            stats.detail_ms[11] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - r1).count();
             host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - r0).count();
             out.W = W;
             out.ctr = hc;

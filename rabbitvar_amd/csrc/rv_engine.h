@@ -57,7 +57,7 @@ class Engine {
 
     // buffers of the realignment / scoring stages
     DevBuf b_gather_ids, b_gather_out, b_cx_pos, b_cx_off, b_cx, b_cx_arena, b_cx_cov, b_cx_key, b_out, b_out_arena,
-        b_score_ctr, b_status, b_keys, b_keys2, b_kpos, b_kpos2, b_sort_tmp, b_kcount, b_kpos3;
+        b_score_ctr, b_status, b_keys, b_keys2, b_kpos, b_kpos2, b_sort_tmp, b_kcount, b_kpos3, b_args;
     hipEvent_t sev[2] = {nullptr, nullptr};
     // figures of the last region: record-list positions, dense rows fetched for the
     // realigner, insertions whose coverage came from the next position

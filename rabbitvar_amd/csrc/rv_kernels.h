@@ -7,11 +7,10 @@
 
 namespace rv {
 
-hipError_t launch_prep(const ReadsDev& R, const Window& W, const Config& cfg, PrepRec* prep, uint32_t* cig,
-                       WalkCounters* ctr, int* max_span, hipStream_t s);
+hipError_t launch_prep(const WalkArgs* A, int64_t n, PrepRec* prep, uint32_t* cig, WalkCounters* ctr, int* max_span,
+                       hipStream_t s);
 hipError_t launch_fill_rows(DenseRow* rows, int n, hipStream_t s);
-hipError_t launch_walk(const ReadsDev& R, const Window& W, const Config& cfg, const PrepRec* prep, const uint32_t* cig,
-                       const WalkSink& sink, hipStream_t s);
+hipError_t launch_walk(const WalkArgs* A, int64_t n, const PrepRec* prep, const uint32_t* cig, hipStream_t s);
 hipError_t launch_anchor(const Anchor* a, const unsigned* n_anchors, unsigned cap, DenseRow* rows, int dense_lo,
                          int dense_n, const Config& cfg, hipStream_t s);
 

@@ -47,6 +47,23 @@ int rh_max_displacement(const int32_t* keys, size_t n, size_t cap) {
     return maxd;
 }
 
+int rh_packed_displacement(const int32_t* keys, size_t n, size_t cap) {
+    // displacements of a packed, home-ordered layout depend only on how many keys share each home
+    std::vector<uint32_t> cnt(cap, 0);
+    const uint64_t mask = (uint64_t)cap - 1;
+    for (size_t i = 0; i < n; i++) cnt[(size_t)(hash_int(keys[i]) & mask)]++;
+    int64_t next = 0;  // first free slot
+    int maxd = 0;
+    for (size_t h = 0; h < cap; h++) {
+        if (!cnt[h]) continue;
+        const int64_t first = (int64_t)h > next ? (int64_t)h : next;
+        const int64_t last = first + (int64_t)cnt[h] - 1;
+        if (last - (int64_t)h > maxd) maxd = (int)(last - (int64_t)h);
+        next = last + 1;
+    }
+    return maxd;
+}
+
 std::vector<int32_t> rh_sort_keys(const std::vector<int32_t>& keys, size_t cap) {
     const size_t n = keys.size();
     const int ib = bits_for(n);
@@ -82,13 +99,54 @@ std::vector<int32_t> rh_sort_keys(const std::vector<int32_t>& keys, size_t cap) 
     return va;
 }
 
-void replay_created(RHOrder& m, const std::vector<int32_t>& created) {
+namespace {
+
+// Applies the operations with the home slots of the next keys prefetched.
+void apply_ops(RHOrder& m, const std::vector<int32_t>& ops, const std::vector<uint8_t>* transient) {
     constexpr size_t AHEAD = 24;
-    const size_t n = created.size();
+    const size_t n = ops.size();
     for (size_t i = 0; i < n; i++) {
-        if (i + AHEAD < n) m.prefetch(created[i + AHEAD]);
-        m.insert(created[i]);
+        if (i + AHEAD < n) m.prefetch(ops[i + AHEAD]);
+        m.insert(ops[i]);
+        if (transient && (*transient)[i]) m.erase(ops[i]);
     }
+}
+
+}  // namespace
+
+void replay_created(RHOrder& m, const std::vector<int32_t>& created) { apply_ops(m, created, nullptr); }
+
+std::vector<int32_t> replay_ops(const std::vector<int32_t>& ops, const std::vector<uint8_t>& transient) {
+    RHOrder m;
+    apply_ops(m, ops, &transient);
+    std::vector<int32_t> out;
+    out.reserve(m.size());
+    m.for_each([&](int32_t k) { out.push_back(k); });
+    return out;
+}
+
+std::vector<int32_t> aligned_order(const std::vector<int32_t>& processing, const uint8_t* status, int status_lo,
+                                   int status_n) {
+    // alignedVariants holds a few of the positions in a scattered order, so the sorted form rarely
+    // applies (its clusters are long); the replay is as fast as sorting when it would
+    std::vector<int32_t> ops;
+    std::vector<uint8_t> transient;
+    ops.reserve(processing.size() / 4);
+    transient.reserve(processing.size() / 4);
+    const size_t n = processing.size();
+    for (size_t i = 0; i < n; i++) {
+        if (i + 32 < n) {
+            const int k = processing[i + 32] - status_lo;
+            if (k >= 0 && k < status_n) __builtin_prefetch(status + k);
+        }
+        const int32_t p = processing[i];
+        const int k = p - status_lo;
+        const int st = (k >= 0 && k < status_n) ? status[k] : 0;
+        if (st == 0) continue;
+        ops.push_back(p);
+        transient.push_back(st == 2 ? 1 : 0);
+    }
+    return replay_ops(ops, transient);
 }
 
 std::vector<int32_t> noninsertion_order(const std::vector<int32_t>& created, const std::vector<KeyOp>& log) {

@@ -36,6 +36,8 @@ size_t rh_capacity(size_t max_count);
 int rh_log2(size_t cap);
 // largest displacement (slot - home) of keys laid out in slot order in a table of `cap` slots
 int rh_max_displacement(const int32_t* slot_ordered, size_t n, size_t cap);
+// the same for any order of `keys`: the largest displacement once they are packed by home
+int rh_packed_displacement(const int32_t* keys, size_t n, size_t cap);
 // keys (in insertion order) sorted by (home, hash bits desc, insertion index): a host radix sort
 std::vector<int32_t> rh_sort_keys(const std::vector<int32_t>& keys, size_t cap);
 // displacement at or above which an info byte overflows at the initial info increment
@@ -58,40 +60,12 @@ bool noninsertion_order_sorted(const std::vector<int32_t>& walk_sorted, const st
                                std::vector<int32_t>& out);
 
 // alignedVariants in slot order, given the positions in processing order and, per position, what
-// ToVarsBuilder did with it: 0 nothing, 1 added, 2 added then erased.
-template <class StatusOf>
-std::vector<int32_t> aligned_order(const std::vector<int32_t>& processing, StatusOf status_of) {
-    // sorted form first
-    std::vector<int32_t> kept;
-    size_t count = 0, max_count = 0;
-    bool transient = false;
-    for (int32_t p : processing) {
-        const int st = status_of(p);
-        if (st == 0) continue;
-        if (count + 1 > max_count) max_count = count + 1;
-        if (st == 2) {
-            transient = true;
-            continue;
-        }
-        kept.push_back(p);
-        ++count;
-    }
-    const size_t cap = rh_capacity(max_count);
-    std::vector<int32_t> sorted = rh_sort_keys(kept, cap);
-    // a transient key can lengthen one cluster by one slot while it is present
-    const int limit = RH_OVERFLOW_DISP - (transient ? 1 : 0);
-    if (rh_max_displacement(sorted.data(), sorted.size(), cap) < limit) return sorted;
-    RHOrder m;
-    for (int32_t p : processing) {
-        const int st = status_of(p);
-        if (st == 0) continue;
-        m.insert(p);
-        if (st == 2) m.erase(p);
-    }
-    std::vector<int32_t> out;
-    out.reserve(m.size());
-    m.for_each([&](int32_t k) { out.push_back(k); });
-    return out;
-}
+// ToVarsBuilder did with it: status[p - status_lo] = 0 nothing, 1 added, 2 added then erased.
+std::vector<int32_t> aligned_order(const std::vector<int32_t>& processing, const uint8_t* status, int status_lo,
+                                   int status_n);
+
+// Replays insert (and insert-then-erase) operations on a robin_hood<int> table from empty and
+// returns its slot order.  ops[i] = key, transient[i] = erased right after its insertion.
+std::vector<int32_t> replay_ops(const std::vector<int32_t>& ops, const std::vector<uint8_t>& transient);
 
 }  // namespace rv

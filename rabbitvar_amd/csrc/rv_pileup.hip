@@ -7,10 +7,10 @@
 
 namespace rv {
 
-__global__ void __launch_bounds__(256) k_prep(ReadsDev R, Window W, Config cfg, PrepRec* prep, uint32_t* cig,
-                                                WalkCounters* ctr, int* max_span) {
+__global__ void __launch_bounds__(256) k_prep(const WalkArgs* A, PrepRec* prep, uint32_t* cig, WalkCounters* ctr,
+                                                int* max_span) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < R.n) prep_one(r, R, W, cfg, prep, cig, ctr, max_span);
+    if (r < A->R.n) prep_one(r, A->R, A->W, A->cfg, prep, cig, ctr, max_span);
 }
 
 __global__ void __launch_bounds__(256) k_fill_rows(DenseRow* rows, int n) {
@@ -25,21 +25,23 @@ constexpr int WALK_THREADS = 1024;
 constexpr int WALK_READS = 64;
 constexpr int TILE_MARGIN = 64;
 
-__global__ void __launch_bounds__(WALK_THREADS) k_walk(ReadsDev R, Window W, Config cfg, const PrepRec* prep,
-                                                         const uint32_t* cig, WalkSink sink) {
-    __shared__ TileAcc tile;
+__global__ void __launch_bounds__(WALK_THREADS) k_walk(const WalkArgs* A, const PrepRec* prep, const uint32_t* cig) {
     const int waves = (int)(blockDim.x / warpSize);
-    const int wave = (int)(threadIdx.x / warpSize), lane = (int)(threadIdx.x % warpSize);
+    // the wavefront index is uniform across its lanes; saying so lets the per-read state live in
+    // scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / warpSize));
+    const int lane = (int)(threadIdx.x % warpSize);
+    const int64_t n = A->R.n;
     const int64_t r0 = (int64_t)blockIdx.x * WALK_READS;
-    const int tile_lo = (r0 < R.n ? prep[r0].position : 0) - TILE_MARGIN;
-    tile_init(&tile, (int)threadIdx.x, (int)blockDim.x);
+    const int tile_lo = (r0 < n ? prep[r0].position : 0) - TILE_MARGIN;
+    tile_init((int)threadIdx.x, (int)blockDim.x);
     __syncthreads();
     for (int k = wave; k < WALK_READS; k += waves) {
         const int64_t r = r0 + k;
-        if (r < R.n) walk_one(r, R, W, cfg, prep, cig, sink, lane, &tile, tile_lo);
+        if (r < n) walk_one(r, A->R, A->W, A->cfg, prep, cig, A->S, lane, true, tile_lo);
     }
     __syncthreads();
-    tile_flush(&tile, tile_lo, sink.dense, sink.dense_lo, sink.dense_n, (int)threadIdx.x, (int)blockDim.x);
+    tile_flush(tile_lo, A->S.dense, A->S.dense_lo, A->S.dense_n, (int)threadIdx.x, (int)blockDim.x);
 }
 
 __global__ void __launch_bounds__(256) k_anchor(const Anchor* A, const unsigned* n_anchors, unsigned cap, DenseRow* rows,
@@ -68,10 +70,10 @@ __global__ void __launch_bounds__(256) k_gather_rows(const DenseRow* rows, int d
 // ---------------------------------------------------------------- launch wrappers
 static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
-hipError_t launch_prep(const ReadsDev& R, const Window& W, const Config& cfg, PrepRec* prep, uint32_t* cig,
-                       WalkCounters* ctr, int* max_span, hipStream_t s) {
-    if (R.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_prep, dim3(grid_for(R.n, 256)), dim3(256), 0, s, R, W, cfg, prep, cig, ctr, max_span);
+hipError_t launch_prep(const WalkArgs* A, int64_t n, PrepRec* prep, uint32_t* cig, WalkCounters* ctr, int* max_span,
+                       hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_prep, dim3(grid_for(n, 256)), dim3(256), 0, s, A, prep, cig, ctr, max_span);
     return hipGetLastError();
 }
 hipError_t launch_fill_rows(DenseRow* rows, int n, hipStream_t s) {
@@ -79,10 +81,9 @@ hipError_t launch_fill_rows(DenseRow* rows, int n, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_rows, dim3(grid_for(n, 256)), dim3(256), 0, s, rows, n);
     return hipGetLastError();
 }
-hipError_t launch_walk(const ReadsDev& R, const Window& W, const Config& cfg, const PrepRec* prep, const uint32_t* cig,
-                       const WalkSink& sink, hipStream_t s) {
-    if (R.n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_walk, dim3(grid_for(R.n, WALK_READS)), dim3(WALK_THREADS), 0, s, R, W, cfg, prep, cig, sink);
+hipError_t launch_walk(const WalkArgs* A, int64_t n, const PrepRec* prep, const uint32_t* cig, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_walk, dim3(grid_for(n, WALK_READS)), dim3(WALK_THREADS), 0, s, A, prep, cig);
     return hipGetLastError();
 }
 hipError_t launch_anchor(const Anchor* a, const unsigned* n_anchors, unsigned cap, DenseRow* rows, int dense_lo,

@@ -27,6 +27,15 @@ struct ReadsDev {
     int64_t n;
 };
 
+// Everything the prep and walk kernels read besides the per-read records, kept in device memory
+// so that uniform fields are read with scalar loads rather than copied to private memory.
+struct WalkArgs {
+    ReadsDev R;
+    Window W;
+    Config cfg;
+    WalkSink S;
+};
+
 __device__ __forceinline__ uint32_t packed_seq_word(const uint8_t* seq, int lseq, int w) {
     uint32_t v = 0;
     for (int b = 0; b < 4; b++) {
@@ -188,13 +197,13 @@ __device__ inline void fill_row_one(DenseRow* rows, int i) {
 // One read, walked by a whole wavefront (lane = this thread's lane; see rv_walk.h).  Contributions
 // to positions [tile_lo, tile_lo + TILE_POS) go to the workgroup's tile.
 __device__ inline void walk_one(int64_t r, const ReadsDev& R, const Window& W, const Config& cfg, const PrepRec* prep,
-                                const uint32_t* cig, WalkSink& sink, int lane, TileAcc* tile, int tile_lo) {
+                                const uint32_t* cig, const WalkSink& sink, int lane, bool use_tile, int tile_lo) {
     const PrepRec p = prep[r];
     if (p.status != RD_PARSE) return;
     Walker w;
     w.lane = lane;
     w.leader = lane == 0;
-    w.tile = tile;
+    w.use_tile = use_tile;
     w.tile_lo = tile_lo;
     w.ref.leader = lane == 0;
     w.c = &cfg;
@@ -240,13 +249,15 @@ __device__ inline void walk_one(int64_t r, const ReadsDev& R, const Window& W, c
 }
 
 // Tile set-up and flush, strided over the workgroup's threads.
-__device__ inline void tile_init(TileAcc* t, int tid, int nthreads) {
-    int32_t* f = &t->f[0][0][0];
-    for (int k = tid; k < TF_N * 4 * TILE_POS; k += nthreads) f[k] = (k / (4 * TILE_POS) == TF_MINORD) ? -1 : 0;
+__device__ inline void tile_init(int tid, int nthreads) {
+    for (int k = tid; k < TF_N * 4 * TILE_POS; k += nthreads) {
+        const int f = k / (4 * TILE_POS), r = k - f * 4 * TILE_POS;
+        walk_tile.f[f][r / TILE_POS][r % TILE_POS] = f == TF_MINORD ? -1 : 0;
+    }
     for (int k = tid; k < TILE_POS; k += nthreads) {
-        t->cov[k] = 0;
-        t->rowflags[k] = 0;
-        t->first_touch[k] = ~0ull;
+        walk_tile.cov[k] = 0;
+        walk_tile.rowflags[k] = 0;
+        walk_tile.first_touch[k] = ~0ull;
     }
 }
 
@@ -254,8 +265,8 @@ __device__ inline void tile_init(TileAcc* t, int tid, int nthreads) {
 // cover pairs inside the workgroup, and comparing the tile's first (position, quality) value with
 // the row's covers pairs across workgroups (all values equal within each side, so the two sides
 // differ exactly when their first values do).
-__device__ inline void tile_flush(const TileAcc* t, int tile_lo, DenseRow* rows, int dense_lo, int dense_n, int tid,
-                                  int nthreads) {
+__device__ inline void tile_flush(int tile_lo, DenseRow* rows, int dense_lo, int dense_n, int tid, int nthreads) {
+    const TileAcc* t = &walk_tile;
     for (int k = tid; k < 4 * TILE_POS; k += nthreads) {
         const int b = k / TILE_POS, i = k - b * TILE_POS;
         const int ri = tile_lo + i - dense_lo;

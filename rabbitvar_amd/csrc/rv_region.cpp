@@ -50,6 +50,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     s = e.key_order((int)std::min<int64_t>(reads.n_reads, INT32_MAX), created, walk_sorted);
     if (s != RV_OK) return s;
     double order_ms = ms_since(k0);
+    e.stats.detail_ms[6] = order_ms;
     auto h0 = std::chrono::steady_clock::now();
     // a second host thread checks whether the sorted slot order provably holds and otherwise
     // replays the walk's insertions, while this one realigns
@@ -115,12 +116,19 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     } catch (const std::exception& ex) {
         return e.fail(RV_ERR_DEVICE, std::string("realignment: ") + ex.what());
     }
+    e.stats.detail_ms[0] = rst.ms_prefetch;
+    e.stats.detail_ms[1] = rst.ms_mnp;
+    e.stats.detail_ms[2] = rst.ms_del;
+    e.stats.detail_ms[3] = rst.ms_ins;
+    e.stats.detail_ms[4] = rst.ms_lgins;
+    const auto c0 = std::chrono::steady_clock::now();
     CxHost cx;
     try {
         build_record_list(S, cx);
     } catch (const std::exception& ex) {
         return e.fail(RV_ERR_DEVICE, std::string("record list: ") + ex.what());
     }
+    e.stats.detail_ms[5] = ms_since(c0);
     double host_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
     const double realign_ms = host_ms;
     // ---- scoring on the device
@@ -207,6 +215,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     auto t0 = std::chrono::steady_clock::now();
     replay.join();
     const auto o1 = std::chrono::steady_clock::now();
+    e.stats.detail_ms[7] = std::chrono::duration<double, std::milli>(o1 - t0).count();
     if (replay_err) return e.fail(RV_ERR_DEVICE, "key order replay failed");
     std::vector<int32_t> proc;
     if (replayed || !noninsertion_order_sorted(walk_sorted, walk_maxd, S.key_log, proc)) {
@@ -214,15 +223,15 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
         proc = replay_finish(nonins, S.key_log);
     }
     e.stats.order_sorted = replayed ? 0 : 1;
+    const auto a0 = std::chrono::steady_clock::now();
     // one status per window position: record-list positions override their dense rows
     for (size_t k = 0; k < cx.pos.size(); k++) {
         const int i = cx.pos[k] - pr.P.dense_lo;
         if (i >= 0 && i < pr.P.dense_n) status[(size_t)i] = status[(size_t)pr.P.dense_n + k];
     }
-    std::vector<int32_t> aligned = aligned_order(proc, [&](int32_t p) -> int {
-        const int i = p - pr.P.dense_lo;
-        return (i >= 0 && i < pr.P.dense_n) ? status[(size_t)i] : 0;
-    });
+    std::vector<int32_t> aligned = aligned_order(proc, status.data(), pr.P.dense_lo, pr.P.dense_n);
+    e.stats.detail_ms[8] = ms_since(a0);
+    const auto k1 = std::chrono::steady_clock::now();
     {
         std::vector<int32_t> rank((size_t)std::max(pr.P.dense_n, 1), INT32_MAX);
         for (size_t i = 0; i < aligned.size(); i++) {
@@ -239,6 +248,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
         if (a.position != b.position) return a.position < b.position;
         return a.order < b.order;
     });
+    e.stats.detail_ms[9] = ms_since(k1);
     order_ms += ms_since(t0);
     const auto f0 = std::chrono::steady_clock::now();
     LineContext ctx{e.sample.c_str(), region.gene ? region.gene : "", region.chr ? region.chr : "", region.start, region.end,

@@ -39,6 +39,8 @@ struct TileAcc {
     uint32_t rowflags[TILE_POS];
     unsigned long long first_touch[TILE_POS];
 };
+// the walk kernel's tile, addressed directly so that its accesses are LDS instructions
+static __shared__ TileAcc walk_tile;
 
 struct Anchor {
     int32_t pos;
@@ -94,7 +96,7 @@ struct Walker {
     int Rs, Re;          // region
     int chrLen;
     RefWin ref;
-    WalkSink* S;
+    const WalkSink* S;
     // read
     const char* seq;
     const uint8_t* qual;
@@ -114,7 +116,7 @@ struct Walker {
     bool bad;
     int lane;      // lane within the wavefront walking this read
     bool leader;   // lane 0: issues the side effects of the lane-uniform steps
-    TileAcc* tile; // workgroup accumulator (nullptr: dense rows directly)
+    bool use_tile; // accumulate into walk_tile (false: dense rows directly)
     int tile_lo;   // position of tile slot 0
 
     __device__ uint32_t cw(int i) const { return (i >= 0 && i < rec_n) ? rec[i] : 0u; }
@@ -148,14 +150,14 @@ struct Walker {
     // order, replayed on the host)
     __device__ int tile_slot(int pos) const {
         const int i = pos - tile_lo;
-        return (tile && i >= 0 && i < TILE_POS) ? i : -1;
+        return (use_tile && i >= 0 && i < TILE_POS) ? i : -1;
     }
     __device__ void touch_key(DenseRow* r, int pos, unsigned t) {
         const unsigned long long key = ((unsigned long long)(unsigned)ord << 24) | (t & 0xffffffu);
         const int ti = tile_slot(pos);
         if (ti >= 0) {
-            if (!(tile->rowflags[ti] & 1u)) atomicOr(&tile->rowflags[ti], 1u);
-            if (key < tile->first_touch[ti]) atomicMin(&tile->first_touch[ti], key);
+            if (!(walk_tile.rowflags[ti] & 1u)) atomicOr(&walk_tile.rowflags[ti], 1u);
+            if (key < walk_tile.first_touch[ti]) atomicMin(&walk_tile.first_touch[ti], key);
             return;
         }
         if (!(r->rowflags & 1u)) atomicOr(&r->rowflags, 1u);
@@ -168,7 +170,7 @@ struct Walker {
     __device__ void add_cov(DenseRow* r, int pos, int k) {
         const int ti = tile_slot(pos);
         if (ti >= 0)
-            atomicAdd(&tile->cov[ti], k);
+            atomicAdd(&walk_tile.cov[ti], k);
         else
             atomicAdd(&r->cov, k);
     }
@@ -188,7 +190,7 @@ struct Walker {
     __device__ void dense_add(DenseRow* r, int pos, int b, int tp, int qv, int nmv) {
         const int ti = tile_slot(pos);
         if (ti >= 0) {
-            auto F = [&](int f) -> int32_t* { return &tile->f[f][b][ti]; };
+            auto F = [&](int f) -> int32_t* { return &walk_tile.f[f][b][ti]; };
             atomicAdd(F(TF_CNT), 1);
             atomicAdd(F(dir ? TF_REV : TF_FWD), 1);
             atomicAdd(F((double)qv >= c->goodq ? TF_HI : TF_LO), 1);
@@ -245,7 +247,7 @@ struct Walker {
         if (!leader) return;
         const int ti = tile_slot(pos);
         if (ti >= 0) {
-            auto F = [&](int f) -> int32_t* { return &tile->f[f][b][ti]; };
+            auto F = [&](int f) -> int32_t* { return &walk_tile.f[f][b][ti]; };
             atomicAdd(F(TF_CNT), 1);
             atomicAdd(F(dir ? TF_REV : TF_FWD), 1);
             atomicAdd(F((double)qv >= (double)(float)c->goodq ? TF_HI : TF_LO), 1);

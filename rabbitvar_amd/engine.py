@@ -127,4 +127,6 @@ class Engine:
     def stats(self):
         s = _lib.RvStats()
         self.lib.rv_get_stats(self.h, ctypes.byref(s))
-        return {k: getattr(s, k) for k, _ in s._fields_}
+        out = {k: getattr(s, k) for k, _ in s._fields_}
+        out["detail_ms"] = list(s.detail_ms)
+        return out

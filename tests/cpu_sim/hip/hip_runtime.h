@@ -11,6 +11,7 @@ using std::log;
 #define __host__
 #define __device__
 #define __global__
+#define __shared__
 #define __forceinline__ inline
 #define __launch_bounds__(x)
 inline int atomicAdd(int* p, int v) { int o = *p; *p += v; return o; }

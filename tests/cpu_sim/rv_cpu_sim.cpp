@@ -7,6 +7,7 @@
 //   rv_cpu_sim --rh-ints < OPS                              robin_hood <int,V> order after "i K" /
 //                                                           "e K" operations (one per line)
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <unordered_map>
 #include <cstdio>
@@ -171,12 +172,11 @@ int main(int argc, char** argv) {
                arena.size(), cl.data(), &ncl, (unsigned)cl.size(), carena.data(), &cu, carena.size(), an.data(), &na,
                (unsigned)an.size(), &ctr};
     // workgroups of 64 reads sharing one tile, as k_walk runs them
-    static TileAcc tile;
     for (int64_t r0 = 0; r0 < m; r0 += 64) {
-        tile_init(&tile, 0, 1);
+        tile_init(0, 1);
         const int tile_lo = prep[(size_t)r0].position - 64;
-        for (int64_t r = r0; r < r0 + 64 && r < m; r++) walk_one(r, R, W, cfg, prep.data(), cg.data(), S, 0, &tile, tile_lo);
-        tile_flush(&tile, tile_lo, rows.data(), W.dense_lo, W.dense_n, 0, 1);
+        for (int64_t r = r0; r < r0 + 64 && r < m; r++) walk_one(r, R, W, cfg, prep.data(), cg.data(), S, 0, true, tile_lo);
+        tile_flush(tile_lo, rows.data(), W.dense_lo, W.dense_n, 0, 1);
     }
     for (unsigned i = 0; i < na; i++) anchor_one(i, an.data(), rows.data(), W.dense_lo, W.dense_n, cfg);
     // nonInsertionVariants keys in creation order
@@ -259,7 +259,19 @@ int main(int argc, char** argv) {
         const int i = p - W.dense_lo;
         return (i >= 0 && i < W.dense_n) ? status[(size_t)i] : 0;
     };
-    std::vector<int32_t> aligned = aligned_order(proc, status_of);
+    std::vector<uint8_t> merged(status.begin(), status.begin() + W.dense_n);
+    for (size_t k = 0; k < cx.pos.size(); k++) {
+        const int i = cx.pos[k] - W.dense_lo;
+        if (i >= 0 && i < W.dense_n) merged[(size_t)i] = status[(size_t)W.dense_n + k];
+        else if (status[(size_t)W.dense_n + k]) {
+            fprintf(stderr, "order: status outside the dense window at %d\n", cx.pos[k]);
+            return 4;
+        }
+    }
+    const auto ta = std::chrono::steady_clock::now();
+    std::vector<int32_t> aligned = aligned_order(proc, merged.data(), W.dense_lo, W.dense_n);
+    fprintf(stderr, "order: alignedVariants %zu keys, %.1f ms\n", aligned.size(),
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ta).count());
     {
         RHOrder m;
         for (int32_t p : proc) {
