@@ -181,6 +181,8 @@ struct PrepRec {
     int32_t total_len;  // M + I + S length
     int32_t lseq;
     int32_t flags;      // bit0: rewrite changed the CIGAR; bit1: rewrite text overflow; bit2: negative length
+    int32_t reflen;     // M + D + N + = + X length of the record's own CIGAR
+    int32_t pad;
 };
 
 // Dense single-base allele accumulators, one row per reference position (AoS, 288-byte rows).

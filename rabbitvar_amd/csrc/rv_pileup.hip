@@ -21,9 +21,6 @@ __global__ void __launch_bounds__(256) k_fill_rows(DenseRow* rows, int n) {
 // One workgroup walks WALK_READS consecutive reads (sorted by position), one wavefront per read at a
 // time, into an LDS tile that starts TILE_MARGIN bases before the first read; the tile is added to
 // the dense rows once at the end.
-constexpr int WALK_THREADS = 1024;
-constexpr int WALK_READS = 64;
-constexpr int TILE_MARGIN = 64;
 
 __global__ void __launch_bounds__(WALK_THREADS) k_walk(const WalkArgs* A, const PrepRec* prep, const uint32_t* cig) {
     const int waves = (int)(blockDim.x / warpSize);
@@ -34,11 +31,11 @@ __global__ void __launch_bounds__(WALK_THREADS) k_walk(const WalkArgs* A, const 
     const int64_t n = A->R.n;
     const int64_t r0 = (int64_t)blockIdx.x * WALK_READS;
     const int tile_lo = (r0 < n ? prep[r0].position : 0) - TILE_MARGIN;
-    tile_init((int)threadIdx.x, (int)blockDim.x);
+    tile_init((int)threadIdx.x, (int)blockDim.x, A->W, tile_lo);
     __syncthreads();
     for (int k = wave; k < WALK_READS; k += waves) {
         const int64_t r = r0 + k;
-        if (r < n) walk_one(r, A->R, A->W, A->cfg, prep, cig, A->S, lane, true, tile_lo);
+        if (r < n) walk_one(r, A->R, A->W, A->cfg, prep, cig, A->S, lane, wave, tile_lo);
     }
     __syncthreads();
     tile_flush(tile_lo, A->S.dense, A->S.dense_lo, A->S.dense_n, (int)threadIdx.x, (int)blockDim.x);

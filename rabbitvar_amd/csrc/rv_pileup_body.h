@@ -56,6 +56,8 @@ __device__ inline void prep_one(int64_t r, const ReadsDev& R, const Window& W, c
     p.nm_eff = 0;
     p.rlen1 = 0;
     p.total_len = 0;
+    p.reflen = 0;
+    p.pad = 0;
     const uint16_t flag = R.flag[r];
     const int64_t so = R.seq_off[r];
     const int lseq = (int)(R.seq_off[r + 1] - so);
@@ -73,6 +75,7 @@ __device__ inline void prep_one(int64_t r, const ReadsDev& R, const Window& W, c
             if (o == OP_M || o == OP_D || o == OP_N || o == OP_EQ || o == OP_X) reflen += clen(R.cigar[co + i]);
         }
         atomicMax(max_span, lseq + reflen + 8);
+        p.reflen = reflen;
     }
     if ((flag & cfg.samfilter) != 0 || (int)R.mapq[r] < cfg.mappingQuality || lseq == 1) {
         p.status = RD_FILTERED;
@@ -194,16 +197,54 @@ __device__ inline void fill_row_one(DenseRow* rows, int i) {
     rows[i] = z;
 }
 
-// One read, walked by a whole wavefront (lane = this thread's lane; see rv_walk.h).  Contributions
-// to positions [tile_lo, tile_lo + TILE_POS) go to the workgroup's tile.
+// One read, walked by a whole wavefront (lane = this thread's lane, wave = its wavefront's index in
+// the workgroup; see rv_walk.h).  Contributions to positions [tile_lo, tile_lo + TILE_POS) go to the
+// workgroup's tile.
 __device__ inline void walk_one(int64_t r, const ReadsDev& R, const Window& W, const Config& cfg, const PrepRec* prep,
-                                const uint32_t* cig, const WalkSink& sink, int lane, bool use_tile, int tile_lo) {
+                                const uint32_t* cig, const WalkSink& sink, int lane, int wave, int tile_lo) {
     const PrepRec p = prep[r];
     if (p.status != RD_PARSE) return;
+    // every per-read field up front, so that the loads are in flight together
+    const int64_t so = R.seq_off[r];
+    const uint8_t mapq = R.mapq[r];
+    const uint16_t flag = R.flag[r];
+    const int tid = R.tid[r], mtid = R.mtid[r], mpos0 = R.mpos0[r], pos0 = R.pos0[r];
+    const char* gseq = (const char*)(R.bases + so);
+    const uint8_t* gqual = R.quals + so;
+    const uint32_t* rec = cig + (size_t)r * MAXC;
+    WaveStage* st = &walk_stage[wave];
+    const bool staged = p.lseq <= STAGE_LEN;
+    // the previous read's steps are done with the copy before it is overwritten, and the copy is
+    // complete before any lane reads it (LDS operations of one wavefront complete in order)
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    for (int i = lane; i < MAXC; i += warpSize) st->cig[i] = rec[i];
+    if (staged) {
+        for (int i0 = 0; i0 < p.lseq; i0 += 8 * warpSize) {
+            char b[8];
+            uint8_t q[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int i = i0 + k * warpSize + lane;
+                if (i < p.lseq) {
+                    b[k] = gseq[i];
+                    q[k] = gqual[i];
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int i = i0 + k * warpSize + lane;
+                if (i < p.lseq) {
+                    st->seq[i] = b[k];
+                    st->qual[i] = q[k];
+                }
+            }
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     Walker w;
     w.lane = lane;
     w.leader = lane == 0;
-    w.use_tile = use_tile;
+    w.use_tile = true;
     w.tile_lo = tile_lo;
     w.ref.leader = lane == 0;
     w.c = &cfg;
@@ -214,42 +255,42 @@ __device__ inline void walk_one(int64_t r, const ReadsDev& R, const Window& W, c
     w.ref.lo = W.ref_lo;
     w.ref.hi = W.ref_hi;
     w.ref.ctr = sink.ctr;
+    w.ref.slo = tile_lo - REF_BEFORE;
+    w.ref.sn = REF_STAGE;
     w.S = &sink;
-    const int64_t so = R.seq_off[r];
-    w.seq = (const char*)(R.bases + so);
-    w.qual = R.quals + so;
+    w.seq = gseq;
+    w.qual = gqual;
+    w.stage = st;
+    w.staged = staged;
     w.lseq = p.lseq;
     {
         int nb = (p.lseq + 1) / 2;
         int last = nb - 1;
         int i0 = 2 * last, i1 = 2 * last + 1;
-        int hi = i0 < p.lseq ? nt16_code(w.seq[i0]) : 0;
-        int lo = i1 < p.lseq ? nt16_code(w.seq[i1]) : 0;
+        int hi = i0 < p.lseq ? nt16_code(w.base_at(i0)) : 0;
+        int lo = i1 < p.lseq ? nt16_code(w.base_at(i1)) : 0;
         w.qual_before0 = nb > 0 ? (uint8_t)((hi << 4) | lo) : 0;
     }
-    w.rec = cig + (size_t)r * MAXC;
     w.rec_n = MAXC;
     w.n_cigar = p.ncig;
     w.ord = (int)r;
     w.evseq = 0;
     w.tseq = 0;
-    w.mapq = R.mapq[r];
+    w.mapq = mapq;
     w.nm = p.nm_eff;
-    w.dir = (R.flag[r] & 0x10) != 0;
+    w.dir = (flag & 0x10) != 0;
     w.position = p.position;
     w.bad = false;
-    int reflen = 0;
-    const int64_t co = R.cig_off[r];
-    const int ncig = (int)(R.cig_off[r + 1] - co);
-    for (int i = 0; i < ncig; i++) {
-        int o = cop(R.cigar[co + i]);
-        if (o == OP_M || o == OP_D || o == OP_N || o == OP_EQ || o == OP_X) reflen += clen(R.cigar[co + i]);
-    }
-    w.walk(p.rlen1, p.total_len, R.flag[r], R.tid[r], R.mtid[r], R.mpos0[r] + 1, reflen, R.pos0[r]);
+    w.walk(p.rlen1, p.total_len, flag, tid, mtid, mpos0 + 1, p.reflen, pos0);
 }
 
 // Tile set-up and flush, strided over the workgroup's threads.
-__device__ inline void tile_init(int tid, int nthreads) {
+__device__ inline void tile_init(int tid, int nthreads, const Window& W, int tile_lo) {
+    const int slo = tile_lo - REF_BEFORE;
+    for (int k = tid; k < REF_STAGE; k += nthreads) {
+        const int pos = slo + k;
+        walk_ref[k] = (pos >= W.ref_lo && pos <= W.ref_hi) ? W.ref[pos - W.ref_lo] : '\0';
+    }
     for (int k = tid; k < TF_N * 4 * TILE_POS; k += nthreads) {
         const int f = k / (4 * TILE_POS), r = k - f * 4 * TILE_POS;
         walk_tile.f[f][r / TILE_POS][r % TILE_POS] = f == TF_MINORD ? -1 : 0;

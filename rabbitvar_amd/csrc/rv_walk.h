@@ -42,6 +42,32 @@ struct TileAcc {
 // the walk kernel's tile, addressed directly so that its accesses are LDS instructions
 static __shared__ TileAcc walk_tile;
 
+// The walk kernel's workgroup: WALK_THREADS lanes (12 wavefronts, 3 per SIMD at the register count
+// the walk needs) walk WALK_READS consecutive reads into a tile starting TILE_MARGIN bases before
+// the first read.
+constexpr int WALK_THREADS = 768;
+constexpr int WALK_READS = 48;
+constexpr int TILE_MARGIN = 64;
+
+// Per-wavefront copy of the read being walked (bases, qualities, rewritten CIGAR), loaded with one
+// coalesced pass so that the lane-uniform steps read LDS instead of waiting on global memory.
+// Reads longer than STAGE_LEN are read from global memory.
+constexpr int STAGE_LEN = 512;
+constexpr int WALK_MAX_WAVES = 16;
+struct WaveStage {
+    char seq[STAGE_LEN];
+    uint8_t qual[STAGE_LEN];
+    uint32_t cig[MAXC];
+};
+static __shared__ WaveStage walk_stage[WALK_MAX_WAVES];
+static_assert(WALK_THREADS / 64 <= WALK_MAX_WAVES, "one read copy per wavefront");
+
+// Per-workgroup copy of the reference bases around the tile: [tile_lo - REF_BEFORE,
+// tile_lo - REF_BEFORE + REF_STAGE).  Positions outside it are read from global memory.
+constexpr int REF_BEFORE = 64;
+constexpr int REF_STAGE = TILE_POS + 512;
+static __shared__ char walk_ref[REF_STAGE];
+
 struct Anchor {
     int32_t pos;
     int32_t ord;
@@ -79,18 +105,24 @@ struct RefWin {
     int lo, hi;
     WalkCounters* ctr;
     bool leader;
-    __device__ bool has(int k) const { return k >= lo && k <= hi; }
-    __device__ char at(int k) const { return ref[k - lo]; }
+    int slo, sn;  // staged copy: walk_ref[k - slo] for k - slo in [0, sn)
+    __device__ __forceinline__ bool has(int k) const { return k >= lo && k <= hi; }
+    __device__ __forceinline__ char at(int k) const {
+        const unsigned j = (unsigned)(k - slo);
+        return j < (unsigned)sn ? walk_ref[j] : ref[k - lo];
+    }
     // robin_hood operator[]: a missing key would be created with '\0'
-    __device__ char get(int k) const {
+    __device__ __forceinline__ char get(int k) const {
         if (has(k)) return at(k);
         if (leader) atomicAdd(&ctr->ref_insert, 1ull);
         return '\0';
     }
-    __device__ bool hasEq(int k, char ch) const { return has(k) && at(k) == ch; }
-    __device__ bool hasNe(int k, char ch) const { return has(k) && at(k) != ch; }
+    __device__ __forceinline__ bool hasEq(int k, char ch) const { return has(k) && at(k) == ch; }
+    __device__ __forceinline__ bool hasNe(int k, char ch) const { return has(k) && at(k) != ch; }
 };
 
+// Every member function is force-inlined: one out-of-line call makes the walker live in per-lane
+// scratch memory, which puts a memory round trip on every state access of the walk.
 struct Walker {
     const Config* c;
     int Rs, Re;          // region
@@ -102,7 +134,8 @@ struct Walker {
     const uint8_t* qual;
     uint8_t qual_before0;
     int lseq;
-    const uint32_t* rec;
+    WaveStage* stage;    // this wavefront's copy of the read
+    bool staged;         // bases and qualities are in stage (else read through seq / qual)
     int rec_n;           // valid emulated words
     int n_cigar;
     int ord;
@@ -119,25 +152,27 @@ struct Walker {
     bool use_tile; // accumulate into walk_tile (false: dense rows directly)
     int tile_lo;   // position of tile slot 0
 
-    __device__ uint32_t cw(int i) const { return (i >= 0 && i < rec_n) ? rec[i] : 0u; }
-    __device__ int op(int i) const { return cop(cw(i)); }
-    __device__ int len(int i) const { return clen(cw(i)); }
-    __device__ int q(int i) {
-        if (i >= 0 && i < lseq) return qual[i];
+    __device__ __forceinline__ uint32_t cw(int i) const { return (i >= 0 && i < rec_n) ? stage->cig[i] : 0u; }
+    __device__ __forceinline__ char base_at(int i) const { return staged ? stage->seq[i] : seq[i]; }
+    __device__ __forceinline__ int qual_at(int i) const { return staged ? stage->qual[i] : qual[i]; }
+    __device__ __forceinline__ int op(int i) const { return cop(cw(i)); }
+    __device__ __forceinline__ int len(int i) const { return clen(cw(i)); }
+    __device__ __forceinline__ int q(int i) {
+        if (i >= 0 && i < lseq) return qual_at(i);
         if (i == -1) return qual_before0;
         bad = true;
         return 0;
     }
-    __device__ char sq(int i) const { return (i >= 0 && i < lseq) ? seq[i] : '\0'; }
-    __device__ void sub(DStr& d, int p, int n) {
+    __device__ __forceinline__ char sq(int i) const { return (i >= 0 && i < lseq) ? base_at(i) : '\0'; }
+    __device__ __forceinline__ void sub(DStr& d, int p, int n) {
         if (p < 0 || p > lseq) {
             bad = true;
             return;
         }
-        for (int i = p; i < p + n && i < lseq; i++) d.push(seq[i]);
+        for (int i = p; i < p + n && i < lseq; i++) d.push(base_at(i));
     }
 
-    __device__ DenseRow* row(int pos) {
+    __device__ __forceinline__ DenseRow* row(int pos) {
         int i = pos - S->dense_lo;
         if (i < 0 || i >= S->dense_n) {
             bad = true;
@@ -148,11 +183,11 @@ struct Walker {
     // a getVariation on nonInsertionVariants at this row: the key exists from here on, and the
     // smallest (read, touch) pair records when the reference created it (its hash-map insertion
     // order, replayed on the host)
-    __device__ int tile_slot(int pos) const {
+    __device__ __forceinline__ int tile_slot(int pos) const {
         const int i = pos - tile_lo;
         return (use_tile && i >= 0 && i < TILE_POS) ? i : -1;
     }
-    __device__ void touch_key(DenseRow* r, int pos, unsigned t) {
+    __device__ __forceinline__ void touch_key(DenseRow* r, int pos, unsigned t) {
         const unsigned long long key = ((unsigned long long)(unsigned)ord << 24) | (t & 0xffffffu);
         const int ti = tile_slot(pos);
         if (ti >= 0) {
@@ -163,31 +198,31 @@ struct Walker {
         if (!(r->rowflags & 1u)) atomicOr(&r->rowflags, 1u);
         if (key < r->first_touch) atomicMin(&r->first_touch, key);
     }
-    __device__ void touch_pos(DenseRow* r, int pos) {
+    __device__ __forceinline__ void touch_pos(DenseRow* r, int pos) {
         const unsigned t = tseq++;
         if (leader) touch_key(r, pos, t);
     }
-    __device__ void add_cov(DenseRow* r, int pos, int k) {
+    __device__ __forceinline__ void add_cov(DenseRow* r, int pos, int k) {
         const int ti = tile_slot(pos);
         if (ti >= 0)
             atomicAdd(&walk_tile.cov[ti], k);
         else
             atomicAdd(&r->cov, k);
     }
-    __device__ void cov(int pos, int k = 1) {
+    __device__ __forceinline__ void cov(int pos, int k = 1) {
         DenseRow* r = row(pos);
         if (r && leader) add_cov(r, pos, k);
     }
 
     // single-base contribution from the match run (updates pp/pq)
-    __device__ void dense_single(int pos, int b, int tp, int qv, int nmv) {
+    __device__ __forceinline__ void dense_single(int pos, int b, int tp, int qv, int nmv) {
         DenseRow* r = row(pos);
         if (!r) return;
         touch_pos(r, pos);
         if (leader) dense_add(r, pos, b, tp, qv, nmv);
     }
     // the pp/pq-updating contribution (first (tp, q) value + differs flags)
-    __device__ void dense_add(DenseRow* r, int pos, int b, int tp, int qv, int nmv) {
+    __device__ __forceinline__ void dense_add(DenseRow* r, int pos, int b, int tp, int qv, int nmv) {
         const int ti = tile_slot(pos);
         if (ti >= 0) {
             auto F = [&](int f) -> int32_t* { return &walk_tile.f[f][b][ti]; };
@@ -240,7 +275,7 @@ struct Walker {
         }
     }
     // addCnt-form contribution (soft-clip rescue): no pp/pq update
-    __device__ void dense_addcnt(int pos, int b, int rp, int qv) {
+    __device__ __forceinline__ void dense_addcnt(int pos, int b, int rp, int qv) {
         DenseRow* r = row(pos);
         if (!r) return;
         touch_pos(r, pos);
@@ -272,7 +307,7 @@ struct Walker {
         if (!(d.flags & 1u)) atomicOr(&d.flags, 1u);
     }
 
-    __device__ void event(int kind, int pos, const DStr& d, int tp, double qv, int nmv, int cntflags, int rp, bool hiq) {
+    __device__ __forceinline__ void event(int kind, int pos, const DStr& d, int tp, double qv, int nmv, int cntflags, int rp, bool hiq) {
         if (d.overflow) {
             if (leader) atomicAdd(&S->ctr->overflow, 1ull);
             return;
@@ -310,7 +345,7 @@ struct Walker {
     }
 
     // a nonInsertionVariants contribution from the match run / deletion handlers
-    __device__ void var_contrib(int pos, const DStr& d, int tp, double qv, int nmv, int cntflags) {
+    __device__ __forceinline__ void var_contrib(int pos, const DStr& d, int tp, double qv, int nmv, int cntflags) {
         if (d.n == 1) {
             int b = base_index(d.c[0]);
             if (b >= 0) {
@@ -321,14 +356,14 @@ struct Walker {
         event(0, pos, d, tp, qv, nmv, cntflags | 8, 0, qv >= c->goodq);
     }
 
-    __device__ int tp_of(int rlen1) const { return rpe < rlen1 - rpe ? rpe + 1 : rlen1 - rpe; }
+    __device__ __forceinline__ int tp_of(int rlen1) const { return rpe < rlen1 - rpe ? rpe + 1 : rlen1 - rpe; }
 
-    __device__ bool isNextMatched(int ci) const { return c->performLocalRealignment && n_cigar > ci + 1 && op(ci + 1) == OP_M; }
-    __device__ bool isNextInsertion(int ci) const {
+    __device__ __forceinline__ bool isNextMatched(int ci) const { return c->performLocalRealignment && n_cigar > ci + 1 && op(ci + 1) == OP_M; }
+    __device__ __forceinline__ bool isNextInsertion(int ci) const {
         return c->performLocalRealignment && n_cigar > ci + 1 && op(ci + 1) == OP_I;
     }
-    __device__ bool isNextAfterNumMatched(int ci, int number) const { return n_cigar > ci + number && op(ci + 1) == OP_M; }
-    __device__ bool isIDWithNextMatched(int ci) const {
+    __device__ __forceinline__ bool isNextAfterNumMatched(int ci, int number) const { return n_cigar > ci + number && op(ci + 1) == OP_M; }
+    __device__ __forceinline__ bool isIDWithNextMatched(int ci) const {
         return c->performLocalRealignment && n_cigar > ci + 2 && len(ci + 1) <= c->vext && op(ci + 1) == OP_M &&
                (op(ci + 2) == OP_I || op(ci + 2) == OP_D) && op(ci + 3) != OP_I && op(ci + 3) != OP_D;
     }
@@ -339,7 +374,7 @@ struct Walker {
         long q_sum;
         int nm;
     };
-    __device__ Off findOffset(int refPos, int readPos, int cigLen) {
+    __device__ __forceinline__ Off findOffset(int refPos, int readPos, int cigLen) {
         Off o{0, readPos, 0, 0};
         int vsn = 0;
         for (int vi = 0; vsn <= c->vext && vi < cigLen; vi++) {
@@ -362,7 +397,7 @@ struct Walker {
         return o;
     }
 
-    __device__ void appendSegments(int ci, DStr& desc, long& qs, int& qc, int mLen, int indelLen, int begin, bool isIns) {
+    __device__ __forceinline__ void appendSegments(int ci, DStr& desc, long& qs, int& qc, int mLen, int indelLen, int begin, bool isIns) {
         desc.push('#');
         sub(desc, begin, mLen);
         for (int i = begin; i < begin + mLen; i++) qs += q(i);
@@ -385,7 +420,7 @@ struct Walker {
         }
     }
 
-    __device__ void emit_clip(int side, int pos, int cel_, int nhq, int sumq, int from, int step) {
+    __device__ __forceinline__ void emit_clip(int side, int pos, int cel_, int nhq, int sumq, int from, int step) {
         for (int k = 0; k < nhq; k++) (void)q(from + step * k);  // range checks on every lane
         if (!leader) return;
         unsigned idx = atomicAdd(S->n_clips, 1u);
@@ -413,7 +448,7 @@ struct Walker {
         S->clips[idx] = r;
     }
 
-    __device__ void rescue(int pos, int rp, int qv) {
+    __device__ __forceinline__ void rescue(int pos, int rp, int qv) {
         char rb = ref.at(pos);
         int b = base_index(rb);
         if (b >= 0) {
@@ -427,7 +462,7 @@ struct Walker {
         cov(pos);
     }
 
-    __device__ void softclip(int ci, int totalLen) {
+    __device__ __forceinline__ void softclip(int ci, int totalLen) {
         if (ci == 0) {
             while (cel - 1 >= 0 && start - 1 > 0 && start - 1 <= chrLen && ref.hasEq(start - 1, sq(cel - 1)) && q(cel - 1) > 10) {
                 rescue(start - 1, cel, q(cel - 1));
@@ -473,7 +508,7 @@ struct Walker {
         start = position;
     }
 
-    __device__ int insertion(int ci, int rlen1) {
+    __device__ __forceinline__ int insertion(int ci, int rlen1) {
         if ((n_cigar > ci + 1 && op(ci + 1) == OP_N) || (ci > 0 && op(ci - 1) == OP_N)) {
             rpi += cel;
             return ci;
@@ -603,7 +638,7 @@ struct Walker {
         return ci;
     }
 
-    __device__ int deletion(int ci, int rlen1) {
+    __device__ __forceinline__ int deletion(int ci, int rlen1) {
         if ((n_cigar > ci && op(ci + 1) == OP_N) || (ci > 1 && op(ci - 1) == OP_N)) {
             rpi += cel;
             return ci;
@@ -711,7 +746,7 @@ struct Walker {
     // A base stops the run when it could start a multi-base allele (a high-quality mismatch with a
     // next base in the run and the region), when it is a high-quality mismatch within the last
     // `vext` bases before an adjacent D/I, or when it is not A/C/G/T/N.
-    __device__ int plain_chunk(int i, int ci, int totalLen, int rlen1, int nmoff) {
+    __device__ __forceinline__ int plain_chunk(int i, int ci, int totalLen, int rlen1, int nmoff) {
         const int span = min((int)warpSize, cel - i);
         const int k = lane;
         const bool act = k < span;
@@ -722,11 +757,11 @@ struct Walker {
             if (rp < 0 || rp >= lseq) {
                 stop = true;
             } else {
-                const char ch = seq[rp];
+                const char ch = base_at(rp);
                 if (ch == 'N') {
                     isN = true;
                 } else {
-                    qv = qual[rp];
+                    qv = qual_at(rp);
                     b = base_index(ch);
                     if (b < 0) {
                         stop = true;
@@ -761,7 +796,7 @@ struct Walker {
     }
 
     // one M (or =, X, P ...) CIGAR element; returns the possibly advanced ci
-    __device__ int match_run(int ci, int c_op, int rlen1, int totalLen) {
+    __device__ __forceinline__ int match_run(int ci, int c_op, int rlen1, int totalLen) {
         int nmoff = 0, moffset = 0;
         int i = offset;
         while (i < cel) {
@@ -941,7 +976,7 @@ struct Walker {
         return ci;
     }
 
-    __device__ bool skip_overlap(uint16_t flag, int tid, int mtid, int mateStart, int reflen, int pos0) const {
+    __device__ __forceinline__ bool skip_overlap(uint16_t flag, int tid, int mtid, int mateStart, int reflen, int pos0) const {
         bool pairedSame = (flag & 1) && tid == mtid;
         if (c->uniqueModeAlignmentEnabled && pairedSame && !dir && start >= mateStart) return true;
         if (c->uniqueModeSecondInPairEnabled && (flag & 1) && pairedSame) {
@@ -952,7 +987,7 @@ struct Walker {
         return false;
     }
 
-    __device__ void walk(int rlen1, int totalLen, uint16_t flag, int tid, int mtid, int mateStart, int reflen, int pos0) {
+    __device__ __forceinline__ void walk(int rlen1, int totalLen, uint16_t flag, int tid, int mtid, int mateStart, int reflen, int pos0) {
         start = position;
         offset = 0;
         rpi = rpe = 0;

@@ -23,6 +23,8 @@ inline unsigned long long atomicMin(unsigned long long* p, unsigned long long v)
 inline unsigned atomicOr(unsigned* p, unsigned v) { unsigned o = *p; *p |= v; return o; }
 inline unsigned atomicCAS(unsigned* p, unsigned c, unsigned v) { unsigned o = *p; if (o == c) *p = v; return o; }
 inline int atomicMax(int* p, int v) { int o = *p; if (v > o) *p = v; return o; }
+// one lane: nothing to order between lanes
+#define __builtin_amdgcn_fence(order, scope) ((void)0)
 // a wavefront of one lane
 constexpr int warpSize = 1;
 inline unsigned long long __ballot(int pred) { return pred ? 1ull : 0ull; }

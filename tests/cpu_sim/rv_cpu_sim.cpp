@@ -171,11 +171,12 @@ int main(int argc, char** argv) {
     WalkSink S{rows.data(), W.dense_lo, W.dense_n, ev.data(), &ne, (unsigned)ev.size(), arena.data(), &au,
                arena.size(), cl.data(), &ncl, (unsigned)cl.size(), carena.data(), &cu, carena.size(), an.data(), &na,
                (unsigned)an.size(), &ctr};
-    // workgroups of 64 reads sharing one tile, as k_walk runs them
-    for (int64_t r0 = 0; r0 < m; r0 += 64) {
-        tile_init(0, 1);
-        const int tile_lo = prep[(size_t)r0].position - 64;
-        for (int64_t r = r0; r < r0 + 64 && r < m; r++) walk_one(r, R, W, cfg, prep.data(), cg.data(), S, 0, true, tile_lo);
+    // workgroups of WALK_READS reads sharing one tile, as k_walk runs them
+    for (int64_t r0 = 0; r0 < m; r0 += WALK_READS) {
+        const int tile_lo = prep[(size_t)r0].position - TILE_MARGIN;
+        tile_init(0, 1, W, tile_lo);
+        for (int64_t r = r0; r < r0 + WALK_READS && r < m; r++)
+            walk_one(r, R, W, cfg, prep.data(), cg.data(), S, 0, 0, tile_lo);
         tile_flush(tile_lo, rows.data(), W.dense_lo, W.dense_n, 0, 1);
     }
     for (unsigned i = 0; i < na; i++) anchor_one(i, an.data(), rows.data(), W.dense_lo, W.dense_n, cfg);
