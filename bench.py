@@ -102,7 +102,7 @@ def main():
         log(f"warmup {i}: {time.perf_counter() - t0:.3f} s, {len(text.splitlines())} calls")
     walk_ms = []
     stage = {k: 0.0 for k in ("prep_ms", "walk_ms", "anchor_ms", "score_ms", "reduce_ms", "realign_ms", "order_ms",
-                              "format_ms", "wall_ms")}
+                              "format_ms", "wall_ms", "call_ms")}
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(a.steps):
@@ -143,7 +143,8 @@ def main():
         "stages_ms_per_step": {k: round(v / a.steps, 3) for k, v in stage.items()},
         "host_detail_ms_last_step": dict(zip(["rl_prefetch", "rl_mnp", "rl_del", "rl_ins", "rl_lgins", "record_list",
                                               "key_sort", "order_wait", "aligned_order", "rank_sort", "reduce_events",
-                                              "reduce_clips"], [round(x, 2) for x in st["detail_ms"]])),
+                                              "reduce_clips", "nopass", "row_miss_fetch", "n_nopass", "n_row_miss_fetches"],
+                                             [round(x, 2) for x in st["detail_ms"]])),
         "roofline": {"kernel": "k_walk", "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": round(avg_walk_s * 1e3, 4)},

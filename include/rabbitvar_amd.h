@@ -145,9 +145,11 @@ typedef struct rv_stats {
     double format_ms;          /* host: text */
     double wall_ms;            /* rv_run_region, end to end */
     int64_t order_sorted;      /* 1: output order by the slot sort, 0: by the full map replay */
-    double detail_ms[12];      /* finer host split: realign prefetch, mnp, del, ins, lgins, record list,
+    double detail_ms[16];      /* finer host split: realign prefetch, mnp, del, ins, lgins, record list,
                                   key sort, order-thread wait, alignedVariants order, rank+sort,
-                                  event reduction, clip reduction */
+                                  event reduction, clip reduction, noPassingReads, dense-row fetches on
+                                  a cache miss, then the counts of those two */
+    double call_ms;            /* rv_run_region including the release of its host working state */
 } rv_stats;
 
 void rv_config_default(rv_config* cfg);

@@ -58,7 +58,8 @@ class RvStats(ctypes.Structure):
         ("ref_key_creations", c_int64), ("walk_errors", c_int64), ("dense_rows", c_int64),
         ("event_bytes", c_int64), ("cx_positions", c_int64), ("rows_fetched", c_int64),
         ("reduce_ms", c_double), ("realign_ms", c_double), ("order_ms", c_double), ("format_ms", c_double),
-        ("wall_ms", c_double), ("order_sorted", c_int64), ("detail_ms", c_double * 12),
+        ("wall_ms", c_double), ("order_sorted", c_int64), ("detail_ms", c_double * 16),
+        ("call_ms", c_double),
     ]
 
 

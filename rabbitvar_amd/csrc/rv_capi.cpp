@@ -1,4 +1,5 @@
 // extern "C" entry points (declared in include/rabbitvar_amd.h).
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <new>
@@ -14,6 +15,28 @@ struct rv_engine {
 struct rv_callset {
     rv::CallSet cs;
 };
+
+namespace {
+
+// No C++ exception crosses the C boundary: allocation failures become RV_ERR_OOM, anything else
+// RV_ERR_DEVICE with its message in rv_last_error.
+template <class F>
+rv_status guarded(rv_engine* e, F&& f) {
+    try {
+        return f();
+    } catch (const std::bad_alloc&) {
+        e->e->err = "host allocation failed";
+        return RV_ERR_OOM;
+    } catch (const std::exception& ex) {
+        e->e->err = ex.what();
+        return RV_ERR_DEVICE;
+    } catch (...) {
+        e->e->err = "unknown error";
+        return RV_ERR_DEVICE;
+    }
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -58,7 +81,7 @@ rv_status rv_open(const rv_config* cfg, int device, rv_engine** out) {
     rv_engine* h = new (std::nothrow) rv_engine;
     if (!h) return RV_ERR_OOM;
     h->e = new (std::nothrow) rv::Engine(*cfg, device);
-    if (!h->e || !h->e->stream) {
+    if (!h->e || !h->e->stream || !h->e->err.empty()) {
         delete h->e;
         delete h;
         return RV_ERR_DEVICE;
@@ -73,7 +96,9 @@ rv_status rv_run_region(rv_engine* e, const rv_region* region, const rv_reads* r
     *out = nullptr;
     rv_callset* cs = new (std::nothrow) rv_callset;
     if (!cs) return RV_ERR_OOM;
-    rv_status s = rv::run_region(*e->e, *region, *reads, *contig, cs->cs);
+    const auto t0 = std::chrono::steady_clock::now();
+    rv_status s = guarded(e, [&] { return rv::run_region(*e->e, *region, *reads, *contig, cs->cs); });
+    e->e->stats.call_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (s != RV_OK) {
         delete cs;
         return s;
@@ -110,9 +135,14 @@ rv_status rv_debug_pileup(rv_engine* e, const rv_region* region, const rv_reads*
                           char** text, size_t* len) {
     if (!e || !region || !reads || !contig || !text) return RV_ERR_ARG;
     rv::PileupResult pr;
-    rv_status s = e->e->pileup(*region, *reads, *contig, true, pr);
+    rv_status s = guarded(e, [&] { return e->e->pileup(*region, *reads, *contig, true, pr); });
     if (s != RV_OK) return s;
-    std::string d = rv::dump_pileup(pr.P);
+    std::string d;
+    s = guarded(e, [&] {
+        d = rv::dump_pileup(pr.P);
+        return RV_OK;
+    });
+    if (s != RV_OK) return s;
     char* p = (char*)std::malloc(d.size() + 1);
     if (!p) return RV_ERR_OOM;
     std::memcpy(p, d.c_str(), d.size() + 1);

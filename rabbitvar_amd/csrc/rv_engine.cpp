@@ -60,31 +60,35 @@ Engine::Engine(const rv_config& c, int dev) {
         err = "hipStreamCreate failed";
         return;
     }
-    for (auto& e : ev) hipEventCreate(&e);
-    for (auto& e : sev) hipEventCreate(&e);
+    for (auto& e : ev)
+        if (hipEventCreate(&e) != hipSuccess) err = "hipEventCreate failed";
+    for (auto& e : sev)
+        if (hipEventCreate(&e) != hipSuccess) err = "hipEventCreate failed";
 }
 
 Engine::~Engine() {
+    if (reaper_.joinable()) reaper_.join();
     DevBuf* all[] = {&b_prep,      &b_cig,     &b_ctr,    &b_span,       &b_rows,     &b_ref,    &b_events,
                      &b_arena,     &b_clips,   &b_clip_arena, &b_anchors, &b_counts,   &b_gather_ids, &b_gather_out,
                      &b_cx_pos,    &b_cx_off,  &b_cx,     &b_cx_arena,   &b_cx_cov,   &b_cx_key, &b_out,
                      &b_out_arena, &b_score_ctr, &b_status, &b_keys, &b_keys2, &b_kpos, &b_kpos2, &b_sort_tmp,
-                     &b_kcount, &b_args, &b_kpos3};
+                     &b_kcount, &b_args, &b_kpos3, &b_nopass};
+    // teardown: nothing to report a failure to
     for (auto* b : all)
-        if (b->p) hipFree(b->p);
+        if (b->p) (void)hipFree(b->p);
     for (auto& b : b_reads)
-        if (b.p) hipFree(b.p);
+        if (b.p) (void)hipFree(b.p);
     for (auto& e : ev)
-        if (e) hipEventDestroy(e);
+        if (e) (void)hipEventDestroy(e);
     for (auto& e : sev)
-        if (e) hipEventDestroy(e);
-    if (stream) hipStreamDestroy(stream);
+        if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
 }
 
 void* Engine::grow(DevBuf& b, size_t bytes) {
     if (bytes == 0) bytes = 8;
     if (b.cap >= bytes) return b.p;
-    if (b.p) hipFree(b.p);
+    if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
     size_t want = bytes + bytes / 4;
@@ -196,6 +200,8 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
     hargs.cfg = cfg;
     RV_HIP(hipStreamSynchronize(stream));
     RV_HIP(hipMemcpy(dargs, &hargs, sizeof hargs, hipMemcpyHostToDevice));
+    args_dev = dargs;
+    args_n = n;
     RV_HIP(hipEventRecord(ev[0], stream));
     RV_HIP(launch_prep(dargs, n, prep, cig, ctr, span, stream));
     RV_HIP(hipEventRecord(ev[1], stream));
@@ -284,22 +290,47 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
                                       stream));
             }
             RV_HIP(hipStreamSynchronize(stream));
+            // the previous region's host state is released while this one waits on the device; wait
+            // for that before the reductions allocate
+            wait_release();
             auto r0 = std::chrono::steady_clock::now();
             out.P.dense_lo = W.dense_lo;
             out.P.dense_n = W.dense_n;
             out.P.maxReadLength = hc.max_read_length;
-            reduce_events(evs.data(), evs.size(), arena.data(), cfg.goodq, out.P);
+            // the three reductions write disjoint maps: events -> nd / ins / counts, clips -> sc5, sc3
+            double clip_ms[2] = {0, 0};
+            std::exception_ptr clip_err[2];
+            auto clips = [&](int k) {
+                try {
+                    const auto c0 = std::chrono::steady_clock::now();
+                    reduce_clips(cls.data(), cls.size(), carena.data(), cfg.goodq, out.P, k == 0 ? 5 : 3);
+                    clip_ms[k] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0).count();
+                } catch (...) {
+                    clip_err[k] = std::current_exception();
+                }
+            };
+            std::thread c5(clips, 0), c3(clips, 1);
+            try {
+                reduce_events(evs.data(), evs.size(), arena.data(), cfg.goodq, out.P);
+            } catch (...) {
+                c5.join();
+                c3.join();
+                throw;
+            }
             const auto r1 = std::chrono::steady_clock::now();
-            reduce_clips(cls.data(), cls.size(), carena.data(), cfg.goodq, out.P);
+            c5.join();
+            c3.join();
+            for (auto& ce : clip_err)
+                if (ce) std::rethrow_exception(ce);
             stats.detail_ms[10] = std::chrono::duration<double, std::milli>(r1 - r0).count();
-            stats.detail_ms[11] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - r1).count();
+            stats.detail_ms[11] = std::max(clip_ms[0], clip_ms[1]);
             host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - r0).count();
             out.W = W;
             out.ctr = hc;
-            float t01 = 0, t23 = 0, t34 = 0;
-            hipEventElapsedTime(&t01, ev[0], ev[1]);
-            hipEventElapsedTime(&t23, ev[2], ev[3]);
-            hipEventElapsedTime(&t34, ev[3], ev[4]);
+            float t01 = 0, t23 = 0, t34 = 0;  // stage timings only: a failed query reads as 0
+            (void)hipEventElapsedTime(&t01, ev[0], ev[1]);
+            (void)hipEventElapsedTime(&t23, ev[2], ev[3]);
+            (void)hipEventElapsedTime(&t34, ev[3], ev[4]);
             stats.prep_ms = t01;
             stats.walk_ms = t23;
             stats.anchor_ms = t34;
@@ -370,6 +401,20 @@ rv_status Engine::key_order(int n_reads, std::vector<int32_t>& created, std::vec
     RV_HIP(hipMemcpyAsync(created.data(), pos2, sizeof(int32_t) * n, hipMemcpyDeviceToHost, stream));
     RV_HIP(hipStreamSynchronize(stream));
     return RV_OK;
+}
+
+bool Engine::no_passing(int start, int end, bool& none) {
+    none = true;
+    if (!args_dev || args_n <= 0) return true;
+    unsigned* cnt = (unsigned*)grow(b_nopass, 64);
+    if (!cnt) return false;
+    unsigned h = 0;
+    if (hipMemsetAsync(cnt, 0, sizeof(unsigned), stream) != hipSuccess) return false;
+    if (launch_no_passing(args_dev, args_n, start, end, cnt, stream) != hipSuccess) return false;
+    if (hipMemcpyAsync(&h, cnt, sizeof(unsigned), hipMemcpyDeviceToHost, stream) != hipSuccess) return false;
+    if (hipStreamSynchronize(stream) != hipSuccess) return false;
+    none = h == 0;
+    return true;
 }
 
 bool Engine::fetch_rows(const std::vector<int>& idx, std::vector<DenseRow>& out) {

@@ -3,7 +3,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <memory>
 #include <string>
+#include <thread>
+#include <tuple>
 #include <vector>
 
 #include "rabbitvar_amd.h"
@@ -50,6 +53,8 @@ class Engine {
     void* grow(DevBuf& b, size_t bytes);
     // copies the listed dense rows (indices into the window) of the last pileup to the host
     bool fetch_rows(const std::vector<int>& rows, std::vector<DenseRow>& out);
+    // noPassingReads over the last pileup's records (device-resident): none = no record passes
+    bool no_passing(int start, int end, bool& none);
 
     // nonInsertionVariants keys of the last pileup in creation order, and the same keys in the
     // slot order they take when sorted (see rv_order.h)
@@ -57,7 +62,9 @@ class Engine {
 
     // buffers of the realignment / scoring stages
     DevBuf b_gather_ids, b_gather_out, b_cx_pos, b_cx_off, b_cx, b_cx_arena, b_cx_cov, b_cx_key, b_out, b_out_arena,
-        b_score_ctr, b_status, b_keys, b_keys2, b_kpos, b_kpos2, b_sort_tmp, b_kcount, b_kpos3, b_args;
+        b_score_ctr, b_status, b_keys, b_keys2, b_kpos, b_kpos2, b_sort_tmp, b_kcount, b_kpos3, b_args, b_nopass;
+    WalkArgs* args_dev = nullptr;  // the last pileup's kernel arguments (its reads stay resident)
+    int64_t args_n = 0;
     hipEvent_t sev[2] = {nullptr, nullptr};
     // figures of the last region: record-list positions, dense rows fetched for the
     // realigner, insertions whose coverage came from the next position
@@ -67,7 +74,21 @@ class Engine {
         return s;
     }
 
+    // Destroys the given objects on a background thread, so that releasing a region's host working
+    // state (hundreds of thousands of map nodes) overlaps the caller's next steps.  At most one
+    // release runs at a time; the engine's destructor waits for it.
+    template <class... T>
+    void release_later(T&&... objs) {
+        auto bundle = std::make_unique<std::tuple<std::decay_t<T>...>>(std::forward<T>(objs)...);
+        wait_release();
+        reaper_ = std::thread([b = std::move(bundle)]() mutable { b.reset(); });
+    }
+    void wait_release() {
+        if (reaper_.joinable()) reaper_.join();
+    }
+
   private:
+    std::thread reaper_;
     DevBuf b_reads[13], b_prep, b_cig, b_ctr, b_span, b_rows, b_ref;
     DevBuf b_events, b_arena, b_clips, b_clip_arena, b_anchors, b_counts;
     hipEvent_t ev[8];

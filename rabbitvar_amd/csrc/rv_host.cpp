@@ -264,9 +264,11 @@ void reduce_events(const Event* ev, size_t n, const char* arena, double goodq, H
     }
 }
 
-void reduce_clips(const ClipRec* cl, size_t n, const char* clip_arena, double goodq, HostPileup& P) {
-    std::vector<size_t> idx(n);
-    std::iota(idx.begin(), idx.end(), 0);
+void reduce_clips(const ClipRec* cl, size_t n, const char* clip_arena, double goodq, HostPileup& P, int side) {
+    std::vector<size_t> idx;
+    idx.reserve(n);
+    for (size_t i = 0; i < n; i++)
+        if (side < 0 || cl[i].side == side) idx.push_back(i);
     std::sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
         if (cl[a].side != cl[b].side) return cl[a].side < cl[b].side;
         if (cl[a].pos != cl[b].pos) return cl[a].pos < cl[b].pos;
@@ -288,23 +290,29 @@ void reduce_clips(const ClipRec* cl, size_t n, const char* clip_arena, double go
         else
             v.locnt++;
     };
+    // records arrive sorted by (side, position, read order): each position's accumulator is
+    // appended at the end of its map
     for (size_t k : idx) {
         const ClipRec& r = cl[k];
-        HSclip& s = (r.side == 5 ? P.sc5 : P.sc3)[r.pos];
+        auto& m = r.side == 5 ? P.sc5 : P.sc3;
+        auto it = m.empty() ? m.end() : std::prev(m.end());
+        if (it == m.end() || it->first != r.pos) it = m.emplace_hint(m.end(), r.pos, HSclip{});
+        HSclip& s = it->second;
         const char* bases = clip_arena + r.bases_off;
         const char* quals = bases + r.nhq;
+        if ((int)s.col.size() < r.nhq) s.col.resize((size_t)r.nhq);
         for (int i = 0; i < r.nhq; i++) {
-            char ch = bases[i];
-            auto& lst = s.nt[i];
+            const char ch = bases[i];
+            ClipColumn& c = s.col[(size_t)i];
             bool found = false;
-            for (auto& pr : lst)
+            for (auto& pr : c.nt)
                 if (pr.first == ch) {
                     pr.second++;
                     found = true;
                 }
-            if (!found) lst.push_back({ch, 1});
-            int rp = r.side == 5 ? r.nhq - 1 - i : r.nhq - i;
-            add(s.seq[i][ch], r.dir != 0, rp, (double)(uint8_t)quals[i], r.mapq, r.nm);
+            if (!found) c.nt.push_back({ch, 1});
+            const int rp = r.side == 5 ? r.nhq - 1 - i : r.nhq - i;
+            add(c.seq_at(ch), r.dir != 0, rp, (double)(uint8_t)quals[i], r.mapq, r.nm);
         }
         add(s.v, r.dir != 0, r.cel, r.sumq / (double)r.nhq, r.mapq, r.nm);
     }
@@ -341,15 +349,16 @@ std::string dump_pileup(const HostPileup& P) {
         for (auto& e : m) {
             const HSclip& s = e.second;
             snprintf(buf, sizeof buf, "%s\t%d\t%d\t%d\t%d\t%d\t%d\t%.17g\t%.17g\t%.17g\t%.17g\t%zu\n", tag, e.first, s.v.cnt,
-                     s.v.fwd, s.v.rev, s.v.hicnt, s.v.locnt, s.v.pmean, s.v.qmean, s.v.mqmean, s.v.nm, s.nt.size());
+                     s.v.fwd, s.v.rev, s.v.hicnt, s.v.locnt, s.v.pmean, s.v.qmean, s.v.mqmean, s.v.nm, s.col.size());
             o += buf;
-            for (auto& nt : s.nt) {
-                o += std::string(tag) + "n\t" + std::to_string(e.first) + "\t" + std::to_string(nt.first);
+            for (size_t ci = 0; ci < s.col.size(); ci++) {
+                const auto& nt = s.col[ci].nt;
+                o += std::string(tag) + "n\t" + std::to_string(e.first) + "\t" + std::to_string(ci);
                 std::vector<char> ins;
-                for (auto& pr : nt.second) ins.push_back(pr.first);
+                for (auto& pr : nt) ins.push_back(pr.first);
                 for (char b : rh_char_iteration_order(ins)) {
                     int cnt = 0;
-                    for (auto& pr : nt.second)
+                    for (auto& pr : nt)
                         if (pr.first == b) cnt = pr.second;
                     o += "\t";
                     o += b;

@@ -2,6 +2,7 @@
 // into per-allele records, the soft-clip aggregates, and text helpers.  None of this re-runs a
 // device stage; it consumes what the device produced.
 #pragma once
+#include <algorithm>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -29,10 +30,30 @@ std::vector<char> rh_char_iteration_order(const std::vector<char>& inserted);
 // the same for at most 7 distinct keys, without allocating (out holds n entries)
 int rh_char_order_small(const char* inserted, int n, char* out);
 
+// One clip index of a soft-clip accumulator: the reference's Sclip.nt[i] (base counts, a
+// robin_hood map kept here in insertion order) and Sclip.seq[i] (per-base variations, an ordered
+// map kept here sorted by base).
+struct ClipColumn {
+    std::vector<std::pair<char, int>> nt;   // (base, count) in first-seen order
+    std::vector<std::pair<char, HVar>> seq;  // sorted by base
+    HVar& seq_at(char b) {  // seq[i][b], created when absent
+        auto it = std::lower_bound(seq.begin(), seq.end(), b,
+                                   [](const std::pair<char, HVar>& e, char k) { return e.first < k; });
+        if (it == seq.end() || it->first != b) it = seq.insert(it, {b, HVar{}});
+        return it->second;
+    }
+    const HVar* seq_find(char b) const {
+        for (auto& e : seq)
+            if (e.first == b) return &e.second;
+        return nullptr;
+    }
+};
+
+// Soft-clip accumulator at one position (the reference's Sclip).  Clip records add indices
+// 0 .. len - 1, so the reference's per-index maps are dense here: col[i] is index i.
 struct HSclip {
     HVar v;
-    std::map<int, std::vector<std::pair<char, int>>> nt;  // per clip index: (base, count) in first-seen order
-    std::map<int, std::map<char, HVar>> seq;
+    std::vector<ClipColumn> col;
     std::string sequence;
     bool used = false;
 };
@@ -49,7 +70,8 @@ struct HostPileup {
 
 // Reduces the walk's records in read order (ord, seq) per (map, position, descriptor).
 void reduce_events(const Event* ev, size_t n, const char* arena, double goodq, HostPileup& P);
-void reduce_clips(const ClipRec* cl, size_t n, const char* clip_arena, double goodq, HostPileup& P);
+// side = 5 or 3: only that side's records (its map is written, the other is untouched); -1: both
+void reduce_clips(const ClipRec* cl, size_t n, const char* clip_arena, double goodq, HostPileup& P, int side = -1);
 
 // HVar view of a dense row's base (pileup-stage values)
 HVar dense_to_hvar(const DenseBase& d);

@@ -11,6 +11,7 @@ hipError_t launch_prep(const WalkArgs* A, int64_t n, PrepRec* prep, uint32_t* ci
                        hipStream_t s);
 hipError_t launch_fill_rows(DenseRow* rows, int n, hipStream_t s);
 hipError_t launch_walk(const WalkArgs* A, int64_t n, const PrepRec* prep, const uint32_t* cig, hipStream_t s);
+hipError_t launch_no_passing(const WalkArgs* A, int64_t n, int start, int end, unsigned* count, hipStream_t s);
 hipError_t launch_anchor(const Anchor* a, const unsigned* n_anchors, unsigned cap, DenseRow* rows, int dense_lo,
                          int dense_n, const Config& cfg, hipStream_t s);
 

@@ -2,6 +2,7 @@
 #include "rv_order.h"
 
 #include <algorithm>
+#include <thread>
 #include <unordered_set>
 
 namespace rv {
@@ -125,28 +126,49 @@ std::vector<int32_t> replay_ops(const std::vector<int32_t>& ops, const std::vect
     return out;
 }
 
+void aligned_table(const std::vector<int32_t>& processing, const uint8_t* status, int status_lo, int status_n,
+                   RHOrder& out, unsigned threads) {
+    // the filter reads one status byte per position in a scattered order: split it over threads,
+    // keeping each chunk's order, then replay the operations in order
+    const size_t n = processing.size();
+    const size_t nt = n < 65536 ? 1 : std::max(1u, std::min<unsigned>(threads, 16));
+    std::vector<std::vector<int32_t>> ops(nt);
+    std::vector<std::vector<uint8_t>> tr(nt);
+    auto filter = [&](size_t t) {
+        const size_t a = n * t / nt, b = n * (t + 1) / nt;
+        ops[t].reserve((b - a) / 4);
+        tr[t].reserve((b - a) / 4);
+        for (size_t i = a; i < b; i++) {
+            if (i + 32 < b) {
+                const int k = processing[i + 32] - status_lo;
+                if (k >= 0 && k < status_n) __builtin_prefetch(status + k);
+            }
+            const int32_t p = processing[i];
+            const int k = p - status_lo;
+            const int st = (k >= 0 && k < status_n) ? status[k] : 0;
+            if (st == 0) continue;
+            ops[t].push_back(p);
+            tr[t].push_back(st == 2 ? 1 : 0);
+        }
+    };
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < nt; t++) pool.emplace_back(filter, t);
+    filter(0);
+    for (auto& th : pool) th.join();
+    out = RHOrder();
+    for (size_t t = 0; t < nt; t++) apply_ops(out, ops[t], &tr[t]);
+}
+
 std::vector<int32_t> aligned_order(const std::vector<int32_t>& processing, const uint8_t* status, int status_lo,
                                    int status_n) {
     // alignedVariants holds a few of the positions in a scattered order, so the sorted form rarely
-    // applies (its clusters are long); the replay is as fast as sorting when it would
-    std::vector<int32_t> ops;
-    std::vector<uint8_t> transient;
-    ops.reserve(processing.size() / 4);
-    transient.reserve(processing.size() / 4);
-    const size_t n = processing.size();
-    for (size_t i = 0; i < n; i++) {
-        if (i + 32 < n) {
-            const int k = processing[i + 32] - status_lo;
-            if (k >= 0 && k < status_n) __builtin_prefetch(status + k);
-        }
-        const int32_t p = processing[i];
-        const int k = p - status_lo;
-        const int st = (k >= 0 && k < status_n) ? status[k] : 0;
-        if (st == 0) continue;
-        ops.push_back(p);
-        transient.push_back(st == 2 ? 1 : 0);
-    }
-    return replay_ops(ops, transient);
+    // applies (its clusters are long): the replay is as fast as sorting when it would
+    RHOrder m;
+    aligned_table(processing, status, status_lo, status_n, m, 1);
+    std::vector<int32_t> out;
+    out.reserve(m.size());
+    m.for_each([&](int32_t k) { out.push_back(k); });
+    return out;
 }
 
 std::vector<int32_t> noninsertion_order(const std::vector<int32_t>& created, const std::vector<KeyOp>& log) {

@@ -59,8 +59,12 @@ bool noninsertion_order_sorted(const std::vector<int32_t>& walk_sorted, int walk
 bool noninsertion_order_sorted(const std::vector<int32_t>& walk_sorted, const std::vector<KeyOp>& log,
                                std::vector<int32_t>& out);
 
-// alignedVariants in slot order, given the positions in processing order and, per position, what
-// ToVarsBuilder did with it: status[p - status_lo] = 0 nothing, 1 added, 2 added then erased.
+// alignedVariants as a table, given the positions in processing order and, per position, what
+// ToVarsBuilder did with it: status[p - status_lo] = 0 nothing, 1 added, 2 added then erased.  The
+// positions are filtered on `threads` threads, then the operations are replayed.
+void aligned_table(const std::vector<int32_t>& processing, const uint8_t* status, int status_lo, int status_n,
+                   RHOrder& out, unsigned threads = 1);
+// the same as a list in slot order
 std::vector<int32_t> aligned_order(const std::vector<int32_t>& processing, const uint8_t* status, int status_lo,
                                    int status_n);
 

@@ -64,6 +64,15 @@ __global__ void __launch_bounds__(256) k_gather_rows(const DenseRow* rows, int d
     reinterpret_cast<uint4*>(out)[w] = v;
 }
 
+// noPassingReads over the region's records: counts the records passes_one accepts.  One lane per
+// record; one counter update per wavefront.
+__global__ void __launch_bounds__(256) k_no_passing(const WalkArgs* A, int start, int end, unsigned* count) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const bool passes = i < A->R.n && passes_one(A->R, i, start, end);
+    const unsigned long long m = __ballot(passes);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(count, (unsigned)__popcll(m));
+}
+
 // ---------------------------------------------------------------- launch wrappers
 static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
@@ -81,6 +90,11 @@ hipError_t launch_fill_rows(DenseRow* rows, int n, hipStream_t s) {
 hipError_t launch_walk(const WalkArgs* A, int64_t n, const PrepRec* prep, const uint32_t* cig, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_walk, dim3(grid_for(n, WALK_READS)), dim3(WALK_THREADS), 0, s, A, prep, cig);
+    return hipGetLastError();
+}
+hipError_t launch_no_passing(const WalkArgs* A, int64_t n, int start, int end, unsigned* count, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_no_passing, dim3(grid_for(n, 256)), dim3(256), 0, s, A, start, end, count);
     return hipGetLastError();
 }
 hipError_t launch_anchor(const Anchor* a, const unsigned* n_anchors, unsigned cap, DenseRow* rows, int dense_lo,

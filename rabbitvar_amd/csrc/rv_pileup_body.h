@@ -284,6 +284,28 @@ __device__ inline void walk_one(int64_t r, const ReadsDev& R, const Window& W, c
     w.walk(p.rlen1, p.total_len, flag, tid, mtid, mpos0 + 1, p.reflen, pos0);
 }
 
+// noPassingReads (reference src/VariationRealigner.cpp:1447-1498), one record: true when the record
+// overlaps [start - 1, end) (0-based, as the region query does), carries no CIGAR deletion of
+// exactly end - start bases, and its M + D span reaches more than two bases past both ends.
+__device__ inline bool passes_one(const ReadsDev& R, int64_t i, int start, int end) {
+    const int beg0 = R.pos0[i];
+    if (beg0 >= end) return false;
+    const uint32_t dq = ((uint32_t)(end - start) << 4) | (uint32_t)OP_D;
+    int reflen_all = 0, al = 0;
+    bool has = false;
+    for (int64_t k = R.cig_off[i], k1 = R.cig_off[i + 1]; k < k1; k++) {
+        const uint32_t w = R.cigar[k];
+        const int o = (int)(w & 0xFu), l = (int)(w >> 4);
+        if (o == OP_M || o == OP_D || o == OP_N || o == OP_EQ || o == OP_X) reflen_all += l;
+        if (o == OP_M || o == OP_D) al += l;
+        if (w == dq) has = true;
+    }
+    const int end0 = beg0 + (reflen_all > 0 ? reflen_all : 1);
+    if (end0 <= start - 1 || has) return false;
+    const int rs = beg0 + 1, re = rs + al;
+    return re > end + 2 && rs < start - 2;
+}
+
 // Tile set-up and flush, strided over the workgroup's threads.
 __device__ inline void tile_init(int tid, int nthreads, const Window& W, int tile_lo) {
     const int slo = tile_lo - REF_BEFORE;

@@ -66,10 +66,13 @@ const DenseRow* RowCache::row(int pos) {
     int i = pos - lo_;
     if (i < 0 || i >= n_) return nullptr;
     if (!have(i)) {
+        const auto t0 = std::chrono::steady_clock::now();
         std::vector<int> want;
         for (int k = std::max(0, i - MISS_RADIUS); k <= std::min(n_ - 1, i + MISS_RADIUS); k++)
             if (!have(k)) want.push_back(k);
         if (!load(want)) throw std::runtime_error("dense row fetch failed");
+        miss_fetches++;
+        miss_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     return &store_[(size_t)slot_[(size_t)i]];
 }
@@ -309,19 +312,18 @@ std::string findconseq(HSclip& sc) {
     int total = 0, match = 0;
     std::string seqq;
     bool flag = false;
-    auto si = sc.seq.begin();
-    for (auto& e : sc.nt) {
-        int idx = e.first;
+    for (size_t idx = 0; idx < sc.col.size(); idx++) {
+        ClipColumn& col = sc.col[idx];
         char firsts[8], order[8];
         std::vector<char> big;
-        int nk = (int)e.second.size(), no;
+        int nk = (int)col.nt.size(), no;
         const char* ord;
         if (nk <= 7) {
-            for (int k = 0; k < nk; k++) firsts[k] = e.second[(size_t)k].first;
+            for (int k = 0; k < nk; k++) firsts[k] = col.nt[(size_t)k].first;
             no = rh_char_order_small(firsts, nk, order);
             ord = order;
         } else {
-            for (auto& pr : e.second) big.push_back(pr.first);
+            for (auto& pr : col.nt) big.push_back(pr.first);
             big = rh_char_iteration_order(big);
             no = (int)big.size();
             ord = big.data();
@@ -330,30 +332,22 @@ std::string findconseq(HSclip& sc) {
         double maxQuality = 0;
         char chosen = 0;
         int tt = 0;
-        while (si != sc.seq.end() && si->first < idx) ++si;
-        const bool seq_here = si != sc.seq.end() && si->first == idx;
         for (int oi = 0; oi < no; oi++) {
             const char base = ord[oi];
             int count = 0;
-            for (auto& pr : e.second)
+            for (auto& pr : col.nt)
                 if (pr.first == base) count = pr.second;
             tt += count;
-            double bq = 0;
-            bool has = false;
-            if (seq_here) {
-                auto bj = si->second.find(base);
-                if (bj != si->second.end()) {
-                    has = true;
-                    bq = bj->second.qmean;
-                }
-            }
+            const HVar* sv = col.seq_find(base);
+            const bool has = sv != nullptr;
+            const double bq = has ? sv->qmean : 0;
             if (count > maxCount || (has && bq > maxQuality)) {
                 maxCount = count;
                 chosen = base;
-                maxQuality = has ? bq : sc.seq[idx][base].qmean;
+                maxQuality = has ? bq : col.seq_at(base).qmean;
             }
         }
-        if (idx == 3 && sc.nt.size() >= 6 && tt / (double)sc.v.cnt < 0.2 && tt <= 2) break;
+        if (idx == 3 && sc.col.size() >= 6 && tt / (double)sc.v.cnt < 0.2 && tt <= 2) break;
         if ((tt - maxCount > 2 || maxCount <= tt - maxCount) && maxCount / (double)tt < 0.8) {
             if (flag) break;
             flag = true;
@@ -363,7 +357,7 @@ std::string findconseq(HSclip& sc) {
         if (chosen != 0) seqq += chosen;
     }
     std::string SEQ;
-    size_t ntSize = sc.nt.size();
+    size_t ntSize = sc.col.size();
     size_t sl = seqq.size();
     if (total != 0 && match / (double)total > 0.9 && sl / 1.5 > (double)(ntSize - sl) &&
         (sl / (double)ntSize > 0.8 || ntSize - sl < 10 || sl > 25)) {
@@ -379,6 +373,58 @@ std::string findconseq(HSclip& sc) {
     }
     sc.sequence = SEQ;
     return SEQ;
+}
+
+void find35match(const std::string& seq5, const std::string& seq3, int& b5, int& b3, int& score) {
+    // The reference tries every start (i, j), i in [0, |seq5| - 9], j in [1, |seq3| - 9] (i outer),
+    // and walks n = 0, 1, ... comparing seq5[i + n] with seq3[|seq3| - j - n] (seq5's end counts as
+    // a mismatch) until the third mismatch or either end, and returns the first start whose run has
+    // more than 8 matches, under 10% mismatches, and reaches an end.  Every start on one
+    // anti-diagonal (seq5 index + seq3 index = D) walks the same cells, so each diagonal's
+    // mismatches are listed once and each start is evaluated in O(1); the smallest qualifying
+    // (i, j) is the reference's answer.
+    b5 = b3 = score = 0;
+    const int l5 = (int)seq5.size(), l3 = (int)seq3.size();
+    if (l5 <= 8 || l3 <= 9) return;
+    int bi = INT32_MAX, bj = INT32_MAX, bs = 0;
+    std::vector<int> M;
+    M.reserve((size_t)l5 + 1);
+    // start (i, j) lies on D = i + l3 - j: i <= l5 - 9 and j >= 1 give D <= l5 - 9 + l3 - 1, j <= l3 - 9
+    // gives D >= i + 9 >= 9
+    for (int D = 9; D <= l5 - 9 + l3 - 1; D++) {
+        const int i_lo = std::max(0, D - l3 + 1), i_hi = std::min(l5 - 9, D - 9);
+        if (i_lo > i_hi || i_lo > bi) continue;
+        const int a_hi = std::min(D, l5);  // last cell: seq3 index 0 or seq5's end
+        M.clear();
+        for (int a = i_lo; a <= a_hi; a++)
+            if (a >= l5 || seq3[(size_t)(D - a)] != seq5[(size_t)a]) M.push_back(a);
+        size_t k = 0;
+        for (int i = i_lo; i <= i_hi && i <= bi; i++) {
+            while (k < M.size() && M[k] < i) k++;
+            int n, nm;
+            if (k + 2 < M.size()) {
+                n = M[k + 2] - i;
+                nm = 3;
+            } else {
+                n = a_hi - i + 1;
+                nm = (int)(M.size() - k);
+            }
+            const int j = i + l3 - D;
+            if (n - nm > 8 && nm / (double)n < 0.1 && (n + j >= l3 || i + n >= l5)) {
+                if (i < bi || (i == bi && j < bj)) {
+                    bi = i;
+                    bj = j;
+                    bs = n - nm;
+                }
+                break;  // later starts on this diagonal have larger i
+            }
+        }
+    }
+    if (bi != INT32_MAX) {
+        b5 = bi;
+        b3 = bj;
+        score = bs;
+    }
 }
 
 namespace {
@@ -448,7 +494,7 @@ struct Realigner {
     int rstart, rend;
     HostRef& ref;
     int chrLen;
-    const HostReads* reads;
+    ReadQuery* reads;
     RState& S;
     RealignStats& st;
 
@@ -575,33 +621,17 @@ struct Realigner {
     // noPassingReads (VariationRealigner.cpp:1447-1498): no record of the region spans [start, end]
     // without carrying exactly this deletion
     bool noPassingReads(int start, int end) {
-        if (!reads) throw std::runtime_error("region reads unavailable");
-        if (reads->n > 0 && !reads->pos0) {
-            HostReads& hr = const_cast<HostReads&>(*reads);
-            if (!hr.load || !hr.load(hr)) throw std::runtime_error("region reads could not be copied to the host");
-        }
-        int cnt = 0;
-        uint32_t dq = ((uint32_t)(end - start) << 4) | OP_D;
-        for (int64_t i = 0; i < reads->n; i++) {
-            const uint32_t* cg = reads->cigar + reads->cigar_off[i];
-            int nc = (int)(reads->cigar_off[i + 1] - reads->cigar_off[i]);
-            int reflenAll = 0, al = 0;
-            bool has = false;
-            for (int k = 0; k < nc; k++) {
-                uint32_t w = cg[k];
-                int o = (int)(w & 0xF), l = (int)(w >> 4);
-                if (o == OP_M || o == OP_D || o == OP_N || o == OP_EQ || o == OP_X) reflenAll += l;
-                if (o == OP_M || o == OP_D) al += l;
-                if (w == dq) has = true;
+        const auto t0 = std::chrono::steady_clock::now();
+        struct Acc {
+            RealignStats& st;
+            std::chrono::steady_clock::time_point t0;
+            ~Acc() {
+                st.n_nopass++;
+                st.ms_nopass += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
             }
-            int beg0 = reads->pos0[i], end0 = reads->pos0[i] + (reflenAll > 0 ? reflenAll : 1);
-            if (!(beg0 < end && end0 > start - 1)) continue;
-            if (has) continue;
-            int rs = reads->pos0[i] + 1;
-            int re = rs + al;
-            if (re > end + 2 && rs < start - 2) cnt++;
-        }
-        return cnt <= 0;
+        } acc{st, t0};
+        if (!reads) throw std::runtime_error("region reads unavailable");
+        return reads->no_passing(start, end);
     }
 
     void adjustMNP() {
@@ -969,30 +999,6 @@ struct Realigner {
         }
     }
 
-    // find35match (VariationRealigner.cpp:1396-1428)
-    void find35match(const std::string& seq5, const std::string& seq3, int& b5, int& b3, int& score) {
-        b5 = b3 = score = 0;
-        for (size_t i = 0; i < seq5.size() - 8; i++) {
-            for (size_t j = 1; j < seq3.size() - 8; j++) {
-                int nm = 0, n = 0;
-                const int l3 = (int)seq3.size(), l5 = (int)seq5.size();
-                while (n + (int)j <= l3 && (int)i + n <= l5) {
-                    // vc_substr(seq3, -j-n, 1) vs vc_substr(seq5, i+n, 1); the latter is empty at the end
-                    if ((int)i + n >= l5 || seq3[(size_t)(l3 - (int)j - n)] != seq5[(size_t)((int)i + n)]) nm++;
-                    if (nm > 2) break;
-                    n++;
-                }
-                if (n - nm > score && n - nm > 8 && nm / (double)n < 0.1 &&
-                    (n + (int)j >= (int)seq3.size() || (int)i + n >= (int)seq5.size())) {
-                    score = n - nm;
-                    b3 = (int)j;
-                    b5 = (int)i;
-                    return;
-                }
-            }
-        }
-    }
-
     void realignlgins30() {
         HostRef rf = ref;  // the reference copies the whole map (VariationRealigner.cpp:1120)
         struct SP {
@@ -1203,7 +1209,7 @@ void build_record_list(RState& S, CxHost& cx) {
 }
 
 
-void realign(const Config& c, int region_start, int region_end, HostRef& ref, int chrLen, const HostReads* reads,
+void realign(const Config& c, int region_start, int region_end, HostRef& ref, int chrLen, ReadQuery* reads,
              RState& S, RealignStats& st) {
     // prefetch, in one transfer, the rows the candidates' own handlers read first: the candidate
     // positions, the few bases after them, and every non-dense / insertion position (read again

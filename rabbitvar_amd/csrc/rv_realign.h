@@ -49,6 +49,8 @@ class RowCache {
     bool prefetch(const std::vector<int>& positions, int radius);
     const DenseRow* row(int pos);  // nullptr outside the dense window
     int64_t fetches = 0, rows_fetched = 0;
+    int64_t miss_fetches = 0;
+    double miss_ms = 0;  // time in fetches made on a miss (not by prefetch)
 
   private:
     bool load(std::vector<int>& rows);
@@ -70,15 +72,14 @@ struct HostRef {
     char get(int k);
 };
 
-// Region reads for noPassingReads (host arrays).  When the batch is device-resident the arrays
-// are copied on first use through `load`.
-struct HostReads {
-    int64_t n = 0;
-    const int32_t* pos0 = nullptr;
-    const int64_t* cigar_off = nullptr;
-    const uint32_t* cigar = nullptr;
-    std::function<bool(HostReads&)> load;
+// noPassingReads (reference src/VariationRealigner.cpp:1447-1498) over the region's records: true
+// when no record overlapping [start, end] spans it without carrying exactly this deletion.
+class ReadQuery {
+  public:
+    virtual ~ReadQuery() = default;
+    virtual bool no_passing(int start, int end) = 0;
 };
+
 
 struct RPos {
     bool loaded = false;
@@ -114,10 +115,12 @@ class RState {
 struct RealignStats {
     int64_t candidates = 0, touched = 0, changed = 0;
     double ms_prefetch = 0, ms_mnp = 0, ms_del = 0, ms_ins = 0, ms_lgins = 0;
+    double ms_nopass = 0;  // noPassingReads, inside the handlers' times
+    int64_t n_nopass = 0;
 };
 
 // adjustMNP, realigndel, realignins, realignlgins30 in the reference's order.
-void realign(const Config& c, int region_start, int region_end, HostRef& ref, int chrLen, const HostReads* reads,
+void realign(const Config& c, int region_start, int region_end, HostRef& ref, int chrLen, ReadQuery* reads,
              RState& S, RealignStats& st);
 
 // Positions the scoring stage reads from a record list instead of the dense rows: every
@@ -133,5 +136,8 @@ void build_record_list(RState& S, CxHost& cx);
 
 // soft-clip consensus (include/VariationUtils.h:403-509)
 std::string findconseq(HSclip& sc);
+// find35match (reference src/VariationRealigner.cpp:1396-1428): where the 5' clip consensus and the
+// reversed 3' clip consensus overlap; score 0 when they do not
+void find35match(const std::string& seq5, const std::string& seq3, int& b5, int& b3, int& score);
 
 }  // namespace rv

@@ -27,6 +27,20 @@ class DeviceRowSource : public RowSource {
     Engine& e_;
 };
 
+// noPassingReads on the device over the pileup's records (they stay resident in HBM)
+class DeviceReadQuery : public ReadQuery {
+  public:
+    explicit DeviceReadQuery(Engine& e) : e_(e) {}
+    bool no_passing(int start, int end) override {
+        bool none = true;
+        if (!e_.no_passing(start, end, none)) throw std::runtime_error("noPassingReads on the device failed");
+        return none;
+    }
+
+  private:
+    Engine& e_;
+};
+
 }  // namespace
 
 #define RV_HIPS(x)                                                                                   \
@@ -84,35 +98,10 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     ref.seq = pr.ref_win.data();
     ref.lo = pr.W.ref_lo;
     ref.hi = pr.W.ref_lo + (int)pr.ref_win.size() - 1;
-    std::vector<int32_t> h_pos0;
-    std::vector<int64_t> h_coff;
-    std::vector<uint32_t> h_cig;
-    HostReads hr;
-    hr.n = reads.n_reads;
-    if (!reads.on_device) {
-        hr.pos0 = reads.pos0;
-        hr.cigar_off = reads.cigar_off;
-        hr.cigar = reads.cigar;
-    } else if (reads.n_reads > 0) {
-        hr.load = [&](HostReads& h) {
-            h_pos0.resize((size_t)reads.n_reads);
-            h_coff.resize((size_t)reads.n_reads + 1);
-            if (hipMemcpy(h_pos0.data(), reads.pos0, 4 * h_pos0.size(), hipMemcpyDeviceToHost) != hipSuccess) return false;
-            if (hipMemcpy(h_coff.data(), reads.cigar_off, 8 * h_coff.size(), hipMemcpyDeviceToHost) != hipSuccess)
-                return false;
-            h_cig.resize((size_t)h_coff.back());
-            if (!h_cig.empty() &&
-                hipMemcpy(h_cig.data(), reads.cigar, 4 * h_cig.size(), hipMemcpyDeviceToHost) != hipSuccess)
-                return false;
-            h.pos0 = h_pos0.data();
-            h.cigar_off = h_coff.data();
-            h.cigar = h_cig.data();
-            return true;
-        };
-    }
+    DeviceReadQuery rq(e);
     RealignStats rst;
     try {
-        realign(e.cfg, region.start, region.end, ref, contig.length, &hr, S, rst);
+        realign(e.cfg, region.start, region.end, ref, contig.length, &rq, S, rst);
     } catch (const std::exception& ex) {
         return e.fail(RV_ERR_DEVICE, std::string("realignment: ") + ex.what());
     }
@@ -121,6 +110,10 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     e.stats.detail_ms[2] = rst.ms_del;
     e.stats.detail_ms[3] = rst.ms_ins;
     e.stats.detail_ms[4] = rst.ms_lgins;
+    e.stats.detail_ms[12] = rst.ms_nopass;
+    e.stats.detail_ms[13] = rc.miss_ms;
+    e.stats.detail_ms[14] = (double)rst.n_nopass;
+    e.stats.detail_ms[15] = (double)rc.miss_fetches;
     const auto c0 = std::chrono::steady_clock::now();
     CxHost cx;
     try {
@@ -211,88 +204,130 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
         max_out = std::max<unsigned>(max_out, n_out) * 2;
         arena_cap = std::max(arena_cap, used) * 2;
     }
-    // ---- text, in the printer's order: alignedVariants slot order, then the variants' order
+    // ---- text, in the printer's order: alignedVariants slot order, then the variants' order.  A
+    // second thread works out that order while this one formats the lines in record order; the
+    // lines are then joined in printing order.
     auto t0 = std::chrono::steady_clock::now();
     replay.join();
-    const auto o1 = std::chrono::steady_clock::now();
-    e.stats.detail_ms[7] = std::chrono::duration<double, std::milli>(o1 - t0).count();
+    e.stats.detail_ms[7] = ms_since(t0);
     if (replay_err) return e.fail(RV_ERR_DEVICE, "key order replay failed");
-    std::vector<int32_t> proc;
-    if (replayed || !noninsertion_order_sorted(walk_sorted, walk_maxd, S.key_log, proc)) {
-        if (!replayed) replay_created(nonins, created);
-        proc = replay_finish(nonins, S.key_log);
-    }
     e.stats.order_sorted = replayed ? 0 : 1;
-    const auto a0 = std::chrono::steady_clock::now();
-    // one status per window position: record-list positions override their dense rows
-    for (size_t k = 0; k < cx.pos.size(); k++) {
-        const int i = cx.pos[k] - pr.P.dense_lo;
-        if (i >= 0 && i < pr.P.dense_n) status[(size_t)i] = status[(size_t)pr.P.dense_n + k];
-    }
-    std::vector<int32_t> aligned = aligned_order(proc, status.data(), pr.P.dense_lo, pr.P.dense_n);
-    e.stats.detail_ms[8] = ms_since(a0);
-    const auto k1 = std::chrono::steady_clock::now();
-    {
-        std::vector<int32_t> rank((size_t)std::max(pr.P.dense_n, 1), INT32_MAX);
-        for (size_t i = 0; i < aligned.size(); i++) {
-            const int k = aligned[i] - pr.P.dense_lo;
-            if (k >= 0 && k < pr.P.dense_n) rank[(size_t)k] = (int32_t)i;
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t n = recs.size();
+    std::vector<uint32_t> perm(n);  // record indices in printing order
+    RHOrder aligned;
+    std::vector<int32_t> proc;
+    std::exception_ptr order_err;
+    double order_thread_ms = 0;
+    std::thread order_th([&] {
+        try {
+            const auto q0 = std::chrono::steady_clock::now();
+            if (replayed || !noninsertion_order_sorted(walk_sorted, walk_maxd, S.key_log, proc)) {
+                if (!replayed) replay_created(nonins, created);
+                proc = replay_finish(nonins, S.key_log);
+            }
+            // one status per window position: record-list positions override their dense rows
+            for (size_t k = 0; k < cx.pos.size(); k++) {
+                const int i = cx.pos[k] - pr.P.dense_lo;
+                if (i >= 0 && i < pr.P.dense_n) status[(size_t)i] = status[(size_t)pr.P.dense_n + k];
+            }
+            aligned_table(proc, status.data(), pr.P.dense_lo, pr.P.dense_n, aligned, std::min(hw, 8u));
+            const auto q1 = std::chrono::steady_clock::now();
+            e.stats.detail_ms[8] = std::chrono::duration<double, std::milli>(q1 - q0).count();
+            // iteration order is slot order: a record's rank is its start position's slot
+            std::vector<int64_t> slot(n);
+            for (size_t i = 0; i < n; i++) {
+                const int64_t sl = aligned.find_slot(recs[i].position);
+                slot[i] = sl < 0 ? INT64_MAX : sl;
+                perm[i] = (uint32_t)i;
+            }
+            std::sort(perm.begin(), perm.end(), [&](uint32_t x, uint32_t y) {
+                if (slot[x] != slot[y]) return slot[x] < slot[y];
+                if (recs[x].position != recs[y].position) return recs[x].position < recs[y].position;
+                return recs[x].order < recs[y].order;
+            });
+            e.stats.detail_ms[9] = ms_since(q1);
+            order_thread_ms = ms_since(q0);
+        } catch (...) {
+            order_err = std::current_exception();
         }
-        for (OutRec& o : recs) {
-            const int k = o.position - pr.P.dense_lo;
-            o.start_rank = (k >= 0 && k < pr.P.dense_n) ? rank[(size_t)k] : INT32_MAX;
-        }
-    }
-    std::sort(recs.begin(), recs.end(), [](const OutRec& a, const OutRec& b) {
-        if (a.start_rank != b.start_rank) return a.start_rank < b.start_rank;
-        if (a.position != b.position) return a.position < b.position;
-        return a.order < b.order;
     });
-    e.stats.detail_ms[9] = ms_since(k1);
-    order_ms += ms_since(t0);
+    struct OrderJoiner {
+        std::thread& t;
+        ~OrderJoiner() {
+            if (t.joinable()) t.join();
+        }
+    } order_joiner{order_th};
     const auto f0 = std::chrono::steady_clock::now();
     LineContext ctx{e.sample.c_str(), region.gene ? region.gene : "", region.chr ? region.chr : "", region.start, region.end,
                     e.cfg.fisher != 0};
-    out.text.clear();
-    out.calls.clear();
-    out.strings.clear();
-    out.strings.resize(recs.size());
-    out.calls.reserve(recs.size());
-    {
-        // lines are independent: format contiguous chunks on several threads, join in order
-        const size_t n = recs.size();
-        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-        const size_t nt = n < 4096 ? 1 : std::min<size_t>(std::min(hw, 16u), n / 2048);
-        std::vector<std::string> parts(nt);
-        auto work = [&](size_t t) {
-            const size_t a = n * t / nt, b = n * (t + 1) / nt;
-            std::string& txt = parts[t];
-            txt.reserve((b - a) * 200);
-            for (size_t i = a; i < b; i++) {
-                const OutRec& o = recs[i];
-                format_line(txt, o, arena.data(), ctx);
-                CallStrings& cs = out.strings[i];
-                const char* p = arena.data() + o.str_off;
-                std::string* dst[6] = {&cs.refallele, &cs.varallele, &cs.genotype, &cs.bias, &cs.leftseq, &cs.rightseq};
-                for (int k = 0; k < 6; k++) {
-                    dst[k]->assign(p, o.len[k]);
-                    p += o.len[k];
-                }
-                cs.vartype = vartype_name(o, arena.data());
+    // lines are independent: format contiguous chunks of records on several threads
+    const size_t nt = n < 4096 ? 1 : std::min<size_t>(std::min(hw, 16u), n / 2048);
+    std::vector<std::string> parts(nt);
+    std::vector<uint64_t> line_at(n + 1);  // (chunk << 40) | offset of record i's line in its chunk
+    std::vector<CallStrings> strs(n);
+    auto work = [&](size_t t) {
+        const size_t a = n * t / nt, b = n * (t + 1) / nt;
+        std::string& txt = parts[t];
+        txt.reserve((b - a) * 200);
+        for (size_t i = a; i < b; i++) {
+            const OutRec& o = recs[i];
+            line_at[i] = ((uint64_t)t << 40) | (uint64_t)txt.size();
+            format_line(txt, o, arena.data(), ctx);
+            CallStrings& cs = strs[i];
+            const char* p = arena.data() + o.str_off;
+            std::string* dst[6] = {&cs.refallele, &cs.varallele, &cs.genotype, &cs.bias, &cs.leftseq, &cs.rightseq};
+            for (int k = 0; k < 6; k++) {
+                dst[k]->assign(p, o.len[k]);
+                p += o.len[k];
             }
-        };
+            cs.vartype = vartype_name(o, arena.data());
+        }
+    };
+    {
         std::vector<std::thread> pool;
         for (size_t t = 1; t < nt; t++) pool.emplace_back(work, t);
         if (nt > 0) work(0);
         for (auto& th : pool) th.join();
-        size_t total = 0;
-        for (auto& p : parts) total += p.size();
-        out.text.reserve(total);
-        for (auto& p : parts) out.text += p;
     }
-    for (size_t i = 0; i < recs.size(); i++) {
-        const OutRec& o = recs[i];
-        const CallStrings& cs = out.strings[i];
+    const double format_ms = ms_since(f0);
+    order_th.join();
+    if (order_err) {
+        try {
+            std::rethrow_exception(order_err);
+        } catch (const std::exception& ex) {
+            return e.fail(RV_ERR_DEVICE, std::string("output order: ") + ex.what());
+        }
+    }
+    order_ms += order_thread_ms;
+    // join the lines in printing order
+    auto line_len = [&](size_t i) {
+        const size_t t = (size_t)(line_at[i] >> 40);
+        const size_t off = (size_t)(line_at[i] & ((1ull << 40) - 1));
+        const size_t b = n * (t + 1) / nt;  // first record of the next chunk
+        size_t end = parts[t].size();
+        if (i + 1 < b) end = (size_t)(line_at[i + 1] & ((1ull << 40) - 1));
+        return std::make_pair(parts[t].data() + off, end - off);
+    };
+    out.text.clear();
+    out.calls.clear();
+    out.strings.clear();
+    {
+        size_t total = 0;
+        for (auto& pt : parts) total += pt.size();
+        out.text.reserve(total);
+    }
+    out.strings.resize(n);
+    out.calls.reserve(n);
+    for (size_t k = 0; k < n; k++) {
+        const size_t i = perm[k];
+        const auto ln = line_len(i);
+        out.text.append(ln.first, ln.second);
+        out.strings[k] = std::move(strs[i]);
+    }
+    for (size_t k = 0; k < n; k++) {
+        const OutRec& o = recs[perm[k]];
+        const CallStrings& cs = out.strings[k];
         rv_call c;
         c.position = o.position;
         c.start = o.start;
@@ -332,9 +367,8 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     e.stats.realign_ms = realign_ms;
     e.stats.order_ms = order_ms;
-    e.stats.format_ms = ms_since(f0);
+    e.stats.format_ms = format_ms;
     e.stats.wall_ms = ms_since(w0);
-    (void)o1;
     e.stats.score_ms = score_ms;
     e.stats.host_ms += host_ms;
     e.stats.total_device_ms = e.stats.prep_ms + e.stats.walk_ms + e.stats.anchor_ms + score_ms;
@@ -343,6 +377,10 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     e.last_cx_positions = ncx;
     e.last_rows_fetched = rc.rows_fetched;
     e.last_insertion_side_effects = (int64_t)hc[3];
+    e.release_later(std::move(S), std::move(rc), std::move(pr), std::move(cx), std::move(created),
+                    std::move(walk_sorted), std::move(nonins), std::move(recs), std::move(arena), std::move(status),
+                    std::move(proc), std::move(aligned), std::move(parts), std::move(line_at), std::move(strs),
+                    std::move(perm));
     return RV_OK;
 }
 

@@ -81,6 +81,20 @@ class RHOrder {
         __builtin_prefetch(&keys_[idx]);
     }
 
+    // slot of `key` (iteration order is slot order), -1 when absent (robin_hood findIdx)
+    int64_t find_slot(int32_t key) const {
+        if (info_.empty()) return -1;
+        size_t idx;
+        uint32_t info;
+        place(key, idx, info);
+        do {
+            if (info == info_[idx] && keys_[idx] == key) return (int64_t)idx;
+            ++idx;
+            info += inc_;
+        } while (info <= info_[idx]);
+        return -1;
+    }
+
     // keys in iteration (slot) order
     template <class F>
     void for_each(F&& f) const {

@@ -1,3 +1,4 @@
+#include <cstdlib>
 // Test harness only: runs the engine's device stages sequentially on the CPU (prep, walk, anchor,
 // then the host reduction and realigner, then one score_position call per position exactly as the
 // two scoring kernels map threads to positions) and prints the simple-mode lines.
@@ -53,6 +54,63 @@ int main(int argc, char** argv) {
             if (op[0] == 'e') m.erase((int32_t)key);
         }
         m.for_each([](int32_t k) { printf("%d\n", k); });
+        return 0;
+    }
+    if (argc >= 4 && std::string(argv[1]) == "--find35-check") {
+        // find35match against a direct restatement of the reference's nested loops, on random
+        // clip consensus pairs (half of them built to overlap).  Prints "pairs matched equal".
+        const int trials = atoi(argv[2]);
+        unsigned seed = (unsigned)atoi(argv[3]);
+        auto rnd = [&]() {
+            seed = seed * 1664525u + 1013904223u;
+            return seed >> 8;
+        };
+        auto brute = [](const std::string& seq5, const std::string& seq3, int& b5, int& b3, int& score) {
+            b5 = b3 = score = 0;
+            for (size_t i = 0; i < seq5.size() - 8; i++) {
+                for (size_t j = 1; j < seq3.size() - 8; j++) {
+                    int nm = 0, n = 0;
+                    const int l3 = (int)seq3.size(), l5 = (int)seq5.size();
+                    while (n + (int)j <= l3 && (int)i + n <= l5) {
+                        if ((int)i + n >= l5 || seq3[(size_t)(l3 - (int)j - n)] != seq5[(size_t)((int)i + n)]) nm++;
+                        if (nm > 2) break;
+                        n++;
+                    }
+                    if (n - nm > score && n - nm > 8 && nm / (double)n < 0.1 &&
+                        (n + (int)j >= (int)seq3.size() || (int)i + n >= (int)seq5.size())) {
+                        score = n - nm;
+                        b3 = (int)j;
+                        b5 = (int)i;
+                        return;
+                    }
+                }
+            }
+        };
+        const char* AL = "ACGT";
+        int matched = 0, equal = 0;
+        for (int t = 0; t < trials; t++) {
+            const int l5 = 11 + (int)(rnd() % 150), l3 = 11 + (int)(rnd() % 150);
+            std::string s5, s3;
+            for (int k = 0; k < l5; k++) s5 += AL[rnd() % (t % 7 == 0 ? 2 : 4)];
+            if (rnd() % 2) {
+                // s3 = reversed stretch of s5 (optionally shifted / mutated), padded to l3
+                const int from = (int)(rnd() % (unsigned)l5);
+                for (int k = from; k >= 0 && (int)s3.size() < l3; k--) s3 += s5[(size_t)k];
+                std::reverse(s3.begin(), s3.end());
+                while ((int)s3.size() < l3) s3.insert(s3.begin(), AL[rnd() % 4]);
+                const int muts = (int)(rnd() % 4);
+                for (int k = 0; k < muts; k++) s3[rnd() % s3.size()] = AL[rnd() % 4];
+            } else {
+                for (int k = 0; k < l3; k++) s3 += AL[rnd() % (t % 7 == 0 ? 2 : 4)];
+            }
+            int a5, a3, as, c5, c3, cs;
+            brute(s5, s3, a5, a3, as);
+            find35match(s5, s3, c5, c3, cs);
+            if (as != 0) matched++;
+            if (a5 == c5 && a3 == c3 && as == cs) equal++;
+            else fprintf(stderr, "differs: %s %s: (%d %d %d) vs (%d %d %d)\n", s5.c_str(), s3.c_str(), a5, a3, as, c5, c3, cs);
+        }
+        printf("%d %d %d\n", trials, matched, equal);
         return 0;
     }
     if (argc >= 4 && std::string(argv[1]) == "--order-check") {
@@ -201,13 +259,33 @@ int main(int argc, char** argv) {
     ref.seq = win.data();
     ref.lo = W.ref_lo;
     ref.hi = W.ref_hi;
-    HostReads hr;
-    hr.n = m;
-    hr.pos0 = P0.data();
-    hr.cigar_off = CO.data();
-    hr.cigar = CG.data();
+    // noPassingReads through the device kernel's per-record predicate, one record at a time
+    struct SimReadQuery : ReadQuery {
+        const ReadsDev& R;
+        explicit SimReadQuery(const ReadsDev& r) : R(r) {}
+        bool no_passing(int start, int end) override {
+            for (int64_t i = 0; i < R.n; i++)
+                if (passes_one(R, i, start, end)) return false;
+            return true;
+        }
+    } rq(R);
+    // RV_SIM_REALIGN_REPS=N: realign N extra times on copies of the state first (profiling)
+    if (const char* reps = getenv("RV_SIM_REALIGN_REPS")) {
+        for (int k = 0; k < atoi(reps); k++) {
+            HostPileup hp2 = HP;
+            HostRowSource src2(hp2);
+            RowCache rc2(&src2, hp2.dense_lo, hp2.dense_n);
+            RState rs2(hp2, rc2);
+            HostRef ref2;
+            ref2.seq = win.data();
+            ref2.lo = W.ref_lo;
+            ref2.hi = W.ref_hi;
+            RealignStats st2;
+            realign(cfg, rs, re, ref2, chrLen, &rq, rs2, st2);
+        }
+    }
     RealignStats rst;
-    realign(cfg, rs, re, ref, chrLen, &hr, RS, rst);
+    realign(cfg, rs, re, ref, chrLen, &rq, RS, rst);
     CxHost cx;
     build_record_list(RS, cx);
     ScoreIn in{};
@@ -300,6 +378,11 @@ int main(int argc, char** argv) {
     for (auto& r : out) format_line(text, r, oarena.data(), ctx);
     std::ofstream o(outp);
     o << text;
+    fprintf(stderr, "realign sizes: mnp %zu del %zu ins %zu nd %zu insv %zu sc5 %zu sc3 %zu rows_fetched %lld fetches %lld touched %zu\n",
+            HP.mnp.size(), HP.delCount.size(), HP.insCount.size(), HP.nd.size(), HP.ins.size(), HP.sc5.size(),
+            HP.sc3.size(), (long long)rc.rows_fetched, (long long)rc.fetches, RS.pos.size());
+    fprintf(stderr, "realign ms: noPassingReads %.1f (%lld calls), row misses %.1f (%lld)\n", rst.ms_nopass,
+            (long long)rst.n_nopass, rc.miss_ms, (long long)rc.miss_fetches);
     fprintf(stderr, "realign ms: prefetch %.1f mnp %.1f del %.1f ins %.1f lgins %.1f\n", rst.ms_prefetch, rst.ms_mnp,
             rst.ms_del, rst.ms_ins, rst.ms_lgins);
     fprintf(stderr, "reads %lld events %u clips %u walk_errors %llu overflow %llu capacity %llu\n", (long long)m, ne, ncl,

@@ -106,3 +106,12 @@ def test_sorted_slot_order_with_realigner_log(sim, n, seed):
     r = subprocess.run([sim, "--order-check", str(n), str(seed)], capture_output=True, text=True, check=True)
     applied, equal = (int(x) for x in r.stdout.split())
     assert applied == equal and applied > 0
+
+
+def test_find35match_against_nested_loops(sim):
+    """find35match's per-diagonal evaluation against the reference's nested loops
+    (src/VariationRealigner.cpp:1396-1428) restated directly, on random clip consensus pairs of
+    11-160 bases, half of them built to overlap: the same (b5, b3, score) for every pair."""
+    r = subprocess.run([sim, "--find35-check", "6000", "11"], capture_output=True, text=True, check=True)
+    trials, matched, equal = (int(x) for x in r.stdout.split())
+    assert equal == trials and matched > 100
