@@ -261,6 +261,7 @@ __device__ inline void walk_one(int64_t r, const ReadsDev& R, const Window& W, c
     w.seq = gseq;
     w.qual = gqual;
     w.stage = st;
+    w.strs = walk_str[wave];
     w.staged = staged;
     w.lseq = p.lseq;
     {

@@ -42,11 +42,11 @@ struct TileAcc {
 // the walk kernel's tile, addressed directly so that its accesses are LDS instructions
 static __shared__ TileAcc walk_tile;
 
-// The walk kernel's workgroup: WALK_THREADS lanes (12 wavefronts, 3 per SIMD at the register count
-// the walk needs) walk WALK_READS consecutive reads into a tile starting TILE_MARGIN bases before
-// the first read.
-constexpr int WALK_THREADS = 768;
-constexpr int WALK_READS = 48;
+// The walk kernel's workgroup: WALK_THREADS lanes (16 wavefronts, 4 per SIMD: the walk fits in 128
+// registers, and the tile, read copies and strings in one CU's LDS) walk WALK_READS consecutive
+// reads into a tile starting TILE_MARGIN bases before the first read.
+constexpr int WALK_THREADS = 1024;
+constexpr int WALK_READS = 64;
 constexpr int TILE_MARGIN = 64;
 
 // Per-wavefront copy of the read being walked (bases, qualities, rewritten CIGAR), loaded with one
@@ -60,6 +60,13 @@ struct WaveStage {
     uint32_t cig[MAXC];
 };
 static __shared__ WaveStage walk_stage[WALK_MAX_WAVES];
+
+// Per-wavefront allele descriptor strings.  The walker's control flow is lane-uniform, so one copy
+// per wavefront serves every lane (per-lane copies would live in scratch memory, 64 times over).
+// At most WALK_STRS are live at once on any path: match_run (s, ss, t), insertion (desc, t / key,
+// d), deletion (desc), rescue (d).
+constexpr int WALK_STRS = 3;
+static __shared__ DStr walk_str[WALK_THREADS / 64][WALK_STRS];
 static_assert(WALK_THREADS / 64 <= WALK_MAX_WAVES, "one read copy per wavefront");
 
 // Per-workgroup copy of the reference bases around the tile: [tile_lo - REF_BEFORE,
@@ -135,6 +142,7 @@ struct Walker {
     uint8_t qual_before0;
     int lseq;
     WaveStage* stage;    // this wavefront's copy of the read
+    DStr* strs;          // this wavefront's descriptor strings (walk_str)
     bool staged;         // bases and qualities are in stage (else read through seq / qual)
     int rec_n;           // valid emulated words
     int n_cigar;
@@ -164,6 +172,13 @@ struct Walker {
         return 0;
     }
     __device__ __forceinline__ char sq(int i) const { return (i >= 0 && i < lseq) ? base_at(i) : '\0'; }
+    // dst = src (distinct strings): the characters are copied lane-parallel
+    __device__ __forceinline__ void copy_str(DStr& dst, const DStr& src) {
+        for (int i = lane; i < src.n; i += warpSize) dst.c[i] = src.c[i];
+        dst.n = src.n;
+        dst.overflow = src.overflow;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
     __device__ __forceinline__ void sub(DStr& d, int p, int n) {
         if (p < 0 || p > lseq) {
             bad = true;
@@ -454,7 +469,7 @@ struct Walker {
         if (b >= 0) {
             dense_addcnt(pos, b, rp, qv);
         } else {
-            DStr d;
+            DStr& d = strs[0];
             d.clear();
             d.push(rb);
             event(2, pos, d, 0, (double)qv, nm, 0, rp, (double)qv >= (double)(float)c->goodq);
@@ -513,7 +528,7 @@ struct Walker {
             rpi += cel;
             return ci;
         }
-        DStr desc;
+        DStr& desc = strs[0];
         desc.clear();
         sub(desc, rpi, cel);
         long qs = 0;
@@ -586,15 +601,15 @@ struct Walker {
                 }
                 if (n > 1) {
                     // rotate right by n-1: last n-1 characters move to the front
-                    DStr t;
+                    DStr& t = strs[1];
                     t.clear();
                     t.append(desc.c + (L - (n - 1)), n - 1);
                     t.append(desc.c, L - (n - 1));
-                    desc = t;
+                    copy_str(desc, t);
                 }
                 if (rpi - 1 - (start - 1 - bi) > 0) inspos = bi;
             }
-            DStr key;
+            DStr& key = strs[1];
             key.clear();
             key.push('+');
             key.append(desc);
@@ -624,7 +639,7 @@ struct Walker {
                         atomicAdd(&S->ctr->overflow, 1ull);
                     }
                 } else if (ab != 0) {
-                    DStr d;
+                    DStr& d = strs[2];
                     d.clear();
                     d.push(ab);
                     event(3, inspos, d, tp, (double)q(index), nm - nmoff, 0, 0, (double)q(index) >= c->goodq);
@@ -643,7 +658,7 @@ struct Walker {
             rpi += cel;
             return ci;
         }
-        DStr desc;
+        DStr& desc = strs[0];
         desc.clear();
         desc.push('-');
         desc.append_int(cel);
@@ -818,14 +833,14 @@ struct Walker {
                 i++;
                 continue;
             }
-            DStr s;
+            DStr& s = strs[0];
             s.clear();
             s.push(ch1);
             bool swd = false;
             double qv = q(rpi);
             int qbases = 1, qibases = 0;
             int ss_n = 0;
-            DStr ss;
+            DStr& ss = strs[1];
             ss.clear();
             while ((start + 1) >= Rs && (start + 1) <= Re && (i + 1) < cel && qv >= c->goodq && ref.hasNe(start, sq(rpi)) &&
                    ref.at(start) != 'N') {
@@ -883,13 +898,13 @@ struct Walker {
                 }
                 int amp = s.find('&');
                 if (amp >= 0) s.erase_at(amp);
-                DStr t;
+                DStr& t = strs[2];
                 t.clear();
                 t.push('-');
                 t.append_int(len(ci + 1));
                 t.push('&');
                 t.append(s);
-                s = t;
+                copy_str(s, t);
                 swd = true;
                 ddlen = len(ci + 1);
                 ci += 1;
