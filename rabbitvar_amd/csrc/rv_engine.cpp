@@ -107,7 +107,7 @@ static void window_bounds(const Config& c, const rv_region& r, int chrLen, int& 
 }
 
 rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_contig& contig, bool download_rows,
-                         PileupResult& out) {
+                         PileupResult& out, const PileupHooks* hooks) {
     if (cfg.removeDuplicatedReads) return fail(RV_ERR_UNSUPPORTED, "-t (duplicate removal) is not implemented");
     if (cfg.crisprCuttingSite != 0) return fail(RV_ERR_UNSUPPORTED, "CRISPR mode is not implemented");
     auto h0 = std::chrono::steady_clock::now();
@@ -275,6 +275,7 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
         bool over = counts[0] > max_events || aused > arena_cap || counts[1] > max_clips || cused > clip_cap ||
                     counts[2] > max_anchors;
         if (!over && hc.overflow == 0) {
+            if (hooks && hooks->after_walk) hooks->after_walk();
             // ---- copy the ordered records back and reduce them
             std::vector<Event> evs(counts[0]);
             std::vector<char> arena((size_t)aused);
@@ -297,6 +298,7 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
             out.P.dense_lo = W.dense_lo;
             out.P.dense_n = W.dense_n;
             out.P.maxReadLength = hc.max_read_length;
+            out.W = W;
             // the three reductions write disjoint maps: events -> nd / ins / counts, clips -> sc5, sc3
             double clip_ms[2] = {0, 0};
             std::exception_ptr clip_err[2];
@@ -318,6 +320,15 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
                 throw;
             }
             const auto r1 = std::chrono::steady_clock::now();
+            if (hooks && hooks->after_events) {
+                try {
+                    hooks->after_events(out);
+                } catch (...) {
+                    c5.join();
+                    c3.join();
+                    throw;
+                }
+            }
             c5.join();
             c3.join();
             for (auto& ce : clip_err)

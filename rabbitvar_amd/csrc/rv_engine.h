@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <memory>
 #include <string>
 #include <thread>
@@ -28,6 +29,12 @@ struct PileupResult {
     int64_t n_reads = 0, walked = 0;
 };
 
+// Points in pileup() where the caller can start work that overlaps the host reductions.
+struct PileupHooks {
+    std::function<void()> after_walk;                 // the dense rows are final on the device
+    std::function<void(PileupResult&)> after_events;  // the event maps are reduced (clips still running)
+};
+
 class Engine {
   public:
     Engine(const rv_config& cfg, int device);
@@ -36,7 +43,7 @@ class Engine {
     // Runs prep + walk + anchor for one region; reduces the ordered records on the host.
     // When download_rows is set the whole dense window is copied back (diagnostics).
     rv_status pileup(const rv_region& region, const rv_reads& reads, const rv_contig& contig, bool download_rows,
-                     PileupResult& out);
+                     PileupResult& out, const PileupHooks* hooks = nullptr);
 
     Config cfg;
     std::string sample;

@@ -52,12 +52,16 @@ bool RowCache::load(std::vector<int>& rows) {
     return true;
 }
 
-bool RowCache::prefetch(const std::vector<int>& positions, int radius) {
+bool RowCache::prefetch(const std::vector<int>& positions, int radius, const std::vector<int>& exact) {
     std::vector<int> want;
     for (int p : positions) {
         int a = std::max(0, p - radius - lo_), b = std::min(n_ - 1, p + radius - lo_);
         for (int i = a; i <= b; i++)
             if (!have(i)) want.push_back(i);
+    }
+    for (int p : exact) {
+        const int i = p - lo_;
+        if (i >= 0 && i < n_ && !have(i)) want.push_back(i);
     }
     return load(want);
 }
@@ -489,6 +493,157 @@ const Re& res() {
     return r;
 }
 
+// findMM5 / findMM3: the mismatches next to a candidate's flanks, and the soft-clip positions to
+// try.  The soft clips found are marked used in `mark`; with mark == nullptr (a probe for the
+// positions a candidate will read) nothing is marked, the clip maps are not read, and no
+// reference key is created.
+MM findMM5(HostRef& rf, int position, const std::string& wupseq, HostPileup* mark) {
+    const bool probe = mark == nullptr;
+    std::string seq = strip_marks(wupseq);
+    MM r;
+    int n = 0, mn = 0, mcnt = 0;
+    std::string str;
+    auto ne = [&](char ch, int k) { return rf.count(k) && rf.at(k) != ch; };
+    auto eq = [&](char ch, int k) { return rf.count(k) && rf.at(k) == ch; };
+    while (ne(char_at(seq, -1 - n), position - n) && mcnt < 3) {
+        str.insert(str.begin(), char_at(seq, -1 - n));
+        r.mm.push_back({str, position - n, 5});
+        n++;
+        mcnt++;
+    }
+    r.scp.push_back(position + 1);
+    char misnt = 0;
+    if (str.size() == 1) {
+        while (eq(char_at(seq, -1 - n), position - n)) {
+            n++;
+            if (n != 0) mn++;
+        }
+        if (mn > 1) {
+            int n2 = 0;
+            while (-1 - n - 1 - n2 >= 0 && eq(char_at(seq, -1 - n - 1 - n2), position - n - 1 - n2)) n2++;
+            if (n2 > 2) {
+                r.scp.push_back(position - n - n2);
+                r.misp = position - n;
+                misnt = char_at(seq, -1 - n);
+                if (!probe) {
+                    auto s = mark->sc5.find(position - n - n2);
+                    if (s != mark->sc5.end()) s->second.used = true;
+                }
+                mn += n2;
+            } else {
+                r.scp.push_back(position - n);
+                if (!probe) {
+                    auto s = mark->sc5.find(position - n);
+                    if (s != mark->sc5.end()) s->second.used = true;
+                }
+            }
+        }
+    }
+    r.nm = mn;
+    r.misnt = misnt == 0 ? "" : std::string(1, misnt);
+    return r;
+}
+
+MM findMM3(HostRef& rf, int p, const std::string& sanpseq, HostPileup* mark) {
+    const bool probe = mark == nullptr;
+    // rf.get creates a missing reference key holding '\0'; a probe reads the same value
+    auto get = [&](int k) { return probe ? (rf.count(k) ? rf.at(k) : '\0') : rf.get(k); };
+    std::string seq = strip_marks(sanpseq);
+    MM r;
+    int n = 0, mn = 0, mcnt = 0;
+    std::string str;
+    while (n < (int)seq.size() && rf.count(p + n) && get(p + n) == seq[(size_t)n]) n++;
+    r.scp.push_back(p + n);
+    int Tbp = p + n;
+    while (mcnt <= 3 && n < (int)seq.size() && get(p + n) != seq[(size_t)n]) {
+        str += seq[(size_t)n];
+        r.mm.push_back({str, Tbp, 3});
+        n++;
+        mcnt++;
+    }
+    char misnt = 0;
+    auto eq = [&](char ch, int k) { return rf.count(k) && rf.at(k) == ch; };
+    if (str.size() == 1) {
+        while (n < (int)seq.size() && eq(seq[(size_t)n], p + n)) {
+            n++;
+            if (n != 0) mn++;
+        }
+        if (mn > 1) {
+            int n2 = 0;
+            while (n + n2 + 1 < (int)seq.size() && eq(seq[(size_t)(n + n2 + 1)], p + n + 1 + n2)) n2++;
+            if (n2 > 2 && n + n2 + 1 < (int)seq.size()) {
+                r.scp.push_back(p + n + n2);
+                r.misp = p + n;
+                misnt = seq[(size_t)n];
+                if (!probe) {
+                    auto s = mark->sc3.find(p + n + n2);
+                    if (s != mark->sc3.end()) s->second.used = true;
+                }
+                mn += n2;
+            } else {
+                r.scp.push_back(p + n);
+                if (!probe) {
+                    auto s = mark->sc3.find(p + n);
+                    if (s != mark->sc3.end()) s->second.used = true;
+                }
+            }
+        }
+    }
+    r.nm = mn;
+    r.misnt = misnt == 0 ? "" : std::string(1, misnt);
+    return r;
+}
+
+// the descriptor parse and flanking sequences of a deletion candidate (realigndel's set-up)
+struct DelSetup {
+    int dellen = 0, el = 0;
+    std::string wupseq, sanpseq;
+};
+DelSetup del_setup(HostRef& ref, int chrLen, int p, const std::string& vn) {
+    const Re& R = res();
+    DelSetup d;
+    std::smatch sm;
+    if (std::regex_search(vn, sm, R.begin_minus_number)) d.dellen = std::atoi(sm.str(1).c_str());
+    if (std::regex_search(vn, sm, R.up_number_end)) d.dellen += std::atoi(sm.str(1).c_str());
+    std::string extrains, extra, inv5, inv3;
+    if (std::regex_search(vn, sm, R.minus_sv)) {
+        inv5 = sm.str(1);
+        inv3 = sm.str(2);
+    } else if (std::regex_search(vn, sm, R.begin_minus_number_any)) {
+        extra = std::regex_replace(sm.str(1), R.strip, "");
+        std::smatch s2;
+        if (std::regex_search(vn, s2, R.caret_atgnc)) extrains = s2.str(1);
+    }
+    int wustart = (p - 200) > 1 ? (p - 200) : 1;
+    d.wupseq = join_ref(ref, wustart, p - 1) + extra;
+    if (!inv3.empty()) d.wupseq = inv3;
+    int sanend = p + 200 > chrLen ? chrLen : p + 200;
+    d.el = (int)extra.size() - (int)extrains.size();
+    d.sanpseq = extra + join_ref(ref, p + d.dellen + d.el, sanend);
+    if (!inv5.empty()) d.sanpseq = inv5;
+    return d;
+}
+
+// Positions realigndel will read for its candidates' mismatches (found from the reference and
+// the descriptors alone), so that they can be fetched with the first batch of rows.
+void del_probe_positions(HostRef& ref, int chrLen, const std::map<int, std::map<std::string, int>>& counts,
+                         std::vector<int>& out) {
+    for (auto& e : counts)
+        for (auto& kv : e.second) {
+            try {
+                const int p = e.first;
+                const DelSetup d = del_setup(ref, chrLen, p, kv.first);
+                const MM r3 = findMM3(ref, p, d.sanpseq, nullptr);
+                const MM r5 = findMM5(ref, p + d.dellen + d.el - 1, d.wupseq, nullptr);
+                for (const MM* r : {&r3, &r5}) {
+                    for (const Mismatch& m : r->mm) out.push_back(m.pos);
+                    if (r->misp != 0) out.push_back(r->misp);
+                }
+            } catch (const std::exception&) {
+            }
+        }
+}
+
 struct Realigner {
     const Config& c;
     int rstart, rend;
@@ -522,91 +677,6 @@ struct Realigner {
             if (char_at(s, dir == 1 ? n : dir * n - 1) != rf.get(position + dir * n)) mm++;
         }
         return mm <= MMx && mm / (double)s.size() < 0.15;
-    }
-
-    MM findMM5(HostRef& rf, int position, const std::string& wupseq) {
-        std::string seq = strip_marks(wupseq);
-        MM r;
-        int n = 0, mn = 0, mcnt = 0;
-        std::string str;
-        auto ne = [&](char ch, int k) { return rf.count(k) && rf.at(k) != ch; };
-        auto eq = [&](char ch, int k) { return rf.count(k) && rf.at(k) == ch; };
-        while (ne(char_at(seq, -1 - n), position - n) && mcnt < 3) {
-            str.insert(str.begin(), char_at(seq, -1 - n));
-            r.mm.push_back({str, position - n, 5});
-            n++;
-            mcnt++;
-        }
-        r.scp.push_back(position + 1);
-        char misnt = 0;
-        if (str.size() == 1) {
-            while (eq(char_at(seq, -1 - n), position - n)) {
-                n++;
-                if (n != 0) mn++;
-            }
-            if (mn > 1) {
-                int n2 = 0;
-                while (-1 - n - 1 - n2 >= 0 && eq(char_at(seq, -1 - n - 1 - n2), position - n - 1 - n2)) n2++;
-                if (n2 > 2) {
-                    r.scp.push_back(position - n - n2);
-                    r.misp = position - n;
-                    misnt = char_at(seq, -1 - n);
-                    auto s = S.P.sc5.find(position - n - n2);
-                    if (s != S.P.sc5.end()) s->second.used = true;
-                    mn += n2;
-                } else {
-                    r.scp.push_back(position - n);
-                    auto s = S.P.sc5.find(position - n);
-                    if (s != S.P.sc5.end()) s->second.used = true;
-                }
-            }
-        }
-        r.nm = mn;
-        r.misnt = misnt == 0 ? "" : std::string(1, misnt);
-        return r;
-    }
-
-    MM findMM3(HostRef& rf, int p, const std::string& sanpseq) {
-        std::string seq = strip_marks(sanpseq);
-        MM r;
-        int n = 0, mn = 0, mcnt = 0;
-        std::string str;
-        while (n < (int)seq.size() && rf.count(p + n) && rf.get(p + n) == seq[(size_t)n]) n++;
-        r.scp.push_back(p + n);
-        int Tbp = p + n;
-        while (mcnt <= 3 && n < (int)seq.size() && rf.get(p + n) != seq[(size_t)n]) {
-            str += seq[(size_t)n];
-            r.mm.push_back({str, Tbp, 3});
-            n++;
-            mcnt++;
-        }
-        char misnt = 0;
-        auto eq = [&](char ch, int k) { return rf.count(k) && rf.at(k) == ch; };
-        if (str.size() == 1) {
-            while (n < (int)seq.size() && eq(seq[(size_t)n], p + n)) {
-                n++;
-                if (n != 0) mn++;
-            }
-            if (mn > 1) {
-                int n2 = 0;
-                while (n + n2 + 1 < (int)seq.size() && eq(seq[(size_t)(n + n2 + 1)], p + n + 1 + n2)) n2++;
-                if (n2 > 2 && n + n2 + 1 < (int)seq.size()) {
-                    r.scp.push_back(p + n + n2);
-                    r.misp = p + n;
-                    misnt = seq[(size_t)n];
-                    auto s = S.P.sc3.find(p + n + n2);
-                    if (s != S.P.sc3.end()) s->second.used = true;
-                    mn += n2;
-                } else {
-                    r.scp.push_back(p + n);
-                    auto s = S.P.sc3.find(p + n);
-                    if (s != S.P.sc3.end()) s->second.used = true;
-                }
-            }
-        }
-        r.nm = mn;
-        r.misnt = misnt == 0 ? "" : std::string(1, misnt);
-        return r;
     }
 
     HSclip* sc5(int p) {
@@ -725,28 +795,12 @@ struct Realigner {
                 std::string vn = t.desc;
                 int dcnt = t.count;
                 HVar* vref = S.get_variation(p, vn);
-                int dellen = 0;
-                std::smatch sm;
-                if (std::regex_search(vn, sm, R.begin_minus_number)) dellen = std::atoi(sm.str(1).c_str());
-                if (std::regex_search(vn, sm, R.up_number_end)) dellen += std::atoi(sm.str(1).c_str());
-                std::string extrains, extra, inv5, inv3;
-                if (std::regex_search(vn, sm, R.minus_sv)) {
-                    inv5 = sm.str(1);
-                    inv3 = sm.str(2);
-                } else if (std::regex_search(vn, sm, R.begin_minus_number_any)) {
-                    extra = std::regex_replace(sm.str(1), R.strip, "");
-                    std::smatch s2;
-                    if (std::regex_search(vn, s2, R.caret_atgnc)) extrains = s2.str(1);
-                }
-                int wustart = (p - 200) > 1 ? (p - 200) : 1;
-                std::string wupseq = join_ref(ref, wustart, p - 1) + extra;
-                if (!inv3.empty()) wupseq = inv3;
-                int sanend = p + 200 > chrLen ? chrLen : p + 200;
-                int el = (int)extra.size() - (int)extrains.size();
-                std::string sanpseq = extra + join_ref(ref, p + dellen + el, sanend);
-                if (!inv5.empty()) sanpseq = inv5;
-                MM r3 = findMM3(ref, p, sanpseq);
-                MM r5 = findMM5(ref, p + dellen + el - 1, wupseq);
+                const DelSetup d = del_setup(ref, chrLen, p, vn);
+                const int dellen = d.dellen, el = d.el;
+                const std::string& wupseq = d.wupseq;
+                const std::string& sanpseq = d.sanpseq;
+                MM r3 = findMM3(ref, p, sanpseq, &S.P);
+                MM r5 = findMM5(ref, p + dellen + el - 1, wupseq, &S.P);
                 std::vector<Mismatch> mmm = r3.mm;
                 mmm.insert(mmm.end(), r5.mm.begin(), r5.mm.end());
                 for (const Mismatch& mis : mmm) {
@@ -878,12 +932,12 @@ struct Realigner {
                     int p3 = position + inslen - (int)ins3.size() + 50;
                     if (ins3.size() > 50) sanpseq = sub1(ins3, 50 - (int)ins3.size());
                     sanpseq += join_ref(ref, position + 1, position + 101);
-                    r3 = findMM3(ref, p3 + 1, sanpseq);
+                    r3 = findMM3(ref, p3 + 1, sanpseq, &S.P);
                 } else {
                     sanpseq = tn + join_ref(ref, position + (int)extra.size() + 1 + (int)compm.size() + newdel, sanend);
-                    r3 = findMM3(ref, position + 1, sanpseq);
+                    r3 = findMM3(ref, position + 1, sanpseq, &S.P);
                 }
-                MM r5 = findMM5(ref, position + (int)extra.size() + (int)compm.size() + newdel, wupseq);
+                MM r5 = findMM5(ref, position + (int)extra.size() + (int)compm.size() + newdel, wupseq, &S.P);
                 std::vector<Mismatch> mmm = r3.mm;
                 mmm.insert(mmm.end(), r5.mm.begin(), r5.mm.end());
                 HVar* vref = S.get_ins(position, vn);
@@ -1209,23 +1263,28 @@ void build_record_list(RState& S, CxHost& cx) {
 }
 
 
-void realign(const Config& c, int region_start, int region_end, HostRef& ref, int chrLen, ReadQuery* reads,
-             RState& S, RealignStats& st) {
-    // prefetch, in one transfer, the rows the candidates' own handlers read first: the candidate
-    // positions, the few bases after them, and every non-dense / insertion position (read again
-    // when the record list is built)
-    std::vector<int> cand, exact;
-    for (auto* m : {&S.P.mnp, &S.P.delCount, &S.P.insCount})
+bool realign_prefetch(const Config& c, HostRef& ref, int chrLen, const HostPileup& P, RowCache& rows) {
+    // the rows the candidates' own handlers read first: the candidate positions, the few bases
+    // after them, every non-dense / insertion position (read again when the record list is
+    // built), and the positions the deletion candidates' flank mismatches will read
+    std::vector<int> cand, probe;
+    for (auto* m : {&P.mnp, &P.delCount, &P.insCount})
         for (auto& e : *m) cand.push_back(e.first);
-    for (auto& e : S.P.nd) exact.push_back(e.first);
-    for (auto& e : S.P.ins) exact.push_back(e.first);
+    for (auto& e : P.nd) cand.push_back(e.first);
+    for (auto& e : P.ins) cand.push_back(e.first);
+    std::vector<int> exact;
+    if (c.performLocalRealignment) del_probe_positions(ref, chrLen, P.delCount, exact);
+    return rows.prefetch(cand, 4, exact);
+}
+
+void realign(const Config& c, int region_start, int region_end, HostRef& ref, int chrLen, ReadQuery* reads,
+             RState& S, RealignStats& st, bool prefetched) {
     auto now = [] { return std::chrono::steady_clock::now(); };
     auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
         return std::chrono::duration<double, std::milli>(b - a).count();
     };
     auto t0 = now();
-    for (int p : exact) cand.push_back(p);
-    S.rows.prefetch(cand, 4);
+    if (!prefetched && !realign_prefetch(c, ref, chrLen, S.P, S.rows)) throw std::runtime_error("dense row fetch failed");
     auto t1 = now();
     Realigner rg{c, region_start, region_end, ref, chrLen, reads, S, st};
     rg.adjustMNP();

@@ -46,7 +46,7 @@ class RowCache {
     static constexpr int MISS_RADIUS = 16;
     RowCache(RowSource* src, int dense_lo, int dense_n) : src_(src), lo_(dense_lo), n_(dense_n) {}
     // fetches, in one transfer, every row within `radius` of the given positions
-    bool prefetch(const std::vector<int>& positions, int radius);
+    bool prefetch(const std::vector<int>& positions, int radius, const std::vector<int>& exact = {});
     const DenseRow* row(int pos);  // nullptr outside the dense window
     int64_t fetches = 0, rows_fetched = 0;
     int64_t miss_fetches = 0;
@@ -119,9 +119,16 @@ struct RealignStats {
     int64_t n_nopass = 0;
 };
 
-// adjustMNP, realigndel, realignins, realignlgins30 in the reference's order.
+// The realigner's first batch of dense rows, in one transfer: every candidate position and the
+// four after it, the non-dense and insertion positions, and the positions the deletion
+// candidates' flank mismatches will read.  Reads only the maps the event reduction fills (not the
+// soft-clip maps), so it can run while the clips are still being reduced.
+bool realign_prefetch(const Config& c, HostRef& ref, int chrLen, const HostPileup& P, RowCache& rows);
+
+// adjustMNP, realigndel, realignins, realignlgins30 in the reference's order (after the prefetch
+// above unless `prefetched`).
 void realign(const Config& c, int region_start, int region_end, HostRef& ref, int chrLen, ReadQuery* reads,
-             RState& S, RealignStats& st);
+             RState& S, RealignStats& st, bool prefetched = false);
 
 // Positions the scoring stage reads from a record list instead of the dense rows: every
 // non-dense / insertion position and every position the realigner changed, ascending, with the

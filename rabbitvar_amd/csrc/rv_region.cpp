@@ -55,16 +55,61 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
     };
     PileupResult pr;
-    rv_status s = e.pileup(region, reads, contig, false, pr);
-    if (s != RV_OK) return s;
-    // ---- nonInsertionVariants keys in creation order; their slot-order replay runs on a second
-    // host thread while this one realigns
+    // Work that overlaps the host reductions inside pileup(): the nonInsertionVariants key order
+    // (a device sort) once the dense rows are final, and the realigner's first row batch once the
+    // event maps are reduced.
     std::vector<int32_t> created, walk_sorted;
-    const auto k0 = std::chrono::steady_clock::now();
-    s = e.key_order((int)std::min<int64_t>(reads.n_reads, INT32_MAX), created, walk_sorted);
+    rv_status key_status = RV_OK;
+    double key_ms = 0;
+    std::thread key_th;
+    DeviceRowSource src(e);
+    std::unique_ptr<RowCache> rcp;
+    HostRef ref;
+    bool prefetch_ok = false;
+    double prefetch_ms = 0;
+    std::exception_ptr prefetch_err;
+    std::thread prefetch_th;
+    struct Joiner {
+        std::thread& t;
+        ~Joiner() {
+            if (t.joinable()) t.join();
+        }
+    } key_joiner{key_th}, prefetch_joiner{prefetch_th};
+    PileupHooks hooks;
+    hooks.after_walk = [&] {
+        key_th = std::thread([&] {
+            const auto k0 = std::chrono::steady_clock::now();
+            key_status = e.key_order((int)std::min<int64_t>(reads.n_reads, INT32_MAX), created, walk_sorted);
+            key_ms = ms_since(k0);
+        });
+    };
+    hooks.after_events = [&](PileupResult& p) {
+        ref.seq = p.ref_win.data();
+        ref.lo = p.W.ref_lo;
+        ref.hi = p.W.ref_lo + (int)p.ref_win.size() - 1;
+        rcp = std::make_unique<RowCache>(&src, p.P.dense_lo, p.P.dense_n);
+        const HostPileup* hp = &p.P;
+        prefetch_th = std::thread([&, hp] {
+            try {
+                const auto f0 = std::chrono::steady_clock::now();
+                prefetch_ok = realign_prefetch(e.cfg, ref, contig.length, *hp, *rcp);
+                prefetch_ms = ms_since(f0);
+            } catch (...) {
+                prefetch_err = std::current_exception();
+            }
+        });
+    };
+    rv_status s = e.pileup(region, reads, contig, false, pr, &hooks);
+    if (key_th.joinable()) key_th.join();
+    if (prefetch_th.joinable()) prefetch_th.join();
     if (s != RV_OK) return s;
-    double order_ms = ms_since(k0);
-    e.stats.detail_ms[6] = order_ms;
+    if (key_status != RV_OK) return key_status;
+    if (!rcp) return e.fail(RV_ERR_DEVICE, "pileup finished without reducing its records");
+    if (prefetch_err) std::rethrow_exception(prefetch_err);
+    if (!prefetch_ok) return e.fail(RV_ERR_DEVICE, "dense row prefetch failed");
+    RowCache& rc = *rcp;
+    double order_ms = 0;
+    e.stats.detail_ms[6] = key_ms;
     auto h0 = std::chrono::steady_clock::now();
     // a second host thread checks whether the sorted slot order provably holds and otherwise
     // replays the walk's insertions, while this one realigns
@@ -84,27 +129,17 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
             replay_err = std::current_exception();
         }
     });
-    struct Joiner {
-        std::thread& t;
-        ~Joiner() {
-            if (t.joinable()) t.join();
-        }
-    } joiner{replay};
+    Joiner joiner{replay};
     // ---- realignment on the host
-    DeviceRowSource src(e);
-    RowCache rc(&src, pr.P.dense_lo, pr.P.dense_n);
     RState S(pr.P, rc);
-    HostRef ref;
-    ref.seq = pr.ref_win.data();
-    ref.lo = pr.W.ref_lo;
-    ref.hi = pr.W.ref_lo + (int)pr.ref_win.size() - 1;
     DeviceReadQuery rq(e);
     RealignStats rst;
     try {
-        realign(e.cfg, region.start, region.end, ref, contig.length, &rq, S, rst);
+        realign(e.cfg, region.start, region.end, ref, contig.length, &rq, S, rst, true);
     } catch (const std::exception& ex) {
         return e.fail(RV_ERR_DEVICE, std::string("realignment: ") + ex.what());
     }
+    rst.ms_prefetch = prefetch_ms;
     e.stats.detail_ms[0] = rst.ms_prefetch;
     e.stats.detail_ms[1] = rst.ms_mnp;
     e.stats.detail_ms[2] = rst.ms_del;
@@ -377,7 +412,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     e.last_cx_positions = ncx;
     e.last_rows_fetched = rc.rows_fetched;
     e.last_insertion_side_effects = (int64_t)hc[3];
-    e.release_later(std::move(S), std::move(rc), std::move(pr), std::move(cx), std::move(created),
+    e.release_later(std::move(S), std::move(rcp), std::move(pr), std::move(cx), std::move(created),
                     std::move(walk_sorted), std::move(nonins), std::move(recs), std::move(arena), std::move(status),
                     std::move(proc), std::move(aligned), std::move(parts), std::move(line_at), std::move(strs),
                     std::move(perm));
