@@ -68,6 +68,7 @@ Engine::Engine(const rv_config& c, int dev) {
 
 Engine::~Engine() {
     if (reaper_.joinable()) reaper_.join();
+    for (auto& blk : pinned_) (void)hipHostFree(blk.first);
     DevBuf* all[] = {&b_prep,      &b_cig,     &b_ctr,    &b_span,       &b_rows,     &b_ref,    &b_events,
                      &b_arena,     &b_clips,   &b_clip_arena, &b_anchors, &b_counts,   &b_gather_ids, &b_gather_out,
                      &b_cx_pos,    &b_cx_off,  &b_cx,     &b_cx_arena,   &b_cx_cov,   &b_cx_key, &b_out,
@@ -428,20 +429,44 @@ bool Engine::no_passing(int start, int end, bool& none) {
     return true;
 }
 
-bool Engine::fetch_rows(const std::vector<int>& idx, std::vector<DenseRow>& out) {
-    out.assign(idx.size(), DenseRow{});
-    if (idx.empty()) return true;
+void* Engine::pinned_alloc(size_t bytes) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    while (pinned_cur_ < pinned_.size()) {
+        auto& blk = pinned_[pinned_cur_];
+        if (pinned_off_ + bytes <= blk.second) {
+            void* p = (char*)blk.first + pinned_off_;
+            pinned_off_ += bytes;
+            return p;
+        }
+        pinned_cur_++;
+        pinned_off_ = 0;
+    }
+    const size_t sz = std::max(bytes, (size_t)64 << 20);
+    void* p = nullptr;
+    if (hipHostMalloc(&p, sz, hipHostMallocDefault) != hipSuccess) return nullptr;
+    pinned_.push_back({p, sz});
+    pinned_cur_ = pinned_.size() - 1;
+    pinned_off_ = bytes;
+    return p;
+}
+
+const DenseRow* Engine::fetch_rows(const std::vector<int>& idx) {
+    if (idx.empty()) return (const DenseRow*)pinned_alloc(sizeof(DenseRow));
     for (int i : idx)
-        if (i < 0 || i >= rows_n) return false;
+        if (i < 0 || i >= rows_n) return nullptr;
     int32_t* ids = (int32_t*)grow(b_gather_ids, sizeof(int32_t) * idx.size());
     DenseRow* dst = (DenseRow*)grow(b_gather_out, sizeof(DenseRow) * idx.size());
-    if (!ids || !dst) return false;
-    if (hipMemcpyAsync(ids, idx.data(), sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice, stream) != hipSuccess)
-        return false;
-    if (launch_gather_rows(rows, rows_n, ids, (int)idx.size(), dst, stream) != hipSuccess) return false;
-    if (hipMemcpyAsync(out.data(), dst, sizeof(DenseRow) * out.size(), hipMemcpyDeviceToHost, stream) != hipSuccess)
-        return false;
-    return hipStreamSynchronize(stream) == hipSuccess;
+    int32_t* h_ids = (int32_t*)pinned_alloc(sizeof(int32_t) * idx.size());
+    DenseRow* out = (DenseRow*)pinned_alloc(sizeof(DenseRow) * idx.size());
+    if (!ids || !dst || !h_ids || !out) return nullptr;
+    std::memcpy(h_ids, idx.data(), sizeof(int32_t) * idx.size());
+    if (hipMemcpyAsync(ids, h_ids, sizeof(int32_t) * idx.size(), hipMemcpyHostToDevice, stream) != hipSuccess)
+        return nullptr;
+    if (launch_gather_rows(rows, rows_n, ids, (int)idx.size(), dst, stream) != hipSuccess) return nullptr;
+    if (hipMemcpyAsync(out, dst, sizeof(DenseRow) * idx.size(), hipMemcpyDeviceToHost, stream) != hipSuccess)
+        return nullptr;
+    if (hipStreamSynchronize(stream) != hipSuccess) return nullptr;
+    return out;
 }
 
 }  // namespace rv

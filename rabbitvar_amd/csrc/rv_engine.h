@@ -58,8 +58,18 @@ class Engine {
     char* ref_dev = nullptr;
 
     void* grow(DevBuf& b, size_t bytes);
-    // copies the listed dense rows (indices into the window) of the last pileup to the host
-    bool fetch_rows(const std::vector<int>& rows, std::vector<DenseRow>& out);
+    // copies the listed dense rows (indices into the window) of the last pileup to pinned host
+    // memory, valid until pinned_reset(); nullptr on failure
+    const DenseRow* fetch_rows(const std::vector<int>& rows);
+    // pinned host memory for one region's transfers: blocks are kept and reused by the next region
+    void* pinned_alloc(size_t bytes);
+    void pinned_reset() {
+        pinned_cur_ = 0;
+        pinned_off_ = 0;
+    }
+    // row index -> fetched row table of the realigner's row cache, reused across regions (all null
+    // between regions)
+    std::vector<const DenseRow*> row_slots;
     // noPassingReads over the last pileup's records (device-resident): none = no record passes
     bool no_passing(int start, int end, bool& none);
 
@@ -96,6 +106,8 @@ class Engine {
 
   private:
     std::thread reaper_;
+    std::vector<std::pair<void*, size_t>> pinned_;
+    size_t pinned_cur_ = 0, pinned_off_ = 0;
     DevBuf b_reads[13], b_prep, b_cig, b_ctr, b_span, b_rows, b_ref;
     DevBuf b_events, b_arena, b_clips, b_clip_arena, b_anchors, b_counts;
     hipEvent_t ev[8];

@@ -27,29 +27,37 @@
 namespace rv {
 
 // ------------------------------------------------------------------ rows
-bool HostRowSource::fetch(const std::vector<int>& rows, std::vector<DenseRow>& out) {
-    out.assign(rows.size(), DenseRow{});
+const DenseRow* HostRowSource::fetch(const std::vector<int>& rows) {
+    blocks_.emplace_back(rows.size(), DenseRow{});
+    std::vector<DenseRow>& out = blocks_.back();
     for (size_t k = 0; k < rows.size(); k++) {
         int r = rows[k];
         if (r >= 0 && r < P_.dense_n && (size_t)r < P_.rows.size()) out[k] = P_.rows[(size_t)r];
     }
-    return true;
+    return out.data();
 }
 
 bool RowCache::load(std::vector<int>& rows) {
     std::sort(rows.begin(), rows.end());
     rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
     if (rows.empty()) return true;
-    std::vector<DenseRow> buf;
-    if (!src_->fetch(rows, buf)) return false;
+    const auto t0 = std::chrono::steady_clock::now();
+    const DenseRow* buf = src_->fetch(rows);
+    fetch_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (!buf) return false;
     fetches++;
     rows_fetched += (int64_t)rows.size();
-    if (slot_.empty()) slot_.assign((size_t)n_, -1);
+    if (slots_->size() < (size_t)n_) slots_->resize((size_t)n_, nullptr);
     for (size_t k = 0; k < rows.size(); k++) {
-        slot_[(size_t)rows[k]] = (int32_t)store_.size();
-        store_.push_back(buf[k]);
+        (*slots_)[(size_t)rows[k]] = buf + k;
+        set_.push_back(rows[k]);
     }
     return true;
+}
+
+void RowCache::release_slots() {
+    for (int i : set_) (*slots_)[(size_t)i] = nullptr;
+    set_.clear();
 }
 
 bool RowCache::prefetch(const std::vector<int>& positions, int radius, const std::vector<int>& exact) {
@@ -78,7 +86,7 @@ const DenseRow* RowCache::row(int pos) {
         miss_fetches++;
         miss_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
-    return &store_[(size_t)slot_[(size_t)i]];
+    return (*slots_)[(size_t)i];
 }
 
 // ------------------------------------------------------------------ reference

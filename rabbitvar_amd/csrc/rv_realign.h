@@ -23,42 +23,51 @@
 
 namespace rv {
 
-// Source of dense rows (row i holds position dense_lo + i).  fetch() copies the listed rows.
+// Source of dense rows (row i holds position dense_lo + i).  fetch() returns a copy of the listed
+// rows that stays valid as long as the source does; nullptr on failure.
 class RowSource {
   public:
     virtual ~RowSource() {}
-    virtual bool fetch(const std::vector<int>& rows, std::vector<DenseRow>& out) = 0;
+    virtual const DenseRow* fetch(const std::vector<int>& rows) = 0;
 };
 
 // Host rows already downloaded (tests, diagnostics).
 class HostRowSource : public RowSource {
   public:
     explicit HostRowSource(const HostPileup& P) : P_(P) {}
-    bool fetch(const std::vector<int>& rows, std::vector<DenseRow>& out) override;
+    const DenseRow* fetch(const std::vector<int>& rows) override;
 
   private:
     const HostPileup& P_;
+    std::deque<std::vector<DenseRow>> blocks_;
 };
 
 // Rows the realigner has read, by position.  A miss copies a small neighbourhood of rows.
+// The row index -> row table can be an external one reused across regions (all null when handed
+// over; release_slots() nulls the entries this cache set), or the cache's own.
 class RowCache {
   public:
     static constexpr int MISS_RADIUS = 16;
-    RowCache(RowSource* src, int dense_lo, int dense_n) : src_(src), lo_(dense_lo), n_(dense_n) {}
-    // fetches, in one transfer, every row within `radius` of the given positions
+    RowCache(RowSource* src, int dense_lo, int dense_n, std::vector<const DenseRow*>* slots = nullptr)
+        : src_(src), lo_(dense_lo), n_(dense_n), slots_(slots ? slots : &own_) {}
+    // fetches, in one transfer, every row within `radius` of the given positions and the rows of
+    // the `exact` positions
     bool prefetch(const std::vector<int>& positions, int radius, const std::vector<int>& exact = {});
     const DenseRow* row(int pos);  // nullptr outside the dense window
+    void release_slots();
     int64_t fetches = 0, rows_fetched = 0;
     int64_t miss_fetches = 0;
-    double miss_ms = 0;  // time in fetches made on a miss (not by prefetch)
+    double miss_ms = 0;   // time in fetches made on a miss (not by prefetch)
+    double fetch_ms = 0;  // time in the source's fetches
 
   private:
     bool load(std::vector<int>& rows);
-    bool have(int i) const { return !slot_.empty() && slot_[(size_t)i] >= 0; }
+    bool have(int i) const { return (size_t)i < slots_->size() && (*slots_)[(size_t)i] != nullptr; }
     RowSource* src_;
     int lo_, n_;
-    std::vector<int32_t> slot_;  // row index -> position in store_, -1 = not fetched
-    std::deque<DenseRow> store_;  // stable addresses
+    std::vector<const DenseRow*> own_;
+    std::vector<const DenseRow*>* slots_;
+    std::vector<int> set_;  // entries of *slots_ this cache filled
 };
 
 // The reference window as the reference's map: keys [lo, hi]; get() of a missing key creates it

@@ -21,7 +21,7 @@ namespace {
 class DeviceRowSource : public RowSource {
   public:
     explicit DeviceRowSource(Engine& e) : e_(e) {}
-    bool fetch(const std::vector<int>& rows, std::vector<DenseRow>& out) override { return e_.fetch_rows(rows, out); }
+    const DenseRow* fetch(const std::vector<int>& rows) override { return e_.fetch_rows(rows); }
 
   private:
     Engine& e_;
@@ -54,6 +54,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     auto ms_since = [](std::chrono::steady_clock::time_point t) {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
     };
+    e.pinned_reset();
     PileupResult pr;
     // Work that overlaps the host reductions inside pileup(): the nonInsertionVariants key order
     // (a device sort) once the dense rows are final, and the realigner's first row batch once the
@@ -75,6 +76,13 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
             if (t.joinable()) t.join();
         }
     } key_joiner{key_th}, prefetch_joiner{prefetch_th};
+    // the engine's row table must be all null again when this region ends, on every path
+    struct SlotRelease {
+        std::unique_ptr<RowCache>& rc;
+        ~SlotRelease() {
+            if (rc) rc->release_slots();
+        }
+    } slot_release{rcp};
     PileupHooks hooks;
     hooks.after_walk = [&] {
         key_th = std::thread([&] {
@@ -87,7 +95,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
         ref.seq = p.ref_win.data();
         ref.lo = p.W.ref_lo;
         ref.hi = p.W.ref_lo + (int)p.ref_win.size() - 1;
-        rcp = std::make_unique<RowCache>(&src, p.P.dense_lo, p.P.dense_n);
+        rcp = std::make_unique<RowCache>(&src, p.P.dense_lo, p.P.dense_n, &e.row_slots);
         const HostPileup* hp = &p.P;
         prefetch_th = std::thread([&, hp] {
             try {
@@ -412,6 +420,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     e.last_cx_positions = ncx;
     e.last_rows_fetched = rc.rows_fetched;
     e.last_insertion_side_effects = (int64_t)hc[3];
+    rcp->release_slots();
     e.release_later(std::move(S), std::move(rcp), std::move(pr), std::move(cx), std::move(created),
                     std::move(walk_sorted), std::move(nonins), std::move(recs), std::move(arena), std::move(status),
                     std::move(proc), std::move(aligned), std::move(parts), std::move(line_at), std::move(strs),
