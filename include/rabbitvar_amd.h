@@ -161,6 +161,15 @@ rv_status rv_open(const rv_config* cfg, int device, rv_engine** out);
 rv_status rv_run_region(rv_engine* e, const rv_region* region, const rv_reads* reads, const rv_contig* contig,
                         rv_callset** out);
 
+/* Somatic mode for one region: the first sample (`tumor`, -b 'T|N' before the bar) and the second
+ * (`normal`) are piled up and realigned over the same reference window, classified and printed
+ * (replaces SomaticMode::one_region_run_somt + output, reference src/modes/somaticMode.cpp:83-128 and
+ * :311-735, declared include/modes/somaticMode.h).  The call runs with the reference's two-sample
+ * setting (Configuration::hasBam2) whatever the engine was opened with; the printed sample name
+ * is the first name of a 'T|N' pair.  The call set carries the text only (count 0). */
+rv_status rv_run_region_somatic(rv_engine* e, const rv_region* region, const rv_reads* tumor,
+                                const rv_reads* normal, const rv_contig* contig, rv_callset** out);
+
 size_t rv_callset_count(const rv_callset* cs);
 const rv_call* rv_callset_get(const rv_callset* cs, size_t i);
 /* The region's output lines exactly as the reference's simple mode prints them. */

@@ -246,10 +246,25 @@ int main(int argc, char** argv) {
         c.freq = -1;
         c.minr = 0;
     }
+    // somatic mode: -b 'first|second' (Launcher.cpp:108-109, getSampleNamesSomatic: the printed
+    // sample is the first name of -N 'first|second')
+    std::string bam2;
+    {
+        const size_t bar = bam.find('|');
+        if (bar != std::string::npos) {
+            bam2 = bam.substr(bar + 1);
+            bam = bam.substr(0, bar);
+            c.hasBam2 = true;  // ToVarsBuilder keeps reference-only and sub-threshold positions
+            const size_t nb = c.sample.find('|');
+            if (nb != std::string::npos) c.sample = c.sample.substr(0, nb);
+        }
+    }
     Fasta fa = read_fasta(fasta);
     for (char& ch : fa.seq) ch = (char)toupper((unsigned char)ch);
     int chrLen = (int)fa.seq.size();
     std::vector<Read> reads = read_rvb(bam);
+    std::vector<Read> reads2;
+    if (!bam2.empty()) reads2 = read_rvb(bam2);
     std::vector<Region> regions;
     if (!regionArg.empty()) {
         Region r;
@@ -290,6 +305,7 @@ int main(int argc, char** argv) {
     }
     int maxRefLen = 1;
     for (const Read& r : reads) maxRefLen = std::max(maxRefLen, ref_len_of(r));
+    for (const Read& r : reads2) maxRefLen = std::max(maxRefLen, ref_len_of(r));
     std::ofstream of(out, std::ios::binary);
     std::ofstream dp, dr;
     if (!dumpP.empty()) dp.open(dumpP);
@@ -307,11 +323,15 @@ int main(int argc, char** argv) {
     auto now = [] { return std::chrono::steady_clock::now(); };
     for (const Region& R : regions) {
         // region iterator: records overlapping [R.start-1, R.end)
-        std::vector<const Read*> regionReads;
-        auto lo = std::lower_bound(reads.begin(), reads.end(), R.start - 1 - maxRefLen,
-                                   [](const Read& r, int v) { return r.pos0 < v; });
-        for (auto it = lo; it != reads.end() && it->pos0 < R.end; ++it)
-            if (it->pos0 + ref_len_of(*it) > R.start - 1) regionReads.push_back(&*it);
+        auto region_reads = [&](const std::vector<Read>& rs) {
+            std::vector<const Read*> out;
+            auto lo = std::lower_bound(rs.begin(), rs.end(), R.start - 1 - maxRefLen,
+                                       [](const Read& r, int v) { return r.pos0 < v; });
+            for (auto it = lo; it != rs.end() && it->pos0 < R.end; ++it)
+                if (it->pos0 + ref_len_of(*it) > R.start - 1) out.push_back(&*it);
+            return out;
+        };
+        std::vector<const Read*> regionReads = region_reads(reads);
         nreads += (long)regionReads.size();
         RefMap ref;
         int ext = c.referenceExtension;
@@ -320,6 +340,25 @@ int main(int argc, char** argv) {
         ref.start = ss;
         ref.end = se - 17;
         if (ref.end >= ref.start) ref.win = fa.seq.substr((size_t)(ss - 1), (size_t)(ref.end - ref.start + 1));
+        if (!bam2.empty()) {
+            // one_region_run_somt (somaticMode.cpp:83-128): the first sample, then the second over
+            // the same reference map, starting from the first sample's maximum read length
+            Aligned A1, A2;
+            for (int k = 0; k < 2; k++) {
+                std::vector<const Read*> rr = k == 0 ? regionReads : region_reads(reads2);
+                if (k == 1) nreads += (long)rr.size();
+                Counters ctr;
+                std::vector<const Read*> filtered;
+                preprocess_filter(c, rr, filtered, ctr);
+                Pileup P;
+                if (k == 1) P.maxReadLength = A1.maxReadLength;
+                pileup_region(c, R, ref, chrLen, filtered, P, ctr);
+                realign_region(c, R, ref, chrLen, rr, P);
+                score_region(c, R, ref, chrLen, P, k == 0 ? A1 : A2);
+            }
+            of << format_somatic(c, R, A1, A2);
+            continue;
+        }
         Counters ctr;
         std::vector<const Read*> filtered;
         auto a0 = now();

@@ -240,6 +240,8 @@ void realign_region(const Config& c, const Region& r, RefMap& ref, int chrLen,
                     const std::vector<const Read*>& regionReads, Pileup& P);
 void score_region(const Config& c, const Region& r, RefMap& ref, int chrLen, Pileup& P, Aligned& A);
 std::string format_simple(const Config& c, const Region& r, Aligned& A);
+// somatic mode (somaticMode.cpp:311-735): A1 = first (tumor) sample, A2 = second (normal) sample
+std::string format_somatic(const Config& c, const Region& r, Aligned& A1, Aligned& A2);
 
 // helpers shared by stages
 double kt_fisher_two_sided(int n11, int n12, int n21, int n22, double* left, double* right);

@@ -28,6 +28,7 @@
 #include <cstdio>
 #include <cstring>
 #include <regex>
+#include <set>
 #include <stdexcept>
 
 #include "rv_oracle.h"
@@ -708,6 +709,276 @@ std::string format_simple(const Config& c, const Region& R, Aligned& A) {
             print_line(out, vref, R, vv->sv, position, c);
         }
     });
+    return out;
+}
+
+}  // namespace orc
+
+// ------------------------------------------------------------------ somatic output (somaticMode.cpp)
+namespace orc {
+namespace {
+
+const std::string kStrongSomatic = "StrongSomatic", kSampleSpecific = "SampleSpecific", kLikelyLOH = "LikelyLOH",
+                  kGermline = "Germline", kStrongLOH = "StrongLOH", kLikelySomatic = "LikelySomatic",
+                  kAFDiff = "AFDiff", kComplex = "Complex", kSNV = "SNV";
+
+// Variant::isNoise (include/Variant.h:75-94): clears the variant's counts when it is noise
+bool isNoise(Variant* v, const Config& c) {
+    const double qual = v->meanQuality;
+    if (((qual < 4.5 || (qual < 12 && !v->hasAtLeast2DiffQualities)) && v->positionCoverage <= 3) ||
+        (qual < c.goodq && v->frequency < 2 * c.lofreq && v->positionCoverage <= 1)) {
+        v->totalPosCoverage -= v->positionCoverage;
+        v->positionCoverage = 0;
+        v->varsCountOnForward = 0;
+        v->varsCountOnReverse = 0;
+        v->frequency = 0;
+        v->highQualityReadsFrequency = 0;
+        return true;
+    }
+    return false;
+}
+
+// MINUS_NUM_NUM (include/patterns.h:18): a '-' followed by two digits
+bool minus_num_num(const std::string& s) {
+    for (size_t i = 0; i + 2 < s.size(); i++)
+        if (s[i] == '-' && isdigit((unsigned char)s[i + 1]) && isdigit((unsigned char)s[i + 2])) return true;
+    return false;
+}
+
+// put_fisher_ext_and_odds (somaticMode.cpp:130-149): two-sided p, then log(ad/bc + bc/ad) in float
+// over the counts plus one half
+void put_fisher_odds(std::string& s, int rf, int rr, int af, int ar) {
+    s += to_string_f(kt_fisher_two_sided(rf, rr, af, ar, nullptr, nullptr));
+    s += '\t';
+    const float a = (float)(rf + 0.5), b = (float)(rr + 0.5), cc = (float)(af + 0.5), d = (float)(ar + 0.5);
+    const float ad = a * d, bc = b * cc;
+    const float ratio = std::log(ad / bc + bc / ad);
+    s += to_string_f((double)ratio);
+    s += '\t';
+}
+
+void sample_fields(std::string& s, const Variant* v, const Variant* genotypeOf) {
+    s += std::to_string(v->totalPosCoverage) + "\t" + std::to_string(v->positionCoverage) + "\t" +
+         std::to_string(v->refForwardCoverage) + "\t" + std::to_string(v->refReverseCoverage) + "\t" +
+         std::to_string(v->varsCountOnForward) + "\t" + std::to_string(v->varsCountOnReverse) + "\t";
+    s += v->genotype.empty() ? std::string("0") : genotypeOf->genotype;
+    s += "\t" + to_string_f(v->frequency) + "\t" + v->strandBiasFlag + "\t" + to_string_f(v->meanPosition) + "\t" +
+         std::to_string(v->isAtLeastAt2Positions ? 1 : 0) + "\t" + to_string_f(v->meanQuality) + "\t" +
+         std::to_string(v->hasAtLeast2DiffQualities ? 1 : 0) + "\t" + to_string_f(v->meanMappingQuality) + "\t" +
+         to_string_f(v->highQualityToLowQualityRatio) + "\t" + to_string_f(v->highQualityReadsFrequency) + "\t" +
+         to_string_f(v->extraFrequency) + "\t" + to_string_f(v->numberOfMismatches) + "\t";
+}
+
+// SomaticMode::print_output_variant_simple (somaticMode.cpp:151-309)
+std::string somatic_line(const Variant* begin, const Variant* end, const Variant* tumor, const Variant* normal,
+                         const Region& R, const std::string& label, const Config& c) {
+    static const std::string null18 = "0\t0\t0\t0\t0\t0\t0\t0\t0\t0\t0\t0\t0\t0\t0\t0\t0\t0\t";
+    std::string s = c.sample + "\t" + R.gene + "\t" + R.chr + "\t";
+    if (begin)
+        s += std::to_string(begin->startPosition) + "\t" + std::to_string(begin->endPosition) + "\t" +
+             begin->refallele + "\t" + begin->varallele + "\t";
+    else
+        s += "0\t0\t0\t0\t";
+    if (tumor)
+        sample_fields(s, tumor, tumor);
+    else
+        s += null18;
+    if (c.fisher) {
+        if (tumor)
+            put_fisher_odds(s, tumor->refForwardCoverage, tumor->refReverseCoverage, tumor->varsCountOnForward,
+                            tumor->varsCountOnReverse);
+        else
+            s += "0\t0\t";
+    }
+    if (normal) {
+        // the reference prints the tumor variant's genotype here whenever the normal one has one
+        if (!normal->genotype.empty() && !tumor) throw std::runtime_error("somatic printer: normal genotype without tumor");
+        sample_fields(s, normal, tumor);
+    } else {
+        s += null18;
+    }
+    if (c.fisher) {
+        if (normal)
+            put_fisher_odds(s, normal->refForwardCoverage, normal->refReverseCoverage, normal->varsCountOnForward,
+                            normal->varsCountOnReverse);
+        else
+            s += "0\t0\t";
+    }
+    if (end)
+        s += std::to_string(end->shift3) + "\t" + to_string_f(end->msi) + "\t" + std::to_string(end->msint) + "\t" +
+             (end->leftseq.empty() ? std::string("0") : end->leftseq) + "\t" +
+             (end->rightseq.empty() ? std::string("0") : end->rightseq) + "\t";
+    else
+        s += "\t\t\t\t\t";
+    s += R.chr + ":" + std::to_string(R.start) + "-" + std::to_string(R.end) + "\t";
+    s += label + "\t";
+    s += begin ? begin->vartype + "\t" : std::string("\t");
+    s += tumor ? to_string_f(tumor->duprate) + "\t" : std::string("0\t");
+    s += "0\t";
+    s += normal ? to_string_f(normal->duprate) + "\t" : std::string("0\t");
+    s += "0\t";
+    if (c.fisher) {
+        const int t_tot = tumor ? tumor->totalPosCoverage : 0, t_var = tumor ? tumor->positionCoverage : 0;
+        const int n_tot = normal ? normal->totalPosCoverage : 0, n_var = normal ? normal->positionCoverage : 0;
+        int tref = t_tot - t_var, rref = n_tot - n_var;
+        if (tref < 0) tref = 0;
+        if (rref < 0) rref = 0;
+        put_fisher_odds(s, t_var, tref, n_var, rref);
+        const double tv = tumor ? tumor->frequency : 0, nv = normal ? normal->frequency : 0;
+        s += to_string_f(std::log(std::max(tv, 0.0001) / std::max(nv, 0.0001))) + "\t";
+        s += to_string_f(std::log(((float)t_var + 0.5) / ((float)n_var + 0.5))) + "\t";
+    }
+    s += "\n";
+    return s;
+}
+
+struct Somatic {
+    const Config& c;
+    const Region& R;
+
+    // determinateType (somaticMode.cpp:598-621)
+    std::string determinateType(Vars* variants, Variant* standard, Variant* cmp) {
+        std::string type;
+        if (isGoodVar(cmp, variants->referenceVariant, standard->vartype, c)) {
+            if (standard->frequency > (1 - c.lofreq) && cmp->frequency < 0.8 && cmp->frequency > 0.2)
+                type = kLikelyLOH;
+            else if (cmp->frequency < c.lofreq || cmp->positionCoverage <= 1)
+                type = kLikelySomatic;
+            else
+                type = kGermline;
+        } else {
+            if (cmp->frequency < c.lofreq || cmp->positionCoverage <= 1)
+                type = kLikelySomatic;
+            else
+                type = kAFDiff;
+        }
+        if (isNoise(cmp, c) && standard->vartype == kSNV) type = kStrongSomatic;
+        return type;
+    }
+
+    // callingForOneSample (:358-380)
+    std::string one_sample(Vars* vs, bool firstCover, const std::string& label) {
+        std::string out;
+        for (Variant* v : vs->variants) {
+            v->vartype = varType(v);
+            if (!isGoodVar(v, vs->referenceVariant, v->vartype, c)) continue;
+            if (v->vartype == kComplex) adjComplex(v);
+            out += firstCover ? somatic_line(v, v, nullptr, v, R, label, c) : somatic_line(v, v, v, nullptr, R, label, c);
+        }
+        return out;
+    }
+
+    // printVariationsFromFirstSample (:400-518)
+    std::string first_sample(Vars* v1, Vars* v2) {
+        std::string out;
+        size_t k = 0;
+        while (k < v1->variants.size()) {
+            Variant* vref = v1->variants[k];
+            const std::string type0 = vref->vartype.empty() ? varType(vref) : vref->vartype;
+            if (!isGoodVar(vref, v1->referenceVariant, type0, c)) break;
+            const std::string nt = vref->descriptionString;
+            vref->vartype = varType(vref);
+            if (vref->vartype == kComplex) adjComplex(vref);
+            auto it = v2->byDesc.find(nt);
+            if (it != v2->byDesc.end()) {
+                Variant* v2nt = it->second;
+                const std::string type = determinateType(v2, vref, v2nt);
+                out += somatic_line(vref, v2nt, vref, v2nt, R, type, c);
+            } else {
+                // tumor only: strong somatic
+                Variant hold;
+                const Variant* forPrint;
+                if (!v2->variants.empty()) {
+                    const Variant* v2r = v2->variants[0];
+                    hold.totalPosCoverage = v2r->totalPosCoverage;
+                    hold.refForwardCoverage = v2r->refForwardCoverage;
+                    hold.refReverseCoverage = v2r->refReverseCoverage;
+                    forPrint = &hold;
+                } else {
+                    forPrint = v2->referenceVariant;
+                }
+                if (vref->vartype != kSNV && (nt.size() > 10 || minus_num_num(nt)) &&
+                    vref->positionCoverage < c.minr + 3 && nt.find('<') != std::string::npos)
+                    throw std::runtime_error("somatic: combineAnalysis (structural descriptors) not restated");
+                out += somatic_line(vref, vref, vref, forPrint, R, kStrongSomatic, c);
+            }
+            k++;
+        }
+        if (k == 0) {
+            if (v2->variants.empty()) return "";
+            for (Variant* v2var : v2->variants) {
+                v2var->vartype = varType(v2var);
+                if (!isGoodVar(v2var, v2->referenceVariant, v2var->vartype, c)) continue;
+                const std::string nt = v2var->descriptionString;
+                auto it = v1->byDesc.find(nt);
+                if (it != v1->byDesc.end()) {
+                    Variant* v1nt = it->second;
+                    const std::string type = v1nt->frequency < c.lofreq ? kLikelyLOH : kGermline;
+                    if (v2var->vartype == kComplex) adjComplex(v1nt);
+                    v1nt->vartype = varType(v1nt);
+                    out += somatic_line(v1nt, v2var, v1nt, v2var, R, type, c);
+                } else {
+                    const Variant* v1var = v1->variants.empty() ? nullptr : v1->variants[0];
+                    const int tcov = v1var && v1var->totalPosCoverage != 0 ? v1var->totalPosCoverage : 0;
+                    const Variant* v1ref = v1->referenceVariant;
+                    Variant hold;
+                    hold.totalPosCoverage = tcov;
+                    hold.refForwardCoverage = v1ref ? v1ref->varsCountOnForward : 0;
+                    hold.refReverseCoverage = v1ref ? v1ref->varsCountOnReverse : 0;
+                    hold.genotype = v1var ? v1var->genotype
+                                          : (v1ref ? v1ref->descriptionString + "/" + v1ref->descriptionString : "N/N");
+                    if (v2var->vartype == kComplex) adjComplex(v2var);
+                    out += somatic_line(v2var, v2var, &hold, v2var, R, kStrongLOH, c);
+                }
+            }
+        }
+        return out;
+    }
+
+    // printVariationsFromSecondSample (:525-596)
+    std::string second_sample(Vars* v1, Vars* v2) {
+        std::string out;
+        for (Variant* v2var : v2->variants) {
+            v2var->vartype = varType(v2var);
+            if (!isGoodVar(v2var, v2->referenceVariant, v2var->vartype, c)) continue;
+            const std::string& nt = v2var->descriptionString;
+            auto it = v1->byDesc.find(nt);
+            Variant fresh;
+            Variant* v1nt = it != v1->byDesc.end() ? it->second : &fresh;
+            v1nt->positionCoverage = 0;
+            if (v2->byDesc.at(nt)->positionCoverage < c.minr + 3 && nt.find('<') != std::string::npos &&
+                (nt.size() > 10 || minus_num_num(nt)))
+                throw std::runtime_error("somatic: combineAnalysis (structural descriptors) not restated");
+            const Variant* forPrint = v1->referenceVariant;
+            if (v2var->vartype == kComplex) adjComplex(v2var);
+            out += somatic_line(v2var, v2var, forPrint, v2var, R, kStrongLOH, c);
+        }
+        return out;
+    }
+};
+
+}  // namespace
+
+std::string format_somatic(const Config& c, const Region& R, Aligned& A1, Aligned& A2) {
+    // positions: the first sample's alignedVariants keys, in order (the reference inserts the first
+    // map's keys twice, somaticMode.cpp:320-325, so the second map's own positions are not visited)
+    std::set<int> positions;
+    A1.vars.for_each([&](int p, Vars*) { positions.insert(p); });
+    Somatic S{c, R};
+    std::string out;
+    for (int p : positions) {
+        if (p < R.start || p > R.end) continue;
+        Vars** v1 = A1.vars.find(p);
+        Vars** v2 = A2.vars.find(p);
+        if (!v1)
+            out += S.one_sample(*v2, true, "Deletion");
+        else if (!v2)
+            out += S.one_sample(*v1, false, kSampleSpecific);
+        else if (!(*v1)->variants.empty())
+            out += S.first_sample(*v1, *v2);
+        else if (!(*v2)->variants.empty())
+            out += S.second_sample(*v1, *v2);
+    }
     return out;
 }
 

@@ -69,6 +69,9 @@ SIGNATURES = {
     "rv_open": (c_int, [ctypes.POINTER(RvConfig), c_int, ctypes.POINTER(c_void_p)]),
     "rv_run_region": (c_int, [c_void_p, ctypes.POINTER(RvRegion), ctypes.POINTER(RvReads),
                               ctypes.POINTER(RvContig), ctypes.POINTER(c_void_p)]),
+    "rv_run_region_somatic": (c_int, [c_void_p, ctypes.POINTER(RvRegion), ctypes.POINTER(RvReads),
+                                      ctypes.POINTER(RvReads), ctypes.POINTER(RvContig),
+                                      ctypes.POINTER(c_void_p)]),
     "rv_callset_count": (c_size_t, [c_void_p]),
     "rv_callset_get": (c_void_p, [c_void_p, c_size_t]),
     "rv_callset_text": (ctypes.POINTER(ctypes.c_char), [c_void_p, ctypes.POINTER(c_size_t)]),

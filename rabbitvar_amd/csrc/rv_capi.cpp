@@ -107,6 +107,24 @@ rv_status rv_run_region(rv_engine* e, const rv_region* region, const rv_reads* r
     return RV_OK;
 }
 
+rv_status rv_run_region_somatic(rv_engine* e, const rv_region* region, const rv_reads* tumor,
+                                const rv_reads* normal, const rv_contig* contig, rv_callset** out) {
+    if (!e || !region || !tumor || !normal || !contig || !out) return RV_ERR_ARG;
+    *out = nullptr;
+    rv_callset* cs = new (std::nothrow) rv_callset;
+    if (!cs) return RV_ERR_OOM;
+    const auto t0 = std::chrono::steady_clock::now();
+    rv_status s =
+        guarded(e, [&] { return rv::run_region_somatic(*e->e, *region, *tumor, *normal, *contig, cs->cs); });
+    e->e->stats.call_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (s != RV_OK) {
+        delete cs;
+        return s;
+    }
+    *out = cs;
+    return RV_OK;
+}
+
 size_t rv_callset_count(const rv_callset* cs) { return cs ? cs->cs.calls.size() : 0; }
 const rv_call* rv_callset_get(const rv_callset* cs, size_t i) {
     if (!cs || i >= cs->cs.calls.size()) return nullptr;

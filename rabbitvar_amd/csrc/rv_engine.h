@@ -70,6 +70,9 @@ class Engine {
     // row index -> fetched row table of the realigner's row cache, reused across regions (all null
     // between regions)
     std::vector<const DenseRow*> row_slots;
+    // the next pileup writes its rows into the other buffer, so the last pileup's rows stay valid
+    // (somatic mode keeps the first sample's rows while the second is piled up)
+    void swap_row_buffers() { std::swap(b_rows, b_rows2); }
     // noPassingReads over the last pileup's records (device-resident): none = no record passes
     bool no_passing(int start, int end, bool& none);
 
@@ -79,7 +82,8 @@ class Engine {
 
     // buffers of the realignment / scoring stages
     DevBuf b_gather_ids, b_gather_out, b_cx_pos, b_cx_off, b_cx, b_cx_arena, b_cx_cov, b_cx_key, b_out, b_out_arena,
-        b_score_ctr, b_status, b_keys, b_keys2, b_kpos, b_kpos2, b_sort_tmp, b_kcount, b_kpos3, b_args, b_nopass;
+        b_score_ctr, b_status, b_keys, b_keys2, b_kpos, b_kpos2, b_sort_tmp, b_kcount, b_kpos3, b_args, b_nopass,
+        b_want;
     WalkArgs* args_dev = nullptr;  // the last pileup's kernel arguments (its reads stay resident)
     int64_t args_n = 0;
     hipEvent_t sev[2] = {nullptr, nullptr};
@@ -108,7 +112,7 @@ class Engine {
     std::thread reaper_;
     std::vector<std::pair<void*, size_t>> pinned_;
     size_t pinned_cur_ = 0, pinned_off_ = 0;
-    DevBuf b_reads[13], b_prep, b_cig, b_ctr, b_span, b_rows, b_ref;
+    DevBuf b_reads[13], b_prep, b_cig, b_ctr, b_span, b_rows, b_rows2, b_ref;
     DevBuf b_events, b_arena, b_clips, b_clip_arena, b_anchors, b_counts;
     hipEvent_t ev[8];
 };

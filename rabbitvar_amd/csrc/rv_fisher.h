@@ -1,0 +1,97 @@
+// htslib's kt_fisher_exact (kfunc.c: Lanczos log-gamma, hypergeometric accumulator, two-sided
+// tail rule), restated for the device scoring kernels and the host's somatic printer.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "rv_common.h"
+
+namespace rv {
+
+RV_HD double kf_lgamma_d(double z) {
+    double x = 0;
+    x += 0.1659470187408462e-06 / (z + 7);
+    x += 0.9934937113930748e-05 / (z + 6);
+    x -= 0.1385710331296526 / (z + 5);
+    x += 12.50734324009056 / (z + 4);
+    x -= 176.6150291498386 / (z + 3);
+    x += 771.3234287757674 / (z + 2);
+    x -= 1259.139216722289 / (z + 1);
+    x += 676.5203681218835 / z;
+    x += 0.9999999999995183;
+    return log(x) - 5.58106146679532777 - z + (z - 0.5) * log(z + 6.5);
+}
+RV_HD double lbinom_d(int n, int k) {
+    if (k == 0 || n == k) return 0;
+    return kf_lgamma_d(n + 1) - kf_lgamma_d(k + 1) - kf_lgamma_d(n - k + 1);
+}
+struct HG {
+    int n11, n1_, n_1, n;
+    double p;
+};
+RV_HD double hg_acc(int n11, int n1_, int n_1, int n, HG& a) {
+    if (n1_ || n_1 || n) {
+        a.n11 = n11;
+        a.n1_ = n1_;
+        a.n_1 = n_1;
+        a.n = n;
+    } else {
+        if (n11 % 11 && n11 + a.n - a.n1_ - a.n_1) {
+            if (n11 == a.n11 + 1) {
+                a.p *= (double)(a.n1_ - a.n11) / n11 * (a.n_1 - a.n11) / (n11 + a.n - a.n1_ - a.n_1);
+                a.n11 = n11;
+                return a.p;
+            }
+            if (n11 == a.n11 - 1) {
+                a.p *= (double)a.n11 / (a.n1_ - n11) * (a.n11 + a.n - a.n1_ - a.n_1) / (a.n_1 - n11);
+                a.n11 = n11;
+                return a.p;
+            }
+        }
+        a.n11 = n11;
+    }
+    a.p = exp(lbinom_d(a.n1_, a.n11) + lbinom_d(a.n - a.n1_, a.n_1 - a.n11) - lbinom_d(a.n, a.n_1));
+    return a.p;
+}
+// two-sided p of the 2x2 table (n11 n12 / n21 n22); left/right tails optional
+RV_HD double fisher_two_sided(int n11, int n12, int n21, int n22, double* pl, double* pr) {
+    int i, j, max, min;
+    double p, q, left, right, two;
+    HG aux{0, 0, 0, 0, 0};
+    int n1_ = n11 + n12, n_1 = n11 + n21, n = n11 + n12 + n21 + n22;
+    max = (n_1 < n1_) ? n_1 : n1_;
+    min = n1_ + n_1 - n;
+    if (min < 0) min = 0;
+    if (pl) *pl = 1.;
+    if (pr) *pr = 1.;
+    if (min == max) return 1.;
+    q = hg_acc(n11, n1_, n_1, n, aux);
+    p = hg_acc(min, 0, 0, 0, aux);
+    for (left = 0., i = min + 1; p < 0.99999999 * q && i <= max; ++i) left += p, p = hg_acc(i, 0, 0, 0, aux);
+    --i;
+    if (p < 1.00000001 * q)
+        left += p;
+    else
+        --i;
+    p = hg_acc(max, 0, 0, 0, aux);
+    for (right = 0., j = max - 1; p < 0.99999999 * q && j >= 0; --j) right += p, p = hg_acc(j, 0, 0, 0, aux);
+    ++j;
+    if (p < 1.00000001 * q)
+        right += p;
+    else
+        ++j;
+    two = left + right;
+    if (two > 1.) two = 1.;
+    int di = i - n11 < 0 ? n11 - i : i - n11;
+    int dj = j - n11 < 0 ? n11 - j : j - n11;
+    if (di < dj)
+        right = 1. - left + q;
+    else
+        left = 1.0 - right + q;
+    if (pl) *pl = left;
+    if (pr) *pr = right;
+    return two;
+}
+
+}  // namespace rv

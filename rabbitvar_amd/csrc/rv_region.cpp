@@ -49,6 +49,53 @@ class DeviceReadQuery : public ReadQuery {
         if (e_ != hipSuccess) return e.fail(RV_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+rv_status score_pass(Engine& e, ScoreIn& in, size_t n_status, std::vector<OutRec>& recs, std::vector<char>& arena,
+                     std::vector<uint8_t>* status, unsigned long long hc[8], float& score_ms) {
+    unsigned max_out = (unsigned)std::min<int64_t>(65536 + 8 * (int64_t)in.n_cx + in.dense_n / 16, 0x7fffffff);
+    unsigned long long arena_cap = 160ull * max_out;
+    for (int attempt = 0; attempt < 3; attempt++) {
+        in.out = (OutRec*)e.grow(e.b_out, sizeof(OutRec) * max_out);
+        in.arena = (char*)e.grow(e.b_out_arena, (size_t)arena_cap);
+        unsigned long long* ctr = (unsigned long long*)e.grow(e.b_score_ctr, 64);
+        if (!in.out || !in.arena || !ctr) return e.fail(RV_ERR_OOM, "device buffers for the scored records failed");
+        RV_HIPS(hipMemsetAsync(ctr, 0, 64, e.stream));
+        RV_HIPS(hipMemsetAsync(in.status, 0, n_status, e.stream));
+        in.n_out = (unsigned*)ctr;
+        in.arena_used = ctr + 1;
+        in.counters = ctr + 2;
+        in.max_out = max_out;
+        in.arena_cap = arena_cap;
+        RV_HIPS(hipEventRecord(e.sev[0], e.stream));
+        RV_HIPS(launch_score(in, e.stream));
+        RV_HIPS(hipEventRecord(e.sev[1], e.stream));
+        RV_HIPS(hipMemcpyAsync(hc, ctr, 64, hipMemcpyDeviceToHost, e.stream));
+        RV_HIPS(hipStreamSynchronize(e.stream));
+        RV_HIPS(hipEventElapsedTime(&score_ms, e.sev[0], e.sev[1]));
+        unsigned n_out = (unsigned)(hc[0] & 0xffffffffu);
+        unsigned long long used = hc[1];
+        if (n_out <= max_out && used <= arena_cap) {
+            if (hc[2] != 0) {
+                // a position needed more alleles or string space than a scoring thread holds
+                return e.fail(RV_ERR_CAPACITY, "scoring: " + std::to_string(hc[2]) +
+                                                   " position(s) exceeded the per-thread allele/string capacity");
+            }
+            recs.resize(n_out);
+            arena.resize((size_t)used + 1);
+            if (n_out) RV_HIPS(hipMemcpyAsync(recs.data(), in.out, sizeof(OutRec) * n_out, hipMemcpyDeviceToHost, e.stream));
+            if (used) RV_HIPS(hipMemcpyAsync(arena.data(), in.arena, (size_t)used, hipMemcpyDeviceToHost, e.stream));
+            if (status) {
+                status->resize(n_status);
+                RV_HIPS(hipMemcpyAsync(status->data(), in.status, n_status, hipMemcpyDeviceToHost, e.stream));
+            }
+            RV_HIPS(hipStreamSynchronize(e.stream));
+            return RV_OK;
+        }
+        max_out = std::max<unsigned>(max_out, n_out) * 2;
+        arena_cap = std::max(arena_cap, used) * 2;
+    }
+    return e.fail(RV_ERR_CAPACITY, "scored record buffers kept overflowing");
+}
+
 rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, const rv_contig& contig, CallSet& out) {
     const auto w0 = std::chrono::steady_clock::now();
     auto ms_since = [](std::chrono::steady_clock::time_point t) {
@@ -199,54 +246,13 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     const size_t n_status = (size_t)pr.P.dense_n + (size_t)ncx;
     in.status = (uint8_t*)e.grow(e.b_status, n_status);
     if (!in.status) return e.fail(RV_ERR_OOM, "device buffer for position status failed");
-    unsigned max_out = (unsigned)std::min<int64_t>(65536 + 8 * (int64_t)ncx + pr.P.dense_n / 16, 0x7fffffff);
-    unsigned long long arena_cap = 160ull * max_out;
     std::vector<OutRec> recs;
     std::vector<char> arena;
     std::vector<uint8_t> status;
     unsigned long long hc[8];
     float score_ms = 0;
-    for (int attempt = 0; attempt < 3; attempt++) {
-        in.out = (OutRec*)e.grow(e.b_out, sizeof(OutRec) * max_out);
-        in.arena = (char*)e.grow(e.b_out_arena, (size_t)arena_cap);
-        unsigned long long* ctr = (unsigned long long*)e.grow(e.b_score_ctr, 64);
-        if (!in.out || !in.arena || !ctr) return e.fail(RV_ERR_OOM, "device buffers for the scored records failed");
-        RV_HIPS(hipMemsetAsync(ctr, 0, 64, e.stream));
-        RV_HIPS(hipMemsetAsync(in.status, 0, n_status, e.stream));
-        in.n_out = (unsigned*)ctr;
-        in.arena_used = ctr + 1;
-        in.counters = ctr + 2;
-        in.max_out = max_out;
-        in.arena_cap = arena_cap;
-        RV_HIPS(hipEventRecord(e.sev[0], e.stream));
-        RV_HIPS(launch_score(in, e.stream));
-        RV_HIPS(hipEventRecord(e.sev[1], e.stream));
-        RV_HIPS(hipMemcpyAsync(hc, ctr, 64, hipMemcpyDeviceToHost, e.stream));
-        RV_HIPS(hipStreamSynchronize(e.stream));
-        RV_HIPS(hipEventElapsedTime(&score_ms, e.sev[0], e.sev[1]));
-        unsigned n_out = (unsigned)(hc[0] & 0xffffffffu);
-        unsigned long long used = hc[1];
-        if (n_out <= max_out && used <= arena_cap) {
-            if (hc[2] != 0) {
-                // a position needed more alleles or string space than a scoring thread holds
-                unsigned long long aused = used;
-                (void)aused;
-                return e.fail(RV_ERR_CAPACITY, "scoring: " + std::to_string(hc[2]) +
-                                                   " position(s) exceeded the per-thread allele/string capacity");
-            }
-            recs.resize(n_out);
-            arena.resize((size_t)used + 1);
-            if (n_out) RV_HIPS(hipMemcpyAsync(recs.data(), in.out, sizeof(OutRec) * n_out, hipMemcpyDeviceToHost, e.stream));
-            if (used) RV_HIPS(hipMemcpyAsync(arena.data(), in.arena, (size_t)used, hipMemcpyDeviceToHost, e.stream));
-            status.resize(n_status);
-            RV_HIPS(hipMemcpyAsync(status.data(), in.status, n_status, hipMemcpyDeviceToHost, e.stream));
-            RV_HIPS(hipStreamSynchronize(e.stream));
-            break;
-        }
-        if (attempt == 2) return e.fail(RV_ERR_CAPACITY, "scored record buffers kept overflowing");
-        max_out = std::max<unsigned>(max_out, n_out) * 2;
-        arena_cap = std::max(arena_cap, used) * 2;
-    }
+    s = score_pass(e, in, n_status, recs, arena, &status, hc, score_ms);
+    if (s != RV_OK) return s;
     // ---- text, in the printer's order: alignedVariants slot order, then the variants' order.  A
     // second thread works out that order while this one formats the lines in record order; the
     // lines are then joined in printing order.

@@ -20,4 +20,16 @@ struct CallSet {
 
 rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, const rv_contig& contig, CallSet& out);
 
+// Somatic mode: the first (tumor) and second (normal) samples of one region, classified and
+// printed as somaticMode.cpp does (one_region_run_somt :83-128, output :311-735).  The call set
+// carries the text only.
+rv_status run_region_somatic(Engine& e, const rv_region& region, const rv_reads& first, const rv_reads& second,
+                             const rv_contig& contig, CallSet& out);
+
+// One launch of the scoring kernels over `in` (record list already uploaded), retried with larger
+// record buffers when they overflow; copies back the records, the string arena and, when asked,
+// the per-slot status.  hc receives the kernel counters.
+rv_status score_pass(Engine& e, ScoreIn& in, size_t n_status, std::vector<OutRec>& recs, std::vector<char>& arena,
+                     std::vector<uint8_t>* status, unsigned long long hc[8], float& score_ms);
+
 }  // namespace rv

@@ -37,9 +37,19 @@ struct OutRec {
     int32_t nfields;  // 5 for a pileup line without a reference variant
     double freq, pmean, qmean, mq, hlr, hifreq, extrafreq, msi, nm, fisher_p, fisher_ratio;
     uint32_t str_off;
-    uint16_t len[6];  // refallele, varallele, genotype, bias, leftseq, rightseq
+    uint16_t len[7];  // refallele, varallele, genotype, bias, leftseq, rightseq, descriptor
+    int32_t kind;     // OUT_VARIANT, or (somatic passes) OUT_REF / OUT_REF_ONLY / OUT_VARIANT_RAW
     int32_t start_rank;  // host: the position's rank in the printer's order
 };
+
+// OutRec kinds.  Simple mode emits printed variants only.  The somatic passes emit a position's
+// Vars: every non-reference variant in CMP_VARI order plus its reference variant (OUT_REF, as
+// createVariant left it), or, on request, the reference-only form of a position without variants
+// (OUT_REF_ONLY, ToVarsBuilder.cpp:1074-1097), and the variants a failed reference lookup left as
+// createVariant made them (OUT_VARIANT_RAW).
+enum : int { OUT_VARIANT = 0, OUT_REF = 1, OUT_REF_ONLY = 2, OUT_VARIANT_RAW = 3 };
+// ScoreIn::somatic
+enum : int { SCORE_SIMPLE = 0, SCORE_SOMATIC_VARS = 1, SCORE_SOMATIC_REF = 2 };
 
 enum : int { VT_SNV = 0, VT_INS = 1, VT_DEL = 2, VT_COMPLEX = 3, VT_EMPTY = 4, VT_SV = 5 };
 
@@ -64,7 +74,10 @@ struct ScoreIn {
     unsigned long long* arena_used;
     unsigned long long arena_cap;
     uint8_t* status;  // per dense row, then per record-list position: 1 = alignedVariants entry kept,
-                      // 2 = added and erased again (nothing above -f), 0 = not added
+                      // 2 = added and erased again (nothing above -f), 0 = not added; the somatic
+                      // passes: 1 = kept with variants, 3 = kept without (reference only)
+    int somatic;                // SCORE_*
+    const uint8_t* want;        // SCORE_SOMATIC_REF: per status slot, emit the reference-only record
     unsigned long long* counters;  // [0] capacity overflows, [1] insertion side effects on p+1,
                                    // [2] sorts above 16 elements, [3] qstd ambiguous rows
 };

@@ -1,6 +1,7 @@
 // Device side of the per-position scoring stage (see rv_score.h for the restated reference code).
 // Included by rv_score.hip only.
 #pragma once
+#include "rv_fisher.h"
 #include "rv_score.h"
 
 namespace rv {
@@ -215,91 +216,7 @@ __device__ MSIRes find_msi(const char* t1, int l1, const char* t2, int l2, const
     return MSIRes{msicnt, shift3, maxlen};
 }
 
-// --------------------------------------------------------------------------- Fisher (FP64)
-__device__ inline double kf_lgamma_d(double z) {
-    double x = 0;
-    x += 0.1659470187408462e-06 / (z + 7);
-    x += 0.9934937113930748e-05 / (z + 6);
-    x -= 0.1385710331296526 / (z + 5);
-    x += 12.50734324009056 / (z + 4);
-    x -= 176.6150291498386 / (z + 3);
-    x += 771.3234287757674 / (z + 2);
-    x -= 1259.139216722289 / (z + 1);
-    x += 676.5203681218835 / z;
-    x += 0.9999999999995183;
-    return log(x) - 5.58106146679532777 - z + (z - 0.5) * log(z + 6.5);
-}
-__device__ inline double lbinom_d(int n, int k) {
-    if (k == 0 || n == k) return 0;
-    return kf_lgamma_d(n + 1) - kf_lgamma_d(k + 1) - kf_lgamma_d(n - k + 1);
-}
-struct HG {
-    int n11, n1_, n_1, n;
-    double p;
-};
-__device__ inline double hg_acc(int n11, int n1_, int n_1, int n, HG& a) {
-    if (n1_ || n_1 || n) {
-        a.n11 = n11;
-        a.n1_ = n1_;
-        a.n_1 = n_1;
-        a.n = n;
-    } else {
-        if (n11 % 11 && n11 + a.n - a.n1_ - a.n_1) {
-            if (n11 == a.n11 + 1) {
-                a.p *= (double)(a.n1_ - a.n11) / n11 * (a.n_1 - a.n11) / (n11 + a.n - a.n1_ - a.n_1);
-                a.n11 = n11;
-                return a.p;
-            }
-            if (n11 == a.n11 - 1) {
-                a.p *= (double)a.n11 / (a.n1_ - n11) * (a.n11 + a.n - a.n1_ - a.n_1) / (a.n_1 - n11);
-                a.n11 = n11;
-                return a.p;
-            }
-        }
-        a.n11 = n11;
-    }
-    a.p = exp(lbinom_d(a.n1_, a.n11) + lbinom_d(a.n - a.n1_, a.n_1 - a.n11) - lbinom_d(a.n, a.n_1));
-    return a.p;
-}
-// two-sided p of the 2x2 table (n11 n12 / n21 n22); left/right tails optional
-__device__ inline double fisher_two_sided(int n11, int n12, int n21, int n22, double* pl, double* pr) {
-    int i, j, max, min;
-    double p, q, left, right, two;
-    HG aux{0, 0, 0, 0, 0};
-    int n1_ = n11 + n12, n_1 = n11 + n21, n = n11 + n12 + n21 + n22;
-    max = (n_1 < n1_) ? n_1 : n1_;
-    min = n1_ + n_1 - n;
-    if (min < 0) min = 0;
-    if (pl) *pl = 1.;
-    if (pr) *pr = 1.;
-    if (min == max) return 1.;
-    q = hg_acc(n11, n1_, n_1, n, aux);
-    p = hg_acc(min, 0, 0, 0, aux);
-    for (left = 0., i = min + 1; p < 0.99999999 * q && i <= max; ++i) left += p, p = hg_acc(i, 0, 0, 0, aux);
-    --i;
-    if (p < 1.00000001 * q)
-        left += p;
-    else
-        --i;
-    p = hg_acc(max, 0, 0, 0, aux);
-    for (right = 0., j = max - 1; p < 0.99999999 * q && j >= 0; --j) right += p, p = hg_acc(j, 0, 0, 0, aux);
-    ++j;
-    if (p < 1.00000001 * q)
-        right += p;
-    else
-        ++j;
-    two = left + right;
-    if (two > 1.) two = 1.;
-    int di = i - n11 < 0 ? n11 - i : i - n11;
-    int dj = j - n11 < 0 ? n11 - j : j - n11;
-    if (di < dj)
-        right = 1. - left + q;
-    else
-        left = 1.0 - right + q;
-    if (pl) *pl = left;
-    if (pr) *pr = right;
-    return two;
-}
+// Fisher (FP64): rv_fisher.h
 
 namespace {
 
@@ -458,6 +375,103 @@ struct Strs {
 
 }  // namespace
 
+// Appends one record and its strings (refallele, varallele, genotype, bias, leftseq, rightseq,
+// descriptor); false when the buffers are full (counted).
+__device__ inline bool emit_out(const ScoreIn& S, OutRec& o, const char* const pc[7], const int pn[7]) {
+    unsigned oi = atomicAdd(S.n_out, 1u);
+    unsigned long long nbytes = 0;
+    for (int k = 0; k < 7; k++) nbytes += (unsigned long long)pn[k];
+    unsigned long long off = atomicAdd(S.arena_used, nbytes);
+    if (oi >= S.max_out || off + nbytes > S.arena_cap) {
+        atomicAdd(&S.counters[0], 1ull);
+        return false;
+    }
+    char* w = S.arena + off;
+    for (int k = 0; k < 7; k++) {
+        for (int i = 0; i < pn[k]; i++) *w++ = pc[k][i];
+        o.len[k] = (uint16_t)pn[k];
+    }
+    o.str_off = (uint32_t)off;
+    S.out[oi] = o;
+    return true;
+}
+
+// A position's reference variant as a somatic record: as createVariant left it (OUT_REF), or in the
+// reference-only form ToVarsBuilder gives a position without variants (OUT_REF_ONLY,
+// ToVarsBuilder.cpp:1074-1097).  x == nullptr: the empty Variant ToVarsBuilder creates when no
+// allele has reads (:1098-1100).
+// OUT_VARIANT_RAW: a variant collectReferenceVariants never completed (createVariant's fields, and
+// the left flank when only the right one failed), in CMP_VARI rank `order`.
+__device__ inline void emit_ref(const ScoreIn& S, int p, int kind, const VarX* x, const char* d, int dl, int tpc,
+                                int rfc, int rrc, char rbase, int order = -1, const char* lfc = nullptr,
+                                int lfn = 0) {
+    OutRec o;
+    o.position = p;
+    o.order = order;
+    o.kind = kind;
+    o.nfields = 0;
+    o.fisher_p = o.fisher_ratio = 0;
+    o.vartype = VT_EMPTY;
+    o.start = o.end = 0;
+    o.tpc = o.poscov = o.rfc = o.rrc = o.fwd = o.rev = o.shift3 = o.msint = o.hicnt = o.hicov = o.pstd = o.qstd = 0;
+    o.freq = o.pmean = o.qmean = o.mq = o.hlr = o.hifreq = o.extrafreq = o.msi = o.nm = 0;
+    char bias[8], allele[2] = {0, 0}, gt[4] = {0, 0, 0, 0};
+    int nb = 0, na = 0, ng = 0;
+    if (x) {
+        o.poscov = x->poscov;
+        o.fwd = x->fwd;
+        o.rev = x->rev;
+        o.hicnt = x->hicnt;
+        o.hicov = x->hicov;
+        o.pstd = x->pstd;
+        o.qstd = x->qstd;
+        o.freq = x->freq;
+        o.pmean = x->pmean;
+        o.qmean = x->qmean;
+        o.mq = x->mq;
+        o.hlr = x->hlr;
+        o.hifreq = x->hifreq;
+        o.extrafreq = x->extrafreq;
+        o.nm = x->nm;
+        bias[nb++] = (char)('0' + x->bias);
+        if (kind == OUT_REF_ONLY) {
+            o.tpc = tpc;
+            o.poscov = 0;
+            o.freq = 0;
+            o.rfc = rfc;
+            o.rrc = rrc;
+            o.fwd = o.rev = 0;
+            o.start = o.end = p;
+            bias[nb++] = ';';
+            bias[nb++] = '0';
+            if (rbase) {
+                // validateRefallele maps IUPAC codes to their alphabetically first base
+                char v = rbase;
+                switch (v) {
+                    case 'M': case 'R': case 'W': case 'V': case 'H': case 'D': v = 'A'; break;
+                    case 'S': case 'Y': case 'B': v = 'C'; break;
+                    case 'K': v = 'G'; break;
+                    default: break;
+                }
+                allele[0] = v;
+                na = 1;
+                gt[0] = rbase;
+                gt[1] = '/';
+                gt[2] = rbase;
+                ng = 3;
+            } else {
+                gt[0] = '/';
+                ng = 1;
+            }
+        }
+    } else {
+        bias[nb++] = '0';
+    }
+    const char* pc[7] = {allele, allele, gt, bias, lfc ? lfc : allele, allele, d};
+    const int pn[7] = {na, na, ng, nb, lfc ? lfn : 0, 0, dl};
+    emit_out(S, o, pc, pn);
+}
+
 template <int CAP, int NA>
 __device__ void score_position(const ScoreIn& S, int p, bool dense_only, int slot) {
     const Config& c = S.cfg;
@@ -533,16 +547,40 @@ __device__ void score_position(const ScoreIn& S, int p, bool dense_only, int slo
             if (f > maxfreq) maxfreq = f;
         }
     }
-    if (nv == 0) return;  // no Vars entry was created for the position
+    if (nv == 0) {
+        // no allele has reads: simple mode creates no Vars entry; with a second sample ToVarsBuilder
+        // still creates one, holding an empty reference variant
+        if (S.somatic == SCORE_SIMPLE) return;
+        S.status[slot] = 3;
+        if (S.somatic == SCORE_SOMATIC_REF && S.want[slot] && !(p < S.region_start || p > S.region_end))
+            emit_ref(S, p, OUT_REF_ONLY, nullptr, nullptr, 0, 0, 0, 0, 0);
+        return;
+    }
     // alignedVariants: the position was added, and is erased again when nothing passes -f
     if (!c.doPileup && maxfreq <= c.freq && !c.hasBam2) {
         S.status[slot] = 2;
         return;
     }
-    S.status[slot] = 1;
+    S.status[slot] = (S.somatic != SCORE_SIMPLE && nonref == 0) ? 3 : 1;
     bool in_region = !(p < S.region_start || p > S.region_end);
     // the output filter skips positions outside the region; nothing else reads them
     if (!in_region) return;
+    if (S.somatic == SCORE_SOMATIC_REF) {
+        // only positions without variants, where the other sample has some
+        if (nonref == 0 && S.want[slot]) {
+            const VarX rx = expand(var[refi], c);
+            const char rb = rf.has(p) ? rf.at(p) : 0;
+            emit_ref(S, p, OUT_REF_ONLY, &rx, var[refi].al->d, var[refi].al->len, tpc, rx.fwd, rx.rev, rb);
+        }
+        return;
+    }
+    if (S.somatic == SCORE_SOMATIC_VARS) {
+        if (nonref == 0) return;
+        if (refi >= 0) {
+            const VarX rx = expand(var[refi], c);
+            emit_ref(S, p, OUT_REF, &rx, var[refi].al->d, var[refi].al->len, 0, 0, 0, 0);
+        }
+    }
     Strs<CAP>* st;
     Strs<CAP> stv;
     st = &stv;
@@ -876,15 +914,34 @@ __device__ void score_position(const ScoreIn& S, int p, bool dense_only, int slo
             // flanks
             SS<64>& lf = st->left;
             SS<64>& rt = st->right;
+            bool left_set = false;
             if (!thrown) {
                 int a = startPosition - 20 < 1 ? 1 : startPosition - 20;
-                if (!rf.join(lf, a, startPosition - 1)) thrown = true;
+                if (!rf.join(lf, a, startPosition - 1))
+                    thrown = true;
+                else
+                    left_set = true;
             }
             if (!thrown) {
                 int b = endPosition + 20 > chr0 ? chr0 : endPosition + 20;
                 if (!rf.join(rt, endPosition + 1, b)) thrown = true;
             }
-            if (thrown) return;  // the reference's per-position catch: later variants keep no alleles
+            if (thrown) {
+                // A reference lookup outside the window throws inside collectReferenceVariants and
+                // the per-position catch (ToVarsBuilder.cpp:105, :183) ends it: this variant and
+                // the later ones keep createVariant's fields (no alleles, so simple mode prints none
+                // of them); the position's Vars entry stays, which the somatic printer reads.
+                if (S.somatic == SCORE_SOMATIC_VARS) {
+                    for (int vj = vi; vj < nv; vj++) {
+                        if (vj == refi) continue;
+                        const VarX xr = expand(var[vj], c);
+                        const bool lf_kept = vj == vi && left_set;
+                        emit_ref(S, p, OUT_VARIANT_RAW, &xr, var[vj].al->d, var[vj].al->len, 0, 0, 0, 0, order++,
+                                 lf_kept ? lf.c : nullptr, lf_kept ? lf.n : 0);
+                    }
+                }
+                return;
+            }
             if (ra.overflow || va.overflow || g1.overflow || g2.overflow || lf.overflow || rt.overflow) {
                 atomicAdd(&S.counters[0], 1ull);
                 return;
@@ -934,7 +991,7 @@ __device__ void score_position(const ScoreIn& S, int p, bool dense_only, int slo
             bool hasN = false;
             for (int i = 0; i < ra.n; i++)
                 if (ra.c[i] == 'N') hasN = true;
-            if (hasN) continue;
+            if (hasN && S.somatic == SCORE_SIMPLE) continue;
             int vt;
             if (ra.n == 1 && va.n == 1)
                 vt = VT_SNV;
@@ -965,10 +1022,10 @@ __device__ void score_position(const ScoreIn& S, int p, bool dense_only, int slo
                     if (vt == VT_SNV || (ra.n < 3 && va.n < 3)) good = false;
                 }
             }
-            if (!good && !c.doPileup) continue;
+            if (!good && !c.doPileup && S.somatic == SCORE_SIMPLE) continue;
             // adjComplex (Variant.h:99-132)
             int sp = startPosition, ep = endPosition;
-            if (vt == VT_COMPLEX && !(va.n > 0 && va.c[0] == '<')) {
+            if (S.somatic == SCORE_SIMPLE && vt == VT_COMPLEX && !(va.n > 0 && va.c[0] == '<')) {
                 int n = 0;
                 while (ra.n - n > 1 && va.n - n > 1 && ra.c[n] == va.c[n]) n++;
                 if (n > 0) {
@@ -1003,22 +1060,8 @@ __device__ void score_position(const ScoreIn& S, int p, bool dense_only, int slo
                 }
             }
             // ---------------- append the printed record
-            unsigned oi = atomicAdd(S.n_out, 1u);
-            unsigned long long nbytes = (unsigned long long)(ra.n + va.n + gt.n + bs.n + lf.n + rt.n);
-            unsigned long long off = atomicAdd(S.arena_used, nbytes);
-            if (oi >= S.max_out || off + nbytes > S.arena_cap) {
-                atomicAdd(&S.counters[0], 1ull);
-                return;
-            }
-            char* w = S.arena + off;
-            const char* pc[6] = {ra.c, va.c, gt.c, bs.c, lf.c, rt.c};
-            const int pn[6] = {ra.n, va.n, gt.n, bs.n, lf.n, rt.n};
             OutRec o;
-            for (int k = 0; k < 6; k++) {
-                for (int i = 0; i < pn[k]; i++) *w++ = pc[k][i];
-                o.len[k] = (uint16_t)pn[k];
-            }
-            o.str_off = (uint32_t)off;
+            o.kind = OUT_VARIANT;
             o.position = p;
             o.order = order++;
             o.start = sp;
@@ -1055,7 +1098,9 @@ __device__ void score_position(const ScoreIn& S, int p, bool dense_only, int slo
                 if (bc != 0 && ad != 0) ratio = ad > bc ? (ad / bc) : (bc / ad);
                 o.fisher_ratio = ratio;
             }
-            S.out[oi] = o;
+            const char* pc[7] = {ra.c, va.c, gt.c, bs.c, lf.c, rt.c, d};
+            const int pn[7] = {ra.n, va.n, gt.n, bs.n, lf.n, rt.n, dl};
+            if (!emit_out(S, o, pc, pn)) return;
         }
     }
     // positions with only a reference allele print only with -p (pileup mode); not modelled here

@@ -80,14 +80,16 @@ class Engine:
     def __del__(self):
         self.close()
 
+    def _reads(self, reads):
+        if isinstance(reads, DeviceReads):
+            return reads.rv_reads()
+        arrays = [np.ascontiguousarray(getattr(reads, k)) for k in _ARRAYS]
+        self._keep.append(arrays)
+        return _lib.RvReads(reads.n, *[_ptr(a) for a in arrays], 0)
+
     def _args(self, chrom, start, end, reads, contig_name, contig_bases, gene=None):
         self._keep = []
-        if isinstance(reads, DeviceReads):
-            r = reads.rv_reads()
-        else:
-            arrays = [np.ascontiguousarray(getattr(reads, k)) for k in _ARRAYS]
-            self._keep.append(arrays)
-            r = _lib.RvReads(reads.n, *[_ptr(a) for a in arrays], 0)
+        r = self._reads(reads)
         g = (gene if gene is not None else chrom).encode()
         c = chrom.encode()
         self._keep += [g, c, contig_bases]
@@ -116,6 +118,22 @@ class Engine:
         cs = ctypes.c_void_p()
         st = self.lib.rv_run_region(self.h, ctypes.byref(region), ctypes.byref(r), ctypes.byref(contig),
                                     ctypes.byref(cs))
+        if st != 0:
+            raise self._error(st)
+        n = ctypes.c_size_t()
+        p = self.lib.rv_callset_text(cs, ctypes.byref(n))
+        text = ctypes.string_at(p, n.value).decode()
+        self.lib.rv_callset_free(cs)
+        return text
+
+    def run_region_somatic(self, chrom, start, end, tumor, normal, contig_name, contig_bases, gene=None):
+        """Somatic mode for one region (the first and second samples of -b 'T|N'); returns the text
+        the reference's somatic printer writes for it."""
+        region, r1, contig = self._args(chrom, start, end, tumor, contig_name, contig_bases, gene)
+        r2 = self._reads(normal)
+        cs = ctypes.c_void_p()
+        st = self.lib.rv_run_region_somatic(self.h, ctypes.byref(region), ctypes.byref(r1), ctypes.byref(r2),
+                                            ctypes.byref(contig), ctypes.byref(cs))
         if st != 0:
             raise self._error(st)
         n = ctypes.c_size_t()

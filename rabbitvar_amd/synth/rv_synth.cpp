@@ -46,7 +46,9 @@ struct Opts {
     int longDelMin = 50, longDelMax = 300;
     double repeatFrac = 0.0;
     double errRate = 0.003;
-    double vafLo = 0.5, vafHi = 0.5;  // fraction of fragments drawn from the alt haplotype
+    double vafLo = 0.5, vafHi = 0.5;  // tumor: per-fragment probability of the somatic haplotype
+    double somaticRate = 0;           // somatic SNVs/indels per bp (haplotype 2 = haplotype 0 + these)
+    double normalDepth = 0;           // > 0: also write normal.rvb (germline haplotypes only)
 };
 
 struct Variant {
@@ -73,6 +75,7 @@ struct Hap {
     std::vector<int32_t> rpos;
 };
 
+// haplotype 2 carries haplotype 0's variants and the somatic ones
 Hap build_hap(const std::string& ref, const std::vector<Variant>& vars, int h) {
     Hap H;
     H.seq.reserve(ref.size() + 1024);
@@ -82,7 +85,7 @@ Hap build_hap(const std::string& ref, const std::vector<Variant>& vars, int h) {
     int n = (int)ref.size();
     std::vector<const Variant*> mine;
     for (const Variant& v : vars)
-        if (v.hap == h) mine.push_back(&v);
+        if (v.hap == h || (h == 2 && v.hap == 0)) mine.push_back(&v);
     while (p < n) {
         while (vi < mine.size() && mine[vi]->pos < p) vi++;
         if (vi < mine.size() && mine[vi]->pos == p) {
@@ -231,7 +234,9 @@ int main(int argc, char** argv) {
         else if (a == "--vaf") {
             o.vafLo = std::stod(nx());
             o.vafHi = std::stod(nx());
-        } else
+        } else if (a == "--somatic-rate") o.somaticRate = std::stod(nx());
+        else if (a == "--normal-depth") o.normalDepth = std::stod(nx());
+        else
             return usage();
     }
     std::mt19937_64 gref(o.seed * 1000003ULL + 1), gvar(o.seed * 1000003ULL + 2), gread(o.seed * 1000003ULL + 3),
@@ -307,17 +312,43 @@ int main(int argc, char** argv) {
             }
         }
     }
+    if (o.somaticRate > 0) {
+        // somatic variants, from their own stream so that the germline set does not change
+        std::mt19937_64 gsom(o.seed * 1000003ULL + 5);
+        for (int p = 150; p < o.len - 150; p++) {
+            double u = U(gsom);
+            if (u >= o.somaticRate) continue;
+            if (u < o.somaticRate * 5 / 6) {
+                if (!free_span(p - 3, p + 4)) continue;
+                char rb = ref[(size_t)p], alt;
+                do {
+                    alt = BASES[gsom() % 4];
+                } while (alt == rb);
+                vars.push_back({p, 0, 1, std::string(1, alt), 2});
+                mark(p - 3, p + 4);
+            } else {
+                int L = std::uniform_int_distribution<int>(1, std::max(1, o.indelMax))(gsom);
+                if (!free_span(p - 5, p + L + 5)) continue;
+                vars.push_back({p, 2, L, "", 2});
+                mark(p - 5, p + L + 5);
+            }
+        }
+    }
     std::sort(vars.begin(), vars.end(), [](const Variant& a, const Variant& b) { return a.pos < b.pos; });
-    Hap H[2] = {build_hap(ref, vars, 0), build_hap(ref, vars, 1)};
+    Hap H[3] = {build_hap(ref, vars, 0), build_hap(ref, vars, 1), Hap{}};
+    if (o.somaticRate > 0) H[2] = build_hap(ref, vars, 2);
     // ---------------- reads
-    long npairs = (long)(o.depth * o.len / (2.0 * o.readLen));
+    // tumor (or the only sample): haplotypes 0/1, and haplotype 2 with probability vaf when there
+    // are somatic variants; normal: haplotypes 0/1 from its own streams
+    auto gen_reads = [&](double depth, bool somatic, std::mt19937_64& gread, std::mt19937_64& gerr) {
+    long npairs = (long)(depth * o.len / (2.0 * o.readLen));
     std::vector<Rec> recs;
     recs.reserve((size_t)npairs * 2);
     std::normal_distribution<double> ins(o.insMu, o.insSd), qn(35, 5);
     for (long f = 0; f < npairs; f++) {
         double vaf = o.vafLo + (o.vafHi - o.vafLo) * U(gread);
-        (void)vaf;
         int h = (gread() & 1) ? 1 : 0;
+        if (somatic && U(gread) < vaf) h = 2;
         const Hap& hp = H[h];
         int hl = (int)hp.seq.size();
         int isz = (int)std::lround(ins(gread));
@@ -376,6 +407,10 @@ int main(int argc, char** argv) {
         recs.push_back(std::move(r2));
     }
     std::stable_sort(recs.begin(), recs.end(), [](const Rec& a, const Rec& b) { return a.pos0 < b.pos0; });
+    return recs;
+    };
+    std::vector<Rec> recs = gen_reads(o.depth, o.somaticRate > 0, gread, gerr);
+
     // ---------------- write
     {
         std::ofstream fa(o.out + "/ref.fa");
@@ -388,7 +423,7 @@ int main(int argc, char** argv) {
         static const char* kn[3] = {"SNV", "INS", "DEL"};
         for (const Variant& v : vars) tv << v.pos + 1 << "\t" << kn[v.kind] << "\t" << v.len << "\t" << v.alt << "\t" << v.hap << "\n";
     }
-    {
+    auto write_rvb = [&](const std::string& path, const std::vector<Rec>& recs) {
         int64_t n = (int64_t)recs.size(), nb = 0, nc = 0;
         for (const Rec& r : recs) {
             nb += (int64_t)r.bases.size();
@@ -416,7 +451,7 @@ int main(int argc, char** argv) {
         }
         so[(size_t)n] = b;
         co[(size_t)n] = c;
-        std::ofstream f(o.out + "/reads.rvb", std::ios::binary);
+        std::ofstream f(path, std::ios::binary);
         f.write("RVB1", 4);
         uint32_t ver = 1;
         f.write(reinterpret_cast<const char*>(&ver), 4);
@@ -436,8 +471,15 @@ int main(int argc, char** argv) {
         put(f, bases.data(), (size_t)nb);
         put(f, quals.data(), (size_t)nb);
         put(f, cig.data(), 4 * (size_t)nc);
-        printf("{\"reads\": %lld, \"bases\": %lld, \"cigar_ops\": %lld, \"variants\": %zu, \"len\": %d}\n", (long long)n,
-               (long long)nb, (long long)nc, vars.size(), o.len);
+        return n;
+    };
+    const int64_t n = write_rvb(o.out + "/reads.rvb", recs);
+    int64_t nn = 0;
+    if (o.normalDepth > 0) {
+        std::mt19937_64 gnread(o.seed * 1000003ULL + 6), gnerr(o.seed * 1000003ULL + 7);
+        nn = write_rvb(o.out + "/normal.rvb", gen_reads(o.normalDepth, false, gnread, gnerr));
     }
+    printf("{\"reads\": %lld, \"normal_reads\": %lld, \"variants\": %zu, \"len\": %d}\n", (long long)n, (long long)nn,
+           vars.size(), o.len);
     return 0;
 }

@@ -17,6 +17,13 @@ CASES = {
     # dense variant edge cases: high indel rate, long indels, low quality tail errors
     "edge": ["--seed", "5", "--len", "60000", "--depth", "40", "--indel-rate", "0.002", "--indel-max", "40",
              "--snv-rate", "0.004", "--long-dels", "6", "--repeat-frac", "0.05", "--err-rate", "0.01"],
+    # tumor/normal pair: 200 kb, tumor 60x, normal 40x, somatic SNVs/indels at 5e-4 with VAF 0.05-0.4
+    "som": ["--seed", "5", "--len", "200000", "--depth", "60", "--normal-depth", "40", "--somatic-rate", "0.0005",
+            "--vaf", "0.05", "0.4", "--indel-rate", "0.0002"],
+    # tumor/normal pair with the dense edge-case mix above and a thin normal
+    "som_edge": ["--seed", "7", "--len", "60000", "--depth", "40", "--normal-depth", "30", "--indel-rate", "0.002",
+                 "--indel-max", "40", "--snv-rate", "0.004", "--long-dels", "6", "--repeat-frac", "0.05",
+                 "--err-rate", "0.01", "--somatic-rate", "0.001"],
 }
 
 
@@ -66,3 +73,13 @@ def run_oracle(d, region, extra=(), dump=None, out="out.txt"):
         with open(os.path.join(d, dump)) as f:
             dump_text = f.read()
     return text, dump_text
+
+
+def run_oracle_somatic(d, region, extra=(), out="somatic.txt"):
+    """The oracle in somatic mode (-b 'tumor|normal' -N 'T|N') over one region."""
+    bams = os.path.join(d, "reads.rvb") + "|" + os.path.join(d, "normal.rvb")
+    cmd = [ORACLE, "-G", os.path.join(d, "ref.fa"), "-b", bams, "-N", "T|N", "-R", region,
+           "--out", os.path.join(d, out)] + list(extra)
+    subprocess.run(cmd, check=True, capture_output=True)
+    with open(os.path.join(d, out)) as f:
+        return f.read()
