@@ -60,8 +60,8 @@ def dataset(tmp_path_factory, tools):
     return get
 
 
-def run_oracle(d, region, extra=(), dump=None, out="out.txt"):
-    cmd = [ORACLE, "-G", os.path.join(d, "ref.fa"), "-b", os.path.join(d, "reads.rvb"), "-N", "S1", "-R", region,
+def run_oracle(d, region, extra=(), dump=None, out="out.txt", reads="reads.rvb"):
+    cmd = [ORACLE, "-G", os.path.join(d, "ref.fa"), "-b", os.path.join(d, reads), "-N", "S1", "-R", region,
            "--out", os.path.join(d, out)] + list(extra)
     if dump:
         cmd += ["--dump-pileup", os.path.join(d, dump)]
