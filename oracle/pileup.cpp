@@ -39,7 +39,7 @@ static int nt16_code(char c) {
 void preprocess_filter(const Config& c, const std::vector<const Read*>& all, std::vector<const Read*>& out,
                        Counters& ctr) {
     // recordPreprocessor.cpp:121-176 (downsampling -Z is not modelled: it is unseeded rand())
-    int firstMatchingPosition = -1;
+    int firstMatchingPosition = 0;
     std::set<std::string> dups;
     for (const Read* rp : all) {
         const Read& r = *rp;
@@ -58,7 +58,8 @@ void preprocess_filter(const Config& c, const std::vector<const Read*>& all, std
         }
         if (c.removeDuplicatedReads) {
             int as = r.pos0 + 1, ms = r.mpos0 + 1;
-            std::string mref = !(r.flag & 1) ? "*" : (r.tid == r.mtid ? "=" : "chrM");
+            // getMateReferenceName (:80-90); a contig other than the read's own is named by its id
+            std::string mref = !(r.flag & 1) ? "*" : (r.tid == r.mtid ? "=" : "tid" + std::to_string(r.mtid));
             if (as != firstMatchingPosition) dups.clear();
             if (ms < 10) {
                 std::string k = std::to_string(as) + "-" + mref + "-" + std::to_string(ms);
@@ -70,8 +71,9 @@ void preprocess_filter(const Config& c, const std::vector<const Read*>& all, std
                 firstMatchingPosition = as;
             } else if ((r.flag & 1) && (r.flag & 4)) {
                 std::string cs;
-                static const char* ops = "MIDNSHP=XB";
-                for (uint32_t w : r.cigar) cs += std::to_string(clen(w)) + ops[cop(w) % 10];
+                // get_cigar_string (VariationUtils.h:563-570) through htslib's bam_cigar_opchr
+                static const char* ops = "MIDNSHP=XB??????";
+                for (uint32_t w : r.cigar) cs += std::to_string(clen(w)) + ops[cop(w)];
                 std::string k = std::to_string(as) + "-" + cs;
                 if (dups.count(k)) {
                     ctr.reads_filtered++;

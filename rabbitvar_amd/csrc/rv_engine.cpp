@@ -73,7 +73,7 @@ Engine::~Engine() {
                      &b_arena,     &b_clips,   &b_clip_arena, &b_anchors, &b_counts,   &b_gather_ids, &b_gather_out,
                      &b_cx_pos,    &b_cx_off,  &b_cx,     &b_cx_arena,   &b_cx_cov,   &b_cx_key, &b_out,
                      &b_out_arena, &b_score_ctr, &b_status, &b_keys, &b_keys2, &b_kpos, &b_kpos2, &b_sort_tmp,
-                     &b_kcount, &b_args, &b_kpos3, &b_nopass, &b_rows2, &b_want};
+                     &b_kcount, &b_args, &b_kpos3, &b_nopass, &b_rows2, &b_want, &b_dup};
     // teardown: nothing to report a failure to
     for (auto* b : all)
         if (b->p) (void)hipFree(b->p);
@@ -109,7 +109,6 @@ static void window_bounds(const Config& c, const rv_region& r, int chrLen, int& 
 
 rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_contig& contig, bool download_rows,
                          PileupResult& out, const PileupHooks* hooks) {
-    if (cfg.removeDuplicatedReads) return fail(RV_ERR_UNSUPPORTED, "-t (duplicate removal) is not implemented");
     if (cfg.crisprCuttingSite != 0) return fail(RV_ERR_UNSUPPORTED, "CRISPR mode is not implemented");
     auto h0 = std::chrono::steady_clock::now();
     double host_ms = 0;
@@ -199,12 +198,17 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
     hargs.R = R;
     hargs.W = W;
     hargs.cfg = cfg;
+    hargs.dup = nullptr;
+    if (cfg.removeDuplicatedReads) {
+        hargs.dup = (uint8_t*)grow(b_dup, (size_t)std::max<int64_t>(n, 1));
+        if (!hargs.dup) return fail(RV_ERR_OOM, "device allocation failed");
+    }
     RV_HIP(hipStreamSynchronize(stream));
     RV_HIP(hipMemcpy(dargs, &hargs, sizeof hargs, hipMemcpyHostToDevice));
     args_dev = dargs;
     args_n = n;
     RV_HIP(hipEventRecord(ev[0], stream));
-    RV_HIP(launch_prep(dargs, n, prep, cig, ctr, span, stream));
+    RV_HIP(launch_prep(dargs, n, prep, cig, ctr, span, cfg.removeDuplicatedReads != 0, stream));
     RV_HIP(hipEventRecord(ev[1], stream));
     int max_span = 0;
     RV_HIP(hipMemcpyAsync(&max_span, span, sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -258,7 +262,7 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
         if (attempt > 0) {
             RV_HIP(launch_fill_rows(rows, W.dense_n, stream));
             RV_HIP(hipMemsetAsync(ctr, 0, sizeof(WalkCounters), stream));
-            RV_HIP(launch_prep(dargs, n, prep, cig, ctr, span, stream));
+            RV_HIP(launch_prep(dargs, n, prep, cig, ctr, span, false, stream));  // duplicate marks kept
         }
         RV_HIP(hipEventRecord(ev[2], stream));
         RV_HIP(launch_walk(dargs, n, prep, cig, stream));

@@ -8,7 +8,7 @@
 namespace rv {
 
 hipError_t launch_prep(const WalkArgs* A, int64_t n, PrepRec* prep, uint32_t* cig, WalkCounters* ctr, int* max_span,
-                       hipStream_t s);
+                       bool dedup, hipStream_t s);
 hipError_t launch_fill_rows(DenseRow* rows, int n, hipStream_t s);
 hipError_t launch_walk(const WalkArgs* A, int64_t n, const PrepRec* prep, const uint32_t* cig, hipStream_t s);
 hipError_t launch_no_passing(const WalkArgs* A, int64_t n, int start, int end, unsigned* count, hipStream_t s);

@@ -10,7 +10,13 @@ namespace rv {
 __global__ void __launch_bounds__(256) k_prep(const WalkArgs* A, PrepRec* prep, uint32_t* cig, WalkCounters* ctr,
                                                 int* max_span) {
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < A->R.n) prep_one(r, A->R, A->W, A->cfg, prep, cig, ctr, max_span);
+    if (r < A->R.n) prep_one(r, A->R, A->W, A->cfg, prep, cig, ctr, max_span, A->dup);
+}
+
+// -t: marks the reads the record filter drops as duplicates (before prep, which then skips them)
+__global__ void __launch_bounds__(256) k_dedup(const WalkArgs* A) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < A->R.n) A->dup[r] = is_duplicate(A->R, A->cfg, r) ? 1 : 0;
 }
 
 __global__ void __launch_bounds__(256) k_fill_rows(DenseRow* rows, int n) {
@@ -77,8 +83,13 @@ __global__ void __launch_bounds__(256) k_no_passing(const WalkArgs* A, int start
 static inline unsigned grid_for(int64_t n, int block) { return (unsigned)((n + block - 1) / block); }
 
 hipError_t launch_prep(const WalkArgs* A, int64_t n, PrepRec* prep, uint32_t* cig, WalkCounters* ctr, int* max_span,
-                       hipStream_t s) {
+                       bool dedup, hipStream_t s) {
     if (n <= 0) return hipSuccess;
+    if (dedup) {
+        hipLaunchKernelGGL(k_dedup, dim3(grid_for(n, 256)), dim3(256), 0, s, A);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(k_prep, dim3(grid_for(n, 256)), dim3(256), 0, s, A, prep, cig, ctr, max_span);
     return hipGetLastError();
 }

@@ -34,7 +34,60 @@ struct WalkArgs {
     Window W;
     Config cfg;
     WalkSink S;
+    uint8_t* dup;  // -t: per read, 1 = dropped as a duplicate (nullptr without -t)
 };
+
+// ---------------------------------------------------------------- duplicate removal (-t)
+// recordPreprocessor.cpp:153-176.  The reference keeps a set of keys, clears it whenever a record's
+// alignment start differs from the start of the key it inserted last, and drops a read whose key
+// is already in the set.  Keys exist only for reads whose mate starts before base 10 (start, mate
+// contig name, mate start) and for paired reads flagged unmapped (start, CIGAR text).  Over the
+// records in iterator order, with the ones the flag/MAPQ/length filters dropped invisible to it,
+// that is: a read is a duplicate when an earlier read with the same key lies in the same run of
+// consecutive records that share its alignment start.
+RV_HD bool rec_filtered(const ReadsDev& R, const Config& c, int64_t i) {
+    const int lseq = (int)(R.seq_off[i + 1] - R.seq_off[i]);
+    return (R.flag[i] & c.samfilter) != 0 || (int)R.mapq[i] < c.mappingQuality || lseq == 1;
+}
+// 0: no key; 1: keyed by the mate; 2: keyed by the CIGAR
+RV_HD int dup_kind(const ReadsDev& R, int64_t i) {
+    if (R.mpos0[i] + 1 < 10) return 1;
+    const uint16_t f = R.flag[i];
+    return ((f & 1) && (f & 4)) ? 2 : 0;
+}
+// getMateReferenceName (recordPreprocessor.cpp:80-90): "*" unpaired, "=" same contig, else the
+// mate contig's name (one name per contig id)
+RV_HD int mate_name(const ReadsDev& R, int64_t i) {
+    if (!(R.flag[i] & 1)) return -3;
+    if (R.tid[i] == R.mtid[i]) return -2;
+    return R.mtid[i];
+}
+RV_HD bool same_dup_key(const ReadsDev& R, int64_t i, int64_t j, int kind) {
+    if (kind == 1) return R.mpos0[i] == R.mpos0[j] && mate_name(R, i) == mate_name(R, j);
+    const int64_t a = R.cig_off[i], b = R.cig_off[j];
+    const int64_t na = R.cig_off[i + 1] - a;
+    if (na != R.cig_off[j + 1] - b) return false;
+    for (int64_t k = 0; k < na; k++) {
+        const uint32_t x = R.cigar[a + k], y = R.cigar[b + k];
+        if ((x >> 4) != (y >> 4)) return false;
+        // bam_cigar_opchr prints every op code past 'B' (9) as '?'
+        const uint32_t ox = (x & 15u) > 9u ? 10u : (x & 15u), oy = (y & 15u) > 9u ? 10u : (y & 15u);
+        if (ox != oy) return false;
+    }
+    return true;
+}
+RV_HD bool is_duplicate(const ReadsDev& R, const Config& c, int64_t i) {
+    if (rec_filtered(R, c, i)) return false;
+    const int kind = dup_kind(R, i);
+    if (kind == 0) return false;
+    const int32_t s = R.pos0[i];
+    for (int64_t j = i - 1; j >= 0; j--) {
+        if (rec_filtered(R, c, j)) continue;
+        if (R.pos0[j] != s) return false;
+        if (dup_kind(R, j) == kind && same_dup_key(R, i, j, kind)) return true;
+    }
+    return false;
+}
 
 __device__ __forceinline__ uint32_t packed_seq_word(const uint8_t* seq, int lseq, int w) {
     uint32_t v = 0;
@@ -49,7 +102,7 @@ __device__ __forceinline__ uint32_t packed_seq_word(const uint8_t* seq, int lseq
 }
 
 __device__ inline void prep_one(int64_t r, const ReadsDev& R, const Window& W, const Config& cfg, PrepRec* prep,
-                                uint32_t* cig, WalkCounters* ctr, int* max_span) {
+                                uint32_t* cig, WalkCounters* ctr, int* max_span, const uint8_t* dup) {
     PrepRec p;
     p.status = RD_PARSE;
     p.flags = 0;
@@ -77,7 +130,7 @@ __device__ inline void prep_one(int64_t r, const ReadsDev& R, const Window& W, c
         atomicMax(max_span, lseq + reflen + 8);
         p.reflen = reflen;
     }
-    if ((flag & cfg.samfilter) != 0 || (int)R.mapq[r] < cfg.mappingQuality || lseq == 1) {
+    if ((flag & cfg.samfilter) != 0 || (int)R.mapq[r] < cfg.mappingQuality || lseq == 1 || (dup && dup[r])) {
         p.status = RD_FILTERED;
         prep[r] = p;
         return;

@@ -49,6 +49,10 @@ struct Opts {
     double vafLo = 0.5, vafHi = 0.5;  // tumor: per-fragment probability of the somatic haplotype
     double somaticRate = 0;           // somatic SNVs/indels per bp (haplotype 2 = haplotype 0 + these)
     double normalDepth = 0;           // > 0: also write normal.rvb (germline haplotypes only)
+    // record-level shapes for the duplicate filter (-t); their own random stream
+    double dupRate = 0;        // each record is followed by an identical copy with this probability
+    double singleEndFrac = 0;  // records written unpaired (no mate: mate start 0)
+    double unmappedFrac = 0;   // paired records flagged unmapped (0x4), alignment kept
 };
 
 struct Variant {
@@ -204,7 +208,9 @@ int usage() {
     fprintf(stderr,
             "rvsynth --out DIR [--seed N] [--len L] [--depth D] [--read-len R] [--snv-rate X]\n"
             "        [--indel-rate X] [--indel-max N] [--long-dels N] [--repeat-frac F]\n"
-            "        [--err-rate X] [--insert-mu M] [--insert-sd S] [--vaf LO HI]\n");
+            "        [--err-rate X] [--insert-mu M] [--insert-sd S] [--vaf LO HI]\n"
+            "        [--somatic-rate X] [--normal-depth D] [--dup-rate X] [--single-end-frac F]\n"
+            "        [--unmapped-frac F]\n");
     return 2;
 }
 
@@ -236,6 +242,9 @@ int main(int argc, char** argv) {
             o.vafHi = std::stod(nx());
         } else if (a == "--somatic-rate") o.somaticRate = std::stod(nx());
         else if (a == "--normal-depth") o.normalDepth = std::stod(nx());
+        else if (a == "--dup-rate") o.dupRate = std::stod(nx());
+        else if (a == "--single-end-frac") o.singleEndFrac = std::stod(nx());
+        else if (a == "--unmapped-frac") o.unmappedFrac = std::stod(nx());
         else
             return usage();
     }
@@ -407,6 +416,23 @@ int main(int argc, char** argv) {
         recs.push_back(std::move(r2));
     }
     std::stable_sort(recs.begin(), recs.end(), [](const Rec& a, const Rec& b) { return a.pos0 < b.pos0; });
+    if (o.dupRate > 0 || o.singleEndFrac > 0 || o.unmappedFrac > 0) {
+        std::mt19937_64 gdup(gread());
+        std::vector<Rec> shaped;
+        shaped.reserve(recs.size() + recs.size() / 4);
+        for (Rec& r : recs) {
+            if (U(gdup) < o.singleEndFrac) {
+                r.flag &= (uint16_t)~(0x1 | 0x2 | 0x20 | 0x40 | 0x80);
+                r.mpos0 = -1;
+            } else if (U(gdup) < o.unmappedFrac) {
+                r.flag |= 0x4;
+            }
+            const bool copy = U(gdup) < o.dupRate;
+            shaped.push_back(r);
+            if (copy) shaped.push_back(r);
+        }
+        recs.swap(shaped);
+    }
     return recs;
     };
     std::vector<Rec> recs = gen_reads(o.depth, o.somaticRate > 0, gread, gerr);

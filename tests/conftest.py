@@ -17,6 +17,9 @@ CASES = {
     # dense variant edge cases: high indel rate, long indels, low quality tail errors
     "edge": ["--seed", "5", "--len", "60000", "--depth", "40", "--indel-rate", "0.002", "--indel-max", "40",
              "--snv-rate", "0.004", "--long-dels", "6", "--repeat-frac", "0.05", "--err-rate", "0.01"],
+    # duplicate-filter shapes (-t): 30% of records duplicated, half unpaired, 5% flagged unmapped
+    "dups": ["--seed", "11", "--len", "60000", "--depth", "40", "--indel-rate", "0.001", "--dup-rate", "0.3",
+             "--single-end-frac", "0.5", "--unmapped-frac", "0.05"],
     # tumor/normal pair: 200 kb, tumor 60x, normal 40x, somatic SNVs/indels at 5e-4 with VAF 0.05-0.4
     "som": ["--seed", "5", "--len", "200000", "--depth", "60", "--normal-depth", "40", "--somatic-rate", "0.0005",
             "--vaf", "0.05", "0.4", "--indel-rate", "0.0002"],

@@ -199,6 +199,8 @@ int main(int argc, char** argv) {
     cfg.vext = 2; cfg.mismatch = 8; cfg.performLocalRealignment = k; cfg.indelsize = 50; cfg.minBiasReads = 2;
     cfg.minr = 2; cfg.samfilter = 0x504; cfg.readPosFilter = 5; cfg.SVMINLEN = 1000; cfg.referenceExtension = 1200;
     cfg.fisher = fisher;
+    if (const char* e = getenv("RV_SIM_DEDUP")) cfg.removeDuplicatedReads = atoi(e);         // -t
+    if (const char* e = getenv("RV_SIM_SAMFILTER")) cfg.samfilter = (int)strtol(e, nullptr, 16);  // -F
     const int chrLen = (int)seq.size();
     Window W;
     W.region_start = rs;
@@ -213,7 +215,13 @@ int main(int argc, char** argv) {
     std::vector<uint32_t> cg((size_t)std::max<int64_t>(m, 1) * MAXC);
     WalkCounters ctr{};
     int span = 0;
-    for (int64_t r = 0; r < m; r++) prep_one(r, R, W, cfg, prep.data(), cg.data(), &ctr, &span);
+    std::vector<uint8_t> dup;
+    if (cfg.removeDuplicatedReads) {
+        dup.resize((size_t)std::max<int64_t>(m, 1));
+        for (int64_t r = 0; r < m; r++) dup[(size_t)r] = is_duplicate(R, cfg, r) ? 1 : 0;
+    }
+    for (int64_t r = 0; r < m; r++)
+        prep_one(r, R, W, cfg, prep.data(), cg.data(), &ctr, &span, dup.empty() ? nullptr : dup.data());
     const int pad = 2 * std::max(span, 64) + 8;
     W.dense_lo = rs - pad;
     W.dense_n = (re + pad) - W.dense_lo + 1;

@@ -113,3 +113,42 @@ def expected_ref_rows(kind, s):
         rows[p] = (10, fwd, 10 - fwd, 10, 0, sum(tps), 300, 600, sum(nm for *_, nm in cov),
                    int(len(set(tps)) > 1), 0)
     return rows
+
+
+# Duplicate filter (-t).  Records in iterator order, as (start offset, flag, mate pos0, SNV carrier):
+# the reference drops a read whose key (start, mate contig, mate start) for a mate starting before
+# base 10 was already seen since the last change of alignment start.
+DUP_RECORDS = [
+    (0, 0x0, -1, True),     # r0 unpaired: key 201-*-0, kept
+    (0, 0x10, -1, True),    # r1 unpaired: same key, dropped
+    (0, 0x1, 500, False),   # r2 paired, mate at 501: no key, kept
+    (0, 0x11, 3, False),    # r3 paired, mate at 4: key 201-=-4, kept
+    (0, 0x1, 3, False),     # r4 same key as r3, dropped
+    (0, 0x0, -1, True),     # r5 key of r0, still the same start: dropped
+    (1, 0x0, -1, False),    # r6 starts at 202: the set is cleared; key 202-*-0, kept
+    (0, 0x10, -1, False),   # r7 back at 201 (a second file's records): cleared again, kept
+    (0, 0x5, 3, True),      # r8 flagged unmapped: removed by -F 0x504 before the duplicate filter
+]
+DUP_KEPT = [0, 2, 3, 6, 7]
+
+
+def build_dups(d):
+    s = reference()
+    alt = "T" if s[EV - 1] != "T" else "A"
+    recs = []
+    for off, flag, mpos0, carrier in DUP_RECORDS:
+        p0 = START + off
+        seq = s[p0:p0 + RLEN]
+        nm = 0
+        if carrier:
+            seq = seq[:EV - 1 - p0] + alt + seq[EV - p0:]
+            nm = 1
+        recs.append({"pos0": p0, "flag": flag, "mapq": 60, "nm": nm, "seq": seq, "qual": [30] * RLEN,
+                     "cigar": [(0, RLEN)], "mpos0": mpos0, "tid": 0, "mtid": 0 if flag & 1 else -1})
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, "ref.fa"), "w") as f:
+        f.write(">chrT\n")
+        for i in range(0, L, 60):
+            f.write(s[i:i + 60] + "\n")
+    write_rvb(os.path.join(d, "reads.rvb"), batch_from_records(recs))
+    return s, alt

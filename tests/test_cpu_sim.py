@@ -42,6 +42,24 @@ def test_engine_stages_match_oracle(sim, dataset, case, k, fisher):
     assert got == want  # same lines in the same (robin_hood slot) order
 
 
+@pytest.mark.parametrize("samfilter", ["504", "0"])
+def test_engine_stages_duplicate_filter(sim, dataset, samfilter):
+    """-t through the device duplicate-filter body, with the default -F and with -F 0 (which lets the
+    reads flagged unmapped reach the CIGAR-keyed branch)."""
+    d = dataset("dups")
+    extra = ["--fisher", "-t", "-F", samfilter]
+    want, _ = run_oracle(d, "chrS:1-60000", extra, out=f"sim_dup_{samfilter}.txt")
+    plain, _ = run_oracle(d, "chrS:1-60000", ["--fisher", "-F", samfilter], out=f"sim_nodup_{samfilter}.txt")
+    out = os.path.join(d, f"cpu_sim_dup_{samfilter}.txt")
+    env = dict(os.environ, RV_SIM_DEDUP="1", RV_SIM_SAMFILTER=samfilter)
+    subprocess.run([sim, os.path.join(d, "ref.fa"), os.path.join(d, "reads.rvb"), "1", "60000", "1", "1", out],
+                   check=True, env=env, capture_output=True)
+    with open(out) as f:
+        got = f.read()
+    assert len(want.splitlines()) > 500 and want != plain  # the filter changes the calls
+    assert got == want
+
+
 @pytest.mark.parametrize("start,end", [(1, 1), (5000, 5150), (59000, 60000)])
 def test_engine_stages_subregions(sim, dataset, start, end):
     d = dataset("edge")

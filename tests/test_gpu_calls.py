@@ -63,3 +63,19 @@ def test_empty_region(dataset):
     got = eng.run_region(name, 100, 200, reads.take([]), name, seq)
     eng.close()
     assert got == ""
+
+
+@pytest.mark.parametrize("samfilter", [0x504, 0])
+def test_duplicate_filter_matches_oracle(dataset, samfilter):
+    """-t over records with duplicates, unpaired reads and reads flagged unmapped."""
+    from rabbitvar_amd.engine import Engine
+
+    d = dataset("dups")
+    name, seq, reads = _inputs(d)
+    want, _ = run_oracle(d, f"{name}:1-{len(seq)}", ["--fisher", "-t", "-F", f"{samfilter:x}"],
+                         out=f"dup_{samfilter}.txt")
+    eng = Engine(sample="S1", fisher=1, removeDuplicatedReads=1, samfilter=samfilter)
+    got = eng.run_region(name, 1, len(seq), reads.select(1, len(seq)), name, seq)
+    eng.close()
+    assert len(want.splitlines()) > 500
+    assert got == want

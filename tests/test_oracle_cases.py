@@ -112,3 +112,22 @@ def test_realignment_off_changes_nothing_here(case):
     a, _ = run(d)
     b, _ = run(d, ["-k", "0"])
     assert a == b
+
+
+@pytest.mark.parametrize("dedup", [False, True])
+def test_duplicate_filter(tmp_path, tools, dedup):
+    """-t (recordPreprocessor.cpp:153-176) on hand-built records: which reads survive follows from the
+    key rule and the clear-on-new-start rule (handbuilt.DUP_RECORDS)."""
+    d = str(tmp_path / "dups")
+    s, alt = hb.build_dups(d)
+    _, dump = run(d, ["-t"] if dedup else [])
+    V, I, C = parse_dump(dump)
+    kept = hb.DUP_KEPT if dedup else [i for i, r in enumerate(hb.DUP_RECORDS) if not r[1] & 0x4]
+    recs = [hb.DUP_RECORDS[i] for i in kept]
+    # coverage at 251 and at 201 (r6 starts at 202)
+    assert C[hb.EV] == len(recs)
+    assert C[hb.START + 1] == sum(1 for r in recs if r[0] == 0)
+    carriers = [r for r in recs if r[3]]
+    assert V[(hb.EV, alt)][0] == len(carriers)
+    assert V[(hb.EV, alt)][1:3] == (sum(1 for r in carriers if not r[1] & 0x10),
+                                   sum(1 for r in carriers if r[1] & 0x10))
