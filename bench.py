@@ -106,7 +106,7 @@ def main():
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(a.steps):
-        step()
+        text = step()
         st = eng.stats()
         walk_ms.append(st["walk_ms"])
         for k in stage:
@@ -149,6 +149,42 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": round(avg_walk_s * 1e3, 4)},
     }
+    # screened mode: the realigner skips the candidates the trained network scores below its
+    # threshold; the same workload, timed the same way, and its call set against the exact one
+    from rabbitvar_amd import screen
+
+    params, threshold, _ = screen.load_model()
+    eng.set_screen(params, threshold)
+    for i in range(a.warmup):
+        step()
+    sc_stage = {k: 0.0 for k in ("realign_ms", "wall_ms")}
+    skipped = cands = 0
+    scr_ms = 0.0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(a.steps):
+        stext = step()
+        st = eng.stats()
+        for k in sc_stage:
+            sc_stage[k] += st[k]
+        skipped, cands = st["screen_skipped"], st["screen_candidates"]
+        scr_ms += st["screen_ms"]
+    torch.cuda.synchronize()
+    sdt = time.perf_counter() - t0
+    eng.set_screen(None, 0)
+    cmp_ = screen.compare(text, stext)
+    res["screened"] = {"value": round(batch.n * a.steps / sdt, 1), "unit": "reads/s",
+                       "speedup": round(dt / sdt, 4), "threshold": threshold,
+                       "recall": round(cmp_["recall"], 6), "precision": round(cmp_["precision"], 6),
+                       "identical_lines": round(cmp_["identical_lines"], 6), "calls": cmp_["exact_calls"],
+                       "candidates": cands, "skipped": skipped, "screen_ms": round(scr_ms / a.steps, 3),
+                       "screen_list_ms": round(st["screen_list_ms"], 3),
+                       "stages_ms_per_step": {k: round(v / a.steps, 3) for k, v in sc_stage.items()},
+                       "realign_detail_ms_last_step": dict(zip(["rl_prefetch", "rl_mnp", "rl_del", "rl_ins",
+                                                                "rl_lgins", "record_list"],
+                                                               [round(x, 2) for x in st["detail_ms"][:6]]))}
+    log(f"screened: {res['screened']['value']} reads/s ({res['screened']['speedup']}x), recall "
+        f"{cmp_['recall']:.6f}, precision {cmp_['precision']:.6f}, skipped {skipped}/{cands}")
     if not a.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(a.data)
     eng.close()

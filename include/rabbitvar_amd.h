@@ -150,6 +150,10 @@ typedef struct rv_stats {
                                   event reduction, clip reduction, noPassingReads, dense-row fetches on
                                   a cache miss, then the counts of those two */
     double call_ms;            /* rv_run_region including the release of its host working state */
+    double screen_ms;          /* realignment screen: candidate list, features, scores (0 when off) */
+    int64_t screen_candidates; /* realignment candidates the screen saw */
+    int64_t screen_skipped;    /* candidates the realigner skipped (screened mode) */
+    double screen_list_ms;     /* of screen_ms: the host's candidate list and neighbourhood tables */
 } rv_stats;
 
 void rv_config_default(rv_config* cfg);
@@ -179,6 +183,38 @@ void rv_callset_free(rv_callset* cs);
 rv_status rv_get_stats(const rv_engine* e, rv_stats* out);
 const char* rv_last_error(const rv_engine* e);
 void rv_close(rv_engine* e);
+
+/* ---- Realignment screen.  No counterpart in the reference, whose realigner
+ * (src/VariationRealigner.cpp:135-161) visits every candidate; exact mode (the default) never uses
+ * it.  The network (32 features -> 64 -> 64 -> 1, BF16 products, FP32 sums) scores each realignment
+ * candidate (an adjustMNP / realigndel / realignins candidate or a realignlgins30 5' clip cluster)
+ * by how likely the realigner is to change a count there. */
+
+/* Screened mode: the realigner skips candidates scoring below `threshold`.  params: the network's
+ * rv_mlp_param_count() floats.  params == NULL or threshold <= 0 returns to exact mode. */
+rv_status rv_set_screen(rv_engine* e, const float* params, size_t n_params, float threshold);
+/* Runs one region exactly and returns its candidates' features (n x 32 floats), change labels
+ * (1: visiting the candidate changed a count) and kinds (0 MNP, 1 deletion, 2 insertion, 3 clip
+ * cluster).  Free the three arrays with rv_free. */
+rv_status rv_screen_dataset(rv_engine* e, const rv_region* region, const rv_reads* reads, const rv_contig* contig,
+                            float** features, uint8_t** labels, int32_t** kinds, size_t* n);
+
+/* The network's trainer on one device: Adam over mini-batches of the loaded samples, loss = mean
+ * over the batch of w * BCE (w = pos_weight for label 1). */
+typedef struct rv_mlp rv_mlp;
+size_t rv_mlp_param_count(void);
+/* init_params == NULL: Glorot-uniform weights from `seed`, zero biases */
+rv_status rv_mlp_open(int device, const float* init_params, uint64_t seed, rv_mlp** out);
+rv_status rv_mlp_load(rv_mlp* m, const float* features, const uint8_t* labels, size_t n);
+/* epoch_loss (may be NULL) receives each epoch's mean batch loss */
+rv_status rv_mlp_train(rv_mlp* m, int epochs, int batch, float lr, float pos_weight, uint64_t seed,
+                       float* epoch_loss);
+/* gradient of the loss over all loaded samples (one batch) at the current parameters */
+rv_status rv_mlp_gradient(rv_mlp* m, float pos_weight, float* grad, float* loss);
+rv_status rv_mlp_predict(rv_mlp* m, const float* features, size_t n, float* scores);
+rv_status rv_mlp_get_params(rv_mlp* m, float* params);
+const char* rv_mlp_last_error(const rv_mlp* m);
+void rv_mlp_close(rv_mlp* m);
 
 /* Diagnostics: the pileup-stage state (after the walk, before realignment) as text, one record per
  * line, in the layout tests compare against.  Free with rv_free. */

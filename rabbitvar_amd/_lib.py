@@ -59,7 +59,8 @@ class RvStats(ctypes.Structure):
         ("event_bytes", c_int64), ("cx_positions", c_int64), ("rows_fetched", c_int64),
         ("reduce_ms", c_double), ("realign_ms", c_double), ("order_ms", c_double), ("format_ms", c_double),
         ("wall_ms", c_double), ("order_sorted", c_int64), ("detail_ms", c_double * 16),
-        ("call_ms", c_double),
+        ("call_ms", c_double), ("screen_ms", c_double), ("screen_candidates", c_int64),
+        ("screen_skipped", c_int64), ("screen_list_ms", c_double),
     ]
 
 
@@ -83,6 +84,22 @@ SIGNATURES = {
                                 ctypes.POINTER(RvContig), ctypes.POINTER(ctypes.POINTER(ctypes.c_char)),
                                 ctypes.POINTER(c_size_t)]),
     "rv_free": (None, [c_void_p]),
+    "rv_set_screen": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_float), c_size_t, ctypes.c_float]),
+    "rv_screen_dataset": (c_int, [c_void_p, ctypes.POINTER(RvRegion), ctypes.POINTER(RvReads),
+                                  ctypes.POINTER(RvContig), ctypes.POINTER(ctypes.POINTER(ctypes.c_float)),
+                                  ctypes.POINTER(ctypes.POINTER(ctypes.c_uint8)),
+                                  ctypes.POINTER(ctypes.POINTER(ctypes.c_int32)), ctypes.POINTER(c_size_t)]),
+    "rv_mlp_param_count": (c_size_t, []),
+    "rv_mlp_open": (c_int, [c_int, ctypes.POINTER(ctypes.c_float), ctypes.c_uint64, ctypes.POINTER(c_void_p)]),
+    "rv_mlp_load": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_uint8), c_size_t]),
+    "rv_mlp_train": (c_int, [c_void_p, c_int, c_int, ctypes.c_float, ctypes.c_float, ctypes.c_uint64,
+                             ctypes.POINTER(ctypes.c_float)]),
+    "rv_mlp_gradient": (c_int, [c_void_p, ctypes.c_float, ctypes.POINTER(ctypes.c_float),
+                                ctypes.POINTER(ctypes.c_float)]),
+    "rv_mlp_predict": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_float), c_size_t, ctypes.POINTER(ctypes.c_float)]),
+    "rv_mlp_get_params": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_float)]),
+    "rv_mlp_last_error": (c_char_p, [c_void_p]),
+    "rv_mlp_close": (None, [c_void_p]),
 }
 
 _lib = None

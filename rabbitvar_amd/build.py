@@ -26,7 +26,7 @@ HIP_FLAGS = [
 ]
 
 LIB_SOURCES = ["rv_pileup.hip", "rv_score.hip", "rv_order.hip", "rv_host.cpp", "rv_engine.cpp", "rv_region.cpp",
-               "rv_realign.cpp", "rv_format.cpp", "rv_order.cpp", "rv_somatic.cpp", "rv_capi.cpp"]
+               "rv_realign.cpp", "rv_format.cpp", "rv_order.cpp", "rv_somatic.cpp", "rv_screen.hip", "rv_mlp.cpp", "rv_capi.cpp"]
 
 
 def _run(cmd):

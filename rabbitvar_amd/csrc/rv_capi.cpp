@@ -12,6 +12,10 @@ struct rv_engine {
     rv::Engine* e;
 };
 
+namespace rv {
+Engine* engine_of(rv_engine* e) { return e ? e->e : nullptr; }
+}  // namespace rv
+
 struct rv_callset {
     rv::CallSet cs;
 };

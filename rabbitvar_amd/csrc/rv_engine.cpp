@@ -73,7 +73,8 @@ Engine::~Engine() {
                      &b_arena,     &b_clips,   &b_clip_arena, &b_anchors, &b_counts,   &b_gather_ids, &b_gather_out,
                      &b_cx_pos,    &b_cx_off,  &b_cx,     &b_cx_arena,   &b_cx_cov,   &b_cx_key, &b_out,
                      &b_out_arena, &b_score_ctr, &b_status, &b_keys, &b_keys2, &b_kpos, &b_kpos2, &b_sort_tmp,
-                     &b_kcount, &b_args, &b_kpos3, &b_nopass, &b_rows2, &b_want, &b_dup};
+                     &b_kcount, &b_args, &b_kpos3, &b_nopass, &b_rows2, &b_want, &b_dup,
+                     &b_scr_cand, &b_scr_pos, &b_scr_cum, &b_scr_feat, &b_scr_score, &b_scr_params};
     // teardown: nothing to report a failure to
     for (auto* b : all)
         if (b->p) (void)hipFree(b->p);

@@ -58,6 +58,21 @@ class Engine {
     char* ref_dev = nullptr;
 
     void* grow(DevBuf& b, size_t bytes);
+
+    // ---- realignment screen (opt-in).  With a model set, the realigner skips the candidates the
+    // network scores below the threshold; collect mode records features and change labels instead
+    // (exact run).
+    bool screen_on = false;
+    float screen_threshold = 0.f;
+    bool screen_collect = false;
+    std::vector<float> collect_feat;     // [n][SCR_F] of the last collected region
+    std::vector<uint8_t> collect_label;  // per candidate
+    std::vector<int32_t> collect_kind;
+    DevBuf b_scr_params;
+    // candidate list, features and (when a model is set) keep flags for the last pileup
+    // (events_done: screen_candidates_events already ran on pr.P)
+    rv_status screen_plan(const PileupResult& pr, struct ScreenPlan& plan, bool want_features,
+                          bool events_done = false);
     // copies the listed dense rows (indices into the window) of the last pileup to pinned host
     // memory, valid until pinned_reset(); nullptr on failure
     const DenseRow* fetch_rows(const std::vector<int>& rows);
@@ -83,7 +98,7 @@ class Engine {
     // buffers of the realignment / scoring stages
     DevBuf b_gather_ids, b_gather_out, b_cx_pos, b_cx_off, b_cx, b_cx_arena, b_cx_cov, b_cx_key, b_out, b_out_arena,
         b_score_ctr, b_status, b_keys, b_keys2, b_kpos, b_kpos2, b_sort_tmp, b_kcount, b_kpos3, b_args, b_nopass,
-        b_want, b_dup;
+        b_want, b_dup, b_scr_cand, b_scr_pos, b_scr_cum, b_scr_feat, b_scr_score;
     WalkArgs* args_dev = nullptr;  // the last pileup's kernel arguments (its reads stay resident)
     int64_t args_n = 0;
     hipEvent_t sev[2] = {nullptr, nullptr};

@@ -20,9 +20,12 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <iterator>
 #include <regex>
 #include <stdexcept>
+#include <thread>
 
 namespace rv {
 
@@ -118,8 +121,28 @@ HVar* RState::new_var() {
 
 RPos& RState::at(int p) {
     RPos& r = pos[p];
-    if (r.loaded) return r;
+    if (r.loaded) {
+        if (track_on_ && !track_n_.count(p)) {
+            Snap& sn = track_n_[p];
+            sn.key = r.key;
+            sn.cov = r.cov;
+            for (auto& kv : r.v) sn.v.emplace_back(kv.first, *kv.second);
+        }
+        return r;
+    }
     r.loaded = true;
+    struct AfterLoad {
+        RState& s;
+        int p;
+        RPos& r;
+        ~AfterLoad() {
+            if (!s.track_on_ || s.track_n_.count(p)) return;
+            Snap& sn = s.track_n_[p];
+            sn.key = r.key;
+            sn.cov = r.cov;
+            for (auto& kv : r.v) sn.v.emplace_back(kv.first, *kv.second);
+        }
+    } after{*this, p, r};
     if (const DenseRow* d = rows.row(p)) {
         r.key = (d->rowflags & 1u) != 0;
         r.cov = d->cov;
@@ -165,6 +188,7 @@ HVar* RState::get_variation(int p, const std::string& d) {
 }
 
 HVar* RState::get_ins(int p, const std::string& d) {
+    track_ins(p);
     HVar*& v = ins[p][d];
     if (!v) v = new_var();
     return v;
@@ -190,6 +214,62 @@ void RState::add_cov(int p, int k) {
     RPos& r = at(p);
     r.dirty = true;
     r.cov += k;
+}
+
+namespace {
+bool same_hvar(const HVar& a, const HVar& b) {
+    return a.cnt == b.cnt && a.fwd == b.fwd && a.rev == b.rev && a.hicnt == b.hicnt && a.locnt == b.locnt &&
+           a.extracnt == b.extracnt && a.pmean == b.pmean && a.qmean == b.qmean && a.mqmean == b.mqmean &&
+           a.nm == b.nm && a.pstd == b.pstd && a.qstd == b.qstd && a.pp == b.pp && a.pq == b.pq;
+}
+bool same_entries(const std::vector<std::pair<std::string, HVar>>& a, const std::map<std::string, HVar*>& b) {
+    if (a.size() != b.size()) return false;
+    size_t i = 0;
+    for (auto& kv : b) {
+        if (a[i].first != kv.first || !same_hvar(a[i].second, *kv.second)) return false;
+        i++;
+    }
+    return true;
+}
+}  // namespace
+
+void RState::track_begin() {
+    track_on_ = true;
+    track_n_.clear();
+    track_i_.clear();
+}
+
+void RState::track_ins(int p) {
+    if (!track_on_ || track_i_.count(p)) return;
+    Snap& sn = track_i_[p];
+    auto it = ins.find(p);
+    sn.present = it != ins.end();
+    if (sn.present)
+        for (auto& kv : it->second) sn.v.emplace_back(kv.first, *kv.second);
+}
+
+bool RState::track_end() {
+    track_on_ = false;
+    bool changed = false;
+    for (auto& e : track_n_) {
+        const RPos& r = pos[e.first];
+        if (r.key != e.second.key || r.cov != e.second.cov || !same_entries(e.second.v, r.v)) {
+            changed = true;
+            break;
+        }
+    }
+    if (!changed)
+        for (auto& e : track_i_) {
+            auto it = ins.find(e.first);
+            const bool present = it != ins.end();
+            if (present != e.second.present || (present && !same_entries(e.second.v, it->second))) {
+                changed = true;
+                break;
+            }
+        }
+    track_n_.clear();
+    track_i_.clear();
+    return changed;
 }
 
 // ------------------------------------------------------------------ string helpers (util.h)
@@ -660,8 +740,31 @@ struct Realigner {
     ReadQuery* reads;
     RState& S;
     RealignStats& st;
+    ScreenPlan* plan;
 
     int maxlen() const { return S.P.maxReadLength; }
+
+    // ---- screen: candidate i of a kind is skipped when the plan says so; with labels on, a
+    // visit's changes are recorded when it ends (every exit of the loop body)
+    bool skip(int kind, int i) {
+        if (!plan || plan->keep.empty() || i < 0) return false;
+        if (plan->keep[(size_t)(plan->off[kind] + i)]) return false;
+        plan->skipped++;
+        return true;
+    }
+    struct Visit {
+        Realigner& rg;
+        int slot;  // -1: not labelled
+        Visit(Realigner& r, int kind, int i) : rg(r), slot(-1) {
+            if (rg.plan && rg.plan->label && i >= 0) {
+                slot = rg.plan->off[kind] + i;
+                rg.S.track_begin();
+            }
+        }
+        ~Visit() {
+            if (slot >= 0) rg.plan->changed[(size_t)slot] = rg.S.track_end() ? 1 : 0;
+        }
+    };
 
     bool ismatch(const std::string& s1, const std::string& s2_in, int dir, int MMx = 3) {
         const bool marked = s2_in.find_first_of("#^") != std::string::npos;
@@ -715,7 +818,11 @@ struct Realigner {
     void adjustMNP() {
         std::vector<Cand> tmp = sorted_candidates(S.P.mnp);
         st.candidates += (int64_t)tmp.size();
+        int ci = -1;
         for (const Cand& t : tmp) {
+            ci++;
+            if (skip(SC_MNP, ci)) continue;
+            Visit visit(*this, SC_MNP, ci);
             try {
                 int position = t.position;
                 const std::string vn = t.desc;
@@ -797,7 +904,11 @@ struct Realigner {
         const Re& R = res();
         std::vector<Cand> tmp = sorted_candidates(counts);
         st.candidates += (int64_t)tmp.size();
+        int ci = -1;
         for (const Cand& t : tmp) {
+            ci++;
+            if (skip(SC_DEL, ci)) continue;
+            Visit visit(*this, SC_DEL, ci);
             try {
                 int p = t.position;
                 std::string vn = t.desc;
@@ -901,7 +1012,11 @@ struct Realigner {
         const Re& R = res();
         std::vector<Cand> tmp = sorted_candidates(counts);
         st.candidates += (int64_t)tmp.size();
+        int ci = -1;
         for (const Cand& t : tmp) {
+            ci++;
+            if (skip(SC_INS, ci)) continue;
+            Visit visit(*this, SC_INS, ci);
             try {
                 int position = t.position;
                 std::string vn = t.desc;
@@ -1014,6 +1129,7 @@ struct Realigner {
                                     }
                                 }
                                 if (flag > 0) {
+                                    S.track_ins(position);
                                     auto& im = S.ins[position];
                                     HVar* vv = im[vn];
                                     im[tvn] = vv;
@@ -1040,6 +1156,7 @@ struct Realigner {
                 const Cand& t = tmp[(size_t)i];
                 int p = t.position;
                 const std::string& vn = t.desc;
+                S.track_ins(p);
                 auto mi = S.ins.find(p);
                 if (mi == S.ins.end()) continue;
                 auto& m = mi->second;
@@ -1067,12 +1184,22 @@ struct Realigner {
             int position;
             HSclip* sc;
             int count;
+            int li;  // index among the screen's cluster candidates (P.sc5 order)
         };
         std::vector<SP> t5, t3;
-        for (auto& e : S.P.sc5)
-            if (!(e.first < rstart - 5000 || e.first > rend + 5000)) t5.push_back({e.first, &e.second, e.second.v.cnt});
+        {
+            int li = -1;
+            for (auto& e : S.P.sc5) {
+                li++;
+                if (e.first < rstart - 5000 || e.first > rend + 5000) continue;
+                // screened mode: a skipped cluster is dropped here; the kept ones keep their order
+                if (skip(SC_LGINS, plan ? li : -1)) continue;
+                t5.push_back({e.first, &e.second, e.second.v.cnt, plan ? li : -1});
+            }
+        }
+        if (t5.empty()) return;  // the 3' clusters are only read from a 5' cluster's visit
         for (auto& e : S.P.sc3)
-            if (!(e.first < rstart - 5000 || e.first > rend + 5000)) t3.push_back({e.first, &e.second, e.second.v.cnt});
+            if (!(e.first < rstart - 5000 || e.first > rend + 5000)) t3.push_back({e.first, &e.second, e.second.v.cnt, -1});
         auto comp3 = [](const SP& a, const SP& b) {
             if (b.count != a.count) return a.count > b.count;
             return a.position < b.position;
@@ -1089,6 +1216,7 @@ struct Realigner {
         std::sort(by_pos.begin(), by_pos.end());
         std::vector<int> ranks;
         for (const SP& a : t5) {
+            Visit visit(*this, SC_LGINS, a.li);
             int p5 = a.position;
             HSclip* sc5v = a.sc;
             int cnt5 = a.count;
@@ -1271,6 +1399,103 @@ void build_record_list(RState& S, CxHost& cx) {
 }
 
 
+namespace {
+
+void screen_table(ScreenPlan& plan, int t, const std::vector<std::pair<int, int64_t>>& pc) {
+    std::vector<int32_t>& tp = plan.tpos[t];
+    std::vector<int64_t>& tc = plan.tcum[t];
+    tp.clear();
+    tc.assign(1, 0);
+    tp.reserve(pc.size());
+    tc.reserve(pc.size() + 1);
+    for (auto& x : pc) {
+        if (!tp.empty() && tp.back() == x.first) {
+            tc.back() += x.second;
+        } else {
+            tp.push_back(x.first);
+            tc.push_back(tc.back() + x.second);
+        }
+    }
+}
+
+void screen_counted(ScreenPlan& plan, int kind, int t, const std::map<int, std::map<std::string, int>>& m) {
+    std::vector<ScreenCand>& out = plan.lists[kind];
+    out.clear();
+    for (const Cand& c : sorted_candidates(m)) {
+        int len = (int)c.desc.size();
+        if (kind == SC_DEL) len = std::max(1, std::atoi(c.desc.c_str() + 1));  // "-N..."
+        if (kind == SC_INS) len = std::max(1, len - 1);                         // "+SEQ..."
+        out.push_back({kind, c.position, c.count, len});
+    }
+    std::vector<std::pair<int, int64_t>> sums;
+    sums.reserve(m.size());
+    plan.per_pos[kind].clear();
+    plan.per_pos[kind].reserve(m.size());
+    for (auto& e : m) {
+        int64_t c = 0;
+        for (auto& kv : e.second) c += kv.second;
+        sums.push_back({e.first, c});
+        plan.per_pos[kind].push_back({e.first, (int64_t)e.second.size()});
+    }
+    screen_table(plan, t, sums);
+}
+
+void screen_clips(ScreenPlan& plan, int t, const std::map<int, HSclip>& m, bool as_candidates) {
+    std::vector<std::pair<int, int64_t>> sums;
+    sums.reserve(m.size());
+    if (as_candidates) {
+        plan.lists[SC_LGINS].clear();
+        plan.per_pos[SC_LGINS].clear();
+    }
+    for (auto& e : m) {
+        sums.push_back({e.first, (int64_t)e.second.v.cnt});
+        if (as_candidates) {
+            plan.lists[SC_LGINS].push_back({SC_LGINS, e.first, e.second.v.cnt, (int)e.second.col.size()});
+            plan.per_pos[SC_LGINS].push_back({e.first, 1});
+        }
+    }
+    screen_table(plan, t, sums);
+}
+
+}  // namespace
+
+void screen_candidates_events(const HostPileup& P, ScreenPlan& plan) {
+    std::thread t1([&] { screen_counted(plan, SC_MNP, ST_MNP, P.mnp); });
+    screen_counted(plan, SC_DEL, ST_DEL, P.delCount);
+    screen_counted(plan, SC_INS, ST_INS, P.insCount);
+    t1.join();
+}
+
+void screen_candidates_finish(const HostPileup& P, ScreenPlan& plan) {
+    std::thread t3([&] { screen_clips(plan, ST_SC3, P.sc3, false); });
+    screen_clips(plan, ST_SC5, P.sc5, true);
+    t3.join();
+    plan.cand.clear();
+    size_t total = 0;
+    for (auto& l : plan.lists) total += l.size();
+    plan.cand.reserve(total);
+    for (int k = 0; k < SC_NKIND; k++) {
+        plan.off[k] = (int)plan.cand.size();
+        plan.cand.insert(plan.cand.end(), plan.lists[k].begin(), plan.lists[k].end());
+    }
+    plan.off[SC_NKIND] = (int)plan.cand.size();
+    // every candidate position: merge of the per-kind position lists (each already sorted)
+    std::vector<std::pair<int, int64_t>> all, tmp;
+    for (int k = 0; k < SC_NKIND; k++) {
+        tmp.clear();
+        tmp.reserve(all.size() + plan.per_pos[k].size());
+        std::merge(all.begin(), all.end(), plan.per_pos[k].begin(), plan.per_pos[k].end(), std::back_inserter(tmp),
+                   [](const std::pair<int, int64_t>& a, const std::pair<int, int64_t>& b) { return a.first < b.first; });
+        all.swap(tmp);
+    }
+    screen_table(plan, ST_ALL, all);
+}
+
+void screen_candidates(const HostPileup& P, ScreenPlan& plan) {
+    screen_candidates_events(P, plan);
+    screen_candidates_finish(P, plan);
+}
+
 bool realign_prefetch(const Config& c, HostRef& ref, int chrLen, const HostPileup& P, RowCache& rows) {
     // the rows the candidates' own handlers read first: the candidate positions, the few bases
     // after them, every non-dense / insertion position (read again when the record list is
@@ -1286,7 +1511,7 @@ bool realign_prefetch(const Config& c, HostRef& ref, int chrLen, const HostPileu
 }
 
 void realign(const Config& c, int region_start, int region_end, HostRef& ref, int chrLen, ReadQuery* reads,
-             RState& S, RealignStats& st, bool prefetched) {
+             RState& S, RealignStats& st, bool prefetched, ScreenPlan* plan) {
     auto now = [] { return std::chrono::steady_clock::now(); };
     auto ms = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
         return std::chrono::duration<double, std::milli>(b - a).count();
@@ -1294,7 +1519,8 @@ void realign(const Config& c, int region_start, int region_end, HostRef& ref, in
     auto t0 = now();
     if (!prefetched && !realign_prefetch(c, ref, chrLen, S.P, S.rows)) throw std::runtime_error("dense row fetch failed");
     auto t1 = now();
-    Realigner rg{c, region_start, region_end, ref, chrLen, reads, S, st};
+    if (plan && plan->label) plan->changed.assign(plan->cand.size(), 0);
+    Realigner rg{c, region_start, region_end, ref, chrLen, reads, S, st, plan};
     rg.adjustMNP();
     auto t2 = now();
     st.ms_prefetch = ms(t0, t1);

@@ -20,6 +20,7 @@
 #include "rv_host.h"
 #include "rv_order.h"
 #include "rv_score.h"
+#include "rv_screen.h"
 
 namespace rv {
 
@@ -119,7 +120,45 @@ class RState {
     std::unordered_map<int, RPos> pos;
     std::map<int, std::map<std::string, HVar*>> ins;
     std::vector<KeyOp> key_log;  // nonInsertionVariants key insertions / erasures, in order
+
+    // Change tracking for the screen's labels: between track_begin() and track_end() every
+    // position read through at() or track_ins() is snapshotted once; track_end() reports whether
+    // any of them now differs (key, coverage, descriptors or their counts and sums).
+    void track_begin();
+    bool track_end();
+    void track_ins(int pos);  // call before reading or changing ins[pos] directly
+
+  private:
+    struct Snap {
+        bool key = false;
+        int cov = 0;
+        bool present = false;
+        std::vector<std::pair<std::string, HVar>> v;
+    };
+    bool track_on_ = false;
+    std::unordered_map<int, Snap> track_n_, track_i_;
 };
+
+// The realignment screen's plan for one region.  Candidates are listed kind by kind: adjustMNP,
+// realigndel and realignins candidates in the order those stages visit them (count descending,
+// position ascending, descriptor descending), then realignlgins30's 5' clip clusters by position.
+struct ScreenPlan {
+    std::vector<ScreenCand> cand;
+    int off[SC_NKIND + 1] = {0, 0, 0, 0, 0};
+    std::vector<int32_t> tpos[ST_N];  // neighbourhood tables (rv_screen.h)
+    std::vector<int64_t> tcum[ST_N];
+    std::vector<ScreenCand> lists[SC_NKIND];                  // per kind, while being built
+    std::vector<std::pair<int, int64_t>> per_pos[SC_NKIND];   // per kind: (position, candidates there)
+    std::vector<uint8_t> keep;     // per candidate; empty: visit every candidate
+    bool label = false;            // fill `changed`
+    std::vector<uint8_t> changed;  // per candidate: the visit changed a count (unvisited: 0)
+    int64_t skipped = 0;
+};
+void screen_candidates(const HostPileup& P, ScreenPlan& plan);
+// the same in two parts: the MNV / deletion / insertion candidates once the event maps are
+// reduced, then the clip clusters and the merged tables once the clip maps are
+void screen_candidates_events(const HostPileup& P, ScreenPlan& plan);
+void screen_candidates_finish(const HostPileup& P, ScreenPlan& plan);
 
 struct RealignStats {
     int64_t candidates = 0, touched = 0, changed = 0;
@@ -137,7 +176,8 @@ bool realign_prefetch(const Config& c, HostRef& ref, int chrLen, const HostPileu
 // adjustMNP, realigndel, realignins, realignlgins30 in the reference's order (after the prefetch
 // above unless `prefetched`).
 void realign(const Config& c, int region_start, int region_end, HostRef& ref, int chrLen, ReadQuery* reads,
-             RState& S, RealignStats& st, bool prefetched = false);
+             RState& S, RealignStats& st, bool prefetched = false,
+             ScreenPlan* plan = nullptr);
 
 // Positions the scoring stage reads from a record list instead of the dense rows: every
 // non-dense / insertion position and every position the realigner changed, ascending, with the

@@ -117,12 +117,17 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     double prefetch_ms = 0;
     std::exception_ptr prefetch_err;
     std::thread prefetch_th;
+    // realignment screen: its MNV / deletion / insertion candidates are listed while the clips
+    // are still being reduced
+    const bool screening = e.screen_on || e.screen_collect;
+    ScreenPlan plan;
+    std::thread screen_th;
     struct Joiner {
         std::thread& t;
         ~Joiner() {
             if (t.joinable()) t.join();
         }
-    } key_joiner{key_th}, prefetch_joiner{prefetch_th};
+    } key_joiner{key_th}, prefetch_joiner{prefetch_th}, screen_joiner{screen_th};
     // the engine's row table must be all null again when this region ends, on every path
     struct SlotRelease {
         std::unique_ptr<RowCache>& rc;
@@ -144,6 +149,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
         ref.hi = p.W.ref_lo + (int)p.ref_win.size() - 1;
         rcp = std::make_unique<RowCache>(&src, p.P.dense_lo, p.P.dense_n, &e.row_slots);
         const HostPileup* hp = &p.P;
+        if (screening) screen_th = std::thread([&, hp] { screen_candidates_events(*hp, plan); });
         prefetch_th = std::thread([&, hp] {
             try {
                 const auto f0 = std::chrono::steady_clock::now();
@@ -157,6 +163,8 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     rv_status s = e.pileup(region, reads, contig, false, pr, &hooks);
     if (key_th.joinable()) key_th.join();
     if (prefetch_th.joinable()) prefetch_th.join();
+    const bool screen_events_done = screen_th.joinable();
+    if (screen_th.joinable()) screen_th.join();
     if (s != RV_OK) return s;
     if (key_status != RV_OK) return key_status;
     if (!rcp) return e.fail(RV_ERR_DEVICE, "pileup finished without reducing its records");
@@ -185,14 +193,37 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
         }
     });
     Joiner joiner{replay};
+    // ---- realignment screen (screened mode, or feature/label collection)
+    ScreenPlan* planp = nullptr;
+    e.stats.screen_ms = 0;
+    e.stats.screen_list_ms = 0;
+    e.stats.screen_candidates = 0;
+    e.stats.screen_skipped = 0;
+    if (screening) {
+        const auto q0 = std::chrono::steady_clock::now();
+        plan.label = e.screen_collect;
+        s = e.screen_plan(pr, plan, e.screen_collect, screen_events_done);
+        if (s != RV_OK) return s;
+        planp = &plan;
+        e.stats.screen_ms = ms_since(q0);
+        e.stats.screen_candidates = (int64_t)plan.cand.size();
+    }
     // ---- realignment on the host
     RState S(pr.P, rc);
     DeviceReadQuery rq(e);
     RealignStats rst;
     try {
-        realign(e.cfg, region.start, region.end, ref, contig.length, &rq, S, rst, true);
+        realign(e.cfg, region.start, region.end, ref, contig.length, &rq, S, rst, true, planp);
     } catch (const std::exception& ex) {
         return e.fail(RV_ERR_DEVICE, std::string("realignment: ") + ex.what());
+    }
+    if (planp) {
+        e.stats.screen_skipped = plan.skipped;
+        if (e.screen_collect) {
+            e.collect_label = plan.changed;
+            e.collect_kind.resize(plan.cand.size());
+            for (size_t i = 0; i < plan.cand.size(); i++) e.collect_kind[i] = plan.cand[i].kind;
+        }
     }
     rst.ms_prefetch = prefetch_ms;
     e.stats.detail_ms[0] = rst.ms_prefetch;

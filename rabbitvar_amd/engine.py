@@ -142,9 +142,107 @@ class Engine:
         self.lib.rv_callset_free(cs)
         return text
 
+    def set_screen(self, params, threshold):
+        """Screened mode: the realigner skips candidates the network scores below `threshold`
+        (threshold <= 0 or params None: exact mode)."""
+        if params is None:
+            st = self.lib.rv_set_screen(self.h, None, 0, 0.0)
+        else:
+            self._screen = np.ascontiguousarray(params, np.float32)
+            st = self.lib.rv_set_screen(self.h, self._screen.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                        self._screen.size, float(threshold))
+        if st != 0:
+            raise self._error(st)
+
+    def screen_dataset(self, chrom, start, end, reads, contig_name, contig_bases):
+        """Runs the region exactly and returns its realignment candidates' features [n, 32], change
+        labels [n] (1: the realigner changed a count there) and kinds [n]."""
+        region, r, contig = self._args(chrom, start, end, reads, contig_name, contig_bases)
+        fp = ctypes.POINTER(ctypes.c_float)()
+        lp = ctypes.POINTER(ctypes.c_uint8)()
+        kp = ctypes.POINTER(ctypes.c_int32)()
+        n = ctypes.c_size_t()
+        st = self.lib.rv_screen_dataset(self.h, ctypes.byref(region), ctypes.byref(r), ctypes.byref(contig),
+                                        ctypes.byref(fp), ctypes.byref(lp), ctypes.byref(kp), ctypes.byref(n))
+        if st != 0:
+            raise self._error(st)
+        m = n.value
+        if m == 0:
+            return np.zeros((0, 32), np.float32), np.zeros(0, np.uint8), np.zeros(0, np.int32)
+        feats = np.ctypeslib.as_array(fp, shape=(m * 32,)).reshape(m, 32).copy()
+        labels = np.ctypeslib.as_array(lp, shape=(m,)).copy()
+        kinds = np.ctypeslib.as_array(kp, shape=(m,)).copy()
+        for p in (fp, lp, kp):
+            self.lib.rv_free(ctypes.cast(p, ctypes.c_void_p))
+        return feats, labels, kinds
+
     def stats(self):
         s = _lib.RvStats()
         self.lib.rv_get_stats(self.h, ctypes.byref(s))
         out = {k: getattr(s, k) for k, _ in s._fields_}
         out["detail_ms"] = list(s.detail_ms)
         return out
+
+
+class Mlp:
+    """The realignment screen's network on the GPU (rv_mlp_*): training with the library's own
+    forward/backward kernels, gradients and scores."""
+
+    def __init__(self, device=0, params=None, seed=1):
+        self.lib = _lib.load()
+        self.n_params = self.lib.rv_mlp_param_count()
+        h = ctypes.c_void_p()
+        init = None
+        if params is not None:
+            self._init = np.ascontiguousarray(params, np.float32)
+            assert self._init.size == self.n_params
+            init = self._init.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        st = self.lib.rv_mlp_open(device, init, seed, ctypes.byref(h))
+        if st != 0:
+            raise EngineError(f"rv_mlp_open failed with status {st}")
+        self.h = h
+
+    def _check(self, st):
+        if st != 0:
+            msg = self.lib.rv_mlp_last_error(self.h)
+            raise EngineError(f"status {st}: {msg.decode() if msg else ''}")
+
+    def load(self, X, y=None):
+        self._X = np.ascontiguousarray(X, np.float32)
+        self._y = None if y is None else np.ascontiguousarray(y, np.uint8)
+        fp = self._X.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        yp = None if self._y is None else self._y.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+        self._check(self.lib.rv_mlp_load(self.h, fp, yp, self._X.shape[0]))
+
+    def train(self, epochs, batch=1024, lr=1e-3, pos_weight=1.0, seed=1):
+        losses = np.zeros(epochs, np.float32)
+        self._check(self.lib.rv_mlp_train(self.h, epochs, batch, lr, pos_weight, seed,
+                                          losses.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return losses
+
+    def gradient(self, pos_weight=1.0):
+        g = np.zeros(self.n_params, np.float32)
+        loss = ctypes.c_float()
+        self._check(self.lib.rv_mlp_gradient(self.h, pos_weight, g.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                             ctypes.byref(loss)))
+        return g, loss.value
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        out = np.zeros(X.shape[0], np.float32)
+        self._check(self.lib.rv_mlp_predict(self.h, X.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), X.shape[0],
+                                            out.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return out
+
+    def params(self):
+        p = np.zeros(self.n_params, np.float32)
+        self._check(self.lib.rv_mlp_get_params(self.h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return p
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rv_mlp_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

@@ -292,8 +292,28 @@ int main(int argc, char** argv) {
             realign(cfg, rs, re, ref2, chrLen, &rq, rs2, st2);
         }
     }
+    // RV_SIM_SCREEN=1: build the realignment screen's candidate list first (timed) and collect
+    // change labels in the realigner
+    ScreenPlan plan;
+    ScreenPlan* planp = nullptr;
+    if (getenv("RV_SIM_SCREEN")) {
+        const auto q0 = std::chrono::steady_clock::now();
+        screen_candidates(HP, plan);
+        const double qms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - q0).count();
+        plan.label = true;
+        planp = &plan;
+        fprintf(stderr, "screen list: %zu candidates (mnp %d del %d ins %d clusters %d), %.2f ms\n", plan.cand.size(),
+                plan.off[1] - plan.off[0], plan.off[2] - plan.off[1], plan.off[3] - plan.off[2],
+                plan.off[4] - plan.off[3], qms);
+    }
     RealignStats rst;
-    realign(cfg, rs, re, ref, chrLen, &rq, RS, rst);
+    realign(cfg, rs, re, ref, chrLen, &rq, RS, rst, false, planp);
+    if (planp) {
+        int64_t pos[4] = {0, 0, 0, 0};
+        for (size_t i = 0; i < plan.cand.size(); i++) pos[plan.cand[i].kind] += plan.changed[i];
+        fprintf(stderr, "screen labels: changed mnp %lld del %lld ins %lld clusters %lld\n", (long long)pos[0],
+                (long long)pos[1], (long long)pos[2], (long long)pos[3]);
+    }
     CxHost cx;
     build_record_list(RS, cx);
     ScoreIn in{};
