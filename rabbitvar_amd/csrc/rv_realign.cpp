@@ -194,6 +194,33 @@ HVar* RState::get_ins(int p, const std::string& d) {
     return v;
 }
 
+const HVar* RState::peek(int p, const std::string& d) {
+    auto it = pos.find(p);
+    if (it != pos.end() && it->second.loaded) {
+        const RPos& r = it->second;
+        if (!r.key) return nullptr;
+        auto j = r.v.find(d);
+        return j == r.v.end() ? nullptr : j->second;
+    }
+    // what at() would load: the dense row's key flag and bases, then the non-dense entries
+    const DenseRow* dr = rows.row(p);
+    auto nit = P.nd.find(p);
+    const bool key = (dr && (dr->rowflags & 1u)) || nit != P.nd.end();
+    if (!key) return nullptr;
+    if (d.size() == 1 && dr) {
+        const int b = base_index(d[0]);
+        if (b >= 0 && (dr->b[b].flags & 1u)) {
+            peek_tmp_ = dense_to_hvar(dr->b[b]);
+            return &peek_tmp_;
+        }
+    }
+    if (nit != P.nd.end()) {
+        auto j = nit->second.find(d);
+        if (j != nit->second.end()) return &j->second;
+    }
+    return nullptr;
+}
+
 HVar* RState::maybe(int p, char refBase) {
     if (refBase == 0) return nullptr;
     return inner(p, std::string(1, refBase));
@@ -826,32 +853,48 @@ struct Realigner {
             try {
                 int position = t.position;
                 const std::string vn = t.desc;
-                RPos& rp = S.at(position);
-                if (!rp.key) continue;
-                std::map<std::string, HVar*> copy = rp.v;
-                if (!copy.count(vn)) continue;
-                rp.dirty = true;
-                HVar* vref = copy.at(vn);
+                // The reference walks a copy of the position's map and erases the left pieces it
+                // used from the copy only (VariationRealigner.cpp:350-400).  The position's own map
+                // does not change during the walk (the right pieces sit at later positions), so
+                // the walk reads it directly and remembers the erased keys.  Positions are read
+                // through peek() and materialized only when an entry is taken over: the output
+                // does not depend on it (the candidate's position is a non-dense one, which the
+                // record list always carries).
+                const HVar* v0 = S.peek(position, vn);
+                if (!v0) continue;  // no key at the position, or no such descriptor
+                HVar vcur = *v0;    // vref's value until the position is materialized
+                HVar* vref = nullptr;
+                auto take = [&]() {
+                    if (vref) return;
+                    RPos& rp = S.at(position);
+                    rp.dirty = true;
+                    vref = rp.v.at(vn);
+                };
+                auto vcnt = [&]() { return vref ? vref->cnt : vcur.cnt; };
                 std::string mnt = vn;
                 replace_first(mnt, "&", "");
+                std::vector<std::string> used_left;
                 for (int i = 0; i < (int)mnt.size() - 1; i++) {
                     std::string left = sub2(mnt, 0, i + 1);
                     if (left.size() > 1) left.insert(1, "&");
                     std::string right = sub1(mnt, -(int)(mnt.size() - (size_t)i - 1));
                     if (right.size() > 1) right.insert(1, "&");
-                    if (copy.count(left)) {
-                        HVar* tref = copy.at(left);
-                        if (tref->cnt <= 0) continue;
-                        if (tref->cnt < vref->cnt && tref->pmean / tref->cnt <= i + 1) {
-                            adj_cnt(vref, tref, nullptr);
-                            copy.erase(left);
+                    const HVar* lt = S.peek(position, left);
+                    if (lt && std::find(used_left.begin(), used_left.end(), left) == used_left.end()) {
+                        if (lt->cnt <= 0) continue;
+                        if (lt->cnt < vcnt() && lt->pmean / lt->cnt <= i + 1) {
+                            take();
+                            adj_cnt(vref, S.at(position).v.at(left), nullptr);
+                            used_left.push_back(left);
                         }
                     }
-                    RPos& rr = S.at(position + i + 1);
-                    if (rr.key && rr.v.count(right)) {
-                        HVar* tref = rr.v.at(right);
-                        if (tref->cnt < 0) continue;
-                        if (tref->cnt < vref->cnt) {
+                    const HVar* rt = S.peek(position + i + 1, right);
+                    if (rt) {
+                        if (rt->cnt < 0) continue;
+                        if (rt->cnt < vcnt()) {
+                            take();
+                            RPos& rr = S.at(position + i + 1);
+                            HVar* tref = rr.v.at(right);
                             adj_cnt(vref, tref, nullptr);
                             S.add_cov(position, tref->cnt);
                             rr.dirty = true;

@@ -110,6 +110,9 @@ class RState {
     HVar* get_variation(int pos, const std::string& d);  // getVariation on nonInsertionVariants
     HVar* get_ins(int pos, const std::string& d);        // getVariation on insertionVariants
     HVar* maybe(int pos, char refBase);                  // getVariationMaybe
+    // nonInsertionVariants[pos][d] as at() would find it, without materializing the position
+    // (valid until the next call); nullptr when absent
+    const HVar* peek(int pos, const std::string& d);
     void erase_inner(int pos, const std::string& d, bool drop_empty);
     void add_cov(int pos, int k);
     HVar* new_var();
@@ -135,6 +138,7 @@ class RState {
         bool present = false;
         std::vector<std::pair<std::string, HVar>> v;
     };
+    HVar peek_tmp_;
     bool track_on_ = false;
     std::unordered_map<int, Snap> track_n_, track_i_;
 };
