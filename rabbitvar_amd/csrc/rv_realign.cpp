@@ -494,6 +494,11 @@ std::string findconseq(HSclip& sc) {
     return SEQ;
 }
 
+const std::string& consensus(HSclip& sc) {
+    if (sc.sequence.empty()) findconseq(sc);
+    return sc.sequence;
+}
+
 void find35match(const std::string& seq5, const std::string& seq3, int& b5, int& b3, int& score) {
     // The reference tries every start (i, j), i in [0, |seq5| - 9], j in [1, |seq3| - 9] (i outer),
     // and walks n = 0, 1, ... comparing seq5[i + n] with seq3[|seq3| - j - n] (seq5's end counts as
@@ -1264,7 +1269,7 @@ struct Realigner {
             HSclip* sc5v = a.sc;
             int cnt5 = a.count;
             if (sc5v->used) continue;
-            std::string seq5 = findconseq(*sc5v);
+            const std::string& seq5 = consensus(*sc5v);
             if (seq5.size() <= 10) continue;
             ranks.clear();
             {
@@ -1284,7 +1289,7 @@ struct Realigner {
                     if (sc3v->used) continue;
                     if (p5 - p3 > maxlen() * 2.5) continue;
                     if (p3 - p5 > maxlen() - 10) continue;
-                    std::string seq3 = findconseq(*sc3v);
+                    const std::string& seq3 = consensus(*sc3v);
                     if (seq3.size() <= 10) continue;
                     if (!(cnt5 / (double)cnt3 >= 0.08 && cnt5 / (double)cnt3 <= 12)) continue;
                     int bp5, bp3, score;

@@ -196,6 +196,8 @@ void build_record_list(RState& S, CxHost& cx);
 
 // soft-clip consensus (include/VariationUtils.h:403-509)
 std::string findconseq(HSclip& sc);
+// the cluster's consensus, computed once (findconseq stores it in the cluster)
+const std::string& consensus(HSclip& sc);
 // find35match (reference src/VariationRealigner.cpp:1396-1428): where the 5' clip consensus and the
 // reversed 3' clip consensus overlap; score 0 when they do not
 void find35match(const std::string& seq5, const std::string& seq3, int& b5, int& b3, int& score);
