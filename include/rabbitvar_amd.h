@@ -216,6 +216,24 @@ rv_status rv_mlp_get_params(rv_mlp* m, float* params);
 const char* rv_mlp_last_error(const rv_mlp* m);
 void rv_mlp_close(rv_mlp* m);
 
+/* ---- BAM input: the records the region iterator returns, decoded to rv_reads.  Replaces the
+ * reference's htslib reads (sam_itr_querys + sam_itr_next, src/recordPreprocessor.cpp:27, :103-114,
+ * with seq_nt16_str decoding, src/parseCigar.cpp:442-459).  rv_bam_open inflates the whole file
+ * (BGZF blocks on `threads` threads) and indexes its records; no .bai is read. */
+typedef struct rv_bam rv_bam;
+/* On failure *out may still be set: read rv_bam_last_error, then rv_bam_close. */
+rv_status rv_bam_open(const char* path, int threads, rv_bam** out);
+const char* rv_bam_last_error(const rv_bam* b);
+int32_t rv_bam_n_targets(const rv_bam* b);
+const char* rv_bam_target_name(const rv_bam* b, int32_t tid);
+int32_t rv_bam_target_len(const rv_bam* b, int32_t tid);
+size_t rv_bam_n_records(const rv_bam* b);
+/* Records of chr:start-end (1-based, inclusive) in file order: those of the contig that start before
+ * `end` and end after `start - 1` (bam_endpos).  The arrays (host memory) stay valid until the next
+ * query or rv_bam_close. */
+rv_status rv_bam_query(rv_bam* b, const char* chr, int32_t start, int32_t end, rv_reads* out);
+void rv_bam_close(rv_bam* b);
+
 /* Diagnostics: the pileup-stage state (after the walk, before realignment) as text, one record per
  * line, in the layout tests compare against.  Free with rv_free. */
 rv_status rv_debug_pileup(rv_engine* e, const rv_region* region, const rv_reads* reads, const rv_contig* contig,

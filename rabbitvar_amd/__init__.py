@@ -4,4 +4,4 @@ The compute path is the native library rabbitvar_amd/_build/librabbitvar_amd.so 
 host engine + C ABI, include/rabbitvar_amd.h).  This package holds the Python host mirror of that
 interface (engine.Engine), the input readers (io), and the build script (build).
 """
-from .io import batch_from_records, read_fasta, read_rvb, write_rvb  # noqa: F401
+from .io import BamFile, batch_from_records, read_fasta, read_rvb, write_rvb  # noqa: F401

@@ -1,0 +1,135 @@
+"""Command line: the calling path with the reference's option names (src/Launcher.cpp:299-494 for
+the options this path reads; region planning as RegionBuilder for -R and -i, src/RegionBuilder.cpp).
+
+    python -m rabbitvar_amd -G ref.fa -b sample.bam -R chr1:1000-2000 -N S1 --fisher [--out calls.txt]
+    python -m rabbitvar_amd -G ref.fa -b 'tumor.bam|normal.bam' -N 'T|N' -i panel.bed --fisher
+
+Reads come from BAM (decoded by the library) or from RVB read-batch files.  The output is the text
+the reference prints, region by region in the order given.  --screen turns on the learned
+realignment screen (opt-in; the default is the exact path).
+"""
+import argparse
+import sys
+
+OPTIONS = [  # (flag, config field, type)
+    ("-f", "freq", float), ("-q", "goodq", float), ("-o", "qratio", float), ("-O", "mapq", float),
+    ("-V", "lofreq", float), ("-X", "vext", int), ("-m", "mismatch", int), ("-T", "trimBasesAfter", int),
+    ("-k", "performLocalRealignment", int), ("-I", "indelsize", int), ("-B", "minBiasReads", int),
+    ("-r", "minr", int), ("-P", "readPosFilter", int), ("-M", "minmatch", int), ("-L", "SVMINLEN", int),
+    ("-Y", "referenceExtension", int), ("-x", "numberNucleotideToExtend", int), ("-Q", "mappingQuality", int),
+    ("-J", "crisprCuttingSite", int),
+]
+
+
+def parse_region(text, extend):
+    """-R chr:start-end[:gene] (commas allowed), widened by -x."""
+    chrom, _, rest = text.partition(":")
+    rng, _, gene = rest.partition(":")
+    a, _, b = rng.partition("-")
+    start = int(a.replace(",", ""))
+    end = int(b.replace(",", "")) if b else start
+    start -= extend
+    end += extend
+    if start > end:
+        start = end
+    return chrom, start, end, gene or chrom
+
+
+def bed_regions(path, c, s, e, g):
+    out = []
+    with open(path) as f:
+        for line in f:
+            line = line.rstrip("\n")
+            if not line or line.startswith("#") or line.startswith("track"):
+                continue
+            cols = line.split("\t")
+            gene = cols[g] if 0 <= g < len(cols) else cols[c]
+            out.append((cols[c], int(cols[s]), int(cols[e]), gene))
+    return out
+
+
+class Reads:
+    def __init__(self, path, threads):
+        from . import BamFile, read_rvb
+
+        self.bam = None
+        self.batch = None
+        if path.endswith(".rvb"):
+            self.batch = read_rvb(path)
+        else:
+            self.bam = BamFile(path, threads=threads)
+
+    def region(self, chrom, start, end):
+        if self.bam is not None:
+            return self.bam.query(chrom, start, end)
+        return self.batch.select(start, end)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="python -m rabbitvar_amd", description=__doc__.splitlines()[0])
+    ap.add_argument("-G", required=True, help="reference FASTA")
+    ap.add_argument("-b", required=True, help="BAM (or .rvb); 'first|second' for somatic mode")
+    ap.add_argument("-N", default="", help="sample name ('T|N' in somatic mode)")
+    ap.add_argument("-R", help="region chr:start-end[:gene]")
+    ap.add_argument("-i", "--bed", help="BED file of regions")
+    ap.add_argument("-c", type=int, default=1)
+    ap.add_argument("-S", type=int, default=2)
+    ap.add_argument("-E", type=int, default=3)
+    ap.add_argument("-g", type=int, default=0)
+    ap.add_argument("-F", default="0x504", help="flag filter (hex)")
+    ap.add_argument("-t", action="store_true", help="duplicate filter")
+    ap.add_argument("-3", dest="move3", action="store_true")
+    ap.add_argument("-u", action="store_true")
+    ap.add_argument("--UN", action="store_true")
+    ap.add_argument("--fisher", action="store_true")
+    ap.add_argument("--out", help="output file (default: stdout)")
+    ap.add_argument("--th", type=int, default=8, help="BAM inflate threads")
+    ap.add_argument("--device", type=int, default=0)
+    ap.add_argument("--screen", action="store_true", help="screened mode with the trained model")
+    for flag, field, typ in OPTIONS:
+        ap.add_argument(flag, dest=field, type=typ)
+    a = ap.parse_args(argv)
+
+    from . import read_fasta
+    from .engine import Engine
+
+    cfg = {field: getattr(a, field) for _, field, _ in OPTIONS if getattr(a, field) is not None}
+    cfg.update(samfilter=int(a.F, 16), removeDuplicatedReads=int(a.t), moveIndelsTo3=int(a.move3),
+               uniqueModeAlignmentEnabled=int(a.u), uniqueModeSecondInPairEnabled=int(a.UN), fisher=int(a.fisher))
+    extend = cfg.get("numberNucleotideToExtend", 0)
+    if a.R:
+        regions = [parse_region(a.R, extend)]
+    elif a.bed:
+        regions = bed_regions(a.bed, a.c - 1, a.S - 1, a.E - 1, a.g - 1)
+    else:
+        ap.error("give -R or -i")
+    paths = a.b.split("|")
+    samples = [Reads(p, a.th) for p in paths]
+    eng = Engine(device=a.device, sample=a.N, **cfg)
+    if a.screen:
+        from . import screen
+
+        params, threshold, _ = screen.load_model()
+        eng.set_screen(params, threshold)
+    out = open(a.out, "w") if a.out else sys.stdout
+    contigs = {}
+    try:
+        for chrom, start, end, gene in regions:
+            if chrom not in contigs:
+                contigs[chrom] = read_fasta(a.G, chrom)[1]
+            seq = contigs[chrom]
+            rs = [s.region(chrom, start, end) for s in samples]
+            if len(rs) == 2:
+                text = eng.run_region_somatic(chrom, start, end, rs[0], rs[1], chrom, seq, gene=gene)
+            else:
+                text = eng.run_region(chrom, start, end, rs[0], chrom, seq, gene=gene)
+            out.write(text)
+    finally:
+        if a.out:
+            out.close()
+        eng.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

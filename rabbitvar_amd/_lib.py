@@ -100,6 +100,14 @@ SIGNATURES = {
     "rv_mlp_get_params": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_float)]),
     "rv_mlp_last_error": (c_char_p, [c_void_p]),
     "rv_mlp_close": (None, [c_void_p]),
+    "rv_bam_open": (c_int, [c_char_p, c_int, ctypes.POINTER(c_void_p)]),
+    "rv_bam_last_error": (c_char_p, [c_void_p]),
+    "rv_bam_n_targets": (ctypes.c_int32, [c_void_p]),
+    "rv_bam_target_name": (c_char_p, [c_void_p, ctypes.c_int32]),
+    "rv_bam_target_len": (ctypes.c_int32, [c_void_p, ctypes.c_int32]),
+    "rv_bam_n_records": (c_size_t, [c_void_p]),
+    "rv_bam_query": (c_int, [c_void_p, c_char_p, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(RvReads)]),
+    "rv_bam_close": (None, [c_void_p]),
 }
 
 _lib = None

@@ -26,7 +26,7 @@ HIP_FLAGS = [
 ]
 
 LIB_SOURCES = ["rv_pileup.hip", "rv_score.hip", "rv_order.hip", "rv_host.cpp", "rv_engine.cpp", "rv_region.cpp",
-               "rv_realign.cpp", "rv_format.cpp", "rv_order.cpp", "rv_somatic.cpp", "rv_screen.hip", "rv_mlp.cpp", "rv_capi.cpp"]
+               "rv_realign.cpp", "rv_format.cpp", "rv_order.cpp", "rv_somatic.cpp", "rv_screen.hip", "rv_mlp.cpp", "rv_bam.cpp", "rv_capi.cpp"]
 
 
 def _run(cmd):
@@ -49,7 +49,7 @@ def build(verbose=True, jobs=None):
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_run, cmds))
     lib = os.path.join(OUT, "librabbitvar_amd.so")
-    _run([HIPCC] + HIP_FLAGS + ["-shared", "-o", lib + ".tmp"] + objs)
+    _run([HIPCC] + HIP_FLAGS + ["-shared", "-o", lib + ".tmp"] + objs + ["-lz"])
     os.replace(lib + ".tmp", lib)
     cli_src = os.path.join(CSRC, "rv_main.cpp")
     if os.path.exists(cli_src):
@@ -59,7 +59,7 @@ def build(verbose=True, jobs=None):
         os.replace(exe + ".tmp", exe)
     gxx = shutil.which("g++") or "g++"
     synth = os.path.join(OUT, "rvsynth")
-    _run([gxx, "-O2", "-std=c++17", "-o", synth + ".tmp", os.path.join(HERE, "synth", "rv_synth.cpp")])
+    _run([gxx, "-O2", "-std=c++17", "-o", synth + ".tmp", os.path.join(HERE, "synth", "rv_synth.cpp"), "-lz"])
     os.replace(synth + ".tmp", synth)
     return lib
 

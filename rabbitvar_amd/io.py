@@ -4,19 +4,22 @@ region iterator's record selection."""
 import numpy as np
 
 
-def read_fasta(path):
-    """Returns (name, sequence bytes) of the first contig."""
+def read_fasta(path, contig=None):
+    """Returns (name, sequence bytes) of the first contig, or of the named one."""
     name = None
     parts = []
     with open(path, "rb") as f:
         for line in f:
             line = line.rstrip(b"\r\n")
             if line.startswith(b">"):
-                if name is not None:
+                if name is not None and (contig is None or name == contig):
                     break
                 name = line[1:].split()[0].decode()
+                parts = []
                 continue
             parts.append(line)
+    if contig is not None and name != contig:
+        raise KeyError(f"{path}: no contig {contig}")
     return name, b"".join(parts)
 
 
@@ -43,12 +46,14 @@ class ReadBatch:
         self.n = int(len(self.pos0))
 
     def ref_len(self):
-        """Reference span of each record (M/D/N/=/X lengths; 1 when empty, as bam_endpos)."""
+        """Reference span of each record (M/D/N/=/X lengths; 1 when empty or unmapped, as
+        bam_endpos)."""
         ops = self.cigar & 0xF
         lens = (self.cigar >> 4).astype(np.int64)
         on_ref = np.isin(ops, (0, 2, 3, 7, 8))
         csum = np.concatenate([[0], np.cumsum(np.where(on_ref, lens, 0))])
         span = csum[self.cigar_off[1:]] - csum[self.cigar_off[:-1]]
+        span = np.where(self.flag & 4, 0, span)
         return np.where(span > 0, span, 1)
 
     def select(self, start, end):
@@ -140,3 +145,69 @@ def read_rvb(path):
     arrays["quals"] = take(np.uint8, nb)
     arrays["cigar"] = take(np.uint32, nc)
     return ReadBatch(arrays)
+
+
+class BamFile:
+    """A BAM file decoded by the native library (rv_bam_*): the whole file is inflated at open, and
+    query() returns the records of a region as a ReadBatch, as the reference's region iterator does."""
+
+    def __init__(self, path, threads=8):
+        import ctypes
+
+        from . import _lib
+
+        self.lib = _lib.load()
+        h = ctypes.c_void_p()
+        st = self.lib.rv_bam_open(path.encode(), threads, ctypes.byref(h))
+        self.h = h
+        if st != 0:
+            msg = self.lib.rv_bam_last_error(h) if h.value else b""
+            self.close()
+            raise OSError(f"{path}: {msg.decode() if msg else 'cannot open'} (status {st})")
+
+    def targets(self):
+        return [(self.lib.rv_bam_target_name(self.h, t).decode(), self.lib.rv_bam_target_len(self.h, t))
+                for t in range(self.lib.rv_bam_n_targets(self.h))]
+
+    def n_records(self):
+        return self.lib.rv_bam_n_records(self.h)
+
+    def query(self, chrom, start, end):
+        import ctypes
+
+        from . import _lib
+
+        r = _lib.RvReads()
+        st = self.lib.rv_bam_query(self.h, chrom.encode(), start, end, ctypes.byref(r))
+        if st != 0:
+            msg = self.lib.rv_bam_last_error(self.h)
+            raise ValueError(msg.decode() if msg else f"status {st}")
+        n = r.n_reads
+
+        def arr(ptr, dtype, count):
+            if count == 0:
+                return np.zeros(0, dtype)
+            return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(np.ctypeslib.as_ctypes_type(dtype))),
+                                         shape=(count,)).copy()
+
+        so = arr(r.seq_off, np.int64, n + 1)
+        co = arr(r.cigar_off, np.int64, n + 1)
+        arrays = {"pos0": arr(r.pos0, np.int32, n), "flag": arr(r.flag, np.uint16, n),
+                  "mapq": arr(r.mapq, np.uint8, n), "has_nm": arr(r.has_nm, np.uint8, n),
+                  "nm": arr(r.nm, np.int32, n), "mpos0": arr(r.mpos0, np.int32, n), "tid": arr(r.tid, np.int32, n),
+                  "mtid": arr(r.mtid, np.int32, n), "seq_off": so, "cigar_off": co,
+                  "bases": arr(r.bases, np.uint8, int(so[-1]) if n else 0),
+                  "quals": arr(r.quals, np.uint8, int(so[-1]) if n else 0),
+                  "cigar": arr(r.cigar, np.uint32, int(co[-1]) if n else 0)}
+        if n == 0:
+            arrays["seq_off"] = np.zeros(1, np.int64)
+            arrays["cigar_off"] = np.zeros(1, np.int64)
+        return ReadBatch(arrays)
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.rv_bam_close(self.h)
+        self.h = None
+
+    def __del__(self):
+        self.close()

@@ -30,6 +30,8 @@
 #include <string>
 #include <vector>
 
+#include <zlib.h>
+
 namespace {
 
 struct Opts {
@@ -53,6 +55,7 @@ struct Opts {
     double dupRate = 0;        // each record is followed by an identical copy with this probability
     double singleEndFrac = 0;  // records written unpaired (no mate: mate start 0)
     double unmappedFrac = 0;   // paired records flagged unmapped (0x4), alignment kept
+    bool bam = false;          // also write the records as BGZF-compressed BAM (reads.bam, normal.bam)
 };
 
 struct Variant {
@@ -133,6 +136,72 @@ struct Rec {
 
 uint32_t cw(int len, int op) { return ((uint32_t)len << 4) | (uint32_t)op; }
 
+// ---------------- BAM output (SAM/BAM specification: BGZF blocks of at most 64 KiB, each a gzip
+// member with a "BC" extra field holding the block size; records after the header)
+class BgzfWriter {
+  public:
+    explicit BgzfWriter(const std::string& path) : f_(path, std::ios::binary) {}
+    ~BgzfWriter() { close(); }
+    void write(const void* p, size_t n) {
+        const uint8_t* c = static_cast<const uint8_t*>(p);
+        while (n) {
+            const size_t k = std::min(n, kMax - buf_.size());
+            buf_.insert(buf_.end(), c, c + k);
+            c += k;
+            n -= k;
+            if (buf_.size() == kMax) flush();
+        }
+    }
+    void close() {
+        if (!f_.is_open()) return;
+        flush();
+        static const uint8_t eof[28] = {0x1f, 0x8b, 8, 4, 0, 0, 0, 0, 0, 0xff, 6, 0, 0x42, 0x43,
+                                        2, 0, 0x1b, 0, 3, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        f_.write(reinterpret_cast<const char*>(eof), 28);
+        f_.close();
+    }
+
+  private:
+    static constexpr size_t kMax = 65280;
+    void flush() {
+        if (buf_.empty()) return;
+        std::vector<uint8_t> z(compressBound((uLong)buf_.size()) + 64);
+        z_stream zs;
+        memset(&zs, 0, sizeof zs);
+        deflateInit2(&zs, 6, Z_DEFLATED, -15, 8, Z_DEFAULT_STRATEGY);
+        zs.next_in = buf_.data();
+        zs.avail_in = (uInt)buf_.size();
+        zs.next_out = z.data();
+        zs.avail_out = (uInt)z.size();
+        deflate(&zs, Z_FINISH);
+        const size_t clen = zs.total_out;
+        deflateEnd(&zs);
+        const uint32_t bsize = (uint32_t)(18 + clen + 8 - 1);
+        const uint32_t crc = (uint32_t)crc32(0L, buf_.data(), (uInt)buf_.size());
+        const uint32_t isize = (uint32_t)buf_.size();
+        const uint8_t h[18] = {0x1f, 0x8b, 8, 4, 0, 0, 0, 0, 0, 0xff, 6, 0, 0x42, 0x43, 2, 0,
+                               (uint8_t)(bsize & 0xff), (uint8_t)(bsize >> 8)};
+        f_.write(reinterpret_cast<const char*>(h), 18);
+        f_.write(reinterpret_cast<const char*>(z.data()), (std::streamsize)clen);
+        f_.write(reinterpret_cast<const char*>(&crc), 4);
+        f_.write(reinterpret_cast<const char*>(&isize), 4);
+        buf_.clear();
+    }
+    std::ofstream f_;
+    std::vector<uint8_t> buf_;
+};
+
+// reg2bin (SAM specification, section 5.3)
+int reg2bin(int beg, int end) {
+    --end;
+    if (beg >> 14 == end >> 14) return ((1 << 15) - 1) / 7 + (beg >> 14);
+    if (beg >> 17 == end >> 17) return ((1 << 12) - 1) / 7 + (beg >> 17);
+    if (beg >> 20 == end >> 20) return ((1 << 9) - 1) / 7 + (beg >> 20);
+    if (beg >> 23 == end >> 23) return ((1 << 6) - 1) / 7 + (beg >> 23);
+    if (beg >> 26 == end >> 26) return ((1 << 3) - 1) / 7 + (beg >> 26);
+    return 0;
+}
+
 // Build the record for haplotype interval [a, b): returns false when the read has no aligned base.
 bool make_record(const std::string& ref, const Hap& H, int a, int b, bool longDelClip,
                  const std::vector<uint8_t>& longDelMask, Rec& r) {
@@ -210,7 +279,7 @@ int usage() {
             "        [--indel-rate X] [--indel-max N] [--long-dels N] [--repeat-frac F]\n"
             "        [--err-rate X] [--insert-mu M] [--insert-sd S] [--vaf LO HI]\n"
             "        [--somatic-rate X] [--normal-depth D] [--dup-rate X] [--single-end-frac F]\n"
-            "        [--unmapped-frac F]\n");
+            "        [--unmapped-frac F] [--bam]\n");
     return 2;
 }
 
@@ -245,6 +314,7 @@ int main(int argc, char** argv) {
         else if (a == "--dup-rate") o.dupRate = std::stod(nx());
         else if (a == "--single-end-frac") o.singleEndFrac = std::stod(nx());
         else if (a == "--unmapped-frac") o.unmappedFrac = std::stod(nx());
+        else if (a == "--bam") o.bam = true;
         else
             return usage();
     }
@@ -499,11 +569,71 @@ int main(int argc, char** argv) {
         put(f, cig.data(), 4 * (size_t)nc);
         return n;
     };
+    auto write_bam = [&](const std::string& path, const std::vector<Rec>& recs) {
+        BgzfWriter w(path);
+        const std::string text = "@HD\tVN:1.6\tSO:coordinate\n@SQ\tSN:chrS\tLN:" + std::to_string(o.len) + "\n";
+        auto u32 = [&](uint32_t v) { w.write(&v, 4); };
+        w.write("BAM\1", 4);
+        u32((uint32_t)text.size());
+        w.write(text.data(), text.size());
+        u32(1);
+        u32(5);
+        w.write("chrS", 5);
+        u32((uint32_t)o.len);
+        std::vector<uint8_t> r;
+        for (size_t i = 0; i < recs.size(); i++) {
+            const Rec& x = recs[i];
+            const std::string name = "r" + std::to_string(i);
+            int rlen = 0;
+            for (uint32_t c : x.cigar) {
+                const int op = (int)(c & 15);
+                if (op == 0 || op == 2 || op == 3 || op == 7 || op == 8) rlen += (int)(c >> 4);
+            }
+            const int l_seq = (int)x.bases.size();
+            r.clear();
+            auto p32 = [&](uint32_t v) { for (int k = 0; k < 4; k++) r.push_back((uint8_t)(v >> (8 * k))); };
+            auto p16 = [&](uint16_t v) { r.push_back((uint8_t)v); r.push_back((uint8_t)(v >> 8)); };
+            p32(0);                    // refID
+            p32((uint32_t)x.pos0);
+            r.push_back((uint8_t)(name.size() + 1));
+            r.push_back(60);           // MAPQ
+            p16((uint16_t)reg2bin(x.pos0, x.pos0 + (rlen > 0 ? rlen : 1)));
+            p16((uint16_t)x.cigar.size());
+            p16(x.flag);
+            p32((uint32_t)l_seq);
+            p32(0);                    // next refID: the same contig (as the RVB records)
+            p32((uint32_t)x.mpos0);
+            p32(0);                    // TLEN
+            r.insert(r.end(), name.begin(), name.end());
+            r.push_back(0);
+            for (uint32_t c : x.cigar) p32(c);
+            static const char* codes = "=ACMGRSVTWYHKDBN";
+            for (int q = 0; q < l_seq; q += 2) {
+                auto code = [&](int k) -> uint8_t {
+                    if (k >= l_seq) return 0;
+                    const char* h = strchr(codes, x.bases[(size_t)k]);
+                    return h ? (uint8_t)(h - codes) : 15;
+                };
+                r.push_back((uint8_t)(code(q) << 4 | code(q + 1)));
+            }
+            for (int q = 0; q < l_seq; q++) r.push_back((uint8_t)x.quals[(size_t)q]);
+            r.push_back('N');
+            r.push_back('M');
+            r.push_back('i');
+            p32((uint32_t)x.nm);
+            u32((uint32_t)r.size());
+            w.write(r.data(), r.size());
+        }
+        w.close();
+    };
     const int64_t n = write_rvb(o.out + "/reads.rvb", recs);
+    if (o.bam) write_bam(o.out + "/reads.bam", recs);
     int64_t nn = 0;
     if (o.normalDepth > 0) {
         std::mt19937_64 gnread(o.seed * 1000003ULL + 6), gnerr(o.seed * 1000003ULL + 7);
-        nn = write_rvb(o.out + "/normal.rvb", gen_reads(o.normalDepth, false, gnread, gnerr));
+        const std::vector<Rec> nrecs = gen_reads(o.normalDepth, false, gnread, gnerr);
+        nn = write_rvb(o.out + "/normal.rvb", nrecs);
+        if (o.bam) write_bam(o.out + "/normal.bam", nrecs);
     }
     printf("{\"reads\": %lld, \"normal_reads\": %lld, \"variants\": %zu, \"len\": %d}\n", (long long)n, (long long)nn,
            vars.size(), o.len);

@@ -40,7 +40,7 @@ def _ensure_tools():
     if not os.path.exists(SYNTH):
         os.makedirs(os.path.dirname(SYNTH), exist_ok=True)
         subprocess.run(["g++", "-O2", "-std=c++17", "-o", SYNTH,
-                        os.path.join(ROOT, "rabbitvar_amd", "synth", "rv_synth.cpp")], check=True)
+                        os.path.join(ROOT, "rabbitvar_amd", "synth", "rv_synth.cpp"), "-lz"], check=True)
 
 
 @pytest.fixture(scope="session")
