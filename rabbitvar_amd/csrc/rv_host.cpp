@@ -4,8 +4,10 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <numeric>
 #include <stdexcept>
+#include <thread>
 #include <unordered_map>
 
 namespace rv {
@@ -265,15 +267,37 @@ void reduce_events(const Event* ev, size_t n, const char* arena, double goodq, H
 }
 
 void reduce_clips(const ClipRec* cl, size_t n, const char* clip_arena, double goodq, HostPileup& P, int side) {
-    std::vector<size_t> idx;
-    idx.reserve(n);
+    // Records are reduced per position in read order.  Sort keys: (side, position, read ordinal),
+    // packed in 64 bits (a read has at most one clip per side, so keys are distinct).
+    std::vector<std::pair<uint64_t, uint32_t>> keys;
+    keys.reserve(n);
     for (size_t i = 0; i < n; i++)
-        if (side < 0 || cl[i].side == side) idx.push_back(i);
-    std::sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
-        if (cl[a].side != cl[b].side) return cl[a].side < cl[b].side;
-        if (cl[a].pos != cl[b].pos) return cl[a].pos < cl[b].pos;
-        return cl[a].ord < cl[b].ord;
-    });
+        if (side < 0 || cl[i].side == side)
+            keys.push_back({((uint64_t)(cl[i].side == 5) << 63) | ((uint64_t)((uint32_t)cl[i].pos ^ 0x80000000u) << 31) |
+                                (uint64_t)(uint32_t)cl[i].ord,
+                            (uint32_t)i});
+    const size_t m = keys.size();
+    const unsigned T = m < 4096 ? 1u : std::min(8u, std::max(1u, std::thread::hardware_concurrency() / 4));
+    // sort: chunks in parallel, then pairwise merges
+    {
+        std::vector<size_t> cut(T + 1);
+        for (unsigned t = 0; t <= T; t++) cut[t] = m * t / T;
+        std::vector<std::thread> pool;
+        for (unsigned t = 1; t < T; t++)
+            pool.emplace_back([&, t] { std::sort(keys.begin() + (long)cut[t], keys.begin() + (long)cut[t + 1]); });
+        std::sort(keys.begin() + (long)cut[0], keys.begin() + (long)cut[1]);
+        for (auto& th : pool) th.join();
+        for (size_t w = 1; w < T; w *= 2) {
+            std::vector<std::thread> mp;
+            for (size_t a = 0; a + w < T; a += 2 * w) {
+                const size_t lo = cut[a], mid = cut[a + w], hi = cut[std::min<size_t>(a + 2 * w, T)];
+                mp.emplace_back([&, lo, mid, hi] {
+                    std::inplace_merge(keys.begin() + (long)lo, keys.begin() + (long)mid, keys.begin() + (long)hi);
+                });
+            }
+            for (auto& th : mp) th.join();
+        }
+    }
     const double gq = (double)(float)goodq;
     auto add = [&](HVar& v, bool dir, int rp, double q, int mapq, int nm) {
         v.cnt++;
@@ -290,31 +314,55 @@ void reduce_clips(const ClipRec* cl, size_t n, const char* clip_arena, double go
         else
             v.locnt++;
     };
-    // records arrive sorted by (side, position, read order): each position's accumulator is
-    // appended at the end of its map
-    for (size_t k : idx) {
-        const ClipRec& r = cl[k];
-        auto& m = r.side == 5 ? P.sc5 : P.sc3;
-        auto it = m.empty() ? m.end() : std::prev(m.end());
-        if (it == m.end() || it->first != r.pos) it = m.emplace_hint(m.end(), r.pos, HSclip{});
-        HSclip& s = it->second;
-        const char* bases = clip_arena + r.bases_off;
-        const char* quals = bases + r.nhq;
-        if ((int)s.col.size() < r.nhq) s.col.resize((size_t)r.nhq);
-        for (int i = 0; i < r.nhq; i++) {
-            const char ch = bases[i];
-            ClipColumn& c = s.col[(size_t)i];
-            bool found = false;
-            for (auto& pr : c.nt)
-                if (pr.first == ch) {
-                    pr.second++;
-                    found = true;
-                }
-            if (!found) c.nt.push_back({ch, 1});
-            const int rp = r.side == 5 ? r.nhq - 1 - i : r.nhq - i;
-            add(c.seq_at(ch), r.dir != 0, rp, (double)(uint8_t)quals[i], r.mapq, r.nm);
+    // accumulate position ranges on separate threads (no position spans two ranges), each into its
+    // own map, then move the nodes into the pileup's maps in order
+    auto same_pos = [&](size_t a, size_t b) { return (keys[a].first >> 31) == (keys[b].first >> 31); };
+    std::vector<size_t> cut(T + 1);
+    cut[0] = 0;
+    cut[T] = m;
+    for (unsigned t = 1; t < T; t++) {
+        size_t c = std::max(cut[t - 1], m * t / T);
+        while (c > cut[t - 1] && c < m && same_pos(c - 1, c)) c++;
+        cut[t] = c;
+    }
+    std::vector<std::map<int, HSclip>> part5(T), part3(T);
+    auto run = [&](unsigned t) {
+        for (size_t q = cut[t]; q < cut[t + 1]; q++) {
+            const ClipRec& r = cl[keys[q].second];
+            auto& mp = r.side == 5 ? part5[t] : part3[t];
+            auto it = mp.empty() ? mp.end() : std::prev(mp.end());
+            if (it == mp.end() || it->first != r.pos) it = mp.emplace_hint(mp.end(), r.pos, HSclip{});
+            HSclip& sc = it->second;
+            const char* bases = clip_arena + r.bases_off;
+            const char* quals = bases + r.nhq;
+            if ((int)sc.col.size() < r.nhq) sc.col.resize((size_t)r.nhq);
+            for (int i = 0; i < r.nhq; i++) {
+                const char ch = bases[i];
+                ClipColumn& c = sc.col[(size_t)i];
+                bool found = false;
+                for (auto& pr : c.nt)
+                    if (pr.first == ch) {
+                        pr.second++;
+                        found = true;
+                    }
+                if (!found) c.nt.push_back({ch, 1});
+                const int rp = r.side == 5 ? r.nhq - 1 - i : r.nhq - i;
+                add(c.seq_at(ch), r.dir != 0, rp, (double)(uint8_t)quals[i], r.mapq, r.nm);
+            }
+            add(sc.v, r.dir != 0, r.cel, r.sumq / (double)r.nhq, r.mapq, r.nm);
         }
-        add(s.v, r.dir != 0, r.cel, r.sumq / (double)r.nhq, r.mapq, r.nm);
+    };
+    {
+        std::vector<std::thread> pool;
+        for (unsigned t = 1; t < T; t++) pool.emplace_back(run, t);
+        run(0);
+        for (auto& th : pool) th.join();
+    }
+    for (unsigned t = 0; t < T; t++) {
+        for (auto* pr : {&part5[t], &part3[t]}) {
+            auto& dst = pr == &part5[t] ? P.sc5 : P.sc3;
+            while (!pr->empty()) dst.insert(dst.end(), pr->extract(pr->begin()));
+        }
     }
 }
 
