@@ -2,6 +2,7 @@
 #include "rv_order.h"
 
 #include <algorithm>
+#include <climits>
 #include <thread>
 #include <unordered_set>
 
@@ -206,16 +207,34 @@ bool noninsertion_order_sorted(const std::vector<int32_t>& walk_sorted, int maxd
         return true;
     }
     // the realigner's operations: erased walk keys leave the order; inserted keys join it after
-    // every earlier key of equal (home, hash bits)
-    std::unordered_set<int32_t> walk_keys(walk_sorted.begin(), walk_sorted.end());
-    std::unordered_set<int32_t> erased, added_set;
+    // every earlier key of equal (home, hash bits).  Walk keys and their erasures are bitmaps over
+    // the walk keys' span (a few million positions: hash sets of them cost more than the rest).
+    int32_t lo = INT32_MAX, hi = INT32_MIN;
+    for (int32_t k : walk_sorted) {
+        lo = k < lo ? k : lo;
+        hi = k > hi ? k : hi;
+    }
+    const size_t span = n_walk ? (size_t)((int64_t)hi - lo + 1) : 0;
+    std::vector<uint64_t> is_walk((span + 63) / 64, 0), was_erased((span + 63) / 64, 0);
+    auto bit = [&](const std::vector<uint64_t>& b, int32_t k) {
+        if (!n_walk || k < lo || k > hi) return false;
+        const size_t i = (size_t)((int64_t)k - lo);
+        return ((b[i >> 6] >> (i & 63)) & 1u) != 0;
+    };
+    auto set_bit = [&](std::vector<uint64_t>& b, int32_t k) {
+        const size_t i = (size_t)((int64_t)k - lo);
+        b[i >> 6] |= 1ull << (i & 63);
+    };
+    for (int32_t k : walk_sorted) set_bit(is_walk, k);
+    std::unordered_set<int32_t> added_set;
     std::vector<int32_t> added;  // in insertion order
     size_t count = n_walk, max_count = n_walk;
     for (const KeyOp& op : log) {
-        const bool present = (walk_keys.count(op.pos) && !erased.count(op.pos)) || added_set.count(op.pos);
+        const bool walk = bit(is_walk, op.pos);
+        const bool present = (walk && !bit(was_erased, op.pos)) || added_set.count(op.pos);
         if (op.insert) {
             if (present) continue;
-            if (walk_keys.count(op.pos)) erased.insert(op.pos);  // re-inserted: now the latest key
+            if (walk) set_bit(was_erased, op.pos);  // re-inserted: now the latest key
             added_set.insert(op.pos);
             added.erase(std::remove(added.begin(), added.end(), op.pos), added.end());
             added.push_back(op.pos);
@@ -227,7 +246,7 @@ bool noninsertion_order_sorted(const std::vector<int32_t>& walk_sorted, int maxd
                 added_set.erase(op.pos);
                 added.erase(std::remove(added.begin(), added.end(), op.pos), added.end());
             } else {
-                erased.insert(op.pos);
+                set_bit(was_erased, op.pos);
             }
             --count;
         }
@@ -250,9 +269,11 @@ bool noninsertion_order_sorted(const std::vector<int32_t>& walk_sorted, int maxd
     size_t e = 0;
     for (size_t i = 0; i < n_walk; i++) {
         const int32_t k = walk_sorted[i];
-        const uint64_t r = rank(k);
-        while (e < extra.size() && extra[e].first < r) out.push_back(extra[e++].second);
-        if (!erased.count(k)) out.push_back(k);
+        if (e < extra.size()) {
+            const uint64_t r = rank(k);
+            while (e < extra.size() && extra[e].first < r) out.push_back(extra[e++].second);
+        }
+        if (!bit(was_erased, k)) out.push_back(k);
     }
     while (e < extra.size()) out.push_back(extra[e++].second);
     return true;
