@@ -51,6 +51,47 @@ def ensure_data(d):
     subprocess.run([SYNTH, "--out", d] + WORKLOAD, check=True, stdout=subprocess.DEVNULL)
 
 
+# amplicon-style panel (a scaled form of the 20,000-region panel: 60 regions, log-normal lengths,
+# depth 500-5000x per region, pairs starting at the amplicon ends)
+PANEL = ["--seed", "77", "--len", "2000000", "--panel", "60", "--panel-len", "7.0", "1.0", "--panel-depth", "500",
+         "5000", "--snv-rate", "0.0005", "--indel-rate", "0.0001", "--bam"]
+
+
+def panel_bench(d, steps):
+    """Region-list throughput: the engine pool (4 engines) against one engine, same regions."""
+    from rabbitvar_amd import BamFile, read_fasta
+    from rabbitvar_amd.engine import EnginePool
+
+    if not os.path.exists(os.path.join(d, "reads.bam")):
+        os.makedirs(d, exist_ok=True)
+        log(f"generating the panel into {d}")
+        subprocess.run([SYNTH, "--out", d] + PANEL, check=True, stdout=subprocess.DEVNULL)
+    regions = []
+    for ln in open(os.path.join(d, "panel.bed")):
+        c, s0, e0, g = ln.rstrip("\n").split("\t")
+        regions.append((c, int(s0), int(e0), g))
+    name, seq = read_fasta(os.path.join(d, "ref.fa"))
+    t0 = time.perf_counter()
+    bam = BamFile(os.path.join(d, "reads.bam"), threads=16)
+    open_s = time.perf_counter() - t0
+    n_reads = sum(bam.query(c, s0, e0).n for c, s0, e0, _ in regions)
+    res = {"regions": len(regions), "reads": n_reads, "bam_bytes": os.path.getsize(os.path.join(d, "reads.bam")),
+           "bam_open_s": round(open_s, 3)}
+    for workers in (1, 4):
+        pool = EnginePool(workers=workers, sample="S1", fisher=1)
+        pool.run_regions(regions, bam, {name: seq})  # warmup: buffers grow to the largest region
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            pool.run_regions(regions, bam, {name: seq})
+        dt = (time.perf_counter() - t0) / steps
+        pool.close()
+        res[f"reads_per_s_{workers}_engine" + ("s" if workers > 1 else "")] = round(n_reads / dt, 1)
+        res[f"s_per_pass_{workers}"] = round(dt, 4)
+    log(f"panel: {len(regions)} regions, {n_reads} reads; 1 engine {res['reads_per_s_1_engine']:.0f} reads/s, "
+        f"4 engines {res['reads_per_s_4_engines']:.0f} reads/s")
+    return res
+
+
 def cpu_baseline(d):
     if not os.path.exists(ORACLE):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
@@ -73,6 +114,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--data", default=os.environ.get("RV_BENCH_DATA", "/tmp/rv_bench_5mb_60x"))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-panel", action="store_true")
+    ap.add_argument("--panel-data", default=os.environ.get("RV_BENCH_PANEL", "/tmp/rv_bench_panel"))
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: the calling path has no multi-GPU mode here (asked for {a.gpus}); "
@@ -185,9 +228,11 @@ def main():
                                                                [round(x, 2) for x in st["detail_ms"][:6]]))}
     log(f"screened: {res['screened']['value']} reads/s ({res['screened']['speedup']}x), recall "
         f"{cmp_['recall']:.6f}, precision {cmp_['precision']:.6f}, skipped {skipped}/{cands}")
+    eng.close()
+    if not a.no_panel:
+        res["panel"] = panel_bench(a.panel_data, max(1, min(a.steps, 3)))
     if not a.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(a.data)
-    eng.close()
     print(json.dumps(res), flush=True)
 
 

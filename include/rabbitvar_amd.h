@@ -234,6 +234,22 @@ size_t rv_bam_n_records(const rv_bam* b);
 rv_status rv_bam_query(rv_bam* b, const char* chr, int32_t start, int32_t end, rv_reads* out);
 void rv_bam_close(rv_bam* b);
 
+/* ---- Region lists.  Replaces the reference's region loop (SimpleMode::process / SomaticMode::process,
+ * src/modes/simpleMode.cpp:210-345, src/modes/somaticMode.cpp:737-957: an OpenMP loop over the BED
+ * regions, one data pool per thread).  A pool holds `workers` engines on one device, each with its own
+ * stream and buffers; each is driven by its own host thread, so one region's host stages overlap
+ * others' device work.  Regions are handed out in order to whichever engine is free; out[i] receives
+ * region i's call set (free each with rv_callset_free), so the text is in region order whatever the
+ * completion order.  second != NULL: somatic mode, as rv_run_region_somatic. */
+typedef struct rv_pool rv_pool;
+rv_status rv_pool_open(const rv_config* cfg, int device, int workers, rv_pool** out);
+rv_status rv_pool_run(rv_pool* p, const rv_region* regions, size_t n, rv_bam* first, rv_bam* second,
+                      const rv_contig* contigs, size_t n_contigs, rv_callset** out);
+/* screened mode on every engine of the pool (as rv_set_screen) */
+rv_status rv_pool_set_screen(rv_pool* p, const float* params, size_t n_params, float threshold);
+const char* rv_pool_last_error(const rv_pool* p);
+void rv_pool_close(rv_pool* p);
+
 /* Diagnostics: the pileup-stage state (after the walk, before realignment) as text, one record per
  * line, in the layout tests compare against.  Free with rv_free. */
 rv_status rv_debug_pileup(rv_engine* e, const rv_region* region, const rv_reads* reads, const rv_contig* contig,

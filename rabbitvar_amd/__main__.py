@@ -84,6 +84,7 @@ def main(argv=None):
     ap.add_argument("--fisher", action="store_true")
     ap.add_argument("--out", help="output file (default: stdout)")
     ap.add_argument("--th", type=int, default=8, help="BAM inflate threads")
+    ap.add_argument("--workers", type=int, default=4, help="engines working on regions at once (BAM input)")
     ap.add_argument("--device", type=int, default=0)
     ap.add_argument("--screen", action="store_true", help="screened mode with the trained model")
     for flag, field, typ in OPTIONS:
@@ -105,18 +106,39 @@ def main(argv=None):
         ap.error("give -R or -i")
     paths = a.b.split("|")
     samples = [Reads(p, a.th) for p in paths]
+    contigs = {}
+    for chrom, _, _, _ in regions:
+        if chrom not in contigs:
+            contigs[chrom] = read_fasta(a.G, chrom)[1]
+    out = open(a.out, "w") if a.out else sys.stdout
+    if all(s.bam is not None for s in samples):
+        # BAM input: the native pool reads each region's records and runs the regions concurrently
+        from .engine import EnginePool
+
+        pool = EnginePool(workers=a.workers, device=a.device, sample=a.N, **cfg)
+        try:
+            if a.screen:
+                from . import screen
+
+                params, threshold, _ = screen.load_model()
+                pool.set_screen(params, threshold)
+            texts = pool.run_regions(regions, samples[0].bam, contigs,
+                                     bam2=samples[1].bam if len(samples) == 2 else None)
+            for t in texts:
+                out.write(t)
+        finally:
+            if a.out:
+                out.close()
+            pool.close()
+        return 0
     eng = Engine(device=a.device, sample=a.N, **cfg)
     if a.screen:
         from . import screen
 
         params, threshold, _ = screen.load_model()
         eng.set_screen(params, threshold)
-    out = open(a.out, "w") if a.out else sys.stdout
-    contigs = {}
     try:
         for chrom, start, end, gene in regions:
-            if chrom not in contigs:
-                contigs[chrom] = read_fasta(a.G, chrom)[1]
             seq = contigs[chrom]
             rs = [s.region(chrom, start, end) for s in samples]
             if len(rs) == 2:

@@ -108,6 +108,12 @@ SIGNATURES = {
     "rv_bam_n_records": (c_size_t, [c_void_p]),
     "rv_bam_query": (c_int, [c_void_p, c_char_p, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(RvReads)]),
     "rv_bam_close": (None, [c_void_p]),
+    "rv_pool_open": (c_int, [ctypes.POINTER(RvConfig), c_int, c_int, ctypes.POINTER(c_void_p)]),
+    "rv_pool_run": (c_int, [c_void_p, ctypes.POINTER(RvRegion), c_size_t, c_void_p, c_void_p,
+                            ctypes.POINTER(RvContig), c_size_t, ctypes.POINTER(c_void_p)]),
+    "rv_pool_set_screen": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_float), c_size_t, ctypes.c_float]),
+    "rv_pool_last_error": (c_char_p, [c_void_p]),
+    "rv_pool_close": (None, [c_void_p]),
 }
 
 _lib = None

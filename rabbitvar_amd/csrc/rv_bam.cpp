@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "rabbitvar_amd.h"
+#include "rv_bam.h"
 
 namespace {
 
@@ -50,15 +51,12 @@ struct rv_bam {
     std::vector<int32_t> lens;
     std::vector<size_t> rec;  // offset of each record's block_size field
     std::vector<int32_t> rec_tid, rec_pos, rec_end;
+    bool sorted = false;     // coordinate order: records of a contig contiguous, starts non-decreasing
+    int32_t max_span = 0;    // longest record (end - start)
     int threads = 1;
     double inflate_ms = 0, index_ms = 0;
     uint64_t bytes_in = 0;
-    // the last query's arrays
-    std::vector<int32_t> pos0, nm, mpos0, tid, mtid;
-    std::vector<uint16_t> flag;
-    std::vector<uint8_t> mapq, has_nm, bases, quals;
-    std::vector<int64_t> seq_off, cigar_off;
-    std::vector<uint32_t> cigar;
+    rv::BamQuery last;  // the arrays of the last rv_bam_query
 };
 
 namespace {
@@ -209,6 +207,19 @@ rv_status load(rv_bam* b, const char* path) {
         b->rec_pos[i] = pos;
         b->rec_end[i] = (int32_t)std::min<int64_t>(INT32_MAX, pos + rlen);
     });
+    b->sorted = true;
+    std::vector<char> seen(b->names.size() + 1, 0);
+    for (size_t i = 0; i < nr; i++) {
+        b->max_span = std::max(b->max_span, b->rec_end[i] - b->rec_pos[i]);
+        if (i == 0 || b->rec_tid[i] != b->rec_tid[i - 1]) {
+            const int32_t t = b->rec_tid[i];
+            const size_t slot = t < 0 || (size_t)t >= b->names.size() ? b->names.size() : (size_t)t;
+            if (seen[slot]) b->sorted = false;
+            seen[slot] = 1;
+        } else if (b->rec_pos[i] < b->rec_pos[i - 1]) {
+            b->sorted = false;
+        }
+    }
     return RV_OK;
 }
 
@@ -298,14 +309,51 @@ size_t rv_bam_n_records(const rv_bam* b) { return b ? b->rec.size() : 0; }
 
 rv_status rv_bam_query(rv_bam* b, const char* chr, int32_t start, int32_t end, rv_reads* out) {
     if (!b || !chr || !out) return RV_ERR_ARG;
+    std::string err;
+    const rv_status s = rv::bam_query_into(b, chr, start, end, b->last, out, err);
+    if (s != RV_OK) b->err = err;
+    return s;
+}
+
+void rv_bam_close(rv_bam* b) { delete b; }
+
+}  // extern "C"
+
+namespace rv {
+
+rv_status bam_query_into(const rv_bam* b, const char* chr, int32_t start, int32_t end, BamQuery& q, rv_reads* out,
+                         std::string& err) {
+    if (!b || !chr || !out) return RV_ERR_ARG;
     int32_t t = -1;
     for (size_t i = 0; i < b->names.size(); i++)
         if (b->names[i] == chr) t = (int32_t)i;
-    if (t < 0) return fail(b, std::string("contig not in the BAM header: ") + chr);
+    if (t < 0) {
+        err = std::string("contig not in the BAM header: ") + chr;
+        return RV_ERR_ARG;
+    }
     // 1-based inclusive region -> [beg, end) 0-based, as htslib's region parser
     const int64_t beg = (int64_t)std::max(start, 1) - 1, en = end;
     std::vector<size_t> sel;
-    for (size_t i = 0; i < b->rec.size(); i++) {
+    size_t lo = 0, hi = b->rec.size();
+    if (b->sorted) {
+        // the contig's block, then the starts that can reach the region: [beg - max_span, end)
+        auto key = [&](size_t i) { return std::make_pair(b->rec_tid[i] < 0 ? INT32_MAX : b->rec_tid[i], b->rec_pos[i]); };
+        auto first_at_least = [&](std::pair<int32_t, int64_t> k) {
+            size_t a = 0, z = b->rec.size();
+            while (a < z) {
+                const size_t m = (a + z) / 2;
+                const auto km = key(m);
+                if (km.first < k.first || (km.first == k.first && km.second < k.second))
+                    a = m + 1;
+                else
+                    z = m;
+            }
+            return a;
+        };
+        lo = first_at_least({t, beg - (int64_t)b->max_span});
+        hi = first_at_least({t, en});
+    }
+    for (size_t i = lo; i < hi; i++) {
         if (b->rec_tid[i] != t) continue;
         if ((int64_t)b->rec_pos[i] >= en) continue;
         if ((int64_t)b->rec_end[i] <= beg) continue;
@@ -314,69 +362,67 @@ rv_status rv_bam_query(rv_bam* b, const char* chr, int32_t start, int32_t end, r
     const size_t n = sel.size();
     const uint8_t* d = b->data.data();
     // sizes, then offsets
-    b->seq_off.assign(n + 1, 0);
-    b->cigar_off.assign(n + 1, 0);
+    q.seq_off.assign(n + 1, 0);
+    q.cigar_off.assign(n + 1, 0);
     for (size_t k = 0; k < n; k++) {
         const uint8_t* r = d + b->rec[sel[k]] + 4;
-        b->seq_off[k + 1] = b->seq_off[k] + rd32s(r + 16);
-        b->cigar_off[k + 1] = b->cigar_off[k] + rd16(r + 12);
+        q.seq_off[k + 1] = q.seq_off[k] + rd32s(r + 16);
+        q.cigar_off[k + 1] = q.cigar_off[k] + rd16(r + 12);
     }
-    b->pos0.resize(n);
-    b->nm.resize(n);
-    b->mpos0.resize(n);
-    b->tid.resize(n);
-    b->mtid.resize(n);
-    b->flag.resize(n);
-    b->mapq.resize(n);
-    b->has_nm.resize(n);
-    b->bases.resize((size_t)b->seq_off[n]);
-    b->quals.resize((size_t)b->seq_off[n]);
-    b->cigar.resize((size_t)b->cigar_off[n]);
+    q.pos0.resize(n);
+    q.nm.resize(n);
+    q.mpos0.resize(n);
+    q.tid.resize(n);
+    q.mtid.resize(n);
+    q.flag.resize(n);
+    q.mapq.resize(n);
+    q.has_nm.resize(n);
+    q.bases.resize((size_t)q.seq_off[n]);
+    q.quals.resize((size_t)q.seq_off[n]);
+    q.cigar.resize((size_t)q.cigar_off[n]);
     parallel_for(n, b->threads, [&](size_t k) {
         const size_t o = b->rec[sel[k]];
         const uint8_t* r = d + o + 4;
         const uint8_t* rend = r + rd32(d + o);
-        b->tid[k] = rd32s(r);
-        b->pos0[k] = rd32s(r + 4);
+        q.tid[k] = rd32s(r);
+        q.pos0[k] = rd32s(r + 4);
         const uint8_t l_read_name = r[8];
-        b->mapq[k] = r[9];
+        q.mapq[k] = r[9];
         const uint16_t n_cigar = rd16(r + 12);
-        b->flag[k] = rd16(r + 14);
+        q.flag[k] = rd16(r + 14);
         const int32_t l_seq = rd32s(r + 16);
-        b->mtid[k] = rd32s(r + 20);
-        b->mpos0[k] = rd32s(r + 24);
+        q.mtid[k] = rd32s(r + 20);
+        q.mpos0[k] = rd32s(r + 24);
         const uint8_t* cg = r + 32 + l_read_name;
-        for (uint16_t c = 0; c < n_cigar; c++) b->cigar[(size_t)b->cigar_off[k] + c] = rd32(cg + 4 * c);
+        for (uint16_t c = 0; c < n_cigar; c++) q.cigar[(size_t)q.cigar_off[k] + c] = rd32(cg + 4 * c);
         const uint8_t* sq = cg + 4 * (size_t)n_cigar;
-        uint8_t* bo = b->bases.data() + b->seq_off[k];
+        uint8_t* bo = q.bases.data() + q.seq_off[k];
         for (int32_t q = 0; q < l_seq; q++) bo[q] = (uint8_t)NT16[(sq[q >> 1] >> ((~q & 1) << 2)) & 15];
         const uint8_t* ql = sq + (l_seq + 1) / 2;
-        memcpy(b->quals.data() + b->seq_off[k], ql, (size_t)l_seq);
+        memcpy(q.quals.data() + q.seq_off[k], ql, (size_t)l_seq);
         int32_t v = 0;
         const bool has = aux_nm(ql + l_seq, rend, v);
-        b->has_nm[k] = has ? 1 : 0;
-        b->nm[k] = has ? v : 0;
+        q.has_nm[k] = has ? 1 : 0;
+        q.nm[k] = has ? v : 0;
     });
     rv_reads rr;
     rr.n_reads = (int64_t)n;
-    rr.pos0 = b->pos0.data();
-    rr.flag = b->flag.data();
-    rr.mapq = b->mapq.data();
-    rr.has_nm = b->has_nm.data();
-    rr.nm = b->nm.data();
-    rr.mpos0 = b->mpos0.data();
-    rr.tid = b->tid.data();
-    rr.mtid = b->mtid.data();
-    rr.seq_off = b->seq_off.data();
-    rr.cigar_off = b->cigar_off.data();
-    rr.bases = b->bases.data();
-    rr.quals = b->quals.data();
-    rr.cigar = b->cigar.data();
+    rr.pos0 = q.pos0.data();
+    rr.flag = q.flag.data();
+    rr.mapq = q.mapq.data();
+    rr.has_nm = q.has_nm.data();
+    rr.nm = q.nm.data();
+    rr.mpos0 = q.mpos0.data();
+    rr.tid = q.tid.data();
+    rr.mtid = q.mtid.data();
+    rr.seq_off = q.seq_off.data();
+    rr.cigar_off = q.cigar_off.data();
+    rr.bases = q.bases.data();
+    rr.quals = q.quals.data();
+    rr.cigar = q.cigar.data();
     rr.on_device = 0;
     *out = rr;
     return RV_OK;
 }
 
-void rv_bam_close(rv_bam* b) { delete b; }
-
-}  // extern "C"
+}  // namespace rv

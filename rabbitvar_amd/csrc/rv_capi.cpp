@@ -100,6 +100,7 @@ rv_status rv_run_region(rv_engine* e, const rv_region* region, const rv_reads* r
     *out = nullptr;
     rv_callset* cs = new (std::nothrow) rv_callset;
     if (!cs) return RV_ERR_OOM;
+    (void)hipSetDevice(e->e->device);  // the calling thread may be new (engine pools)
     const auto t0 = std::chrono::steady_clock::now();
     rv_status s = guarded(e, [&] { return rv::run_region(*e->e, *region, *reads, *contig, cs->cs); });
     e->e->stats.call_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -117,6 +118,7 @@ rv_status rv_run_region_somatic(rv_engine* e, const rv_region* region, const rv_
     *out = nullptr;
     rv_callset* cs = new (std::nothrow) rv_callset;
     if (!cs) return RV_ERR_OOM;
+    (void)hipSetDevice(e->e->device);
     const auto t0 = std::chrono::steady_clock::now();
     rv_status s =
         guarded(e, [&] { return rv::run_region_somatic(*e->e, *region, *tumor, *normal, *contig, cs->cs); });
@@ -174,5 +176,117 @@ rv_status rv_debug_pileup(rv_engine* e, const rv_region* region, const rv_reads*
 }
 
 void rv_free(void* p) { std::free(p); }
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ engine pool (region lists)
+#include <atomic>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "rv_bam.h"
+
+struct rv_pool {
+    std::vector<rv_engine*> engines;
+    std::string err;
+};
+
+extern "C" {
+
+rv_status rv_pool_open(const rv_config* cfg, int device, int workers, rv_pool** out) {
+    if (!cfg || !out || workers <= 0) return RV_ERR_ARG;
+    *out = nullptr;
+    rv_pool* p = new (std::nothrow) rv_pool;
+    if (!p) return RV_ERR_OOM;
+    for (int w = 0; w < workers; w++) {
+        rv_engine* e = nullptr;
+        const rv_status s = rv_open(cfg, device, &e);
+        if (s != RV_OK) {
+            for (rv_engine* x : p->engines) rv_close(x);
+            delete p;
+            return s;
+        }
+        p->engines.push_back(e);
+    }
+    *out = p;
+    return RV_OK;
+}
+
+rv_status rv_pool_run(rv_pool* p, const rv_region* regions, size_t n, rv_bam* first, rv_bam* second,
+                      const rv_contig* contigs, size_t n_contigs, rv_callset** out) {
+    if (!p || (n && (!regions || !first || !contigs || !out))) return RV_ERR_ARG;
+    for (size_t i = 0; i < n; i++) out[i] = nullptr;
+    std::atomic<size_t> next{0};
+    std::atomic<bool> stop{false};
+    std::mutex mu;
+    rv_status first_err = RV_OK;
+    auto worker = [&](rv_engine* eng) {
+        rv::BamQuery q1, q2;
+        for (;;) {
+            if (stop) return;
+            const size_t i = next.fetch_add(1);
+            if (i >= n) return;
+            const rv_region& rg = regions[i];
+            const rv_contig* ct = nullptr;
+            for (size_t c = 0; c < n_contigs; c++)
+                if (contigs[c].name && rg.chr && std::strcmp(contigs[c].name, rg.chr) == 0) ct = &contigs[c];
+            rv_status s = RV_OK;
+            std::string msg;
+            rv_reads r1{}, r2{};
+            if (!ct) {
+                s = RV_ERR_ARG;
+                msg = std::string("no reference sequence for ") + (rg.chr ? rg.chr : "(null)");
+            }
+            if (s == RV_OK) s = rv::bam_query_into(first, rg.chr, rg.start, rg.end, q1, &r1, msg);
+            if (s == RV_OK && second) s = rv::bam_query_into(second, rg.chr, rg.start, rg.end, q2, &r2, msg);
+            rv_callset* cs = nullptr;
+            if (s == RV_OK) {
+                s = second ? rv_run_region_somatic(eng, &rg, &r1, &r2, ct, &cs) : rv_run_region(eng, &rg, &r1, ct, &cs);
+                if (s != RV_OK) msg = rv_last_error(eng);
+            }
+            if (s != RV_OK) {
+                std::lock_guard<std::mutex> lk(mu);
+                if (first_err == RV_OK) {
+                    first_err = s;
+                    p->err = "region " + std::to_string(i) + ": " + msg;
+                }
+                stop = true;
+                return;
+            }
+            out[i] = cs;
+        }
+    };
+    std::vector<std::thread> pool;
+    for (size_t w = 1; w < p->engines.size(); w++) pool.emplace_back(worker, p->engines[w]);
+    worker(p->engines[0]);
+    for (auto& th : pool) th.join();
+    if (first_err != RV_OK) {
+        for (size_t i = 0; i < n; i++) {
+            rv_callset_free(out[i]);
+            out[i] = nullptr;
+        }
+    }
+    return first_err;
+}
+
+const char* rv_pool_last_error(const rv_pool* p) { return p ? p->err.c_str() : ""; }
+
+rv_status rv_pool_set_screen(rv_pool* p, const float* params, size_t n_params, float threshold) {
+    if (!p) return RV_ERR_ARG;
+    for (rv_engine* e : p->engines) {
+        const rv_status s = rv_set_screen(e, params, n_params, threshold);
+        if (s != RV_OK) return s;
+    }
+    return RV_OK;
+}
+
+void rv_pool_close(rv_pool* p) {
+    if (!p) return;
+    for (rv_engine* e : p->engines) rv_close(e);
+    delete p;
+}
 
 }  // extern "C"

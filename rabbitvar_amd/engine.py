@@ -246,3 +246,75 @@ class Mlp:
 
     def __del__(self):
         self.close()
+
+
+class EnginePool:
+    """Several engines on one device, one host thread each (rv_pool_*): a list of regions runs with
+    one region's host stages overlapping others' device work; the texts come back in region order."""
+
+    def __init__(self, workers=4, device=0, sample="S1", **cfg_fields):
+        self.lib = _lib.load()
+        cfg = _lib.RvConfig()
+        self.lib.rv_config_default(ctypes.byref(cfg))
+        for k, v in cfg_fields.items():
+            if not hasattr(cfg, k):
+                raise KeyError(f"unknown configuration field {k}")
+            setattr(cfg, k, v)
+        if cfg.doPileup:
+            cfg.freq = -1.0
+            cfg.minr = 0
+        self._sample = sample.encode()
+        cfg.sample = self._sample
+        self.cfg = cfg
+        h = ctypes.c_void_p()
+        st = self.lib.rv_pool_open(ctypes.byref(cfg), device, workers, ctypes.byref(h))
+        if st != 0:
+            raise EngineError(f"rv_pool_open failed with status {st}")
+        self.h = h
+
+    def set_screen(self, params, threshold):
+        if params is None:
+            st = self.lib.rv_pool_set_screen(self.h, None, 0, 0.0)
+        else:
+            self._screen = np.ascontiguousarray(params, np.float32)
+            st = self.lib.rv_pool_set_screen(self.h, self._screen.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                             self._screen.size, float(threshold))
+        if st != 0:
+            raise EngineError(f"rv_pool_set_screen failed with status {st}")
+
+    def run_regions(self, regions, bam, contigs, bam2=None):
+        """regions: [(chrom, start, end, gene)]; bam, bam2: rabbitvar_amd.BamFile (bam2 for somatic
+        mode); contigs: {name: sequence bytes}.  Returns the texts, one per region."""
+        n = len(regions)
+        keep = []
+        regs = (_lib.RvRegion * max(n, 1))()
+        for i, (chrom, start, end, gene) in enumerate(regions):
+            c, g = chrom.encode(), (gene or chrom).encode()
+            keep += [c, g]
+            regs[i] = _lib.RvRegion(c, start, end, g)
+        names = list(contigs)
+        cts = (_lib.RvContig * max(len(names), 1))()
+        for i, name in enumerate(names):
+            nb = name.encode()
+            keep += [nb, contigs[name]]
+            cts[i] = _lib.RvContig(nb, len(contigs[name]), contigs[name])
+        out = (ctypes.c_void_p * max(n, 1))()
+        st = self.lib.rv_pool_run(self.h, regs, n, bam.h, bam2.h if bam2 is not None else None, cts, len(names), out)
+        if st != 0:
+            msg = self.lib.rv_pool_last_error(self.h)
+            raise EngineError(f"status {st}: {msg.decode() if msg else ''}")
+        texts = []
+        for i in range(n):
+            ln = ctypes.c_size_t()
+            p = self.lib.rv_callset_text(out[i], ctypes.byref(ln))
+            texts.append(ctypes.string_at(p, ln.value).decode())
+            self.lib.rv_callset_free(out[i])
+        return texts
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rv_pool_close(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

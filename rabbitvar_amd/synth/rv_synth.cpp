@@ -56,6 +56,14 @@ struct Opts {
     double singleEndFrac = 0;  // records written unpaired (no mate: mate start 0)
     double unmappedFrac = 0;   // paired records flagged unmapped (0x4), alignment kept
     bool bam = false;          // also write the records as BGZF-compressed BAM (reads.bam, normal.bam)
+    // amplicon panel: reads only inside `panel` regions (written to panel.bed, 1-based inclusive),
+    // region lengths log-normal (mu, sigma) clamped to [100, 50000], depth per region uniform in
+    // [panelDepthLo, panelDepthHi]; each region is tiled by amplicons whose read pairs all start at
+    // the amplicon's ends
+    int panel = 0;
+    double panelMu = 7.71, panelSigma = 1.21;
+    double panelDepthLo = 500, panelDepthHi = 5000;
+    int ampLen = 200;
 };
 
 struct Variant {
@@ -168,7 +176,7 @@ class BgzfWriter {
         std::vector<uint8_t> z(compressBound((uLong)buf_.size()) + 64);
         z_stream zs;
         memset(&zs, 0, sizeof zs);
-        deflateInit2(&zs, 6, Z_DEFLATED, -15, 8, Z_DEFAULT_STRATEGY);
+        deflateInit2(&zs, 1, Z_DEFLATED, -15, 8, Z_DEFAULT_STRATEGY);  // fast: test data
         zs.next_in = buf_.data();
         zs.avail_in = (uInt)buf_.size();
         zs.next_out = z.data();
@@ -279,7 +287,8 @@ int usage() {
             "        [--indel-rate X] [--indel-max N] [--long-dels N] [--repeat-frac F]\n"
             "        [--err-rate X] [--insert-mu M] [--insert-sd S] [--vaf LO HI]\n"
             "        [--somatic-rate X] [--normal-depth D] [--dup-rate X] [--single-end-frac F]\n"
-            "        [--unmapped-frac F] [--bam]\n");
+            "        [--unmapped-frac F] [--bam] [--panel N] [--panel-len MU SIGMA] [--panel-depth LO HI]\n"
+            "        [--amp-len L]\n");
     return 2;
 }
 
@@ -315,6 +324,14 @@ int main(int argc, char** argv) {
         else if (a == "--single-end-frac") o.singleEndFrac = std::stod(nx());
         else if (a == "--unmapped-frac") o.unmappedFrac = std::stod(nx());
         else if (a == "--bam") o.bam = true;
+        else if (a == "--panel") o.panel = std::stoi(nx());
+        else if (a == "--panel-len") {
+            o.panelMu = std::stod(nx());
+            o.panelSigma = std::stod(nx());
+        } else if (a == "--panel-depth") {
+            o.panelDepthLo = std::stod(nx());
+            o.panelDepthHi = std::stod(nx());
+        } else if (a == "--amp-len") o.ampLen = std::stoi(nx());
         else
             return usage();
     }
@@ -419,25 +436,46 @@ int main(int argc, char** argv) {
     // ---------------- reads
     // tumor (or the only sample): haplotypes 0/1, and haplotype 2 with probability vaf when there
     // are somatic variants; normal: haplotypes 0/1 from its own streams
+    // panel regions (1-based inclusive) from their own stream; hap index of every reference
+    // position (the next kept base for a deleted one)
+    std::vector<std::pair<int, int>> panel_regions;
+    std::vector<std::vector<int32_t>> ref2hap(3);
+    if (o.panel > 0) {
+        std::mt19937_64 gpan(o.seed * 1000003ULL + 9);
+        std::normal_distribution<double> ln(o.panelMu, o.panelSigma);
+        std::vector<std::pair<int, int>> cand;
+        for (int k = 0; k < o.panel; k++) {
+            int L = (int)std::lround(std::exp(ln(gpan)));
+            L = std::max(100, std::min(50000, std::min(L, o.len - 2)));
+            const int s0 = 1 + (int)(gpan() % (uint64_t)(o.len - L));
+            cand.push_back({s0, s0 + L - 1});
+        }
+        std::sort(cand.begin(), cand.end());
+        for (auto& r : cand)  // disjoint regions, in order
+            if (panel_regions.empty() || r.first > panel_regions.back().second + 300) panel_regions.push_back(r);
+        for (int h = 0; h < 3; h++) {
+            if (H[h].seq.empty()) continue;
+            std::vector<int32_t>& m = ref2hap[(size_t)h];
+            m.assign((size_t)o.len + 1, (int32_t)H[h].seq.size());
+            for (size_t i = 0; i < H[h].rpos.size(); i++)
+                if (H[h].rpos[i] >= 0 && m[(size_t)H[h].rpos[i]] == (int32_t)H[h].seq.size()) m[(size_t)H[h].rpos[i]] = (int32_t)i;
+            for (int p = o.len - 1; p >= 0; p--) m[(size_t)p] = std::min(m[(size_t)p], m[(size_t)p + 1]);
+        }
+        std::ofstream bed(o.out + "/panel.bed");
+        int k = 0;
+        for (auto& r : panel_regions) bed << "chrS\t" << r.first << "\t" << r.second << "\tamp" << k++ << "\n";
+    }
     auto gen_reads = [&](double depth, bool somatic, std::mt19937_64& gread, std::mt19937_64& gerr) {
     long npairs = (long)(depth * o.len / (2.0 * o.readLen));
     std::vector<Rec> recs;
     recs.reserve((size_t)npairs * 2);
     std::normal_distribution<double> ins(o.insMu, o.insSd), qn(35, 5);
-    for (long f = 0; f < npairs; f++) {
-        double vaf = o.vafLo + (o.vafHi - o.vafLo) * U(gread);
-        int h = (gread() & 1) ? 1 : 0;
-        if (somatic && U(gread) < vaf) h = 2;
-        const Hap& hp = H[h];
-        int hl = (int)hp.seq.size();
-        int isz = (int)std::lround(ins(gread));
-        if (isz < o.readLen) isz = o.readLen;
-        if (isz > hl) isz = hl;
-        int a = std::uniform_int_distribution<int>(0, hl - isz)(gread);
+    // one read pair of the fragment [a, a + isz) of haplotype hp
+    auto emit_pair = [&](const Hap& hp, int a, int isz) {
         Rec r1, r2;
         bool ok1 = make_record(ref, hp, a, a + o.readLen, o.longDels > 0, longDelMask, r1);
         bool ok2 = make_record(ref, hp, a + isz - o.readLen, a + isz, o.longDels > 0, longDelMask, r2);
-        if (!ok1 || !ok2) continue;
+        if (!ok1 || !ok2) return;
         bool firstIsFwd = (gread() & 1) != 0;
         for (int k = 0; k < 2; k++) {
             Rec& r = k == 0 ? r1 : r2;
@@ -484,6 +522,38 @@ int main(int argc, char** argv) {
         }
         recs.push_back(std::move(r1));
         recs.push_back(std::move(r2));
+    };
+    if (o.panel > 0) {
+        // amplicons: fixed start and end per tile, every pair of the tile on one of them
+        for (const auto& reg : panel_regions) {
+            const double depth = o.panelDepthLo + (o.panelDepthHi - o.panelDepthLo) * U(gread);
+            for (int t = reg.first - 1; t < reg.second; t += o.ampLen * 3 / 4) {
+                const int t_end = std::min(t + o.ampLen, o.len);
+                if (t_end - t < o.readLen) break;
+                const long pairs = std::max(1L, (long)(depth * (t_end - t) / (2.0 * o.readLen)));
+                for (long f = 0; f < pairs; f++) {
+                    int h = (gread() & 1) ? 1 : 0;
+                    double vaf = o.vafLo + (o.vafHi - o.vafLo) * U(gread);
+                    if (somatic && U(gread) < vaf) h = 2;
+                    const Hap& hp = H[h];
+                    const int a = ref2hap[h][(size_t)t], b = ref2hap[h][(size_t)t_end];
+                    if (b - a >= o.readLen) emit_pair(hp, a, b - a);
+                }
+            }
+        }
+    } else {
+        for (long f = 0; f < npairs; f++) {
+            double vaf = o.vafLo + (o.vafHi - o.vafLo) * U(gread);
+            int h = (gread() & 1) ? 1 : 0;
+            if (somatic && U(gread) < vaf) h = 2;
+            const Hap& hp = H[h];
+            int hl = (int)hp.seq.size();
+            int isz = (int)std::lround(ins(gread));
+            if (isz < o.readLen) isz = o.readLen;
+            if (isz > hl) isz = hl;
+            int a = std::uniform_int_distribution<int>(0, hl - isz)(gread);
+            emit_pair(hp, a, isz);
+        }
     }
     std::stable_sort(recs.begin(), recs.end(), [](const Rec& a, const Rec& b) { return a.pos0 < b.pos0; });
     if (o.dupRate > 0 || o.singleEndFrac > 0 || o.unmappedFrac > 0) {
