@@ -78,6 +78,7 @@ def main(argv=None):
     ap.add_argument("-g", type=int, default=0)
     ap.add_argument("-F", default="0x504", help="flag filter (hex)")
     ap.add_argument("-t", action="store_true", help="duplicate filter")
+    ap.add_argument("-p", action="store_true", help="pileup mode: print every covered position")
     ap.add_argument("-3", dest="move3", action="store_true")
     ap.add_argument("-u", action="store_true")
     ap.add_argument("--UN", action="store_true")
@@ -95,7 +96,7 @@ def main(argv=None):
     from .engine import Engine
 
     cfg = {field: getattr(a, field) for _, field, _ in OPTIONS if getattr(a, field) is not None}
-    cfg.update(samfilter=int(a.F, 16), removeDuplicatedReads=int(a.t), moveIndelsTo3=int(a.move3),
+    cfg.update(samfilter=int(a.F, 16), removeDuplicatedReads=int(a.t), moveIndelsTo3=int(a.move3), doPileup=int(a.p),
                uniqueModeAlignmentEnabled=int(a.u), uniqueModeSecondInPairEnabled=int(a.UN), fisher=int(a.fisher))
     extend = cfg.get("numberNucleotideToExtend", 0)
     if a.R:

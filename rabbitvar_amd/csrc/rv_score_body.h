@@ -412,6 +412,7 @@ __device__ inline void emit_ref(const ScoreIn& S, int p, int kind, const VarX* x
     o.nfields = 0;
     o.fisher_p = o.fisher_ratio = 0;
     o.vartype = VT_EMPTY;
+    o.msi = 0;
     o.start = o.end = 0;
     o.tpc = o.poscov = o.rfc = o.rrc = o.fwd = o.rev = o.shift3 = o.msint = o.hicnt = o.hicov = o.pstd = o.qstd = 0;
     o.freq = o.pmean = o.qmean = o.mq = o.hlr = o.hifreq = o.extrafreq = o.msi = o.nm = 0;
@@ -466,6 +467,12 @@ __device__ inline void emit_ref(const ScoreIn& S, int p, int kind, const VarX* x
         }
     } else {
         bias[nb++] = '0';
+    }
+    if (S.cfg.fisher) {
+        // the simple printer's Fisher columns (a pileup line of a reference-only position)
+        o.fisher_p = fisher_two_sided(o.rfc, o.rrc, o.fwd, o.rev, nullptr, nullptr);
+        const double ad = (double)(o.rfc * o.rev), bc = (double)(o.rrc * o.fwd);
+        if (bc != 0 && ad != 0) o.fisher_ratio = ad > bc ? (ad / bc) : (bc / ad);
     }
     const char* pc[7] = {allele, allele, gt, bias, lfc ? lfc : allele, allele, d};
     const int pn[7] = {na, na, ng, nb, lfc ? lfn : 0, 0, dl};
@@ -548,9 +555,15 @@ __device__ void score_position(const ScoreIn& S, int p, bool dense_only, int slo
         }
     }
     if (nv == 0) {
-        // no allele has reads: simple mode creates no Vars entry; with a second sample ToVarsBuilder
-        // still creates one, holding an empty reference variant
-        if (S.somatic == SCORE_SIMPLE) return;
+        // no allele has reads: simple mode creates no Vars entry; with a second sample, or in
+        // pileup mode (-p), ToVarsBuilder still creates one, holding an empty reference variant
+        // (ToVarsBuilder.cpp:1098-1100), which -p prints (simpleMode.cpp:160-170)
+        if (S.somatic == SCORE_SIMPLE) {
+            if (!c.doPileup) return;
+            S.status[slot] = 1;
+            if (!(p < S.region_start || p > S.region_end)) emit_ref(S, p, OUT_REF_ONLY, nullptr, nullptr, 0, 0, 0, 0, 0);
+            return;
+        }
         S.status[slot] = 3;
         if (S.somatic == SCORE_SOMATIC_REF && S.want[slot] && !(p < S.region_start || p > S.region_end))
             emit_ref(S, p, OUT_REF_ONLY, nullptr, nullptr, 0, 0, 0, 0, 0);
@@ -572,6 +585,14 @@ __device__ void score_position(const ScoreIn& S, int p, bool dense_only, int slo
             const char rb = rf.has(p) ? rf.at(p) : 0;
             emit_ref(S, p, OUT_REF_ONLY, &rx, var[refi].al->d, var[refi].al->len, tpc, rx.fwd, rx.rev, rb);
         }
+        return;
+    }
+    if (S.somatic == SCORE_SIMPLE && c.doPileup && nonref == 0) {
+        // -p prints a reference-only position's reference variant in its else-branch form
+        // (ToVarsBuilder.cpp:1074-1097; simpleMode.cpp:160-170)
+        const VarX rx = expand(var[refi], c);
+        const char rb = rf.has(p) ? rf.at(p) : 0;
+        emit_ref(S, p, OUT_REF_ONLY, &rx, var[refi].al->d, var[refi].al->len, tpc, rx.fwd, rx.rev, rb);
         return;
     }
     if (S.somatic == SCORE_SOMATIC_VARS) {
@@ -1103,7 +1124,6 @@ __device__ void score_position(const ScoreIn& S, int p, bool dense_only, int slo
             if (!emit_out(S, o, pc, pn)) return;
         }
     }
-    // positions with only a reference allele print only with -p (pileup mode); not modelled here
 }
 
 }  // namespace rv

@@ -201,6 +201,11 @@ int main(int argc, char** argv) {
     cfg.fisher = fisher;
     if (const char* e = getenv("RV_SIM_DEDUP")) cfg.removeDuplicatedReads = atoi(e);         // -t
     if (const char* e = getenv("RV_SIM_SAMFILTER")) cfg.samfilter = (int)strtol(e, nullptr, 16);  // -F
+    if (getenv("RV_SIM_PILEUP")) {  // -p (the reference then also sets -f -1 and -r 0)
+        cfg.doPileup = 1;
+        cfg.freq = -1;
+        cfg.minr = 0;
+    }
     const int chrLen = (int)seq.size();
     Window W;
     W.region_start = rs;

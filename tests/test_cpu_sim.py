@@ -60,6 +60,21 @@ def test_engine_stages_duplicate_filter(sim, dataset, samfilter):
     assert got == want
 
 
+@pytest.mark.parametrize("case,start,end", [("c1", 1, 20000), ("edge", 29001, 31000)])
+def test_engine_stages_pileup_mode(sim, dataset, case, start, end):
+    """-p: every covered position prints, reference-only ones in their reference-variant form."""
+    d = dataset(case)
+    want, _ = run_oracle(d, f"chrS:{start}-{end}", ["--fisher", "-p"], out=f"sim_pileup_{start}.txt")
+    out = os.path.join(d, f"cpu_sim_pileup_{start}.txt")
+    env = dict(os.environ, RV_SIM_PILEUP="1")
+    subprocess.run([sim, os.path.join(d, "ref.fa"), os.path.join(d, "reads.rvb"), str(start), str(end), "1", "1", out],
+                   check=True, env=env, capture_output=True)
+    with open(out) as f:
+        got = f.read()
+    assert len(want.splitlines()) > 0.9 * (end - start + 1)
+    assert got == want
+
+
 @pytest.mark.parametrize("start,end", [(1, 1), (5000, 5150), (59000, 60000)])
 def test_engine_stages_subregions(sim, dataset, start, end):
     d = dataset("edge")

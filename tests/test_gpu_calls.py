@@ -79,3 +79,18 @@ def test_duplicate_filter_matches_oracle(dataset, samfilter):
     eng.close()
     assert len(want.splitlines()) > 500
     assert got == want
+
+
+@pytest.mark.parametrize("case,start,end", [("c1", 1, 20000), ("edge", 29001, 31000)])
+def test_pileup_mode_matches_oracle(dataset, case, start, end):
+    """-p: every covered position prints (reference-only ones as their reference variant)."""
+    from rabbitvar_amd.engine import Engine
+
+    d = dataset(case)
+    name, seq, reads = _inputs(d)
+    want, _ = run_oracle(d, f"{name}:{start}-{end}", ["--fisher", "-p"], out=f"pileup_{start}.txt")
+    eng = Engine(sample="S1", fisher=1, doPileup=1)
+    got = eng.run_region(name, start, end, reads.select(start, end), name, seq)
+    eng.close()
+    assert len(want.splitlines()) > 0.9 * (end - start + 1)
+    assert got == want
