@@ -21,31 +21,9 @@ OPTIONS = [  # (flag, config field, type)
 ]
 
 
-def parse_region(text, extend):
-    """-R chr:start-end[:gene] (commas allowed), widened by -x."""
-    chrom, _, rest = text.partition(":")
-    rng, _, gene = rest.partition(":")
-    a, _, b = rng.partition("-")
-    start = int(a.replace(",", ""))
-    end = int(b.replace(",", "")) if b else start
-    start -= extend
-    end += extend
-    if start > end:
-        start = end
-    return chrom, start, end, gene or chrom
-
-
-def bed_regions(path, c, s, e, g):
-    out = []
-    with open(path) as f:
-        for line in f:
-            line = line.rstrip("\n")
-            if not line or line.startswith("#") or line.startswith("track"):
-                continue
-            cols = line.split("\t")
-            gene = cols[g] if 0 <= g < len(cols) else cols[c]
-            out.append((cols[c], int(cols[s]), int(cols[e]), gene))
-    return out
+def fasta_names(path):
+    with open(path, "rb") as f:
+        return {ln[1:].split()[0].decode() for ln in f if ln.startswith(b">")}
 
 
 class Reads:
@@ -72,10 +50,15 @@ def main(argv=None):
     ap.add_argument("-N", default="", help="sample name ('T|N' in somatic mode)")
     ap.add_argument("-R", help="region chr:start-end[:gene]")
     ap.add_argument("-i", "--bed", help="BED file of regions")
-    ap.add_argument("-c", type=int, default=1)
-    ap.add_argument("-S", type=int, default=2)
-    ap.add_argument("-E", type=int, default=3)
-    ap.add_argument("-g", type=int, default=0)
+    ap.add_argument("-c", type=int, help="BED column of the chromosome (1-based)")
+    ap.add_argument("-S", type=int, help="BED column of the region start")
+    ap.add_argument("-E", type=int, help="BED column of the region end")
+    ap.add_argument("-s", type=int, help="BED column of the segment starts")
+    ap.add_argument("-e", type=int, help="BED column of the segment ends")
+    ap.add_argument("-g", type=int, help="BED column of the gene name")
+    ap.add_argument("-d", default="\t", help="BED delimiter")
+    ap.add_argument("-z", action="store_true", help="zero-based coordinates")
+    ap.add_argument("--auto_resize", action="store_true", help="split BED regions longer than 10 kb")
     ap.add_argument("-F", default="0x504", help="flag filter (hex)")
     ap.add_argument("-t", action="store_true", help="duplicate filter")
     ap.add_argument("-p", action="store_true", help="pileup mode: print every covered position")
@@ -98,15 +81,26 @@ def main(argv=None):
     cfg = {field: getattr(a, field) for _, field, _ in OPTIONS if getattr(a, field) is not None}
     cfg.update(samfilter=int(a.F, 16), removeDuplicatedReads=int(a.t), moveIndelsTo3=int(a.move3), doPileup=int(a.p),
                uniqueModeAlignmentEnabled=int(a.u), uniqueModeSecondInPairEnabled=int(a.UN), fisher=int(a.fisher))
+    from . import regions as rg
+
     extend = cfg.get("numberNucleotideToExtend", 0)
-    if a.R:
-        regions = [parse_region(a.R, extend)]
-    elif a.bed:
-        regions = bed_regions(a.bed, a.c - 1, a.S - 1, a.E - 1, a.g - 1)
-    else:
-        ap.error("give -R or -i")
     paths = a.b.split("|")
     samples = [Reads(p, a.th) for p in paths]
+    names = fasta_names(a.G)
+    if samples[0].bam is not None:
+        names |= {n for n, _ in samples[0].bam.targets()}
+    if a.R:
+        segs = rg.from_option(a.R, names, extend, a.z)
+    elif a.bed:
+        with open(a.bed) as f:
+            segs = rg.from_bed(f.readlines(), names, {"c": a.c, "S": a.S, "E": a.E, "s": a.s, "e": a.e, "g": a.g},
+                               extend, a.z, a.d)
+        if a.auto_resize:
+            segs = rg.auto_resize(segs)
+    else:
+        ap.error("give -R or -i")
+    # a region that ends before it starts reads no records and prints nothing
+    regions = [r for r in rg.flatten(segs) if r[1] <= r[2]]
     contigs = {}
     for chrom, _, _, _ in regions:
         if chrom not in contigs:
