@@ -3,21 +3,78 @@
 #include "rv_format.h"
 
 #include <cstdio>
+#include <cstring>
 
 namespace rv {
 
 namespace {
 
-void put_f(std::string& o, double v) {
-    char b[512];
-    int n = snprintf(b, sizeof b, "%f", v);
-    o.append(b, (size_t)n);
+void put_u(std::string& o, unsigned long long v) {
+    char b[24];
+    int n = 0;
+    do {
+        b[n++] = (char)('0' + (int)(v % 10));
+        v /= 10;
+    } while (v);
+    while (n) o += b[--n];
 }
 void put_i(std::string& o, long long v) {
-    char b[32];
-    int n = snprintf(b, sizeof b, "%lld", v);
-    o.append(b, (size_t)n);
+    if (v < 0) {
+        o += '-';
+        put_u(o, (unsigned long long)(-(v + 1)) + 1ull);
+    } else {
+        put_u(o, (unsigned long long)v);
+    }
 }
+}  // namespace
+
+// "%f" (six decimals) of a finite double, exactly as printf prints it: the binary value times 10^6
+// rounded to the nearest integer, ties to even, computed in 128-bit integers from the value's
+// significand and exponent.  Values of 2^43 and above, and non-finite ones, go through snprintf.
+void put_f(std::string& o, double v) {
+    uint64_t bits;
+    std::memcpy(&bits, &v, 8);
+    const bool neg = (bits >> 63) != 0;
+    const int bexp = (int)((bits >> 52) & 0x7ff);
+    uint64_t m = bits & ((1ull << 52) - 1);
+    if (bexp == 0x7ff || (bexp != 0 && bexp - 1075 >= -10)) {  // non-finite, or |v| >= 2^43
+        char b[512];
+        const int n = snprintf(b, sizeof b, "%f", v);
+        o.append(b, (size_t)n);
+        return;
+    }
+    int e;
+    if (bexp == 0) {
+        e = -1074;  // subnormal
+    } else {
+        m |= 1ull << 52;
+        e = bexp - 1075;
+    }
+    // |v| = m * 2^e with e < -10 here
+    const unsigned __int128 x = (unsigned __int128)m * 1000000u;
+    const int k = -e;
+    unsigned long long q;
+    if (k >= 100) {
+        q = 0;  // x < 2^73 is below half of 2^k
+    } else {
+        q = (unsigned long long)(x >> k);
+        const unsigned __int128 r = x & ((((unsigned __int128)1) << k) - 1);
+        const unsigned __int128 half = ((unsigned __int128)1) << (k - 1);
+        if (r > half || (r == half && (q & 1))) q++;
+    }
+    if (neg) o += '-';
+    put_u(o, q / 1000000u);
+    o += '.';
+    char f[6];
+    unsigned long long fr = q % 1000000u;
+    for (int i = 5; i >= 0; i--) {
+        f[i] = (char)('0' + (int)(fr % 10));
+        fr /= 10;
+    }
+    o.append(f, 6);
+}
+
+namespace {
 void put_s0(std::string& o, const char* p, int n) {  // empty strings print as "0"
     if (n == 0)
         o += '0';

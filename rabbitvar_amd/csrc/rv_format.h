@@ -19,6 +19,9 @@ struct LineContext {
 // the printed variant type (Variant.h:75-97) of a scored record
 std::string vartype_name(const OutRec& o, const char* arena);
 
+// appends v as printf's "%f" (std::to_string of a double)
+void put_f(std::string& out, double v);
+
 // appends one output line for a scored record
 void format_line(std::string& out, const OutRec& o, const char* arena, const LineContext& ctx);
 

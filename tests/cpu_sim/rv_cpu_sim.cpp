@@ -1,3 +1,4 @@
+#include <cmath>
 #include <cstdlib>
 // Test harness only: runs the engine's device stages sequentially on the CPU (prep, walk, anchor,
 // then the host reduction and realigner, then one score_position call per position exactly as the
@@ -111,6 +112,57 @@ int main(int argc, char** argv) {
             else fprintf(stderr, "differs: %s %s: (%d %d %d) vs (%d %d %d)\n", s5.c_str(), s3.c_str(), a5, a3, as, c5, c3, cs);
         }
         printf("%d %d %d\n", trials, matched, equal);
+        return 0;
+    }
+    if (argc >= 4 && std::string(argv[1]) == "--fmt-check") {
+        // put_f against snprintf("%f") on random doubles of every magnitude, exact ties (k / 2^j),
+        // signed zeros, subnormals and values just around the rounding points; prints mismatches
+        const long n = atol(argv[2]);
+        unsigned long long s = (unsigned long long)atoll(argv[3]) | 1ull;
+        auto rnd = [&]() {
+            s ^= s << 13;
+            s ^= s >> 7;
+            s ^= s << 17;
+            return s;
+        };
+        long bad = 0;
+        auto check = [&](double v) {
+            std::string a;
+            put_f(a, v);
+            char b[512];
+            const int k = snprintf(b, sizeof b, "%f", v);
+            if (a != std::string(b, (size_t)k)) {
+                if (bad < 5) fprintf(stderr, "mismatch %.17g: %s vs %s\n", v, a.c_str(), b);
+                bad++;
+            }
+        };
+        for (double v : {0.0, -0.0, 1.0, -1.0, 0.5, 0.0000005, 0.0000015, 0.0000025, 4.9406564584124654e-324,
+                         2.2250738585072014e-308, 8796093022207.0, 8796093022208.0, 1e15, -1e-9})
+            check(v);
+        for (long i = 0; i < n; i++) {
+            const uint64_t r = rnd();
+            double v;
+            switch (i % 4) {
+                case 0: {  // any finite bit pattern
+                    uint64_t b = r;
+                    std::memcpy(&v, &b, 8);
+                    if (!std::isfinite(v)) continue;
+                    break;
+                }
+                case 1:  // k / 2^j: exact ties at the sixth decimal
+                    v = (double)(r % 100000000) / (double)(1ull << (1 + rnd() % 40));
+                    break;
+                case 2:  // counts over counts, as the printed fields are
+                    v = (double)(r % 100000) / (double)(1 + rnd() % 5000);
+                    break;
+                default:  // next to a printed rounding point
+                    v = std::nextafter((double)(r % 1000000000) / 1e6 + 5e-7, (rnd() & 1) ? 1e300 : -1e300);
+                    break;
+            }
+            if (rnd() & 1) v = -v;
+            check(v);
+        }
+        printf("%ld\n", bad);
         return 0;
     }
     if (argc >= 4 && std::string(argv[1]) == "--order-check") {

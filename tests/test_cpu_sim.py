@@ -148,3 +148,10 @@ def test_find35match_against_nested_loops(sim):
     r = subprocess.run([sim, "--find35-check", "6000", "11"], capture_output=True, text=True, check=True)
     trials, matched, equal = (int(x) for x in r.stdout.split())
     assert equal == trials and matched > 100
+
+
+def test_fixed_point_formatting_matches_printf(sim):
+    """The lines' "%f" fields come from an integer formatter (rv_format.cpp put_f); it must print
+    exactly what printf prints, ties to even included."""
+    r = subprocess.run([sim, "--fmt-check", "400000", "7"], capture_output=True, text=True, check=True)
+    assert r.stdout.strip() == "0", r.stderr
