@@ -351,6 +351,13 @@ void reduce_clips(const ClipRec* cl, size_t n, const char* clip_arena, double go
             }
             add(sc.v, r.dir != 0, r.cel, r.sumq / (double)r.nhq, r.mapq, r.nm);
         }
+        // the clusters are complete: their consensus, here on this range's thread rather than
+        // one by one in the realigner
+        for (auto* mp : {&part5[t], &part3[t]})
+            for (auto& e : *mp) {
+                e.second.pre = clip_consensus(e.second, e.second.pre_used);
+                e.second.pre_valid = true;
+            }
     };
     {
         std::vector<std::thread> pool;
@@ -364,7 +371,107 @@ void reduce_clips(const ClipRec* cl, size_t n, const char* clip_arena, double go
             while (!pr->empty()) dst.insert(dst.end(), pr->extract(pr->begin()));
         }
     }
+    for (int s5 = 0; s5 < 2; s5++) {
+        if (side >= 0 && side != (s5 ? 5 : 3)) continue;
+        auto& m = s5 ? P.sc5 : P.sc3;
+        auto& ord = s5 ? P.sc5_order : P.sc3_order;
+        clip_order(m, ord);
+        (s5 ? P.sc5_of : P.sc3_of) = &m;
+    }
 }
+
+void clip_order(std::map<int, HSclip>& m, std::vector<std::pair<int, HSclip*>>& ord) {
+    ord.clear();
+    ord.reserve(m.size());
+    for (auto& e : m) ord.push_back({e.first, &e.second});
+    std::sort(ord.begin(), ord.end(), [](const std::pair<int, HSclip*>& a, const std::pair<int, HSclip*>& b) {
+        if (a.second->v.cnt != b.second->v.cnt) return a.second->v.cnt > b.second->v.cnt;
+        return a.first < b.first;
+    });
+}
+
+bool low_complexity(const std::string& s) {
+    int len = (int)s.size();
+    if (len == 0) return true;
+    int ntcnt = 0;
+    const char order[4] = {'A', 'T', 'G', 'C'};
+    for (char b : order) {
+        int n = (int)std::count(s.begin(), s.end(), b);
+        if (n > 0) ntcnt++;
+        if (n / (double)len > 0.75) return true;
+    }
+    return ntcnt < 3;
+}
+
+// findconseq's computation (include/VariationUtils.h:403-509) without its side effects: the
+// caller stores the sequence and, when mark_used is set, marks the cluster used
+std::string clip_consensus(const HSclip& sc, bool& mark_used) {
+    mark_used = false;
+    int total = 0, match = 0;
+    std::string seqq;
+    bool flag = false;
+    for (size_t idx = 0; idx < sc.col.size(); idx++) {
+        const ClipColumn& col = sc.col[idx];
+        char firsts[8], order[8];
+        std::vector<char> big;
+        int nk = (int)col.nt.size(), no;
+        const char* ord;
+        if (nk <= 7) {
+            for (int k = 0; k < nk; k++) firsts[k] = col.nt[(size_t)k].first;
+            no = rh_char_order_small(firsts, nk, order);
+            ord = order;
+        } else {
+            for (auto& pr : col.nt) big.push_back(pr.first);
+            big = rh_char_iteration_order(big);
+            no = (int)big.size();
+            ord = big.data();
+        }
+        int maxCount = 0;
+        double maxQuality = 0;
+        char chosen = 0;
+        int tt = 0;
+        for (int oi = 0; oi < no; oi++) {
+            const char base = ord[oi];
+            int count = 0;
+            for (auto& pr : col.nt)
+                if (pr.first == base) count = pr.second;
+            tt += count;
+            const HVar* sv = col.seq_find(base);
+            const bool has = sv != nullptr;
+            const double bq = has ? sv->qmean : 0;
+            if (count > maxCount || (has && bq > maxQuality)) {
+                maxCount = count;
+                chosen = base;
+                maxQuality = has ? bq : 0.0;  // seq[i][base] would be created empty
+            }
+        }
+        if (idx == 3 && sc.col.size() >= 6 && tt / (double)sc.v.cnt < 0.2 && tt <= 2) break;
+        if ((tt - maxCount > 2 || maxCount <= tt - maxCount) && maxCount / (double)tt < 0.8) {
+            if (flag) break;
+            flag = true;
+        }
+        total += tt;
+        match += maxCount;
+        if (chosen != 0) seqq += chosen;
+    }
+    std::string SEQ;
+    size_t ntSize = sc.col.size();
+    size_t sl = seqq.size();
+    if (total != 0 && match / (double)total > 0.9 && sl / 1.5 > (double)(ntSize - sl) &&
+        (sl / (double)ntSize > 0.8 || ntSize - sl < 10 || sl > 25)) {
+        SEQ = seqq;
+    } else {
+        SEQ = " ";
+    }
+    if (!SEQ.empty() && SEQ.size() > 12) {
+        bool mm1 = SEQ.size() >= 8 && SEQ.compare(1, 7, "AAAAAAA") == 0;
+        bool mm2 = SEQ.size() >= 8 && SEQ.compare(1, 7, "TTTTTTT") == 0;
+        if (mm1 || mm2) mark_used = true;
+        if (low_complexity(SEQ)) mark_used = true;
+    }
+    return SEQ;
+}
+
 
 std::string dump_pileup(const HostPileup& P) {
     std::string o;

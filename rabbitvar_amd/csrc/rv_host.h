@@ -56,7 +56,15 @@ struct HSclip {
     std::vector<ClipColumn> col;
     std::string sequence;
     bool used = false;
+    // the consensus as the reduction computed it (findconseq takes it over on its first call)
+    std::string pre;
+    bool pre_valid = false, pre_used = false;
 };
+
+// VariationUtils.h:367-394 islowcomplexseq
+bool low_complexity(const std::string& s);
+// findconseq's computation without its side effects (see rv_realign.h findconseq)
+std::string clip_consensus(const HSclip& sc, bool& mark_used);
 
 struct HostPileup {
     int dense_lo = 0, dense_n = 0;
@@ -64,6 +72,11 @@ struct HostPileup {
     std::map<int, std::map<std::string, HVar>> nd;        // other nonInsertionVariants entries
     std::map<int, std::map<std::string, HVar>> ins;       // insertionVariants
     std::map<int, HSclip> sc5, sc3;
+    // the clusters in realignlgins30's order (count descending, position ascending), built by the
+    // reduction; valid while sc5_of / sc3_of name these maps (a copied pileup rebuilds them)
+    std::vector<std::pair<int, HSclip*>> sc5_order, sc3_order;
+    const void* sc5_of = nullptr;
+    const void* sc3_of = nullptr;
     std::map<int, std::map<std::string, int>> delCount, insCount, mnp;
     int maxReadLength = 0;
 };
@@ -72,6 +85,9 @@ struct HostPileup {
 void reduce_events(const Event* ev, size_t n, const char* arena, double goodq, HostPileup& P);
 // side = 5 or 3: only that side's records (its map is written, the other is untouched); -1: both
 void reduce_clips(const ClipRec* cl, size_t n, const char* clip_arena, double goodq, HostPileup& P, int side = -1);
+
+// the clusters of a clip map in count-descending, position-ascending order
+void clip_order(std::map<int, HSclip>& m, std::vector<std::pair<int, HSclip*>>& ord);
 
 // HVar view of a dense row's base (pileup-stage values)
 HVar dense_to_hvar(const DenseBase& d);

@@ -411,85 +411,21 @@ void adj_ref_factor(HVar* refv, double f) {
     correct_cnt(refv);
 }
 
-bool low_complexity(const std::string& s) {
-    int len = (int)s.size();
-    if (len == 0) return true;
-    int ntcnt = 0;
-    const char order[4] = {'A', 'T', 'G', 'C'};
-    for (char b : order) {
-        int n = (int)std::count(s.begin(), s.end(), b);
-        if (n > 0) ntcnt++;
-        if (n / (double)len > 0.75) return true;
-    }
-    return ntcnt < 3;
-}
 
 }  // namespace
 
 std::string findconseq(HSclip& sc) {
     if (!sc.sequence.empty()) return sc.sequence;
-    int total = 0, match = 0;
-    std::string seqq;
-    bool flag = false;
-    for (size_t idx = 0; idx < sc.col.size(); idx++) {
-        ClipColumn& col = sc.col[idx];
-        char firsts[8], order[8];
-        std::vector<char> big;
-        int nk = (int)col.nt.size(), no;
-        const char* ord;
-        if (nk <= 7) {
-            for (int k = 0; k < nk; k++) firsts[k] = col.nt[(size_t)k].first;
-            no = rh_char_order_small(firsts, nk, order);
-            ord = order;
-        } else {
-            for (auto& pr : col.nt) big.push_back(pr.first);
-            big = rh_char_iteration_order(big);
-            no = (int)big.size();
-            ord = big.data();
-        }
-        int maxCount = 0;
-        double maxQuality = 0;
-        char chosen = 0;
-        int tt = 0;
-        for (int oi = 0; oi < no; oi++) {
-            const char base = ord[oi];
-            int count = 0;
-            for (auto& pr : col.nt)
-                if (pr.first == base) count = pr.second;
-            tt += count;
-            const HVar* sv = col.seq_find(base);
-            const bool has = sv != nullptr;
-            const double bq = has ? sv->qmean : 0;
-            if (count > maxCount || (has && bq > maxQuality)) {
-                maxCount = count;
-                chosen = base;
-                maxQuality = has ? bq : col.seq_at(base).qmean;
-            }
-        }
-        if (idx == 3 && sc.col.size() >= 6 && tt / (double)sc.v.cnt < 0.2 && tt <= 2) break;
-        if ((tt - maxCount > 2 || maxCount <= tt - maxCount) && maxCount / (double)tt < 0.8) {
-            if (flag) break;
-            flag = true;
-        }
-        total += tt;
-        match += maxCount;
-        if (chosen != 0) seqq += chosen;
-    }
+    bool mark = false;
     std::string SEQ;
-    size_t ntSize = sc.col.size();
-    size_t sl = seqq.size();
-    if (total != 0 && match / (double)total > 0.9 && sl / 1.5 > (double)(ntSize - sl) &&
-        (sl / (double)ntSize > 0.8 || ntSize - sl < 10 || sl > 25)) {
-        SEQ = seqq;
+    if (sc.pre_valid) {  // computed when the clips were reduced
+        SEQ.swap(sc.pre);
+        sc.pre_valid = false;
+        mark = sc.pre_used;
     } else {
-        SEQ = " ";
+        SEQ = clip_consensus(sc, mark);
     }
-    if (!SEQ.empty() && SEQ.size() > 12) {
-        bool mm1 = SEQ.size() >= 8 && SEQ.compare(1, 7, "AAAAAAA") == 0;
-        bool mm2 = SEQ.size() >= 8 && SEQ.compare(1, 7, "TTTTTTT") == 0;
-        if (mm1 || mm2) sc.used = true;
-        if (low_complexity(SEQ)) sc.used = true;
-    }
+    if (mark) sc.used = true;
     sc.sequence = SEQ;
     return SEQ;
 }
@@ -775,6 +711,15 @@ struct Realigner {
     ScreenPlan* plan;
 
     int maxlen() const { return S.P.maxReadLength; }
+
+    // index of a 5' cluster among the screen's cluster candidates (those are listed by position)
+    int lgins_index(int position) const {
+        const ScreenCand* b = plan->cand.data() + plan->off[SC_LGINS];
+        const ScreenCand* e = plan->cand.data() + plan->off[SC_LGINS + 1];
+        const ScreenCand* it =
+            std::lower_bound(b, e, position, [](const ScreenCand& x, int v) { return x.position < v; });
+        return (it != e && it->position == position) ? (int)(it - b) : -1;
+    }
 
     // ---- screen: candidate i of a kind is skipped when the plan says so; with labels on, a
     // visit's changes are recorded when it ends (every exit of the loop body)
@@ -1235,25 +1180,28 @@ struct Realigner {
             int li;  // index among the screen's cluster candidates (P.sc5 order)
         };
         std::vector<SP> t5, t3;
-        {
-            int li = -1;
-            for (auto& e : S.P.sc5) {
-                li++;
-                if (e.first < rstart - 5000 || e.first > rend + 5000) continue;
-                // screened mode: a skipped cluster is dropped here; the kept ones keep their order
-                if (skip(SC_LGINS, plan ? li : -1)) continue;
-                t5.push_back({e.first, &e.second, e.second.v.cnt, plan ? li : -1});
-            }
+        // the clusters in COMP3 order (count descending, position ascending), as the reduction
+        // sorted them; a pileup copied since then sorts again
+        if (S.P.sc5_of != &S.P.sc5) {
+            clip_order(S.P.sc5, S.P.sc5_order);
+            S.P.sc5_of = &S.P.sc5;
+        }
+        if (S.P.sc3_of != &S.P.sc3) {
+            clip_order(S.P.sc3, S.P.sc3_order);
+            S.P.sc3_of = &S.P.sc3;
+        }
+        t5.reserve(S.P.sc5_order.size());
+        for (const auto& e : S.P.sc5_order) {
+            if (e.first < rstart - 5000 || e.first > rend + 5000) continue;
+            // screened mode: a skipped cluster is dropped here; the kept ones keep their order
+            const int li = plan ? lgins_index(e.first) : -1;
+            if (skip(SC_LGINS, li)) continue;
+            t5.push_back({e.first, e.second, e.second->v.cnt, li});
         }
         if (t5.empty()) return;  // the 3' clusters are only read from a 5' cluster's visit
-        for (auto& e : S.P.sc3)
-            if (!(e.first < rstart - 5000 || e.first > rend + 5000)) t3.push_back({e.first, &e.second, e.second.v.cnt, -1});
-        auto comp3 = [](const SP& a, const SP& b) {
-            if (b.count != a.count) return a.count > b.count;
-            return a.position < b.position;
-        };
-        std::sort(t5.begin(), t5.end(), comp3);
-        std::sort(t3.begin(), t3.end(), comp3);
+        t3.reserve(S.P.sc3_order.size());
+        for (const auto& e : S.P.sc3_order)
+            if (!(e.first < rstart - 5000 || e.first > rend + 5000)) t3.push_back({e.first, e.second, e.second->v.cnt, -1});
         st.candidates += (int64_t)(t5.size() + t3.size());
         // The reference scans every 3' cluster for every 5' cluster and skips those outside
         // [p5 - 2.5 * maxReadLength, p5 + maxReadLength - 10] before anything else; visiting only
