@@ -65,6 +65,13 @@ typedef struct rv_config {
     int removeDuplicatedReads;    /* -t */
     int crisprCuttingSite;        /* -J, 0 */
     const char* sample;  /* -N */
+    /* -Z (recordPreprocessor.cpp:133, Launcher.cpp:444-447): each record that passes -F draws the C
+     * library's rand() in iterator order and is dropped when the draw is <= RAND_MAX * downsampling.
+     * The library calls the process's own rand(), as the reference does, so the draws continue one
+     * sequence across calls (the first sample's records before the second's); the pool makes its
+     * draws in region order, which is the reference's order with one thread. */
+    int isDownsampling;
+    double downsampling;
 } rv_config;
 
 /* Records of one region as the region iterator returns them (unfiltered, in order).  Bases are

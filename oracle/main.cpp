@@ -229,6 +229,10 @@ int main(int argc, char** argv) {
             c.doPileup = true;
         } else if (a == "-3") c.moveIndelsTo3 = true;
         else if (a == "-t") c.removeDuplicatedReads = true;
+        else if (a == "-Z") {
+            c.isDownsampling = true;
+            c.downsampling = std::stod(nxt());
+        }
         else if (a == "-Y") c.referenceExtension = std::stoi(nxt());
         else if (a == "-x") c.numberNucleotideToExtend = std::stoi(nxt());
         else if (a == "--out") out = nxt();

@@ -18,6 +18,7 @@
 // Parity unpinned: the soft-clip rescue and consensus tables, findOffset / appendSegments (complex
 // M-run descriptors) and the insertion-anchor bookkeeping have no independent check.
 #include <algorithm>
+#include <cstdlib>
 #include <regex>
 #include <stdexcept>
 
@@ -38,13 +39,19 @@ static int nt16_code(char c) {
 
 void preprocess_filter(const Config& c, const std::vector<const Read*>& all, std::vector<const Read*>& out,
                        Counters& ctr) {
-    // recordPreprocessor.cpp:121-176 (downsampling -Z is not modelled: it is unseeded rand())
+    // recordPreprocessor.cpp:121-176.  -Z (:133) draws the C library's rand() once per record that
+    // passed the flag filter; the reference never seeds it, so the draws continue one process-wide
+    // sequence (this process: one region, the first sample before the second).
     int firstMatchingPosition = 0;
     std::set<std::string> dups;
     for (const Read* rp : all) {
         const Read& r = *rp;
         ctr.reads_in++;
         if (r.flag & c.samfilter) {
+            ctr.reads_filtered++;
+            continue;
+        }
+        if (c.isDownsampling && rand() <= RAND_MAX * c.downsampling) {
             ctr.reads_filtered++;
             continue;
         }

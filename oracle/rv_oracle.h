@@ -63,6 +63,8 @@ struct Config {
     int numberNucleotideToExtend = 0;  // -x
     int mappingQuality = 0;          // -Q
     bool removeDuplicatedReads = false;  // -t
+    bool isDownsampling = false;     // -Z given
+    double downsampling = 0;         // -Z fraction
     int crisprCuttingSite = 0;       // -J
     bool hasBam2 = false;            // somatic (-b 'T|N')
     std::string sample;              // -N

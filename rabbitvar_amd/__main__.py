@@ -61,6 +61,9 @@ def main(argv=None):
     ap.add_argument("--auto_resize", action="store_true", help="split BED regions longer than 10 kb")
     ap.add_argument("-F", default="0x504", help="flag filter (hex)")
     ap.add_argument("-t", action="store_true", help="duplicate filter")
+    ap.add_argument("-Z", type=float, dest="downsample",
+                    help="downsampling fraction: each read passing -F is dropped when the C library's rand() "
+                         "draw is <= RAND_MAX * fraction (draws in region order)")
     ap.add_argument("-p", action="store_true", help="pileup mode: print every covered position")
     ap.add_argument("-3", dest="move3", action="store_true")
     ap.add_argument("-u", action="store_true")
@@ -81,6 +84,8 @@ def main(argv=None):
     cfg = {field: getattr(a, field) for _, field, _ in OPTIONS if getattr(a, field) is not None}
     cfg.update(samfilter=int(a.F, 16), removeDuplicatedReads=int(a.t), moveIndelsTo3=int(a.move3), doPileup=int(a.p),
                uniqueModeAlignmentEnabled=int(a.u), uniqueModeSecondInPairEnabled=int(a.UN), fisher=int(a.fisher))
+    if a.downsample is not None:
+        cfg.update(isDownsampling=1, downsampling=a.downsample)
     from . import regions as rg
 
     extend = cfg.get("numberNucleotideToExtend", 0)

@@ -29,7 +29,7 @@ class RvConfig(ctypes.Structure):
         ("uniqueModeSecondInPairEnabled", c_int), ("fisher", c_int), ("SVMINLEN", c_int),
         ("referenceExtension", c_int), ("numberNucleotideToExtend", c_int),
         ("mappingQuality", c_int), ("removeDuplicatedReads", c_int), ("crisprCuttingSite", c_int),
-        ("sample", c_char_p),
+        ("sample", c_char_p), ("isDownsampling", c_int), ("downsampling", c_double),
     ]
 
 

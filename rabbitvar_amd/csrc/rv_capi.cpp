@@ -75,6 +75,8 @@ void rv_config_default(rv_config* c) {
     c->removeDuplicatedReads = 0;
     c->crisprCuttingSite = 0;
     c->sample = "";
+    c->isDownsampling = 0;
+    c->downsampling = 0;
 }
 
 rv_status rv_open(const rv_config* cfg, int device, rv_engine** out) {
@@ -182,6 +184,7 @@ void rv_free(void* p) { std::free(p); }
 // ------------------------------------------------------------------ engine pool (region lists)
 #include <atomic>
 #include <memory>
+#include <condition_variable>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -223,6 +226,11 @@ rv_status rv_pool_run(rv_pool* p, const rv_region* regions, size_t n, rv_bam* fi
     std::atomic<bool> stop{false};
     std::mutex mu;
     rv_status first_err = RV_OK;
+    // -Z draws: made in region order (a turn per region), which is the reference's order with one
+    // thread; the masks are queued on the engine that runs the region
+    const double ds = p->engines.empty() ? -1 : rv::engine_of(p->engines[0])->downsampling;
+    size_t draw_turn = 0;
+    std::condition_variable draw_cv;
     auto worker = [&](rv_engine* eng) {
         rv::BamQuery q1, q2;
         for (;;) {
@@ -242,11 +250,31 @@ rv_status rv_pool_run(rv_pool* p, const rv_region* regions, size_t n, rv_bam* fi
             }
             if (s == RV_OK) s = rv::bam_query_into(first, rg.chr, rg.start, rg.end, q1, &r1, msg);
             if (s == RV_OK && second) s = rv::bam_query_into(second, rg.chr, rg.start, rg.end, q2, &r2, msg);
+            rv::Engine* E = rv::engine_of(eng);
+            if (ds >= 0) {
+                std::unique_lock<std::mutex> lk(mu);
+                draw_cv.wait(lk, [&] { return draw_turn == i || stop; });
+                if (draw_turn != i) return;
+                E->drop_queue.clear();
+                E->drop_queue_pos = 0;
+                if (s == RV_OK) {
+                    const rv_reads* rs[2] = {&r1, second ? &r2 : nullptr};
+                    for (const rv_reads* r : rs) {
+                        if (!r) continue;
+                        E->drop_queue.emplace_back((size_t)r->n_reads);
+                        rv::downsample_draws(r->flag, r->n_reads, E->cfg.samfilter, ds, E->drop_queue.back().data());
+                    }
+                }
+                draw_turn++;
+                draw_cv.notify_all();
+            }
             rv_callset* cs = nullptr;
             if (s == RV_OK) {
                 s = second ? rv_run_region_somatic(eng, &rg, &r1, &r2, ct, &cs) : rv_run_region(eng, &rg, &r1, ct, &cs);
                 if (s != RV_OK) msg = rv_last_error(eng);
             }
+            E->drop_queue.clear();
+            E->drop_queue_pos = 0;
             if (s != RV_OK) {
                 std::lock_guard<std::mutex> lk(mu);
                 if (first_err == RV_OK) {
@@ -254,6 +282,7 @@ rv_status rv_pool_run(rv_pool* p, const rv_region* regions, size_t n, rv_bam* fi
                     p->err = "region " + std::to_string(i) + ": " + msg;
                 }
                 stop = true;
+                draw_cv.notify_all();
                 return;
             }
             out[i] = cs;

@@ -48,9 +48,19 @@ static Config to_config(const rv_config& c) {
     return k;
 }
 
+// -Z (recordPreprocessor.cpp:133): every record that passes the flag filter draws one value of the
+// C library's rand(), in iterator order, and is dropped when the draw is <= RAND_MAX * fraction.
+// The reference never seeds rand(), so its draws form one process-wide sequence; calling the same
+// rand() here continues that sequence.
+void downsample_draws(const uint16_t* flag, int64_t n, int samfilter, double fraction, uint8_t* drop) {
+    const double limit = RAND_MAX * fraction;
+    for (int64_t i = 0; i < n; i++) drop[i] = ((flag[i] & samfilter) == 0 && rand() <= limit) ? 1 : 0;
+}
+
 Engine::Engine(const rv_config& c, int dev) {
     cfg = to_config(c);
     sample = c.sample ? c.sample : "";
+    if (c.isDownsampling) downsampling = c.downsampling;
     device = dev;
     if (hipSetDevice(dev) != hipSuccess) {
         err = "hipSetDevice failed";
@@ -73,7 +83,7 @@ Engine::~Engine() {
                      &b_arena,     &b_clips,   &b_clip_arena, &b_anchors, &b_counts,   &b_gather_ids, &b_gather_out,
                      &b_cx_pos,    &b_cx_off,  &b_cx,     &b_cx_arena,   &b_cx_cov,   &b_cx_key, &b_out,
                      &b_out_arena, &b_score_ctr, &b_status, &b_keys, &b_keys2, &b_kpos, &b_kpos2, &b_sort_tmp,
-                     &b_kcount, &b_args, &b_kpos3, &b_nopass, &b_rows2, &b_want, &b_dup,
+                     &b_kcount, &b_args, &b_kpos3, &b_nopass, &b_rows2, &b_want, &b_dup, &b_drop,
                      &b_scr_cand, &b_scr_pos, &b_scr_cum, &b_scr_feat, &b_scr_score, &b_scr_params};
     // teardown: nothing to report a failure to
     for (auto* b : all)
@@ -161,6 +171,28 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
         R.bases = (const uint8_t*)dst[10];
         R.quals = (const uint8_t*)dst[11];
         R.cigar = (const uint32_t*)dst[12];
+    }
+    // ---- -Z drop mask (uploaded before the window's synchronisation below)
+    R.drop = nullptr;
+    if (downsampling >= 0) {
+        if (drop_queue_pos < drop_queue.size()) {
+            drop_host.swap(drop_queue[drop_queue_pos++]);
+            if ((int64_t)drop_host.size() != n) return fail(RV_ERR_ARG, "queued -Z mask does not match the reads");
+        } else {
+            drop_host.assign((size_t)n, 0);
+            std::vector<uint16_t> fl;
+            const uint16_t* flags = rd.flag;
+            if (rd.on_device && n > 0) {
+                fl.resize((size_t)n);
+                RV_HIP(hipMemcpy(fl.data(), rd.flag, 2 * (size_t)n, hipMemcpyDeviceToHost));
+                flags = fl.data();
+            }
+            downsample_draws(flags, n, cfg.samfilter, downsampling, drop_host.data());
+        }
+        uint8_t* d = (uint8_t*)grow(b_drop, (size_t)std::max<int64_t>(n, 1));
+        if (!d) return fail(RV_ERR_OOM, "device allocation failed");
+        if (n) RV_HIP(hipMemcpyAsync(d, drop_host.data(), (size_t)n, hipMemcpyHostToDevice, stream));
+        R.drop = d;
     }
     // ---- reference window
     Window W;

@@ -47,6 +47,12 @@ class Engine {
 
     Config cfg;
     std::string sample;
+    // -Z: the fraction, or a negative value when off.  Masks queued by the caller (the pool, which
+    // draws in region order) are used first, one per pileup call, before drawing here.
+    double downsampling = -1;
+    std::vector<std::vector<uint8_t>> drop_queue;
+    size_t drop_queue_pos = 0;
+    std::vector<uint8_t> drop_host;
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;
@@ -98,7 +104,7 @@ class Engine {
     // buffers of the realignment / scoring stages
     DevBuf b_gather_ids, b_gather_out, b_cx_pos, b_cx_off, b_cx, b_cx_arena, b_cx_cov, b_cx_key, b_out, b_out_arena,
         b_score_ctr, b_status, b_keys, b_keys2, b_kpos, b_kpos2, b_sort_tmp, b_kcount, b_kpos3, b_args, b_nopass,
-        b_want, b_dup, b_scr_cand, b_scr_pos, b_scr_cum, b_scr_feat, b_scr_score;
+        b_want, b_dup, b_drop, b_scr_cand, b_scr_pos, b_scr_cum, b_scr_feat, b_scr_score;
     WalkArgs* args_dev = nullptr;  // the last pileup's kernel arguments (its reads stay resident)
     int64_t args_n = 0;
     hipEvent_t sev[2] = {nullptr, nullptr};
@@ -131,5 +137,9 @@ class Engine {
     DevBuf b_events, b_arena, b_clips, b_clip_arena, b_anchors, b_counts;
     hipEvent_t ev[8];
 };
+
+// -Z: draws the C library's rand() for each record that passes the flag filter, in order; drop[i]
+// is 1 when the draw is <= RAND_MAX * fraction (recordPreprocessor.cpp:133)
+void downsample_draws(const uint16_t* flag, int64_t n, int samfilter, double fraction, uint8_t* drop);
 
 }  // namespace rv

@@ -25,6 +25,7 @@ struct ReadsDev {
     const uint8_t* quals;
     const uint32_t* cigar;
     int64_t n;
+    const uint8_t* drop;  // -Z: per read, 1 = dropped by the downsampling draw (nullptr without -Z)
 };
 
 // Everything the prep and walk kernels read besides the per-read records, kept in device memory
@@ -47,7 +48,8 @@ struct WalkArgs {
 // consecutive records that share its alignment start.
 RV_HD bool rec_filtered(const ReadsDev& R, const Config& c, int64_t i) {
     const int lseq = (int)(R.seq_off[i + 1] - R.seq_off[i]);
-    return (R.flag[i] & c.samfilter) != 0 || (int)R.mapq[i] < c.mappingQuality || lseq == 1;
+    return (R.flag[i] & c.samfilter) != 0 || (R.drop && R.drop[i]) || (int)R.mapq[i] < c.mappingQuality ||
+           lseq == 1;
 }
 // 0: no key; 1: keyed by the mate; 2: keyed by the CIGAR
 RV_HD int dup_kind(const ReadsDev& R, int64_t i) {
@@ -130,7 +132,8 @@ __device__ inline void prep_one(int64_t r, const ReadsDev& R, const Window& W, c
         atomicMax(max_span, lseq + reflen + 8);
         p.reflen = reflen;
     }
-    if ((flag & cfg.samfilter) != 0 || (int)R.mapq[r] < cfg.mappingQuality || lseq == 1 || (dup && dup[r])) {
+    if ((flag & cfg.samfilter) != 0 || (R.drop && R.drop[r]) || (int)R.mapq[r] < cfg.mappingQuality || lseq == 1 ||
+        (dup && dup[r])) {
         p.status = RD_FILTERED;
         prep[r] = p;
         return;
