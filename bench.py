@@ -92,6 +92,65 @@ def panel_bench(d, steps):
     return res
 
 
+# tumor/normal pair shaped as the 10 Mb, 200x/100x somatic workload, at 2 Mb: somatic SNVs and indels
+# at 6e-5/bp with VAF U(0.01, 0.2), germline variants shared by both samples, 2% duplicate pairs; a
+# seed that no screen training or validation genome uses
+SOMATIC = ["--seed", "2026", "--len", "2000000", "--depth", "200", "--normal-depth", "100", "--somatic-rate",
+           "0.00006", "--vaf", "0.01", "0.2", "--indel-rate", "0.0001", "--dup-rate", "0.02"]
+
+
+def somatic_bench(d, steps, warmup):
+    """Somatic mode on the tumor/normal pair: exact and screened throughput (tumor + normal reads,
+    resident in HBM), and the screened call set against the exact one."""
+    import torch
+
+    from rabbitvar_amd import read_fasta, read_rvb, screen
+    from rabbitvar_amd.engine import DeviceReads, Engine
+
+    if not os.path.exists(os.path.join(d, "normal.rvb")):
+        os.makedirs(d, exist_ok=True)
+        log(f"generating the tumor/normal pair into {d}")
+        subprocess.run([SYNTH, "--out", d] + SOMATIC, check=True, stdout=subprocess.DEVNULL)
+    name, seq = read_fasta(os.path.join(d, "ref.fa"))
+    tumor, normal = read_rvb(os.path.join(d, "reads.rvb")), read_rvb(os.path.join(d, "normal.rvb"))
+    n_reads = tumor.n + normal.n
+    dtum, dnor = DeviceReads(tumor), DeviceReads(normal)
+    eng = Engine(sample="T|N", fisher=1)
+
+    def timed():
+        out = None
+        for _ in range(warmup):
+            out = eng.run_region_somatic(name, 1, len(seq), dtum, dnor, name, seq)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            out = eng.run_region_somatic(name, 1, len(seq), dtum, dnor, name, seq)
+        torch.cuda.synchronize()
+        return out, (time.perf_counter() - t0) / steps
+
+    text, dt = timed()
+    params, threshold, _ = screen.load_model()
+    eng.set_screen(params, threshold)
+    stext, sdt = timed()
+    st = eng.stats()
+    eng.set_screen(None, 0)
+    eng.close()
+    cmp_ = screen.compare(text, stext)
+    res = {"workload": "2 Mb contig, tumor 200x + normal 100x 150 bp PE, somatic SNVs/indels 6e-5 at VAF 0.01-0.2, "
+                       "germline SNV 1e-3 + indels 1e-4, 2% duplicates, seed 2026; -b 'T|N' --fisher",
+           "reads": n_reads, "exact_reads_per_s": round(n_reads / dt, 1), "exact_ms": round(dt * 1e3, 2),
+           "screened_reads_per_s": round(n_reads / sdt, 1), "screened_ms": round(sdt * 1e3, 2),
+           "speedup": round(dt / sdt, 4), "recall": round(cmp_["recall"], 6),
+           "precision": round(cmp_["precision"], 6), "identical_lines": round(cmp_["identical_lines"], 6),
+           "calls": cmp_["exact_calls"], "candidates": st["screen_candidates"], "skipped": st["screen_skipped"],
+           "host_detail_ms_last_step": dict(zip(["first_sample", "second_sample", "ref_passes", "group_by_position",
+                                                 "print"], [round(x, 2) for x in st["detail_ms"][:5]]))}
+    log(f"somatic: {n_reads} reads, exact {res['exact_reads_per_s']:.0f} reads/s, screened "
+        f"{res['screened_reads_per_s']:.0f} reads/s ({res['speedup']}x), recall {res['recall']}, "
+        f"precision {res['precision']}")
+    return res
+
+
 def cpu_baseline(d):
     if not os.path.exists(ORACLE):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
@@ -116,6 +175,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-panel", action="store_true")
     ap.add_argument("--panel-data", default=os.environ.get("RV_BENCH_PANEL", "/tmp/rv_bench_panel"))
+    ap.add_argument("--no-somatic", action="store_true")
+    ap.add_argument("--somatic-data", default=os.environ.get("RV_BENCH_SOMATIC", "/tmp/rv_bench_somatic"))
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: the calling path has no multi-GPU mode here (asked for {a.gpus}); "
@@ -231,6 +292,8 @@ def main():
     eng.close()
     if not a.no_panel:
         res["panel"] = panel_bench(a.panel_data, max(1, min(a.steps, 3)))
+    if not a.no_somatic:
+        res["somatic"] = somatic_bench(a.somatic_data, max(1, min(a.steps, 3)), 1)
     if not a.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(a.data)
     print(json.dumps(res), flush=True)

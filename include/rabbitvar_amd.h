@@ -155,7 +155,10 @@ typedef struct rv_stats {
     double detail_ms[16];      /* finer host split: realign prefetch, mnp, del, ins, lgins, record list,
                                   key sort, order-thread wait, alignedVariants order, rank+sort,
                                   event reduction, clip reduction, noPassingReads, dense-row fetches on
-                                  a cache miss, then the counts of those two */
+                                  a cache miss, then the counts of those two.  After a somatic
+                                  region: the first sample (pileup to first scoring pass), the
+                                  second, the reference-only scoring passes, grouping both samples'
+                                  records by position, printing */
     double call_ms;            /* rv_run_region including the release of its host working state */
     double screen_ms;          /* realignment screen: candidate list, features, scores (0 when off) */
     int64_t screen_candidates; /* realignment candidates the screen saw */

@@ -14,6 +14,7 @@
 // samples' position states are known, scored again for the reference-only positions the printer
 // will read: those where the other sample has variants.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <map>
@@ -21,6 +22,7 @@
 #include <set>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "rv_fisher.h"
@@ -46,10 +48,11 @@ struct SVars {  // Vars (include/Vars.h): reference variant, variants in CMP_VAR
     bool has_ref = false;
     SV ref;
     std::vector<SV> vars;
-    std::map<std::string, size_t> by_desc;
+    // the variant stored under descriptor d (the first one added with it)
     SV* find(const std::string& d) {
-        auto it = by_desc.find(d);
-        return it == by_desc.end() ? nullptr : &vars[it->second];
+        for (SV& v : vars)
+            if (v.desc == d) return &v;
+        return nullptr;
     }
 };
 
@@ -474,8 +477,20 @@ rv_status run_sample(Engine& e, const rv_region& region, const rv_reads& reads, 
     sm.S = std::make_unique<RState>(sm.pr.P, *sm.rc);
     DeviceReads rq(e);
     RealignStats rst;
+    // screened mode: each sample's candidates are scored after its own pileup
+    ScreenPlan plan;
+    ScreenPlan* planp = nullptr;
+    if (e.screen_on) {
+        const auto q0 = std::chrono::steady_clock::now();
+        s = e.screen_plan(sm.pr, plan, false);
+        if (s != RV_OK) return s;
+        planp = &plan;
+        e.stats.screen_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - q0).count();
+        e.stats.screen_candidates += (int64_t)plan.cand.size();
+    }
     try {
-        realign(e.cfg, region.start, region.end, sm.ref, contig.length, &rq, *sm.S, rst);
+        realign(e.cfg, region.start, region.end, sm.ref, contig.length, &rq, *sm.S, rst, false, planp);
+        if (planp) e.stats.screen_skipped += plan.skipped;
         build_record_list(*sm.S, sm.cx);
     } catch (const std::exception& ex) {
         return e.fail(RV_ERR_DEVICE, std::string("realignment: ") + ex.what());
@@ -483,29 +498,75 @@ rv_status run_sample(Engine& e, const rv_region& region, const rv_reads& reads, 
     return score_sample(e, sm, region, contig, SCORE_SOMATIC_VARS, nullptr, sm.recs, sm.arena, &sm.status);
 }
 
-// a sample's positions, from its records
-std::map<int, SVars> collect(const Sample& sm) {
-    std::map<int, SVars> m;
-    auto add = [&](const std::vector<OutRec>& recs, const std::vector<char>& arena) {
-        std::vector<const OutRec*> order;
-        for (const OutRec& o : recs) order.push_back(&o);
-        std::sort(order.begin(), order.end(), [](const OutRec* a, const OutRec* b) {
-            return a->position != b->position ? a->position < b->position : a->order < b->order;
-        });
-        for (const OutRec* o : order) {
-            SVars& vs = m[o->position];
-            if (o->kind == OUT_VARIANT || o->kind == OUT_VARIANT_RAW) {
-                vs.by_desc.emplace(to_sv(*o, arena.data()).desc, vs.vars.size());
-                vs.vars.push_back(to_sv(*o, arena.data()));
+// per-position status of a sample over [lo, hi] (Sample::state for every position at once)
+std::vector<uint8_t> states_of(const Sample& sm, int lo, int hi) {
+    std::vector<uint8_t> st((size_t)(hi - lo + 1), 0);
+    for (int i = 0; i < sm.dense_n; i++) {
+        const int p = sm.dense_lo + i;
+        if (p >= lo && p <= hi) st[(size_t)(p - lo)] = sm.status[(size_t)i];
+    }
+    for (size_t k = 0; k < sm.cx.pos.size(); k++) {
+        const int p = sm.cx.pos[k];
+        if (p >= lo && p <= hi) st[(size_t)(p - lo)] = sm.status[(size_t)sm.dense_n + k];
+    }
+    return st;
+}
+
+// a sample's records at the positions the printer reads (`need`, over [lo, hi]), bucketed by
+// position: at each position the scoring pass's records in creation order, then the
+// reference-only pass's
+struct Bucketed {
+    struct Item {
+        const OutRec* o;
+        const char* arena;
+        int source;
+    };
+    std::vector<uint32_t> off;  // per position, into items
+    std::vector<Item> items;
+
+    // the position's Vars (include/Vars.h) as the reference's map holds them
+    bool vars_at(size_t i, SVars& vs) const {
+        if (off[i] == off[i + 1]) return false;
+        for (uint32_t k = off[i]; k < off[i + 1]; k++) {
+            const Item& it = items[k];
+            SV v = to_sv(*it.o, it.arena);
+            if (it.o->kind == OUT_VARIANT || it.o->kind == OUT_VARIANT_RAW) {
+                vs.vars.push_back(std::move(v));
             } else {
                 vs.has_ref = true;
-                vs.ref = to_sv(*o, arena.data());
+                vs.ref = std::move(v);
             }
         }
+        return true;
+    }
+};
+
+Bucketed bucket(const Sample& sm, int lo, const std::vector<uint8_t>& need) {
+    Bucketed b;
+    b.off.assign(need.size() + 1, 0);
+    auto slot = [&](const OutRec& o) -> long {
+        const long i = (long)o.position - lo;
+        return (i >= 0 && i < (long)need.size() && need[(size_t)i]) ? i : -1;
     };
-    add(sm.recs, sm.arena);
-    add(sm.ref_recs, sm.ref_arena);
-    return m;
+    for (const auto* recs : {&sm.recs, &sm.ref_recs})
+        for (const OutRec& o : *recs)
+            if (const long i = slot(o); i >= 0) b.off[(size_t)i + 1]++;
+    for (size_t i = 0; i < need.size(); i++) b.off[i + 1] += b.off[i];
+    b.items.resize(b.off.back());
+    std::vector<uint32_t> fill(b.off.begin(), b.off.end() - 1);
+    for (int src = 0; src < 2; src++) {
+        const std::vector<OutRec>& recs = src ? sm.ref_recs : sm.recs;
+        const char* arena = (src ? sm.ref_arena : sm.arena).data();
+        for (const OutRec& o : recs)
+            if (const long i = slot(o); i >= 0) b.items[fill[(size_t)i]++] = {&o, arena, src};
+    }
+    for (size_t i = 0; i < need.size(); i++)
+        if (b.off[i + 1] - b.off[i] > 1)
+            std::sort(b.items.begin() + b.off[i], b.items.begin() + b.off[i + 1],
+                      [](const Bucketed::Item& x, const Bucketed::Item& y) {
+                          return x.source != y.source ? x.source < y.source : x.o->order < y.o->order;
+                      });
+    return b;
 }
 
 }  // namespace
@@ -522,12 +583,26 @@ rv_status run_region_somatic(Engine& e, const rv_region& region, const rv_reads&
     e.cfg.hasBam2 = 1;
     e.pinned_reset();
     e.stats.score_ms = 0;
+    e.stats.screen_ms = 0;
+    e.stats.screen_list_ms = 0;
+    e.stats.screen_candidates = 0;
+    e.stats.screen_skipped = 0;
+    for (double& d : e.stats.detail_ms) d = 0;
+    auto lap = [](std::chrono::steady_clock::time_point& t0) {
+        const auto t1 = std::chrono::steady_clock::now();
+        const double ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        t0 = t1;
+        return ms;
+    };
+    auto tl = w0;
     Sample t, n;
     rv_status s = run_sample(e, region, first, contig, 0, nullptr, t);
     if (s != RV_OK) return s;
+    e.stats.detail_ms[0] = lap(tl);
     e.swap_row_buffers();  // the second pileup must not overwrite the first sample's rows
     s = run_sample(e, region, second, contig, t.pr.P.maxReadLength, &t.ref.created, n);
     if (s != RV_OK) return s;
+    e.stats.detail_ms[1] = lap(tl);
     // the reference-only positions the printer will read: where the other sample has variants
     auto wants = [&](const Sample& self, const Sample& other) {
         std::vector<uint8_t> w(self.status.size(), 0);
@@ -552,41 +627,78 @@ rv_status run_region_somatic(Engine& e, const rv_region& region, const rv_reads&
         s = score_sample(e, t, region, contig, SCORE_SOMATIC_REF, &wt.first, t.ref_recs, t.ref_arena, nullptr);
         if (s != RV_OK) return s;
     }
-    // ---- the printer: the first sample's kept positions, ascending
-    std::map<int, SVars> v1 = collect(t), v2 = collect(n);
+    e.stats.detail_ms[2] = lap(tl);
+    // ---- the printer: the first sample's kept positions, ascending.  Only positions where one
+    // sample has variants, or only the first is kept, print anything; the rest (both reference
+    // only) are passed over without their records.
+    const int lo = region.start, hi = std::max(region.start, region.end);
+    const std::vector<uint8_t> s1 = states_of(t, lo, hi), s2 = states_of(n, lo, hi);
+    std::vector<uint8_t> need1(s1.size(), 0), need2(s1.size(), 0);
+    for (size_t i = 0; i < s1.size(); i++) {
+        if (s1[i] == 0) continue;
+        need1[i] = (s2[i] == 0 || s1[i] == 1 || s2[i] == 1) ? 1 : 0;
+        need2[i] = (s2[i] != 0 && (s1[i] == 1 || s2[i] == 1)) ? 1 : 0;
+    }
+    const Bucketed b1 = bucket(t, lo, need1), b2 = bucket(n, lo, need2);
+    e.stats.detail_ms[3] = lap(tl);
     // -N 'first|second' prints the first name (Launcher.cpp getSampleNamesSomatic)
     const std::string sample = e.sample.substr(0, e.sample.find('|'));
-    Printer P{e.cfg, sample, region, {}};
-    std::vector<int> positions;
-    for (int i = 0; i < t.dense_n; i++) {
-        const int p = t.dense_lo + i;
-        if (p >= region.start && p <= region.end && t.status[(size_t)i] != 0) positions.push_back(p);
+    // positions print independently: chunks of them on host threads, texts joined in order
+    const size_t npos = s1.size();
+    const size_t n_chunks = std::max<size_t>(1, std::min<size_t>(256, npos / 4096));
+    std::vector<std::string> texts(n_chunks);
+    std::vector<std::exception_ptr> errs(n_chunks);
+    std::atomic<size_t> next_chunk{0};
+    auto print_chunks = [&]() {
+        for (;;) {
+            const size_t ch = next_chunk.fetch_add(1);
+            if (ch >= n_chunks) return;
+            Printer P{e.cfg, sample, region, {}};
+            try {
+                for (size_t i = npos * ch / n_chunks; i < npos * (ch + 1) / n_chunks; i++) {
+                    const int st1 = s1[i], st2 = s2[i];
+                    if (st1 == 0) continue;
+                    SVars a, b;
+                    const bool has_a = b1.vars_at(i, a);
+                    if (st2 == 0) {
+                        if (has_a) P.one_sample(a, false, "SampleSpecific");
+                        continue;
+                    }
+                    if (st1 != 1 && st2 != 1) continue;
+                    b2.vars_at(i, b);  // a sample without records here takes an empty Vars
+                    if (st1 == 1)
+                        P.first_sample(a, b);
+                    else
+                        P.second_sample(a, b);
+                }
+            } catch (...) {
+                errs[ch] = std::current_exception();
+            }
+            texts[ch] = std::move(P.out);
+        }
+    };
+    {
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const size_t nt = std::min<size_t>({(size_t)std::min(hw, 16u), n_chunks});
+        std::vector<std::thread> th;
+        for (size_t k = 1; k < nt; k++) th.emplace_back(print_chunks);
+        print_chunks();
+        for (auto& x : th) x.join();
     }
-    for (size_t k = 0; k < t.cx.pos.size(); k++) {
-        const int p = t.cx.pos[k];
-        if (p >= region.start && p <= region.end && t.status[(size_t)t.dense_n + k] != 0) positions.push_back(p);
-    }
-    std::sort(positions.begin(), positions.end());
-    positions.erase(std::unique(positions.begin(), positions.end()), positions.end());
+    std::string text;
+    size_t total = 0;
+    for (const auto& x : texts) total += x.size();
+    text.reserve(total);
     try {
-        for (int p : positions) {
-            const int st1 = t.state(p), st2 = n.state(p);
-            if (st1 == 0) continue;
-            if (st2 == 0) {
-                auto it = v1.find(p);
-                if (it != v1.end()) P.one_sample(it->second, false, "SampleSpecific");
-                continue;
-            }
-            if (st1 == 1) {
-                P.first_sample(v1[p], v2[p]);
-            } else if (st2 == 1) {
-                P.second_sample(v1[p], v2[p]);
-            }
+        for (size_t ch = 0; ch < n_chunks; ch++) {
+            if (errs[ch]) std::rethrow_exception(errs[ch]);
+            text += texts[ch];
         }
     } catch (const std::exception& ex) {
         return e.fail(RV_ERR_UNSUPPORTED, ex.what());
     }
-    out.text = std::move(P.out);
+    e.stats.detail_ms[4] = lap(tl);
+    out.text = std::move(text);
     out.calls.clear();
     out.strings.clear();
     e.stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();

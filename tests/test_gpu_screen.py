@@ -125,3 +125,43 @@ def test_screened_mode_extremes(dataset):
     back = eng.run_region(name, 1, L, reads.select(1, L), name, seq)
     eng.close()
     assert back == exact
+
+
+def test_screened_somatic(dataset):
+    """Screened mode in somatic mode: each sample's candidates are screened after its own pileup.
+    Keeping every candidate reproduces the exact text; skipping every one skips both samples' lists;
+    the trained model keeps the call set."""
+    import os
+
+    from rabbitvar_amd import read_fasta, read_rvb, screen
+    from rabbitvar_amd.engine import Engine
+
+    d = dataset("som_edge")
+    name, seq = read_fasta(os.path.join(d, "ref.fa"))
+    L = len(seq)
+    tumor = read_rvb(os.path.join(d, "reads.rvb")).select(1, L)
+    normal = read_rvb(os.path.join(d, "normal.rvb")).select(1, L)
+    eng = Engine(sample="T|N", fisher=1)
+    exact = eng.run_region_somatic(name, 1, L, tumor, normal, name, seq)
+    P = np.zeros(omlp.NPARAM, np.float32)
+    P[omlp.B3] = 30.0
+    eng.set_screen(P, 0.5)
+    keep_all = eng.run_region_somatic(name, 1, L, tumor, normal, name, seq)
+    st = eng.stats()
+    assert keep_all == exact
+    assert st["screen_candidates"] > 100 and st["screen_skipped"] == 0
+    P[omlp.B3] = -30.0
+    eng.set_screen(P, 0.5)
+    eng.run_region_somatic(name, 1, L, tumor, normal, name, seq)
+    st = eng.stats()
+    assert st["screen_skipped"] == st["screen_candidates"] > 100
+    params, threshold, _ = screen.load_model()
+    eng.set_screen(params, threshold)
+    screened = eng.run_region_somatic(name, 1, L, tumor, normal, name, seq)
+    eng.set_screen(None, 0)
+    back = eng.run_region_somatic(name, 1, L, tumor, normal, name, seq)
+    eng.close()
+    assert back == exact
+    m = screen.compare(exact, screened)
+    assert m["exact_calls"] > 100
+    assert m["recall"] >= 0.99 and m["precision"] >= 0.99
