@@ -639,7 +639,25 @@ rv_status run_region_somatic(Engine& e, const rv_region& region, const rv_reads&
         need1[i] = (s2[i] == 0 || s1[i] == 1 || s2[i] == 1) ? 1 : 0;
         need2[i] = (s2[i] != 0 && (s1[i] == 1 || s2[i] == 1)) ? 1 : 0;
     }
-    const Bucketed b1 = bucket(t, lo, need1), b2 = bucket(n, lo, need2);
+    Bucketed b1, b2;
+    {
+        std::exception_ptr berr;
+        std::thread tb([&] {
+            try {
+                b2 = bucket(n, lo, need2);
+            } catch (...) {
+                berr = std::current_exception();
+            }
+        });
+        try {
+            b1 = bucket(t, lo, need1);
+        } catch (...) {
+            tb.join();
+            throw;
+        }
+        tb.join();
+        if (berr) std::rethrow_exception(berr);
+    }
     e.stats.detail_ms[3] = lap(tl);
     // -N 'first|second' prints the first name (Launcher.cpp getSampleNamesSomatic)
     const std::string sample = e.sample.substr(0, e.sample.find('|'));
