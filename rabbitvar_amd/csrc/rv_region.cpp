@@ -50,7 +50,7 @@ class DeviceReadQuery : public ReadQuery {
     } while (0)
 
 rv_status score_pass(Engine& e, ScoreIn& in, size_t n_status, std::vector<OutRec>& recs, std::vector<char>& arena,
-                     std::vector<uint8_t>* status, unsigned long long hc[8], float& score_ms) {
+                     std::vector<uint8_t>* status, unsigned long long hc[8], float& score_ms, RecSpan* pinned) {
     unsigned max_out = (unsigned)std::min<int64_t>(65536 + 8 * (int64_t)in.n_cx + in.dense_n / 16, 0x7fffffff);
     unsigned long long arena_cap = 160ull * max_out;
     for (int attempt = 0; attempt < 3; attempt++) {
@@ -79,10 +79,22 @@ rv_status score_pass(Engine& e, ScoreIn& in, size_t n_status, std::vector<OutRec
                 return e.fail(RV_ERR_CAPACITY, "scoring: " + std::to_string(hc[2]) +
                                                    " position(s) exceeded the per-thread allele/string capacity");
             }
-            recs.resize(n_out);
-            arena.resize((size_t)used + 1);
-            if (n_out) RV_HIPS(hipMemcpyAsync(recs.data(), in.out, sizeof(OutRec) * n_out, hipMemcpyDeviceToHost, e.stream));
-            if (used) RV_HIPS(hipMemcpyAsync(arena.data(), in.arena, (size_t)used, hipMemcpyDeviceToHost, e.stream));
+            OutRec* rdst;
+            char* adst;
+            if (pinned) {
+                rdst = (OutRec*)e.pinned_alloc(sizeof(OutRec) * std::max(n_out, 1u));
+                adst = (char*)e.pinned_alloc((size_t)used + 1);
+                if (!rdst || !adst) return e.fail(RV_ERR_OOM, "pinned host memory for the scored records failed");
+                adst[used] = 0;
+                *pinned = RecSpan{rdst, n_out, adst};
+            } else {
+                recs.resize(n_out);
+                arena.resize((size_t)used + 1);
+                rdst = recs.data();
+                adst = arena.data();
+            }
+            if (n_out) RV_HIPS(hipMemcpyAsync(rdst, in.out, sizeof(OutRec) * n_out, hipMemcpyDeviceToHost, e.stream));
+            if (used) RV_HIPS(hipMemcpyAsync(adst, in.arena, (size_t)used, hipMemcpyDeviceToHost, e.stream));
             if (status) {
                 status->resize(n_status);
                 RV_HIPS(hipMemcpyAsync(status->data(), in.status, n_status, hipMemcpyDeviceToHost, e.stream));

@@ -29,7 +29,17 @@ rv_status run_region_somatic(Engine& e, const rv_region& region, const rv_reads&
 // One launch of the scoring kernels over `in` (record list already uploaded), retried with larger
 // record buffers when they overflow; copies back the records, the string arena and, when asked,
 // the per-slot status.  hc receives the kernel counters.
+// With `pinned` set, the records and the arena go to the engine's pinned host memory instead of
+// the vectors (valid until its pinned_reset()).
+struct RecSpan {
+    const OutRec* p = nullptr;
+    size_t n = 0;
+    const char* arena = nullptr;
+    const OutRec* begin() const { return p; }
+    const OutRec* end() const { return p + n; }
+};
 rv_status score_pass(Engine& e, ScoreIn& in, size_t n_status, std::vector<OutRec>& recs, std::vector<char>& arena,
-                     std::vector<uint8_t>* status, unsigned long long hc[8], float& score_ms);
+                     std::vector<uint8_t>* status, unsigned long long hc[8], float& score_ms,
+                     RecSpan* pinned = nullptr);
 
 }  // namespace rv

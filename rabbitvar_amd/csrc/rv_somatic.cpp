@@ -373,11 +373,9 @@ struct Sample {
     HostRef ref;
     DenseRow* rows = nullptr;
     int dense_lo = 0, dense_n = 0;
-    std::vector<OutRec> recs;
-    std::vector<char> arena;
+    RecSpan recs;                 // first scoring pass (pinned host memory of the engine)
     std::vector<uint8_t> status;  // per slot: dense rows, then record-list positions
-    std::vector<OutRec> ref_recs;
-    std::vector<char> ref_arena;
+    RecSpan ref_recs;             // reference-only pass
 
     // status of position p: 1 variants, 3 reference only, 0 not kept
     int state(int p) const {
@@ -412,8 +410,7 @@ class DeviceReads : public ReadQuery {
 
 // uploads the sample's record list and runs one scoring pass in `mode`
 rv_status score_sample(Engine& e, Sample& sm, const rv_region& region, const rv_contig& contig, int mode,
-                       const std::vector<uint8_t>* want, std::vector<OutRec>& recs, std::vector<char>& arena,
-                       std::vector<uint8_t>* status) {
+                       const std::vector<uint8_t>* want, RecSpan& recs, std::vector<uint8_t>* status) {
     const CxHost& cx = sm.cx;
     ScoreIn in{};
     in.rows = sm.rows;
@@ -447,7 +444,9 @@ rv_status score_sample(Engine& e, Sample& sm, const rv_region& region, const rv_
         return e.fail(RV_ERR_OOM, "device buffers for somatic scoring failed");
     unsigned long long hc[8];
     float ms = 0;
-    rv_status s = score_pass(e, in, n_status, recs, arena, status, hc, ms);
+    std::vector<OutRec> unused_recs;
+    std::vector<char> unused_arena;
+    rv_status s = score_pass(e, in, n_status, unused_recs, unused_arena, status, hc, ms, &recs);
     e.stats.score_ms += ms;
     // the pageable uploads above are read by the kernels before score_pass returns
     return s;
@@ -495,7 +494,7 @@ rv_status run_sample(Engine& e, const rv_region& region, const rv_reads& reads, 
     } catch (const std::exception& ex) {
         return e.fail(RV_ERR_DEVICE, std::string("realignment: ") + ex.what());
     }
-    return score_sample(e, sm, region, contig, SCORE_SOMATIC_VARS, nullptr, sm.recs, sm.arena, &sm.status);
+    return score_sample(e, sm, region, contig, SCORE_SOMATIC_VARS, nullptr, sm.recs, &sm.status);
 }
 
 // per-position status of a sample over [lo, hi] (Sample::state for every position at once)
@@ -555,8 +554,8 @@ Bucketed bucket(const Sample& sm, int lo, const std::vector<uint8_t>& need) {
     b.items.resize(b.off.back());
     std::vector<uint32_t> fill(b.off.begin(), b.off.end() - 1);
     for (int src = 0; src < 2; src++) {
-        const std::vector<OutRec>& recs = src ? sm.ref_recs : sm.recs;
-        const char* arena = (src ? sm.ref_arena : sm.arena).data();
+        const RecSpan& recs = src ? sm.ref_recs : sm.recs;
+        const char* arena = recs.arena;
         for (const OutRec& o : recs)
             if (const long i = slot(o); i >= 0) b.items[fill[(size_t)i]++] = {&o, arena, src};
     }
@@ -620,11 +619,11 @@ rv_status run_region_somatic(Engine& e, const rv_region& region, const rv_reads&
     };
     auto wt = wants(t, n), wn = wants(n, t);
     if (wn.second) {
-        s = score_sample(e, n, region, contig, SCORE_SOMATIC_REF, &wn.first, n.ref_recs, n.ref_arena, nullptr);
+        s = score_sample(e, n, region, contig, SCORE_SOMATIC_REF, &wn.first, n.ref_recs, nullptr);
         if (s != RV_OK) return s;
     }
     if (wt.second) {
-        s = score_sample(e, t, region, contig, SCORE_SOMATIC_REF, &wt.first, t.ref_recs, t.ref_arena, nullptr);
+        s = score_sample(e, t, region, contig, SCORE_SOMATIC_REF, &wt.first, t.ref_recs, nullptr);
         if (s != RV_OK) return s;
     }
     e.stats.detail_ms[2] = lap(tl);
