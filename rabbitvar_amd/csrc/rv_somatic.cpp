@@ -603,12 +603,14 @@ rv_status run_region_somatic(Engine& e, const rv_region& region, const rv_reads&
     if (s != RV_OK) return s;
     e.stats.detail_ms[1] = lap(tl);
     // the reference-only positions the printer will read: where the other sample has variants
-    auto wants = [&](const Sample& self, const Sample& other) {
+    const int lo = region.start, hi = std::max(region.start, region.end);
+    const std::vector<uint8_t> s1 = states_of(t, lo, hi), s2 = states_of(n, lo, hi);
+    auto wants = [&](const Sample& self, const std::vector<uint8_t>& other) {
         std::vector<uint8_t> w(self.status.size(), 0);
         bool any = false;
         auto mark = [&](size_t slot, int p) {
             if (p < region.start || p > region.end) return;
-            if (self.status[slot] == 3 && other.state(p) == 1) {
+            if (self.status[slot] == 3 && other[(size_t)(p - lo)] == 1) {
                 w[slot] = 1;
                 any = true;
             }
@@ -617,7 +619,7 @@ rv_status run_region_somatic(Engine& e, const rv_region& region, const rv_reads&
         for (size_t k = 0; k < self.cx.pos.size(); k++) mark((size_t)self.dense_n + k, self.cx.pos[k]);
         return std::make_pair(w, any);
     };
-    auto wt = wants(t, n), wn = wants(n, t);
+    auto wt = wants(t, s2), wn = wants(n, s1);
     if (wn.second) {
         s = score_sample(e, n, region, contig, SCORE_SOMATIC_REF, &wn.first, n.ref_recs, nullptr);
         if (s != RV_OK) return s;
@@ -630,8 +632,6 @@ rv_status run_region_somatic(Engine& e, const rv_region& region, const rv_reads&
     // ---- the printer: the first sample's kept positions, ascending.  Only positions where one
     // sample has variants, or only the first is kept, print anything; the rest (both reference
     // only) are passed over without their records.
-    const int lo = region.start, hi = std::max(region.start, region.end);
-    const std::vector<uint8_t> s1 = states_of(t, lo, hi), s2 = states_of(n, lo, hi);
     std::vector<uint8_t> need1(s1.size(), 0), need2(s1.size(), 0);
     for (size_t i = 0; i < s1.size(); i++) {
         if (s1[i] == 0) continue;
