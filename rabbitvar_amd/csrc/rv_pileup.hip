@@ -37,11 +37,18 @@ __global__ void __launch_bounds__(WALK_THREADS) k_walk(const WalkArgs* A, const 
     const int64_t n = A->R.n;
     const int64_t r0 = (int64_t)blockIdx.x * WALK_READS;
     const int tile_lo = (r0 < n ? prep[r0].position : 0) - TILE_MARGIN;
+    // reads after each wavefront's first are handed out as wavefronts finish (reads differ in cost;
+    // every contribution merges the same way whichever wavefront walks it)
+    __shared__ int walk_next;
+    if (threadIdx.x == 0) walk_next = waves;
     tile_init((int)threadIdx.x, (int)blockDim.x, A->W, tile_lo);
     __syncthreads();
-    for (int k = wave; k < WALK_READS; k += waves) {
+    for (int k = wave; k < WALK_READS;) {
         const int64_t r = r0 + k;
         if (r < n) walk_one(r, A->R, A->W, A->cfg, prep, cig, A->S, lane, wave, tile_lo);
+        int nk = 0;
+        if (lane == 0) nk = atomicAdd(&walk_next, 1);
+        k = __builtin_amdgcn_readfirstlane(__shfl(nk, 0));
     }
     __syncthreads();
     tile_flush(tile_lo, A->S.dense, A->S.dense_lo, A->S.dense_n, (int)threadIdx.x, (int)blockDim.x);
