@@ -53,6 +53,10 @@ class Engine {
     std::vector<std::vector<uint8_t>> drop_queue;
     size_t drop_queue_pos = 0;
     std::vector<uint8_t> drop_host;
+    // scored-record capacity that sufficed before (grow-only): later passes start there instead of
+    // overflowing and relaunching
+    unsigned score_out_hint = 0;
+    unsigned long long score_arena_hint = 0;
     int device = 0;
     hipStream_t stream = nullptr;
     std::string err;

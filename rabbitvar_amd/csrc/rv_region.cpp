@@ -52,7 +52,8 @@ class DeviceReadQuery : public ReadQuery {
 rv_status score_pass(Engine& e, ScoreIn& in, size_t n_status, std::vector<OutRec>& recs, std::vector<char>& arena,
                      std::vector<uint8_t>* status, unsigned long long hc[8], float& score_ms, RecSpan* pinned) {
     unsigned max_out = (unsigned)std::min<int64_t>(65536 + 8 * (int64_t)in.n_cx + in.dense_n / 16, 0x7fffffff);
-    unsigned long long arena_cap = 160ull * max_out;
+    max_out = std::max(max_out, e.score_out_hint);
+    unsigned long long arena_cap = std::max(160ull * max_out, e.score_arena_hint);
     for (int attempt = 0; attempt < 3; attempt++) {
         in.out = (OutRec*)e.grow(e.b_out, sizeof(OutRec) * max_out);
         in.arena = (char*)e.grow(e.b_out_arena, (size_t)arena_cap);
@@ -74,6 +75,8 @@ rv_status score_pass(Engine& e, ScoreIn& in, size_t n_status, std::vector<OutRec
         unsigned n_out = (unsigned)(hc[0] & 0xffffffffu);
         unsigned long long used = hc[1];
         if (n_out <= max_out && used <= arena_cap) {
+            e.score_out_hint = std::max(e.score_out_hint, max_out);
+            e.score_arena_hint = std::max(e.score_arena_hint, arena_cap);
             if (hc[2] != 0) {
                 // a position needed more alleles or string space than a scoring thread holds
                 return e.fail(RV_ERR_CAPACITY, "scoring: " + std::to_string(hc[2]) +
