@@ -377,11 +377,40 @@ struct Strs {
 
 // Appends one record and its strings (refallele, varallele, genotype, bias, leftseq, rightseq,
 // descriptor); false when the buffers are full (counted).
+// The lanes of a wavefront that reach this call together take their slots with one pair of atomics
+// (issued by the lowest of them): every scoring thread appends to the same two counters, and one
+// atomic per record serialised them.
 __device__ inline bool emit_out(const ScoreIn& S, OutRec& o, const char* const pc[7], const int pn[7]) {
-    unsigned oi = atomicAdd(S.n_out, 1u);
     unsigned long long nbytes = 0;
     for (int k = 0; k < 7; k++) nbytes += (unsigned long long)pn[k];
-    unsigned long long off = atomicAdd(S.arena_used, nbytes);
+    const unsigned long long active = __builtin_amdgcn_read_exec();
+    const int lane = (int)__lane_id();
+    unsigned rank = 0, cnt = 0;
+    unsigned long long before = 0, total = 0;
+    for (unsigned long long m = active; m; m &= m - 1) {
+        const int l = __builtin_ctzll(m);  // wave-uniform
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)nbytes, l);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(nbytes >> 32), l);
+        const unsigned long long v = ((unsigned long long)hi << 32) | lo;
+        if (l < lane) {
+            before += v;
+            rank++;
+        }
+        total += v;
+        cnt++;
+    }
+    const int leader = __builtin_ctzll(active);
+    unsigned base = 0;
+    unsigned long long abase = 0;
+    if (lane == leader) {
+        base = atomicAdd(S.n_out, cnt);
+        abase = atomicAdd(S.arena_used, total);
+    }
+    base = (unsigned)__builtin_amdgcn_readlane((int)base, leader);
+    abase = ((unsigned long long)(unsigned)__builtin_amdgcn_readlane((int)(unsigned)(abase >> 32), leader) << 32) |
+            (unsigned)__builtin_amdgcn_readlane((int)(unsigned)abase, leader);
+    const unsigned oi = base + rank;
+    const unsigned long long off = abase + before;
     if (oi >= S.max_out || off + nbytes > S.arena_cap) {
         atomicAdd(&S.counters[0], 1ull);
         return false;

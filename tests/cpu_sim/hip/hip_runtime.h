@@ -30,4 +30,7 @@ constexpr int warpSize = 1;
 inline unsigned long long __ballot(int pred) { return pred ? 1ull : 0ull; }
 inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 inline int __ffsll(unsigned long long x) { return __builtin_ffsll((long long)x); }
+inline unsigned long long __builtin_amdgcn_read_exec() { return 1ull; }
+inline unsigned __lane_id() { return 0u; }
+inline int __builtin_amdgcn_readlane(int v, int) { return v; }
 using std::min;
