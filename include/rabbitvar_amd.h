@@ -266,6 +266,12 @@ rv_status rv_debug_pileup(rv_engine* e, const rv_region* region, const rv_reads*
                           char** text, size_t* len);
 void rv_free(void* p);
 
+/* Diagnostics: the device's two-sided Fisher exact test (the FP64 step of the scoring kernels) on n
+ * host 2x2 tables tables[4i..4i+3] = (n11, n12, n21, n22); out[3i..3i+2] = (two-sided p, left
+ * tail, right tail).  Replaces htslib's kt_fisher_exact as the reference calls it
+ * (src/modes/simpleMode.cpp:96-108, src/modes/somaticMode.cpp:130-149). */
+rv_status rv_debug_fisher(rv_engine* e, const int32_t* tables, size_t n, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -188,8 +188,90 @@ static int fisher_table(const std::string& in, const std::string& out) {
     return 0;
 }
 
+namespace orc {
+bool modify_cigar_record(const RefMap& ref, int& position, std::vector<uint32_t>& rec, int n_orig,
+                         int& base, int& n_new, const std::string& seq, const uint8_t* qual, int lseq,
+                         int indelLen);
+std::string cigar_to_text(const uint32_t* w, int n);
+}  // namespace orc
+
+// --cigar-kat IN OUT: the CIGAR rewrite on single reads, for known-answer tests.  IN: a line
+// "REF SEQ" (the reference window, keys 1..len), then one read per line "POS CIGAR SEQ Q" (POS
+// 1-based alignment start, Q one Phred value for every base).  OUT, per read:
+// "changed position cigar", the CIGAR parseCigar then walks (Q may also be a comma-separated
+// list, one value per base): modifyCigar's result read through
+// the record's own CIGAR pointer with the returned operation count (parseCigar.cpp:548-574),
+// then cleanupCigar (parseCigar.cpp:575, :1858-1892).
+static int cigar_kat(const std::string& in, const std::string& out) {
+    std::ifstream f(in);
+    std::ofstream o(out);
+    std::string tag, refseq;
+    f >> tag >> refseq;
+    orc::RefMap ref;
+    ref.start = 1;
+    ref.end = (int)refseq.size();
+    ref.win = refseq;
+    int pos;
+    std::string cig, seq, qtok;
+    static const std::string codes = "=ACMGRSVTWYHKDBN";
+    while (f >> pos >> cig >> seq >> qtok) {
+        std::vector<uint32_t> w(64);
+        int n = 0;
+        {
+            std::string s = cig;
+            while (!s.empty()) {
+                size_t k = 0;
+                long v = std::stol(s, &k);
+                const std::string ops = "MIDNSHP=X";
+                w[(size_t)n++] = ((uint32_t)v << 4) | (uint32_t)ops.find(s[k]);
+                s = s.substr(k + 1);
+            }
+        }
+        const int lseq = (int)seq.size();
+        std::vector<uint32_t> rec((size_t)n + (size_t)(lseq + 8) / 8 + 1, 0u);
+        for (int i = 0; i < n; i++) rec[(size_t)i] = w[(size_t)i];
+        for (int i = 0; i < lseq; i++) {
+            size_t cpos = codes.find(seq[(size_t)i]);
+            uint32_t code = cpos == std::string::npos ? 15u : (uint32_t)cpos;
+            int b = i / 2;
+            uint32_t v = (i % 2 == 0) ? (code << 4) : code;
+            rec[(size_t)n + (size_t)(b / 4)] |= v << (8 * (b % 4));
+        }
+        std::vector<uint8_t> qual((size_t)lseq, 0);
+        if (qtok.find(',') == std::string::npos) {
+            for (auto& x : qual) x = (uint8_t)std::stoi(qtok);
+        } else {
+            std::stringstream qs(qtok);
+            std::string one;
+            for (size_t i = 0; i < qual.size() && std::getline(qs, one, ','); i++) qual[i] = (uint8_t)std::stoi(one);
+        }
+        int idlen = 0;
+        for (int i = 0; i < n; i++)
+            if ((w[(size_t)i] & 15) == 1 || (w[(size_t)i] & 15) == 2) idlen += (int)(w[(size_t)i] >> 4);
+        int base = 0, nn = 0, p = pos, n_cigar = n;
+        bool changed = orc::modify_cigar_record(ref, p, rec, n, base, nn, seq, qual.data(), lseq, idlen);
+        if (changed) n_cigar = nn;
+        else p = pos;
+        auto op = [&](int i) { return (int)(rec[(size_t)i] & 15); };
+        auto m3 = [](int o) { return o == 0 || o == 7 || o == 8; };
+        for (int i = 0; i < n_cigar; i++) {
+            if (op(i) == 1) rec[(size_t)i] = (rec[(size_t)i] & ~15u) | 4u;
+            else if (op(i) == 5) continue;
+            else if (m3(op(i))) break;
+        }
+        for (int i = n_cigar - 1; i >= 0; i--) {
+            if (op(i) == 1) rec[(size_t)i] = (rec[(size_t)i] & ~15u) | 4u;
+            else if (op(i) == 5) continue;
+            else if (m3(op(i))) break;
+        }
+        o << (changed ? 1 : 0) << " " << p << " " << orc::cigar_to_text(rec.data(), n_cigar) << "\n";
+    }
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc == 4 && std::string(argv[1]) == "--fisher-table") return fisher_table(argv[2], argv[3]);
+    if (argc == 4 && std::string(argv[1]) == "--cigar-kat") return cigar_kat(argv[2], argv[3]);
     Config c;
     std::string fasta, bam, regionArg, bed, out = "./out.txt", dumpP, dumpR, dumpO;
     int colC = -1, colS = -1, colE = -1, colG = -1;

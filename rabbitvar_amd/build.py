@@ -1,7 +1,6 @@
 """Builds the native parts of rabbitvar_amd in-tree.
 
   rabbitvar_amd/_build/librabbitvar_amd.so   HIP kernels for gfx950 + host engine + C ABI
-  rabbitvar_amd/_build/rabbitvar-amd          command-line driver (same options as RabbitVar)
   rabbitvar_amd/_build/rvsynth                seeded synthetic input generator
 
 Run:  python -m rabbitvar_amd.build      (or import and call build())
@@ -51,12 +50,6 @@ def build(verbose=True, jobs=None):
     lib = os.path.join(OUT, "librabbitvar_amd.so")
     _run([HIPCC] + HIP_FLAGS + ["-shared", "-o", lib + ".tmp"] + objs + ["-lz"])
     os.replace(lib + ".tmp", lib)
-    cli_src = os.path.join(CSRC, "rv_main.cpp")
-    if os.path.exists(cli_src):
-        exe = os.path.join(OUT, "rabbitvar-amd")
-        _run([HIPCC, "-x", "hip"] + HIP_FLAGS + inc + [cli_src, "-o", exe + ".tmp", "-L", OUT, "-lrabbitvar_amd",
-                                                       "-Wl,-rpath,$ORIGIN"])
-        os.replace(exe + ".tmp", exe)
     gxx = shutil.which("g++") or "g++"
     synth = os.path.join(OUT, "rvsynth")
     _run([gxx, "-O2", "-std=c++17", "-o", synth + ".tmp", os.path.join(HERE, "synth", "rv_synth.cpp"), "-lz"])

@@ -179,6 +179,12 @@ rv_status rv_debug_pileup(rv_engine* e, const rv_region* region, const rv_reads*
 
 void rv_free(void* p) { std::free(p); }
 
+rv_status rv_debug_fisher(rv_engine* e, const int32_t* tables, size_t n, double* out) {
+    if (!e || (n && (!tables || !out)) || n > (size_t)0x7fffffff / 4) return RV_ERR_ARG;
+    if (n == 0) return RV_OK;
+    return guarded(e, [&] { return e->e->fisher_tables(tables, (int)n, out); });
+}
+
 }  // extern "C"
 
 // ------------------------------------------------------------------ engine pool (region lists)

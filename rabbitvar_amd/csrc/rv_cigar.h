@@ -15,11 +15,20 @@ namespace rv {
 
 typedef FStr<CIGTXT> CText;
 
+// Capture slots: the largest rule (threeIndelsPattern's core, "nMnxnMnxnMnxnM") captures 10.
+constexpr int RX_MAXCAP = 12;
+
 struct RxMatch {
     int start, end;
     int ncap;
-    int cs[8], ce[8];
+    int cs[RX_MAXCAP], ce[RX_MAXCAP];
 };
+
+// number of captures a pattern makes (every code letter captures)
+constexpr int rx_ncap(const char* p) {
+    return *p == 0 ? 0 : ((*p == 'n' || *p == 'd' || *p == 'x' || *p == '!' || *p == '?') ? 1 : 0) + rx_ncap(p + 1);
+}
+static_assert(rx_ncap("nMnxnMnxnMnxnM") <= RX_MAXCAP, "capture slots");
 
 RV_HD bool is_digit(char c) { return c >= '0' && c <= '9'; }
 
@@ -696,7 +705,8 @@ RV_HD int op_code_of(char c) {
 // Applies the rewrite to an emulated record `rec` (CIGAR words followed by packed SEQ words, as the
 // reference's in-place rewrite and its reads past the CIGAR see them).  Returns true when the
 // CIGAR text changed and was written back; n_new/position are updated in that case.  `flags`
-// gains bit1 on text overflow and bit2 when a negative length restored the original.
+// gains bit1 when the text or the words would exceed their capacity (the caller refuses the read)
+// and bit2 when a negative length restored the original.
 RV_HD bool modify_cigar_record(const RefView& rf, const ReadView& rd, int& position, uint32_t* rec, int rec_cap,
                                int n_orig, int indel_len, int& n_new, int& flags) {
     int base = 0;
@@ -755,17 +765,18 @@ RV_HD bool modify_cigar_record(const RefView& rf, const ReadView& rd, int& posit
     }
     if (!ok) {
         flags |= 4;
-        for (int t = 0; t < n_orig && base + t < rec_cap; t++) {
-            // restore from the original text (its words equal the original record words)
-        }
-        // re-encode the original words at rec[base..]
+        // re-encode the original text's words at rec[base..], as cigarstr_2cigar(originalCigar) does
         int kk = 0, ii = 0;
         while (ii < original.n) {
             long long v = 0;
             int j = ii;
             while (j < original.n && is_digit(original.c[j])) v = v * 10 + (original.c[j++] - '0');
             char opc = j < original.n ? original.c[j] : '\0';
-            if (base + kk < rec_cap) rec[base + kk] = ((uint32_t)v << 4) | (uint32_t)op_code_of(opc);
+            if (base + kk >= rec_cap) {
+                flags |= 2;
+                return false;
+            }
+            rec[base + kk] = ((uint32_t)v << 4) | (uint32_t)op_code_of(opc);
             kk++;
             ii = j + 1;
         }

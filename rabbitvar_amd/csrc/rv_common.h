@@ -170,6 +170,8 @@ enum : int {
     RD_NOWALK = 3,     // parsed but returns before the CIGAR walk (both-end clips, -M, supplementary)
 };
 constexpr int MAXC = 48;        // CIGAR words kept per read (rewritten CIGAR + record tail emulation)
+constexpr int MAXC_IN = MAXC - 4;  // longest input CIGAR accepted: the rewrite adds at most 2 ops and
+                                   // may read one word past the CIGAR
 constexpr int CIGTXT = 192;     // CIGAR text capacity for the rewrite rules
 
 struct PrepRec {
@@ -252,6 +254,7 @@ struct WalkCounters {
     unsigned long long walk_errors;  // reads whose walk hit an out-of-range access
     int max_read_length;
     int ttref;                       // reads reaching the insertion-after-clip branch
+    unsigned long long capacity;     // reads whose CIGAR or its rewrite exceeds MAXC words / CIGTXT chars
 };
 
 }  // namespace rv

@@ -84,7 +84,8 @@ Engine::~Engine() {
                      &b_cx_pos,    &b_cx_off,  &b_cx,     &b_cx_arena,   &b_cx_cov,   &b_cx_key, &b_out,
                      &b_out_arena, &b_score_ctr, &b_status, &b_keys, &b_keys2, &b_kpos, &b_kpos2, &b_sort_tmp,
                      &b_kcount, &b_args, &b_kpos3, &b_nopass, &b_rows2, &b_want, &b_dup, &b_drop,
-                     &b_scr_cand, &b_scr_pos, &b_scr_cum, &b_scr_feat, &b_scr_score, &b_scr_params};
+                     &b_scr_cand, &b_scr_pos, &b_scr_cum, &b_scr_feat, &b_scr_score, &b_scr_params,
+                     &b_fisher_in, &b_fisher_out};
     // teardown: nothing to report a failure to
     for (auto* b : all)
         if (b->p) (void)hipFree(b->p);
@@ -312,6 +313,10 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
         unsigned long long aused = hcnt[2], cused = hcnt[3];
         bool over = counts[0] > max_events || aused > arena_cap || counts[1] > max_clips || cused > clip_cap ||
                     counts[2] > max_anchors;
+        if (hc.capacity)
+            return fail(RV_ERR_CAPACITY, std::to_string(hc.capacity) + " read(s) have a CIGAR longer than " +
+                                             std::to_string(MAXC_IN) + " operations (or a rewrite beyond " +
+                                             std::to_string(CIGTXT) + " characters); region not processed");
         if (!over && hc.overflow == 0) {
             if (hooks && hooks->after_walk) hooks->after_walk();
             // ---- copy the ordered records back and reduce them
@@ -404,6 +409,17 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
         max_anchors = std::max<unsigned>(max_anchors, counts[2]) * 2;
     }
     return fail(RV_ERR_CAPACITY, "walk record buffers kept overflowing");
+}
+
+rv_status Engine::fisher_tables(const int32_t* tab, int n, double* out) {
+    int* din = (int*)grow(b_fisher_in, sizeof(int) * 4 * (size_t)n);
+    double* dout = (double*)grow(b_fisher_out, sizeof(double) * 3 * (size_t)n);
+    if (!din || !dout) return fail(RV_ERR_OOM, "device buffers for the Fisher tables failed");
+    RV_HIP(hipMemcpyAsync(din, tab, sizeof(int) * 4 * (size_t)n, hipMemcpyHostToDevice, stream));
+    RV_HIP(launch_fisher_table(din, n, dout, stream));
+    RV_HIP(hipMemcpyAsync(out, dout, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost, stream));
+    RV_HIP(hipStreamSynchronize(stream));
+    return RV_OK;
 }
 
 rv_status Engine::key_order(int n_reads, std::vector<int32_t>& created, std::vector<int32_t>& slot_sorted) {

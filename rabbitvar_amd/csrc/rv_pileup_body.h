@@ -164,10 +164,18 @@ __device__ inline void prep_one(int64_t r, const ReadsDev& R, const Window& W, c
     p.nm_eff = tnm;
     // emulated record: CIGAR words, then the packed SEQ bytes as little-endian words
     const uint8_t* seq = R.bases + so;
+    // a CIGAR longer than the record slot is refused before anything is written: the region then
+    // fails with a capacity error (the host checks the counter) instead of walking a cut record
+    if (ncig > MAXC_IN) {
+        p.status = RD_NOWALK;
+        p.flags |= 2;
+        atomicAdd(&ctr->capacity, 1ull);
+        prep[r] = p;
+        return;
+    }
     int seq_words = ((lseq + 1) / 2 + 3) / 4;
     int nrec = ncig + seq_words;
     if (nrec > MAXC) nrec = MAXC;
-    if (ncig + 4 > MAXC) p.flags |= 2;
     for (int i = 0; i < nrec; i++) rec[i] = i < ncig ? R.cigar[co + i] : packed_seq_word(seq, lseq, i - ncig);
     for (int i = nrec; i < MAXC; i++) rec[i] = 0;
     int n_cigar = ncig;
@@ -181,6 +189,12 @@ __device__ inline void prep_one(int64_t r, const ReadsDev& R, const Window& W, c
             p.flags |= 1;
         }
         p.flags |= fl;
+        if (fl & 2) {  // rewrite text or words beyond capacity: refused as above
+            p.status = RD_NOWALK;
+            atomicAdd(&ctr->capacity, 1ull);
+            prep[r] = p;
+            return;
+        }
     }
     // cleanupCigar: leading/trailing I become S until the first M/=/X
     {

@@ -36,11 +36,12 @@ struct OutRec {
     int32_t tpc, poscov, rfc, rrc, fwd, rev, shift3, msint, hicnt, hicov, pstd, qstd, vartype;
     int32_t nfields;  // 5 for a pileup line without a reference variant
     double freq, pmean, qmean, mq, hlr, hifreq, extrafreq, msi, nm, fisher_p, fisher_ratio;
-    uint32_t str_off;
+    uint64_t str_off;  // byte offset into the string arena (the arena can exceed 4 GiB)
     uint16_t len[7];  // refallele, varallele, genotype, bias, leftseq, rightseq, descriptor
     int32_t kind;     // OUT_VARIANT, or (somatic passes) OUT_REF / OUT_REF_ONLY / OUT_VARIANT_RAW
     int32_t start_rank;  // host: the position's rank in the printer's order
 };
+static_assert(sizeof(OutRec) == 192, "OutRec layout (records are copied to the host as they are)");
 
 // OutRec kinds.  Simple mode emits printed variants only.  The somatic passes emit a position's
 // Vars: every non-reference variant in CMP_VARI order plus its reference variant (OUT_REF, as

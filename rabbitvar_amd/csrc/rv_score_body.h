@@ -420,7 +420,7 @@ __device__ inline bool emit_out(const ScoreIn& S, OutRec& o, const char* const p
         for (int i = 0; i < pn[k]; i++) *w++ = pc[k][i];
         o.len[k] = (uint16_t)pn[k];
     }
-    o.str_off = (uint32_t)off;
+    o.str_off = (uint64_t)off;
     S.out[oi] = o;
     return true;
 }

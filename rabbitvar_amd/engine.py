@@ -113,6 +113,17 @@ class Engine:
         self.lib.rv_free(buf)
         return text
 
+    def fisher(self, tables):
+        """Device two-sided Fisher test on an (n, 4) integer array of 2x2 tables (n11 n12 n21 n22);
+        returns an (n, 3) float64 array of (two-sided p, left tail, right tail)."""
+        t = np.ascontiguousarray(tables, np.int32).reshape(-1, 4)
+        out = np.zeros((len(t), 3), np.float64)
+        st = self.lib.rv_debug_fisher(self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), len(t),
+                                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        if st != 0:
+            raise self._error(st)
+        return out
+
     def run_region(self, chrom, start, end, reads, contig_name, contig_bases, gene=None):
         region, r, contig = self._args(chrom, start, end, reads, contig_name, contig_bases, gene)
         cs = ctypes.c_void_p()

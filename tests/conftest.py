@@ -23,6 +23,9 @@ CASES = {
     # tumor/normal pair: 200 kb, tumor 60x, normal 40x, somatic SNVs/indels at 5e-4 with VAF 0.05-0.4
     "som": ["--seed", "5", "--len", "200000", "--depth", "60", "--normal-depth", "40", "--somatic-rate", "0.0005",
             "--vaf", "0.05", "0.4", "--indel-rate", "0.0002"],
+    # deep tumor/normal pair: 40 kb, tumor 250x, normal 120x, somatic variants at VAF 0.01-0.2, 2% duplicates
+    "deep": ["--seed", "13", "--len", "40000", "--depth", "250", "--normal-depth", "120", "--somatic-rate", "0.001",
+             "--vaf", "0.01", "0.2", "--indel-rate", "0.0005", "--dup-rate", "0.02"],
     # tumor/normal pair with the dense edge-case mix above and a thin normal
     "som_edge": ["--seed", "7", "--len", "60000", "--depth", "40", "--normal-depth", "30", "--indel-rate", "0.002",
                  "--indel-max", "40", "--snv-rate", "0.004", "--long-dels", "6", "--repeat-frac", "0.05",

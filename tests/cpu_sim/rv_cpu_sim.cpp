@@ -279,6 +279,22 @@ int main(int argc, char** argv) {
     }
     for (int64_t r = 0; r < m; r++)
         prep_one(r, R, W, cfg, prep.data(), cg.data(), &ctr, &span, dup.empty() ? nullptr : dup.data());
+    if (const char* pd = getenv("RV_SIM_PREP_DUMP")) {
+        // per read: status, position, CIGAR word count and the words of its record slot
+        FILE* fp = fopen(pd, "w");
+        for (int64_t r = 0; r < m; r++) {
+            const PrepRec& p = prep[(size_t)r];
+            fprintf(fp, "%d %d %d", p.status, p.position, p.ncig);
+            if (p.status == RD_PARSE)
+                for (int i = 0; i < p.ncig; i++) fprintf(fp, " %u", cg[(size_t)r * MAXC + (size_t)i]);
+            fprintf(fp, "\n");
+        }
+        fclose(fp);
+    }
+    if (ctr.capacity) {
+        fprintf(stderr, "capacity: %llu read(s) refused\n", (unsigned long long)ctr.capacity);
+        return 4;
+    }
     const int pad = 2 * std::max(span, 64) + 8;
     W.dense_lo = rs - pad;
     W.dense_n = (re + pad) - W.dense_lo + 1;

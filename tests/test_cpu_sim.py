@@ -20,8 +20,10 @@ def sim():
     srcs = [os.path.join(SIM_DIR, "rv_cpu_sim.cpp")] + [os.path.join(CSRC, f) for f in
                                                         ("rv_host.cpp", "rv_realign.cpp", "rv_format.cpp",
                                                          "rv_order.cpp")]
+    tmp = f"{SIM}.{os.getpid()}"
     subprocess.run(["g++", "-O2", "-std=c++17", "-I", SIM_DIR, "-I", CSRC, "-I", os.path.join(ROOT, "include"), "-o",
-                    SIM] + srcs, check=True)
+                    tmp] + srcs, check=True)
+    os.replace(tmp, SIM)  # atomic: parallel sessions never run a half-written binary
     return SIM
 
 
@@ -155,3 +157,119 @@ def test_fixed_point_formatting_matches_printf(sim):
     exactly what printf prints, ties to even included."""
     r = subprocess.run([sim, "--fmt-check", "400000", "7"], capture_output=True, text=True, check=True)
     assert r.stdout.strip() == "0", r.stderr
+
+
+@pytest.fixture(scope="session")
+def sim_asan():
+    """The same harness built with AddressSanitizer (host code only)."""
+    out = os.path.join(SIM_DIR, "_build", "rv_cpu_sim_asan")
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    srcs = [os.path.join(SIM_DIR, "rv_cpu_sim.cpp")] + [os.path.join(CSRC, f) for f in
+                                                        ("rv_host.cpp", "rv_realign.cpp", "rv_format.cpp",
+                                                         "rv_order.cpp")]
+    tmp = f"{out}.{os.getpid()}"
+    subprocess.run(["g++", "-O1", "-g", "-fsanitize=address", "-fno-omit-frame-pointer", "-std=c++17", "-I", SIM_DIR,
+                    "-I", CSRC, "-I", os.path.join(ROOT, "include"), "-o", tmp] + srcs, check=True)
+    os.replace(tmp, out)
+    return out
+
+
+def _long_cigar_records(ref, long_at, n=12, start=200, rlen=100):
+    """n reads at start+1; those at indexes `long_at` carry 59 CIGAR operations (29 x 2M1D, then 42M)."""
+    from rabbitvar_amd import batch_from_records
+
+    recs = []
+    for i in range(n):
+        if i in long_at:
+            cig = [(0, 2), (2, 1)] * 29 + [(0, rlen - 58)]
+        else:
+            cig = [(0, rlen)]
+        seq, rp = [], start
+        for op, ln in cig:
+            if op == 0:
+                seq.append(ref[rp:rp + ln])
+                rp += ln
+            else:
+                rp += ln
+        seq = "".join(seq)
+        nm = sum(ln for op, ln in cig if op == 2)
+        recs.append({"pos0": start, "flag": 16 if i % 2 else 0, "mapq": 60, "nm": nm, "seq": seq,
+                     "qual": [30] * len(seq), "cigar": cig})
+    return batch_from_records(recs)
+
+
+@pytest.mark.parametrize("long_at", [(5,), (11,), (5, 11)])
+def test_long_cigar_reads_are_refused(sim_asan, tmp_path, long_at):
+    """A read whose CIGAR has more operations than its 48-word record slot allows (here 59, in the
+    middle of the batch and as its last read) is refused before the slot is written, and the
+    region fails with a capacity error (exit 4) instead of walking a cut record.  Under ASan no
+    access leaves the record buffer, and every other read's prep record (status, position,
+    rewritten CIGAR words) equals the one it gets in a batch without the long reads."""
+    import handbuilt as hb
+    from rabbitvar_amd import write_rvb
+
+    ref = hb.reference()
+    fa = str(tmp_path / "ref.fa")
+    with open(fa, "w") as f:
+        f.write(">chrS\n" + ref + "\n")
+    with_long, without = str(tmp_path / "a.rvb"), str(tmp_path / "b.rvb")
+    write_rvb(with_long, _long_cigar_records(ref, long_at))
+    write_rvb(without, _long_cigar_records(ref, (), n=12 - len(long_at)))
+    da, db = str(tmp_path / "a.txt"), str(tmp_path / "b.txt")
+    ra = subprocess.run([sim_asan, fa, with_long, "1", "600", "1", "1", str(tmp_path / "oa.txt")],
+                        env=dict(os.environ, RV_SIM_PREP_DUMP=da), capture_output=True, text=True)
+    assert ra.returncode == 4, ra.stderr
+    assert f"capacity: {len(long_at)} read(s) refused" in ra.stderr
+    assert "AddressSanitizer" not in ra.stderr
+    rb = subprocess.run([sim_asan, fa, without, "1", "600", "1", "1", str(tmp_path / "ob.txt")],
+                        env=dict(os.environ, RV_SIM_PREP_DUMP=db), capture_output=True, text=True)
+    assert rb.returncode == 0, rb.stderr
+    a = open(da).read().splitlines()
+    b = open(db).read().splitlines()
+    kept = [ln for i, ln in enumerate(a) if i not in long_at]
+    assert kept == b
+    for i in long_at:
+        assert a[i].split()[0] == "3"  # RD_NOWALK: never walked
+
+
+def test_cigar_rewrite_known_answers_engine(sim, tmp_path):
+    """The engine's CIGAR rewrite (rv_cigar.h through prep_one, as k_prep runs it) on the hand-worked
+    known-answer reads of test_oracle_cigar_kat.py: the position and the CIGAR it leaves for the walk
+    equal the hand-derived answers."""
+    import test_oracle_cigar_kat as kat
+    from rabbitvar_amd import batch_from_records, write_rvb
+
+    names = list(kat.CASES)
+    rows = [kat.CASES[n]() for n in names]
+    recs = []
+    for P, cig, seq, q, _ in rows:
+        ops, num = [], ""
+        for ch in cig:
+            if ch.isdigit():
+                num += ch
+            else:
+                ops.append(("MIDNSHP=X".index(ch), int(num)))
+                num = ""
+        quals = [int(x) for x in q.split(",")] if isinstance(q, str) else [q] * len(seq)
+        idlen = sum(n for o, n in ops if o in (1, 2))
+        recs.append({"pos0": P - 1, "flag": 0, "mapq": 60, "nm": idlen, "seq": seq, "qual": quals, "cigar": ops})
+    fa = str(tmp_path / "ref.fa")
+    with open(fa, "w") as f:
+        f.write(">chrS\n" + kat.REF + "\n")
+    rvb = str(tmp_path / "kat.rvb")
+    write_rvb(rvb, batch_from_records(recs))
+    dump = str(tmp_path / "prep.txt")
+    subprocess.run([sim, fa, rvb, "1", "600", "1", "1", str(tmp_path / "o.txt")], check=True, capture_output=True,
+                   env=dict(os.environ, RV_SIM_PREP_DUMP=dump))
+    # the harness keeps the region's records in iterator order (sorted input is not required here:
+    # every KAT read overlaps the region)
+    lines = open(dump).read().splitlines()
+    assert len(lines) == len(rows)
+    bad = []
+    for name, (P, cig, seq, q, want), ln in zip(names, rows, lines):
+        f = [int(x) for x in ln.split()]
+        status, pos, n = f[0], f[1], f[2]
+        text = "".join(f"{w >> 4}{'MIDNSHP=XB'[w & 15]}" for w in f[3:3 + n])
+        if status != 0 or (pos, text) != want[1:]:
+            bad.append((name, P, cig, status, pos, text, want))
+    assert not bad, bad

@@ -32,9 +32,15 @@ def test_two_sided_matches_exact_rationals(tools, golden, tmp_path):
     want = np.array([g["two_f"] for g in golden])
     near = np.array([g["near_tie"] for g in golden])
     rel = np.abs(got - want) / np.maximum(want, 1e-300)
-    # Lanczos log-gamma and the incremental recurrence: ~1e-13 relative at these sizes
-    assert np.all(rel[~near] < 1e-10), (np.max(rel[~near]), tables[int(np.argmax(np.where(near, 0, rel)))])
-    assert (~near).sum() > 2000
+    # htslib's method (Lanczos log-gamma, then a ratio recurrence) loses accuracy as the table total n
+    # grows: the log-probabilities it exponentiates are sums of log-gammas of size ~n ln n, each
+    # carrying a relative rounding error near 1e-16.  Below n = 500 it stays within 1e-12 relative
+    # of the exact rational value; on the golden set's largest tables (n up to ~2000) within 5e-12.
+    n = np.array([sum(t) for t in tables])
+    small = ~near & (n < 500)
+    assert np.all(rel[small] < 1e-12), (np.max(rel[small]), tables[int(np.argmax(np.where(small, rel, 0)))])
+    assert np.all(rel[~near] < 5e-12), (np.max(rel[~near]), tables[int(np.argmax(np.where(near, 0, rel)))])
+    assert small.sum() > 2000 and (~near & (n >= 500)).sum() > 100
 
 
 def test_properties(tools, tmp_path):
