@@ -14,6 +14,9 @@
 // --dump-order writes the iteration order of nonInsertionVariants after the pileup and after
 // realignment, and of alignedVariants, one line each per region.
 #include <algorithm>
+#include <atomic>
+#include <mutex>
+#include <thread>
 #include <chrono>
 #include <cstdio>
 #include <cstring>
@@ -193,6 +196,7 @@ bool modify_cigar_record(const RefMap& ref, int& position, std::vector<uint32_t>
                          int& base, int& n_new, const std::string& seq, const uint8_t* qual, int lseq,
                          int indelLen);
 std::string cigar_to_text(const uint32_t* w, int n);
+std::string findmsi_kat(const std::string& t1, const std::string& t2, const std::string& left);
 }  // namespace orc
 
 // --cigar-kat IN OUT: the CIGAR rewrite on single reads, for known-answer tests.  IN: a line
@@ -272,7 +276,14 @@ static int cigar_kat(const std::string& in, const std::string& out) {
 int main(int argc, char** argv) {
     if (argc == 4 && std::string(argv[1]) == "--fisher-table") return fisher_table(argv[2], argv[3]);
     if (argc == 4 && std::string(argv[1]) == "--cigar-kat") return cigar_kat(argv[2], argv[3]);
+    // --findmsi TSEQ1 TSEQ2 LEFT ("-" for an empty string): findMSI's "msi shift3 msint"
+    if (argc == 5 && std::string(argv[1]) == "--findmsi") {
+        auto arg = [&](int i) { return std::string(argv[i]) == "-" ? std::string() : std::string(argv[i]); };
+        printf("%s\n", orc::findmsi_kat(arg(2), arg(3), arg(4)).c_str());
+        return 0;
+    }
     Config c;
+    int threads = 1;
     std::string fasta, bam, regionArg, bed, out = "./out.txt", dumpP, dumpR, dumpO;
     int colC = -1, colS = -1, colE = -1, colG = -1;
     bool timing = false;
@@ -322,7 +333,7 @@ int main(int argc, char** argv) {
         else if (a == "--dump-realigned") dumpR = nxt();
         else if (a == "--dump-order") dumpO = nxt();
         else if (a == "--time") timing = true;
-        else if (a == "--th") nxt();
+        else if (a == "--th") threads = std::max(1, std::stoi(nxt()));
         else {
             std::cerr << "unknown option " << a << "\n";
             return 2;
@@ -404,10 +415,16 @@ int main(int argc, char** argv) {
         dord << '\n';
     };
     auto t0 = std::chrono::steady_clock::now();
-    long nreads = 0;
+    std::atomic<long> nreads{0};
     double st[5] = {0, 0, 0, 0, 0};
+    std::mutex st_mu;
     auto now = [] { return std::chrono::steady_clock::now(); };
-    for (const Region& R : regions) {
+    // --th N: regions on N threads, as the reference's OpenMP region loop (simpleMode.cpp:320);
+    // the texts are written in region order.  Dumps and -Z (one process-wide rand() sequence) keep
+    // one thread.
+    if (dp.is_open() || dr.is_open() || dord.is_open() || c.isDownsampling) threads = 1;
+    std::vector<std::string> texts(regions.size());
+    auto region_text = [&](const Region& R, std::string& text_out) {
         // region iterator: records overlapping [R.start-1, R.end)
         auto region_reads = [&](const std::vector<Read>& rs) {
             std::vector<const Read*> out;
@@ -442,8 +459,8 @@ int main(int argc, char** argv) {
                 realign_region(c, R, ref, chrLen, rr, P);
                 score_region(c, R, ref, chrLen, P, k == 0 ? A1 : A2);
             }
-            of << format_somatic(c, R, A1, A2);
-            continue;
+            text_out = format_somatic(c, R, A1, A2);
+            return;
         }
         Counters ctr;
         std::vector<const Read*> filtered;
@@ -466,18 +483,37 @@ int main(int argc, char** argv) {
         std::string text = format_simple(c, R, A);
         if (dord.is_open()) dump_keys("vars", A.vars.keys_in_order());
         auto a6 = now();
-        st[0] += std::chrono::duration<double>(a1 - a0).count();
-        st[1] += std::chrono::duration<double>(a3 - a2).count();
-        st[2] += std::chrono::duration<double>(a5 - a4).count();
-        st[3] += std::chrono::duration<double>(a6 - a5).count();
-        of << text;
-        if (ctr.exceptions) std::cerr << "[rvcpu] " << ctr.exceptions << " reads raised during parse\n";
+        {
+            std::lock_guard<std::mutex> lk(st_mu);
+            st[0] += std::chrono::duration<double>(a1 - a0).count();
+            st[1] += std::chrono::duration<double>(a3 - a2).count();
+            st[2] += std::chrono::duration<double>(a5 - a4).count();
+            st[3] += std::chrono::duration<double>(a6 - a5).count();
+            if (ctr.exceptions) std::cerr << "[rvcpu] " << ctr.exceptions << " reads raised during parse\n";
+        }
+        text_out = std::move(text);
+    };
+    if (threads <= 1) {
+        for (size_t i = 0; i < regions.size(); i++) {
+            region_text(regions[i], texts[i]);
+            of << texts[i];
+            texts[i].clear();
+        }
+    } else {
+        std::atomic<size_t> next{0};
+        std::vector<std::thread> pool;
+        for (int t = 0; t < threads; t++)
+            pool.emplace_back([&] {
+                for (size_t i; (i = next.fetch_add(1)) < regions.size();) region_text(regions[i], texts[i]);
+            });
+        for (auto& th : pool) th.join();
+        for (auto& t : texts) of << t;
     }
     auto t1 = std::chrono::steady_clock::now();
     double sec = std::chrono::duration<double>(t1 - t0).count();
     if (timing)
         printf("{\"reads\": %ld, \"seconds\": %.6f, \"reads_per_s\": %.3f, \"pileup_s\": %.6f, \"realign_s\": %.6f, "
                "\"score_s\": %.6f, \"output_s\": %.6f}\n",
-               nreads, sec, nreads / sec, st[0], st[1], st[2], st[3]);
+               nreads.load(), sec, nreads.load() / sec, st[0], st[1], st[2], st[3]);
     return 0;
 }

@@ -553,6 +553,14 @@ struct Scorer {
 
 }  // namespace
 
+// findMSI's result as "msi shift3 msint", for known-answer tests (rvcpu --findmsi)
+std::string findmsi_kat(const std::string& t1, const std::string& t2, const std::string& left) {
+    MSI m = findMSI(t1, t2, left);
+    char b[64];
+    snprintf(b, sizeof b, "%.17g %d ", m.msi, m.shift3);
+    return std::string(b) + m.msint;
+}
+
 void score_region(const Config& c, const Region& r, RefMap& ref, int chrLen, Pileup& P, Aligned& A) {
     Scorer s{c, r, ref, chrLen, P, A};
     s.process();

@@ -52,7 +52,7 @@ def build(verbose=True, jobs=None):
     os.replace(lib + ".tmp", lib)
     gxx = shutil.which("g++") or "g++"
     synth = os.path.join(OUT, "rvsynth")
-    _run([gxx, "-O2", "-std=c++17", "-o", synth + ".tmp", os.path.join(HERE, "synth", "rv_synth.cpp"), "-lz"])
+    _run([gxx, "-O2", "-std=c++17", "-pthread", "-o", synth + ".tmp", os.path.join(HERE, "synth", "rv_synth.cpp"), "-lz"])
     os.replace(synth + ".tmp", synth)
     return lib
 

@@ -2,6 +2,7 @@
 #include "rv_engine.h"
 
 #include "rv_order.h"
+#include "rv_fisher.h"
 
 #include <algorithm>
 #include <chrono>
@@ -85,7 +86,7 @@ Engine::~Engine() {
                      &b_out_arena, &b_score_ctr, &b_status, &b_keys, &b_keys2, &b_kpos, &b_kpos2, &b_sort_tmp,
                      &b_kcount, &b_args, &b_kpos3, &b_nopass, &b_rows2, &b_want, &b_dup, &b_drop,
                      &b_scr_cand, &b_scr_pos, &b_scr_cum, &b_scr_feat, &b_scr_score, &b_scr_params,
-                     &b_fisher_in, &b_fisher_out};
+                     &b_fisher_in, &b_fisher_out, &b_lgamma};
     // teardown: nothing to report a failure to
     for (auto* b : all)
         if (b->p) (void)hipFree(b->p);
@@ -411,12 +412,30 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
     return fail(RV_ERR_CAPACITY, "walk record buffers kept overflowing");
 }
 
+const double* Engine::lgamma_table() {
+    if (b_lgamma.p && b_lgamma.cap >= sizeof(double) * FISHER_LGT_N) return (const double*)b_lgamma.p;
+    std::vector<double> t(FISHER_LGT_N);
+    fisher_lgamma_table(t.data(), FISHER_LGT_N);
+    double* d = (double*)grow(b_lgamma, sizeof(double) * FISHER_LGT_N);
+    if (!d) {
+        fail(RV_ERR_OOM, "device buffer for the log-gamma table failed");
+        return nullptr;
+    }
+    if (hipMemcpy(d, t.data(), sizeof(double) * FISHER_LGT_N, hipMemcpyHostToDevice) != hipSuccess) {
+        fail(RV_ERR_DEVICE, "upload of the log-gamma table failed");
+        return nullptr;
+    }
+    return d;
+}
+
 rv_status Engine::fisher_tables(const int32_t* tab, int n, double* out) {
+    const double* lg = lgamma_table();
+    if (!lg) return RV_ERR_DEVICE;
     int* din = (int*)grow(b_fisher_in, sizeof(int) * 4 * (size_t)n);
     double* dout = (double*)grow(b_fisher_out, sizeof(double) * 3 * (size_t)n);
     if (!din || !dout) return fail(RV_ERR_OOM, "device buffers for the Fisher tables failed");
     RV_HIP(hipMemcpyAsync(din, tab, sizeof(int) * 4 * (size_t)n, hipMemcpyHostToDevice, stream));
-    RV_HIP(launch_fisher_table(din, n, dout, stream));
+    RV_HIP(launch_fisher_table(din, n, dout, lg, FISHER_LGT_N, stream));
     RV_HIP(hipMemcpyAsync(out, dout, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost, stream));
     RV_HIP(hipStreamSynchronize(stream));
     return RV_OK;

@@ -45,13 +45,30 @@ RV_HD double kf_lgamma_d(double z) {
     x += 0.9999999999995183;
     return log(x) - 5.58106146679532777 - z + (z - 0.5) * log(z + 6.5);
 }
-RV_HD double lbinom_d(int n, int k) {
+// Log-gamma of the integers 1..n-1 as the host's C library computes kf_lgamma_d (table[m] =
+// kf_lgamma_d(m), table[0] unused).  The device's log differs from the host's by an ulp now and
+// then; kf_lgamma multiplies log(z + 6.5) by z - 0.5, so at a table total of a few thousand such
+// an ulp moves a p-value by ~1e-11 relative.  Scoring kernels look the values up in this
+// host-built table instead, so their p-values agree with the host's to the last few ulps (only
+// exp and IEEE arithmetic remain, and exp's error is not amplified).
+constexpr int FISHER_LGT_N = 1 << 17;
+inline void fisher_lgamma_table(double* table, int n) {
+    if (n > 0) table[0] = 0;
+    for (int m = 1; m < n; m++) table[m] = kf_lgamma_d((double)m);
+}
+struct LGTable {
+    const double* t;  // nullptr: compute
+    int n;
+};
+RV_HD double lgamma_int(int m, LGTable lg) { return (lg.t && m < lg.n) ? lg.t[m] : kf_lgamma_d(m); }
+RV_HD double lbinom_d(int n, int k, LGTable lg = LGTable{nullptr, 0}) {
     if (k == 0 || n == k) return 0;
-    return kf_lgamma_d(n + 1) - kf_lgamma_d(k + 1) - kf_lgamma_d(n - k + 1);
+    return lgamma_int(n + 1, lg) - lgamma_int(k + 1, lg) - lgamma_int(n - k + 1, lg);
 }
 struct HG {
     int n11, n1_, n_1, n;
     double p;
+    LGTable lg;
 };
 RV_HD double hg_acc(int n11, int n1_, int n_1, int n, HG& a) {
     if (n1_ || n_1 || n) {
@@ -74,14 +91,15 @@ RV_HD double hg_acc(int n11, int n1_, int n_1, int n, HG& a) {
         }
         a.n11 = n11;
     }
-    a.p = exp(lbinom_d(a.n1_, a.n11) + lbinom_d(a.n - a.n1_, a.n_1 - a.n11) - lbinom_d(a.n, a.n_1));
+    a.p = exp(lbinom_d(a.n1_, a.n11, a.lg) + lbinom_d(a.n - a.n1_, a.n_1 - a.n11, a.lg) - lbinom_d(a.n, a.n_1, a.lg));
     return a.p;
 }
 // two-sided p of the 2x2 table (n11 n12 / n21 n22); left/right tails optional
-RV_HD double fisher_two_sided(int n11, int n12, int n21, int n22, double* pl, double* pr) {
+RV_HD double fisher_two_sided(int n11, int n12, int n21, int n22, double* pl, double* pr,
+                              LGTable lg = LGTable{nullptr, 0}) {
     int i, j, max, min;
     double p, q, left, right, two;
-    HG aux{0, 0, 0, 0, 0};
+    HG aux{0, 0, 0, 0, 0, lg};
     int n1_ = n11 + n12, n_1 = n11 + n21, n = n11 + n12 + n21 + n22;
     max = (n_1 < n1_) ? n_1 : n1_;
     min = n1_ + n_1 - n;

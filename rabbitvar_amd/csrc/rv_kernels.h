@@ -28,6 +28,6 @@ hipError_t sort_key_pairs(void* temp, size_t& temp_bytes, const unsigned long lo
                           const int32_t* vin, int32_t* vout, int n, int end_bit, hipStream_t s);
 
 hipError_t launch_score(const ScoreIn& S, hipStream_t s);
-hipError_t launch_fisher_table(const int* tab, int n, double* out, hipStream_t s);
+hipError_t launch_fisher_table(const int* tab, int n, double* out, const double* lgt, int lgt_n, hipStream_t s);
 
 }  // namespace rv

@@ -2,6 +2,7 @@
 // (device) -> simple-mode text.  Replaces the body of one_region_run and the printer that follows
 // it (src/modes/simpleMode.cpp:18-64, :144-208; src/ToVarsBuilder.cpp:97-205).
 #include "rv_region.h"
+#include "rv_fisher.h"
 
 #include <algorithm>
 #include <chrono>
@@ -54,6 +55,9 @@ rv_status score_pass(Engine& e, ScoreIn& in, size_t n_status, std::vector<OutRec
     unsigned max_out = (unsigned)std::min<int64_t>(65536 + 8 * (int64_t)in.n_cx + in.dense_n / 16, 0x7fffffff);
     max_out = std::max(max_out, e.score_out_hint);
     unsigned long long arena_cap = std::max(160ull * max_out, e.score_arena_hint);
+    in.lgamma_tab = e.lgamma_table();
+    in.lgamma_n = FISHER_LGT_N;
+    if (!in.lgamma_tab) return RV_ERR_DEVICE;
     for (int attempt = 0; attempt < 3; attempt++) {
         in.out = (OutRec*)e.grow(e.b_out, sizeof(OutRec) * max_out);
         in.arena = (char*)e.grow(e.b_out_arena, (size_t)arena_cap);

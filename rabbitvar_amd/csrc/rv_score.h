@@ -81,6 +81,8 @@ struct ScoreIn {
     const uint8_t* want;        // SCORE_SOMATIC_REF: per status slot, emit the reference-only record
     unsigned long long* counters;  // [0] capacity overflows, [1] insertion side effects on p+1,
                                    // [2] sorts above 16 elements, [3] qstd ambiguous rows
+    const double* lgamma_tab;      // host-built kf_lgamma of the integers (fisher_lgamma_table)
+    int lgamma_n;
 };
 
 }  // namespace rv

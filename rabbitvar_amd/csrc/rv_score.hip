@@ -22,11 +22,12 @@ __global__ void __launch_bounds__(64) k_score_cx(ScoreIn S) {
     if (i < S.n_cx) score_position<1536, MAXV>(S, S.cx_pos[i], false, S.dense_n + i);
 }
 
-__global__ void __launch_bounds__(256) k_fisher_table(const int* tab, int n, double* out) {
+__global__ void __launch_bounds__(256) k_fisher_table(const int* tab, int n, double* out, const double* lgt, int lgt_n) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
         double l, r;
-        double two = fisher_two_sided(tab[4 * i], tab[4 * i + 1], tab[4 * i + 2], tab[4 * i + 3], &l, &r);
+        double two = fisher_two_sided(tab[4 * i], tab[4 * i + 1], tab[4 * i + 2], tab[4 * i + 3], &l, &r,
+                                      LGTable{lgt, lgt_n});
         out[3 * i] = two;
         out[3 * i + 1] = l;
         out[3 * i + 2] = r;
@@ -43,9 +44,9 @@ hipError_t launch_score(const ScoreIn& S, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_fisher_table(const int* tab, int n, double* out, hipStream_t s) {
+hipError_t launch_fisher_table(const int* tab, int n, double* out, const double* lgt, int lgt_n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fisher_table, dim3((n + 255) / 256), dim3(256), 0, s, tab, n, out);
+    hipLaunchKernelGGL(k_fisher_table, dim3((n + 255) / 256), dim3(256), 0, s, tab, n, out, lgt, lgt_n);
     return hipGetLastError();
 }
 

@@ -499,7 +499,7 @@ __device__ inline void emit_ref(const ScoreIn& S, int p, int kind, const VarX* x
     }
     if (S.cfg.fisher) {
         // the simple printer's Fisher columns (a pileup line of a reference-only position)
-        o.fisher_p = fisher_two_sided(o.rfc, o.rrc, o.fwd, o.rev, nullptr, nullptr);
+        o.fisher_p = fisher_two_sided(o.rfc, o.rrc, o.fwd, o.rev, nullptr, nullptr, LGTable{S.lgamma_tab, S.lgamma_n});
         const double ad = (double)(o.rfc * o.rev), bc = (double)(o.rrc * o.fwd);
         if (bc != 0 && ad != 0) o.fisher_ratio = ad > bc ? (ad / bc) : (bc / ad);
     }
@@ -1142,7 +1142,7 @@ __device__ void score_position(const ScoreIn& S, int p, bool dense_only, int slo
             o.fisher_p = 0;
             o.fisher_ratio = 0;
             if (c.fisher) {
-                o.fisher_p = fisher_two_sided(orfc, orrc, fwd, rev, nullptr, nullptr);
+                o.fisher_p = fisher_two_sided(orfc, orrc, fwd, rev, nullptr, nullptr, LGTable{S.lgamma_tab, S.lgamma_n});
                 double ad = (double)(orfc * rev), bc = (double)(orrc * fwd);
                 double ratio = 0.0;
                 if (bc != 0 && ad != 0) ratio = ad > bc ? (ad / bc) : (bc / ad);

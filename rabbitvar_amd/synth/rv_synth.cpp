@@ -20,6 +20,9 @@
 // i32 mtid[n], i64 seq_off[n+1], i64 cigar_off[n+1], u8 bases[nb] (ASCII), u8 quals[nb] (Phred),
 // u32 cigar[nc] (BAM op encoding); every array is zero-padded to a multiple of 8 bytes.
 #include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstdlib>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -29,6 +32,7 @@
 #include <stdexcept>
 #include <string>
 #include <vector>
+#include <thread>
 
 #include <zlib.h>
 
@@ -56,6 +60,7 @@ struct Opts {
     double singleEndFrac = 0;  // records written unpaired (no mate: mate start 0)
     double unmappedFrac = 0;   // paired records flagged unmapped (0x4), alignment kept
     bool bam = false;          // also write the records as BGZF-compressed BAM (reads.bam, normal.bam)
+    bool rvb = true;           // write the read batches (reads.rvb, normal.rvb); --no-rvb: BAM only
     // amplicon panel: reads only inside `panel` regions (written to panel.bed, 1-based inclusive),
     // region lengths log-normal (mu, sigma) clamped to [100, 50000], depth per region uniform in
     // [panelDepthLo, panelDepthHi]; each region is tiled by amplicons whose read pairs all start at
@@ -163,6 +168,7 @@ class BgzfWriter {
     void close() {
         if (!f_.is_open()) return;
         flush();
+        drain();
         static const uint8_t eof[28] = {0x1f, 0x8b, 8, 4, 0, 0, 0, 0, 0, 0xff, 6, 0, 0x42, 0x43,
                                         2, 0, 0x1b, 0, 3, 0, 0, 0, 0, 0, 0, 0, 0, 0};
         f_.write(reinterpret_cast<const char*>(eof), 28);
@@ -171,32 +177,53 @@ class BgzfWriter {
 
   private:
     static constexpr size_t kMax = 65280;
+    static constexpr size_t kBatch = 512;  // blocks compressed together, on several threads
     void flush() {
         if (buf_.empty()) return;
-        std::vector<uint8_t> z(compressBound((uLong)buf_.size()) + 64);
+        pending_.push_back(std::move(buf_));
+        buf_.clear();
+        if (pending_.size() == kBatch) drain();
+    }
+    // one BGZF block (header, raw deflate, CRC32, ISIZE) of `in`
+    static std::vector<uint8_t> block(const std::vector<uint8_t>& in) {
+        std::vector<uint8_t> z(compressBound((uLong)in.size()) + 64);
         z_stream zs;
         memset(&zs, 0, sizeof zs);
         deflateInit2(&zs, 1, Z_DEFLATED, -15, 8, Z_DEFAULT_STRATEGY);  // fast: test data
-        zs.next_in = buf_.data();
-        zs.avail_in = (uInt)buf_.size();
-        zs.next_out = z.data();
-        zs.avail_out = (uInt)z.size();
+        zs.next_in = const_cast<uint8_t*>(in.data());
+        zs.avail_in = (uInt)in.size();
+        zs.next_out = z.data() + 18;
+        zs.avail_out = (uInt)(z.size() - 26);
         deflate(&zs, Z_FINISH);
         const size_t clen = zs.total_out;
         deflateEnd(&zs);
         const uint32_t bsize = (uint32_t)(18 + clen + 8 - 1);
-        const uint32_t crc = (uint32_t)crc32(0L, buf_.data(), (uInt)buf_.size());
-        const uint32_t isize = (uint32_t)buf_.size();
+        const uint32_t crc = (uint32_t)crc32(0L, in.data(), (uInt)in.size());
+        const uint32_t isize = (uint32_t)in.size();
         const uint8_t h[18] = {0x1f, 0x8b, 8, 4, 0, 0, 0, 0, 0, 0xff, 6, 0, 0x42, 0x43, 2, 0,
                                (uint8_t)(bsize & 0xff), (uint8_t)(bsize >> 8)};
-        f_.write(reinterpret_cast<const char*>(h), 18);
-        f_.write(reinterpret_cast<const char*>(z.data()), (std::streamsize)clen);
-        f_.write(reinterpret_cast<const char*>(&crc), 4);
-        f_.write(reinterpret_cast<const char*>(&isize), 4);
-        buf_.clear();
+        memcpy(z.data(), h, 18);
+        memcpy(z.data() + 18 + clen, &crc, 4);
+        memcpy(z.data() + 22 + clen, &isize, 4);
+        z.resize(26 + clen);
+        return z;
+    }
+    void drain() {
+        std::vector<std::vector<uint8_t>> out(pending_.size());
+        std::atomic<size_t> next{0};
+        std::vector<std::thread> pool;
+        const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        for (unsigned t = 0; t < nt; t++)
+            pool.emplace_back([&] {
+                for (size_t i; (i = next.fetch_add(1)) < pending_.size();) out[i] = block(pending_[i]);
+            });
+        for (auto& th : pool) th.join();
+        for (auto& b : out) f_.write(reinterpret_cast<const char*>(b.data()), (std::streamsize)b.size());
+        pending_.clear();
     }
     std::ofstream f_;
     std::vector<uint8_t> buf_;
+    std::vector<std::vector<uint8_t>> pending_;
 };
 
 // reg2bin (SAM specification, section 5.3)
@@ -274,12 +301,6 @@ bool make_record(const std::string& ref, const Hap& H, int a, int b, bool longDe
     return true;
 }
 
-void put(std::ofstream& f, const void* p, size_t bytes) {
-    f.write(static_cast<const char*>(p), (std::streamsize)bytes);
-    static const char z[8] = {0};
-    size_t pad = (8 - bytes % 8) % 8;
-    if (pad) f.write(z, (std::streamsize)pad);
-}
 
 int usage() {
     fprintf(stderr,
@@ -287,7 +308,7 @@ int usage() {
             "        [--indel-rate X] [--indel-max N] [--long-dels N] [--repeat-frac F]\n"
             "        [--err-rate X] [--insert-mu M] [--insert-sd S] [--vaf LO HI]\n"
             "        [--somatic-rate X] [--normal-depth D] [--dup-rate X] [--single-end-frac F]\n"
-            "        [--unmapped-frac F] [--bam] [--panel N] [--panel-len MU SIGMA] [--panel-depth LO HI]\n"
+            "        [--unmapped-frac F] [--bam] [--no-rvb] [--panel N] [--panel-len MU SIGMA] [--panel-depth LO HI]\n"
             "        [--amp-len L]\n");
     return 2;
 }
@@ -325,6 +346,7 @@ int main(int argc, char** argv) {
         else if (a == "--unmapped-frac") o.unmappedFrac = std::stod(nx());
         else if (a == "--bam") o.bam = true;
         else if (a == "--panel") o.panel = std::stoi(nx());
+        else if (a == "--no-rvb") o.rvb = false;
         else if (a == "--panel-len") {
             o.panelMu = std::stod(nx());
             o.panelSigma = std::stod(nx());
@@ -465,10 +487,17 @@ int main(int argc, char** argv) {
         int k = 0;
         for (auto& r : panel_regions) bed << "chrS\t" << r.first << "\t" << r.second << "\tamp" << k++ << "\n";
     }
-    auto gen_reads = [&](double depth, bool somatic, std::mt19937_64& gread, std::mt19937_64& gerr) {
+    // Reads are made in chunks (a range of fragments, or one panel region) on several threads; every
+    // chunk draws from its own two streams seeded from (stream seed, chunk index), so the output
+    // depends on the seed only, not on the thread count.
+    auto gen_reads = [&](double depth, bool somatic, uint64_t stream_seed) {
     long npairs = (long)(depth * o.len / (2.0 * o.readLen));
-    std::vector<Rec> recs;
-    recs.reserve((size_t)npairs * 2);
+    const long CHUNK = 1 << 15;
+    const size_t nchunks = o.panel > 0 ? panel_regions.size() : (size_t)((npairs + CHUNK - 1) / CHUNK);
+    std::vector<std::vector<Rec>> parts(nchunks);
+    auto make_chunk = [&](size_t ci) {
+    std::mt19937_64 gread(stream_seed * 0x9E3779B97F4A7C15ULL + 2 * ci + 1), gerr(stream_seed * 0x9E3779B97F4A7C15ULL + 2 * ci + 2);
+    std::vector<Rec>& recs = parts[ci];
     std::normal_distribution<double> ins(o.insMu, o.insSd), qn(35, 5);
     // one read pair of the fragment [a, a + isz) of haplotype hp
     auto emit_pair = [&](const Hap& hp, int a, int isz) {
@@ -525,7 +554,8 @@ int main(int argc, char** argv) {
     };
     if (o.panel > 0) {
         // amplicons: fixed start and end per tile, every pair of the tile on one of them
-        for (const auto& reg : panel_regions) {
+        {
+            const auto& reg = panel_regions[ci];
             const double depth = o.panelDepthLo + (o.panelDepthHi - o.panelDepthLo) * U(gread);
             for (int t = reg.first - 1; t < reg.second; t += o.ampLen * 3 / 4) {
                 const int t_end = std::min(t + o.ampLen, o.len);
@@ -542,7 +572,7 @@ int main(int argc, char** argv) {
             }
         }
     } else {
-        for (long f = 0; f < npairs; f++) {
+        for (long f = (long)ci * CHUNK; f < std::min(npairs, (long)(ci + 1) * CHUNK); f++) {
             double vaf = o.vafLo + (o.vafHi - o.vafLo) * U(gread);
             int h = (gread() & 1) ? 1 : 0;
             if (somatic && U(gread) < vaf) h = 2;
@@ -555,9 +585,33 @@ int main(int argc, char** argv) {
             emit_pair(hp, a, isz);
         }
     }
-    std::stable_sort(recs.begin(), recs.end(), [](const Rec& a, const Rec& b) { return a.pos0 < b.pos0; });
+    };
+    {
+        std::atomic<size_t> next{0};
+        std::vector<std::thread> pool;
+        const unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        for (unsigned t = 0; t < nt; t++)
+            pool.emplace_back([&] {
+                for (size_t ci; (ci = next.fetch_add(1)) < nchunks;) make_chunk(ci);
+            });
+        for (auto& th : pool) th.join();
+    }
+    if (getenv("RV_SYNTH_TIMING")) fprintf(stderr, "[rvsynth] chunks done\n");
+    // records sorted by start, ties in chunk order (a stable sort of the concatenation)
+    std::vector<std::pair<int32_t, uint32_t>> key;
+    std::vector<std::pair<uint32_t, uint32_t>> at;  // (chunk, index in chunk)
+    for (size_t ci = 0; ci < nchunks; ci++)
+        for (size_t i = 0; i < parts[ci].size(); i++) {
+            key.push_back({parts[ci][i].pos0, (uint32_t)at.size()});
+            at.push_back({(uint32_t)ci, (uint32_t)i});
+        }
+    std::sort(key.begin(), key.end());
+    std::vector<Rec> recs;
+    recs.reserve(key.size());
+    for (const auto& k : key) recs.push_back(std::move(parts[at[k.second].first][at[k.second].second]));
+    parts.clear();
     if (o.dupRate > 0 || o.singleEndFrac > 0 || o.unmappedFrac > 0) {
-        std::mt19937_64 gdup(gread());
+        std::mt19937_64 gdup(stream_seed * 0x9E3779B97F4A7C15ULL + 0x5bd1e995ULL);
         std::vector<Rec> shaped;
         shaped.reserve(recs.size() + recs.size() / 4);
         for (Rec& r : recs) {
@@ -568,14 +622,20 @@ int main(int argc, char** argv) {
                 r.flag |= 0x4;
             }
             const bool copy = U(gdup) < o.dupRate;
-            shaped.push_back(r);
-            if (copy) shaped.push_back(r);
+            if (copy) shaped.push_back(r);  // the record and its identical copy
+            shaped.push_back(std::move(r));
         }
         recs.swap(shaped);
     }
     return recs;
     };
-    std::vector<Rec> recs = gen_reads(o.depth, o.somaticRate > 0, gread, gerr);
+    auto T0 = std::chrono::steady_clock::now();
+    auto tlog = [&](const char* what) {
+        if (getenv("RV_SYNTH_TIMING"))
+            fprintf(stderr, "[rvsynth] %s %.2f s\n", what, std::chrono::duration<double>(std::chrono::steady_clock::now() - T0).count());
+    };
+    std::vector<Rec> recs = gen_reads(o.depth, o.somaticRate > 0, o.seed * 1000003ULL + 3);
+    tlog("tumor reads");
 
     // ---------------- write
     {
@@ -590,33 +650,12 @@ int main(int argc, char** argv) {
         for (const Variant& v : vars) tv << v.pos + 1 << "\t" << kn[v.kind] << "\t" << v.len << "\t" << v.alt << "\t" << v.hap << "\n";
     }
     auto write_rvb = [&](const std::string& path, const std::vector<Rec>& recs) {
+        // streamed field by field through one buffer (the layout of the file header comment)
         int64_t n = (int64_t)recs.size(), nb = 0, nc = 0;
         for (const Rec& r : recs) {
             nb += (int64_t)r.bases.size();
             nc += (int64_t)r.cigar.size();
         }
-        std::vector<int32_t> pos((size_t)n), nm((size_t)n), mpos((size_t)n), tid((size_t)n, 0), mtid((size_t)n, 0);
-        std::vector<uint16_t> flag((size_t)n);
-        std::vector<uint8_t> mapq((size_t)n, 60), hasnm((size_t)n, 1);
-        std::vector<int64_t> so((size_t)n + 1), co((size_t)n + 1);
-        std::vector<uint8_t> bases((size_t)nb), quals((size_t)nb);
-        std::vector<uint32_t> cig((size_t)nc);
-        int64_t b = 0, c = 0;
-        for (int64_t i = 0; i < n; i++) {
-            const Rec& r = recs[(size_t)i];
-            pos[(size_t)i] = r.pos0;
-            flag[(size_t)i] = r.flag;
-            nm[(size_t)i] = r.nm;
-            mpos[(size_t)i] = r.mpos0;
-            so[(size_t)i] = b;
-            co[(size_t)i] = c;
-            memcpy(bases.data() + b, r.bases.data(), r.bases.size());
-            memcpy(quals.data() + b, r.quals.data(), r.quals.size());
-            b += (int64_t)r.bases.size();
-            for (uint32_t w : r.cigar) cig[(size_t)c++] = w;
-        }
-        so[(size_t)n] = b;
-        co[(size_t)n] = c;
         std::ofstream f(path, std::ios::binary);
         f.write("RVB1", 4);
         uint32_t ver = 1;
@@ -624,19 +663,63 @@ int main(int argc, char** argv) {
         f.write(reinterpret_cast<const char*>(&n), 8);
         f.write(reinterpret_cast<const char*>(&nb), 8);
         f.write(reinterpret_cast<const char*>(&nc), 8);
-        put(f, pos.data(), 4 * (size_t)n);
-        put(f, flag.data(), 2 * (size_t)n);
-        put(f, mapq.data(), (size_t)n);
-        put(f, hasnm.data(), (size_t)n);
-        put(f, nm.data(), 4 * (size_t)n);
-        put(f, mpos.data(), 4 * (size_t)n);
-        put(f, tid.data(), 4 * (size_t)n);
-        put(f, mtid.data(), 4 * (size_t)n);
-        put(f, so.data(), 8 * ((size_t)n + 1));
-        put(f, co.data(), 8 * ((size_t)n + 1));
-        put(f, bases.data(), (size_t)nb);
-        put(f, quals.data(), (size_t)nb);
-        put(f, cig.data(), 4 * (size_t)nc);
+        std::vector<char> buf;
+        buf.reserve(1 << 22);
+        size_t field_bytes = 0;
+        auto emit = [&](const void* p, size_t k) {
+            const char* c = static_cast<const char*>(p);
+            buf.insert(buf.end(), c, c + k);
+            field_bytes += k;
+            if (buf.size() >= (1 << 22)) {
+                f.write(buf.data(), (std::streamsize)buf.size());
+                buf.clear();
+            }
+        };
+        auto end_field = [&]() {
+            static const char z[8] = {0};
+            const size_t pad = (8 - field_bytes % 8) % 8;
+            if (pad) emit(z, pad);
+            field_bytes = 0;
+        };
+        for (const Rec& r : recs) emit(&r.pos0, 4);
+        end_field();
+        for (const Rec& r : recs) emit(&r.flag, 2);
+        end_field();
+        const uint8_t mq = 60, one = 1;
+        for (int64_t i = 0; i < n; i++) emit(&mq, 1);
+        end_field();
+        for (int64_t i = 0; i < n; i++) emit(&one, 1);
+        end_field();
+        for (const Rec& r : recs) emit(&r.nm, 4);
+        end_field();
+        for (const Rec& r : recs) emit(&r.mpos0, 4);
+        end_field();
+        const int32_t zero = 0;
+        for (int64_t i = 0; i < n; i++) emit(&zero, 4);  // tid
+        end_field();
+        for (int64_t i = 0; i < n; i++) emit(&zero, 4);  // mtid
+        end_field();
+        int64_t off = 0;
+        for (const Rec& r : recs) {
+            emit(&off, 8);
+            off += (int64_t)r.bases.size();
+        }
+        emit(&off, 8);
+        end_field();
+        off = 0;
+        for (const Rec& r : recs) {
+            emit(&off, 8);
+            off += (int64_t)r.cigar.size();
+        }
+        emit(&off, 8);
+        end_field();
+        for (const Rec& r : recs) emit(r.bases.data(), r.bases.size());
+        end_field();
+        for (const Rec& r : recs) emit(r.quals.data(), r.quals.size());
+        end_field();
+        for (const Rec& r : recs) emit(r.cigar.data(), 4 * r.cigar.size());
+        end_field();
+        f.write(buf.data(), (std::streamsize)buf.size());
         return n;
     };
     auto write_bam = [&](const std::string& path, const std::vector<Rec>& recs) {
@@ -696,13 +779,17 @@ int main(int argc, char** argv) {
         }
         w.close();
     };
-    const int64_t n = write_rvb(o.out + "/reads.rvb", recs);
+    const int64_t n = o.rvb ? write_rvb(o.out + "/reads.rvb", recs) : (int64_t)recs.size();
+    tlog("tumor written");
     if (o.bam) write_bam(o.out + "/reads.bam", recs);
     int64_t nn = 0;
     if (o.normalDepth > 0) {
-        std::mt19937_64 gnread(o.seed * 1000003ULL + 6), gnerr(o.seed * 1000003ULL + 7);
-        const std::vector<Rec> nrecs = gen_reads(o.normalDepth, false, gnread, gnerr);
-        nn = write_rvb(o.out + "/normal.rvb", nrecs);
+        recs.clear();
+        recs.shrink_to_fit();
+        const std::vector<Rec> nrecs = gen_reads(o.normalDepth, false, o.seed * 1000003ULL + 6);
+        tlog("normal reads");
+        nn = o.rvb ? write_rvb(o.out + "/normal.rvb", nrecs) : (int64_t)nrecs.size();
+        tlog("normal written");
         if (o.bam) write_bam(o.out + "/normal.bam", nrecs);
     }
     printf("{\"reads\": %lld, \"normal_reads\": %lld, \"variants\": %zu, \"len\": %d}\n", (long long)n, (long long)nn,

@@ -26,6 +26,13 @@ CASES = {
     # deep tumor/normal pair: 40 kb, tumor 250x, normal 120x, somatic variants at VAF 0.01-0.2, 2% duplicates
     "deep": ["--seed", "13", "--len", "40000", "--depth", "250", "--normal-depth", "120", "--somatic-rate", "0.001",
              "--vaf", "0.01", "0.2", "--indel-rate", "0.0005", "--dup-rate", "0.02"],
+    # the 5 Mb, 60x germline workload at its full size (the bench workload's shape, another seed)
+    "c2full": ["--seed", "8", "--len", "5000000", "--depth", "60", "--indel-rate", "0.000125", "--indel-max", "30",
+               "--long-dels", "200", "--repeat-frac", "0.02"],
+    # the deep somatic pair at full depth on a 1 Mb slice: tumor 200x, normal 100x, somatic SNVs and
+    # indels at 6e-5/bp with VAF 0.01-0.2, 2% duplicated records
+    "c3slice": ["--seed", "31", "--len", "1000000", "--depth", "200", "--normal-depth", "100", "--somatic-rate",
+                "0.00006", "--vaf", "0.01", "0.2", "--indel-rate", "0.0001", "--dup-rate", "0.02"],
     # tumor/normal pair with the dense edge-case mix above and a thin normal
     "som_edge": ["--seed", "7", "--len", "60000", "--depth", "40", "--normal-depth", "30", "--indel-rate", "0.002",
                  "--indel-max", "40", "--snv-rate", "0.004", "--long-dels", "6", "--repeat-frac", "0.05",
@@ -42,7 +49,7 @@ def _ensure_tools():
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
     if not os.path.exists(SYNTH):
         os.makedirs(os.path.dirname(SYNTH), exist_ok=True)
-        subprocess.run(["g++", "-O2", "-std=c++17", "-o", SYNTH,
+        subprocess.run(["g++", "-O2", "-std=c++17", "-pthread", "-o", SYNTH,
                         os.path.join(ROOT, "rabbitvar_amd", "synth", "rv_synth.cpp"), "-lz"], check=True)
 
 

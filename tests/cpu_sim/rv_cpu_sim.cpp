@@ -46,6 +46,14 @@ int main(int argc, char** argv) {
         printf("\n");
         return 0;
     }
+    if (argc == 5 && std::string(argv[1]) == "--findmsi") {
+        // the scoring kernels' find_msi: "msi shift3 msint" (arguments as rvcpu --findmsi)
+        auto arg = [&](int i) { return std::string(argv[i]) == "-" ? std::string() : std::string(argv[i]); };
+        std::string t1 = arg(2), t2 = arg(3), left = arg(4);
+        MSIRes m = find_msi(t1.c_str(), (int)t1.size(), t2.c_str(), (int)t2.size(), left.c_str(), (int)left.size());
+        printf("%.17g %d %s\n", m.msi, m.shift3, t1.substr(t1.size() - (size_t)m.msint_len).c_str());
+        return 0;
+    }
     if (argc >= 2 && std::string(argv[1]) == "--rh-ints") {
         RHOrder m;
         char op[8];

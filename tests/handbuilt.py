@@ -152,3 +152,65 @@ def build_dups(d):
             f.write(s[i:i + 60] + "\n")
     write_rvb(os.path.join(d, "reads.rvb"), batch_from_records(recs))
     return s, alt
+
+
+def build_quirks(d, copies=3):
+    """Reads that exercise the reference's rarely taken paths, on the known-answer reference of
+    test_oracle_cigar_kat.py (chrS, 600 bp):
+      * every CIGAR-rewrite known-answer read, `copies` times each (forward and reverse);
+      * spliced reads (N operations, parseCigar.cpp:1733-1753);
+      * reads without an NM tag (parseCigar.cpp:525-533), mapped and flagged unmapped;
+      * 30 plain 100M reads for background coverage.
+    Records are written in alignment-start order."""
+    import test_oracle_cigar_kat as kat
+
+    ref = kat.REF
+    recs = []
+
+    def ops_of(cig):
+        ops, num = [], ""
+        for ch in cig:
+            if ch.isdigit():
+                num += ch
+            else:
+                ops.append(("MIDNSHP=X".index(ch), int(num)))
+                num = ""
+        return ops
+
+    for name in kat.CASES:
+        P, cig, seq, q, _ = kat.CASES[name]()
+        ops = ops_of(cig)
+        quals = [int(x) for x in q.split(",")] if isinstance(q, str) else [q] * len(seq)
+        idlen = sum(n for o, n in ops if o in (1, 2))
+        for c in range(copies):
+            recs.append({"pos0": P - 1, "flag": 16 if c % 2 else 0, "mapq": 60, "nm": idlen, "seq": seq,
+                         "qual": quals, "cigar": ops})
+    rnd = random.Random(77)
+    for i in range(30):
+        p0 = 140 + i * 10
+        recs.append({"pos0": p0, "flag": 16 if i % 2 else 0, "mapq": 60, "nm": 0, "seq": ref[p0:p0 + 100],
+                     "qual": [rnd.randint(20, 40) for _ in range(100)], "cigar": [(0, 100)]})
+    for i in range(6):  # spliced: 40M 100N 60M, 35M 2I 20M 150N 43M
+        p0 = 150 + 7 * i
+        if i % 2 == 0:
+            seq = ref[p0:p0 + 40] + ref[p0 + 140:p0 + 200]
+            cig = [(0, 40), (3, 100), (0, 60)]
+        else:
+            seq = ref[p0:p0 + 35] + "AC" + ref[p0 + 35:p0 + 55] + ref[p0 + 205:p0 + 248]
+            cig = [(0, 35), (1, 2), (0, 20), (3, 150), (0, 43)]
+        recs.append({"pos0": p0, "flag": 0, "mapq": 60, "nm": 2 if i % 2 else 0, "seq": seq,
+                     "qual": [30] * len(seq), "cigar": cig})
+    for i in range(4):  # no NM tag: mapped reads are walked (NM counted as 0), unmapped ones dropped
+        p0 = 260 + 5 * i
+        seq = ref[p0:p0 + 50] + ("A" if ref[p0 + 50] != "A" else "C") + ref[p0 + 51:p0 + 100]
+        rec = {"pos0": p0, "flag": 0x8 if i < 2 else 0x4, "mapq": 60, "seq": seq, "qual": [30] * 100,
+               "cigar": [(0, 100)]}
+        recs.append(rec)
+    recs.sort(key=lambda r: r["pos0"])
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, "ref.fa"), "w") as f:
+        f.write(">chrS\n")
+        for i in range(0, len(ref), 60):
+            f.write(ref[i:i + 60] + "\n")
+    write_rvb(os.path.join(d, "reads.rvb"), batch_from_records(recs))
+    return ref

@@ -1,0 +1,59 @@
+"""The reference's rarely taken paths on the GPU: the CIGAR-rewrite known-answer reads (every rule,
+the leading D/I and trailing I/D quirks, the negative-length fallback), spliced reads (N operations,
+parseCigar.cpp:1733-1753), reads without an NM tag (parseCigar.cpp:525-533), against the oracle
+whose rewrite those known answers pin (tests/test_oracle_cigar_kat.py).  Also the capacity error for
+a CIGAR longer than the record slot."""
+import os
+
+import pytest
+
+import handbuilt as hb
+from conftest import run_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("extra", [[], ["-k", "0"], ["-F", "0"], ["-p"]])
+def test_quirk_reads_match_oracle(tools, tmp_path, extra):
+    from rabbitvar_amd import read_fasta, read_rvb
+    from rabbitvar_amd.engine import Engine
+
+    d = str(tmp_path / "quirks")
+    hb.build_quirks(d)
+    want_out, want_dump = run_oracle(d, "chrS:1-600", ["--fisher"] + extra, dump="pile.txt")
+    name, seq = read_fasta(os.path.join(d, "ref.fa"))
+    reads = read_rvb(os.path.join(d, "reads.rvb"))
+    kw = {}
+    if "-k" in extra:
+        kw["performLocalRealignment"] = 0
+    if "-F" in extra:
+        kw["samfilter"] = 0
+    if "-p" in extra:
+        kw["doPileup"] = 1
+    eng = Engine(sample="S1", fisher=1, **kw)
+    dump = eng.debug_pileup(name, 1, len(seq), reads, name, seq)
+    out = eng.run_region(name, 1, len(seq), reads, name, seq)
+    st = eng.stats()
+    eng.close()
+    print(f"quirks {extra}: {len(want_out.splitlines())} lines, {len(want_dump.splitlines())} pileup records",
+          flush=True)
+    assert st["walk_errors"] == 0
+    assert sorted(dump.splitlines()) == sorted(want_dump.splitlines())
+    assert out == want_out
+    assert len(want_out.splitlines()) >= 10
+
+
+def test_long_cigar_fails_with_capacity_error(tools, tmp_path):
+    from rabbitvar_amd.engine import Engine, EngineError
+    from test_cpu_sim import _long_cigar_records
+
+    ref = hb.reference()
+    reads = _long_cigar_records(ref, (5, 11))
+    eng = Engine(sample="S1", fisher=1)
+    with pytest.raises(EngineError) as ei:
+        eng.run_region("chrT", 1, hb.L, reads, "chrT", ref.encode())
+    assert "CIGAR longer than" in str(ei.value)
+    # the engine stays usable: the same batch without the long reads runs
+    ok = eng.run_region("chrT", 1, hb.L, _long_cigar_records(ref, ()), "chrT", ref.encode())
+    eng.close()
+    assert isinstance(ok, str)
