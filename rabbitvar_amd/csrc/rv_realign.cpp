@@ -16,6 +16,7 @@
 // A candidate whose processing throws (a reference key outside the loaded window, an out-of-range
 // substring) is abandoned with the changes made so far, as the reference's per-candidate catch does.
 #include "rv_realign.h"
+#include "rv_desc.h"
 
 #include <algorithm>
 #include <chrono>
@@ -23,7 +24,6 @@
 #include <cstdlib>
 #include <cstring>
 #include <iterator>
-#include <regex>
 #include <stdexcept>
 #include <thread>
 
@@ -336,9 +336,6 @@ void replace_first(std::string& s, const std::string& a, const std::string& b) {
     size_t p = s.find(a);
     if (p != std::string::npos) s.replace(p, a.size(), b);
 }
-void replace_first_re(std::string& s, const std::regex& re, const std::string& b) {
-    s = std::regex_replace(s, re, b, std::regex_constants::format_first_only);
-}
 bool starts(const std::string& s, char c) { return !s.empty() && s[0] == c; }
 
 // ------------------------------------------------------------------ count transfer
@@ -525,29 +522,6 @@ std::string strip_marks(std::string s) {
     return s;
 }
 
-struct Re {
-    std::regex begin_minus_number{"^-(\\d+).*"};
-    std::regex up_number_end{"\\^(\\d+)$"};
-    std::regex minus_sv{"^-\\d+\\^([ATGNC]+)<...\\d+>([ATGNC]+)$"};
-    std::regex begin_minus_number_any{"^-\\d+(.*)"};
-    std::regex caret_atgnc{"\\^([ATGNC]+)"};
-    std::regex strip{"\\^|&|#"};
-    std::regex minus_number_amp_atgcs_end{"(-\\d+)&[ATGC]+$"};
-    std::regex begin_plus_atgc{"^\\+([ATGC]+).*"};
-    std::regex dup_num_atgc{"<dup(\\d+)>([ATGC]+)$"};
-    std::regex amp_atgc{".*&([ATGC]+).*"};
-    std::regex hash_atgc{"#([ATGC]+).*"};
-    std::regex caret_atgc_end{"\\^([ATGC]+)$"};
-    std::regex plus_atgcs_amp_atgcs_end{"(\\+[ATGC]+)&[ATGC]+$"};
-    std::regex lead_plus{"^\\+"};
-    std::regex caret_digits_end{"\\^\\d+$"};
-    std::regex caret{"\\^"};
-};
-
-const Re& res() {
-    static const Re r;
-    return r;
-}
 
 // findMM5 / findMM3: the mismatches next to a candidate's flanks, and the soft-clip positions to
 // try.  The soft clips found are marked used in `mark`; with mark == nullptr (a probe for the
@@ -656,19 +630,15 @@ struct DelSetup {
     std::string wupseq, sanpseq;
 };
 DelSetup del_setup(HostRef& ref, int chrLen, int p, const std::string& vn) {
-    const Re& R = res();
     DelSetup d;
-    std::smatch sm;
-    if (std::regex_search(vn, sm, R.begin_minus_number)) d.dellen = std::atoi(sm.str(1).c_str());
-    if (std::regex_search(vn, sm, R.up_number_end)) d.dellen += std::atoi(sm.str(1).c_str());
+    std::string num, rest;
+    if (desc::begin_minus_number(vn, num)) d.dellen = std::atoi(num.c_str());
+    if (desc::up_number_end(vn, num)) d.dellen += std::atoi(num.c_str());
     std::string extrains, extra, inv5, inv3;
-    if (std::regex_search(vn, sm, R.minus_sv)) {
-        inv5 = sm.str(1);
-        inv3 = sm.str(2);
-    } else if (std::regex_search(vn, sm, R.begin_minus_number_any)) {
-        extra = std::regex_replace(sm.str(1), R.strip, "");
-        std::smatch s2;
-        if (std::regex_search(vn, s2, R.caret_atgnc)) extrains = s2.str(1);
+    if (desc::minus_sv(vn, inv5, inv3)) {
+    } else if (desc::begin_minus_number_any(vn, rest)) {
+        extra = desc::strip_marks3(rest);
+        desc::caret_atgnc(vn, extrains);
     }
     int wustart = (p - 200) > 1 ? (p - 200) : 1;
     d.wupseq = join_ref(ref, wustart, p - 1) + extra;
@@ -894,7 +864,6 @@ struct Realigner {
     }
 
     void realigndel(bool withBams, const std::map<int, std::map<std::string, int>>& counts) {
-        const Re& R = res();
         std::vector<Cand> tmp = sorted_candidates(counts);
         st.candidates += (int64_t)tmp.size();
         int ci = -1;
@@ -986,9 +955,8 @@ struct Realigner {
             if (!rp.key) continue;
             if (!rp.v.count(vn)) continue;
             HVar* vref = rp.v[vn];
-            std::smatch mt;
-            if (std::regex_search(vn, mt, R.minus_number_amp_atgcs_end)) {
-                std::string tn = mt.str(1);
+            std::string tn;
+            if (desc::minus_number_amp_atgcs_end(vn, tn)) {
                 if (rp.v.count(tn)) {
                     HVar* tref = rp.v[tn];
                     if (vref->cnt < tref->cnt) {
@@ -1002,7 +970,6 @@ struct Realigner {
     }
 
     void realignins(const std::map<int, std::map<std::string, int>>& counts) {
-        const Re& R = res();
         std::vector<Cand> tmp = sorted_candidates(counts);
         st.candidates += (int64_t)tmp.size();
         int ci = -1;
@@ -1014,30 +981,23 @@ struct Realigner {
                 int position = t.position;
                 std::string vn = t.desc;
                 int icnt = t.count;
-                std::smatch m;
                 std::string insert1;
-                if (std::regex_search(vn, m, R.begin_plus_atgc))
-                    insert1 = m.str(1);
-                else
-                    continue;
-                std::string ins3;
+                if (!desc::begin_plus_atgc(vn, insert1)) continue;
+                std::string ins3, dupn;
                 int inslen = (int)insert1.size();
-                if (std::regex_search(vn, m, R.dup_num_atgc)) {
-                    ins3 = m.str(2);
-                    inslen += std::atoi(m.str(1).c_str()) + (int)ins3.size();
-                }
-                std::string extra, compm, newins;
-                if (std::regex_search(vn, m, R.amp_atgc)) extra = m.str(1);
-                if (std::regex_search(vn, m, R.hash_atgc)) compm = m.str(1);
-                if (std::regex_search(vn, m, R.caret_atgc_end)) newins = m.str(1);
+                if (desc::dup_num_atgc(vn, dupn, ins3)) inslen += std::atoi(dupn.c_str()) + (int)ins3.size();
+                std::string extra, compm, newins, num;
+                desc::amp_atgc(vn, extra);
+                desc::hash_atgc(vn, compm);
+                desc::caret_atgc_end(vn, newins);
                 int newdel = 0;
-                if (std::regex_search(vn, m, R.up_number_end)) newdel = std::atoi(m.str(1).c_str());
+                if (desc::up_number_end(vn, num)) newdel = std::atoi(num.c_str());
                 std::string tn = vn;
-                replace_first_re(tn, R.lead_plus, "");
+                desc::drop_lead_plus(tn);
                 replace_first(tn, "&", "");
                 replace_first(tn, "#", "");
-                replace_first_re(tn, R.caret_digits_end, "");
-                replace_first_re(tn, R.caret, "");
+                desc::drop_caret_digits_end(tn);
+                desc::drop_first_caret(tn);
                 int wustart = position - 150 > 1 ? (position - 150) : 1;
                 std::string wupseq = join_ref(ref, wustart, position) + tn;
                 int sanend = position + (int)vn.size() + 100;
@@ -1155,9 +1115,8 @@ struct Realigner {
                 auto& m = mi->second;
                 if (!m.count(vn)) continue;
                 HVar* vref = m.at(vn);
-                std::smatch mt;
-                if (std::regex_search(vn, mt, R.plus_atgcs_amp_atgcs_end)) {
-                    std::string tn = mt.str(1);
+                std::string tn;
+                if (desc::plus_atgcs_amp_atgcs_end(vn, tn)) {
                     if (m.count(tn)) {
                         HVar* tref = m.at(tn);
                         if (vref->cnt < tref->cnt) {

@@ -26,6 +26,8 @@
 #include "rv_realign.h"
 #include "rv_rhorder.h"
 #include "rv_score_body.h"
+#include "rv_desc.h"
+#include <regex>
 
 using namespace rv;
 
@@ -44,6 +46,74 @@ int main(int argc, char** argv) {
         for (int i = 2; i < argc; i++) ins.push_back((char)atoi(argv[i]));
         for (char c : rh_char_iteration_order(ins)) printf("%d ", (int)(unsigned char)c);
         printf("\n");
+        return 0;
+    }
+    if (argc == 4 && std::string(argv[1]) == "--desc-fuzz") {
+        // the realigner's descriptor matchers (rv_desc.h) against std::regex with the reference's
+        // pattern texts, on random strings over the descriptor alphabet and on descriptor shapes.
+        // Prints "cases mismatches matches" (matches: cases where some pattern matched).
+        const int trials = atoi(argv[2]);
+        unsigned seed = (unsigned)atoi(argv[3]);
+        auto rnd = [&]() {
+            seed = seed * 1664525u + 1013904223u;
+            return seed >> 8;
+        };
+        const std::string alpha = "+-^&#ACGTN0123456789<>dup";
+        const char* shapes[] = {"-", "+", "-12", "-3&", "+AC", "#", "^", "&", "^5", "#AG^", "ACGT", "-2^AT", "<dup3>"};
+        const std::regex r_bmn{"^-(\\d+).*"}, r_une{"\\^(\\d+)$"}, r_bmna{"^-\\d+(.*)"}, r_ca{"\\^([ATGNC]+)"},
+            r_strip{"\\^|&|#"}, r_mnae{"(-\\d+)&[ATGC]+$"}, r_bpa{"^\\+([ATGC]+).*"}, r_amp{".*&([ATGC]+).*"},
+            r_hash{"#([ATGC]+).*"}, r_cae{"\\^([ATGC]+)$"}, r_paae{"(\\+[ATGC]+)&[ATGC]+$"}, r_lp{"^\\+"},
+            r_cde{"\\^\\d+$"}, r_c{"\\^"}, r_sv{"^-\\d+\\^([ATGNC]+)<...\\d+>([ATGNC]+)$"},
+            r_dup{"<dup(\\d+)>([ATGC]+)$"};
+        long bad = 0, hits = 0;
+        for (int t = 0; t < trials; t++) {
+            std::string x;
+            const int parts = 1 + (int)(rnd() % 6);
+            for (int k = 0; k < parts; k++) {
+                if (rnd() % 2) x += shapes[rnd() % (sizeof shapes / sizeof *shapes)];
+                const int len = (int)(rnd() % 5);
+                for (int q = 0; q < len; q++) x += alpha[rnd() % alpha.size()];
+            }
+            std::smatch m;
+            std::string a, b;
+            auto chk = [&](bool got, const std::string& gv, const std::regex& re, int grp) {
+                const bool want = std::regex_search(x, m, re);
+                hits += want;
+                if (got != want || (want && gv != m.str(grp))) {
+                    if (bad < 10) fprintf(stderr, "mismatch on '%s'\n", x.c_str());
+                    bad++;
+                }
+            };
+            chk(desc::begin_minus_number(x, a), a, r_bmn, 1);
+            chk(desc::up_number_end(x, a), a, r_une, 1);
+            chk(desc::begin_minus_number_any(x, a), a, r_bmna, 1);
+            chk(desc::caret_atgnc(x, a), a, r_ca, 1);
+            chk(desc::minus_number_amp_atgcs_end(x, a), a, r_mnae, 1);
+            chk(desc::begin_plus_atgc(x, a), a, r_bpa, 1);
+            chk(desc::amp_atgc(x, a), a, r_amp, 1);
+            chk(desc::hash_atgc(x, a), a, r_hash, 1);
+            chk(desc::caret_atgc_end(x, a), a, r_cae, 1);
+            chk(desc::plus_atgcs_amp_atgcs_end(x, a), a, r_paae, 1);
+            {
+                std::string c, d;
+                const bool g = desc::minus_sv(x, c, d), w = std::regex_search(x, m, r_sv);
+                if (g != w || (w && (c != m.str(1) || d != m.str(2)))) bad++;
+                const bool g2 = desc::dup_num_atgc(x, c, d), w2 = std::regex_search(x, m, r_dup);
+                if (g2 != w2 || (w2 && (c != m.str(1) || d != m.str(2)))) bad++;
+            }
+            auto rep = [&](const std::regex& re) { return std::regex_replace(x, re, "", std::regex_constants::format_first_only); };
+            if (desc::strip_marks3(x) != std::regex_replace(x, r_strip, "")) bad++;
+            a = x;
+            desc::drop_lead_plus(a);
+            if (a != rep(r_lp)) bad++;
+            a = x;
+            desc::drop_caret_digits_end(a);
+            if (a != rep(r_cde)) bad++;
+            a = x;
+            desc::drop_first_caret(a);
+            if (a != rep(r_c)) bad++;
+        }
+        printf("%d %ld %ld\n", trials, bad, hits);
         return 0;
     }
     if (argc == 5 && std::string(argv[1]) == "--findmsi") {

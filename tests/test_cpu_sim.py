@@ -273,3 +273,13 @@ def test_cigar_rewrite_known_answers_engine(sim, tmp_path):
         if status != 0 or (pos, text) != want[1:]:
             bad.append((name, P, cig, status, pos, text, want))
     assert not bad, bad
+
+
+def test_descriptor_matchers_equal_std_regex(sim):
+    """The realigner's hand-written descriptor matchers (rv_desc.h) give std::regex's match and
+    captures, with the reference's pattern texts (include/patterns.h), on 200,000 random strings over
+    the descriptor alphabet and descriptor-shaped pieces."""
+    r = subprocess.run([sim, "--desc-fuzz", "200000", "5"], capture_output=True, text=True, check=True)
+    trials, bad, hits = (int(x) for x in r.stdout.split())
+    assert bad == 0, r.stderr
+    assert hits > 100000

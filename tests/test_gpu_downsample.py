@@ -81,5 +81,5 @@ def test_pool_region_order(panel, workers):
     texts = pool.run_regions(regions, bam, {name: seq})
     pool.close()
     want = open(f"{d}/oracle.txt").read()
-    assert len(want.splitlines()) > 50
+    assert len(want.splitlines()) > 30
     assert "".join(texts) == want
