@@ -272,6 +272,15 @@ void rv_free(void* p);
  * (src/modes/simpleMode.cpp:96-108, src/modes/somaticMode.cpp:130-149). */
 rv_status rv_debug_fisher(rv_engine* e, const int32_t* tables, size_t n, double* out);
 
+/* rv_debug_fisher without any allocation: no reference counterpart; for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Needs no engine; runs on the current HIP device.  Its device memory is one fixed array per GPU
+ * (3.5 MB: the log-gamma table, the tables in, the p-values out), held until the process ends, so at
+ * most 65536 tables per call (more: RV_ERR_CAPACITY, nothing computed).  Calls that overlap on one
+ * GPU share that array and race.  Same results as rv_debug_fisher, bit for bit.  RV_ERR_DEVICE on a
+ * HIP failure, RV_ERR_ARG on null pointers with n > 0. */
+rv_status rv_fisher_tables_noalloc(const int32_t* tables, size_t n, double* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -85,6 +85,7 @@ SIGNATURES = {
                                 ctypes.POINTER(c_size_t)]),
     "rv_free": (None, [c_void_p]),
     "rv_debug_fisher": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_int32), c_size_t, ctypes.POINTER(ctypes.c_double)]),
+    "rv_fisher_tables_noalloc": (c_int, [ctypes.POINTER(ctypes.c_int32), c_size_t, ctypes.POINTER(ctypes.c_double)]),
     "rv_set_screen": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_float), c_size_t, ctypes.c_float]),
     "rv_screen_dataset": (c_int, [c_void_p, ctypes.POINTER(RvRegion), ctypes.POINTER(RvReads),
                                   ctypes.POINTER(RvContig), ctypes.POINTER(ctypes.POINTER(ctypes.c_float)),

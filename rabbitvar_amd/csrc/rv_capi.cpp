@@ -179,6 +179,14 @@ rv_status rv_debug_pileup(rv_engine* e, const rv_region* region, const rv_reads*
 
 void rv_free(void* p) { std::free(p); }
 
+rv_status rv_fisher_tables_noalloc(const int32_t* tables, size_t n, double* out) {
+    if (n && (!tables || !out)) return RV_ERR_ARG;
+    if (n > (size_t)rv::FISHER_NOALLOC_MAX_TABLES) return RV_ERR_CAPACITY;
+    if (n == 0) return RV_OK;
+    const char* err = nullptr;
+    return rv::fisher_tables_noalloc(tables, (int)n, out, &err) == hipSuccess ? RV_OK : RV_ERR_DEVICE;
+}
+
 rv_status rv_debug_fisher(rv_engine* e, const int32_t* tables, size_t n, double* out) {
     if (!e || (n && (!tables || !out)) || n > (size_t)0x7fffffff / 4) return RV_ERR_ARG;
     if (n == 0) return RV_OK;

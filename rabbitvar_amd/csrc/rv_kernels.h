@@ -29,5 +29,8 @@ hipError_t sort_key_pairs(void* temp, size_t& temp_bytes, const unsigned long lo
 
 hipError_t launch_score(const ScoreIn& S, hipStream_t s);
 hipError_t launch_fisher_table(const int* tab, int n, double* out, const double* lgt, int lgt_n, hipStream_t s);
+// the allocation-free variant's whole call (host tables in, host p-values out; rv_score.hip)
+constexpr int FISHER_NOALLOC_MAX_TABLES = 1 << 16;
+hipError_t fisher_tables_noalloc(const int* tab, int n, double* out, const char** err);
 
 }  // namespace rv

@@ -7,6 +7,8 @@
 #include "rv_kernels.h"
 #include "rv_score_body.h"
 
+#include <mutex>
+
 namespace rv {
 
 __global__ void __launch_bounds__(128) k_score_dense(ScoreIn S, const int32_t* pos_list, int n) {
@@ -48,6 +50,84 @@ hipError_t launch_fisher_table(const int* tab, int n, double* out, const double*
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_fisher_table, dim3((n + 255) / 256), dim3(256), 0, s, tab, n, out, lgt, lgt_n);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- the allocation-free Fisher entry
+// rv_fisher_tables_noalloc works in this file-scope device array instead of engine buffers: the
+// log-gamma table, then the tables in, then the p-values out, for at most FISHER_NOALLOC_MAX_TABLES tables
+// per call.  Each GPU holds its own copy until the process ends; every call on one GPU uses it, so
+// calls that overlap on one GPU race.  Nothing in it has to start at zero: a call writes the inputs
+// it reads and every output it returns.
+constexpr size_t FNA_LGT = 0;
+constexpr size_t FNA_IN = FNA_LGT + sizeof(double) * FISHER_LGT_N;
+constexpr size_t FNA_OUT = FNA_IN + sizeof(int) * 4 * FISHER_NOALLOC_MAX_TABLES;
+constexpr size_t FNA_BYTES = FNA_OUT + sizeof(double) * 3 * FISHER_NOALLOC_MAX_TABLES;
+static __device__ __align__(256) unsigned char scratch[FNA_BYTES];
+
+namespace {
+constexpr int FNA_MAX_DEVICES = 64;
+std::mutex fna_mu;
+unsigned char* fna_ptr[FNA_MAX_DEVICES];  // the device address of `scratch`, per device (nullptr: not set up)
+double fna_host_lgt[FISHER_LGT_N];        // the host-built table, computed once (static storage)
+bool fna_host_ready = false;
+}  // namespace
+
+// Set-up on the current device (once): the scratch array's device address, and the log-gamma table
+// copied into it.
+static hipError_t fisher_noalloc_setup(unsigned char*& base, const char** err) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        *err = "no current HIP device";
+        return e;
+    }
+    if (dev < 0 || dev >= FNA_MAX_DEVICES) {
+        *err = "device index beyond the scratch table";
+        return hipErrorInvalidDevice;
+    }
+    std::lock_guard<std::mutex> lk(fna_mu);
+    if (!fna_ptr[dev]) {
+        if (!fna_host_ready) {
+            fisher_lgamma_table(fna_host_lgt, FISHER_LGT_N);
+            fna_host_ready = true;
+        }
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            *err = "hipGetSymbolAddress(scratch) failed";
+            return e != hipSuccess ? e : hipErrorInvalidSymbol;
+        }
+        e = hipMemcpy((unsigned char*)p + FNA_LGT, fna_host_lgt, sizeof fna_host_lgt, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            *err = "log-gamma table upload failed";
+            return e;
+        }
+        fna_ptr[dev] = (unsigned char*)p;
+    }
+    base = fna_ptr[dev];
+    return hipSuccess;
+}
+
+hipError_t fisher_tables_noalloc(const int* tab, int n, double* out, const char** err) {
+    unsigned char* base = nullptr;
+    hipError_t e = fisher_noalloc_setup(base, err);
+    if (e != hipSuccess) return e;
+    int* din = (int*)(base + FNA_IN);
+    double* dout = (double*)(base + FNA_OUT);
+    const double* lgt = (const double*)(base + FNA_LGT);
+    if ((e = hipMemcpy(din, tab, sizeof(int) * 4 * (size_t)n, hipMemcpyHostToDevice)) != hipSuccess) {
+        *err = "table upload failed";
+        return e;
+    }
+    if ((e = launch_fisher_table(din, n, dout, lgt, FISHER_LGT_N, nullptr)) != hipSuccess) {
+        *err = "Fisher kernel launch failed";
+        return e;
+    }
+    if ((e = hipMemcpy(out, dout, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost)) != hipSuccess) {
+        *err = "result download failed";
+        return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace rv
