@@ -164,6 +164,8 @@ typedef struct rv_stats {
     int64_t screen_candidates; /* realignment candidates the screen saw */
     int64_t screen_skipped;    /* candidates the realigner skipped (screened mode) */
     double screen_list_ms;     /* of screen_ms: the host's candidate list and neighbourhood tables */
+    int64_t realign_device_pairs; /* realignlgins30 clip-consensus pairs scored by the device kernel */
+    double realign_device_ms;     /* that kernel's time */
 } rv_stats;
 
 void rv_config_default(rv_config* cfg);
@@ -271,6 +273,13 @@ void rv_free(void* p);
  * tail, right tail).  Replaces htslib's kt_fisher_exact as the reference calls it
  * (src/modes/simpleMode.cpp:96-108, src/modes/somaticMode.cpp:130-149). */
 rv_status rv_debug_fisher(rv_engine* e, const int32_t* tables, size_t n, double* out);
+
+/* Diagnostics: the device's find35match (realignlgins30's clip-consensus pair scoring, reference
+ * src/VariationRealigner.cpp:1396-1428) on n pairs of strings in `arena`: pairs[4k..4k+3] = offset
+ * and length of the 5' consensus, offset and length of the 3' consensus; out[3k..3k+2] = b5, b3,
+ * score (score -1: a string longer than 1024, not scored on the device). */
+rv_status rv_debug_find35(rv_engine* e, const char* arena, size_t arena_len, const int32_t* pairs, size_t n,
+                          int32_t* out);
 
 /* rv_debug_fisher without any allocation: no reference counterpart; for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).

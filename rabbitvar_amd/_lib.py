@@ -61,6 +61,7 @@ class RvStats(ctypes.Structure):
         ("wall_ms", c_double), ("order_sorted", c_int64), ("detail_ms", c_double * 16),
         ("call_ms", c_double), ("screen_ms", c_double), ("screen_candidates", c_int64),
         ("screen_skipped", c_int64), ("screen_list_ms", c_double),
+        ("realign_device_pairs", c_int64), ("realign_device_ms", c_double),
     ]
 
 
@@ -85,6 +86,8 @@ SIGNATURES = {
                                 ctypes.POINTER(c_size_t)]),
     "rv_free": (None, [c_void_p]),
     "rv_debug_fisher": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_int32), c_size_t, ctypes.POINTER(ctypes.c_double)]),
+    "rv_debug_find35": (c_int, [c_void_p, ctypes.c_char_p, c_size_t, ctypes.POINTER(ctypes.c_int32), c_size_t,
+                                ctypes.POINTER(ctypes.c_int32)]),
     "rv_fisher_tables_noalloc": (c_int, [ctypes.POINTER(ctypes.c_int32), c_size_t, ctypes.POINTER(ctypes.c_double)]),
     "rv_set_screen": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_float), c_size_t, ctypes.c_float]),
     "rv_screen_dataset": (c_int, [c_void_p, ctypes.POINTER(RvRegion), ctypes.POINTER(RvReads),

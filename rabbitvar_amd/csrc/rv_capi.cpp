@@ -187,6 +187,28 @@ rv_status rv_fisher_tables_noalloc(const int32_t* tables, size_t n, double* out)
     return rv::fisher_tables_noalloc(tables, (int)n, out, &err) == hipSuccess ? RV_OK : RV_ERR_DEVICE;
 }
 
+rv_status rv_debug_find35(rv_engine* e, const char* arena, size_t arena_len, const int32_t* pairs, size_t n,
+                          int32_t* out) {
+    if (!e || (n && (!arena || !pairs || !out))) return RV_ERR_ARG;
+    std::vector<std::string> strs(2 * n);
+    for (size_t k = 0; k < n; k++)
+        for (int h = 0; h < 2; h++) {
+            const int64_t off = pairs[4 * k + 2 * h], len = pairs[4 * k + 2 * h + 1];
+            if (off < 0 || len < 0 || (size_t)(off + len) > arena_len) return RV_ERR_ARG;
+            strs[2 * k + h].assign(arena + off, (size_t)len);
+        }
+    std::vector<rv::F35Job> jobs(n);
+    for (size_t k = 0; k < n; k++) jobs[k] = {&strs[2 * k], &strs[2 * k + 1]};
+    std::vector<rv::F35> res;
+    if (n && !e->e->find35(jobs, res)) return RV_ERR_DEVICE;
+    for (size_t k = 0; k < n; k++) {
+        out[3 * k] = res[k].b5;
+        out[3 * k + 1] = res[k].b3;
+        out[3 * k + 2] = res[k].score;
+    }
+    return RV_OK;
+}
+
 rv_status rv_debug_fisher(rv_engine* e, const int32_t* tables, size_t n, double* out) {
     if (!e || (n && (!tables || !out)) || n > (size_t)0x7fffffff / 4) return RV_ERR_ARG;
     if (n == 0) return RV_OK;

@@ -3,6 +3,7 @@
 
 #include "rv_order.h"
 #include "rv_fisher.h"
+#include "rv_realign_kernels.h"
 
 #include <algorithm>
 #include <chrono>
@@ -86,7 +87,7 @@ Engine::~Engine() {
                      &b_out_arena, &b_score_ctr, &b_status, &b_keys, &b_keys2, &b_kpos, &b_kpos2, &b_sort_tmp,
                      &b_kcount, &b_args, &b_kpos3, &b_nopass, &b_rows2, &b_want, &b_dup, &b_drop,
                      &b_scr_cand, &b_scr_pos, &b_scr_cum, &b_scr_feat, &b_scr_score, &b_scr_params,
-                     &b_fisher_in, &b_fisher_out, &b_lgamma};
+                     &b_fisher_in, &b_fisher_out, &b_lgamma, &b_f35_pairs, &b_f35_arena, &b_f35_out};
     // teardown: nothing to report a failure to
     for (auto* b : all)
         if (b->p) (void)hipFree(b->p);
@@ -426,6 +427,44 @@ const double* Engine::lgamma_table() {
         return nullptr;
     }
     return d;
+}
+
+bool Engine::find35(const std::vector<F35Job>& jobs, std::vector<F35>& out) {
+    static_assert(sizeof(F35Result) == sizeof(F35) && sizeof(F35) == 12, "F35 layout");
+    const size_t n = jobs.size();
+    if (n == 0 || n > (size_t)INT32_MAX / 4) return false;
+    std::vector<int32_t> pairs(4 * n);
+    size_t bytes = 0;
+    for (const F35Job& j : jobs) bytes += j.s5->size() + j.s3->size();
+    if (bytes > (size_t)INT32_MAX) return false;
+    std::string arena;
+    arena.reserve(bytes);
+    for (size_t k = 0; k < n; k++) {
+        pairs[4 * k] = (int32_t)arena.size();
+        pairs[4 * k + 1] = (int32_t)jobs[k].s5->size();
+        arena += *jobs[k].s5;
+        pairs[4 * k + 2] = (int32_t)arena.size();
+        pairs[4 * k + 3] = (int32_t)jobs[k].s3->size();
+        arena += *jobs[k].s3;
+    }
+    int32_t* dp = (int32_t*)grow(b_f35_pairs, sizeof(int32_t) * 4 * n);
+    char* da = (char*)grow(b_f35_arena, std::max<size_t>(arena.size(), 1));
+    F35Result* dout = (F35Result*)grow(b_f35_out, sizeof(F35Result) * n);
+    if (!dp || !da || !dout) return false;
+    out.resize(n);
+    if (hipMemcpyAsync(dp, pairs.data(), sizeof(int32_t) * 4 * n, hipMemcpyHostToDevice, stream) != hipSuccess) return false;
+    if (!arena.empty() && hipMemcpyAsync(da, arena.data(), arena.size(), hipMemcpyHostToDevice, stream) != hipSuccess)
+        return false;
+    if (hipEventRecord(sev[0], stream) != hipSuccess) return false;
+    if (launch_find35(dp, (int)n, da, dout, stream) != hipSuccess) return false;
+    if (hipEventRecord(sev[1], stream) != hipSuccess) return false;
+    if (hipMemcpyAsync(out.data(), dout, sizeof(F35Result) * n, hipMemcpyDeviceToHost, stream) != hipSuccess) return false;
+    if (hipStreamSynchronize(stream) != hipSuccess) return false;
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, sev[0], sev[1]);
+    f35_ms += ms;
+    f35_device_pairs += (int64_t)n;
+    return true;
 }
 
 rv_status Engine::fisher_tables(const int32_t* tab, int n, double* out) {

@@ -13,6 +13,7 @@
 #include "rabbitvar_amd.h"
 #include "rv_host.h"
 #include "rv_kernels.h"
+#include "rv_realign.h"
 
 namespace rv {
 
@@ -107,6 +108,11 @@ class Engine {
 
     // the device Fisher test on host tables (n11 n12 n21 n22) -> (two-sided, left, right) per table
     rv_status fisher_tables(const int32_t* tab, int n, double* out);
+    // find35match for a batch of consensus pairs on the device (realignlgins30's phase A); false on a
+    // device failure (the realigner then scores the pairs itself)
+    bool find35(const std::vector<F35Job>& jobs, std::vector<F35>& out);
+    int64_t f35_device_pairs = 0;  // pairs scored on the device in the last region
+    float f35_ms = 0;              // their kernel time
     // the host-built log-gamma table the device Fisher test reads (FISHER_LGT_N entries), uploaded
     // on first use; nullptr on failure (err set)
     const double* lgamma_table();
@@ -115,7 +121,7 @@ class Engine {
     DevBuf b_gather_ids, b_gather_out, b_cx_pos, b_cx_off, b_cx, b_cx_arena, b_cx_cov, b_cx_key, b_out, b_out_arena,
         b_score_ctr, b_status, b_keys, b_keys2, b_kpos, b_kpos2, b_sort_tmp, b_kcount, b_kpos3, b_args, b_nopass,
         b_want, b_dup, b_drop, b_scr_cand, b_scr_pos, b_scr_cum, b_scr_feat, b_scr_score, b_fisher_in, b_fisher_out,
-        b_lgamma;
+        b_lgamma, b_f35_pairs, b_f35_arena, b_f35_out;
     WalkArgs* args_dev = nullptr;  // the last pileup's kernel arguments (its reads stay resident)
     int64_t args_n = 0;
     hipEvent_t sev[2] = {nullptr, nullptr};

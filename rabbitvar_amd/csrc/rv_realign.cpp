@@ -26,6 +26,7 @@
 #include <iterator>
 #include <stdexcept>
 #include <thread>
+#include <unordered_map>
 
 namespace rv {
 
@@ -1170,7 +1171,48 @@ struct Realigner {
         for (size_t i = 0; i < t3.size(); i++) by_pos.push_back({t3[i].position, (int)i});
         std::sort(by_pos.begin(), by_pos.end());
         std::vector<int> ranks;
-        for (const SP& a : t5) {
+        auto window = [&](int p5, std::vector<int>& out) {
+            out.clear();
+            const double lo = p5 - maxlen() * 2.5;
+            auto it = std::lower_bound(by_pos.begin(), by_pos.end(), std::make_pair((int)std::floor(lo) - 1, -1));
+            for (; it != by_pos.end() && it->first - p5 <= maxlen() - 10; ++it)
+                if (!(p5 - it->first > maxlen() * 2.5)) out.push_back(it->second);
+            std::sort(out.begin(), out.end());
+        };
+        // Phase A: every pair the ordered pass below can score (both consensus longer than 10, the
+        // distance window and the count ratio; the `used` marks are left to that pass) is scored
+        // as one batch on the device.  The consensus strings are read without findconseq's side
+        // effects, which the ordered pass applies as the reference does.
+        std::unordered_map<uint64_t, int> f35_of;
+        std::vector<F35> f35_res;
+        if (reads) {
+            auto peek_cons = [](const HSclip& sc) -> const std::string* {
+                if (!sc.sequence.empty()) return &sc.sequence;
+                if (sc.pre_valid) return &sc.pre;
+                return nullptr;
+            };
+            std::vector<F35Job> jobs;
+            for (size_t ai = 0; ai < t5.size(); ai++) {
+                const std::string* s5 = peek_cons(*t5[ai].sc);
+                if (!s5 || s5->size() <= 10) continue;
+                window(t5[ai].position, ranks);
+                for (int ri : ranks) {
+                    const SP& b = t3[(size_t)ri];
+                    const std::string* s3 = peek_cons(*b.sc);
+                    if (!s3 || s3->size() <= 10) continue;
+                    const int cnt5 = t5[ai].count, cnt3 = b.count;
+                    if (!(cnt5 / (double)cnt3 >= 0.08 && cnt5 / (double)cnt3 <= 12)) continue;
+                    f35_of[((uint64_t)ai << 32) | (uint32_t)ri] = (int)jobs.size();
+                    jobs.push_back({s5, s3});
+                }
+            }
+            if (jobs.size() < 8 || !reads->find35_batch(jobs, f35_res) || f35_res.size() != jobs.size()) {
+                f35_of.clear();
+                f35_res.clear();
+            }
+        }
+        for (size_t ai = 0; ai < t5.size(); ai++) {
+            const SP& a = t5[ai];
             Visit visit(*this, SC_LGINS, a.li);
             int p5 = a.position;
             HSclip* sc5v = a.sc;
@@ -1178,14 +1220,7 @@ struct Realigner {
             if (sc5v->used) continue;
             const std::string& seq5 = consensus(*sc5v);
             if (seq5.size() <= 10) continue;
-            ranks.clear();
-            {
-                const double lo = p5 - maxlen() * 2.5;
-                auto it = std::lower_bound(by_pos.begin(), by_pos.end(), std::make_pair((int)std::floor(lo) - 1, -1));
-                for (; it != by_pos.end() && it->first - p5 <= maxlen() - 10; ++it)
-                    if (!(p5 - it->first > maxlen() * 2.5)) ranks.push_back(it->second);
-                std::sort(ranks.begin(), ranks.end());
-            }
+            window(p5, ranks);
             for (int ri : ranks) {
                 const SP& b = t3[(size_t)ri];
                 try {
@@ -1200,7 +1235,15 @@ struct Realigner {
                     if (seq3.size() <= 10) continue;
                     if (!(cnt5 / (double)cnt3 >= 0.08 && cnt5 / (double)cnt3 <= 12)) continue;
                     int bp5, bp3, score;
-                    find35match(seq5, seq3, bp5, bp3, score);
+                    auto f = f35_of.find(((uint64_t)ai << 32) | (uint32_t)ri);
+                    if (f != f35_of.end() && f35_res[(size_t)f->second].score >= 0) {
+                        const F35& r = f35_res[(size_t)f->second];
+                        bp5 = r.b5;
+                        bp3 = r.b3;
+                        score = r.score;
+                    } else {
+                        find35match(seq5, seq3, bp5, bp3, score);
+                    }
                     if (score == 0) continue;
                     int smscore = (int)(score / 2);
                     std::string ins = bp3 + smscore > 1 ? sub2(seq3, 0, -(bp3 + smscore) + 1) : seq3;

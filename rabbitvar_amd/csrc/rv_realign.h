@@ -84,10 +84,27 @@ struct HostRef {
 
 // noPassingReads (reference src/VariationRealigner.cpp:1447-1498) over the region's records: true
 // when no record overlapping [start, end] spans it without carrying exactly this deletion.
+// find35match's result for one (5' consensus, 3' consensus) pair
+struct F35 {
+    int b5 = 0, b3 = 0, score = 0;
+};
+struct F35Job {
+    const std::string* s5;
+    const std::string* s3;
+};
+
 class ReadQuery {
   public:
     virtual ~ReadQuery() = default;
     virtual bool no_passing(int start, int end) = 0;
+    // find35match for a batch of pairs on the device (realignlgins30's candidate pairs, scored
+    // before its ordered pass); false: not available, the realigner computes each pair itself.
+    // A result with score -1 was not computed (the host computes it).
+    virtual bool find35_batch(const std::vector<F35Job>& jobs, std::vector<F35>& out) {
+        (void)jobs;
+        (void)out;
+        return false;
+    }
 };
 
 

@@ -37,6 +37,7 @@ class DeviceReadQuery : public ReadQuery {
         if (!e_.no_passing(start, end, none)) throw std::runtime_error("noPassingReads on the device failed");
         return none;
     }
+    bool find35_batch(const std::vector<F35Job>& jobs, std::vector<F35>& out) override { return e_.find35(jobs, out); }
 
   private:
     Engine& e_;
@@ -121,6 +122,8 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
     };
     e.pinned_reset();
+    e.f35_device_pairs = 0;
+    e.f35_ms = 0;
     PileupResult pr;
     // Work that overlaps the host reductions inside pileup(): the nonInsertionVariants key order
     // (a device sort) once the dense rows are final, and the realigner's first row batch once the
@@ -468,6 +471,8 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     e.stats.order_ms = order_ms;
     e.stats.format_ms = format_ms;
     e.stats.wall_ms = ms_since(w0);
+    e.stats.realign_device_pairs = e.f35_device_pairs;
+    e.stats.realign_device_ms = e.f35_ms;
     e.stats.score_ms = score_ms;
     e.stats.host_ms += host_ms;
     e.stats.total_device_ms = e.stats.prep_ms + e.stats.walk_ms + e.stats.anchor_ms + score_ms;

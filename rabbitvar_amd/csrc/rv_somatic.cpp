@@ -403,6 +403,7 @@ class DeviceReads : public ReadQuery {
         if (!e_.no_passing(start, end, none)) throw std::runtime_error("noPassingReads on the device failed");
         return none;
     }
+    bool find35_batch(const std::vector<F35Job>& jobs, std::vector<F35>& out) override { return e_.find35(jobs, out); }
 
   private:
     Engine& e_;
@@ -581,6 +582,8 @@ rv_status run_region_somatic(Engine& e, const rv_region& region, const rv_reads&
     } guard{e, e.cfg.hasBam2};
     e.cfg.hasBam2 = 1;
     e.pinned_reset();
+    e.f35_device_pairs = 0;
+    e.f35_ms = 0;
     e.stats.score_ms = 0;
     e.stats.screen_ms = 0;
     e.stats.screen_list_ms = 0;
@@ -719,6 +722,8 @@ rv_status run_region_somatic(Engine& e, const rv_region& region, const rv_reads&
     out.calls.clear();
     out.strings.clear();
     e.stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+    e.stats.realign_device_pairs = e.f35_device_pairs;
+    e.stats.realign_device_ms = e.f35_ms;
     // the row table may hold the second sample's rows no longer: both caches released above
     e.release_later(std::move(t.S), std::move(t.rc), std::move(n.S), std::move(n.rc), std::move(t.pr),
                     std::move(n.pr));

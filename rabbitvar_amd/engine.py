@@ -124,6 +124,22 @@ class Engine:
             raise self._error(st)
         return out
 
+    def find35(self, pairs):
+        """The device's find35match on a list of (seq5, seq3) string pairs -> list of (b5, b3, score)."""
+        arena, idx = bytearray(), []
+        for a, b in pairs:
+            idx += [len(arena), len(a)]
+            arena += a.encode()
+            idx += [len(arena), len(b)]
+            arena += b.encode()
+        p = np.ascontiguousarray(idx, np.int32)
+        out = np.zeros(3 * len(pairs), np.int32)
+        st = self.lib.rv_debug_find35(self.h, bytes(arena), len(arena), p.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                      len(pairs), out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+        if st != 0:
+            raise self._error(st)
+        return [tuple(int(x) for x in out[3 * k:3 * k + 3]) for k in range(len(pairs))]
+
     def run_region(self, chrom, start, end, reads, contig_name, contig_bases, gene=None):
         region, r, contig = self._args(chrom, start, end, reads, contig_name, contig_bases, gene)
         cs = ctypes.c_void_p()

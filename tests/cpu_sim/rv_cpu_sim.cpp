@@ -48,6 +48,17 @@ int main(int argc, char** argv) {
         printf("\n");
         return 0;
     }
+    if (argc == 3 && std::string(argv[1]) == "--find35-file") {
+        // the realigner's find35match on the pairs of a file ("seq5 seq3" per line): "b5 b3 score"
+        std::ifstream f(argv[2]);
+        std::string a, b;
+        while (f >> a >> b) {
+            int b5, b3, sc;
+            find35match(a, b, b5, b3, sc);
+            printf("%d %d %d\n", b5, b3, sc);
+        }
+        return 0;
+    }
     if (argc == 4 && std::string(argv[1]) == "--desc-fuzz") {
         // the realigner's descriptor matchers (rv_desc.h) against std::regex with the reference's
         // pattern texts, on random strings over the descriptor alphabet and on descriptor shapes.

@@ -57,3 +57,48 @@ def test_long_cigar_fails_with_capacity_error(tools, tmp_path):
     ok = eng.run_region("chrT", 1, hb.L, _long_cigar_records(ref, ()), "chrT", ref.encode())
     eng.close()
     assert isinstance(ok, str)
+
+
+@pytest.mark.parametrize("side,region", [(5, (1, 600)), (5, (251, 600)), (3, (1, 600))])
+@pytest.mark.parametrize("k", [0, 1])
+def test_softclip_rescue_reads_match_oracle(tools, tmp_path, side, region, k):
+    """The soft-clip rescue known-answer reads (tests/test_oracle_softclip_kat.py) on the GPU: the
+    pileup (rescued counts, coverage, Sclip aggregates) and the calls equal the oracle's, with and
+    without realignment."""
+    from rabbitvar_amd import read_fasta, read_rvb
+    from rabbitvar_amd.engine import Engine
+    import test_oracle_softclip_kat as sk
+
+    d = str(tmp_path / f"s{side}")
+    sk.build(d, side)
+    s, e = region
+    want_out, want_dump = run_oracle(d, f"chrS:{s}-{e}", ["--fisher", "-k", str(k)], dump="pile.txt")
+    name, seq = read_fasta(os.path.join(d, "ref.fa"))
+    reads = read_rvb(os.path.join(d, "reads.rvb"))
+    eng = Engine(sample="S1", fisher=1, performLocalRealignment=k)
+    dump = eng.debug_pileup(name, s, e, reads, name, seq)
+    out = eng.run_region(name, s, e, reads, name, seq)
+    eng.close()
+    assert "S%d\t" % side in want_dump or region[0] > 1
+    assert sorted(dump.splitlines()) == sorted(want_dump.splitlines())
+    assert out == want_out
+
+
+def test_adjust_mnp_reads_match_oracle(tools, tmp_path):
+    """The adjustMNP known-answer reads (tests/test_oracle_realign_kat.py) on the GPU: pileup and
+    calls equal the oracle's."""
+    from rabbitvar_amd import read_fasta, read_rvb
+    from rabbitvar_amd.engine import Engine
+    import test_oracle_realign_kat as rk
+
+    d = str(tmp_path / "mnp")
+    rk.build(d)
+    want_out, want_dump = run_oracle(d, "chrS:1-600", ["--fisher"], dump="pile.txt")
+    name, seq = read_fasta(os.path.join(d, "ref.fa"))
+    reads = read_rvb(os.path.join(d, "reads.rvb"))
+    eng = Engine(sample="S1", fisher=1)
+    dump = eng.debug_pileup(name, 1, 600, reads, name, seq)
+    out = eng.run_region(name, 1, 600, reads, name, seq)
+    eng.close()
+    assert sorted(dump.splitlines()) == sorted(want_dump.splitlines())
+    assert out == want_out and "\t301\t302\t" in out
