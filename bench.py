@@ -324,6 +324,8 @@ def main():
                                               "key_sort", "order_wait", "aligned_order", "rank_sort", "reduce_events",
                                               "reduce_clips", "nopass", "row_miss_fetch", "n_nopass", "n_row_miss_fetches"],
                                              [round(x, 2) for x in st["detail_ms"]])),
+        "realign_device": {"kernel": "k_find35", "pairs": st["realign_device_pairs"],
+                           "ms": round(st["realign_device_ms"], 4)},
         "roofline": {"kernel": "k_walk", "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": round(avg_walk_s * 1e3, 4)},

@@ -6,9 +6,12 @@
 // tokens, so the two are independent formulations of one specification).  The regex texts are
 // the rules' specification and are kept identical.
 //
-// Parity unpinned: no golden vector, known answer or closed form fixes these rules independently
-// (the reference ships no tests and cannot be built here); agreement with the engine's separate
-// formulation on synthetic data is the only check.
+// Pinned by tests/test_oracle_cigar_kat.py: 35 reads, each answer (the walked CIGAR, its start and
+// modifyCigar's return) worked out by hand from the reference text rule by rule, cover the indel
+// loop (clip/indel folds at both ends, short-M shifts, close-indel merges into complex forms, the
+// three-indel forms, the negative-length restore), the record quirks (leading or trailing I/D) and
+// the tail rules (end mismatches to clips, 5' and 3' clip rescue, low-quality absorption).  The
+// engine's separate token matcher runs the same reads through the CPU harness.
 #include <regex>
 #include <stdexcept>
 #include <string>

@@ -15,8 +15,12 @@
 //
 // Pinned by tests/test_oracle_cases.py (hand-built reads: match runs, substitutions, deletions,
 // insertions, coverage, strand and quality sums, read-position statistic, pstd/qstd).
-// Parity unpinned: the soft-clip rescue and consensus tables, findOffset / appendSegments (complex
-// M-run descriptors) and the insertion-anchor bookkeeping have no independent check.
+// Pinned by tests/test_oracle_softclip_kat.py: the soft-clip rescue loops and the Sclip consensus
+// counts of process_softclip (5' and 3', a low-quality stop, the region check), worked out by hand.
+// The MNV join of the M loop is pinned by tests/test_oracle_realign_kat.py.
+// Parity unpinned: findOffset / appendSegments on long complex M-run descriptors and the
+// insertion-anchor bookkeeping have no known answer of their own; agreement with the engine's
+// separate formulation on synthetic data is their only check.
 #include <algorithm>
 #include <cstdlib>
 #include <regex>

@@ -15,9 +15,12 @@
 // and include/VariationUtils.h (adjCnt :291-332, correctCnt :263-280, findconseq :403-509,
 // islowcomplexseq :367-394, joinRef :336-349), include/util.h (vc_substr :27-57, charAt :171-179).
 //
-// Parity unpinned: the realigner's decisions have no golden vector or closed-form check (the
-// reference ships no tests and cannot be built here); agreement with the engine's separate host
-// restatement on synthetic data is the only check.
+// Pinned by tests/test_oracle_realign_kat.py: adjustMNP absorbing the right-hand piece of an MNV,
+// worked out by hand.  The candidate order the realigner visits is pinned by
+// tests/test_oracle_rhref.py against the reference's own robin_hood.h.
+// Parity unpinned: the other realigner decisions (realigndel / realignins / realignlgdel /
+// realignlgins30 and the soft-clip consensus matching) have no known answer of their own; agreement
+// with the engine's separate host restatement on synthetic data is their only check.
 #include <algorithm>
 #include <cmath>
 #include <regex>

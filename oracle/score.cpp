@@ -21,8 +21,10 @@
 //
 // Pinned: kt_fisher_two_sided by tests/test_oracle_fisher.py (exact rational enumeration, symmetry,
 // scipy); the printed fields of SNV / deletion / insertion calls by tests/test_oracle_cases.py.
-// Parity unpinned: complex-allele construction in collectReferenceVariants (&, #, ^ descriptors)
-// and findMSI have no independent check.
+// findMSI by tests/test_oracle_findmsi_kat.py (hand-worked microsatellite counts and 3' shifts);
+// the MNV allele printed from an '&' descriptor by tests/test_oracle_realign_kat.py.
+// Parity unpinned: the '#' and '^' descriptor forms of collectReferenceVariants have no known
+// answer of their own.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
