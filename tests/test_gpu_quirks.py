@@ -84,15 +84,17 @@ def test_softclip_rescue_reads_match_oracle(tools, tmp_path, side, region, k):
     assert out == want_out
 
 
-def test_adjust_mnp_reads_match_oracle(tools, tmp_path):
-    """The adjustMNP known-answer reads (tests/test_oracle_realign_kat.py) on the GPU: pileup and
+@pytest.mark.parametrize("case", ["mnp", "realigndel", "realignins"])
+def test_realigner_known_answer_reads_match_oracle(tools, tmp_path, case):
+    """The realigner known-answer reads (tests/test_oracle_realign_kat.py: adjustMNP taking over an
+    MNV's right piece, realigndel and realignins absorbing end mismatches) on the GPU: pileup and
     calls equal the oracle's."""
     from rabbitvar_amd import read_fasta, read_rvb
     from rabbitvar_amd.engine import Engine
     import test_oracle_realign_kat as rk
 
-    d = str(tmp_path / "mnp")
-    rk.build(d)
+    d = str(tmp_path / case)
+    {"mnp": rk.build, "realigndel": rk.build_realigndel, "realignins": rk.build_realignins}[case](d)
     want_out, want_dump = run_oracle(d, "chrS:1-600", ["--fisher"], dump="pile.txt")
     name, seq = read_fasta(os.path.join(d, "ref.fa"))
     reads = read_rvb(os.path.join(d, "reads.rvb"))
@@ -100,5 +102,9 @@ def test_adjust_mnp_reads_match_oracle(tools, tmp_path):
     dump = eng.debug_pileup(name, 1, 600, reads, name, seq)
     out = eng.run_region(name, 1, 600, reads, name, seq)
     eng.close()
+    print(f"realigner known answer {case}: {len(want_out.splitlines())} lines", flush=True)
     assert sorted(dump.splitlines()) == sorted(want_dump.splitlines())
-    assert out == want_out and "\t301\t302\t" in out
+    assert out == want_out
+    anchor = {"mnp": "\t301\t302\t", "realigndel": f"\t{rk.P_DEL - 1}\t{rk.P_DEL + 2}\t",
+              "realignins": f"\t{rk.P_INS}\t{rk.P_INS}\t"}[case]
+    assert anchor in out

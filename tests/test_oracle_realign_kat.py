@@ -14,6 +14,44 @@ MNV (extracnt += 2, pstd = qstd = true), refCoverage[301] grows by 2, and "Y" is
 No soft clip sits at 301 or 303, and no deletion or insertion exists, so nothing else changes.
 The printed MNV: refallele R(301..302), varallele "XY" (collectReferenceVariants: the '&' removed,
 one reference base appended), start 301, end 302, type Complex (the first bases differ).
+
+A second case pins a realigndel absorption (src/VariationRealigner.cpp:489-649, findMM3 :1917-1970):
+a 3 bp deletion at p (deleted bases p..p+2) carried by six reads, and two reads that end at p+3
+without the gap, so that their last four bases (R(p+3..p+6), the sequence after the deletion)
+mismatch R(p..p+3).  p is the first position where all four differ.
+  * The CIGAR rewrite keeps "100M" for the two reads (four end mismatches, more than three), and the
+    M-loop joins the four high-quality mismatches into the MNV "X&YZW" at p, read position 1 from
+    the 3' end (tp = rlen1 - rpe = 1), quality 30.
+  * adjustMNP finds none of its pieces ("X", "X&Y", "X&YZ" at p; "Y&ZW", "Z&W", "W" at p+1..p+3):
+    no read carries them.
+  * realigndel visits "-3" at p (dcnt 6, dellen 3, no extra bases).  findMM3(ref, p, R(p+3..))
+    skips no match (R(p) != R(p+3)), so Tbp = p, and collects "X", "XY", "XYZ", "XYZW" at p (four
+    mismatches, mcnt <= 3); str has four bases, so nm3 = 0.  findMM5's candidates sit at p+2, p+1
+    (no key there) and p (no such allele).  "X&YZW" passes every filter: quality 30 >= goodq,
+    mean read position 1 <= nm3 + 4, count 2 < dcnt + dellen and 2 / 6 < 8.  mp = p is not > p,
+    so adjCnt(vref, tv) without a reference allele: every count and sum of the MNV is added to the
+    deletion (extracnt += 2, pstd = qstd = true) and the MNV is erased from p (p keeps "-3").
+  * The clip loops find no soft clip, and pe - p = 3 < 5 skips the noPassingReads branch.
+The printed deletion: start p - 1, end p + 2, refallele R(p-1..p+2), varallele R(p-1), variant
+count 8 (forward 4, reverse 4); no line is left at p for the MNV.
+
+A third case pins a realignins absorption (src/VariationRealigner.cpp:785-1066): a 3 bp insertion
+"+ABC" after p carried by six reads ("50M3I50M"), and two reads ("100M" over p-95..p+4) that hold
+A, B, C, R(p+1) at p+1..p+4 instead.  The premises: A != R(p+1), B != R(p+2), C != R(p+3),
+R(p+1) != R(p+4) (four end mismatches), C != R(p) (adjInsPos shifts nothing), B != R(p-1) and
+A != R(p-2) (findMM5 collects three mismatches, so none of its single-mismatch bookkeeping runs).
+  * The two reads keep "100M" and carry the MNV "A&BCR" at p+1 (read position 1, quality 30).
+  * adjustMNP finds none of its pieces (R(p+1) != R(p+4) keeps "R" off p+4).
+  * realignins visits "+ABC" (insertionCount 6, insert1 "ABC", tn "ABC").  findMM3(ref, p+1,
+    "ABC" + R(p+1..)) has Tbp = p+1 and collects "A", "AB", "ABC", "ABCR" (nm3 = 0); findMM5's
+    candidates "C" at p, "B&C" at p-1, "A&BC" at p-2 carry no reads.  "A&BCR" passes the filters
+    (2 < 6 + 3, 2 / 6 < 8); mp = p+1 > p at the 3' end and p holds the reference allele R(p), so
+    adjCnt(vref, tv, lref): the MNV's counts and sums move to the insertion (extracnt += 2,
+    pstd = qstd = true) and are taken off R(p) at p (no field goes negative, so correctCnt changes
+    nothing); the MNV is erased from p+1.
+  * No soft clip; first3 = first5 = p+1, so the adjRefFactor step does not run.
+The printed insertion: start p, end p, refallele R(p), varallele R(p) + "ABC", variant count 8
+(forward 4, reverse 4); no line is left at p+1 for the MNV.
 """
 import os
 import subprocess
@@ -93,3 +131,135 @@ def test_adjust_mnp_takes_over_right_piece(tools, tmp_path):
     assert f[7:9] == ["12", "7"]  # refCoverage at 301 after the takeover (10 + 2), the MNV's count
     assert f[35] == "Complex"
     assert not [g for g in lines if g[3] == "302"]
+
+
+P_DEL = kat.pick(lambda p: all(R(p + k) != R(p + 3 + k) for k in range(4)), 200, 400)
+
+
+def build_realigndel(d):
+    from rabbitvar_amd import batch_from_records, write_rvb
+
+    p = P_DEL
+    recs = []
+    for r in range(6):  # "50M3D50M": deleted bases p..p+2
+        s0 = p - 50
+        seq = kat.REF[s0 - 1:p - 1] + kat.REF[p + 2:p + 52]
+        recs.append({"pos0": s0 - 1, "flag": 16 if r % 2 else 0, "mapq": 60, "nm": 3, "seq": seq,
+                     "qual": [30] * 100, "cigar": [(0, 50), (2, 3), (0, 50)]})
+    for r in range(2):  # "100M" over p-96..p+3, the last four bases from after the deletion
+        s0 = p - 96
+        seq = kat.REF[s0 - 1:p - 1] + kat.REF[p + 2:p + 6]
+        assert len(seq) == 100
+        recs.append({"pos0": s0 - 1, "flag": 16 if r % 2 else 0, "mapq": 60, "nm": 4, "seq": seq,
+                     "qual": [30] * 100, "cigar": [(0, 100)]})
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, "ref.fa"), "w") as f:
+        f.write(">chrS\n" + kat.REF + "\n")
+    write_rvb(os.path.join(d, "reads.rvb"), batch_from_records(recs))
+
+
+def test_realigndel_absorbs_end_mismatches(tools, tmp_path):
+    p = P_DEL
+    d = str(tmp_path / "del")
+    build_realigndel(d)
+    out, pile, real = (os.path.join(d, x) for x in ("o.txt", "p.txt", "r.txt"))
+    subprocess.run([ORACLE, "-G", os.path.join(d, "ref.fa"), "-b", os.path.join(d, "reads.rvb"), "-N", "S1", "-R",
+                    "chrS:1-600", "--fisher", "--out", out, "--dump-pileup", pile, "--dump-realigned", real],
+                   check=True, capture_output=True)
+    V0, _ = parse(pile)
+    V1, _ = parse(real)
+    dl = (p, "-3")
+    mnv = (p, R(p + 3) + "&" + R(p + 4) + R(p + 5) + R(p + 6))
+    # before: six deletion reads (3 forward, 3 reverse), two MNV reads (1 and 1), all high quality
+    assert V0[dl][:5] == [6, 3, 3, 6, 0]
+    assert V0[mnv][:5] == [2, 1, 1, 2, 0]
+    assert V0[mnv][5] == 2.0 and V0[mnv][6] == 60.0  # read position 1 and quality 30, per read
+    # after: the MNV's counts and sums added to the deletion, extracnt 2, pstd = qstd = 1
+    want = [a + b for a, b in zip(V0[dl][:9], V0[mnv][:9])] + [1, 1, V0[dl][11] + 2]
+    assert V1[dl] == want
+    assert mnv not in V1
+    assert {k: v for k, v in V1.items() if k != dl} == {k: v for k, v in V0.items() if k not in (dl, mnv)}
+    lines = [ln.split("\t") for ln in open(out).read().splitlines()]
+    dline = [f for f in lines if f[3] == str(p - 1)]
+    assert len(dline) == 1
+    f = dline[0]
+    assert f[3:7] == [str(p - 1), str(p + 2), kat.Rs(p - 1, p + 2), R(p - 1)]
+    assert f[8] == "8" and f[11:13] == ["4", "4"]
+    assert not [g for g in lines if g[3] == str(p)]
+
+
+
+def parse_all(path):
+    V, I = {}, {}
+    for ln in open(path).read().splitlines():
+        f = ln.split("\t")
+        if f[0] in ("V", "I"):
+            (V if f[0] == "V" else I)[(int(f[1]), f[2])] = ([int(x) for x in f[3:8]] + [float(x) for x in f[8:12]] +
+                                                             [int(x) for x in f[12:15]])
+    return V, I
+
+
+def _ins_premises(p):
+    a = next((b for b in "ACGT" if b != R(p + 1) and b != R(p - 2)), None)
+    b = next((x for x in "ACGT" if x != R(p + 2) and x != R(p - 1)), None)
+    c = next((x for x in "ACGT" if x != R(p + 3) and x != R(p)), None)
+    if None in (a, b, c) or R(p + 1) == R(p + 4):
+        return None
+    return a + b + c
+
+
+P_INS = kat.pick(lambda p: _ins_premises(p) is not None, 200, 400)
+INS = _ins_premises(P_INS)
+
+
+def build_realignins(d):
+    from rabbitvar_amd import batch_from_records, write_rvb
+
+    p = P_INS
+    recs = []
+    for r in range(6):  # "50M3I50M": the insertion after p
+        s0 = p - 49
+        seq = kat.Rs(s0, p) + INS + kat.Rs(p + 1, p + 50)
+        recs.append({"pos0": s0 - 1, "flag": 16 if r % 2 else 0, "mapq": 60, "nm": 3, "seq": seq,
+                     "qual": [30] * 103, "cigar": [(0, 50), (1, 3), (0, 50)]})
+    for r in range(2):  # "100M" over p-95..p+4
+        s0 = p - 95
+        seq = kat.Rs(s0, p) + INS + R(p + 1)
+        assert len(seq) == 100
+        recs.append({"pos0": s0 - 1, "flag": 16 if r % 2 else 0, "mapq": 60, "nm": 4, "seq": seq,
+                     "qual": [30] * 100, "cigar": [(0, 100)]})
+    os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, "ref.fa"), "w") as f:
+        f.write(">chrS\n" + kat.REF + "\n")
+    write_rvb(os.path.join(d, "reads.rvb"), batch_from_records(recs))
+
+
+def test_realignins_absorbs_end_mismatches(tools, tmp_path):
+    p = P_INS
+    d = str(tmp_path / "ins")
+    build_realignins(d)
+    out, pile, real = (os.path.join(d, x) for x in ("o.txt", "p.txt", "r.txt"))
+    subprocess.run([ORACLE, "-G", os.path.join(d, "ref.fa"), "-b", os.path.join(d, "reads.rvb"), "-N", "S1", "-R",
+                    "chrS:1-600", "--fisher", "--out", out, "--dump-pileup", pile, "--dump-realigned", real],
+                   check=True, capture_output=True)
+    V0, I0 = parse_all(pile)
+    V1, I1 = parse_all(real)
+    ins = (p, "+" + INS)
+    mnv = (p + 1, INS[0] + "&" + INS[1:] + R(p + 1))
+    rf = (p, R(p))
+    assert I0[ins][:5] == [6, 3, 3, 6, 0]
+    assert V0[mnv][:5] == [2, 1, 1, 2, 0]
+    assert V0[mnv][5] == 2.0 and V0[mnv][6] == 60.0
+    assert all(a >= b for a, b in zip(V0[rf][:9], V0[mnv][:9]))  # nothing for correctCnt to clamp
+    assert I1[ins] == [a + b for a, b in zip(I0[ins][:9], V0[mnv][:9])] + [1, 1, I0[ins][11] + 2]
+    assert V1[rf] == [a - b for a, b in zip(V0[rf][:9], V0[mnv][:9])] + V0[rf][9:]
+    assert mnv not in V1
+    assert {k: v for k, v in V1.items() if k != rf} == {k: v for k, v in V0.items() if k not in (rf, mnv)}
+    assert {k: v for k, v in I1.items() if k != ins} == {k: v for k, v in I0.items() if k != ins}
+    lines = [ln.split("\t") for ln in open(out).read().splitlines()]
+    iline = [f for f in lines if f[3] == str(p) and f[6] == R(p) + INS]
+    assert len(iline) == 1
+    f = iline[0]
+    assert f[3:7] == [str(p), str(p), R(p), R(p) + INS]
+    assert f[8] == "8" and f[11:13] == ["4", "4"]
+    assert not [g for g in lines if g[3] == str(p + 1)]
