@@ -4,6 +4,7 @@
 // host engine (RV_HD = __host__ __device__).  The host-side uses are bookkeeping on records the
 // device produced (descriptor strings, counts); no stage of the path runs on the CPU twice.
 #pragma once
+#include <cstddef>
 // Includers provide the __host__/__device__ qualifiers (hip/hip_runtime.h in the HIP sources).
 #include <stddef.h>
 #include <stdint.h>
@@ -201,6 +202,11 @@ struct DenseBase {
     uint32_t pad[3];
 };
 static_assert(sizeof(DenseBase) == 64, "DenseBase layout");
+// the walk's flush adds (cnt, fwd), (rev, hi), (lo, sum_tp) and (sum_q, sum_mq) as 64-bit pairs
+static_assert(offsetof(DenseBase, cnt) == 0 && offsetof(DenseBase, fwd) == 4 && offsetof(DenseBase, rev) == 8 &&
+                  offsetof(DenseBase, hi) == 12 && offsetof(DenseBase, lo) == 16 && offsetof(DenseBase, sum_tp) == 20 &&
+                  offsetof(DenseBase, sum_q) == 24 && offsetof(DenseBase, sum_mq) == 28,
+              "DenseBase pairs for 64-bit atomics");
 struct DenseRow {
     DenseBase b[4];
     int32_t cov;       // refCoverage

@@ -9,8 +9,12 @@ namespace rv {
 
 __global__ void __launch_bounds__(256) k_prep(const WalkArgs* A, PrepRec* prep, uint32_t* cig, WalkCounters* ctr,
                                                 int* max_span) {
+    // arguments as values, so that the stores below cannot force reloads (see k_walk)
+    const ReadsDev R = A->R;
+    const Window W = A->W;
+    const Config cfg = A->cfg;
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < A->R.n) prep_one(r, A->R, A->W, A->cfg, prep, cig, ctr, max_span, A->dup);
+    if (r < R.n) prep_one(r, R, W, cfg, prep, cig, ctr, max_span, A->dup);
 }
 
 // -t: marks the reads the record filter drops as duplicates (before prep, which then skips them)
@@ -34,24 +38,30 @@ __global__ void __launch_bounds__(WALK_THREADS) k_walk(const WalkArgs* A, const 
     // scalar registers
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / warpSize));
     const int lane = (int)(threadIdx.x % warpSize);
-    const int64_t n = A->R.n;
+    // the launch arguments as values: read through a pointer, the compiler would reload them from
+    // global memory after every store and atomic of the walk (they could alias)
+    const ReadsDev R = A->R;
+    const Window W = A->W;
+    const Config cfg = A->cfg;
+    const WalkSink S = A->S;
+    const int64_t n = R.n;
     const int64_t r0 = (int64_t)blockIdx.x * WALK_READS;
     const int tile_lo = (r0 < n ? prep[r0].position : 0) - TILE_MARGIN;
     // reads after each wavefront's first are handed out as wavefronts finish (reads differ in cost;
     // every contribution merges the same way whichever wavefront walks it)
     __shared__ int walk_next;
     if (threadIdx.x == 0) walk_next = waves;
-    tile_init((int)threadIdx.x, (int)blockDim.x, A->W, tile_lo);
+    tile_init((int)threadIdx.x, (int)blockDim.x, W, tile_lo);
     __syncthreads();
     for (int k = wave; k < WALK_READS;) {
         const int64_t r = r0 + k;
-        if (r < n) walk_one(r, A->R, A->W, A->cfg, prep, cig, A->S, lane, wave, tile_lo);
+        if (r < n) walk_one(r, R, W, cfg, prep, cig, S, lane, wave, tile_lo);
         int nk = 0;
         if (lane == 0) nk = atomicAdd(&walk_next, 1);
         k = __builtin_amdgcn_readfirstlane(__shfl(nk, 0));
     }
     __syncthreads();
-    tile_flush(tile_lo, A->S.dense, A->S.dense_lo, A->S.dense_n, (int)threadIdx.x, (int)blockDim.x);
+    tile_flush(tile_lo, S.dense, S.dense_lo, S.dense_n, (int)threadIdx.x, (int)blockDim.x);
 }
 
 __global__ void __launch_bounds__(256) k_anchor(const Anchor* A, const unsigned* n_anchors, unsigned cap, DenseRow* rows,

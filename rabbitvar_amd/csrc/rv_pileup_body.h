@@ -317,7 +317,7 @@ __device__ inline void walk_one(int64_t r, const ReadsDev& R, const Window& W, c
     w.use_tile = true;
     w.tile_lo = tile_lo;
     w.ref.leader = lane == 0;
-    w.c = &cfg;
+    w.cf = cfg;
     w.Rs = W.region_start;
     w.Re = W.region_end;
     w.chrLen = W.chr_len;
@@ -327,11 +327,10 @@ __device__ inline void walk_one(int64_t r, const ReadsDev& R, const Window& W, c
     w.ref.ctr = sink.ctr;
     w.ref.slo = tile_lo - REF_BEFORE;
     w.ref.sn = REF_STAGE;
-    w.S = &sink;
+    w.S = sink;
     w.seq = gseq;
     w.qual = gqual;
-    w.stage = st;
-    w.strs = walk_str[wave];
+    w.wave = wave;
     w.staged = staged;
     w.lseq = p.lseq;
     {
@@ -386,19 +385,21 @@ __device__ inline void tile_init(int tid, int nthreads, const Window& W, int til
     }
     for (int k = tid; k < TF_N * 4 * TILE_POS; k += nthreads) {
         const int f = k / (4 * TILE_POS), r = k - f * 4 * TILE_POS;
-        walk_tile.f[f][r / TILE_POS][r % TILE_POS] = f == TF_MINORD ? -1 : 0;
+        walk_tile.f[f][r / TILE_POS][r % TILE_POS] = (f == TF_MINORD || f == TF_AND) ? ~0u : 0u;
     }
     for (int k = tid; k < TILE_POS; k += nthreads) {
         walk_tile.cov[k] = 0;
-        walk_tile.rowflags[k] = 0;
         walk_tile.first_touch[k] = ~0ull;
     }
 }
 
-// Adds the tile into the dense rows.  For the pstd/qstd state: the tile's own "differs" flags
-// cover pairs inside the workgroup, and comparing the tile's first (position, quality) value with
-// the row's covers pairs across workgroups (all values equal within each side, so the two sides
-// differ exactly when their first values do).
+// Adds the tile into the dense rows.  Pairs of adjacent 32-bit row fields are added with one 64-bit
+// atomic (the lower field of each pair never goes negative during the walk, so no carry crosses
+// into the upper one; the upper field wraps as its own 32-bit value).  For the pstd/qstd state: the
+// tile's AND/OR of its (position, quality) values covers pairs inside the workgroup, and comparing
+// one tile value with the row's first value covers pairs across workgroups.  The tile value is the
+// AND: exact in each field where the tile's values are all equal, and where they are not, that
+// field's flag is already set, so a comparison against it can only set a flag that is set anyway.
 __device__ inline void tile_flush(int tile_lo, DenseRow* rows, int dense_lo, int dense_n, int tid, int nthreads) {
     const TileAcc* t = &walk_tile;
     for (int k = tid; k < 4 * TILE_POS; k += nthreads) {
@@ -408,30 +409,39 @@ __device__ inline void tile_flush(int tile_lo, DenseRow* rows, int dense_lo, int
         DenseRow& r = rows[ri];
         if (b == 0) {
             if (t->cov[i]) atomicAdd(&r.cov, t->cov[i]);
-            if (t->rowflags[i]) atomicOr(&r.rowflags, t->rowflags[i]);
-            if (t->first_touch[i] != ~0ull) atomicMin(&r.first_touch, t->first_touch[i]);
-        }
-        const unsigned fl = (unsigned)t->f[TF_FLAGS][b][i];
-        if (fl == 0) continue;  // no contribution to this base
-        DenseBase& d = r.b[b];
-        const int32_t* v[9] = {&t->f[TF_CNT][b][i], &t->f[TF_FWD][b][i], &t->f[TF_REV][b][i], &t->f[TF_HI][b][i],
-                               &t->f[TF_LO][b][i],  &t->f[TF_TP][b][i],  &t->f[TF_Q][b][i],   &t->f[TF_MQ][b][i],
-                               &t->f[TF_NM][b][i]};
-        int32_t* g[9] = {&d.cnt, &d.fwd, &d.rev, &d.hi, &d.lo, &d.sum_tp, &d.sum_q, &d.sum_mq, &d.sum_nm};
-        for (int f = 0; f < 9; f++)
-            if (*v[f]) atomicAdd(g[f], *v[f]);
-        atomicMin(&d.min_ord, (unsigned)t->f[TF_MINORD][b][i]);
-        unsigned add = fl;
-        const unsigned first = (unsigned)t->f[TF_FIRST][b][i];
-        if (first != 0u) {
-            const unsigned old = atomicCAS(&d.first_tpq, 0u, first);
-            if (old != 0u && old != first) {
-                const unsigned o = old - 1u, nv = first - 1u;
-                if ((o >> 8) != (nv >> 8)) add |= 2u;
-                if ((o & 0xffu) != (nv & 0xffu)) add |= 4u;
+            if (t->first_touch[i] != ~0ull) {
+                atomicOr(&r.rowflags, 1u);
+                atomicMin(&r.first_touch, t->first_touch[i]);
             }
         }
-        if ((d.flags & add) != add) atomicOr(&d.flags, add);
+        const unsigned fl = t->f[TF_FLAGS][b][i];
+        if (fl == 0) continue;  // no contribution to this base
+        DenseBase& d = r.b[b];
+        const unsigned cw = t->f[TF_CNT][b][i];
+        const unsigned fwd = cw & 0x3ffu, rev = (cw >> 10) & 0x3ffu, hi = (cw >> 20) & 0x3ffu;
+        const unsigned cnt = fwd + rev, lo = cnt - hi;
+        const unsigned qmq = t->f[TF_QMQ][b][i];
+        auto pair = [](unsigned lo32, unsigned hi32) { return (unsigned long long)lo32 | ((unsigned long long)hi32 << 32); };
+        atomicAdd((unsigned long long*)&d.cnt, pair(cnt, fwd));
+        atomicAdd((unsigned long long*)&d.rev, pair(rev, hi));
+        atomicAdd((unsigned long long*)&d.lo, pair(lo, t->f[TF_TP][b][i]));
+        atomicAdd((unsigned long long*)&d.sum_q, pair(qmq & 0xffffu, qmq >> 16));
+        if (t->f[TF_NM][b][i]) atomicAdd((unsigned*)&d.sum_nm, t->f[TF_NM][b][i]);
+        atomicMin(&d.min_ord, t->f[TF_MINORD][b][i]);
+        unsigned add = fl & 9u;
+        if (fl & TF_HASPQ) {
+            const unsigned va = t->f[TF_AND][b][i], diff = va ^ t->f[TF_OR][b][i];
+            if (diff >> 8) add |= 2u;
+            if (diff & 0xffu) add |= 4u;
+            const unsigned first = va + 1u;
+            const unsigned old = atomicCAS(&d.first_tpq, 0u, first);
+            if (old != 0u && old != first) {
+                const unsigned o = old - 1u;
+                if ((o >> 8) != (va >> 8)) add |= 2u;
+                if ((o & 0xffu) != (va & 0xffu)) add |= 4u;
+            }
+        }
+        atomicOr(&d.flags, add);
     }
 }
 

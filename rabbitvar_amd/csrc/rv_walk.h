@@ -27,17 +27,23 @@ constexpr int DCAP = 640;  // allele descriptor capacity
 typedef FStr<DCAP> DStr;
 
 // Workgroup-local accumulator for a window of TILE_POS consecutive positions (LDS).  The reads of
-// one workgroup add their single-base contributions here with LDS atomics; the tile is then added
-// to the dense rows once per touched (position, base).  Every field merges exactly: counts and sums
-// add, creation ordinals take the minimum, flags OR, and the first (position, quality) value is
-// compared with the rows' own first value (see tile_flush).
+// one workgroup add their single-base contributions here with LDS atomics that return nothing (no
+// lane waits on them); the tile is then added to the dense rows once per touched (position, base).
+// Every field merges exactly: counts and sums add, creation ordinals take the minimum, flags OR.
+// The per-tile fields are packed: TF_CNT holds the forward, reverse and high-quality counts in
+// 10-bit fields (cnt = fwd + rev and lo = cnt - hi always hold), TF_QMQ the quality and mapping
+// quality sums in 16-bit halves, and the (position, quality) values that decide pstd / qstd are
+// kept as their AND and OR (all equal exactly when AND == OR, field by field).  A read adds to a
+// (position, base) at most twice (its aligned base and a rescued clip base lie on different
+// positions, so in practice once), so a workgroup's sums stay within the packed widths.
 constexpr int TILE_POS = 512;
-enum : int { TF_CNT, TF_FWD, TF_REV, TF_HI, TF_LO, TF_TP, TF_Q, TF_MQ, TF_NM, TF_MINORD, TF_FLAGS, TF_FIRST, TF_N };
+enum : int { TF_CNT, TF_TP, TF_QMQ, TF_NM, TF_MINORD, TF_FLAGS, TF_AND, TF_OR, TF_N };
+constexpr unsigned TF_REV1 = 1u << 10, TF_HI1 = 1u << 20;  // TF_CNT increments (forward: 1)
+constexpr unsigned TF_HASPQ = 16u;                         // TF_FLAGS: a pp/pq-updating contribution
 struct TileAcc {
-    int32_t f[TF_N][4][TILE_POS];  // field, base, position
+    uint32_t f[TF_N][4][TILE_POS];  // field, base, position
     int32_t cov[TILE_POS];
-    uint32_t rowflags[TILE_POS];
-    unsigned long long first_touch[TILE_POS];
+    unsigned long long first_touch[TILE_POS];  // ~0: no key created (the row's rowflags bit stays 0)
 };
 // the walk kernel's tile, addressed directly so that its accesses are LDS instructions
 static __shared__ TileAcc walk_tile;
@@ -48,6 +54,7 @@ static __shared__ TileAcc walk_tile;
 constexpr int WALK_THREADS = 1024;
 constexpr int WALK_READS = 64;
 constexpr int TILE_MARGIN = 64;
+static_assert(2 * WALK_READS < 1024 && 2 * WALK_READS * 255 < 65536, "packed tile fields hold a workgroup's sums");
 
 // Per-wavefront copy of the read being walked (bases, qualities, rewritten CIGAR), loaded with one
 // coalesced pass so that the lane-uniform steps read LDS instead of waiting on global memory.
@@ -114,9 +121,13 @@ struct RefWin {
     bool leader;
     int slo, sn;  // staged copy: walk_ref[k - slo] for k - slo in [0, sn)
     __device__ __forceinline__ bool has(int k) const { return k >= lo && k <= hi; }
+    // The staged copy is read with LDS instructions and the rare position outside it with a global
+    // load.  The volatile access keeps the compiler from merging the two loads into one generic
+    // (flat) load of a selected pointer, which would put a global-memory wait on every read.
     __device__ __forceinline__ char at(int k) const {
         const unsigned j = (unsigned)(k - slo);
-        return j < (unsigned)sn ? walk_ref[j] : ref[k - lo];
+        if (j < (unsigned)sn) return walk_ref[j];
+        return *(volatile const char*)(ref + (k - lo));
     }
     // robin_hood operator[]: a missing key would be created with '\0'
     __device__ __forceinline__ char get(int k) const {
@@ -131,18 +142,18 @@ struct RefWin {
 // Every member function is force-inlined: one out-of-line call makes the walker live in per-lane
 // scratch memory, which puts a memory round trip on every state access of the walk.
 struct Walker {
-    const Config* c;
+    Config cf;           // by value: no reload through memory the walk writes
     int Rs, Re;          // region
     int chrLen;
     RefWin ref;
-    const WalkSink* S;
+    WalkSink S;
     // read
     const char* seq;
     const uint8_t* qual;
     uint8_t qual_before0;
     int lseq;
-    WaveStage* stage;    // this wavefront's copy of the read
-    DStr* strs;          // this wavefront's descriptor strings (walk_str)
+    int wave;            // wavefront index in the workgroup: its read copy (walk_stage) and
+                         // descriptor strings (walk_str)
     bool staged;         // bases and qualities are in stage (else read through seq / qual)
     int rec_n;           // valid emulated words
     int n_cigar;
@@ -160,9 +171,17 @@ struct Walker {
     bool use_tile; // accumulate into walk_tile (false: dense rows directly)
     int tile_lo;   // position of tile slot 0
 
-    __device__ __forceinline__ uint32_t cw(int i) const { return (i >= 0 && i < rec_n) ? stage->cig[i] : 0u; }
-    __device__ __forceinline__ char base_at(int i) const { return staged ? stage->seq[i] : seq[i]; }
-    __device__ __forceinline__ int qual_at(int i) const { return staged ? stage->qual[i] : qual[i]; }
+    __device__ __forceinline__ uint32_t cw(int i) const { return (i >= 0 && i < rec_n) ? walk_stage[wave].cig[i] : 0u; }
+    // staged: the wavefront's LDS copy; otherwise global memory (volatile for the reason RefWin::at
+    // gives)
+    __device__ __forceinline__ char base_at(int i) const {
+        if (staged) return walk_stage[wave].seq[i];
+        return *(volatile const char*)(seq + i);
+    }
+    __device__ __forceinline__ int qual_at(int i) const {
+        if (staged) return walk_stage[wave].qual[i];
+        return *(volatile const uint8_t*)(qual + i);
+    }
     __device__ __forceinline__ int op(int i) const { return cop(cw(i)); }
     __device__ __forceinline__ int len(int i) const { return clen(cw(i)); }
     __device__ __forceinline__ int q(int i) {
@@ -188,12 +207,12 @@ struct Walker {
     }
 
     __device__ __forceinline__ DenseRow* row(int pos) {
-        int i = pos - S->dense_lo;
-        if (i < 0 || i >= S->dense_n) {
+        int i = pos - S.dense_lo;
+        if (i < 0 || i >= S.dense_n) {
             bad = true;
             return nullptr;
         }
-        return &S->dense[i];
+        return &S.dense[i];
     }
     // a getVariation on nonInsertionVariants at this row: the key exists from here on, and the
     // smallest (read, touch) pair records when the reference created it (its hash-map insertion
@@ -206,8 +225,7 @@ struct Walker {
         const unsigned long long key = ((unsigned long long)(unsigned)ord << 24) | (t & 0xffffffu);
         const int ti = tile_slot(pos);
         if (ti >= 0) {
-            if (!(walk_tile.rowflags[ti] & 1u)) atomicOr(&walk_tile.rowflags[ti], 1u);
-            if (key < walk_tile.first_touch[ti]) atomicMin(&walk_tile.first_touch[ti], key);
+            atomicMin(&walk_tile.first_touch[ti], key);
             return;
         }
         if (!(r->rowflags & 1u)) atomicOr(&r->rowflags, 1u);
@@ -240,36 +258,22 @@ struct Walker {
     __device__ __forceinline__ void dense_add(DenseRow* r, int pos, int b, int tp, int qv, int nmv) {
         const int ti = tile_slot(pos);
         if (ti >= 0) {
-            auto F = [&](int f) -> int32_t* { return &walk_tile.f[f][b][ti]; };
-            atomicAdd(F(TF_CNT), 1);
-            atomicAdd(F(dir ? TF_REV : TF_FWD), 1);
-            atomicAdd(F((double)qv >= c->goodq ? TF_HI : TF_LO), 1);
-            atomicAdd(F(TF_TP), tp);
-            atomicAdd(F(TF_Q), qv);
-            atomicAdd(F(TF_MQ), (int)mapq);
-            atomicAdd(F(TF_NM), nmv);
-            unsigned* mo = (unsigned*)F(TF_MINORD);
-            if ((unsigned)ord < *mo) atomicMin(mo, (unsigned)ord);
-            unsigned* fl = (unsigned*)F(TF_FLAGS);
-            const unsigned want = 1u | ((tp <= 0 || qv <= 0) ? 8u : 0u);
-            if ((*fl & want) != want) atomicOr(fl, want);
-            if ((*fl & 6u) != 6u) {
-                const unsigned v = ((((unsigned)tp & 0xffffffu) << 8) | ((unsigned)qv & 0xffu)) + 1u;
-                const unsigned old = atomicCAS((unsigned*)F(TF_FIRST), 0u, v);
-                if (old != 0u && old != v) {
-                    const unsigned o = old - 1u, nv = v - 1u;
-                    unsigned add = 0;
-                    if ((o >> 8) != (nv >> 8)) add |= 2u;
-                    if ((o & 0xffu) != (nv & 0xffu)) add |= 4u;
-                    if (add) atomicOr(fl, add);
-                }
-            }
+            auto F = [&](int f) -> uint32_t* { return &walk_tile.f[f][b][ti]; };
+            atomicAdd(F(TF_CNT), (dir ? TF_REV1 : 1u) + ((double)qv >= cf.goodq ? TF_HI1 : 0u));
+            atomicAdd(F(TF_TP), (unsigned)tp);
+            atomicAdd(F(TF_QMQ), ((unsigned)qv & 0xffffu) | ((unsigned)mapq << 16));
+            atomicAdd(F(TF_NM), (unsigned)nmv);
+            atomicMin(F(TF_MINORD), (unsigned)ord);
+            atomicOr(F(TF_FLAGS), 1u | TF_HASPQ | ((tp <= 0 || qv <= 0) ? 8u : 0u));
+            const unsigned v = (((unsigned)tp & 0xffffffu) << 8) | ((unsigned)qv & 0xffu);
+            atomicAnd(F(TF_AND), v);
+            atomicOr(F(TF_OR), v);
             return;
         }
         DenseBase& d = r->b[b];
         atomicAdd(&d.cnt, 1);
         atomicAdd(dir ? &d.rev : &d.fwd, 1);
-        atomicAdd((double)qv >= c->goodq ? &d.hi : &d.lo, 1);
+        atomicAdd((double)qv >= cf.goodq ? &d.hi : &d.lo, 1);
         atomicAdd(&d.sum_tp, tp);
         atomicAdd(&d.sum_q, qv);
         atomicAdd(&d.sum_mq, (int)mapq);
@@ -297,23 +301,19 @@ struct Walker {
         if (!leader) return;
         const int ti = tile_slot(pos);
         if (ti >= 0) {
-            auto F = [&](int f) -> int32_t* { return &walk_tile.f[f][b][ti]; };
-            atomicAdd(F(TF_CNT), 1);
-            atomicAdd(F(dir ? TF_REV : TF_FWD), 1);
-            atomicAdd(F((double)qv >= (double)(float)c->goodq ? TF_HI : TF_LO), 1);
-            atomicAdd(F(TF_TP), rp);
-            atomicAdd(F(TF_Q), qv);
-            atomicAdd(F(TF_MQ), (int)mapq);
-            atomicAdd(F(TF_NM), nm);
-            atomicMin((unsigned*)F(TF_MINORD), (unsigned)ord);
-            unsigned* fl = (unsigned*)F(TF_FLAGS);
-            if (!(*fl & 1u)) atomicOr(fl, 1u);
+            auto F = [&](int f) -> uint32_t* { return &walk_tile.f[f][b][ti]; };
+            atomicAdd(F(TF_CNT), (dir ? TF_REV1 : 1u) + ((double)qv >= (double)(float)cf.goodq ? TF_HI1 : 0u));
+            atomicAdd(F(TF_TP), (unsigned)rp);
+            atomicAdd(F(TF_QMQ), ((unsigned)qv & 0xffffu) | ((unsigned)mapq << 16));
+            atomicAdd(F(TF_NM), (unsigned)nm);
+            atomicMin(F(TF_MINORD), (unsigned)ord);
+            atomicOr(F(TF_FLAGS), 1u);
             return;
         }
         DenseBase& d = r->b[b];
         atomicAdd(&d.cnt, 1);
         atomicAdd(dir ? &d.rev : &d.fwd, 1);
-        atomicAdd((double)qv >= (double)(float)c->goodq ? &d.hi : &d.lo, 1);
+        atomicAdd((double)qv >= (double)(float)cf.goodq ? &d.hi : &d.lo, 1);
         atomicAdd(&d.sum_tp, rp);
         atomicAdd(&d.sum_q, qv);
         atomicAdd(&d.sum_mq, (int)mapq);
@@ -324,7 +324,7 @@ struct Walker {
 
     __device__ __forceinline__ void event(int kind, int pos, const DStr& d, int tp, double qv, int nmv, int cntflags, int rp, bool hiq) {
         if (d.overflow) {
-            if (leader) atomicAdd(&S->ctr->overflow, 1ull);
+            if (leader) atomicAdd(&S.ctr->overflow, 1ull);
             return;
         }
         const int myseq = evseq++;
@@ -333,13 +333,13 @@ struct Walker {
             if (r) touch_pos(r, pos);
         }
         if (!leader) return;
-        unsigned idx = atomicAdd(S->n_events, 1u);
-        unsigned long long off = atomicAdd(S->arena_used, (unsigned long long)d.n);
-        if (idx >= S->max_events || off + (unsigned long long)d.n > S->arena_cap) {
-            atomicAdd(&S->ctr->overflow, 1ull);
+        unsigned idx = atomicAdd(S.n_events, 1u);
+        unsigned long long off = atomicAdd(S.arena_used, (unsigned long long)d.n);
+        if (idx >= S.max_events || off + (unsigned long long)d.n > S.arena_cap) {
+            atomicAdd(&S.ctr->overflow, 1ull);
             return;
         }
-        for (int i = 0; i < d.n; i++) S->arena[off + i] = d.c[i];
+        for (int i = 0; i < d.n; i++) S.arena[off + i] = d.c[i];
         Event e;
         e.pos = pos;
         e.ord = ord;
@@ -356,7 +356,7 @@ struct Walker {
         e.hiq = hiq ? 1 : 0;
         e.pad[0] = e.pad[1] = 0;
         e.rp = rp;
-        S->events[idx] = e;
+        S.events[idx] = e;
     }
 
     // a nonInsertionVariants contribution from the match run / deletion handlers
@@ -368,18 +368,18 @@ struct Walker {
                 return;
             }
         }
-        event(0, pos, d, tp, qv, nmv, cntflags | 8, 0, qv >= c->goodq);
+        event(0, pos, d, tp, qv, nmv, cntflags | 8, 0, qv >= cf.goodq);
     }
 
     __device__ __forceinline__ int tp_of(int rlen1) const { return rpe < rlen1 - rpe ? rpe + 1 : rlen1 - rpe; }
 
-    __device__ __forceinline__ bool isNextMatched(int ci) const { return c->performLocalRealignment && n_cigar > ci + 1 && op(ci + 1) == OP_M; }
+    __device__ __forceinline__ bool isNextMatched(int ci) const { return cf.performLocalRealignment && n_cigar > ci + 1 && op(ci + 1) == OP_M; }
     __device__ __forceinline__ bool isNextInsertion(int ci) const {
-        return c->performLocalRealignment && n_cigar > ci + 1 && op(ci + 1) == OP_I;
+        return cf.performLocalRealignment && n_cigar > ci + 1 && op(ci + 1) == OP_I;
     }
     __device__ __forceinline__ bool isNextAfterNumMatched(int ci, int number) const { return n_cigar > ci + number && op(ci + 1) == OP_M; }
     __device__ __forceinline__ bool isIDWithNextMatched(int ci) const {
-        return c->performLocalRealignment && n_cigar > ci + 2 && len(ci + 1) <= c->vext && op(ci + 1) == OP_M &&
+        return cf.performLocalRealignment && n_cigar > ci + 2 && len(ci + 1) <= cf.vext && op(ci + 1) == OP_M &&
                (op(ci + 2) == OP_I || op(ci + 2) == OP_D) && op(ci + 3) != OP_I && op(ci + 3) != OP_D;
     }
 
@@ -392,9 +392,9 @@ struct Walker {
     __device__ __forceinline__ Off findOffset(int refPos, int readPos, int cigLen) {
         Off o{0, readPos, 0, 0};
         int vsn = 0;
-        for (int vi = 0; vsn <= c->vext && vi < cigLen; vi++) {
+        for (int vi = 0; vsn <= cf.vext && vi < cigLen; vi++) {
             if (sq(readPos + vi) == 'N') break;
-            if (q(readPos + vi) < c->goodq) break;
+            if (q(readPos + vi) < cf.goodq) break;
             if (ref.has(refPos + vi)) {
                 if (sq(readPos + vi) != ref.at(refPos + vi)) {
                     o.offset = vi + 1;
@@ -438,16 +438,16 @@ struct Walker {
     __device__ __forceinline__ void emit_clip(int side, int pos, int cel_, int nhq, int sumq, int from, int step) {
         for (int k = 0; k < nhq; k++) (void)q(from + step * k);  // range checks on every lane
         if (!leader) return;
-        unsigned idx = atomicAdd(S->n_clips, 1u);
-        unsigned long long off = atomicAdd(S->clip_used, (unsigned long long)(2 * nhq));
-        if (idx >= S->max_clips || off + (unsigned long long)(2 * nhq) > S->clip_cap) {
-            atomicAdd(&S->ctr->overflow, 1ull);
+        unsigned idx = atomicAdd(S.n_clips, 1u);
+        unsigned long long off = atomicAdd(S.clip_used, (unsigned long long)(2 * nhq));
+        if (idx >= S.max_clips || off + (unsigned long long)(2 * nhq) > S.clip_cap) {
+            atomicAdd(&S.ctr->overflow, 1ull);
             return;
         }
         for (int k = 0; k < nhq; k++) {
             int ri = from + step * k;
-            S->clip_arena[off + k] = sq(ri);
-            S->clip_arena[off + nhq + k] = (char)q(ri);
+            S.clip_arena[off + k] = sq(ri);
+            S.clip_arena[off + nhq + k] = (char)q(ri);
         }
         ClipRec r;
         r.pos = pos;
@@ -460,7 +460,7 @@ struct Walker {
         r.nhq = nhq;
         r.sumq = sumq;
         r.bases_off = (uint32_t)off;
-        S->clips[idx] = r;
+        S.clips[idx] = r;
     }
 
     __device__ __forceinline__ void rescue(int pos, int rp, int qv) {
@@ -469,10 +469,10 @@ struct Walker {
         if (b >= 0) {
             dense_addcnt(pos, b, rp, qv);
         } else {
-            DStr& d = strs[0];
+            DStr& d = walk_str[wave][0];
             d.clear();
             d.push(rb);
-            event(2, pos, d, 0, (double)qv, nm, 0, rp, (double)qv >= (double)(float)c->goodq);
+            event(2, pos, d, 0, (double)qv, nm, 0, rp, (double)qv >= (double)(float)cf.goodq);
         }
         cov(pos);
     }
@@ -528,7 +528,7 @@ struct Walker {
             rpi += cel;
             return ci;
         }
-        DStr& desc = strs[0];
+        DStr& desc = walk_str[wave][0];
         desc.clear();
         sub(desc, rpi, cel);
         long qs = 0;
@@ -555,9 +555,9 @@ struct Walker {
             ci += 2;
         } else if (isNextMatched(ci)) {
             int vsn = 0;
-            for (int vi = 0; vsn <= c->vext && vi < len(ci + 1); vi++) {
+            for (int vi = 0; vsn <= cf.vext && vi < len(ci + 1); vi++) {
                 if (sq(rpi + cel + vi) == 'N') break;
-                if (q(rpi + cel + vi) < c->goodq) break;
+                if (q(rpi + cel + vi) < cf.goodq) break;
                 if (ref.has(start + vi)) {
                     if (sq(rpi + cel + vi) != ref.at(start + vi)) {
                         offset = vi + 1;
@@ -601,7 +601,7 @@ struct Walker {
                 }
                 if (n > 1) {
                     // rotate right by n-1: last n-1 characters move to the front
-                    DStr& t = strs[1];
+                    DStr& t = walk_str[wave][1];
                     t.clear();
                     t.append(desc.c + (L - (n - 1)), n - 1);
                     t.append(desc.c, L - (n - 1));
@@ -609,13 +609,13 @@ struct Walker {
                 }
                 if (rpi - 1 - (start - 1 - bi) > 0) inspos = bi;
             }
-            DStr& key = strs[1];
+            DStr& key = walk_str[wave][1];
             key.clear();
             key.push('+');
             key.append(desc);
             int tp = tp_of(rlen1);
             double tmpq = (double)qs / qc;
-            event(1, inspos, key, tp, tmpq, nm - nmoff, 2 | 8, 0, tmpq >= c->goodq);
+            event(1, inspos, key, tp, tmpq, nm - nmoff, 2 | 8, 0, tmpq >= cf.goodq);
             int index = rpi - 1 - (start - 1 - inspos);
             if (inspos > position && ref.hasEq(inspos, sq(index))) {
                 char ab = sq(index);
@@ -623,8 +623,8 @@ struct Walker {
                 const int qa = q(index);
                 if (b >= 0 && !leader) {
                 } else if (b >= 0) {
-                    unsigned ai = atomicAdd(S->n_anchors, 1u);
-                    if (ai < S->max_anchors) {
+                    unsigned ai = atomicAdd(S.n_anchors, 1u);
+                    if (ai < S.max_anchors) {
                         Anchor a;
                         a.pos = inspos;
                         a.ord = ord;
@@ -634,18 +634,18 @@ struct Walker {
                         a.q = qa;
                         a.mapq = mapq;
                         a.nmv = nm - nmoff;
-                        S->anchors[ai] = a;
+                        S.anchors[ai] = a;
                     } else {
-                        atomicAdd(&S->ctr->overflow, 1ull);
+                        atomicAdd(&S.ctr->overflow, 1ull);
                     }
                 } else if (ab != 0) {
-                    DStr& d = strs[2];
+                    DStr& d = walk_str[wave][2];
                     d.clear();
                     d.push(ab);
-                    event(3, inspos, d, tp, (double)q(index), nm - nmoff, 0, 0, (double)q(index) >= c->goodq);
+                    event(3, inspos, d, tp, (double)q(index), nm - nmoff, 0, 0, (double)q(index) >= cf.goodq);
                 }
             }
-            if (ci == 1 && (op(0) == OP_S || op(0) == OP_H) && leader) atomicAdd(&S->ctr->ttref, 1);
+            if (ci == 1 && (op(0) == OP_S || op(0) == OP_H) && leader) atomicAdd(&S.ctr->ttref, 1);
         }
         rpi += cel + offset + multoffp;
         rpe += cel + offset + multoffp;
@@ -658,7 +658,7 @@ struct Walker {
             rpi += cel;
             return ci;
         }
-        DStr& desc = strs[0];
+        DStr& desc = walk_str[wave][0];
         desc.clear();
         desc.push('-');
         desc.append_int(cel);
@@ -676,9 +676,9 @@ struct Walker {
                 int vsn = 0;
                 int tn = rpi + multoffp;
                 int ts = start + multoffs + cel;
-                for (int vi = 0; vsn <= c->vext && vi < len(ci + 3); vi++) {
+                for (int vi = 0; vsn <= cf.vext && vi < len(ci + 3); vi++) {
                     if (sq(tn + vi) == 'N') break;
-                    if (q(tn + vi) < c->goodq) break;
+                    if (q(tn + vi) < cf.goodq) break;
                     if (ref.hasEq(ts + vi, 'N')) break;
                     if (ref.has(ts + vi)) {
                         if (sq(tn + vi) != ref.at(ts + vi)) {
@@ -709,10 +709,10 @@ struct Walker {
         } else if (isNextMatched(ci)) {
             int mLen = len(ci + 1);
             int vsn = 0;
-            for (int vi = 0; vsn <= c->vext && vi < mLen; vi++) {
+            for (int vi = 0; vsn <= cf.vext && vi < mLen; vi++) {
                 char sc = sq(rpi + vi);
                 if (sc == 'N') break;
-                if (q(rpi + vi) < c->goodq) break;
+                if (q(rpi + vi) < cf.goodq) break;
                 if (ref.has(start + cel + vi)) {
                     char rc = ref.at(start + cel + vi);
                     if (rc == 'N') break;
@@ -747,7 +747,7 @@ struct Walker {
         if (start >= Rs && start <= Re) {
             int tp = tp_of(rlen1);
             double tmpq = (double)qs / qc;
-            event(0, start, desc, tp, tmpq, nm - nmoff, 1 | 8, 0, tmpq >= c->goodq);
+            event(0, start, desc, tp, tmpq, nm - nmoff, 1 | 8, 0, tmpq >= cf.goodq);
             for (int i = 0; i < cel; i++) cov(start + i);
         }
         start += cel + offset + multoffs;
@@ -780,9 +780,9 @@ struct Walker {
                     b = base_index(ch);
                     if (b < 0) {
                         stop = true;
-                    } else if (ref.hasNe(st, ch) && (double)qv >= c->goodq) {
+                    } else if (ref.hasNe(st, ch) && (double)qv >= cf.goodq) {
                         if ((st + 1) >= Rs && (st + 1) <= Re && j + 1 < cel && ref.at(st) != 'N') stop = true;
-                        if (c->performLocalRealignment && cel - j <= c->vext && ci + 1 < n_cigar &&
+                        if (cf.performLocalRealignment && cel - j <= cf.vext && ci + 1 < n_cigar &&
                             (op(ci + 1) == OP_D || op(ci + 1) == OP_I))
                             stop = true;
                     }
@@ -792,14 +792,14 @@ struct Walker {
         const unsigned long long stops = __ballot(act && stop);
         const int f = stops ? (__ffsll(stops) - 1) : span;
         bool doit = act && k < f && !isN;
-        if (doit && c->trimBasesAfter != 0) doit = !(!dir ? rp > c->trimBasesAfter : totalLen - rp > c->trimBasesAfter);
+        if (doit && cf.trimBasesAfter != 0) doit = !(!dir ? rp > cf.trimBasesAfter : totalLen - rp > cf.trimBasesAfter);
         doit = doit && st >= Rs && st <= Re;
-        const int ri = st - S->dense_lo;
-        const bool inwin = ri >= 0 && ri < S->dense_n;
+        const int ri = st - S.dense_lo;
+        const bool inwin = ri >= 0 && ri < S.dense_n;
         if (__ballot(doit && !inwin)) bad = true;  // as row() would flag it
         const unsigned long long md = __ballot(doit && inwin);
         if (doit && inwin) {
-            DenseRow* r = &S->dense[ri];
+            DenseRow* r = &S.dense[ri];
             const unsigned t = tseq + (unsigned)__popcll(md & ((1ull << k) - 1ull));
             const int tp = re1 < rlen1 - re1 ? re1 + 1 : rlen1 - re1;
             touch_key(r, st, t);
@@ -824,7 +824,7 @@ struct Walker {
                 if (i >= cel) break;
             }
             bool trim = false;
-            if (c->trimBasesAfter != 0) trim = !dir ? rpi > c->trimBasesAfter : totalLen - rpi > c->trimBasesAfter;
+            if (cf.trimBasesAfter != 0) trim = !dir ? rpi > cf.trimBasesAfter : totalLen - rpi > cf.trimBasesAfter;
             const char ch1 = sq(rpi);
             if (ch1 == 'N') {
                 start++;
@@ -833,18 +833,18 @@ struct Walker {
                 i++;
                 continue;
             }
-            DStr& s = strs[0];
+            DStr& s = walk_str[wave][0];
             s.clear();
             s.push(ch1);
             bool swd = false;
             double qv = q(rpi);
             int qbases = 1, qibases = 0;
             int ss_n = 0;
-            DStr& ss = strs[1];
+            DStr& ss = walk_str[wave][1];
             ss.clear();
-            while ((start + 1) >= Rs && (start + 1) <= Re && (i + 1) < cel && qv >= c->goodq && ref.hasNe(start, sq(rpi)) &&
+            while ((start + 1) >= Rs && (start + 1) <= Re && (i + 1) < cel && qv >= cf.goodq && ref.hasNe(start, sq(rpi)) &&
                    ref.at(start) != 'N') {
-                if (q(rpi + 1) < c->goodq + 5) break;
+                if (q(rpi + 1) < cf.goodq + 5) break;
                 char nuc = sq(rpi + 1);
                 if (nuc == 'N') break;
                 if (ref.hasEq(start + 1, 'N')) break;
@@ -859,7 +859,7 @@ struct Walker {
                     nmoff++;
                 } else {
                     int ssn = 0;
-                    for (int ssi = 1; ssi <= c->vext; ssi++) {
+                    for (int ssi = 1; ssi <= cf.vext; ssi++) {
                         if (i + 1 + ssi >= cel) break;
                         if (rpi + 1 + ssi < lseq && ref.hasNe(start + 1 + ssi, sq(rpi + 1 + ssi))) {
                             ssn = ssi + 1;
@@ -867,7 +867,7 @@ struct Walker {
                         }
                     }
                     if (ssn == 0) break;
-                    if (q(rpi + ssn) < c->goodq + 5) break;
+                    if (q(rpi + ssn) < cf.goodq + 5) break;
                     for (int ssi = 1; ssi <= ssn; ssi++) {
                         ss.push(sq(rpi + ssi));
                         qv += q(rpi + ssi);
@@ -885,8 +885,8 @@ struct Walker {
                 s.append(ss);
             }
             int ddlen = 0;
-            if (c->performLocalRealignment && cel - i <= c->vext && ci + 1 < n_cigar && op(ci + 1) == OP_D && ref.has(start) &&
-                (ss_n > 0 || sq(rpi) != ref.at(start)) && q(rpi) >= c->goodq) {
+            if (cf.performLocalRealignment && cel - i <= cf.vext && ci + 1 < n_cigar && op(ci + 1) == OP_D && ref.has(start) &&
+                (ss_n > 0 || sq(rpi) != ref.at(start)) && q(rpi) >= cf.goodq) {
                 while (i + 1 < cel) {
                     s.push(sq(rpi + 1));
                     qv += q(rpi + 1);
@@ -898,7 +898,7 @@ struct Walker {
                 }
                 int amp = s.find('&');
                 if (amp >= 0) s.erase_at(amp);
-                DStr& t = strs[2];
+                DStr& t = walk_str[wave][2];
                 t.clear();
                 t.push('-');
                 t.append_int(len(ci + 1));
@@ -929,8 +929,8 @@ struct Walker {
                         sub(s, o.from, o.offset);
                     }
                 }
-            } else if (c->performLocalRealignment && cel - i <= c->vext && ci + 1 < n_cigar && op(ci + 1) == OP_I &&
-                       ref.has(start) && (ss_n > 0 || sq(rpi) != ref.at(start)) && q(rpi) >= c->goodq) {
+            } else if (cf.performLocalRealignment && cel - i <= cf.vext && ci + 1 < n_cigar && op(ci + 1) == OP_I &&
+                       ref.has(start) && (ss_n > 0 || sq(rpi) != ref.at(start)) && q(rpi) >= cf.goodq) {
                 while (i + 1 < cel) {
                     s.push(sq(rpi + 1));
                     qv += q(rpi + 1);
@@ -993,8 +993,8 @@ struct Walker {
 
     __device__ __forceinline__ bool skip_overlap(uint16_t flag, int tid, int mtid, int mateStart, int reflen, int pos0) const {
         bool pairedSame = (flag & 1) && tid == mtid;
-        if (c->uniqueModeAlignmentEnabled && pairedSame && !dir && start >= mateStart) return true;
-        if (c->uniqueModeSecondInPairEnabled && (flag & 1) && pairedSame) {
+        if (cf.uniqueModeAlignmentEnabled && pairedSame && !dir && start >= mateStart) return true;
+        if (cf.uniqueModeSecondInPairEnabled && (flag & 1) && pairedSame) {
             int endpos = pos0 + (reflen > 0 ? reflen : 1);
             if (position >= mateStart) return start >= mateStart && start <= mateStart + reflen - 1;
             return start >= mateStart && mateStart <= endpos;
@@ -1040,7 +1040,7 @@ struct Walker {
             ci = match_run(ci, c_op, rlen1, totalLen);
             if (start > Re) break;
         }
-        if (bad && leader) atomicAdd(&S->ctr->walk_errors, 1ull);
+        if (bad && leader) atomicAdd(&S.ctr->walk_errors, 1ull);
     }
 };
 

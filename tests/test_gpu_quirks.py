@@ -79,7 +79,9 @@ def test_softclip_rescue_reads_match_oracle(tools, tmp_path, side, region, k):
     dump = eng.debug_pileup(name, s, e, reads, name, seq)
     out = eng.run_region(name, s, e, reads, name, seq)
     eng.close()
-    assert "S%d\t" % side in want_dump or region[0] > 1
+    # with -k 0 the clips survive to Sclip accumulation (with -k 1 the CIGAR rewrite rescues the
+    # matching clip bases first); a region starting at 251 leaves the 5' Sclip outside
+    assert "S%d\t" % side in want_dump or region[0] > 1 or k == 1
     assert sorted(dump.splitlines()) == sorted(want_dump.splitlines())
     assert out == want_out
 
