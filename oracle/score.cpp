@@ -22,9 +22,12 @@
 // Pinned: kt_fisher_two_sided by tests/test_oracle_fisher.py (exact rational enumeration, symmetry,
 // scipy); the printed fields of SNV / deletion / insertion calls by tests/test_oracle_cases.py.
 // findMSI by tests/test_oracle_findmsi_kat.py (hand-worked microsatellite counts and 3' shifts);
-// the MNV allele printed from an '&' descriptor by tests/test_oracle_realign_kat.py.
-// Parity unpinned: the '#' and '^' descriptor forms of collectReferenceVariants have no known
-// answer of their own.
+// the MNV allele printed from an '&' descriptor by tests/test_oracle_realign_kat.py; a deletion
+// followed by an insertion ("-2^XY") and by a mismatch ("-2&T") by tests/test_oracle_allele_kat.py.
+// Parity unpinned: the '#' form (an indel, at most vext matched bases, another indel) has no known
+// answer: with local realignment on, the CIGAR rewrite merges every such run (M <= 15 between the
+// indels) into one deletion plus insertion before the walk, and with it off the walk never builds
+// the form, so the default configuration does not reach it.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>

@@ -285,15 +285,17 @@ def test_descriptor_matchers_equal_std_regex(sim):
     assert hits > 100000
 
 
-@pytest.mark.parametrize("case", ["mnp", "realigndel", "realignins", "lgins30"])
+@pytest.mark.parametrize("case", ["mnp", "realigndel", "realignins", "lgins30", "caret", "amp"])
 def test_realigner_known_answers_engine(sim, tmp_path, case):
-    """The realigner known-answer reads through the engine's device stages and host realigner on
-    the CPU: the same lines as the oracle, whose decisions tests/test_oracle_realign_kat.py pins."""
+    """The realigner and deletion-descriptor known-answer reads through the engine's device stages
+    and host realigner on the CPU: the same lines as the oracle, whose answers
+    tests/test_oracle_realign_kat.py and tests/test_oracle_allele_kat.py pin."""
+    import test_oracle_allele_kat as ak
     import test_oracle_realign_kat as rk
 
     d = str(tmp_path / case)
     {"mnp": rk.build, "realigndel": rk.build_realigndel, "realignins": rk.build_realignins,
-     "lgins30": rk.build_lgins30}[case](d)
+     "lgins30": rk.build_lgins30, "caret": lambda x: ak.build(x, "caret"), "amp": lambda x: ak.build(x, "amp")}[case](d)
     want, _ = run_oracle(d, "chrS:1-600", ["--fisher"])
     out = os.path.join(d, "cpu_sim.txt")
     subprocess.run([sim, os.path.join(d, "ref.fa"), os.path.join(d, "reads.rvb"), "1", "600", "1", "1", out],

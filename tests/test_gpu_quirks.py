@@ -86,19 +86,21 @@ def test_softclip_rescue_reads_match_oracle(tools, tmp_path, side, region, k):
     assert out == want_out
 
 
-@pytest.mark.parametrize("case", ["mnp", "realigndel", "realignins", "lgins30"])
+@pytest.mark.parametrize("case", ["mnp", "realigndel", "realignins", "lgins30", "caret", "amp"])
 def test_realigner_known_answer_reads_match_oracle(tools, tmp_path, case):
     """The realigner known-answer reads (tests/test_oracle_realign_kat.py: adjustMNP taking over an
     MNV's right piece, realigndel and realignins absorbing end mismatches, realignlgins30 pairing two
-    soft clips into an insertion) on the GPU: pileup and
+    soft clips into an insertion) and the deletion-descriptor known answers
+    (tests/test_oracle_allele_kat.py) on the GPU: pileup and
     calls equal the oracle's."""
     from rabbitvar_amd import read_fasta, read_rvb
     from rabbitvar_amd.engine import Engine
+    import test_oracle_allele_kat as ak
     import test_oracle_realign_kat as rk
 
     d = str(tmp_path / case)
     {"mnp": rk.build, "realigndel": rk.build_realigndel, "realignins": rk.build_realignins,
-     "lgins30": rk.build_lgins30}[case](d)
+     "lgins30": rk.build_lgins30, "caret": lambda x: ak.build(x, "caret"), "amp": lambda x: ak.build(x, "amp")}[case](d)
     want_out, want_dump = run_oracle(d, "chrS:1-600", ["--fisher"], dump="pile.txt")
     name, seq = read_fasta(os.path.join(d, "ref.fa"))
     reads = read_rvb(os.path.join(d, "reads.rvb"))
@@ -110,5 +112,6 @@ def test_realigner_known_answer_reads_match_oracle(tools, tmp_path, case):
     assert sorted(dump.splitlines()) == sorted(want_dump.splitlines())
     assert out == want_out
     anchor = {"mnp": "\t301\t302\t", "realigndel": f"\t{rk.P_DEL - 1}\t{rk.P_DEL + 2}\t",
-              "realignins": f"\t{rk.P_INS}\t{rk.P_INS}\t", "lgins30": f"\t{rk.Q_LG}\t{rk.Q_LG}\t"}[case]
+              "realignins": f"\t{rk.P_INS}\t{rk.P_INS}\t", "lgins30": f"\t{rk.Q_LG}\t{rk.Q_LG}\t",
+              "caret": f"\t{ak.P_CARET}\t{ak.P_CARET + 1}\t", "amp": f"\t{ak.P_AMP}\t{ak.P_AMP + 2}\t"}[case]
     assert anchor in out
