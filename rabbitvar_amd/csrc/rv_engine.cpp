@@ -301,7 +301,7 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
             RV_HIP(launch_prep(dargs, n, prep, cig, ctr, span, false, stream));  // duplicate marks kept
         }
         RV_HIP(hipEventRecord(ev[2], stream));
-        RV_HIP(launch_walk(dargs, n, prep, cig, stream));
+        RV_HIP(launch_walk(dargs, n, prep, cig, walk_reads_for(n, W.dense_n, max_span), stream));
         RV_HIP(hipEventRecord(ev[3], stream));
         RV_HIP(launch_anchor(S.anchors, S.n_anchors, max_anchors, rows, W.dense_lo, W.dense_n, cfg, stream));
         RV_HIP(hipEventRecord(ev[4], stream));

@@ -10,7 +10,8 @@ namespace rv {
 hipError_t launch_prep(const WalkArgs* A, int64_t n, PrepRec* prep, uint32_t* cig, WalkCounters* ctr, int* max_span,
                        bool dedup, hipStream_t s);
 hipError_t launch_fill_rows(DenseRow* rows, int n, hipStream_t s);
-hipError_t launch_walk(const WalkArgs* A, int64_t n, const PrepRec* prep, const uint32_t* cig, hipStream_t s);
+hipError_t launch_walk(const WalkArgs* A, int64_t n, const PrepRec* prep, const uint32_t* cig, int nreads,
+                       hipStream_t s);
 hipError_t launch_no_passing(const WalkArgs* A, int64_t n, int start, int end, unsigned* count, hipStream_t s);
 hipError_t launch_anchor(const Anchor* a, const unsigned* n_anchors, unsigned cap, DenseRow* rows, int dense_lo,
                          int dense_n, const Config& cfg, hipStream_t s);

@@ -28,11 +28,12 @@ __global__ void __launch_bounds__(256) k_fill_rows(DenseRow* rows, int n) {
     if (i < n) fill_row_one(rows, i);
 }
 
-// One workgroup walks WALK_READS consecutive reads (sorted by position), one wavefront per read at a
+// One workgroup walks nreads consecutive reads (sorted by position), one wavefront per read at a
 // time, into an LDS tile that starts TILE_MARGIN bases before the first read; the tile is added to
 // the dense rows once at the end.
 
-__global__ void __launch_bounds__(WALK_THREADS) k_walk(const WalkArgs* A, const PrepRec* prep, const uint32_t* cig) {
+__global__ void __launch_bounds__(WALK_THREADS) k_walk(const WalkArgs* A, const PrepRec* prep, const uint32_t* cig,
+                                                       int nreads) {
     const int waves = (int)(blockDim.x / warpSize);
     // the wavefront index is uniform across its lanes; saying so lets the per-read state live in
     // scalar registers
@@ -45,7 +46,7 @@ __global__ void __launch_bounds__(WALK_THREADS) k_walk(const WalkArgs* A, const 
     const Config cfg = A->cfg;
     const WalkSink S = A->S;
     const int64_t n = R.n;
-    const int64_t r0 = (int64_t)blockIdx.x * WALK_READS;
+    const int64_t r0 = (int64_t)blockIdx.x * nreads;
     const int tile_lo = (r0 < n ? prep[r0].position : 0) - TILE_MARGIN;
     // reads after each wavefront's first are handed out as wavefronts finish (reads differ in cost;
     // every contribution merges the same way whichever wavefront walks it)
@@ -53,7 +54,7 @@ __global__ void __launch_bounds__(WALK_THREADS) k_walk(const WalkArgs* A, const 
     if (threadIdx.x == 0) walk_next = waves;
     tile_init((int)threadIdx.x, (int)blockDim.x, W, tile_lo);
     __syncthreads();
-    for (int k = wave; k < WALK_READS;) {
+    for (int k = wave; k < nreads;) {
         const int64_t r = r0 + k;
         if (r < n) walk_one(r, R, W, cfg, prep, cig, S, lane, wave, tile_lo);
         int nk = 0;
@@ -115,9 +116,11 @@ hipError_t launch_fill_rows(DenseRow* rows, int n, hipStream_t s) {
     hipLaunchKernelGGL(k_fill_rows, dim3(grid_for(n, 256)), dim3(256), 0, s, rows, n);
     return hipGetLastError();
 }
-hipError_t launch_walk(const WalkArgs* A, int64_t n, const PrepRec* prep, const uint32_t* cig, hipStream_t s) {
+hipError_t launch_walk(const WalkArgs* A, int64_t n, const PrepRec* prep, const uint32_t* cig, int nreads,
+                       hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_walk, dim3(grid_for(n, WALK_READS)), dim3(WALK_THREADS), 0, s, A, prep, cig);
+    if (nreads < 1 || nreads > WALK_READS_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_walk, dim3(grid_for(n, nreads)), dim3(WALK_THREADS), 0, s, A, prep, cig, nreads);
     return hipGetLastError();
 }
 hipError_t launch_no_passing(const WalkArgs* A, int64_t n, int start, int end, unsigned* count, hipStream_t s) {

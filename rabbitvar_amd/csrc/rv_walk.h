@@ -36,7 +36,7 @@ typedef FStr<DCAP> DStr;
 // kept as their AND and OR (all equal exactly when AND == OR, field by field).  A read adds to a
 // (position, base) at most twice (its aligned base and a rescued clip base lie on different
 // positions, so in practice once), so a workgroup's sums stay within the packed widths.
-constexpr int TILE_POS = 512;
+constexpr int TILE_POS = 768;
 enum : int { TF_CNT, TF_TP, TF_QMQ, TF_NM, TF_MINORD, TF_FLAGS, TF_AND, TF_OR, TF_N };
 constexpr unsigned TF_REV1 = 1u << 10, TF_HI1 = 1u << 20;  // TF_CNT increments (forward: 1)
 constexpr unsigned TF_HASPQ = 16u;                         // TF_FLAGS: a pp/pq-updating contribution
@@ -49,12 +49,26 @@ struct TileAcc {
 static __shared__ TileAcc walk_tile;
 
 // The walk kernel's workgroup: WALK_THREADS lanes (16 wavefronts, 4 per SIMD: the walk fits in 128
-// registers, and the tile, read copies and strings in one CU's LDS) walk WALK_READS consecutive
-// reads into a tile starting TILE_MARGIN bases before the first read.
+// registers, and the tile, read copies and strings in one CU's LDS) walk a launch-wide number of
+// consecutive reads (walk_reads_for) into a tile starting TILE_MARGIN bases before the first read.
 constexpr int WALK_THREADS = 1024;
-constexpr int WALK_READS = 64;
+constexpr int WALK_READS_MAX = 128;
+constexpr int WALK_READS_MIN = 16;
 constexpr int TILE_MARGIN = 64;
-static_assert(2 * WALK_READS < 1024 && 2 * WALK_READS * 255 < 65536, "packed tile fields hold a workgroup's sums");
+static_assert(2 * WALK_READS_MAX < 1024 && 2 * WALK_READS_MAX * 255 < 65536, "packed tile fields hold a workgroup's sums");
+// Reads per walk workgroup for one launch: as many as keep a workgroup's reads inside its tile on
+// average (their starts spread over n_reads / span_bp per base, each reaching at most max_span
+// bases past its start), between WALK_READS_MIN and WALK_READS_MAX.  More reads per workgroup
+// means fewer tiles flushed per position; reads that reach past the tile still add correctly,
+// through device-scope atomics on the rows.  Host code.
+inline int walk_reads_for(long long n_reads, long long span_bp, int max_span) {
+    const int reach = max_span < TILE_POS / 2 ? max_span : TILE_POS / 2;
+    const double density = span_bp > 0 ? (double)n_reads / (double)span_bp : 0.0;
+    const double fit = (double)(TILE_POS - TILE_MARGIN - reach) * density;
+    if (fit >= WALK_READS_MAX) return WALK_READS_MAX;
+    if (fit <= WALK_READS_MIN) return WALK_READS_MIN;
+    return (int)fit;
+}
 
 // Per-wavefront copy of the read being walked (bases, qualities, rewritten CIGAR), loaded with one
 // coalesced pass so that the lane-uniform steps read LDS instead of waiting on global memory.
@@ -254,20 +268,24 @@ struct Walker {
         touch_pos(r, pos);
         if (leader) dense_add(r, pos, b, tp, qv, nmv);
     }
+    // the pp/pq-updating contribution to tile slot ti (LDS atomics that return nothing)
+    __device__ __forceinline__ void tile_single(int ti, int b, int tp, int qv, int nmv) {
+        auto F = [&](int f) -> uint32_t* { return &walk_tile.f[f][b][ti]; };
+        atomicAdd(F(TF_CNT), (dir ? TF_REV1 : 1u) + ((double)qv >= cf.goodq ? TF_HI1 : 0u));
+        atomicAdd(F(TF_TP), (unsigned)tp);
+        atomicAdd(F(TF_QMQ), ((unsigned)qv & 0xffffu) | ((unsigned)mapq << 16));
+        atomicAdd(F(TF_NM), (unsigned)nmv);
+        atomicMin(F(TF_MINORD), (unsigned)ord);
+        atomicOr(F(TF_FLAGS), 1u | TF_HASPQ | ((tp <= 0 || qv <= 0) ? 8u : 0u));
+        const unsigned v = (((unsigned)tp & 0xffffffu) << 8) | ((unsigned)qv & 0xffu);
+        atomicAnd(F(TF_AND), v);
+        atomicOr(F(TF_OR), v);
+    }
     // the pp/pq-updating contribution (first (tp, q) value + differs flags)
     __device__ __forceinline__ void dense_add(DenseRow* r, int pos, int b, int tp, int qv, int nmv) {
         const int ti = tile_slot(pos);
         if (ti >= 0) {
-            auto F = [&](int f) -> uint32_t* { return &walk_tile.f[f][b][ti]; };
-            atomicAdd(F(TF_CNT), (dir ? TF_REV1 : 1u) + ((double)qv >= cf.goodq ? TF_HI1 : 0u));
-            atomicAdd(F(TF_TP), (unsigned)tp);
-            atomicAdd(F(TF_QMQ), ((unsigned)qv & 0xffffu) | ((unsigned)mapq << 16));
-            atomicAdd(F(TF_NM), (unsigned)nmv);
-            atomicMin(F(TF_MINORD), (unsigned)ord);
-            atomicOr(F(TF_FLAGS), 1u | TF_HASPQ | ((tp <= 0 || qv <= 0) ? 8u : 0u));
-            const unsigned v = (((unsigned)tp & 0xffffffu) << 8) | ((unsigned)qv & 0xffu);
-            atomicAnd(F(TF_AND), v);
-            atomicOr(F(TF_OR), v);
+            tile_single(ti, b, tp, qv, nmv);
             return;
         }
         DenseBase& d = r->b[b];
@@ -766,30 +784,28 @@ struct Walker {
         const int k = lane;
         const bool act = k < span;
         const int rp = rpi + k, st = start + k, j = i + k, re1 = rpe + k;
-        bool stop = false, isN = false;
-        int qv = 0, b = -1;
-        if (act) {
-            if (rp < 0 || rp >= lseq) {
-                stop = true;
-            } else {
-                const char ch = base_at(rp);
-                if (ch == 'N') {
-                    isN = true;
-                } else {
-                    qv = qual_at(rp);
-                    b = base_index(ch);
-                    if (b < 0) {
-                        stop = true;
-                    } else if (ref.hasNe(st, ch) && (double)qv >= cf.goodq) {
-                        if ((st + 1) >= Rs && (st + 1) <= Re && j + 1 < cel && ref.at(st) != 'N') stop = true;
-                        if (cf.performLocalRealignment && cel - j <= cf.vext && ci + 1 < n_cigar &&
-                            (op(ci + 1) == OP_D || op(ci + 1) == OP_I))
-                            stop = true;
-                    }
-                }
-            }
-        }
-        const unsigned long long stops = __ballot(act && stop);
+        // Every lane loads (indices clamped into range) and the conditions combine without branches:
+        // a branch per condition costs exec-mask bookkeeping and a wait per load on every lane.
+        const bool inread = rp >= 0 && rp < lseq;
+        const int rpc = inread ? rp : 0;
+        const char ch = base_at(rpc);
+        const int qv = qual_at(rpc);
+        const int b = ch == 'A' ? 0 : ch == 'C' ? 1 : ch == 'G' ? 2 : ch == 'T' ? 3 : -1;
+        const bool isN = inread && ch == 'N';
+        const bool hasr = ref.has(st);
+        const unsigned js = (unsigned)(st - ref.slo);
+        char rc = walk_ref[js < (unsigned)ref.sn ? js : 0u];
+        if (hasr && js >= (unsigned)ref.sn) rc = *(volatile const char*)(ref.ref + (st - ref.lo));
+        const bool hq_mis = hasr && rc != ch && (double)qv >= cf.goodq;
+        // the next operation is the same for every lane
+        const bool next_indel =
+            cf.performLocalRealignment && ci + 1 < n_cigar && (op(ci + 1) == OP_D || op(ci + 1) == OP_I);
+        // a high-quality mismatch stops the run where it could start a multi-base allele (a next base
+        // in the run and the region) or lies within vext of an adjacent indel
+        const bool stop_mnv = (st + 1) >= Rs && (st + 1) <= Re && j + 1 < cel && rc != 'N';
+        const bool stop_ind = next_indel && cel - j <= cf.vext;
+        const bool stop = act && (!inread || (!isN && (b < 0 || (hq_mis && (stop_mnv || stop_ind)))));
+        const unsigned long long stops = __ballot(stop);
         const int f = stops ? (__ffsll(stops) - 1) : span;
         bool doit = act && k < f && !isN;
         if (doit && cf.trimBasesAfter != 0) doit = !(!dir ? rp > cf.trimBasesAfter : totalLen - rp > cf.trimBasesAfter);
@@ -799,12 +815,19 @@ struct Walker {
         if (__ballot(doit && !inwin)) bad = true;  // as row() would flag it
         const unsigned long long md = __ballot(doit && inwin);
         if (doit && inwin) {
-            DenseRow* r = &S.dense[ri];
             const unsigned t = tseq + (unsigned)__popcll(md & ((1ull << k) - 1ull));
             const int tp = re1 < rlen1 - re1 ? re1 + 1 : rlen1 - re1;
-            touch_key(r, st, t);
-            dense_add(r, st, b, tp, qv, nm - nmoff);
-            add_cov(r, st, 1);
+            const int ti = tile_slot(st);
+            if (ti >= 0) {  // key creation, counts and coverage in one tile branch
+                atomicMin(&walk_tile.first_touch[ti], ((unsigned long long)(unsigned)ord << 24) | (t & 0xffffffu));
+                tile_single(ti, b, tp, qv, nm - nmoff);
+                atomicAdd(&walk_tile.cov[ti], 1);
+            } else {
+                DenseRow* r = &S.dense[ri];
+                touch_key(r, st, t);
+                dense_add(r, st, b, tp, qv, nm - nmoff);
+                add_cov(r, st, 1);
+            }
         }
         tseq += (unsigned)__popcll(md);
         return f;

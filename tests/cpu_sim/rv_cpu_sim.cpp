@@ -399,11 +399,12 @@ int main(int argc, char** argv) {
     WalkSink S{rows.data(), W.dense_lo, W.dense_n, ev.data(), &ne, (unsigned)ev.size(), arena.data(), &au,
                arena.size(), cl.data(), &ncl, (unsigned)cl.size(), carena.data(), &cu, carena.size(), an.data(), &na,
                (unsigned)an.size(), &ctr};
-    // workgroups of WALK_READS reads sharing one tile, as k_walk runs them
-    for (int64_t r0 = 0; r0 < m; r0 += WALK_READS) {
+    // workgroups of nreads reads sharing one tile, as k_walk runs them
+    const int nreads = walk_reads_for(m, W.dense_n, span);
+    for (int64_t r0 = 0; r0 < m; r0 += nreads) {
         const int tile_lo = prep[(size_t)r0].position - TILE_MARGIN;
         tile_init(0, 1, W, tile_lo);
-        for (int64_t r = r0; r < r0 + WALK_READS && r < m; r++)
+        for (int64_t r = r0; r < r0 + nreads && r < m; r++)
             walk_one(r, R, W, cfg, prep.data(), cg.data(), S, 0, 0, tile_lo);
         tile_flush(tile_lo, rows.data(), W.dense_lo, W.dense_n, 0, 1);
     }
