@@ -59,6 +59,13 @@ rv_status score_pass(Engine& e, ScoreIn& in, size_t n_status, std::vector<OutRec
     in.lgamma_tab = e.lgamma_table();
     in.lgamma_n = FISHER_LGT_N;
     if (!in.lgamma_tab) return RV_ERR_DEVICE;
+    if (in.dense_n > 0) {
+        // the picked dense positions, then the count (64-entry aligned)
+        const size_t list_n = ((size_t)in.dense_n + 63) & ~(size_t)63;
+        in.pick_list = (int32_t*)e.grow(e.b_pick, sizeof(int32_t) * (list_n + 64));
+        if (!in.pick_list) return e.fail(RV_ERR_OOM, "device buffer for the scored positions failed");
+        in.pick_count = (unsigned*)(in.pick_list + list_n);
+    }
     for (int attempt = 0; attempt < 3; attempt++) {
         in.out = (OutRec*)e.grow(e.b_out, sizeof(OutRec) * max_out);
         in.arena = (char*)e.grow(e.b_out_arena, (size_t)arena_cap);

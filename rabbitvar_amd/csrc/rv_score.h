@@ -83,6 +83,10 @@ struct ScoreIn {
                                    // [2] sorts above 16 elements, [3] qstd ambiguous rows
     const double* lgamma_tab;      // host-built kf_lgamma of the integers (fisher_lgamma_table)
     int lgamma_n;
+    // dense positions to score, picked by k_score_pick (dense_n entries, then the count); nullptr:
+    // the dense kernel visits every dense position
+    int32_t* pick_list;
+    unsigned* pick_count;
 };
 
 }  // namespace rv

@@ -11,10 +11,34 @@
 
 namespace rv {
 
-__global__ void __launch_bounds__(128) k_score_dense(ScoreIn S, const int32_t* pos_list, int n) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        const int p = pos_list ? pos_list[i] : S.dense_lo + i;
+// Picks the dense positions that need scoring (score_needed) into S.pick_list.  Most positions
+// end at those checks, and a wavefront of the scoring kernel takes as long as its slowest lane, so
+// the scoring kernel then runs full wavefronts of positions that all do the work.  One ballot and
+// one counter update per wavefront; the list's order does not matter (every record carries its
+// position and rank).
+__global__ void __launch_bounds__(256) k_score_pick(ScoreIn S) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool need = false;
+    const int p = S.dense_lo + i;
+    if (i < S.dense_n) {
+        PosView<4> V;
+        load_pos<4>(S, p, V);
+        need = score_needed<4>(S, p, true, V);
+    }
+    const unsigned long long m = __ballot(need);
+    const int lane = (int)(threadIdx.x & 63u);
+    unsigned base = 0;
+    if (lane == 0 && m) base = atomicAdd(S.pick_count, (unsigned)__popcll(m));
+    base = __shfl(base, 0);
+    if (need) S.pick_list[base + (unsigned)__popcll(m & ((1ull << lane) - 1ull))] = p;
+}
+
+// n: the number of dense positions (the upper bound of the picked count).
+__global__ void __launch_bounds__(128) k_score_dense(ScoreIn S, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = S.pick_list ? (int)*S.pick_count : n;
+    if (i < m) {
+        const int p = S.pick_list ? S.pick_list[i] : S.dense_lo + i;
         score_position<128, 4>(S, p, true, p - S.dense_lo);
     }
 }
@@ -36,10 +60,21 @@ __global__ void __launch_bounds__(256) k_fisher_table(const int* tab, int n, dou
     }
 }
 
+__global__ void __launch_bounds__(64) k_zero_u32(unsigned* p, int n) {
+    const int i = (int)threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
 hipError_t launch_score(const ScoreIn& S, hipStream_t s) {
-    if (S.dense_n > 0)
-        hipLaunchKernelGGL(k_score_dense, dim3((S.dense_n + 127) / 128), dim3(128), 0, s, S, (const int32_t*)nullptr,
-                           S.dense_n);
+    if (S.dense_n > 0) {
+        if (S.pick_list) {
+            if (!S.pick_count) return hipErrorInvalidValue;
+            // the count starts at zero with a fill kernel (no memset node if a caller captures this)
+            hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, S.pick_count, 1);
+            hipLaunchKernelGGL(k_score_pick, dim3((S.dense_n + 255) / 256), dim3(256), 0, s, S);
+        }
+        hipLaunchKernelGGL(k_score_dense, dim3((S.dense_n + 127) / 128), dim3(128), 0, s, S, S.dense_n);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (S.n_cx > 0) hipLaunchKernelGGL(k_score_cx, dim3((S.n_cx + 63) / 64), dim3(64), 0, s, S);

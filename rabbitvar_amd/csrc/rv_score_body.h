@@ -508,24 +508,35 @@ __device__ inline void emit_ref(const ScoreIn& S, int p, int kind, const VarX* x
     emit_out(S, o, pc, pn);
 }
 
+// The checks ToVarsBuilder::process makes before building any variant at p (ToVarsBuilder.cpp:
+// 103-136), on the loaded view: false when the position ends there (no key, no allele, no coverage,
+// the reference base alone), or when the record-list kernel scores it.  An overflowing view counts
+// as work: score_position reports it.
+template <int NA>
+__device__ __forceinline__ bool score_needed(const ScoreIn& S, int p, bool dense_only, const PosView<NA>& V) {
+    if (dense_only && V.complex_) return false;  // scored by the record-list kernel
+    if (!V.key) return false;
+    if (V.overflow) return true;
+    if (V.n == 0 && V.nins == 0) return false;
+    if (V.cov == 0) return false;
+    const RefWindow rf{S.ref, S.ref_lo, S.ref_hi};
+    // isTheSameVariationOnRef (ToVarsBuilder.cpp:213-233)
+    if (V.n == 1 && V.nins == 0 && rf.has(p) && V.a[0].len == 1 && V.a[0].d[0] == rf.at(p))
+        return S.cfg.doPileup || S.cfg.hasBam2;
+    return true;
+}
+
 template <int CAP, int NA>
 __device__ void score_position(const ScoreIn& S, int p, bool dense_only, int slot) {
     const Config& c = S.cfg;
     PosView<NA> V;
     load_pos<NA>(S, p, V);
-    if (dense_only && V.complex_) return;  // scored by the record-list kernel
-    if (!V.key) return;
+    if (!score_needed<NA>(S, p, dense_only, V)) return;
     if (V.overflow) {
         atomicAdd(&S.counters[0], 1ull);
         return;
     }
-    if (V.n == 0 && V.nins == 0) return;
-    if (V.cov == 0) return;
     RefWindow rf{S.ref, S.ref_lo, S.ref_hi};
-    // isTheSameVariationOnRef (ToVarsBuilder.cpp:213-233)
-    if (V.n == 1 && V.nins == 0 && rf.has(p) && V.a[0].len == 1 && V.a[0].d[0] == rf.at(p)) {
-        if (!c.doPileup && !c.hasBam2) return;
-    }
     int tpc = V.cov;
     int hicov = 0;
     for (int i = 0; i < V.n; i++) hicov += V.a[i].v->hicnt;
