@@ -230,11 +230,20 @@ void rv_mlp_close(rv_mlp* m);
 
 /* ---- BAM input: the records the region iterator returns, decoded to rv_reads.  Replaces the
  * reference's htslib reads (sam_itr_querys + sam_itr_next, src/recordPreprocessor.cpp:27, :103-114,
- * with seq_nt16_str decoding, src/parseCigar.cpp:442-459).  rv_bam_open inflates the whole file
- * (BGZF blocks on `threads` threads) and indexes its records; no .bai is read. */
+ * with seq_nt16_str decoding, src/parseCigar.cpp:442-459).  No .bai is read.  rv_bam_open reads
+ * the BGZF block footers first: a file whose inflated size is at most RV_BAM_DEFAULT_CACHE_BYTES is
+ * inflated whole (blocks on `threads` threads) and its records indexed; a larger one is inflated
+ * once, chunk by chunk, to index its blocks, and each query then reads and inflates only the blocks
+ * whose records can reach the region (all blocks, chunk by chunk, when the file is not
+ * coordinate-sorted).  Memory is then the block index (about 48 bytes per 64 KiB block) plus one
+ * region's records. */
 typedef struct rv_bam rv_bam;
+#define RV_BAM_DEFAULT_CACHE_BYTES (4ull << 30)
 /* On failure *out may still be set: read rv_bam_last_error, then rv_bam_close. */
 rv_status rv_bam_open(const char* path, int threads, rv_bam** out);
+/* rv_bam_open with the cached/streamed bound chosen by the caller (0: always stream).  No
+ * counterpart in the reference: for callers that must bound host memory. */
+rv_status rv_bam_open_bounded(const char* path, int threads, uint64_t max_cached_bytes, rv_bam** out);
 const char* rv_bam_last_error(const rv_bam* b);
 int32_t rv_bam_n_targets(const rv_bam* b);
 const char* rv_bam_target_name(const rv_bam* b, int32_t tid);

@@ -106,6 +106,7 @@ SIGNATURES = {
     "rv_mlp_last_error": (c_char_p, [c_void_p]),
     "rv_mlp_close": (None, [c_void_p]),
     "rv_bam_open": (c_int, [c_char_p, c_int, ctypes.POINTER(c_void_p)]),
+    "rv_bam_open_bounded": (c_int, [c_char_p, c_int, ctypes.c_uint64, ctypes.POINTER(c_void_p)]),
     "rv_bam_last_error": (c_char_p, [c_void_p]),
     "rv_bam_n_targets": (ctypes.c_int32, [c_void_p]),
     "rv_bam_target_name": (c_char_p, [c_void_p, ctypes.c_int32]),

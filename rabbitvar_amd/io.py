@@ -148,17 +148,22 @@ def read_rvb(path):
 
 
 class BamFile:
-    """A BAM file decoded by the native library (rv_bam_*): the whole file is inflated at open, and
-    query() returns the records of a region as a ReadBatch, as the reference's region iterator does."""
+    """A BAM file decoded by the native library (rv_bam_*): query() returns the records of a region
+    as a ReadBatch, as the reference's region iterator does.  Files whose inflated size is at most
+    max_cached_bytes (default: the library's, 4 GiB) are inflated whole at open; larger ones keep a
+    block index and inflate only the blocks a query needs."""
 
-    def __init__(self, path, threads=8):
+    def __init__(self, path, threads=8, max_cached_bytes=None):
         import ctypes
 
         from . import _lib
 
         self.lib = _lib.load()
         h = ctypes.c_void_p()
-        st = self.lib.rv_bam_open(path.encode(), threads, ctypes.byref(h))
+        if max_cached_bytes is None:
+            st = self.lib.rv_bam_open(path.encode(), threads, ctypes.byref(h))
+        else:
+            st = self.lib.rv_bam_open_bounded(path.encode(), threads, int(max_cached_bytes), ctypes.byref(h))
         self.h = h
         if st != 0:
             msg = self.lib.rv_bam_last_error(h) if h.value else b""

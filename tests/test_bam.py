@@ -68,11 +68,17 @@ def parse_bam(path):
     return refs, recs
 
 
-def test_decode_matches_spec_parser_and_rvb(bamdir):
+# None: the library's default (this small file is inflated whole); 0: the streamed mode (block
+# index, each query inflates only its blocks)
+MODES = [None, 0]
+
+
+@pytest.mark.parametrize("cache", MODES)
+def test_decode_matches_spec_parser_and_rvb(bamdir, cache):
     from rabbitvar_amd import BamFile, read_rvb
 
     refs, recs = parse_bam(os.path.join(bamdir, "reads.bam"))
-    b = BamFile(os.path.join(bamdir, "reads.bam"), threads=4)
+    b = BamFile(os.path.join(bamdir, "reads.bam"), threads=4, max_cached_bytes=cache)
     assert b.targets() == refs == [("chrS", 60000)]
     assert b.n_records() == len(recs) > 10000
     got = b.query("chrS", 1, 60000)
@@ -92,12 +98,13 @@ def test_decode_matches_spec_parser_and_rvb(bamdir):
     b.close()
 
 
+@pytest.mark.parametrize("cache", MODES)
 @pytest.mark.parametrize("start,end", [(1, 1), (1, 150), (20001, 20001), (30000, 30500), (59850, 60000),
                                        (60001, 70000)])
-def test_region_queries(bamdir, start, end):
+def test_region_queries(bamdir, start, end, cache):
     from rabbitvar_amd import BamFile, read_rvb
 
-    b = BamFile(os.path.join(bamdir, "reads.bam"), threads=3)
+    b = BamFile(os.path.join(bamdir, "reads.bam"), threads=3, max_cached_bytes=cache)
     got = b.query("chrS", start, end)
     want = read_rvb(os.path.join(bamdir, "reads.rvb")).select(start, end)
     for k in FIELDS:
@@ -133,5 +140,75 @@ def test_threads_agree_and_errors(bamdir, tmp_path):
     open(bad2, "wb").write(bytes(flip))
     with pytest.raises(OSError):
         BamFile(bad2)
+    for path in (bad, bad2):  # the streamed mode finds the same damage in its indexing pass
+        with pytest.raises(OSError):
+            BamFile(path, max_cached_bytes=0)
     with pytest.raises(OSError):
         BamFile(str(tmp_path / "missing.bam"))
+
+
+def write_bgzf(path, data, block=60000):
+    """BGZF from the specification: gzip members of at most 64 KiB input with a "BC" extra field
+    holding the member size - 1, then the empty end-of-file member."""
+    import zlib
+
+    out = bytearray()
+
+    def member(chunk):
+        c = zlib.compressobj(6, zlib.DEFLATED, -15)
+        body = c.compress(chunk) + c.flush()
+        bsize = 12 + 6 + len(body) + 8
+        out.extend(b"\x1f\x8b\x08\x04" + b"\0" * 4 + b"\0\xff" + struct.pack("<H", 6) + b"BC" +
+                   struct.pack("<HH", 2, bsize - 1) + body + struct.pack("<II", zlib.crc32(chunk), len(chunk)))
+
+    for i in range(0, len(data), block):
+        member(data[i:i + block])
+    member(b"")
+    open(path, "wb").write(bytes(out))
+
+
+def record_spans(data):
+    l_text = struct.unpack_from("<i", data, 4)[0]
+    p = 8 + l_text
+    n_ref = struct.unpack_from("<i", data, p)[0]
+    p += 4
+    for _ in range(n_ref):
+        p += 8 + struct.unpack_from("<i", data, p)[0]
+    head, spans = data[:p], []
+    while p < len(data):
+        bs = struct.unpack_from("<i", data, p)[0]
+        spans.append((p, p + 4 + bs))
+        p += 4 + bs
+    return head, spans
+
+
+@pytest.mark.parametrize("blocksize", [60000, 977])
+def test_unsorted_and_small_blocks_both_modes(bamdir, tmp_path, blocksize):
+    """An unsorted file (records in reverse order) and records split over many small blocks: both
+    modes return the same records, in file order, as the region rule applied to the spec parser's
+    records."""
+    from rabbitvar_amd import BamFile
+
+    data = gzip.open(os.path.join(bamdir, "reads.bam"), "rb").read()
+    head, spans = record_spans(data)
+    rev = head + b"".join(data[a:b] for a, b in reversed(spans))
+    paths = {"rev": str(tmp_path / "rev.bam"), "fwd": str(tmp_path / "fwd.bam")}
+    write_bgzf(paths["rev"], rev, blocksize)
+    write_bgzf(paths["fwd"], data, blocksize)
+    for name, path in paths.items():
+        _, recs = parse_bam(path)
+        a = BamFile(path, threads=3)
+        z = BamFile(path, threads=3, max_cached_bytes=0)
+        assert a.n_records() == z.n_records() == len(recs)
+        for start, end in [(1, 60000), (30000, 30500), (59850, 60000), (1, 1)]:
+            ga, gz = a.query("chrS", start, end), z.query("chrS", start, end)
+            for k in FIELDS:
+                assert np.array_equal(getattr(ga, k), getattr(gz, k)), (name, start, end, k)
+            exp = []
+            for r in recs:
+                rl = 0 if r["flag"] & 4 else sum(c >> 4 for c in r["cigar"] if (c & 15) in (0, 2, 3, 7, 8))
+                if r["pos0"] < end and r["pos0"] + (rl if rl > 0 else 1) > start - 1:
+                    exp.append(r["pos0"])
+            assert list(gz.pos0) == exp, (name, start, end)
+        a.close()
+        z.close()
