@@ -331,7 +331,12 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     std::thread order_th([&] {
         try {
             const auto q0 = std::chrono::steady_clock::now();
-            if (replayed || !noninsertion_order_sorted(walk_sorted, walk_maxd, S.key_log, proc)) {
+            // nonInsertionVariants' slot order: the device's sorted walk keys as they are when the
+            // realigner changed no key and the sorted form provably holds, else rebuilt
+            const std::vector<int32_t>* procp = &proc;
+            if (!replayed && S.key_log.empty() && walk_maxd < RH_OVERFLOW_DISP) {
+                procp = &walk_sorted;
+            } else if (replayed || !noninsertion_order_sorted(walk_sorted, walk_maxd, S.key_log, proc)) {
                 if (!replayed) replay_created(nonins, created);
                 proc = replay_finish(nonins, S.key_log);
             }
@@ -340,7 +345,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
                 const int i = cx.pos[k] - pr.P.dense_lo;
                 if (i >= 0 && i < pr.P.dense_n) status[(size_t)i] = status[(size_t)pr.P.dense_n + k];
             }
-            aligned_table(proc, status.data(), pr.P.dense_lo, pr.P.dense_n, aligned, std::min(hw, 8u));
+            aligned_table(*procp, status.data(), pr.P.dense_lo, pr.P.dense_n, aligned, std::min(hw, 8u));
             const auto q1 = std::chrono::steady_clock::now();
             e.stats.detail_ms[8] = std::chrono::duration<double, std::milli>(q1 - q0).count();
             // iteration order is slot order: a record's rank is its start position's slot
