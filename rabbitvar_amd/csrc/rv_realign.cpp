@@ -107,6 +107,9 @@ char HostRef::get(int k) {
 
 // ------------------------------------------------------------------ position store
 RState::RState(HostPileup& P_, RowCache& rc) : P(P_), rows(rc) {
+    nd_ix.build(P.nd);
+    sc5_ix.build(P.sc5);
+    sc3_ix.build(P.sc3);
     for (auto& e : P.ins)
         for (auto& kv : e.second) {
             HVar* v = new_var();
@@ -154,10 +157,9 @@ RPos& RState::at(int p) {
                 r.v[std::string(1, index_base(b))] = v;
             }
     }
-    auto it = P.nd.find(p);
-    if (it != P.nd.end()) {
+    if (const auto* nd = nd_ix.find(p)) {
         r.key = true;
-        for (auto& kv : it->second) {
+        for (auto& kv : *nd) {
             HVar* v = new_var();
             *v = kv.second;
             r.v[kv.first] = v;
@@ -205,8 +207,8 @@ const HVar* RState::peek(int p, const std::string& d) {
     }
     // what at() would load: the dense row's key flag and bases, then the non-dense entries
     const DenseRow* dr = rows.row(p);
-    auto nit = P.nd.find(p);
-    const bool key = (dr && (dr->rowflags & 1u)) || nit != P.nd.end();
+    const auto* nd = nd_ix.find(p);
+    const bool key = (dr && (dr->rowflags & 1u)) || nd != nullptr;
     if (!key) return nullptr;
     if (d.size() == 1 && dr) {
         const int b = base_index(d[0]);
@@ -215,9 +217,9 @@ const HVar* RState::peek(int p, const std::string& d) {
             return &peek_tmp_;
         }
     }
-    if (nit != P.nd.end()) {
-        auto j = nit->second.find(d);
-        if (j != nit->second.end()) return &j->second;
+    if (nd) {
+        auto j = nd->find(d);
+        if (j != nd->end()) return &j->second;
     }
     return nullptr;
 }
@@ -738,13 +740,9 @@ struct Realigner {
         return mm <= MMx && mm / (double)s.size() < 0.15;
     }
 
-    HSclip* sc5(int p) {
-        auto it = S.P.sc5.find(p);
-        return it == S.P.sc5.end() ? nullptr : &it->second;
-    }
+    HSclip* sc5(int p) { return S.sc5_ix.find(p); }
     HSclip* sc3(int p) {
-        auto it = S.P.sc3.find(p);
-        return it == S.P.sc3.end() ? nullptr : &it->second;
+        return S.sc3_ix.find(p);
     }
 
     // noPassingReads (VariationRealigner.cpp:1447-1498): no record of the region spans [start, end]

@@ -7,6 +7,7 @@
 // changed; the scoring kernels then read changed positions from a record list and every other
 // position straight from the dense rows still resident in HBM.
 #pragma once
+#include <climits>
 #include <cstdint>
 #include <deque>
 #include <functional>
@@ -108,6 +109,42 @@ class ReadQuery {
 };
 
 
+// Read-only position -> value index over a std::map built once (open addressing, linear probing):
+// one or two cache lines per lookup where the map's tree walk takes a dozen.  The map must not gain
+// or lose entries while the index is used (values may change).
+template <class V>
+class PosIndex {
+  public:
+    template <class M>
+    void build(M& m) {
+        size_t cap = 16;
+        while (cap < 2 * m.size() + 1) cap *= 2;
+        keys_.assign(cap, EMPTY);
+        vals_.assign(cap, nullptr);
+        mask_ = cap - 1;
+        for (auto& kv : m) {
+            size_t i = slot(kv.first);
+            while (keys_[i] != EMPTY) i = (i + 1) & mask_;
+            keys_[i] = kv.first;
+            vals_[i] = &kv.second;
+        }
+    }
+    V* find(int p) const {
+        if (keys_.empty()) return nullptr;
+        for (size_t i = slot(p);; i = (i + 1) & mask_) {
+            if (keys_[i] == p) return vals_[i];
+            if (keys_[i] == EMPTY) return nullptr;
+        }
+    }
+
+  private:
+    static constexpr int EMPTY = INT32_MIN;  // never a position
+    size_t slot(int p) const { return (size_t)(((uint32_t)p * 0x9E3779B1u) >> 7) & mask_; }
+    std::vector<int> keys_;
+    std::vector<V*> vals_;
+    size_t mask_ = 0;
+};
+
 struct RPos {
     bool loaded = false;
     bool key = false;   // nonInsertionVariants has the position
@@ -136,6 +173,8 @@ class RState {
 
     HostPileup& P;
     RowCache& rows;
+    PosIndex<std::map<std::string, HVar>> nd_ix;  // P.nd by position
+    PosIndex<HSclip> sc5_ix, sc3_ix;               // P.sc5 / P.sc3 by position
     std::deque<HVar> pool;
     std::unordered_map<int, RPos> pos;
     std::map<int, std::map<std::string, HVar*>> ins;

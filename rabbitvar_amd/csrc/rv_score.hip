@@ -20,11 +20,7 @@ __global__ void __launch_bounds__(256) k_score_pick(ScoreIn S) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     bool need = false;
     const int p = S.dense_lo + i;
-    if (i < S.dense_n) {
-        PosView<4> V;
-        load_pos<4>(S, p, V);
-        need = score_needed<4>(S, p, true, V);
-    }
+    if (i < S.dense_n) need = dense_needed(S, p);
     const unsigned long long m = __ballot(need);
     const int lane = (int)(threadIdx.x & 63u);
     unsigned base = 0;

@@ -526,6 +526,26 @@ __device__ __forceinline__ bool score_needed(const ScoreIn& S, int p, bool dense
     return true;
 }
 
+// score_needed for a dense-kernel position, from the row's flags alone (no allele view): what the
+// pick pass evaluates for every dense position.
+__device__ __forceinline__ bool dense_needed(const ScoreIn& S, int p) {
+    if (cx_find(S, p) >= 0) return false;  // scored by the record-list kernel
+    const int i = p - S.dense_lo;
+    if (i < 0 || i >= S.dense_n) return false;
+    const DenseRow& r = S.rows[i];
+    if (!(r.rowflags & 1u)) return false;
+    int n = 0, only = -1;
+    for (int b = 0; b < 4; b++)
+        if (r.b[b].flags & 1u) {
+            n++;
+            only = b;
+        }
+    if (n == 0 || r.cov == 0) return false;
+    const RefWindow rf{S.ref, S.ref_lo, S.ref_hi};
+    if (n == 1 && rf.has(p) && "ACGT"[only] == rf.at(p)) return S.cfg.doPileup || S.cfg.hasBam2;
+    return true;
+}
+
 template <int CAP, int NA>
 __device__ void score_position(const ScoreIn& S, int p, bool dense_only, int slot) {
     const Config& c = S.cfg;

@@ -35,6 +35,18 @@ TRAIN_CASES = [
 TRAIN_SEEDS = [1001, 1002, 1003]
 VALID_SEED = 1101
 
+# The full-scale set (--scale full, about 2e6 candidates): genomes shaped as the three germline and
+# somatic workloads the screen serves, at their full sizes, generated, collected and deleted one at
+# a time so that disk and memory stay at one genome.  Seeds disjoint from every test, bench and
+# validation genome.
+FULL_CASES = [
+    ("C1 1 Mb 30x", ["--len", "1000000", "--depth", "30"], list(range(2001, 2011))),
+    ("C2 5 Mb 60x", ["--len", "5000000", "--depth", "60", "--indel-rate", "0.000125", "--indel-max", "30",
+                     "--long-dels", "200", "--repeat-frac", "0.02"], list(range(2101, 2109))),
+    ("C3 tumor 10 Mb 200x", ["--len", "10000000", "--depth", "200", "--somatic-rate", "0.00006", "--vaf", "0.01",
+                             "0.2", "--indel-rate", "0.0001", "--dup-rate", "0.02"], list(range(2201, 2210))),
+]
+
 
 def synth(out, args, seed):
     os.makedirs(out, exist_ok=True)
@@ -128,13 +140,31 @@ def train_main(args):
     from .engine import Engine, Mlp
 
     t0 = time.time()
-    dirs = []
-    for ci, case in enumerate(TRAIN_CASES):
-        for seed in TRAIN_SEEDS:
-            dirs.append(synth(os.path.join(args.work, f"train_{ci}_{seed}"), case, seed))
-    print(f"[screen] {len(dirs)} training genomes generated ({time.time() - t0:.1f} s)", flush=True)
     eng = Engine(sample="S1", fisher=1)
-    X, y, kinds = collect(eng, dirs)
+    if args.scale == "full":
+        import shutil
+
+        parts, per_case = [], {}
+        for label, case, seeds in FULL_CASES:
+            for seed in seeds:
+                d = synth(os.path.join(args.work, f"full_{seed}"), case, seed)
+                Xi, yi, ki = collect(eng, [d])
+                shutil.rmtree(d)
+                parts.append((Xi, yi, ki))
+                per_case[label] = per_case.get(label, 0) + int(len(yi))
+                print(f"[screen] {label} seed {seed}: {len(yi)} candidates ({time.time() - t0:.0f} s)", flush=True)
+        X = np.concatenate([p[0] for p in parts])
+        y = np.concatenate([p[1] for p in parts])
+        kinds = np.concatenate([p[2] for p in parts])
+        del parts
+    else:
+        dirs = []
+        for ci, case in enumerate(TRAIN_CASES):
+            for seed in TRAIN_SEEDS:
+                dirs.append(synth(os.path.join(args.work, f"train_{ci}_{seed}"), case, seed))
+        print(f"[screen] {len(dirs)} training genomes generated ({time.time() - t0:.1f} s)", flush=True)
+        X, y, kinds = collect(eng, dirs)
+        per_case = None
     pos = float(y.mean())
     print(f"[screen] {len(y)} candidates, {pos:.4f} positive; per kind "
           + ", ".join(f"{k}: {int((kinds == k).sum())} ({float(y[kinds == k].mean()) if (kinds == k).any() else 0:.3f})"
@@ -174,8 +204,11 @@ def train_main(args):
     threshold = max(ok) if ok else min(float(t) for t in cand)
     meta = {"samples": int(len(y)), "positive_fraction": pos, "pos_weight": pos_weight, "epochs": args.epochs,
             "batch": args.batch, "lr": args.lr, "seed": args.seed, "epoch_loss": [float(v) for v in losses],
-            "train_seconds": train_s, "heldout_candidates": cand, "validation_sweep": sweep,
-            "train_cases": TRAIN_CASES, "train_seeds": TRAIN_SEEDS, "valid_seed": VALID_SEED,
+            "train_seconds": train_s, "heldout_candidates": cand, "validation_sweep": sweep, "scale": args.scale,
+            "train_cases": ([[lab, case, seeds] for lab, case, seeds in FULL_CASES] if args.scale == "full"
+                            else TRAIN_CASES),
+            "train_seeds": None if args.scale == "full" else TRAIN_SEEDS, "candidates_per_case": per_case,
+            "valid_seed": VALID_SEED,
             "threshold_rule": "largest grid threshold with held-out candidate recall and validation call-set "
                               "recall both >= 0.999"}
     save_model(args.out, params, threshold, meta)
@@ -195,6 +228,9 @@ def main(argv=None):
     t.add_argument("--lr", type=float, default=2e-3)
     t.add_argument("--pos-weight", type=float, default=0.0)
     t.add_argument("--seed", type=int, default=1)
+    t.add_argument("--scale", choices=["small", "full"], default="small",
+                   help="small: the 12 small genomes; full: the full-size C1/C2/C3-shaped set (about 2e6 "
+                        "candidates)")
     args = ap.parse_args(argv)
     if args.cmd == "train":
         train_main(args)

@@ -510,7 +510,16 @@ int main(int argc, char** argv) {
     in.counters = sctr;
     std::vector<uint8_t> status((size_t)W.dense_n + cx.pos.size(), 0);
     in.status = status.data();
-    for (int i = 0; i < W.dense_n; i++) score_position<128, 4>(in, W.dense_lo + i, true, i);
+    for (int i = 0; i < W.dense_n; i++) {
+        // the pick pass's flags-only check must agree with the scoring kernel's own checks
+        PosView<4> pv;
+        load_pos<4>(in, W.dense_lo + i, pv);
+        if (dense_needed(in, W.dense_lo + i) != score_needed<4>(in, W.dense_lo + i, true, pv)) {
+            fprintf(stderr, "pick check differs at %d\n", W.dense_lo + i);
+            return 4;
+        }
+        score_position<128, 4>(in, W.dense_lo + i, true, i);
+    }
     for (int i = 0; i < in.n_cx; i++) score_position<1536, MAXV>(in, cx.pos[(size_t)i], false, W.dense_n + i);
     out.resize(std::min<unsigned>(nout, (unsigned)out.size()));
     // the printer's order: alignedVariants slot order, then the variants' order within a position
