@@ -132,6 +132,13 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise OSError(f"{LIB_PATH} is missing: build it with `python -m rabbitvar_amd.build`")
+    # PyTorch (DeviceReads, the tests' device buffers) ships its own HIP runtime.  A process that
+    # initialises the system runtime through this library first finds no GPU when PyTorch starts
+    # later, so PyTorch is loaded first when it is installed and the library binds to its runtime.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)
