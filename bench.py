@@ -292,7 +292,9 @@ def main():
             stage[k] += st[k]
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    log(f"timed {a.steps} steps in {dt:.3f} s (output order by {'slot sort' if st['order_sorted'] else 'map replay'})")
+    osort = st["order_sorted"]
+    log(f"timed {a.steps} steps in {dt:.3f} s (nonInsertionVariants order by {'slot sort' if osort & 1 else 'map replay'},"
+        f" alignedVariants order by {'per-growth sort' if osort & 2 else 'map replay'})")
     reads_per_s = batch.n * a.steps / dt
 
     alg_bytes = walk_alg_bytes(batch, L)

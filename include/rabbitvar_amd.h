@@ -151,7 +151,9 @@ typedef struct rv_stats {
     double order_ms;           /* key-order sort and the map-order replay left on the critical path */
     double format_ms;          /* host: text */
     double wall_ms;            /* rv_run_region, end to end */
-    int64_t order_sorted;      /* 1: output order by the slot sort, 0: by the full map replay */
+    int64_t order_sorted;      /* how the output order was found: bit 0 nonInsertionVariants by the slot
+                                  sort (else the map replay), bit 1 alignedVariants by the per-growth
+                                  sort (else the map replay) */
     double detail_ms[16];      /* finer host split: realign prefetch, mnp, del, ins, lgins, record list,
                                   key sort, order-thread wait, alignedVariants order, rank+sort,
                                   event reduction, clip reduction, noPassingReads, dense-row fetches on

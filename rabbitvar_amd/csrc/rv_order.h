@@ -72,4 +72,26 @@ std::vector<int32_t> aligned_order(const std::vector<int32_t>& processing, const
 // returns its slot order.  ops[i] = key, transient[i] = erased right after its insertion.
 std::vector<int32_t> replay_ops(const std::vector<int32_t>& ops, const std::vector<uint8_t>& transient);
 
+// The same slot order without replaying, for operations on distinct keys.  Between two growths
+// (an epoch) the table re-inserts the previous table's keys in its slot order, then takes the
+// epoch's operations; robin_hood keeps keys sorted by home slot (no wrap-around) and, within a
+// home, by the info byte's hash bits descending, equal bytes in placement order.  An info byte
+// that would overflow (displacement 6, 14, 30, 62 at increments 32, 16, 8, 4) makes the next
+// insertion halve the increment, dropping a hash bit.  Two keys of one home keep the order fixed
+// when the later one was placed: by the hash bits at the level then in force, ties by placement.
+// So the epoch's slot order is a sort by (home, hash bits of the final level descending, then per
+// earlier level l: placed after halving l+1, else bit l descending; then placement index).
+// The halving steps depend only on displacements, which depend only on how many keys share each
+// home: every set present in an epoch is a subset of its final keys plus at most one transient
+// key, so each run of the final packing is simulated on its own, and only runs that can reach
+// a displacement of 6.  Returns false (`out` unspecified) when a displacement of 126 is reached
+// (the table would grow early; the replay covers that case).
+bool sorted_ops_order(const int32_t* ops, const uint8_t* transient, size_t n, std::vector<int32_t>& out,
+                      unsigned threads = 1);
+
+// alignedVariants in slot order (as aligned_table), by sorted_ops_order where it applies and the
+// replay otherwise.  `sorted` reports which.
+std::vector<int32_t> aligned_slot_order(const std::vector<int32_t>& processing, const uint8_t* status, int status_lo,
+                                        int status_n, unsigned threads, bool* sorted = nullptr);
+
 }  // namespace rv

@@ -19,6 +19,44 @@ namespace rv {
 
 namespace {
 
+// slot[i] = index of recs[i].position in `order` (distinct keys); left as it is when absent.  A
+// bitmap of the records' positions filters the scan, which is split over threads.
+void slot_ranks(const std::vector<int32_t>& order, const std::vector<OutRec>& recs, std::vector<int64_t>& slot,
+                unsigned threads) {
+    const size_t n = recs.size();
+    if (n == 0 || order.empty()) return;
+    std::vector<std::pair<int32_t, uint32_t>> by_pos(n);
+    int32_t lo = INT32_MAX, hi = INT32_MIN;
+    for (size_t i = 0; i < n; i++) {
+        by_pos[i] = {recs[i].position, (uint32_t)i};
+        lo = std::min(lo, recs[i].position);
+        hi = std::max(hi, recs[i].position);
+    }
+    std::sort(by_pos.begin(), by_pos.end());
+    const size_t span = (size_t)((int64_t)hi - lo + 1);
+    std::vector<uint64_t> bits((span + 63) / 64, 0);
+    for (const auto& p : by_pos) {
+        const size_t k = (size_t)((int64_t)p.first - lo);
+        bits[k >> 6] |= 1ull << (k & 63);
+    }
+    auto scan = [&](size_t a, size_t b) {
+        for (size_t r = a; r < b; r++) {
+            const int32_t k = order[r];
+            if (k < lo || k > hi) continue;
+            const size_t x = (size_t)((int64_t)k - lo);
+            if (!((bits[x >> 6] >> (x & 63)) & 1u)) continue;
+            auto it = std::lower_bound(by_pos.begin(), by_pos.end(), std::make_pair(k, 0u));
+            for (; it != by_pos.end() && it->first == k; ++it) slot[it->second] = (int64_t)r;
+        }
+    };
+    const size_t m = order.size();
+    const size_t nt = m < (1u << 18) ? 1 : std::max(1u, std::min(threads, 8u));
+    std::vector<std::thread> pool;
+    for (size_t t = 1; t < nt; t++) pool.emplace_back(scan, m * t / nt, m * (t + 1) / nt);
+    scan(0, m / nt);
+    for (auto& th : pool) th.join();
+}
+
 class DeviceRowSource : public RowSource {
   public:
     explicit DeviceRowSource(Engine& e) : e_(e) {}
@@ -320,11 +358,12 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     replay.join();
     e.stats.detail_ms[7] = ms_since(t0);
     if (replay_err) return e.fail(RV_ERR_DEVICE, "key order replay failed");
-    e.stats.order_sorted = replayed ? 0 : 1;
+    e.stats.order_sorted = replayed ? 0 : 1;  // alignedVariants' bit is added below
     const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
     const size_t n = recs.size();
     std::vector<uint32_t> perm(n);  // record indices in printing order
-    RHOrder aligned;
+    std::vector<int32_t> aligned;  // alignedVariants' keys in slot order
+    bool aligned_sorted = false;
     std::vector<int32_t> proc;
     std::exception_ptr order_err;
     double order_thread_ms = 0;
@@ -345,16 +384,14 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
                 const int i = cx.pos[k] - pr.P.dense_lo;
                 if (i >= 0 && i < pr.P.dense_n) status[(size_t)i] = status[(size_t)pr.P.dense_n + k];
             }
-            aligned_table(*procp, status.data(), pr.P.dense_lo, pr.P.dense_n, aligned, std::min(hw, 8u));
+            aligned = aligned_slot_order(*procp, status.data(), pr.P.dense_lo, pr.P.dense_n, std::min(hw, 8u),
+                                         &aligned_sorted);
             const auto q1 = std::chrono::steady_clock::now();
             e.stats.detail_ms[8] = std::chrono::duration<double, std::milli>(q1 - q0).count();
-            // iteration order is slot order: a record's rank is its start position's slot
-            std::vector<int64_t> slot(n);
-            for (size_t i = 0; i < n; i++) {
-                const int64_t sl = aligned.find_slot(recs[i].position);
-                slot[i] = sl < 0 ? INT64_MAX : sl;
-                perm[i] = (uint32_t)i;
-            }
+            // iteration order is slot order: a record's rank is its start position's index there
+            std::vector<int64_t> slot(n, INT64_MAX);
+            slot_ranks(aligned, recs, slot, hw);
+            for (size_t i = 0; i < n; i++) perm[i] = (uint32_t)i;
             std::sort(perm.begin(), perm.end(), [&](uint32_t x, uint32_t y) {
                 if (slot[x] != slot[y]) return slot[x] < slot[y];
                 if (recs[x].position != recs[y].position) return recs[x].position < recs[y].position;
@@ -414,6 +451,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
         }
     }
     order_ms += order_thread_ms;
+    if (aligned_sorted) e.stats.order_sorted |= 2;
     // join the lines in printing order
     auto line_len = [&](size_t i) {
         const size_t t = (size_t)(line_at[i] >> 40);

@@ -20,6 +20,7 @@ namespace rv {
 class RHOrder {
   public:
     size_t size() const { return num_; }
+    uint32_t info_increment() const { return inc_; }  // 32 until an overflow halves it
 
     // operator[] / insert: adds the key when absent
     void insert(int32_t key) {

@@ -143,6 +143,20 @@ def test_sorted_slot_order_with_realigner_log(sim, n, seed):
     assert applied == equal and applied > 0
 
 
+@pytest.mark.parametrize("n,seed", [(2000, 1), (60000, 2), (400000, 3), (2500000, 4)])
+def test_per_growth_sorted_order_matches_replay(sim, n, seed):
+    """alignedVariants' slot order by the per-growth sort (growth epochs, info-increment halvings
+    found by simulating only the runs that can reach them) against the full robin_hood replay, on
+    insert and insert-then-erase sequences with crowded home ranges: identical whenever it claims
+    to apply, and the sequences reach halved increments."""
+    r = subprocess.run([sim, "--aligned-check", str(n), str(seed)], capture_output=True, text=True, check=True)
+    applied, equal, halved, bailed, min_inc = (int(x) for x in r.stdout.split())
+    print(f"n={n}: applied {applied} equal {equal} halved {halved} bailed {bailed} smallest increment {min_inc}")
+    assert applied == equal and applied > 0
+    if n >= 60000:
+        assert halved > 0 and min_inc <= 8  # two or more halvings in a sequence the sort applied to
+
+
 def test_find35match_against_nested_loops(sim):
     """find35match's per-diagonal evaluation against the reference's nested loops
     (src/VariationRealigner.cpp:1396-1428) restated directly, on random clip consensus pairs of
