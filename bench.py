@@ -294,7 +294,7 @@ def main():
     dt = time.perf_counter() - t0
     osort = st["order_sorted"]
     log(f"timed {a.steps} steps in {dt:.3f} s (nonInsertionVariants order by {'slot sort' if osort & 1 else 'map replay'},"
-        f" alignedVariants order by {'per-growth sort' if osort & 2 else 'map replay'})")
+        f" alignedVariants order by {('device' if osort & 4 else 'host') + ' per-growth sort' if osort & 2 else 'map replay'})")
     reads_per_s = batch.n * a.steps / dt
 
     alg_bytes = walk_alg_bytes(batch, L)
@@ -328,6 +328,8 @@ def main():
                                              [round(x, 2) for x in st["detail_ms"]])),
         "realign_device": {"kernel": "k_find35", "pairs": st["realign_device_pairs"],
                            "ms": round(st["realign_device_ms"], 4)},
+        "aligned_order_split": {"operations": st["order_ops"], "list_ms": round(st["order_ops_ms"], 3),
+                                "device_ms": round(st["order_device_ms"], 3)},
         "roofline": {"kernel": "k_walk", "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "alg_bytes_per_launch": alg_bytes, "avg_launch_ms": round(avg_walk_s * 1e3, 4)},

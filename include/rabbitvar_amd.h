@@ -152,8 +152,8 @@ typedef struct rv_stats {
     double format_ms;          /* host: text */
     double wall_ms;            /* rv_run_region, end to end */
     int64_t order_sorted;      /* how the output order was found: bit 0 nonInsertionVariants by the slot
-                                  sort (else the map replay), bit 1 alignedVariants by the per-growth
-                                  sort (else the map replay) */
+                                  sort (else the map replay); bit 1 alignedVariants by the per-growth
+                                  sort (else the map replay), bit 2 that sort ran on the device */
     double detail_ms[16];      /* finer host split: realign prefetch, mnp, del, ins, lgins, record list,
                                   key sort, order-thread wait, alignedVariants order, rank+sort,
                                   event reduction, clip reduction, noPassingReads, dense-row fetches on
@@ -168,6 +168,10 @@ typedef struct rv_stats {
     double screen_list_ms;     /* of screen_ms: the host's candidate list and neighbourhood tables */
     int64_t realign_device_pairs; /* realignlgins30 clip-consensus pairs scored by the device kernel */
     double realign_device_ms;     /* that kernel's time */
+    double order_ops_ms;          /* of aligned_order: listing alignedVariants' operations (device), with
+                                     the transfers and host steps around the slot order */
+    double order_device_ms;       /* of aligned_order: the slot order's growth epochs on the device */
+    int64_t order_ops;            /* those operations */
 } rv_stats;
 
 void rv_config_default(rv_config* cfg);
@@ -291,6 +295,16 @@ rv_status rv_debug_fisher(rv_engine* e, const int32_t* tables, size_t n, double*
  * score (score -1: a string longer than 1024, not scored on the device). */
 rv_status rv_debug_find35(rv_engine* e, const char* arena, size_t arena_len, const int32_t* pairs, size_t n,
                           int32_t* out);
+
+/* Diagnostics: the device's robin_hood slot order, the order in which the reference prints
+ * alignedVariants (src/modes/simpleMode.cpp:146, the map filled in src/ToVarsBuilder.cpp:97-205):
+ * n operations on distinct keys applied to an empty robin_hood<int> map, inserting keys[i] and
+ * erasing it again right away when transient[i] is nonzero.  out (room for n keys) receives the
+ * keys present at the end in slot order, *n_out their count.  *on_device = 0 when the device path
+ * declines the sequence (a run too long for its kernels, or a displacement at which robin_hood
+ * grows its table early); nothing is written then. */
+rv_status rv_debug_slot_order(rv_engine* e, const int32_t* keys, const uint8_t* transient, size_t n, int32_t* out,
+                              size_t* n_out, int* on_device);
 
 /* rv_debug_fisher without any allocation: no reference counterpart; for callers that cannot allocate
  * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).

@@ -62,6 +62,7 @@ class RvStats(ctypes.Structure):
         ("call_ms", c_double), ("screen_ms", c_double), ("screen_candidates", c_int64),
         ("screen_skipped", c_int64), ("screen_list_ms", c_double),
         ("realign_device_pairs", c_int64), ("realign_device_ms", c_double),
+        ("order_ops_ms", c_double), ("order_device_ms", c_double), ("order_ops", c_int64),
     ]
 
 
@@ -88,6 +89,8 @@ SIGNATURES = {
     "rv_debug_fisher": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_int32), c_size_t, ctypes.POINTER(ctypes.c_double)]),
     "rv_debug_find35": (c_int, [c_void_p, ctypes.c_char_p, c_size_t, ctypes.POINTER(ctypes.c_int32), c_size_t,
                                 ctypes.POINTER(ctypes.c_int32)]),
+    "rv_debug_slot_order": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_uint8), c_size_t,
+                                    ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(c_size_t), ctypes.POINTER(c_int)]),
     "rv_fisher_tables_noalloc": (c_int, [ctypes.POINTER(ctypes.c_int32), c_size_t, ctypes.POINTER(ctypes.c_double)]),
     "rv_set_screen": (c_int, [c_void_p, ctypes.POINTER(ctypes.c_float), c_size_t, ctypes.c_float]),
     "rv_screen_dataset": (c_int, [c_void_p, ctypes.POINTER(RvRegion), ctypes.POINTER(RvReads),

@@ -24,7 +24,7 @@ HIP_FLAGS = [
     "-Wno-unused-command-line-argument",
 ]
 
-LIB_SOURCES = ["rv_pileup.hip", "rv_score.hip", "rv_order.hip", "rv_realign.hip", "rv_host.cpp", "rv_engine.cpp", "rv_region.cpp",
+LIB_SOURCES = ["rv_pileup.hip", "rv_score.hip", "rv_order.hip", "rv_slotorder.hip", "rv_realign.hip", "rv_host.cpp", "rv_engine.cpp", "rv_region.cpp",
                "rv_realign.cpp", "rv_format.cpp", "rv_order.cpp", "rv_somatic.cpp", "rv_screen.hip", "rv_mlp.cpp", "rv_bam.cpp", "rv_capi.cpp"]
 
 

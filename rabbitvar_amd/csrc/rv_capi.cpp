@@ -209,6 +209,23 @@ rv_status rv_debug_find35(rv_engine* e, const char* arena, size_t arena_len, con
     return RV_OK;
 }
 
+rv_status rv_debug_slot_order(rv_engine* e, const int32_t* keys, const uint8_t* transient, size_t n, int32_t* out,
+                              size_t* n_out, int* on_device) {
+    if (!e || !n_out || !on_device || (n && (!keys || !transient || !out))) return RV_ERR_ARG;
+    *n_out = 0;
+    *on_device = 0;
+    return guarded(e, [&] {
+        std::vector<int32_t> order;
+        bool applied = false;
+        const rv_status st = e->e->slot_order(keys, transient, n, order, applied);
+        if (st != RV_OK || !applied) return st;
+        std::copy(order.begin(), order.end(), out);
+        *n_out = order.size();
+        *on_device = 1;
+        return RV_OK;
+    });
+}
+
 rv_status rv_debug_fisher(rv_engine* e, const int32_t* tables, size_t n, double* out) {
     if (!e || (n && (!tables || !out)) || n > (size_t)0x7fffffff / 4) return RV_ERR_ARG;
     if (n == 0) return RV_OK;

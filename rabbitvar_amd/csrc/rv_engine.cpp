@@ -76,6 +76,8 @@ Engine::Engine(const rv_config& c, int dev) {
         if (hipEventCreate(&e) != hipSuccess) err = "hipEventCreate failed";
     for (auto& e : sev)
         if (hipEventCreate(&e) != hipSuccess) err = "hipEventCreate failed";
+    for (auto& e : so_ev)
+        if (hipEventCreate(&e) != hipSuccess) err = "hipEventCreate failed";
 }
 
 Engine::~Engine() {
@@ -87,7 +89,10 @@ Engine::~Engine() {
                      &b_out_arena, &b_score_ctr, &b_status, &b_keys, &b_keys2, &b_kpos, &b_kpos2, &b_sort_tmp,
                      &b_kcount, &b_args, &b_kpos3, &b_nopass, &b_rows2, &b_want, &b_dup, &b_drop,
                      &b_scr_cand, &b_scr_pos, &b_scr_cum, &b_scr_feat, &b_scr_score, &b_scr_params,
-                     &b_fisher_in, &b_fisher_out, &b_lgamma, &b_f35_pairs, &b_f35_arena, &b_f35_out};
+                     &b_fisher_in, &b_fisher_out, &b_lgamma, &b_f35_pairs, &b_f35_arena, &b_f35_out, &b_pick,
+                     &b_so_ops, &b_so_tr, &b_so_ord[0], &b_so_ord[1], &b_so_k[0], &b_so_k[1], &b_so_k[2], &b_so_k[3],
+                     &b_so_w[0], &b_so_w[1], &b_so_w[2], &b_so_w[3], &b_so_runs, &b_so_small, &b_so_tmp,
+                     &b_so_proc};
     // teardown: nothing to report a failure to
     for (auto* b : all)
         if (b->p) (void)hipFree(b->p);
@@ -96,6 +101,8 @@ Engine::~Engine() {
     for (auto& e : ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : sev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : so_ev)
         if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
 }
@@ -523,6 +530,148 @@ rv_status Engine::key_order(int n_reads, std::vector<int32_t>& created, std::vec
     created.resize(n);
     RV_HIP(hipMemcpyAsync(created.data(), pos2, sizeof(int32_t) * n, hipMemcpyDeviceToHost, stream));
     RV_HIP(hipStreamSynchronize(stream));
+    return RV_OK;
+}
+
+rv_status Engine::slot_order(const int32_t* ops, const uint8_t* tr, size_t m, std::vector<int32_t>& out, bool& applied) {
+    applied = false;
+    out.clear();
+    if (m >= ((size_t)1 << 30)) return RV_OK;
+    auto* ops_d = (int32_t*)grow(b_so_ops, 4 * m);
+    auto* tr_d = (uint8_t*)grow(b_so_tr, m);
+    if (!ops_d || !tr_d) return fail(RV_ERR_OOM, "device buffers for the slot order failed");
+    if (m) {
+        RV_HIP(hipMemcpyAsync(ops_d, ops, 4 * m, hipMemcpyHostToDevice, stream));
+        RV_HIP(hipMemcpyAsync(tr_d, tr, m, hipMemcpyHostToDevice, stream));
+    }
+    return slot_order_dev(ops_d, tr_d, tr, m, out, applied);
+}
+
+rv_status Engine::aligned_slot_order(const int32_t* proc_dev, size_t n_proc, uint8_t* status_dev, int dense_lo,
+                                     int dense_n, const int32_t* cx_pos_dev, int ncx, std::vector<int32_t>& out,
+                                     bool& applied, std::vector<int32_t>& ops, std::vector<uint8_t>& tr) {
+    applied = false;
+    out.clear();
+    ops.clear();
+    tr.clear();
+    if (n_proc >= ((size_t)1 << 30)) return fail(RV_ERR_CAPACITY, "processing order too long for the slot order");
+    const int nb = (int)((n_proc + SO_CB - 1) / SO_CB);
+    auto* ops_d = (int32_t*)grow(b_so_ops, 4 * std::max<size_t>(n_proc, 1));
+    auto* tr_d = (uint8_t*)grow(b_so_tr, std::max<size_t>(n_proc, 1));
+    auto* bc = (int32_t*)grow(b_so_w[0], 4 * ((size_t)nb + 2));
+    auto* bo = (int32_t*)grow(b_so_w[1], 4 * ((size_t)nb + 2));
+    size_t tb = 0;
+    RV_HIP(so_ops_scan_bytes(nb, tb));
+    void* tmp = grow(b_so_tmp, tb);
+    if (!ops_d || !tr_d || !bc || !bo || !tmp) return fail(RV_ERR_OOM, "device buffers for the slot order failed");
+    RV_HIP(launch_so_ops(proc_dev, (int)n_proc, status_dev, dense_lo, dense_n, cx_pos_dev, ncx, bc, bo, tmp, tb, ops_d,
+                         tr_d, stream));
+    int32_t m = 0;
+    RV_HIP(hipMemcpyAsync(&m, bo + nb, sizeof m, hipMemcpyDeviceToHost, stream));
+    RV_HIP(hipStreamSynchronize(stream));
+    tr.resize((size_t)m);
+    if (m) {
+        RV_HIP(hipMemcpyAsync(tr.data(), tr_d, (size_t)m, hipMemcpyDeviceToHost, stream));
+        RV_HIP(hipStreamSynchronize(stream));
+    }
+    const rv_status st = slot_order_dev(ops_d, tr_d, tr.data(), (size_t)m, out, applied);
+    if (st != RV_OK || applied) return st;
+    // declined: the caller orders on the host
+    ops.resize((size_t)m);
+    if (m) {
+        RV_HIP(hipMemcpyAsync(ops.data(), ops_d, 4 * (size_t)m, hipMemcpyDeviceToHost, stream));
+        RV_HIP(hipStreamSynchronize(stream));
+    }
+    return RV_OK;
+}
+
+rv_status Engine::slot_order_dev(const int32_t* ops_d, const uint8_t* tr_d, const uint8_t* tr, size_t m,
+                                 std::vector<int32_t>& out, bool& applied) {
+    applied = false;
+    out.clear();
+    slot_order_ms = 0;
+    if (m == 0) {
+        applied = true;
+        return RV_OK;
+    }
+    // growth schedule: an insertion that finds 80% of the slots used doubles the table first
+    struct Ep {
+        size_t cap, a, b, np, S, keep;
+    };
+    std::vector<Ep> eps;
+    size_t cap = 8, count = 0, a = 0, max_keep = 1, max_S = 1;
+    for (size_t i = 0; i < m; i++) {
+        if (count >= cap * 80 / 100) {
+            eps.push_back({cap, a, i, 0, 0, count});
+            a = i;
+            cap *= 2;
+        }
+        count += tr[i] ? 0 : 1;
+    }
+    eps.push_back({cap, a, m, 0, 0, count});
+    if (rh_log2(cap) > 30) return RV_OK;
+    for (size_t e = 0; e < eps.size(); e++) {
+        eps[e].np = e ? eps[e - 1].keep : 0;
+        eps[e].S = eps[e].np + (eps[e].b - eps[e].a);
+        max_S = std::max(max_S, eps[e].S);
+        max_keep = std::max(max_keep, eps[e].keep);
+    }
+    int32_t* ord[2] = {(int32_t*)grow(b_so_ord[0], 4 * max_keep), (int32_t*)grow(b_so_ord[1], 4 * max_keep)};
+    uint32_t* k[4];
+    for (int i = 0; i < 4; i++) k[i] = (uint32_t*)grow(b_so_k[i], 4 * max_S);
+    int32_t* w[4];
+    for (int i = 0; i < 4; i++) w[i] = (int32_t*)grow(b_so_w[i], 4 * max_keep);
+    auto* runs = (int32_t*)grow(b_so_runs, 7 * 4 * max_keep);
+    auto* small = (unsigned*)grow(b_so_small, 64);
+    if (!ord[0] || !ord[1] || !k[0] || !k[1] || !k[2] || !k[3] || !w[0] || !w[1] || !w[2] || !w[3] || !runs || !small)
+        return fail(RV_ERR_OOM, "device buffers for the slot order failed");
+    size_t sort_bytes = 0, scan_bytes = 0;
+    RV_HIP(so_sort_pairs(nullptr, sort_bytes, k[0], k[1], k[2], k[3], (int)max_S, 31, stream));
+    RV_HIP(so_scan_bytes((int)max_keep, scan_bytes));
+    const size_t tmp_bytes = std::max(sort_bytes, scan_bytes);
+    void* tmp = grow(b_so_tmp, tmp_bytes);
+    if (!tmp) return fail(RV_ERR_OOM, "device buffer for the slot order's sorts failed");
+    SoRuns R{runs, runs + max_keep, runs + 2 * max_keep, runs + 3 * max_keep, runs + 4 * max_keep,
+             runs + 5 * max_keep, runs + 6 * max_keep, (int*)(small + 7)};
+    RV_HIP(hipEventRecord(so_ev[0], stream));
+    // small: [0, 5) first steps, [5] a run too long to simulate, [6] a home group too large, [7] runs
+    RV_HIP(hipMemsetAsync(small, 0, 64, stream));
+    int cur = 0;
+    for (const Ep& ep : eps) {
+        const int lcap = rh_log2(ep.cap);
+        SoEpoch E{ord[cur], (int)ep.np, ops_d + ep.a, tr_d + ep.a, (int)ep.S, (int)ep.keep, (uint32_t)(ep.cap - 1), lcap};
+        RV_HIP(hipMemsetAsync(small, 0xff, 5 * sizeof(unsigned), stream));
+        RV_HIP(hipMemsetAsync(small + 5, 0, sizeof(unsigned), stream));
+        RV_HIP(hipMemsetAsync(small + 7, 0, sizeof(unsigned), stream));
+        RV_HIP(launch_so_epoch_keys(E, k[0], k[1], stream));
+        size_t tb = tmp_bytes;
+        RV_HIP(so_sort_pairs(tmp, tb, k[0], k[2], k[1], k[3], (int)ep.S, lcap + 1, stream));
+        RV_HIP(launch_so_runs(k[2], (int)ep.keep, (int)ep.S, (uint32_t)(ep.cap - 1), w[0], w[1], w[2], w[3], R, tmp,
+                              tmp_bytes, stream));
+        RV_HIP(launch_so_simulate(k[2], k[3], (uint32_t)(ep.cap - 1), R, (int)ep.keep, small, small + 5, stream));
+        unsigned h[6];
+        RV_HIP(hipMemcpyAsync(h, small, sizeof h, hipMemcpyDeviceToHost, stream));
+        RV_HIP(hipStreamSynchronize(stream));
+        if (h[5] || h[4] != 0xffffffffu) return RV_OK;  // too long to simulate, or an early growth
+        SoHalve H{0, {0, 0, 0, 0}};
+        for (int l = 0; l < 4; l++)
+            if (h[l] != 0xffffffffu && (size_t)h[l] + 1 < ep.S) H.at[H.nh++] = h[l] + 1;
+        RV_HIP(launch_so_final(k[2], k[3], E, H, ord[cur ^ 1], small + 6, stream));
+        cur ^= 1;
+    }
+    const size_t n_out = eps.back().keep;
+    out.resize(n_out);
+    unsigned big = 0;
+    RV_HIP(hipMemcpyAsync(out.data(), ord[cur], 4 * n_out, hipMemcpyDeviceToHost, stream));
+    RV_HIP(hipMemcpyAsync(&big, small + 6, sizeof big, hipMemcpyDeviceToHost, stream));
+    RV_HIP(hipEventRecord(so_ev[1], stream));
+    RV_HIP(hipStreamSynchronize(stream));
+    RV_HIP(hipEventElapsedTime(&slot_order_ms, so_ev[0], so_ev[1]));
+    if (big) {
+        out.clear();
+        return RV_OK;
+    }
+    applied = true;
     return RV_OK;
 }
 

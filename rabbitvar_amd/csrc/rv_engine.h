@@ -106,6 +106,26 @@ class Engine {
     // slot order they take when sorted (see rv_order.h)
     rv_status key_order(int n_reads, std::vector<int32_t>& created, std::vector<int32_t>& slot_sorted);
 
+    // The slot order of a robin_hood<int> table after inserting ops[0, m) in order from empty,
+    // each key with tr[i] set erased right after its insertion (distinct keys), computed on the
+    // device growth by growth (rv_slotorder.hip).  applied = false (out unspecified) when a run is
+    // too long for the kernels or the table would grow early; the caller then orders on the host.
+    rv_status slot_order(const int32_t* ops, const uint8_t* tr, size_t m, std::vector<int32_t>& out, bool& applied);
+    // The same for alignedVariants of the last scoring pass, from device data: the processing
+    // order proc_dev (n_proc positions) filtered by the scoring status (status_dev: dense_n rows then
+    // ncx record-list positions cx_pos_dev, which override their rows; status 2 = erased after its
+    // insertion).  When the device declines, applied = false and ops / tr hold the operations for
+    // the host; otherwise tr holds the transient flags only.
+    rv_status aligned_slot_order(const int32_t* proc_dev, size_t n_proc, uint8_t* status_dev, int dense_lo, int dense_n,
+                                 const int32_t* cx_pos_dev, int ncx, std::vector<int32_t>& out, bool& applied,
+                                 std::vector<int32_t>& ops, std::vector<uint8_t>& tr);
+    // the epochs over operations already on the device (tr: the same flags on the host)
+    rv_status slot_order_dev(const int32_t* ops_d, const uint8_t* tr_d, const uint8_t* tr, size_t m,
+                             std::vector<int32_t>& out, bool& applied);
+    float slot_order_ms = 0;  // device time of the last slot order's epochs
+    // the last key_order's keys in slot order on the device (valid until the next key_order)
+    const int32_t* slot_sorted_dev() const { return (const int32_t*)b_kpos3.p; }
+
     // the device Fisher test on host tables (n11 n12 n21 n22) -> (two-sided, left, right) per table
     rv_status fisher_tables(const int32_t* tab, int n, double* out);
     // find35match for a batch of consensus pairs on the device (realignlgins30's phase A); false on a
@@ -121,10 +141,12 @@ class Engine {
     DevBuf b_gather_ids, b_gather_out, b_cx_pos, b_cx_off, b_cx, b_cx_arena, b_cx_cov, b_cx_key, b_out, b_out_arena,
         b_score_ctr, b_status, b_keys, b_keys2, b_kpos, b_kpos2, b_sort_tmp, b_kcount, b_kpos3, b_args, b_nopass,
         b_want, b_dup, b_drop, b_scr_cand, b_scr_pos, b_scr_cum, b_scr_feat, b_scr_score, b_fisher_in, b_fisher_out,
-        b_lgamma, b_f35_pairs, b_f35_arena, b_f35_out, b_pick;
+        b_lgamma, b_f35_pairs, b_f35_arena, b_f35_out, b_pick, b_so_ops, b_so_tr, b_so_ord[2], b_so_k[4], b_so_w[4],
+        b_so_runs, b_so_small, b_so_tmp, b_so_proc;
     WalkArgs* args_dev = nullptr;  // the last pileup's kernel arguments (its reads stay resident)
     int64_t args_n = 0;
     hipEvent_t sev[2] = {nullptr, nullptr};
+    hipEvent_t so_ev[2] = {nullptr, nullptr};  // slot_order's own timing (it runs beside other stages)
     // figures of the last region: record-list positions, dense rows fetched for the
     // realigner, insertions whose coverage came from the next position
     int64_t last_cx_positions = 0, last_rows_fetched = 0, last_insertion_side_effects = 0;

@@ -28,6 +28,49 @@ hipError_t launch_slot_keys(const int32_t* keys, int n, int cap_log2, int idx_bi
 hipError_t sort_key_pairs(void* temp, size_t& temp_bytes, const unsigned long long* kin, unsigned long long* kout,
                           const int32_t* vin, int32_t* vout, int n, int end_bit, hipStream_t s);
 
+// ---- robin_hood slot order by growth epochs (rv_slotorder.hip)
+constexpr int SO_EV_MAX = 1024;    // events of one simulated run (more: too_big)
+constexpr int SO_LEN_MAX = 1024;   // slots of one simulated run
+constexpr int SO_GROUP_MAX = 64;   // keys of one home
+struct SoEpoch {
+    const int32_t* prev;  // the previous table's keys in slot order (re-inserted first)
+    int np;
+    const int32_t* ops;  // the epoch's operations (keys) and transient flags
+    const uint8_t* tr;
+    int S;       // np + operations
+    int n_keep;  // keys present at the epoch's end
+    uint32_t mask;
+    int lcap;
+};
+struct SoRuns {  // per run of the epoch's final packing (sized for n_keep runs)
+    int32_t *lo, *hi, *a, *b, *maxd, *tlo, *tn;
+    int* n_runs;
+};
+struct SoHalve {  // steps before which the info increment halves
+    int nh;
+    uint32_t at[4];
+};
+// alignedVariants' operations: the entries of `proc` (n positions, the processing order) whose
+// status is nonzero, in order, after the record-list positions' status overrides their dense rows'
+// (status holds dense_n rows then ncx record-list positions).  bcount/boff: n / SO_CB + 2 entries;
+// boff[n / SO_CB rounded up] = the number of operations.
+constexpr int SO_CB = 2048;
+hipError_t launch_so_ops(const int32_t* proc, int n, uint8_t* status, int dense_lo, int dense_n, const int32_t* cx_pos,
+                         int ncx, int32_t* bcount, int32_t* boff, void* tmp, size_t tmp_bytes, int32_t* ops, uint8_t* tr,
+                         hipStream_t s);
+hipError_t so_ops_scan_bytes(int nb, size_t& tmp_bytes);
+hipError_t launch_so_epoch_keys(const SoEpoch& E, uint32_t* skey, uint32_t* sval, hipStream_t s);
+// slots, runs (maxd, transient ranges) of the sorted steps
+hipError_t launch_so_runs(const uint32_t* skey, int n_keep, int S, uint32_t mask, int32_t* v, int32_t* pm,
+                          int32_t* flag, int32_t* rid, const SoRuns& R, void* tmp, size_t tmp_bytes, hipStream_t s);
+hipError_t launch_so_simulate(const uint32_t* skey, const uint32_t* sval, uint32_t mask, const SoRuns& R, int max_runs,
+                              unsigned* first, unsigned* too_big, hipStream_t s);
+hipError_t launch_so_final(const uint32_t* skey, const uint32_t* sval, const SoEpoch& E, const SoHalve& H, int32_t* out,
+                           unsigned* too_big, hipStream_t s);
+hipError_t so_sort_pairs(void* tmp, size_t& tmp_bytes, const uint32_t* kin, uint32_t* kout, const uint32_t* vin,
+                         uint32_t* vout, int n, int end_bit, hipStream_t s);
+hipError_t so_scan_bytes(int n, size_t& tmp_bytes);
+
 hipError_t launch_score(const ScoreIn& S, hipStream_t s);
 hipError_t launch_fisher_table(const int* tab, int n, double* out, const double* lgt, int lgt_n, hipStream_t s);
 // the allocation-free variant's whole call (host tables in, host p-values out; rv_score.hip)

@@ -177,17 +177,28 @@ void aligned_table(const std::vector<int32_t>& processing, const uint8_t* status
     for (size_t t = 0; t < ops.size(); t++) apply_ops(out, ops[t], &tr[t]);
 }
 
-std::vector<int32_t> aligned_slot_order(const std::vector<int32_t>& processing, const uint8_t* status, int status_lo,
-                                        int status_n, unsigned threads, bool* sorted) {
+void aligned_op_list(const std::vector<int32_t>& processing, const uint8_t* status, int status_lo, int status_n,
+                     unsigned threads, std::vector<int32_t>& all, std::vector<uint8_t>& all_tr) {
     std::vector<std::vector<int32_t>> ops;
     std::vector<std::vector<uint8_t>> tr;
     aligned_ops(processing, status, status_lo, status_n, threads, ops, tr);
-    std::vector<int32_t> all;
-    std::vector<uint8_t> all_tr;
+    size_t n = 0;
+    for (auto& v : ops) n += v.size();
+    all.clear();
+    all_tr.clear();
+    all.reserve(n);
+    all_tr.reserve(n);
     for (size_t t = 0; t < ops.size(); t++) {
         all.insert(all.end(), ops[t].begin(), ops[t].end());
         all_tr.insert(all_tr.end(), tr[t].begin(), tr[t].end());
     }
+}
+
+std::vector<int32_t> aligned_slot_order(const std::vector<int32_t>& processing, const uint8_t* status, int status_lo,
+                                        int status_n, unsigned threads, bool* sorted) {
+    std::vector<int32_t> all;
+    std::vector<uint8_t> all_tr;
+    aligned_op_list(processing, status, status_lo, status_n, threads, all, all_tr);
     std::vector<int32_t> out;
     const bool ok = sorted_ops_order(all.data(), all_tr.data(), all.size(), out, threads);
     if (sorted) *sorted = ok;

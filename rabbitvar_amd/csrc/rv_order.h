@@ -89,6 +89,11 @@ std::vector<int32_t> replay_ops(const std::vector<int32_t>& ops, const std::vect
 bool sorted_ops_order(const int32_t* ops, const uint8_t* transient, size_t n, std::vector<int32_t>& out,
                       unsigned threads = 1);
 
+// The operations ToVarsBuilder makes on alignedVariants (see aligned_table): keys in order, and
+// whether each is erased right after its insertion.
+void aligned_op_list(const std::vector<int32_t>& processing, const uint8_t* status, int status_lo, int status_n,
+                     unsigned threads, std::vector<int32_t>& ops, std::vector<uint8_t>& transient);
+
 // alignedVariants in slot order (as aligned_table), by sorted_ops_order where it applies and the
 // replay otherwise.  `sorted` reports which.
 std::vector<int32_t> aligned_slot_order(const std::vector<int32_t>& processing, const uint8_t* status, int status_lo,

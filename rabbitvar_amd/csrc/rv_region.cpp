@@ -363,7 +363,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     const size_t n = recs.size();
     std::vector<uint32_t> perm(n);  // record indices in printing order
     std::vector<int32_t> aligned;  // alignedVariants' keys in slot order
-    bool aligned_sorted = false;
+    int aligned_how = 0;           // 0 replay, 1 per-growth sort on the host, 3 on the device
     std::vector<int32_t> proc;
     std::exception_ptr order_err;
     double order_thread_ms = 0;
@@ -379,13 +379,32 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
                 if (!replayed) replay_created(nonins, created);
                 proc = replay_finish(nonins, S.key_log);
             }
-            // one status per window position: record-list positions override their dense rows
-            for (size_t k = 0; k < cx.pos.size(); k++) {
-                const int i = cx.pos[k] - pr.P.dense_lo;
-                if (i >= 0 && i < pr.P.dense_n) status[(size_t)i] = status[(size_t)pr.P.dense_n + k];
+            // alignedVariants: its operations listed on the device from the processing order and
+            // the scoring status (record-list positions override their dense rows), its slot order
+            // there too; else the per-growth sort on the host, else the replay
+            const auto l0 = std::chrono::steady_clock::now();
+            const int32_t* proc_dev = procp == &walk_sorted ? e.slot_sorted_dev() : nullptr;
+            if (!proc_dev && !procp->empty()) {
+                auto* d = (int32_t*)e.grow(e.b_so_proc, 4 * procp->size());
+                if (!d || hipMemcpyAsync(d, procp->data(), 4 * procp->size(), hipMemcpyHostToDevice, e.stream) != hipSuccess)
+                    throw std::runtime_error("upload of the processing order failed");
+                proc_dev = d;
             }
-            aligned = aligned_slot_order(*procp, status.data(), pr.P.dense_lo, pr.P.dense_n, std::min(hw, 8u),
-                                         &aligned_sorted);
+            std::vector<int32_t> aops;
+            std::vector<uint8_t> atr;
+            bool on_device = false;
+            if (e.aligned_slot_order(proc_dev, procp->size(), in.status, pr.P.dense_lo, pr.P.dense_n, in.cx_pos, ncx,
+                                     aligned, on_device, aops, atr) != RV_OK)
+                throw std::runtime_error(e.err);
+            e.stats.order_ops = (int64_t)atr.size();
+            e.stats.order_device_ms = e.slot_order_ms;
+            e.stats.order_ops_ms = ms_since(l0) - e.slot_order_ms;
+            if (on_device)
+                aligned_how = 3;
+            else if (sorted_ops_order(aops.data(), atr.data(), aops.size(), aligned, std::min(hw, 8u)))
+                aligned_how = 1;
+            else
+                aligned = replay_ops(aops, atr);
             const auto q1 = std::chrono::steady_clock::now();
             e.stats.detail_ms[8] = std::chrono::duration<double, std::milli>(q1 - q0).count();
             // iteration order is slot order: a record's rank is its start position's index there
@@ -451,7 +470,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
         }
     }
     order_ms += order_thread_ms;
-    if (aligned_sorted) e.stats.order_sorted |= 2;
+    e.stats.order_sorted |= (int64_t)aligned_how << 1;
     // join the lines in printing order
     auto line_len = [&](size_t i) {
         const size_t t = (size_t)(line_at[i] >> 40);

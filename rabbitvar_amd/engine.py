@@ -124,6 +124,26 @@ class Engine:
             raise self._error(st)
         return out
 
+    def slot_order(self, keys, transient):
+        """The device's robin_hood slot order after inserting the distinct `keys` in order into an
+        empty map, erasing each key whose `transient` flag is set right after its insertion.
+        Returns the keys present at the end in slot order, or None when the device path declines
+        the sequence."""
+        k = np.ascontiguousarray(keys, np.int32)
+        t = np.ascontiguousarray(transient, np.uint8)
+        if len(k) != len(t):
+            raise ValueError("keys and transient flags differ in length")
+        out = np.zeros(max(1, len(k)), np.int32)
+        n_out = ctypes.c_size_t()
+        on = ctypes.c_int()
+        st = self.lib.rv_debug_slot_order(self.h, k.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                          t.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), len(k),
+                                          out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(n_out),
+                                          ctypes.byref(on))
+        if st != 0:
+            raise self._error(st)
+        return out[:n_out.value].copy() if on.value else None
+
     def find35(self, pairs):
         """The device's find35match on a list of (seq5, seq3) string pairs -> list of (b5, b3, score)."""
         arena, idx = bytearray(), []
