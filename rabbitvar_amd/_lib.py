@@ -63,6 +63,7 @@ class RvStats(ctypes.Structure):
         ("screen_skipped", c_int64), ("screen_list_ms", c_double),
         ("realign_device_pairs", c_int64), ("realign_device_ms", c_double),
         ("order_ops_ms", c_double), ("order_device_ms", c_double), ("order_ops", c_int64),
+        ("timeline_ms", c_double * 16),
     ]
 
 

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cstring>
 
 namespace rv {
@@ -81,7 +82,7 @@ Engine::Engine(const rv_config& c, int dev) {
 }
 
 Engine::~Engine() {
-    if (reaper_.joinable()) reaper_.join();
+    wait_release();
     for (auto& blk : pinned_) (void)hipHostFree(blk.first);
     DevBuf* all[] = {&b_prep,      &b_cig,     &b_ctr,    &b_span,       &b_rows,     &b_ref,    &b_events,
                      &b_arena,     &b_clips,   &b_clip_arena, &b_anchors, &b_counts,   &b_gather_ids, &b_gather_out,
@@ -223,6 +224,8 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
         RV_HIP(hipStreamSynchronize(stream));  // `up` is pageable and about to go out of scope
         out.ref_win = up.substr(0, (size_t)win_len);
     }
+    auto since_h0 = [&]() { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count(); };
+    stats.timeline_ms[8] = since_h0();
     W.ref = ref_dev;
     W.dense_lo = 0;
     W.dense_n = 0;
@@ -256,6 +259,7 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
     int max_span = 0;
     RV_HIP(hipMemcpyAsync(&max_span, span, sizeof(int), hipMemcpyDeviceToHost, stream));
     RV_HIP(hipStreamSynchronize(stream));
+    stats.timeline_ms[9] = since_h0();
     // ---- dense window: every position a region read can touch
     int pad = 2 * std::max(max_span, 64) + 8;
     W.dense_lo = region.start - pad;
@@ -316,6 +320,7 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
         RV_HIP(hipMemcpyAsync(hcnt, cnt64, 32, hipMemcpyDeviceToHost, stream));
         RV_HIP(hipMemcpyAsync(&hc, ctr, sizeof hc, hipMemcpyDeviceToHost, stream));
         RV_HIP(hipStreamSynchronize(stream));
+        stats.timeline_ms[10] = since_h0();
         counts[0] = (unsigned)(hcnt[0] & 0xffffffffu);
         counts[1] = (unsigned)(hcnt[0] >> 32);
         counts[2] = (unsigned)(hcnt[1] & 0xffffffffu);
@@ -343,9 +348,11 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
                                       stream));
             }
             RV_HIP(hipStreamSynchronize(stream));
+            stats.timeline_ms[11] = since_h0();
             // the previous region's host state is released while this one waits on the device; wait
             // for that before the reductions allocate
             wait_release();
+            stats.timeline_ms[12] = since_h0();
             auto r0 = std::chrono::steady_clock::now();
             out.P.dense_lo = W.dense_lo;
             out.P.dense_n = W.dense_n;
@@ -385,6 +392,7 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
             c3.join();
             for (auto& ce : clip_err)
                 if (ce) std::rethrow_exception(ce);
+            stats.timeline_ms[13] = since_h0();
             stats.detail_ms[10] = std::chrono::duration<double, std::milli>(r1 - r0).count();
             stats.detail_ms[11] = std::max(clip_ms[0], clip_ms[1]);
             host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - r0).count();
@@ -487,22 +495,34 @@ rv_status Engine::fisher_tables(const int32_t* tab, int n, double* out) {
     return RV_OK;
 }
 
-rv_status Engine::key_order(int n_reads, std::vector<int32_t>& created, std::vector<int32_t>& slot_sorted) {
-    created.clear();
-    slot_sorted.clear();
-    if (rows_n <= 0) return RV_OK;
+rv_status Engine::key_order(int n_reads, int64_t& n_keys, int& max_disp) {
+    n_keys = 0;
+    max_disp = INT_MAX;
+    key_n_ = 0;
+    key_slot_ok_ = false;
+    if (rows_n <= 0) {
+        max_disp = 0;
+        key_slot_ok_ = true;
+        return RV_OK;
+    }
     auto* keys = (unsigned long long*)grow(b_keys, sizeof(unsigned long long) * (size_t)rows_n);
     auto* keys2 = (unsigned long long*)grow(b_keys2, sizeof(unsigned long long) * (size_t)rows_n);
     auto* pos = (int32_t*)grow(b_kpos, sizeof(int32_t) * (size_t)rows_n);
     auto* pos2 = (int32_t*)grow(b_kpos2, sizeof(int32_t) * (size_t)rows_n);
-    auto* cnt = (unsigned*)grow(b_kcount, sizeof(unsigned));
+    auto* cnt = (unsigned*)grow(b_kcount, 2 * sizeof(unsigned));
     if (!keys || !keys2 || !pos || !pos2 || !cnt) return fail(RV_ERR_OOM, "device buffers for the key order failed");
-    RV_HIP(hipMemsetAsync(cnt, 0, sizeof(unsigned), stream));
+    RV_HIP(hipMemsetAsync(cnt, 0, 2 * sizeof(unsigned), stream));
     RV_HIP(launch_key_list(rows, rows_n, rows_lo, keys, pos, cnt, stream));
     unsigned n = 0;
     RV_HIP(hipMemcpyAsync(&n, cnt, sizeof n, hipMemcpyDeviceToHost, stream));
     RV_HIP(hipStreamSynchronize(stream));
-    if (n == 0) return RV_OK;
+    key_n_ = n;
+    n_keys = n;
+    if (n == 0) {
+        max_disp = 0;
+        key_slot_ok_ = true;
+        return RV_OK;
+    }
     int end_bit = 25;  // touch index (24 bits) + read ordinal
     while (end_bit < 64 && (1ll << (end_bit - 24)) <= (long long)n_reads) end_bit++;
     size_t tb = 0;
@@ -510,25 +530,45 @@ rv_status Engine::key_order(int n_reads, std::vector<int32_t>& created, std::vec
     void* tmp = grow(b_sort_tmp, tb);
     if (!tmp) return fail(RV_ERR_OOM, "device buffer for the key sort failed");
     RV_HIP(sort_key_pairs(tmp, tb, keys, keys2, pos, pos2, (int)n, end_bit, stream));
-    // the same keys in sorted slot order: pos2 holds them in creation order
+    // the same keys in sorted slot order (pos2 holds them in creation order), and the largest
+    // displacement of that layout
     auto* pos3 = (int32_t*)grow(b_kpos3, sizeof(int32_t) * (size_t)rows_n);
     if (!pos3) return fail(RV_ERR_OOM, "device buffers for the key order failed");
     const int cap_log2 = rh_log2(rh_capacity(n));
     int idx_bits = 1;
     while (idx_bits < 40 && (1ll << idx_bits) <= (long long)n) idx_bits++;
     const int slot_bits = cap_log2 + 5 + idx_bits;
-    if (slot_bits <= 64) {
-        RV_HIP(launch_slot_keys(pos2, (int)n, cap_log2, idx_bits, keys, pos, stream));
-        size_t tb2 = 0;
-        RV_HIP(sort_key_pairs(nullptr, tb2, keys, keys2, pos, pos3, (int)n, slot_bits, stream));
-        void* tmp2 = grow(b_sort_tmp, std::max(tb, tb2));
-        if (!tmp2) return fail(RV_ERR_OOM, "device buffer for the key sort failed");
-        RV_HIP(sort_key_pairs(tmp2, tb2, keys, keys2, pos, pos3, (int)n, slot_bits, stream));
-        slot_sorted.resize(n);
-        RV_HIP(hipMemcpyAsync(slot_sorted.data(), pos3, sizeof(int32_t) * n, hipMemcpyDeviceToHost, stream));
+    if (slot_bits > 64) {
+        RV_HIP(hipStreamSynchronize(stream));
+        return RV_OK;
     }
-    created.resize(n);
-    RV_HIP(hipMemcpyAsync(created.data(), pos2, sizeof(int32_t) * n, hipMemcpyDeviceToHost, stream));
+    RV_HIP(launch_slot_keys(pos2, (int)n, cap_log2, idx_bits, keys, pos, stream));
+    size_t tb2 = 0, tb3 = 0;
+    RV_HIP(sort_key_pairs(nullptr, tb2, keys, keys2, pos, pos3, (int)n, slot_bits, stream));
+    RV_HIP(launch_max_displacement(pos3, (int)n, cap_log2, nullptr, nullptr, nullptr, nullptr, tb3, stream));
+    void* tmp2 = grow(b_sort_tmp, std::max(tb, std::max(tb2, tb3)));
+    if (!tmp2) return fail(RV_ERR_OOM, "device buffer for the key sort failed");
+    RV_HIP(sort_key_pairs(tmp2, tb2, keys, keys2, pos, pos3, (int)n, slot_bits, stream));
+    // keys / keys2 are free again: two int32 arrays of n each
+    int32_t* v = (int32_t*)keys;
+    int32_t* pm = (int32_t*)keys2;
+    int* md = (int*)(cnt + 1);
+    RV_HIP(launch_max_displacement(pos3, (int)n, cap_log2, v, pm, md, tmp2, tb3, stream));
+    int h = 0;
+    RV_HIP(hipMemcpyAsync(&h, md, sizeof h, hipMemcpyDeviceToHost, stream));
+    RV_HIP(hipStreamSynchronize(stream));
+    max_disp = h;
+    key_slot_ok_ = true;
+    return RV_OK;
+}
+
+rv_status Engine::download_keys(bool slot_order, std::vector<int32_t>& out) {
+    out.clear();
+    if (slot_order && !key_slot_ok_) return fail(RV_ERR_DEVICE, "no slot-ordered keys on the device");
+    if (key_n_ == 0) return RV_OK;
+    out.resize(key_n_);
+    RV_HIP(hipMemcpyAsync(out.data(), slot_order ? b_kpos3.p : b_kpos2.p, sizeof(int32_t) * key_n_, hipMemcpyDeviceToHost,
+                          stream));
     RV_HIP(hipStreamSynchronize(stream));
     return RV_OK;
 }

@@ -103,8 +103,11 @@ class Engine {
     bool no_passing(int start, int end, bool& none);
 
     // nonInsertionVariants keys of the last pileup in creation order, and the same keys in the
-    // slot order they take when sorted (see rv_order.h)
-    rv_status key_order(int n_reads, std::vector<int32_t>& created, std::vector<int32_t>& slot_sorted);
+    // slot order they take when sorted (see rv_order.h), both kept on the device; n_keys = their
+    // count, max_disp = the largest displacement of the sorted layout (INT_MAX: no sorted layout)
+    rv_status key_order(int n_reads, int64_t& n_keys, int& max_disp);
+    // copies those keys to the host: creation order, or slot order
+    rv_status download_keys(bool slot_order, std::vector<int32_t>& out);
 
     // The slot order of a robin_hood<int> table after inserting ops[0, m) in order from empty,
     // each key with tr[i] set erased right after its insertion (distinct keys), computed on the
@@ -124,7 +127,9 @@ class Engine {
                              std::vector<int32_t>& out, bool& applied);
     float slot_order_ms = 0;  // device time of the last slot order's epochs
     // the last key_order's keys in slot order on the device (valid until the next key_order)
-    const int32_t* slot_sorted_dev() const { return (const int32_t*)b_kpos3.p; }
+    const int32_t* slot_sorted_dev() const { return key_slot_ok_ ? (const int32_t*)b_kpos3.p : nullptr; }
+    size_t key_n_ = 0;
+    bool key_slot_ok_ = false;
 
     // the device Fisher test on host tables (n11 n12 n21 n22) -> (two-sided, left, right) per table
     rv_status fisher_tables(const int32_t* tab, int n, double* out);
@@ -155,21 +160,23 @@ class Engine {
         return s;
     }
 
-    // Destroys the given objects on a background thread, so that releasing a region's host working
-    // state (hundreds of thousands of map nodes) overlaps the caller's next steps.  At most one
-    // release runs at a time; the engine's destructor waits for it.
+    // Destroys the given objects on background threads, one per object, so that releasing a
+    // region's host working state (maps of hundreds of thousands of nodes) overlaps the caller's
+    // next steps and finishes before the next region's reductions allocate.  At most one release
+    // runs at a time; the engine's destructor waits for it.
     template <class... T>
     void release_later(T&&... objs) {
-        auto bundle = std::make_unique<std::tuple<std::decay_t<T>...>>(std::forward<T>(objs)...);
         wait_release();
-        reaper_ = std::thread([b = std::move(bundle)]() mutable { b.reset(); });
+        (reapers_.emplace_back([o = std::make_unique<std::decay_t<T>>(std::forward<T>(objs))]() mutable { o.reset(); }),
+         ...);
     }
     void wait_release() {
-        if (reaper_.joinable()) reaper_.join();
+        for (auto& t : reapers_) t.join();
+        reapers_.clear();
     }
 
   private:
-    std::thread reaper_;
+    std::vector<std::thread> reapers_;
     std::vector<std::pair<void*, size_t>> pinned_;
     size_t pinned_cur_ = 0, pinned_off_ = 0;
     DevBuf b_reads[13], b_prep, b_cig, b_ctr, b_span, b_rows, b_rows2, b_ref;

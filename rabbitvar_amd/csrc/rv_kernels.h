@@ -71,6 +71,11 @@ hipError_t so_sort_pairs(void* tmp, size_t& tmp_bytes, const uint32_t* kin, uint
                          uint32_t* vout, int n, int end_bit, hipStream_t s);
 hipError_t so_scan_bytes(int n, size_t& tmp_bytes);
 
+// the largest displacement of n keys laid out in slot order in a table of 2^cap_log2 slots, into
+// *out (which must hold 0 beforehand); v, pm: n entries; temp == nullptr sets temp_bytes only
+hipError_t launch_max_displacement(const int32_t* keys, int n, int cap_log2, int32_t* v, int32_t* pm, int* out,
+                                   void* temp, size_t& temp_bytes, hipStream_t s);
+
 hipError_t launch_score(const ScoreIn& S, hipStream_t s);
 hipError_t launch_fisher_table(const int* tab, int n, double* out, const double* lgt, int lgt_n, hipStream_t s);
 // the allocation-free variant's whole call (host tables in, host p-values out; rv_score.hip)

@@ -63,4 +63,40 @@ hipError_t sort_key_pairs(void* temp, size_t& temp_bytes, const unsigned long lo
     return hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, kin, kout, vin, vout, n, 0, end_bit, s);
 }
 
+// Largest displacement of keys laid out in slot order (sorted by home): a key's slot is its index
+// plus the running max of (home - index), so displacement_i = i + max_{j<=i}(home_j - j) - home_i.
+__global__ void __launch_bounds__(256) k_disp_in(const int32_t* keys, int n, unsigned long long mask, int32_t* v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long x = (unsigned long long)(long long)keys[i];
+    const unsigned long long h = __umul64hi(x, 0xde5fb9d2630458e9ull) + x * 0xde5fb9d2630458e9ull;
+    v[i] = (int32_t)(h & mask) - i;
+}
+
+__global__ void __launch_bounds__(256) k_disp_max(const int32_t* v, const int32_t* pm, int n, int* out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int d = 0;
+    if (i < n) d = pm[i] - v[i];  // (i + pm_i) - home_i = pm_i - (home_i - i)
+    for (int o = 32; o > 0; o >>= 1) {
+        const int w = __shfl_xor(d, o);
+        d = w > d ? w : d;
+    }
+    if ((threadIdx.x & 63) == 0 && d > 0) atomicMax(out, d);
+}
+
+hipError_t launch_max_displacement(const int32_t* keys, int n, int cap_log2, int32_t* v, int32_t* pm, int* out,
+                                   void* temp, size_t& temp_bytes, hipStream_t s) {
+    if (temp == nullptr)
+        return hipcub::DeviceScan::InclusiveScan(nullptr, temp_bytes, v, pm, hipcub::Max(), n, s);
+    if (n <= 0) return hipSuccess;
+    const dim3 g((unsigned)((n + 255) / 256));
+    hipLaunchKernelGGL(k_disp_in, g, dim3(256), 0, s, keys, n, (1ull << cap_log2) - 1ull, v);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = hipcub::DeviceScan::InclusiveScan(temp, temp_bytes, v, pm, hipcub::Max(), n, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_disp_max, g, dim3(256), 0, s, v, pm, n, out);
+    return hipGetLastError();
+}
+
 }  // namespace rv

@@ -173,7 +173,9 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     // Work that overlaps the host reductions inside pileup(): the nonInsertionVariants key order
     // (a device sort) once the dense rows are final, and the realigner's first row batch once the
     // event maps are reduced.
-    std::vector<int32_t> created, walk_sorted;
+    std::vector<int32_t> created, walk_sorted;  // copied from the device only when the host needs them
+    int64_t n_walk_keys = 0;
+    int walk_maxd_dev = INT_MAX;
     rv_status key_status = RV_OK;
     double key_ms = 0;
     std::thread key_th;
@@ -206,7 +208,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     hooks.after_walk = [&] {
         key_th = std::thread([&] {
             const auto k0 = std::chrono::steady_clock::now();
-            key_status = e.key_order((int)std::min<int64_t>(reads.n_reads, INT32_MAX), created, walk_sorted);
+            key_status = e.key_order((int)std::min<int64_t>(reads.n_reads, INT32_MAX), n_walk_keys, walk_maxd_dev);
             key_ms = ms_since(k0);
         });
     };
@@ -228,6 +230,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
         });
     };
     rv_status s = e.pileup(region, reads, contig, false, pr, &hooks);
+    e.stats.timeline_ms[0] = ms_since(w0);
     if (key_th.joinable()) key_th.join();
     if (prefetch_th.joinable()) prefetch_th.join();
     const bool screen_events_done = screen_th.joinable();
@@ -237,6 +240,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     if (!rcp) return e.fail(RV_ERR_DEVICE, "pileup finished without reducing its records");
     if (prefetch_err) std::rethrow_exception(prefetch_err);
     if (!prefetch_ok) return e.fail(RV_ERR_DEVICE, "dense row prefetch failed");
+    e.stats.timeline_ms[1] = ms_since(w0);
     RowCache& rc = *rcp;
     double order_ms = 0;
     e.stats.detail_ms[6] = key_ms;
@@ -249,9 +253,9 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     std::exception_ptr replay_err;
     std::thread replay([&] {
         try {
-            if (walk_sorted.size() == created.size())
-                walk_maxd = rh_max_displacement(walk_sorted.data(), walk_sorted.size(), rh_capacity(walk_sorted.size()));
+            walk_maxd = walk_maxd_dev;
             if (walk_maxd >= RH_OVERFLOW_DISP) {
+                if (e.download_keys(false, created) != RV_OK) throw std::runtime_error(e.err);
                 replay_created(nonins, created);
                 replayed = true;
             }
@@ -303,6 +307,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     e.stats.detail_ms[14] = (double)rst.n_nopass;
     e.stats.detail_ms[15] = (double)rc.miss_fetches;
     const auto c0 = std::chrono::steady_clock::now();
+    e.stats.timeline_ms[2] = ms_since(w0);
     CxHost cx;
     try {
         build_record_list(S, cx);
@@ -349,8 +354,10 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     std::vector<uint8_t> status;
     unsigned long long hc[8];
     float score_ms = 0;
+    e.stats.timeline_ms[3] = ms_since(w0);
     s = score_pass(e, in, n_status, recs, arena, &status, hc, score_ms);
     if (s != RV_OK) return s;
+    e.stats.timeline_ms[4] = ms_since(w0);
     // ---- text, in the printer's order: alignedVariants slot order, then the variants' order.  A
     // second thread works out that order while this one formats the lines in record order; the
     // lines are then joined in printing order.
@@ -373,18 +380,25 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
             // nonInsertionVariants' slot order: the device's sorted walk keys as they are when the
             // realigner changed no key and the sorted form provably holds, else rebuilt
             const std::vector<int32_t>* procp = &proc;
-            if (!replayed && S.key_log.empty() && walk_maxd < RH_OVERFLOW_DISP) {
-                procp = &walk_sorted;
-            } else if (replayed || !noninsertion_order_sorted(walk_sorted, walk_maxd, S.key_log, proc)) {
-                if (!replayed) replay_created(nonins, created);
-                proc = replay_finish(nonins, S.key_log);
+            const bool direct = !replayed && S.key_log.empty() && walk_maxd < RH_OVERFLOW_DISP;
+            if (!direct) {
+                if (!replayed && e.download_keys(true, walk_sorted) != RV_OK) throw std::runtime_error(e.err);
+                if (replayed || !noninsertion_order_sorted(walk_sorted, walk_maxd, S.key_log, proc)) {
+                    if (!replayed) {
+                        if (e.download_keys(false, created) != RV_OK) throw std::runtime_error(e.err);
+                        replay_created(nonins, created);
+                    }
+                    proc = replay_finish(nonins, S.key_log);
+                }
             }
             // alignedVariants: its operations listed on the device from the processing order and
             // the scoring status (record-list positions override their dense rows), its slot order
             // there too; else the per-growth sort on the host, else the replay
             const auto l0 = std::chrono::steady_clock::now();
-            const int32_t* proc_dev = procp == &walk_sorted ? e.slot_sorted_dev() : nullptr;
-            if (!proc_dev && !procp->empty()) {
+            const int32_t* proc_dev = direct ? e.slot_sorted_dev() : nullptr;
+            const size_t n_proc = direct ? (size_t)n_walk_keys : procp->size();
+            if (direct && !proc_dev && n_proc) throw std::runtime_error("walk keys missing on the device");
+            if (!direct && !procp->empty()) {
                 auto* d = (int32_t*)e.grow(e.b_so_proc, 4 * procp->size());
                 if (!d || hipMemcpyAsync(d, procp->data(), 4 * procp->size(), hipMemcpyHostToDevice, e.stream) != hipSuccess)
                     throw std::runtime_error("upload of the processing order failed");
@@ -393,7 +407,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
             std::vector<int32_t> aops;
             std::vector<uint8_t> atr;
             bool on_device = false;
-            if (e.aligned_slot_order(proc_dev, procp->size(), in.status, pr.P.dense_lo, pr.P.dense_n, in.cx_pos, ncx,
+            if (e.aligned_slot_order(proc_dev, n_proc, in.status, pr.P.dense_lo, pr.P.dense_n, in.cx_pos, ncx,
                                      aligned, on_device, aops, atr) != RV_OK)
                 throw std::runtime_error(e.err);
             e.stats.order_ops = (int64_t)atr.size();
@@ -461,7 +475,9 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
         for (auto& th : pool) th.join();
     }
     const double format_ms = ms_since(f0);
+    e.stats.timeline_ms[5] = ms_since(w0);
     order_th.join();
+    e.stats.timeline_ms[6] = ms_since(w0);
     if (order_err) {
         try {
             std::rethrow_exception(order_err);
@@ -540,6 +556,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     e.stats.order_ms = order_ms;
     e.stats.format_ms = format_ms;
     e.stats.wall_ms = ms_since(w0);
+    e.stats.timeline_ms[7] = e.stats.wall_ms;
     e.stats.realign_device_pairs = e.f35_device_pairs;
     e.stats.realign_device_ms = e.f35_ms;
     e.stats.score_ms = score_ms;

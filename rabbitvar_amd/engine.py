@@ -228,6 +228,7 @@ class Engine:
         self.lib.rv_get_stats(self.h, ctypes.byref(s))
         out = {k: getattr(s, k) for k, _ in s._fields_}
         out["detail_ms"] = list(s.detail_ms)
+        out["timeline_ms"] = list(s.timeline_ms)
         return out
 
 
