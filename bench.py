@@ -331,8 +331,9 @@ def main():
         "aligned_order_split": {"operations": st["order_ops"], "list_ms": round(st["order_ops_ms"], 3),
                                 "device_ms": round(st["order_device_ms"], 3)},
         "timeline_ms_last_step": dict(zip(["pileup", "side_threads", "realign", "record_list", "score", "format",
-                                           "order", "end", "p_ref", "p_prep", "p_walk", "p_copy", "p_wait_release",
-                                           "p_reduce"], [round(x, 2) for x in st["timeline_ms"]])),
+                                           "order", "released", "p_ref", "p_prep", "p_walk", "p_copy", "p_wait_release",
+                                           "p_reduce", "line_offsets", "lines_and_calls"],
+                                          [round(x, 2) for x in st["timeline_ms"]])),
         "pileup_records_bytes": st["event_bytes"],
         "roofline": {"kernel": "k_walk", "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,

@@ -174,10 +174,13 @@ typedef struct rv_stats {
     int64_t order_ops;            /* those operations */
     double timeline_ms[16];       /* simple mode, from the call's start: [0-7] pileup returned, its side
                                      threads joined, realignment done, record list built, scoring done,
-                                     lines formatted, output order known, the call's end; inside the
+                                     lines formatted, output order known, the region's state handed to the
+                                     release threads; inside the
                                      pileup, from its start: [8] reference window on the device, [9]
                                      prep done, [10] walk and anchors done, [11] records copied back,
-                                     [12] previous region's release waited for, [13] reductions done */
+                                     [12] previous region's release waited for, [13] reductions done;
+                                     from the call's start: [14] line offsets in printing order, [15]
+                                     lines joined and call records built */
 } rv_stats;
 
 void rv_config_default(rv_config* cfg);

@@ -674,14 +674,16 @@ rv_status Engine::slot_order_dev(const int32_t* ops_d, const uint8_t* tr_d, cons
     SoRuns R{runs, runs + max_keep, runs + 2 * max_keep, runs + 3 * max_keep, runs + 4 * max_keep,
              runs + 5 * max_keep, runs + 6 * max_keep, (int*)(small + 7)};
     RV_HIP(hipEventRecord(so_ev[0], stream));
-    // small: [0, 5) first steps, [5] a run too long to simulate, [6] a home group too large, [7] runs
+    // small: [0, 5) first steps, [5] a run too long to simulate, [6] a home group too large, [7] runs,
+    // [8] an early growth, [9, 14) the halving steps; the flags stay set across epochs and are read
+    // once at the end (no host synchronisation between epochs)
     RV_HIP(hipMemsetAsync(small, 0, 64, stream));
+    SoHalve* H = (SoHalve*)(small + 9);
     int cur = 0;
     for (const Ep& ep : eps) {
         const int lcap = rh_log2(ep.cap);
         SoEpoch E{ord[cur], (int)ep.np, ops_d + ep.a, tr_d + ep.a, (int)ep.S, (int)ep.keep, (uint32_t)(ep.cap - 1), lcap};
         RV_HIP(hipMemsetAsync(small, 0xff, 5 * sizeof(unsigned), stream));
-        RV_HIP(hipMemsetAsync(small + 5, 0, sizeof(unsigned), stream));
         RV_HIP(hipMemsetAsync(small + 7, 0, sizeof(unsigned), stream));
         RV_HIP(launch_so_epoch_keys(E, k[0], k[1], stream));
         size_t tb = tmp_bytes;
@@ -689,25 +691,19 @@ rv_status Engine::slot_order_dev(const int32_t* ops_d, const uint8_t* tr_d, cons
         RV_HIP(launch_so_runs(k[2], (int)ep.keep, (int)ep.S, (uint32_t)(ep.cap - 1), w[0], w[1], w[2], w[3], R, tmp,
                               tmp_bytes, stream));
         RV_HIP(launch_so_simulate(k[2], k[3], (uint32_t)(ep.cap - 1), R, (int)ep.keep, small, small + 5, stream));
-        unsigned h[6];
-        RV_HIP(hipMemcpyAsync(h, small, sizeof h, hipMemcpyDeviceToHost, stream));
-        RV_HIP(hipStreamSynchronize(stream));
-        if (h[5] || h[4] != 0xffffffffu) return RV_OK;  // too long to simulate, or an early growth
-        SoHalve H{0, {0, 0, 0, 0}};
-        for (int l = 0; l < 4; l++)
-            if (h[l] != 0xffffffffu && (size_t)h[l] + 1 < ep.S) H.at[H.nh++] = h[l] + 1;
-        RV_HIP(launch_so_final(k[2], k[3], E, H, ord[cur ^ 1], small + 6, stream));
+        RV_HIP(launch_so_final(k[2], k[3], E, small, H, ord[cur ^ 1], small + 6, small + 8, stream));
         cur ^= 1;
     }
     const size_t n_out = eps.back().keep;
     out.resize(n_out);
-    unsigned big = 0;
+    unsigned flags[4] = {0, 0, 0, 0};
     RV_HIP(hipMemcpyAsync(out.data(), ord[cur], 4 * n_out, hipMemcpyDeviceToHost, stream));
-    RV_HIP(hipMemcpyAsync(&big, small + 6, sizeof big, hipMemcpyDeviceToHost, stream));
+    RV_HIP(hipMemcpyAsync(flags, small + 5, sizeof flags, hipMemcpyDeviceToHost, stream));
     RV_HIP(hipEventRecord(so_ev[1], stream));
     RV_HIP(hipStreamSynchronize(stream));
     RV_HIP(hipEventElapsedTime(&slot_order_ms, so_ev[0], so_ev[1]));
-    if (big) {
+    // a run too long to simulate, a home group too large, or an early growth: the host orders
+    if (flags[0] || flags[1] || flags[3]) {
         out.clear();
         return RV_OK;
     }

@@ -65,8 +65,9 @@ hipError_t launch_so_runs(const uint32_t* skey, int n_keep, int S, uint32_t mask
                           int32_t* flag, int32_t* rid, const SoRuns& R, void* tmp, size_t tmp_bytes, hipStream_t s);
 hipError_t launch_so_simulate(const uint32_t* skey, const uint32_t* sval, uint32_t mask, const SoRuns& R, int max_runs,
                               unsigned* first, unsigned* too_big, hipStream_t s);
-hipError_t launch_so_final(const uint32_t* skey, const uint32_t* sval, const SoEpoch& E, const SoHalve& H, int32_t* out,
-                           unsigned* too_big, hipStream_t s);
+// the halving steps from `first` (into *H, on the device), then each key's slot order into out
+hipError_t launch_so_final(const uint32_t* skey, const uint32_t* sval, const SoEpoch& E, const unsigned* first, SoHalve* H,
+                           int32_t* out, unsigned* too_big, unsigned* declined, hipStream_t s);
 hipError_t so_sort_pairs(void* tmp, size_t& tmp_bytes, const uint32_t* kin, uint32_t* kout, const uint32_t* vin,
                          uint32_t* vout, int n, int end_bit, hipStream_t s);
 hipError_t so_scan_bytes(int n, size_t& tmp_bytes);

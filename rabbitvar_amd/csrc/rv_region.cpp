@@ -476,6 +476,16 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     }
     const double format_ms = ms_since(f0);
     e.stats.timeline_ms[5] = ms_since(w0);
+    // while the order thread finishes: the output's buffers, whose sizes are known now
+    out.text.clear();
+    out.calls.clear();
+    out.strings.clear();
+    {
+        size_t total = 0;
+        for (auto& pt : parts) total += pt.size();
+        out.text.resize(total);
+        out.calls.resize(n);
+    }
     order_th.join();
     e.stats.timeline_ms[6] = ms_since(w0);
     if (order_err) {
@@ -496,67 +506,70 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
         if (i + 1 < b) end = (size_t)(line_at[i + 1] & ((1ull << 40) - 1));
         return std::make_pair(parts[t].data() + off, end - off);
     };
-    out.text.clear();
-    out.calls.clear();
-    out.strings.clear();
+    // each line's offset in the text (printing order), then lines and call records in parallel; the
+    // records' strings stay where formatting put them (the vector moves, its elements do not)
+    std::vector<size_t> text_at(n + 1, 0);
+    for (size_t k = 0; k < n; k++) text_at[k + 1] = text_at[k] + line_len(perm[k]).second;
+    if (text_at[n] != out.text.size()) return e.fail(RV_ERR_DEVICE, "output lines do not add up to the text");
+    out.strings = std::move(strs);
+    auto assemble = [&](size_t a, size_t b) {
+        char* txt = n ? &out.text[0] : nullptr;
+        for (size_t k = a; k < b; k++) {
+            const size_t i = perm[k];
+            const auto ln = line_len(i);
+            std::memcpy(txt + text_at[k], ln.first, ln.second);
+            const OutRec& o = recs[i];
+            const CallStrings& cs = out.strings[i];
+            rv_call& c = out.calls[k];
+            c.position = o.position;
+            c.start = o.start;
+            c.end = o.end;
+            c.total_pos_coverage = o.tpc;
+            c.position_coverage = o.poscov;
+            c.ref_fwd = o.rfc;
+            c.ref_rev = o.rrc;
+            c.var_fwd = o.fwd;
+            c.var_rev = o.rev;
+            c.shift3 = o.shift3;
+            c.msint = o.msint;
+            c.hicnt = o.hicnt;
+            c.hicov = o.hicov;
+            c.frequency = o.freq;
+            c.mean_position = o.pmean;
+            c.mean_quality = o.qmean;
+            c.mean_mapq = o.mq;
+            c.hi_lo_ratio = o.hlr;
+            c.hi_freq = o.hifreq;
+            c.extra_freq = o.extrafreq;
+            c.msi = o.msi;
+            c.nm = o.nm;
+            c.fisher_p = o.fisher_p;
+            c.fisher_ratio = o.fisher_ratio;
+            c.pstd = o.pstd;
+            c.qstd = o.qstd;
+            c.refallele = cs.refallele.c_str();
+            c.varallele = cs.varallele.c_str();
+            c.genotype = cs.genotype.c_str();
+            c.bias = cs.bias.c_str();
+            c.leftseq = cs.leftseq.c_str();
+            c.rightseq = cs.rightseq.c_str();
+            c.vartype = cs.vartype.c_str();
+        }
+    };
+    e.stats.timeline_ms[14] = ms_since(w0);
     {
-        size_t total = 0;
-        for (auto& pt : parts) total += pt.size();
-        out.text.reserve(total);
-    }
-    out.strings.resize(n);
-    out.calls.reserve(n);
-    for (size_t k = 0; k < n; k++) {
-        const size_t i = perm[k];
-        const auto ln = line_len(i);
-        out.text.append(ln.first, ln.second);
-        out.strings[k] = std::move(strs[i]);
-    }
-    for (size_t k = 0; k < n; k++) {
-        const OutRec& o = recs[perm[k]];
-        const CallStrings& cs = out.strings[k];
-        rv_call c;
-        c.position = o.position;
-        c.start = o.start;
-        c.end = o.end;
-        c.total_pos_coverage = o.tpc;
-        c.position_coverage = o.poscov;
-        c.ref_fwd = o.rfc;
-        c.ref_rev = o.rrc;
-        c.var_fwd = o.fwd;
-        c.var_rev = o.rev;
-        c.shift3 = o.shift3;
-        c.msint = o.msint;
-        c.hicnt = o.hicnt;
-        c.hicov = o.hicov;
-        c.frequency = o.freq;
-        c.mean_position = o.pmean;
-        c.mean_quality = o.qmean;
-        c.mean_mapq = o.mq;
-        c.hi_lo_ratio = o.hlr;
-        c.hi_freq = o.hifreq;
-        c.extra_freq = o.extrafreq;
-        c.msi = o.msi;
-        c.nm = o.nm;
-        c.fisher_p = o.fisher_p;
-        c.fisher_ratio = o.fisher_ratio;
-        c.pstd = o.pstd;
-        c.qstd = o.qstd;
-        c.refallele = cs.refallele.c_str();
-        c.varallele = cs.varallele.c_str();
-        c.genotype = cs.genotype.c_str();
-        c.bias = cs.bias.c_str();
-        c.leftseq = cs.leftseq.c_str();
-        c.rightseq = cs.rightseq.c_str();
-        c.vartype = cs.vartype.c_str();
-        out.calls.push_back(c);
+        const size_t na = n < 4096 ? 1 : std::min<size_t>(std::min(hw, 16u), n / 2048);
+        std::vector<std::thread> pool;
+        for (size_t t = 1; t < na; t++) pool.emplace_back(assemble, n * t / na, n * (t + 1) / na);
+        assemble(0, n / na);
+        for (auto& th : pool) th.join();
     }
     host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     e.stats.realign_ms = realign_ms;
     e.stats.order_ms = order_ms;
     e.stats.format_ms = format_ms;
+    e.stats.timeline_ms[15] = ms_since(w0);
     e.stats.wall_ms = ms_since(w0);
-    e.stats.timeline_ms[7] = e.stats.wall_ms;
     e.stats.realign_device_pairs = e.f35_device_pairs;
     e.stats.realign_device_ms = e.f35_ms;
     e.stats.score_ms = score_ms;
@@ -572,6 +585,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
                     std::move(walk_sorted), std::move(nonins), std::move(recs), std::move(arena), std::move(status),
                     std::move(proc), std::move(aligned), std::move(parts), std::move(line_at), std::move(strs),
                     std::move(perm));
+    e.stats.timeline_ms[7] = ms_since(w0);
     return RV_OK;
 }
 

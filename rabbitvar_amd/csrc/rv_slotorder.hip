@@ -219,12 +219,25 @@ __global__ void __launch_bounds__(64) k_so_simulate(const uint32_t* skey, const 
     }
 }
 
+// The halving steps of an epoch from the first steps reaching each displacement: a flag raised at
+// step t halves the increment before step t + 1 (when there is one).  Reaching the last level's
+// displacement (the table would grow early) sets *declined.
+__global__ void k_so_halve(const unsigned* first, int S, SoHalve* H, unsigned* declined) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    int nh = 0;
+    for (int l = 0; l < 4; l++)
+        if (first[l] != 0xffffffffu && first[l] + 1u < (unsigned)S) H->at[nh++] = first[l] + 1u;
+    H->nh = nh;
+    if (first[4] != 0xffffffffu) *declined = 1u;
+}
+
 // Each kept key's place within its home group: ranked by the level key then the step; groups of
 // more than SO_GROUP_MAX keys flag too_big.
-__global__ void __launch_bounds__(256) k_so_final(const uint32_t* skey, const uint32_t* sval, SoEpoch E, SoHalve H,
-                                                   int32_t* out, unsigned* too_big) {
+__global__ void __launch_bounds__(256) k_so_final(const uint32_t* skey, const uint32_t* sval, SoEpoch E,
+                                                   const SoHalve* Hp, int32_t* out, unsigned* too_big) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= E.n_keep) return;
+    const SoHalve H = *Hp;
     const uint32_t home = skey[i];
     int g0 = i;
     while (g0 > 0 && skey[g0 - 1] == home && i - g0 < SO_GROUP_MAX) g0--;
@@ -393,11 +406,13 @@ hipError_t launch_so_simulate(const uint32_t* skey, const uint32_t* sval, uint32
     return hipGetLastError();
 }
 
-hipError_t launch_so_final(const uint32_t* skey, const uint32_t* sval, const SoEpoch& E, const SoHalve& H, int32_t* out,
-                           unsigned* too_big, hipStream_t s) {
-    if (E.n_keep <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_so_final, dim3((unsigned)((E.n_keep + 255) / 256)), dim3(256), 0, s, skey, sval, E, H, out,
-                       too_big);
+hipError_t launch_so_final(const uint32_t* skey, const uint32_t* sval, const SoEpoch& E, const unsigned* first, SoHalve* H,
+                           int32_t* out, unsigned* too_big, unsigned* declined, hipStream_t s) {
+    hipLaunchKernelGGL(k_so_halve, dim3(1), dim3(64), 0, s, first, E.S, H, declined);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess || E.n_keep <= 0) return e;
+    hipLaunchKernelGGL(k_so_final, dim3((unsigned)((E.n_keep + 255) / 256)), dim3(256), 0, s, skey, sval, E,
+                       (const SoHalve*)H, out, too_big);
     return hipGetLastError();
 }
 
