@@ -73,15 +73,28 @@ __global__ void __launch_bounds__(256) k_disp_in(const int32_t* keys, int n, uns
     v[i] = (int32_t)(h & mask) - i;
 }
 
+// each block reduces 8 keys per lane, then one atomic per block, skipped when it cannot raise *out
 __global__ void __launch_bounds__(256) k_disp_max(const int32_t* v, const int32_t* pm, int n, int* out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ int wmax[4];
     int d = 0;
-    if (i < n) d = pm[i] - v[i];  // (i + pm_i) - home_i = pm_i - (home_i - i)
+    for (int j = 0; j < 8; j++) {
+        const int i = (blockIdx.x * 8 + j) * 256 + (int)threadIdx.x;
+        if (i < n) {
+            const int x = pm[i] - v[i];  // (i + pm_i) - home_i = pm_i - (home_i - i)
+            d = x > d ? x : d;
+        }
+    }
     for (int o = 32; o > 0; o >>= 1) {
         const int w = __shfl_xor(d, o);
         d = w > d ? w : d;
     }
-    if ((threadIdx.x & 63) == 0 && d > 0) atomicMax(out, d);
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = d;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int m = wmax[0];
+        for (int w = 1; w < 4; w++) m = wmax[w] > m ? wmax[w] : m;
+        if (m > *(volatile int*)out) atomicMax(out, m);
+    }
 }
 
 hipError_t launch_max_displacement(const int32_t* keys, int n, int cap_log2, int32_t* v, int32_t* pm, int* out,
@@ -95,7 +108,7 @@ hipError_t launch_max_displacement(const int32_t* keys, int n, int cap_log2, int
     if (e != hipSuccess) return e;
     e = hipcub::DeviceScan::InclusiveScan(temp, temp_bytes, v, pm, hipcub::Max(), n, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_disp_max, g, dim3(256), 0, s, v, pm, n, out);
+    hipLaunchKernelGGL(k_disp_max, dim3((unsigned)((n + 2047) / 2048)), dim3(256), 0, s, v, pm, n, out);
     return hipGetLastError();
 }
 

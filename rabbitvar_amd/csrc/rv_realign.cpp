@@ -107,9 +107,17 @@ char HostRef::get(int k) {
 
 // ------------------------------------------------------------------ position store
 RState::RState(HostPileup& P_, RowCache& rc) : P(P_), rows(rc) {
+    // the three position indexes are independent: the clip maps' on their own threads
+    std::thread t5([this] { sc5_ix.build(P.sc5); });
+    std::thread t3([this] { sc3_ix.build(P.sc3); });
+    struct Join {
+        std::thread &a, &b;
+        ~Join() {
+            a.join();
+            b.join();
+        }
+    } join{t5, t3};
     nd_ix.build(P.nd);
-    sc5_ix.build(P.sc5);
-    sc3_ix.build(P.sc3);
     for (auto& e : P.ins)
         for (auto& kv : e.second) {
             HVar* v = new_var();
