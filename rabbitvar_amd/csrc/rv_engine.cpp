@@ -93,7 +93,7 @@ Engine::~Engine() {
                      &b_fisher_in, &b_fisher_out, &b_lgamma, &b_f35_pairs, &b_f35_arena, &b_f35_out, &b_pick,
                      &b_so_ops, &b_so_tr, &b_so_ord[0], &b_so_ord[1], &b_so_k[0], &b_so_k[1], &b_so_k[2], &b_so_k[3],
                      &b_so_w[0], &b_so_w[1], &b_so_w[2], &b_so_w[3], &b_so_runs, &b_so_small, &b_so_tmp,
-                     &b_so_proc};
+                     &b_so_proc, &b_so_rank, &b_so_rk[0], &b_so_rk[1], &b_so_rk[2], &b_so_rk[3]};
     // teardown: nothing to report a failure to
     for (auto* b : all)
         if (b->p) (void)hipFree(b->p);
@@ -589,7 +589,7 @@ rv_status Engine::slot_order(const int32_t* ops, const uint8_t* tr, size_t m, st
 
 rv_status Engine::aligned_slot_order(const int32_t* proc_dev, size_t n_proc, uint8_t* status_dev, int dense_lo,
                                      int dense_n, const int32_t* cx_pos_dev, int ncx, std::vector<int32_t>& out,
-                                     bool& applied, std::vector<int32_t>& ops, std::vector<uint8_t>& tr) {
+                                     bool& applied, std::vector<int32_t>& ops, std::vector<uint8_t>& tr, bool download) {
     applied = false;
     out.clear();
     ops.clear();
@@ -614,7 +614,7 @@ rv_status Engine::aligned_slot_order(const int32_t* proc_dev, size_t n_proc, uin
         RV_HIP(hipMemcpyAsync(tr.data(), tr_d, (size_t)m, hipMemcpyDeviceToHost, stream));
         RV_HIP(hipStreamSynchronize(stream));
     }
-    const rv_status st = slot_order_dev(ops_d, tr_d, tr.data(), (size_t)m, out, applied);
+    const rv_status st = slot_order_dev(ops_d, tr_d, tr.data(), (size_t)m, out, applied, download);
     if (st != RV_OK || applied) return st;
     // declined: the caller orders on the host
     ops.resize((size_t)m);
@@ -625,13 +625,66 @@ rv_status Engine::aligned_slot_order(const int32_t* proc_dev, size_t n_proc, uin
     return RV_OK;
 }
 
+rv_status Engine::download_slot_order(std::vector<int32_t>& out) {
+    out.clear();
+    if (!so_valid_) return fail(RV_ERR_DEVICE, "no slot order on the device");
+    out.resize(so_n_);
+    if (so_n_) {
+        RV_HIP(hipMemcpyAsync(out.data(), so_final_, 4 * so_n_, hipMemcpyDeviceToHost, stream));
+        RV_HIP(hipStreamSynchronize(stream));
+    }
+    return RV_OK;
+}
+
+rv_status Engine::record_order(const OutRec* recs_dev, int n, int dense_lo, int dense_n, std::vector<uint32_t>& perm,
+                               bool& ok) {
+    ok = false;
+    perm.clear();
+    if (!so_valid_ || n < 0) return RV_OK;
+    if (n == 0) {
+        ok = true;
+        return RV_OK;
+    }
+    auto* rank_of = (int32_t*)grow(b_so_rank, sizeof(int32_t) * (size_t)std::max(dense_n, 1));
+    auto* key = (unsigned long long*)grow(b_so_rk[0], 8 * (size_t)n);
+    auto* key2 = (unsigned long long*)grow(b_so_rk[1], 8 * (size_t)n);
+    auto* idx = (int32_t*)grow(b_so_rk[2], 4 * (size_t)n);
+    auto* idx2 = (int32_t*)grow(b_so_rk[3], 4 * (size_t)n);
+    auto* bad = (unsigned*)grow(b_so_small, 64) + 15;
+    if (!rank_of || !key || !key2 || !idx || !idx2) return fail(RV_ERR_OOM, "device buffers for the record order failed");
+    size_t tb = 0;
+    RV_HIP(sort_key_pairs(nullptr, tb, key, key2, idx, idx2, n, 64, stream));
+    void* tmp = grow(b_so_tmp, tb);
+    if (!tmp) return fail(RV_ERR_OOM, "device buffer for the record order failed");
+    RV_HIP(hipMemsetAsync(bad, 0, sizeof(unsigned), stream));
+    const long long pos_base = (long long)dense_lo - (1ll << 28);
+    RV_HIP(launch_so_rank(so_final_, (int)so_n_, recs_dev, n, dense_lo, dense_n, pos_base, rank_of, key, idx, bad,
+                          stream));
+    RV_HIP(sort_key_pairs(tmp, tb, key, key2, idx, idx2, n, 64, stream));
+    perm.resize((size_t)n);
+    unsigned hbad = 0;
+    RV_HIP(hipMemcpyAsync(perm.data(), idx2, 4 * (size_t)n, hipMemcpyDeviceToHost, stream));
+    RV_HIP(hipMemcpyAsync(&hbad, bad, sizeof hbad, hipMemcpyDeviceToHost, stream));
+    RV_HIP(hipStreamSynchronize(stream));
+    if (hbad) {
+        perm.clear();
+        return RV_OK;
+    }
+    ok = true;
+    return RV_OK;
+}
+
 rv_status Engine::slot_order_dev(const int32_t* ops_d, const uint8_t* tr_d, const uint8_t* tr, size_t m,
-                                 std::vector<int32_t>& out, bool& applied) {
+                                 std::vector<int32_t>& out, bool& applied, bool download) {
     applied = false;
     out.clear();
     slot_order_ms = 0;
+    so_valid_ = false;
     if (m == 0) {
         applied = true;
+        so_final_ = nullptr;
+        so_n_ = 0;
+        so_valid_ = true;
         return RV_OK;
     }
     // growth schedule: an insertion that finds 80% of the slots used doubles the table first
@@ -695,9 +748,11 @@ rv_status Engine::slot_order_dev(const int32_t* ops_d, const uint8_t* tr_d, cons
         cur ^= 1;
     }
     const size_t n_out = eps.back().keep;
-    out.resize(n_out);
     unsigned flags[4] = {0, 0, 0, 0};
-    RV_HIP(hipMemcpyAsync(out.data(), ord[cur], 4 * n_out, hipMemcpyDeviceToHost, stream));
+    if (download) {
+        out.resize(n_out);
+        RV_HIP(hipMemcpyAsync(out.data(), ord[cur], 4 * n_out, hipMemcpyDeviceToHost, stream));
+    }
     RV_HIP(hipMemcpyAsync(flags, small + 5, sizeof flags, hipMemcpyDeviceToHost, stream));
     RV_HIP(hipEventRecord(so_ev[1], stream));
     RV_HIP(hipStreamSynchronize(stream));
@@ -708,6 +763,9 @@ rv_status Engine::slot_order_dev(const int32_t* ops_d, const uint8_t* tr_d, cons
         return RV_OK;
     }
     applied = true;
+    so_final_ = ord[cur];
+    so_n_ = n_out;
+    so_valid_ = true;
     return RV_OK;
 }
 

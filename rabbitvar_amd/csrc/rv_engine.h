@@ -124,10 +124,22 @@ class Engine {
     // the host; otherwise tr holds the transient flags only.
     rv_status aligned_slot_order(const int32_t* proc_dev, size_t n_proc, uint8_t* status_dev, int dense_lo, int dense_n,
                                  const int32_t* cx_pos_dev, int ncx, std::vector<int32_t>& out, bool& applied,
-                                 std::vector<int32_t>& ops, std::vector<uint8_t>& tr);
-    // the epochs over operations already on the device (tr: the same flags on the host)
+                                 std::vector<int32_t>& ops, std::vector<uint8_t>& tr, bool download = true);
+    // the epochs over operations already on the device (tr: the same flags on the host); with
+    // download unset the order stays on the device for record_order
     rv_status slot_order_dev(const int32_t* ops_d, const uint8_t* tr_d, const uint8_t* tr, size_t m,
-                             std::vector<int32_t>& out, bool& applied);
+                             std::vector<int32_t>& out, bool& applied, bool download = true);
+    // The printer's record order from the last applied slot order on the device: the n scored
+    // records (recs_dev, as scoring wrote them) by (their position's index in that order, records
+    // of positions without a key last; position; allele order).  ok = false when there is no such
+    // order or a record's fields do not fit the sort key (the caller sorts on the host).
+    rv_status record_order(const OutRec* recs_dev, int n, int dense_lo, int dense_n, std::vector<uint32_t>& perm,
+                           bool& ok);
+    // copies the last applied slot order to the host
+    rv_status download_slot_order(std::vector<int32_t>& out);
+    const int32_t* so_final_ = nullptr;  // the last applied slot order (device) and its length
+    size_t so_n_ = 0;
+    bool so_valid_ = false;
     float slot_order_ms = 0;  // device time of the last slot order's epochs
     // the last key_order's keys in slot order on the device (valid until the next key_order)
     const int32_t* slot_sorted_dev() const { return key_slot_ok_ ? (const int32_t*)b_kpos3.p : nullptr; }
@@ -150,7 +162,7 @@ class Engine {
         b_score_ctr, b_status, b_keys, b_keys2, b_kpos, b_kpos2, b_sort_tmp, b_kcount, b_kpos3, b_args, b_nopass,
         b_want, b_dup, b_drop, b_scr_cand, b_scr_pos, b_scr_cum, b_scr_feat, b_scr_score, b_fisher_in, b_fisher_out,
         b_lgamma, b_f35_pairs, b_f35_arena, b_f35_out, b_pick, b_so_ops, b_so_tr, b_so_ord[2], b_so_k[4], b_so_w[4],
-        b_so_runs, b_so_small, b_so_tmp, b_so_proc;
+        b_so_runs, b_so_small, b_so_tmp, b_so_proc, b_so_rank, b_so_rk[4];
     WalkArgs* args_dev = nullptr;  // the last pileup's kernel arguments (its reads stay resident)
     int64_t args_n = 0;
     hipEvent_t sev[2] = {nullptr, nullptr};

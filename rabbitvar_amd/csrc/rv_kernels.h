@@ -60,6 +60,12 @@ hipError_t launch_so_ops(const int32_t* proc, int n, uint8_t* status, int dense_
                          hipStream_t s);
 hipError_t so_ops_scan_bytes(int nb, size_t& tmp_bytes);
 hipError_t launch_so_epoch_keys(const SoEpoch& E, uint32_t* skey, uint32_t* sval, hipStream_t s);
+// record sort keys for the printer's order: (index of the record's position in `order`, or last;
+// position - pos_base; allele order), with idx = record index; *bad set when a field does not fit
+// (rank_of: dense_n entries, overwritten)
+hipError_t launch_so_rank(const int32_t* order, int n_keep, const OutRec* recs, int n, int dense_lo, int dense_n,
+                          long long pos_base, int32_t* rank_of, unsigned long long* key, int32_t* idx, unsigned* bad,
+                          hipStream_t s);
 // slots, runs (maxd, transient ranges) of the sorted steps
 hipError_t launch_so_runs(const uint32_t* skey, int n_keep, int S, uint32_t mask, int32_t* v, int32_t* pm,
                           int32_t* flag, int32_t* rid, const SoRuns& R, void* tmp, size_t tmp_bytes, hipStream_t s);

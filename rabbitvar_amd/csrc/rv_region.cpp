@@ -355,7 +355,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     unsigned long long hc[8];
     float score_ms = 0;
     e.stats.timeline_ms[3] = ms_since(w0);
-    s = score_pass(e, in, n_status, recs, arena, &status, hc, score_ms);
+    s = score_pass(e, in, n_status, recs, arena, nullptr, hc, score_ms);  // the status stays on the device
     if (s != RV_OK) return s;
     e.stats.timeline_ms[4] = ms_since(w0);
     // ---- text, in the printer's order: alignedVariants slot order, then the variants' order.  A
@@ -408,7 +408,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
             std::vector<uint8_t> atr;
             bool on_device = false;
             if (e.aligned_slot_order(proc_dev, n_proc, in.status, pr.P.dense_lo, pr.P.dense_n, in.cx_pos, ncx,
-                                     aligned, on_device, aops, atr) != RV_OK)
+                                     aligned, on_device, aops, atr, false) != RV_OK)
                 throw std::runtime_error(e.err);
             e.stats.order_ops = (int64_t)atr.size();
             e.stats.order_device_ms = e.slot_order_ms;
@@ -421,15 +421,28 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
                 aligned = replay_ops(aops, atr);
             const auto q1 = std::chrono::steady_clock::now();
             e.stats.detail_ms[8] = std::chrono::duration<double, std::milli>(q1 - q0).count();
-            // iteration order is slot order: a record's rank is its start position's index there
-            std::vector<int64_t> slot(n, INT64_MAX);
-            slot_ranks(aligned, recs, slot, hw);
-            for (size_t i = 0; i < n; i++) perm[i] = (uint32_t)i;
-            std::sort(perm.begin(), perm.end(), [&](uint32_t x, uint32_t y) {
-                if (slot[x] != slot[y]) return slot[x] < slot[y];
-                if (recs[x].position != recs[y].position) return recs[x].position < recs[y].position;
-                return recs[x].order < recs[y].order;
-            });
+            // iteration order is slot order: a record's rank is its start position's index there.
+            // With the order on the device, the records (still in HBM as scoring wrote them) are
+            // ranked and sorted there; otherwise on the host.
+            bool perm_done = false;
+            if (on_device) {
+                std::vector<uint32_t> dp;
+                if (e.record_order(in.out, (int)n, pr.P.dense_lo, pr.P.dense_n, dp, perm_done) != RV_OK)
+                    throw std::runtime_error(e.err);
+                if (perm_done) perm.swap(dp);
+                else if (e.download_slot_order(aligned) != RV_OK)
+                    throw std::runtime_error(e.err);
+            }
+            if (!perm_done) {
+                std::vector<int64_t> slot(n, INT64_MAX);
+                slot_ranks(aligned, recs, slot, hw);
+                for (size_t i = 0; i < n; i++) perm[i] = (uint32_t)i;
+                std::sort(perm.begin(), perm.end(), [&](uint32_t x, uint32_t y) {
+                    if (slot[x] != slot[y]) return slot[x] < slot[y];
+                    if (recs[x].position != recs[y].position) return recs[x].position < recs[y].position;
+                    return recs[x].order < recs[y].order;
+                });
+            }
             e.stats.detail_ms[9] = ms_since(q1);
             order_thread_ms = ms_since(q0);
         } catch (...) {

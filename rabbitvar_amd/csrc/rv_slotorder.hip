@@ -367,6 +367,50 @@ hipError_t so_ops_scan_bytes(int nb, size_t& tmp_bytes) {
                                             (hipStream_t)0);
 }
 
+// ---- the printer's record order: records by (their position's index in the slot order, position,
+// allele order); positions without an alignedVariants key sort after all others
+
+__global__ void __launch_bounds__(256) k_so_rank_scatter(const int32_t* order, int n_keep, int dense_lo, int dense_n,
+                                                          int32_t* rank_of) {
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r >= n_keep) return;
+    const long long i = (long long)order[r] - dense_lo;
+    if (i >= 0 && i < dense_n) rank_of[i] = r;
+}
+
+__global__ void __launch_bounds__(256) k_so_rank_keys(const OutRec* recs, int n, const int32_t* rank_of, int dense_lo,
+                                                       int dense_n, long long pos_base, unsigned long long* key,
+                                                       int32_t* idx, unsigned* bad) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    const int32_t p = recs[j].position, ord = recs[j].order;
+    const long long i = (long long)p - dense_lo;
+    const int32_t r = (i >= 0 && i < dense_n) ? rank_of[i] : -1;
+    const unsigned long long rk = r < 0 ? 0xffffffull : (unsigned long long)r;
+    const long long off = (long long)p - pos_base;
+    if (r >= 0xffffff || off < 0 || off >= (1ll << 30) || ord < 0 || ord >= 1024) atomicOr(bad, 1u);
+    key[j] = (rk << 40) | ((unsigned long long)(off & ((1ll << 30) - 1)) << 10) | (unsigned long long)(ord & 1023);
+    idx[j] = j;
+}
+
+hipError_t launch_so_rank(const int32_t* order, int n_keep, const OutRec* recs, int n, int dense_lo, int dense_n,
+                          long long pos_base, int32_t* rank_of, unsigned long long* key, int32_t* idx, unsigned* bad,
+                          hipStream_t s) {
+    hipError_t e;
+    if (dense_n > 0 && (e = hipMemsetAsync(rank_of, 0xff, sizeof(int32_t) * (size_t)dense_n, s)) != hipSuccess) return e;
+    if (n_keep > 0) {
+        hipLaunchKernelGGL(k_so_rank_scatter, dim3((unsigned)((n_keep + 255) / 256)), dim3(256), 0, s, order, n_keep,
+                           dense_lo, dense_n, rank_of);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    if (n > 0) {
+        hipLaunchKernelGGL(k_so_rank_keys, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, recs, n, rank_of, dense_lo,
+                           dense_n, pos_base, key, idx, bad);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 hipError_t launch_so_epoch_keys(const SoEpoch& E, uint32_t* skey, uint32_t* sval, hipStream_t s) {
     if (E.S <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_so_keys, dim3((unsigned)((E.S + 255) / 256)), dim3(256), 0, s, E, skey, sval);
