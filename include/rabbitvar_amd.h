@@ -205,6 +205,8 @@ size_t rv_callset_count(const rv_callset* cs);
 const rv_call* rv_callset_get(const rv_callset* cs, size_t i);
 /* The region's output lines exactly as the reference's simple mode prints them. */
 const char* rv_callset_text(const rv_callset* cs, size_t* len);
+/* Releases a call set (any thread; the engine may already be closed).  Returns at once: its
+ * contents are destroyed on the engine's release threads. */
 void rv_callset_free(rv_callset* cs);
 
 rv_status rv_get_stats(const rv_engine* e, rv_stats* out);

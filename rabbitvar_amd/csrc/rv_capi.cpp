@@ -18,6 +18,9 @@ Engine* engine_of(rv_engine* e) { return e ? e->e : nullptr; }
 
 struct rv_callset {
     rv::CallSet cs;
+    // the engine's release threads: freeing a call set (tens of thousands of strings, megabytes
+    // of text) hands its contents to them and returns at once
+    std::shared_ptr<rv::ReleasePool> pool;
 };
 
 namespace {
@@ -110,6 +113,7 @@ rv_status rv_run_region(rv_engine* e, const rv_region* region, const rv_reads* r
         delete cs;
         return s;
     }
+    cs->pool = e->e->release_pool();
     *out = cs;
     return RV_OK;
 }
@@ -129,6 +133,7 @@ rv_status rv_run_region_somatic(rv_engine* e, const rv_region* region, const rv_
         delete cs;
         return s;
     }
+    cs->pool = e->e->release_pool();
     *out = cs;
     return RV_OK;
 }
@@ -143,7 +148,17 @@ const char* rv_callset_text(const rv_callset* cs, size_t* len) {
     if (len) *len = cs->cs.text.size();
     return cs->cs.text.c_str();
 }
-void rv_callset_free(rv_callset* cs) { delete cs; }
+void rv_callset_free(rv_callset* cs) {
+    if (!cs) return;
+    if (cs->pool) {
+        try {
+            cs->pool->post(std::make_unique<rv::Held<rv::CallSet>>(std::move(cs->cs)));
+        } catch (...) {
+            // no memory for the job: the call set is destroyed here instead
+        }
+    }
+    delete cs;
+}
 
 rv_status rv_get_stats(const rv_engine* e, rv_stats* out) {
     if (!e || !out) return RV_ERR_ARG;

@@ -214,15 +214,18 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
     int win_len = W.ref_hi >= W.ref_lo ? W.ref_hi - W.ref_lo + 1 : 0;
     {
         std::string up((size_t)std::max(win_len, 1), 'N');
-        for (int i = 0; i < win_len; i++) {
-            char ch = contig.bases[W.ref_lo - 1 + i];
-            up[(size_t)i] = (ch >= 'a' && ch <= 'z') ? (char)(ch - 32) : ch;
+        const unsigned char* src = (const unsigned char*)contig.bases + (W.ref_lo - 1);
+        char* dst = &up[0];
+        for (int i = 0; i < win_len; i++) {  // branch-free upper case (vectorised)
+            const unsigned char ch = src[i];
+            dst[i] = (char)(ch - (unsigned char)(((unsigned)(ch - 'a') < 26u) << 5));
         }
         ref_dev = (char*)grow(b_ref, up.size());
         if (!ref_dev) return fail(RV_ERR_OOM, "device allocation for the reference window failed");
         RV_HIP(hipMemcpyAsync(ref_dev, up.data(), up.size(), hipMemcpyHostToDevice, stream));
-        RV_HIP(hipStreamSynchronize(stream));  // `up` is pageable and about to go out of scope
-        out.ref_win = up.substr(0, (size_t)win_len);
+        RV_HIP(hipStreamSynchronize(stream));  // `up` is pageable: the copy must finish before it moves
+        up.resize((size_t)win_len);
+        out.ref_win = std::move(up);
     }
     auto since_h0 = [&]() { return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count(); };
     stats.timeline_ms[8] = since_h0();

@@ -39,6 +39,66 @@ struct PileupHooks {
     std::function<void(PileupResult&)> after_events;  // the event maps are reduced (clips still running)
 };
 
+
+// A fixed set of threads destroying posted objects (an engine's region state, the call sets it
+// returned), shared by the engine and its call sets.
+struct Job {
+    virtual ~Job() = default;
+};
+template <class T>
+struct Held : Job {
+    explicit Held(T&& v) : obj(std::move(v)) {}
+    T obj;
+};
+class ReleasePool {
+  public:
+    explicit ReleasePool(unsigned n) {
+        for (unsigned i = 0; i < n; i++) th_.emplace_back([this] { loop(); });
+    }
+    ~ReleasePool() {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : th_) t.join();
+    }
+    void post(std::unique_ptr<Job> j) {
+        {
+            std::lock_guard<std::mutex> g(m_);
+            q_.push_back(std::move(j));
+        }
+        cv_.notify_one();
+    }
+    void wait() {
+        std::unique_lock<std::mutex> g(m_);
+        idle_.wait(g, [&] { return q_.empty() && busy_ == 0; });
+    }
+
+  private:
+    void loop() {
+        std::unique_lock<std::mutex> g(m_);
+        for (;;) {
+            cv_.wait(g, [&] { return stop_ || !q_.empty(); });
+            if (q_.empty()) return;  // stopping with nothing left
+            std::unique_ptr<Job> j = std::move(q_.front());
+            q_.pop_front();
+            busy_++;
+            g.unlock();
+            j.reset();
+            g.lock();
+            busy_--;
+            if (q_.empty() && busy_ == 0) idle_.notify_all();
+        }
+    }
+    std::mutex m_;
+    std::condition_variable cv_, idle_;
+    std::deque<std::unique_ptr<Job>> q_;
+    unsigned busy_ = 0;
+    bool stop_ = false;
+    std::vector<std::thread> th_;
+};
+
 class Engine {
   public:
     Engine(const rv_config& cfg, int device);
@@ -181,69 +241,13 @@ class Engine {
     // wait_release() returns once every posted object is destroyed; so does the engine's destructor.
     template <class... T>
     void release_later(T&&... objs) {
-        (release_.post(std::make_unique<Held<std::decay_t<T>>>(std::forward<T>(objs))), ...);
+        (release_->post(std::make_unique<Held<std::decay_t<T>>>(std::forward<T>(objs))), ...);
     }
-    void wait_release() { release_.wait(); }
+    void wait_release() { release_->wait(); }
+    const std::shared_ptr<ReleasePool>& release_pool() const { return release_; }
 
   private:
-    // a fixed set of threads destroying posted objects
-    struct Job {
-        virtual ~Job() = default;
-    };
-    template <class T>
-    struct Held : Job {
-        explicit Held(T&& v) : obj(std::move(v)) {}
-        T obj;
-    };
-    class ReleasePool {
-      public:
-        explicit ReleasePool(unsigned n) {
-            for (unsigned i = 0; i < n; i++) th_.emplace_back([this] { loop(); });
-        }
-        ~ReleasePool() {
-            {
-                std::lock_guard<std::mutex> g(m_);
-                stop_ = true;
-            }
-            cv_.notify_all();
-            for (auto& t : th_) t.join();
-        }
-        void post(std::unique_ptr<Job> j) {
-            {
-                std::lock_guard<std::mutex> g(m_);
-                q_.push_back(std::move(j));
-            }
-            cv_.notify_one();
-        }
-        void wait() {
-            std::unique_lock<std::mutex> g(m_);
-            idle_.wait(g, [&] { return q_.empty() && busy_ == 0; });
-        }
-
-      private:
-        void loop() {
-            std::unique_lock<std::mutex> g(m_);
-            for (;;) {
-                cv_.wait(g, [&] { return stop_ || !q_.empty(); });
-                if (q_.empty()) return;  // stopping with nothing left
-                std::unique_ptr<Job> j = std::move(q_.front());
-                q_.pop_front();
-                busy_++;
-                g.unlock();
-                j.reset();
-                g.lock();
-                busy_--;
-                if (q_.empty() && busy_ == 0) idle_.notify_all();
-            }
-        }
-        std::mutex m_;
-        std::condition_variable cv_, idle_;
-        std::deque<std::unique_ptr<Job>> q_;
-        unsigned busy_ = 0;
-        bool stop_ = false;
-        std::vector<std::thread> th_;
-    };
-    ReleasePool release_{8};
+    std::shared_ptr<ReleasePool> release_ = std::make_shared<ReleasePool>(8);
     std::vector<std::pair<void*, size_t>> pinned_;
     size_t pinned_cur_ = 0, pinned_off_ = 0;
     DevBuf b_reads[13], b_prep, b_cig, b_ctr, b_span, b_rows, b_rows2, b_ref;

@@ -47,6 +47,14 @@ class DeviceReads:
         return _lib.RvReads(self.n, *[ctypes.c_void_p(self.t[k].data_ptr()) for k in _ARRAYS], 1)
 
 
+def _text(p, n):
+    """The n bytes of UTF-8 text at p (a C pointer or address) as a str, decoded in place."""
+    if n == 0:
+        return ""
+    addr = ctypes.cast(p, ctypes.c_void_p).value
+    return str(memoryview((ctypes.c_char * n).from_address(addr)).cast("B"), "utf-8")
+
+
 def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data) if a is not None and a.size else ctypes.c_void_p(0)
 
@@ -109,7 +117,7 @@ class Engine:
                                       ctypes.byref(buf), ctypes.byref(n))
         if st != 0:
             raise self._error(st)
-        text = ctypes.string_at(buf, n.value).decode()
+        text = _text(buf, n.value)
         self.lib.rv_free(buf)
         return text
 
@@ -169,7 +177,7 @@ class Engine:
             raise self._error(st)
         n = ctypes.c_size_t()
         p = self.lib.rv_callset_text(cs, ctypes.byref(n))
-        text = ctypes.string_at(p, n.value).decode()
+        text = _text(p, n.value)
         self.lib.rv_callset_free(cs)
         return text
 
@@ -185,7 +193,7 @@ class Engine:
             raise self._error(st)
         n = ctypes.c_size_t()
         p = self.lib.rv_callset_text(cs, ctypes.byref(n))
-        text = ctypes.string_at(p, n.value).decode()
+        text = _text(p, n.value)
         self.lib.rv_callset_free(cs)
         return text
 
@@ -355,7 +363,7 @@ class EnginePool:
         for i in range(n):
             ln = ctypes.c_size_t()
             p = self.lib.rv_callset_text(out[i], ctypes.byref(ln))
-            texts.append(ctypes.string_at(p, ln.value).decode())
+            texts.append(_text(p, ln.value))
             self.lib.rv_callset_free(out[i])
         return texts
 
