@@ -41,9 +41,11 @@ const DenseRow* HostRowSource::fetch(const std::vector<int>& rows) {
     return out.data();
 }
 
-bool RowCache::load(std::vector<int>& rows) {
-    std::sort(rows.begin(), rows.end());
-    rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
+bool RowCache::load(std::vector<int>& rows, bool sorted_unique) {
+    if (!sorted_unique) {
+        std::sort(rows.begin(), rows.end());
+        rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
+    }
     if (rows.empty()) return true;
     const auto t0 = std::chrono::steady_clock::now();
     const DenseRow* buf = src_->fetch(rows);
@@ -65,17 +67,24 @@ void RowCache::release_slots() {
 }
 
 bool RowCache::prefetch(const std::vector<int>& positions, int radius, const std::vector<int>& exact) {
-    std::vector<int> want;
+    // the wanted rows as a bitmap over the window: the list comes out sorted and unique
+    if (n_ <= 0) return true;
+    std::vector<uint64_t> bits(((size_t)n_ + 63) / 64, 0);
+    auto mark = [&](int i) {
+        if (!have(i)) bits[(size_t)i >> 6] |= 1ull << (i & 63);
+    };
     for (int p : positions) {
-        int a = std::max(0, p - radius - lo_), b = std::min(n_ - 1, p + radius - lo_);
-        for (int i = a; i <= b; i++)
-            if (!have(i)) want.push_back(i);
+        const int a = std::max(0, p - radius - lo_), b = std::min(n_ - 1, p + radius - lo_);
+        for (int i = a; i <= b; i++) mark(i);
     }
     for (int p : exact) {
         const int i = p - lo_;
-        if (i >= 0 && i < n_ && !have(i)) want.push_back(i);
+        if (i >= 0 && i < n_) mark(i);
     }
-    return load(want);
+    std::vector<int> want;
+    for (size_t w = 0; w < bits.size(); w++)
+        for (uint64_t x = bits[w]; x; x &= x - 1) want.push_back((int)(w * 64 + (size_t)__builtin_ctzll(x)));
+    return load(want, true);
 }
 
 const DenseRow* RowCache::row(int pos) {

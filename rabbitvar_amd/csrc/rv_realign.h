@@ -63,7 +63,7 @@ class RowCache {
     double fetch_ms = 0;  // time in the source's fetches
 
   private:
-    bool load(std::vector<int>& rows);
+    bool load(std::vector<int>& rows, bool sorted_unique = false);
     bool have(int i) const { return (size_t)i < slots_->size() && (*slots_)[(size_t)i] != nullptr; }
     RowSource* src_;
     int lo_, n_;
