@@ -786,6 +786,18 @@ bool Engine::no_passing(int start, int end, bool& none) {
     return true;
 }
 
+void Engine::pinned_reset() {
+    // blocks the last region used stay (pinning gigabytes again costs far more than keeping them);
+    // unused blocks stay while the total is within PINNED_KEEP
+    size_t keep = 0, total = 0;
+    while (keep < pinned_.size() && (keep <= pinned_cur_ || total + pinned_[keep].second <= PINNED_KEEP))
+        total += pinned_[keep++].second;
+    for (size_t i = keep; i < pinned_.size(); i++) (void)hipHostFree(pinned_[i].first);
+    pinned_.resize(keep);
+    pinned_cur_ = 0;
+    pinned_off_ = 0;
+}
+
 void* Engine::pinned_alloc(size_t bytes) {
     bytes = (bytes + 255) & ~(size_t)255;
     while (pinned_cur_ < pinned_.size()) {

@@ -150,12 +150,12 @@ class Engine {
     // copies the listed dense rows (indices into the window) of the last pileup to pinned host
     // memory, valid until pinned_reset(); nullptr on failure
     const DenseRow* fetch_rows(const std::vector<int>& rows);
-    // pinned host memory for one region's transfers: blocks are kept and reused by the next region
+    // pinned host memory for one region's transfers: blocks are kept and reused by the next region;
+    // when a region starts, blocks the previous region did not use are returned beyond PINNED_KEEP
+    // bytes in all (a large region's transfers do not stay page-locked while smaller ones follow)
+    static constexpr size_t PINNED_KEEP = (size_t)1 << 30;
     void* pinned_alloc(size_t bytes);
-    void pinned_reset() {
-        pinned_cur_ = 0;
-        pinned_off_ = 0;
-    }
+    void pinned_reset();
     // row index -> fetched row table of the realigner's row cache, reused across regions (all null
     // between regions)
     std::vector<const DenseRow*> row_slots;
