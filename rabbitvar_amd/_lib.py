@@ -42,6 +42,20 @@ class RvReads(ctypes.Structure):
     ]
 
 
+class RvCall(ctypes.Structure):
+    _fields_ = [
+        ("position", c_int32), ("start", c_int32), ("end", c_int32),
+        ("total_pos_coverage", c_int32), ("position_coverage", c_int32),
+        ("ref_fwd", c_int32), ("ref_rev", c_int32), ("var_fwd", c_int32), ("var_rev", c_int32),
+        ("shift3", c_int32), ("msint", c_int32), ("hicnt", c_int32), ("hicov", c_int32),
+        ("frequency", c_double), ("mean_position", c_double), ("mean_quality", c_double), ("mean_mapq", c_double),
+        ("hi_lo_ratio", c_double), ("hi_freq", c_double), ("extra_freq", c_double), ("msi", c_double), ("nm", c_double),
+        ("fisher_p", c_double), ("fisher_ratio", c_double), ("pstd", c_int32), ("qstd", c_int32),
+        ("refallele", c_char_p), ("varallele", c_char_p), ("genotype", c_char_p), ("bias", c_char_p),
+        ("leftseq", c_char_p), ("rightseq", c_char_p), ("vartype", c_char_p),
+    ]
+
+
 class RvRegion(ctypes.Structure):
     _fields_ = [("chr", c_char_p), ("start", c_int32), ("end", c_int32), ("gene", c_char_p)]
 

@@ -4,44 +4,56 @@
 
 #include <cstdio>
 #include <cstring>
+#include <vector>
 
 namespace rv {
 
 namespace {
 
-void put_u(std::string& o, unsigned long long v) {
+// Writers into a caller's buffer: each returns the end of what it wrote.
+char* w_u(char* p, unsigned long long v) {
     char b[24];
     int n = 0;
     do {
         b[n++] = (char)('0' + (int)(v % 10));
         v /= 10;
     } while (v);
-    while (n) o += b[--n];
+    while (n) *p++ = b[--n];
+    return p;
 }
-void put_i(std::string& o, long long v) {
+char* w_i(char* p, long long v) {
     if (v < 0) {
-        o += '-';
-        put_u(o, (unsigned long long)(-(v + 1)) + 1ull);
-    } else {
-        put_u(o, (unsigned long long)v);
+        *p++ = '-';
+        return w_u(p, (unsigned long long)(-(v + 1)) + 1ull);
     }
+    return w_u(p, (unsigned long long)v);
 }
-}  // namespace
+char* w_s(char* p, const char* s, size_t n) {
+    std::memcpy(p, s, n);
+    return p + n;
+}
+char* w_s0(char* p, const char* s, int n) {  // empty strings print as "0"
+    if (n == 0) {
+        *p++ = '0';
+        return p;
+    }
+    return w_s(p, s, (size_t)n);
+}
 
-// "%f" (six decimals) of a finite double, exactly as printf prints it: the binary value times 10^6
+constexpr size_t W_F_MAX = 400;  // the longest "%f" of a double (1.8e308: 309 digits + ".000000")
+
+// "%f" (six decimals) of a double, exactly as printf prints it: the binary value times 10^6
 // rounded to the nearest integer, ties to even, computed in 128-bit integers from the value's
 // significand and exponent.  Values of 2^43 and above, and non-finite ones, go through snprintf.
-void put_f(std::string& o, double v) {
+char* w_f(char* p, double v) {
     uint64_t bits;
     std::memcpy(&bits, &v, 8);
     const bool neg = (bits >> 63) != 0;
     const int bexp = (int)((bits >> 52) & 0x7ff);
     uint64_t m = bits & ((1ull << 52) - 1);
     if (bexp == 0x7ff || (bexp != 0 && bexp - 1075 >= -10)) {  // non-finite, or |v| >= 2^43
-        char b[512];
-        const int n = snprintf(b, sizeof b, "%f", v);
-        o.append(b, (size_t)n);
-        return;
+        const int n = snprintf(p, W_F_MAX, "%f", v);
+        return p + (n > 0 ? (n < (int)W_F_MAX ? n : (int)W_F_MAX - 1) : 0);
     }
     int e;
     if (bexp == 0) {
@@ -62,27 +74,23 @@ void put_f(std::string& o, double v) {
         const unsigned __int128 half = ((unsigned __int128)1) << (k - 1);
         if (r > half || (r == half && (q & 1))) q++;
     }
-    if (neg) o += '-';
-    put_u(o, q / 1000000u);
-    o += '.';
-    char f[6];
+    if (neg) *p++ = '-';
+    p = w_u(p, q / 1000000u);
+    *p++ = '.';
     unsigned long long fr = q % 1000000u;
     for (int i = 5; i >= 0; i--) {
-        f[i] = (char)('0' + (int)(fr % 10));
+        p[i] = (char)('0' + (int)(fr % 10));
         fr /= 10;
     }
-    o.append(f, 6);
-}
-
-namespace {
-void put_s0(std::string& o, const char* p, int n) {  // empty strings print as "0"
-    if (n == 0)
-        o += '0';
-    else
-        o.append(p, (size_t)n);
+    return p + 6;
 }
 
 }  // namespace
+
+void put_f(std::string& o, double v) {
+    char b[W_F_MAX];
+    o.append(b, (size_t)(w_f(b, v) - b));
+}
 
 std::string vartype_name(const OutRec& o, const char* arena) {
     switch (o.vartype) {
@@ -101,95 +109,110 @@ std::string vartype_name(const OutRec& o, const char* arena) {
 void format_line(std::string& out, const OutRec& o, const char* arena, const LineContext& ctx) {
     const char* s = arena + o.str_off;
     const char* part[6];
+    size_t strs = 0;
     for (int k = 0, off = 0; k < 6; k++) {
         part[k] = s + off;
         off += o.len[k];
+        strs += o.len[k];
     }
-    out += ctx.sample;
-    out += '\t';
-    out += ctx.gene;
-    out += '\t';
-    out += ctx.chr;
-    out += '\t';
-    put_i(out, o.start);
-    out += '\t';
-    put_i(out, o.end);
-    out += '\t';
-    out.append(part[0], o.len[0]);  // refallele
-    out += '\t';
-    out.append(part[1], o.len[1]);  // varallele
-    out += '\t';
-    put_i(out, o.tpc);
-    out += '\t';
-    put_i(out, o.poscov);
-    out += '\t';
-    put_i(out, o.rfc);
-    out += '\t';
-    put_i(out, o.rrc);
-    out += '\t';
-    put_i(out, o.fwd);
-    out += '\t';
-    put_i(out, o.rev);
-    out += '\t';
-    put_s0(out, part[2], o.len[2]);  // genotype
-    out += '\t';
-    put_f(out, o.freq);
-    out += '\t';
-    out.append(part[3], o.len[3]);  // strand bias flags
-    out += '\t';
-    put_f(out, o.pmean);
-    out += '\t';
-    out += o.pstd ? '1' : '0';
-    out += '\t';
-    put_f(out, o.qmean);
-    out += '\t';
-    out += o.qstd ? '1' : '0';
-    out += '\t';
+    const size_t ls = std::strlen(ctx.sample), lg = std::strlen(ctx.gene), lc = std::strlen(ctx.chr);
+    const std::string vt = vartype_name(o, arena);
+    // the line's length is at most: its strings, 16 numbers of "%f", 18 integers, the separators
+    const size_t bound = ls + lg + 2 * lc + strs + vt.size() + 16 * W_F_MAX + 18 * 24 + 96;
+    char stackbuf[8192];
+    std::vector<char> heapbuf;
+    char* b = stackbuf;
+    if (bound > sizeof stackbuf) {
+        heapbuf.resize(bound);
+        b = heapbuf.data();
+    }
+    char* p = b;
+    p = w_s(p, ctx.sample, ls);
+    *p++ = '\t';
+    p = w_s(p, ctx.gene, lg);
+    *p++ = '\t';
+    p = w_s(p, ctx.chr, lc);
+    *p++ = '\t';
+    p = w_i(p, o.start);
+    *p++ = '\t';
+    p = w_i(p, o.end);
+    *p++ = '\t';
+    p = w_s(p, part[0], (size_t)o.len[0]);  // refallele
+    *p++ = '\t';
+    p = w_s(p, part[1], (size_t)o.len[1]);  // varallele
+    *p++ = '\t';
+    p = w_i(p, o.tpc);
+    *p++ = '\t';
+    p = w_i(p, o.poscov);
+    *p++ = '\t';
+    p = w_i(p, o.rfc);
+    *p++ = '\t';
+    p = w_i(p, o.rrc);
+    *p++ = '\t';
+    p = w_i(p, o.fwd);
+    *p++ = '\t';
+    p = w_i(p, o.rev);
+    *p++ = '\t';
+    p = w_s0(p, part[2], o.len[2]);  // genotype
+    *p++ = '\t';
+    p = w_f(p, o.freq);
+    *p++ = '\t';
+    p = w_s(p, part[3], (size_t)o.len[3]);  // strand bias flags
+    *p++ = '\t';
+    p = w_f(p, o.pmean);
+    *p++ = '\t';
+    *p++ = o.pstd ? '1' : '0';
+    *p++ = '\t';
+    p = w_f(p, o.qmean);
+    *p++ = '\t';
+    *p++ = o.qstd ? '1' : '0';
+    *p++ = '\t';
     if (ctx.fisher) {
-        put_f(out, o.fisher_p);
-        out += '\t';
-        put_f(out, o.fisher_ratio);
-        out += '\t';
+        p = w_f(p, o.fisher_p);
+        *p++ = '\t';
+        p = w_f(p, o.fisher_ratio);
+        *p++ = '\t';
     }
-    put_f(out, o.mq);
-    out += '\t';
-    put_f(out, o.hlr);
-    out += '\t';
-    put_f(out, o.hifreq);
-    out += '\t';
-    put_f(out, o.extrafreq);
-    out += '\t';
-    put_i(out, o.shift3);
-    out += '\t';
-    put_f(out, o.msi);
-    out += '\t';
-    put_i(out, o.msint);
-    out += '\t';
+    p = w_f(p, o.mq);
+    *p++ = '\t';
+    p = w_f(p, o.hlr);
+    *p++ = '\t';
+    p = w_f(p, o.hifreq);
+    *p++ = '\t';
+    p = w_f(p, o.extrafreq);
+    *p++ = '\t';
+    p = w_i(p, o.shift3);
+    *p++ = '\t';
+    p = w_f(p, o.msi);
+    *p++ = '\t';
+    p = w_i(p, o.msint);
+    *p++ = '\t';
     if (o.nm > 0)
-        put_f(out, o.nm);
+        p = w_f(p, o.nm);
     else
-        out += '0';
-    out += '\t';
-    put_i(out, o.hicnt);
-    out += '\t';
-    put_i(out, o.hicov);
-    out += '\t';
-    put_s0(out, part[4], o.len[4]);  // leftseq
-    out += '\t';
-    put_s0(out, part[5], o.len[5]);  // rightseq
-    out += '\t';
-    out += ctx.chr;
-    out += ':';
-    put_i(out, ctx.region_start);
-    out += '-';
-    put_i(out, ctx.region_end);
-    out += '\t';
-    out += vartype_name(o, arena);
-    out += '\t';
-    out += "0.000000";  // duprate (0 unless -t)
-    out += '\t';
-    out += '0';  // structural-variant field: the path calls no SVs
-    out += '\n';
+        *p++ = '0';
+    *p++ = '\t';
+    p = w_i(p, o.hicnt);
+    *p++ = '\t';
+    p = w_i(p, o.hicov);
+    *p++ = '\t';
+    p = w_s0(p, part[4], o.len[4]);  // leftseq
+    *p++ = '\t';
+    p = w_s0(p, part[5], o.len[5]);  // rightseq
+    *p++ = '\t';
+    p = w_s(p, ctx.chr, lc);
+    *p++ = ':';
+    p = w_i(p, ctx.region_start);
+    *p++ = '-';
+    p = w_i(p, ctx.region_end);
+    *p++ = '\t';
+    p = w_s(p, vt.data(), vt.size());
+    *p++ = '\t';
+    p = w_s(p, "0.000000", 8);  // duprate (0 unless -t)
+    *p++ = '\t';
+    *p++ = '0';  // structural-variant field: the path calls no SVs
+    *p++ = '\n';
+    out.append(b, (size_t)(p - b));
 }
 
 }  // namespace rv

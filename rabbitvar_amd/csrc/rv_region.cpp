@@ -5,6 +5,7 @@
 #include "rv_fisher.h"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <climits>
 #include <cstring>
@@ -462,23 +463,30 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     const size_t nt = n < 4096 ? 1 : std::min<size_t>(std::min(hw, 16u), n / 2048);
     std::vector<std::string> parts(nt);
     std::vector<uint64_t> line_at(n + 1);  // (chunk << 40) | offset of record i's line in its chunk
-    std::vector<CallStrings> strs(n);
+    // the call records' strings (alleles, genotype, bias, flanks, type), NUL-terminated, in one
+    // arena per chunk; str_at[i] = their offsets in record i's chunk arena
+    std::vector<std::string> sarenas(nt);
+    std::vector<std::array<uint32_t, 7>> str_at(n);
     auto work = [&](size_t t) {
         const size_t a = n * t / nt, b = n * (t + 1) / nt;
         std::string& txt = parts[t];
+        std::string& sa = sarenas[t];
         txt.reserve((b - a) * 200);
+        sa.reserve((b - a) * 64);
         for (size_t i = a; i < b; i++) {
             const OutRec& o = recs[i];
             line_at[i] = ((uint64_t)t << 40) | (uint64_t)txt.size();
             format_line(txt, o, arena.data(), ctx);
-            CallStrings& cs = strs[i];
             const char* p = arena.data() + o.str_off;
-            std::string* dst[6] = {&cs.refallele, &cs.varallele, &cs.genotype, &cs.bias, &cs.leftseq, &cs.rightseq};
             for (int k = 0; k < 6; k++) {
-                dst[k]->assign(p, o.len[k]);
+                str_at[i][k] = (uint32_t)sa.size();
+                sa.append(p, o.len[k]);
+                sa += '\0';
                 p += o.len[k];
             }
-            cs.vartype = vartype_name(o, arena.data());
+            str_at[i][6] = (uint32_t)sa.size();
+            sa += vartype_name(o, arena.data());
+            sa += '\0';
         }
     };
     {
@@ -492,7 +500,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     // while the order thread finishes: the output's buffers, whose sizes are known now
     out.text.clear();
     out.calls.clear();
-    out.strings.clear();
+    out.string_arenas.clear();
     {
         size_t total = 0;
         for (auto& pt : parts) total += pt.size();
@@ -524,7 +532,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     std::vector<size_t> text_at(n + 1, 0);
     for (size_t k = 0; k < n; k++) text_at[k + 1] = text_at[k] + line_len(perm[k]).second;
     if (text_at[n] != out.text.size()) return e.fail(RV_ERR_DEVICE, "output lines do not add up to the text");
-    out.strings = std::move(strs);
+    out.string_arenas = std::move(sarenas);
     auto assemble = [&](size_t a, size_t b) {
         char* txt = n ? &out.text[0] : nullptr;
         for (size_t k = a; k < b; k++) {
@@ -532,7 +540,8 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
             const auto ln = line_len(i);
             std::memcpy(txt + text_at[k], ln.first, ln.second);
             const OutRec& o = recs[i];
-            const CallStrings& cs = out.strings[i];
+            const char* sb = out.string_arenas[(size_t)(line_at[i] >> 40)].data();
+            const std::array<uint32_t, 7>& so = str_at[i];
             rv_call& c = out.calls[k];
             c.position = o.position;
             c.start = o.start;
@@ -560,13 +569,13 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
             c.fisher_ratio = o.fisher_ratio;
             c.pstd = o.pstd;
             c.qstd = o.qstd;
-            c.refallele = cs.refallele.c_str();
-            c.varallele = cs.varallele.c_str();
-            c.genotype = cs.genotype.c_str();
-            c.bias = cs.bias.c_str();
-            c.leftseq = cs.leftseq.c_str();
-            c.rightseq = cs.rightseq.c_str();
-            c.vartype = cs.vartype.c_str();
+            c.refallele = sb + so[0];
+            c.varallele = sb + so[1];
+            c.genotype = sb + so[2];
+            c.bias = sb + so[3];
+            c.leftseq = sb + so[4];
+            c.rightseq = sb + so[5];
+            c.vartype = sb + so[6];
         }
     };
     e.stats.timeline_ms[14] = ms_since(w0);
@@ -600,7 +609,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
                     std::move(pr.P.mnp), std::move(pr.P.rows), std::move(pr.ref_win));
     e.release_later(std::move(S), std::move(rcp), std::move(pr), std::move(cx), std::move(created),
                     std::move(walk_sorted), std::move(nonins), std::move(recs), std::move(arena), std::move(status),
-                    std::move(proc), std::move(aligned), std::move(parts), std::move(line_at), std::move(strs),
+                    std::move(proc), std::move(aligned), std::move(parts), std::move(line_at), std::move(str_at),
                     std::move(perm));
     e.stats.timeline_ms[7] = ms_since(w0);
     return RV_OK;

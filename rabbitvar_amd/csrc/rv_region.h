@@ -8,13 +8,10 @@
 
 namespace rv {
 
-struct CallStrings {
-    std::string refallele, varallele, genotype, bias, leftseq, rightseq, vartype;
-};
 
 struct CallSet {
     std::vector<rv_call> calls;
-    std::vector<CallStrings> strings;  // backing storage for the rv_call string pointers
+    std::vector<std::string> string_arenas;  // backing storage for the rv_call string pointers
     std::string text;
 };
 

@@ -720,7 +720,7 @@ rv_status run_region_somatic(Engine& e, const rv_region& region, const rv_reads&
     e.stats.detail_ms[4] = lap(tl);
     out.text = std::move(text);
     out.calls.clear();
-    out.strings.clear();
+    out.string_arenas.clear();
     e.stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
     e.stats.realign_device_pairs = e.f35_device_pairs;
     e.stats.realign_device_ms = e.f35_ms;

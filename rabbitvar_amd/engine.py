@@ -181,6 +181,29 @@ class Engine:
         self.lib.rv_callset_free(cs)
         return text
 
+    def run_region_calls(self, chrom, start, end, reads, contig_name, contig_bases, gene=None):
+        """run_region, returning (text, calls): the structured records (rv_call) as dicts, in the
+        text's line order."""
+        region, r, contig = self._args(chrom, start, end, reads, contig_name, contig_bases, gene)
+        cs = ctypes.c_void_p()
+        st = self.lib.rv_run_region(self.h, ctypes.byref(region), ctypes.byref(r), ctypes.byref(contig),
+                                    ctypes.byref(cs))
+        if st != 0:
+            raise self._error(st)
+        try:
+            n = ctypes.c_size_t()
+            text = _text(self.lib.rv_callset_text(cs, ctypes.byref(n)), n.value)
+            calls = []
+            for i in range(self.lib.rv_callset_count(cs)):
+                c = ctypes.cast(self.lib.rv_callset_get(cs, i), ctypes.POINTER(_lib.RvCall)).contents
+                d = {k: getattr(c, k) for k, _ in c._fields_}
+                for k in ("refallele", "varallele", "genotype", "bias", "leftseq", "rightseq", "vartype"):
+                    d[k] = d[k].decode() if d[k] is not None else None
+                calls.append(d)
+        finally:
+            self.lib.rv_callset_free(cs)
+        return text, calls
+
     def run_region_somatic(self, chrom, start, end, tumor, normal, contig_name, contig_bases, gene=None):
         """Somatic mode for one region (the first and second samples of -b 'T|N'); returns the text
         the reference's somatic printer writes for it."""

@@ -94,3 +94,39 @@ def test_pileup_mode_matches_oracle(dataset, case, start, end):
     eng.close()
     assert len(want.splitlines()) > 0.9 * (end - start + 1)
     assert got == want
+
+
+def test_structured_records_match_the_text(dataset):
+    """rv_callset_get's records (rv_call) carry, in the text's line order, the printed fields of
+    each line: positions, counts, the %f values, alleles, genotype, bias, flanks and type."""
+    from rabbitvar_amd.engine import Engine
+
+    d = dataset("c1")
+    name, seq, reads = _inputs(d)
+    L = len(seq)
+    want, _ = run_oracle(d, f"{name}:1-{L}", ["--fisher"], out="calls_structured.txt")
+    eng = Engine(sample="S1", fisher=1)
+    text, calls = eng.run_region_calls(name, 1, L, reads.select(1, L), name, seq)
+    eng.close()
+    assert text == want
+    lines = text.splitlines()
+    assert len(calls) == len(lines) > 100
+
+    def f6(x):
+        return "%f" % x
+
+    for c, ln in zip(calls, lines):
+        f = ln.split("\t")
+        assert (int(f[3]), int(f[4])) == (c["start"], c["end"])
+        assert (f[5], f[6]) == (c["refallele"], c["varallele"])
+        assert [int(x) for x in f[7:13]] == [c["total_pos_coverage"], c["position_coverage"], c["ref_fwd"],
+                                             c["ref_rev"], c["var_fwd"], c["var_rev"]]
+        assert f[13] == (c["genotype"] or "0")
+        assert (f[14], f[15]) == (f6(c["frequency"]), c["bias"])
+        assert (f[16], f[18]) == (f6(c["mean_position"]), f6(c["mean_quality"]))
+        assert (f[20], f[21]) == (f6(c["fisher_p"]), f6(c["fisher_ratio"]))
+        assert f[22] == f6(c["mean_mapq"])
+        assert (f[26], f[27], f[28]) == (str(c["shift3"]), f6(c["msi"]), str(c["msint"]))
+        assert (f[30], f[31]) == (str(c["hicnt"]), str(c["hicov"]))
+        assert (f[32], f[33]) == (c["leftseq"] or "0", c["rightseq"] or "0")
+        assert f[35] == c["vartype"]
