@@ -162,6 +162,40 @@ struct ReadView {
     RV_HD int qq(int i) const { return (i >= 0 && i < lseq) ? (int)q[i] : 0; }
 };
 
+// The end-mismatch scans of the tail rules (cigarModifier.cpp:529-568, the 5' dig): the length of
+// the mismatching run at the 3' end of an M run ending at refoff / rdoff, and at the 5' end of an
+// M run starting at pos (read offset 0); both stop after 3 matches past the last mismatch.
+RV_HD int tail_mismatch_run(const RefView& rf, const ReadView& rd, int refoff, int rdoff, int mch) {
+    int rn = 0, rrn = 0, rmch = 0;
+    while (rrn < mch && rn < mch) {
+        if (!rf.has(refoff - rrn - 1)) break;
+        if (rrn < rdoff && rf.at(refoff - rrn - 1) != rd.s(rdoff - rrn - 1)) {
+            rn = rrn + 1;
+            rmch = 0;
+        } else if (rrn < rdoff && rf.at(refoff - rrn - 1) == rd.s(rdoff - rrn - 1)) {
+            rmch++;
+        }
+        rrn++;
+        if (rmch >= 3) break;
+    }
+    return rn;
+}
+RV_HD int head_mismatch_run(const RefView& rf, const ReadView& rd, int pos, int mch) {
+    int rn = 0, rrn = 0, rmch = 0;
+    while (rrn < mch && rn < mch) {
+        if (!rf.has(pos + rrn)) break;
+        if (rf.hasNe(pos + rrn, rd.s(rrn))) {
+            rn = rrn + 1;
+            rmch = 0;
+        } else if (rf.hasEq(pos + rrn, rd.s(rrn))) {
+            rmch++;
+        }
+        rrn++;
+        if (rmch >= 3) break;
+    }
+    return rn;
+}
+
 struct Rewriter {
     RefView rf;
     ReadView rd;
@@ -183,18 +217,7 @@ struct Rewriter {
 
     // cigarModifier.cpp:412-439
     RV_HD void begin_dig_m(int mch) {
-        int rn = 0, rrn = 0, rmch = 0;
-        while (rrn < mch && rn < mch) {
-            if (!rf.has(pos + rrn)) break;
-            if (rf.hasNe(pos + rrn, rd.s(rrn))) {
-                rn = rrn + 1;
-                rmch = 0;
-            } else if (rf.hasEq(pos + rrn, rd.s(rrn))) {
-                rmch++;
-            }
-            rrn++;
-            if (rmch >= 3) break;
-        }
+        const int rn = head_mismatch_run(rf, rd, pos, mch);
         if (rn > 0 && rn <= 3) {
             mch -= rn;
             CText r;
@@ -267,22 +290,6 @@ struct Rewriter {
         }
     }
 
-    RV_HD int tail_mismatch_run(int refoff, int rdoff, int mch) {
-        int rn = 0, rrn = 0, rmch = 0;
-        while (rrn < mch && rn < mch) {
-            if (!rf.has(refoff - rrn - 1)) break;
-            if (rrn < rdoff && rf.at(refoff - rrn - 1) != rd.s(rdoff - rrn - 1)) {
-                rn = rrn + 1;
-                rmch = 0;
-            } else if (rrn < rdoff && rf.at(refoff - rrn - 1) == rd.s(rdoff - rrn - 1)) {
-                rmch++;
-            }
-            rrn++;
-            if (rmch >= 3) break;
-        }
-        return rn;
-    }
-
     // cigarModifier.cpp:529-568 (prefix = text before the last M run)
     RV_HD void three_mismatches(int prefix_len, int mch) {
         int refoff = pos + mch, rdoff = mch;
@@ -290,7 +297,7 @@ struct Rewriter {
             refoff += rx_sum_prefix(cs, prefix_len, "MND");
             rdoff += rx_sum_prefix(cs, prefix_len, "MIS");
         }
-        int rn = tail_mismatch_run(refoff, rdoff, mch);
+        int rn = tail_mismatch_run(rf, rd, refoff, rdoff, mch);
         mch -= rn;
         if (rn > 0 && rn <= 3) {
             CText r;
@@ -342,7 +349,7 @@ struct Rewriter {
                 rx_replace_first(cs, "nMnS", false, true, r);
             }
             if (rn == 0) {
-                int r2 = tail_mismatch_run(refoff, rdoff, mch);
+                int r2 = tail_mismatch_run(rf, rd, refoff, rdoff, mch);
                 if (r2 > 0 && r2 < mch) {
                     soft += r2;
                     mch -= r2;
@@ -709,6 +716,32 @@ RV_HD int op_code_of(char c) {
 // and bit2 when a negative length restored the original.
 RV_HD bool modify_cigar_record(const RefView& rf, const ReadView& rd, int& position, uint32_t* rec, int rec_cap,
                                int n_orig, int indel_len, int& n_new, int& flags) {
+    // A single M run of 8 or more bases (most reads): no indel rule applies, and the tail rules
+    // reduce to the 3' then the 5' end-mismatch clip ("nM" -> "aMbS" -> "cSdMbS", 1-3 bases each),
+    // computed here without the CIGAR text.  Lengths stay positive, so the text path's parse has
+    // nothing to refuse.
+    if (n_orig == 1 && cop(rec[0]) == OP_M && clen(rec[0]) >= 8) {
+        int m = (int)clen(rec[0]);
+        const int rn3 = tail_mismatch_run(rf, rd, position + m, m, m);
+        const int s3 = (rn3 > 0 && rn3 <= 3) ? rn3 : 0;
+        m -= s3;
+        const int rn5 = head_mismatch_run(rf, rd, position, m);
+        const int s5 = (rn5 > 0 && rn5 <= 3) ? rn5 : 0;
+        m -= s5;
+        if (s3 == 0 && s5 == 0) return false;  // the text is unchanged
+        const int k = 1 + (s5 > 0) + (s3 > 0);
+        if (k > rec_cap) {
+            flags |= 2;
+            return false;
+        }
+        int w = 0;
+        if (s5) rec[w++] = ((uint32_t)s5 << 4) | (uint32_t)OP_S;
+        rec[w++] = ((uint32_t)m << 4) | (uint32_t)OP_M;
+        if (s3) rec[w++] = ((uint32_t)s3 << 4) | (uint32_t)OP_S;
+        n_new = w;
+        position += s5;
+        return true;
+    }
     int base = 0;
     int n = n_orig;
     CText original;
