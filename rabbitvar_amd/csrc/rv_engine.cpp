@@ -213,7 +213,11 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
     window_bounds(cfg, region, contig.length, W.ref_lo, W.ref_hi);
     int win_len = W.ref_hi >= W.ref_lo ? W.ref_hi - W.ref_lo + 1 : 0;
     {
-        std::string up((size_t)std::max(win_len, 1), 'N');
+        // the window string of an earlier region, when one was handed back: its pages are
+        // already mapped
+        std::string up = std::move(ref_spare);
+        ref_spare.clear();
+        up.resize((size_t)std::max(win_len, 1), 'N');
         const unsigned char* src = (const unsigned char*)contig.bases + (W.ref_lo - 1);
         char* dst = &up[0];
         for (int i = 0; i < win_len; i++) {  // branch-free upper case (vectorised)

@@ -130,6 +130,8 @@ class Engine {
     DenseRow* rows = nullptr;
     int rows_n = 0, rows_lo = 0;
     char* ref_dev = nullptr;
+    // a finished region's reference window string, handed back for the next pileup to reuse
+    std::string ref_spare;
 
     void* grow(DevBuf& b, size_t bytes);
 

@@ -603,10 +603,12 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     e.last_rows_fetched = rc.rows_fetched;
     e.last_insertion_side_effects = (int64_t)hc[3];
     rcp->release_slots();
-    // the largest maps each go to their own release job
+    // the reference window string goes back to the engine; the largest maps each go to their own
+    // release job
+    e.ref_spare = std::move(pr.ref_win);
     e.release_later(std::move(S.pos), std::move(S.pool), std::move(S.ins), std::move(pr.P.nd), std::move(pr.P.ins),
                     std::move(pr.P.sc5), std::move(pr.P.sc3), std::move(pr.P.delCount), std::move(pr.P.insCount),
-                    std::move(pr.P.mnp), std::move(pr.P.rows), std::move(pr.ref_win));
+                    std::move(pr.P.mnp), std::move(pr.P.rows));
     e.release_later(std::move(S), std::move(rcp), std::move(pr), std::move(cx), std::move(created),
                     std::move(walk_sorted), std::move(nonins), std::move(recs), std::move(arena), std::move(status),
                     std::move(proc), std::move(aligned), std::move(parts), std::move(line_at), std::move(str_at),
