@@ -241,6 +241,37 @@ const HVar* RState::peek(int p, const std::string& d) {
     return nullptr;
 }
 
+const HVar* RState::peek_loaded(int p, const std::string& d, bool* hit) const {
+    auto it = pos.find(p);
+    *hit = it != pos.end() && it->second.loaded;
+    if (!*hit || !it->second.key) return nullptr;
+    auto j = it->second.v.find(d);
+    return j == it->second.v.end() ? nullptr : j->second;
+}
+
+int RState::peek_static(int p, const std::string& d, HVar* out) const {
+    const DenseRow* dr;
+    if (!rows.row_loaded(p, dr)) return 2;
+    const auto* nd = nd_ix.find(p);
+    const bool key = (dr && (dr->rowflags & 1u)) || nd != nullptr;
+    if (!key) return 0;
+    if (d.size() == 1 && dr) {
+        const int b = base_index(d[0]);
+        if (b >= 0 && (dr->b[b].flags & 1u)) {
+            *out = dense_to_hvar(dr->b[b]);
+            return 1;
+        }
+    }
+    if (nd) {
+        auto j = nd->find(d);
+        if (j != nd->end()) {
+            *out = j->second;
+            return 1;
+        }
+    }
+    return 0;
+}
+
 HVar* RState::maybe(int p, char refBase) {
     if (refBase == 0) return nullptr;
     return inner(p, std::string(1, refBase));
@@ -778,14 +809,84 @@ struct Realigner {
         return reads->no_passing(start, end);
     }
 
+    // Every read adjustMNP makes of a position that at() has not loaded comes from the dense rows
+    // and the non-dense map, which the realigner never changes.  Those reads (the candidate's
+    // descriptor, then each split's left and right piece) and the candidates' clip clusters are
+    // resolved on several threads before the ordered walk, which then reads them in order
+    // instead of from hash slots and rows scattered over the region.
+    struct MnpPre {
+        std::vector<size_t> off;      // candidate k's reads: [off[k], off[k+1])
+        std::vector<uint8_t> state;   // peek_static's result per read
+        std::vector<HVar> v;
+        std::vector<HSclip*> s3, s5;  // sc3(position), sc5(position + length)
+    };
+    void mnp_pre(const std::vector<Cand>& tmp, MnpPre& pre) {
+        const size_t n = tmp.size();
+        pre.off.assign(n + 1, 0);
+        for (size_t k = 0; k < n; k++) {
+            const std::string& d = tmp[k].desc;
+            const size_t len = d.size() - (d.find('&') != std::string::npos ? 1 : 0);
+            pre.off[k + 1] = pre.off[k] + 1 + 2 * (len > 0 ? len - 1 : 0);
+        }
+        pre.state.resize(pre.off[n]);
+        pre.v.resize(pre.off[n]);
+        pre.s3.resize(n);
+        pre.s5.resize(n);
+        auto work = [&](size_t a, size_t b) {
+            for (size_t k = a; k < b; k++) {
+                const int position = tmp[k].position;
+                uint8_t* st = pre.state.data() + pre.off[k];
+                HVar* v = pre.v.data() + pre.off[k];
+                const size_t nq = pre.off[k + 1] - pre.off[k];
+                std::fill(st, st + nq, (uint8_t)2);  // unknown until resolved
+                std::string mnt = tmp[k].desc;
+                replace_first(mnt, "&", "");
+                pre.s3[k] = sc3(position);
+                pre.s5[k] = sc5(position + (int)mnt.size());
+                try {
+                    st[0] = (uint8_t)S.peek_static(position, tmp[k].desc, &v[0]);
+                    for (int i = 0; i < (int)mnt.size() - 1; i++) {
+                        std::string left = sub2(mnt, 0, i + 1);
+                        if (left.size() > 1) left.insert(1, "&");
+                        std::string right = sub1(mnt, -(int)(mnt.size() - (size_t)i - 1));
+                        if (right.size() > 1) right.insert(1, "&");
+                        st[1 + 2 * i] = (uint8_t)S.peek_static(position, left, &v[1 + 2 * i]);
+                        st[2 + 2 * i] = (uint8_t)S.peek_static(position + i + 1, right, &v[2 + 2 * i]);
+                    }
+                } catch (const std::exception&) {  // the ordered walk meets the same error itself
+                }
+            }
+        };
+        const size_t nt = n < 2048 ? 1 : std::min<size_t>(8, n / 1024);
+        std::vector<std::thread> th;
+        for (size_t t = 1; t < nt; t++) th.emplace_back(work, n * t / nt, n * (t + 1) / nt);
+        work(0, n / nt);
+        for (auto& x : th) x.join();
+    }
+
     void adjustMNP() {
         std::vector<Cand> tmp = sorted_candidates(S.P.mnp);
         st.candidates += (int64_t)tmp.size();
+        MnpPre pre;
+        mnp_pre(tmp, pre);
         int ci = -1;
         for (const Cand& t : tmp) {
             ci++;
+            if (ci + 8 < (int)tmp.size()) {
+                if (const HSclip* f = pre.s3[(size_t)ci + 8]) __builtin_prefetch(f);
+                if (const HSclip* f = pre.s5[(size_t)ci + 8]) __builtin_prefetch(f);
+            }
             if (skip(SC_MNP, ci)) continue;
             Visit visit(*this, SC_MNP, ci);
+            // peek() through the resolved reads: the position's entries once at() has loaded it
+            const size_t q0 = pre.off[(size_t)ci];
+            auto look = [&](int p, const std::string& d, size_t q) -> const HVar* {
+                bool hit;
+                const HVar* r = S.peek_loaded(p, d, &hit);
+                if (hit) return r;
+                if (pre.state[q] == 2) return S.peek(p, d);
+                return pre.state[q] ? &pre.v[q] : nullptr;
+            };
             try {
                 int position = t.position;
                 const std::string vn = t.desc;
@@ -796,7 +897,7 @@ struct Realigner {
                 // through peek() and materialized only when an entry is taken over: the output
                 // does not depend on it (the candidate's position is a non-dense one, which the
                 // record list always carries).
-                const HVar* v0 = S.peek(position, vn);
+                const HVar* v0 = look(position, vn, q0);
                 if (!v0) continue;  // no key at the position, or no such descriptor
                 HVar vcur = *v0;    // vref's value until the position is materialized
                 HVar* vref = nullptr;
@@ -815,7 +916,7 @@ struct Realigner {
                     if (left.size() > 1) left.insert(1, "&");
                     std::string right = sub1(mnt, -(int)(mnt.size() - (size_t)i - 1));
                     if (right.size() > 1) right.insert(1, "&");
-                    const HVar* lt = S.peek(position, left);
+                    const HVar* lt = look(position, left, q0 + 1 + 2 * (size_t)i);
                     if (lt && std::find(used_left.begin(), used_left.end(), left) == used_left.end()) {
                         if (lt->cnt <= 0) continue;
                         if (lt->cnt < vcnt() && lt->pmean / lt->cnt <= i + 1) {
@@ -824,7 +925,7 @@ struct Realigner {
                             used_left.push_back(left);
                         }
                     }
-                    const HVar* rt = S.peek(position + i + 1, right);
+                    const HVar* rt = look(position + i + 1, right, q0 + 2 + 2 * (size_t)i);
                     if (rt) {
                         if (rt->cnt < 0) continue;
                         if (rt->cnt < vcnt()) {
@@ -838,7 +939,7 @@ struct Realigner {
                         }
                     }
                 }
-                if (HSclip* s3 = sc3(position)) {
+                if (HSclip* s3 = pre.s3[(size_t)ci]) {
                     if (!s3->used) {
                         std::string seq = findconseq(*s3);
                         if (sub2(seq, 0, (int)mnt.size()) == mnt) {
@@ -854,7 +955,7 @@ struct Realigner {
                         }
                     }
                 }
-                if (HSclip* s5 = sc5(position + (int)mnt.size())) {
+                if (HSclip* s5 = pre.s5[(size_t)ci]) {
                     if (!s5->used) {
                         std::string seq = findconseq(*s5);
                         if (seq != " " && seq.size() >= mnt.size()) {

@@ -56,6 +56,15 @@ class RowCache {
     // the `exact` positions
     bool prefetch(const std::vector<int>& positions, int radius, const std::vector<int>& exact = {});
     const DenseRow* row(int pos);  // nullptr outside the dense window
+    // row() without fetching: false when the position's row is in the window but not loaded
+    bool row_loaded(int pos, const DenseRow*& out) const {
+        const int i = pos - lo_;
+        out = nullptr;
+        if (i < 0 || i >= n_) return true;
+        if (!have(i)) return false;
+        out = (*slots_)[(size_t)i];
+        return true;
+    }
     void release_slots();
     int64_t fetches = 0, rows_fetched = 0;
     int64_t miss_fetches = 0;
@@ -167,6 +176,13 @@ class RState {
     // nonInsertionVariants[pos][d] as at() would find it, without materializing the position
     // (valid until the next call); nullptr when absent
     const HVar* peek(int pos, const std::string& d);
+    // peek() split in two.  peek_loaded: the position's materialized entries (*hit false when
+    // at() has not loaded it).  peek_static: what peek() returns for a position at() has not
+    // loaded, from the dense rows and the non-dense map alone, without fetching and without
+    // writing anything (safe on several threads): 0 absent, 1 found (copied to *out), 2 unknown
+    // (the position's dense row is not loaded).
+    const HVar* peek_loaded(int pos, const std::string& d, bool* hit) const;
+    int peek_static(int pos, const std::string& d, HVar* out) const;
     void erase_inner(int pos, const std::string& d, bool drop_empty);
     void add_cov(int pos, int k);
     HVar* new_var();
