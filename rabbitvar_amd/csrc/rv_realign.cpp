@@ -1486,29 +1486,118 @@ void build_record_list(RState& S, CxHost& cx) {
         if (e.second.dirty) ps.push_back(e.first);
     std::sort(ps.begin(), ps.end());
     ps.erase(std::unique(ps.begin(), ps.end()), ps.end());
-    S.rows.prefetch(ps, 0);
-    cx.off.push_back(0);
-    auto add = [&](int kind, const std::string& d, const HVar& v) {
-        CxEntry c;
-        c.kind = kind;
-        c.desc_off = (uint32_t)cx.arena.size();
-        c.desc_len = (int32_t)d.size();
-        c.pad = 0;
-        c.v = to_dev(v);
-        cx.arena += d;
-        cx.ent.push_back(c);
+    if (!S.rows.prefetch(ps, 0)) throw std::runtime_error("dense row fetch failed");
+    // A position's entries are read, never materialized: one that at() has loaded gives its own;
+    // any other gives what at() would load, the dense row's flagged bases and the non-dense
+    // map's descriptors merged in the map's (string) order, the non-dense entry taking a key
+    // both hold.  Nothing is written to the store, so ranges of positions are listed on several
+    // threads and joined in order.
+    const size_t n = ps.size();
+    const size_t nt = n < 4096 ? 1 : std::min<size_t>(8, n / 2048);
+    std::vector<CxHost> part(nt);
+    std::vector<char> missing(nt, 0);
+    auto work = [&](size_t t) {
+        CxHost& o = part[t];
+        const size_t a = n * t / nt, b = n * (t + 1) / nt;
+        o.pos.reserve(b - a);
+        o.cov.reserve(b - a);
+        o.key.reserve(b - a);
+        o.off.reserve(b - a);
+        auto add = [&](int kind, const std::string& d, const HVar& v) {
+            CxEntry c;
+            c.kind = kind;
+            c.desc_off = (uint32_t)o.arena.size();
+            c.desc_len = (int32_t)d.size();
+            c.pad = 0;
+            c.v = to_dev(v);
+            o.arena += d;
+            o.ent.push_back(c);
+        };
+        std::string one(1, ' ');
+        for (size_t k = a; k < b; k++) {
+            const int p = ps[k];
+            o.pos.push_back(p);
+            auto lp = S.pos.find(p);
+            if (lp != S.pos.end() && lp->second.loaded) {
+                const RPos& r = lp->second;
+                o.cov.push_back(r.cov);
+                o.key.push_back(r.key ? 1 : 0);
+                if (r.key)
+                    for (auto& kv : r.v) add(0, kv.first, *kv.second);
+            } else {
+                const DenseRow* dr;
+                if (!S.rows.row_loaded(p, dr)) {
+                    missing[t] = 1;
+                    return;
+                }
+                const auto* nd = S.nd_ix.find(p);
+                const bool key = (dr && (dr->rowflags & 1u)) || nd != nullptr;
+                o.cov.push_back(dr ? dr->cov : 0);
+                o.key.push_back(key ? 1 : 0);
+                if (key) {
+                    // the flagged bases as single-character keys, in string order
+                    char db[4];
+                    int nb = 0;
+                    if (dr)
+                        for (int x = 0; x < 4; x++)
+                            if (dr->b[x].flags & 1u) db[nb++] = index_base(x);
+                    std::sort(db, db + nb);
+                    int i = 0;
+                    auto dense_out = [&](char c) {
+                        for (int x = 0; x < 4; x++)
+                            if (index_base(x) == c) {
+                                one[0] = c;
+                                add(0, one, dense_to_hvar(dr->b[x]));
+                                return;
+                            }
+                    };
+                    if (nd)
+                        for (auto& kv : *nd) {
+                            for (; i < nb && std::string(1, db[i]) < kv.first; i++) dense_out(db[i]);
+                            if (i < nb && kv.first.size() == 1 && kv.first[0] == db[i]) i++;  // the map's
+                            add(0, kv.first, kv.second);
+                        }
+                    for (; i < nb; i++) dense_out(db[i]);
+                }
+            }
+            auto it = S.ins.find(p);
+            if (it != S.ins.end())
+                for (auto& kv : it->second) add(1, kv.first, *kv.second);
+            o.off.push_back((int32_t)o.ent.size());
+        }
     };
-    for (int p : ps) {
-        RPos& r = S.at(p);
-        cx.pos.push_back(p);
-        cx.cov.push_back(r.cov);
-        cx.key.push_back(r.key ? 1 : 0);
-        if (r.key)
-            for (auto& kv : r.v) add(0, kv.first, *kv.second);
-        auto it = S.ins.find(p);
-        if (it != S.ins.end())
-            for (auto& kv : it->second) add(1, kv.first, *kv.second);
-        cx.off.push_back((int32_t)cx.ent.size());
+    {
+        std::vector<std::thread> th;
+        for (size_t t = 1; t < nt; t++) th.emplace_back(work, t);
+        work(0);
+        for (auto& x : th) x.join();
+    }
+    for (char m : missing)
+        if (m) throw std::runtime_error("dense row not loaded for the record list");
+    size_t ne = 0, na = 0;
+    for (auto& o : part) {
+        ne += o.ent.size();
+        na += o.arena.size();
+    }
+    cx.pos.reserve(n);
+    cx.cov.reserve(n);
+    cx.key.reserve(n);
+    cx.off.reserve(n + 1);
+    cx.ent.reserve(ne);
+    cx.arena.reserve(na);
+    cx.off.push_back(0);
+    for (auto& o : part) {
+        const int32_t eb = (int32_t)cx.ent.size();
+        const uint32_t ab = (uint32_t)cx.arena.size();
+        cx.pos.insert(cx.pos.end(), o.pos.begin(), o.pos.end());
+        cx.cov.insert(cx.cov.end(), o.cov.begin(), o.cov.end());
+        cx.key.insert(cx.key.end(), o.key.begin(), o.key.end());
+        for (int32_t x : o.off) cx.off.push_back(eb + x);
+        for (CxEntry c : o.ent) {
+            c.desc_off += ab;
+            cx.ent.push_back(c);
+        }
+        cx.arena += o.arena;
     }
 }
 
