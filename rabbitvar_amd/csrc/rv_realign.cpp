@@ -117,8 +117,8 @@ char HostRef::get(int k) {
 // ------------------------------------------------------------------ position store
 RState::RState(HostPileup& P_, RowCache& rc) : P(P_), rows(rc) {
     // the three position indexes are independent: the clip maps' on their own threads
-    std::thread t5([this] { sc5_ix.build(P.sc5); });
-    std::thread t3([this] { sc3_ix.build(P.sc3); });
+    std::thread t5([this] { sc5_ix.build(P.sc5, 5); });
+    std::thread t3([this] { sc3_ix.build(P.sc3, 5); });
     struct Join {
         std::thread &a, &b;
         ~Join() {
@@ -126,7 +126,7 @@ RState::RState(HostPileup& P_, RowCache& rc) : P(P_), rows(rc) {
             b.join();
         }
     } join{t5, t3};
-    nd_ix.build(P.nd);
+    nd_ix.build(P.nd, 5);
     for (auto& e : P.ins)
         for (auto& kv : e.second) {
             HVar* v = new_var();

@@ -11,10 +11,12 @@
 #include <cstdint>
 #include <deque>
 #include <functional>
+#include <iterator>
 #include <map>
 #include <memory>
 #include <set>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -124,19 +126,33 @@ class ReadQuery {
 template <class V>
 class PosIndex {
   public:
+    // m: an ordered map keyed by position.  With threads > 1 the map's nodes (scattered over the
+    // heap) are walked on that many threads, each over its own key range, and the entries then
+    // inserted in one pass; lookups do not depend on the insertion order.
     template <class M>
-    void build(M& m) {
+    void build(M& m, unsigned threads = 1) {
         size_t cap = 16;
         while (cap < 2 * m.size() + 1) cap *= 2;
         keys_.assign(cap, EMPTY);
         vals_.assign(cap, nullptr);
         mask_ = cap - 1;
-        for (auto& kv : m) {
-            size_t i = slot(kv.first);
-            while (keys_[i] != EMPTY) i = (i + 1) & mask_;
-            keys_[i] = kv.first;
-            vals_[i] = &kv.second;
+        if (threads <= 1 || m.size() < 4096) {
+            for (auto& kv : m) put(kv.first, &kv.second);
+            return;
         }
+        const int64_t lo = m.begin()->first, span = (int64_t)std::prev(m.end())->first - lo + 1;
+        std::vector<std::vector<std::pair<int, V*>>> part(threads);
+        auto walk = [&](unsigned t) {
+            auto a = m.lower_bound((int)(lo + span * t / threads));
+            auto b = t + 1 == threads ? m.end() : m.lower_bound((int)(lo + span * (t + 1) / threads));
+            for (; a != b; ++a) part[t].emplace_back(a->first, &a->second);
+        };
+        std::vector<std::thread> th;
+        for (unsigned t = 1; t < threads; t++) th.emplace_back(walk, t);
+        walk(0);
+        for (auto& x : th) x.join();
+        for (auto& pt : part)
+            for (auto& kv : pt) put(kv.first, kv.second);
     }
     V* find(int p) const {
         if (keys_.empty()) return nullptr;
@@ -147,6 +163,12 @@ class PosIndex {
     }
 
   private:
+    void put(int p, V* v) {
+        size_t i = slot(p);
+        while (keys_[i] != EMPTY) i = (i + 1) & mask_;
+        keys_[i] = p;
+        vals_[i] = v;
+    }
     static constexpr int EMPTY = INT32_MIN;  // never a position
     size_t slot(int p) const { return (size_t)(((uint32_t)p * 0x9E3779B1u) >> 7) & mask_; }
     std::vector<int> keys_;
