@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <iterator>
 #include <map>
 #include <numeric>
 #include <stdexcept>
@@ -169,10 +170,49 @@ HVar dense_to_hvar(const DenseBase& d) {
     return v;
 }
 
+namespace {
+// Splits elements into `T` position ranges holding about equal numbers of them (cut points from a
+// sample of their positions): out holds range 0's elements, then range 1's, ..., each range in
+// input order; seg (T + 1 entries) gives where each range starts.  No position spans two ranges.
+template <class E, class F>
+void range_scatter(const std::vector<E>& in, F pos_of, unsigned T, std::vector<E>& out, std::vector<size_t>& seg) {
+    const size_t n = in.size();
+    seg.assign((size_t)T + 1, 0);
+    if (T <= 1) {
+        out = in;
+        seg[1] = n;
+        return;
+    }
+    std::vector<int> smp;
+    const size_t step = std::max<size_t>(1, n / (64 * (size_t)T));
+    for (size_t i = 0; i < n; i += step) smp.push_back(pos_of(in[i]));
+    std::sort(smp.begin(), smp.end());
+    std::vector<int> cut(T - 1);
+    for (unsigned t = 1; t < T; t++) cut[t - 1] = smp[smp.size() * t / T];
+    std::vector<uint8_t> bk(n);
+    for (size_t i = 0; i < n; i++) {
+        const int b = (int)(std::upper_bound(cut.begin(), cut.end(), pos_of(in[i])) - cut.begin());
+        bk[i] = (uint8_t)b;
+        seg[(size_t)b + 1]++;
+    }
+    for (unsigned t = 0; t < T; t++) seg[t + 1] += seg[t];
+    std::vector<size_t> at(seg.begin(), seg.end() - 1);
+    out.resize(n);
+    for (size_t i = 0; i < n; i++) out[at[bk[i]]++] = in[i];
+}
+
+unsigned reduce_threads(size_t n) {
+    return n < 8192 ? 1u : std::min(8u, std::max(1u, std::thread::hardware_concurrency() / 4));
+}
+
+template <class M>
+void move_nodes(M& src, M& dst) {  // src's keys all follow dst's
+    while (!src.empty()) dst.insert(dst.end(), src.extract(src.begin()));
+}
+}  // namespace
+
 void reduce_events(const Event* ev, size_t n, const char* arena, double goodq, HostPileup& P) {
     (void)goodq;
-    std::vector<size_t> idx(n);
-    std::iota(idx.begin(), idx.end(), 0);
     auto cls = [&](const Event& e) { return e.kind == 1 ? 1 : 0; };
     auto cmp = [&](size_t a, size_t b) {
         const Event& x = ev[a];
@@ -186,118 +226,131 @@ void reduce_events(const Event* ev, size_t n, const char* arena, double goodq, H
         if (x.ord != y.ord) return x.ord < y.ord;
         return x.seq < y.seq;
     };
-    std::sort(idx.begin(), idx.end(), cmp);
-    size_t i = 0;
-    while (i < n) {
-        const Event& h = ev[idx[i]];
-        std::string desc(arena + h.desc_off, h.desc_len);
-        int c = cls(h);
-        size_t j = i;
-        HVar v;
-        bool created = false;
-        for (; j < n; j++) {
-            const Event& e = ev[idx[j]];
-            if (cls(e) != c || e.pos != h.pos || e.desc_len != h.desc_len ||
-                memcmp(arena + e.desc_off, arena + h.desc_off, h.desc_len) != 0)
-                break;
-            if (e.kind == 3) {  // anchor subtraction against an existing entry
-                if (!created) continue;
-                v.cnt--;
-                if (e.dir)
-                    v.rev--;
-                else
-                    v.fwd--;
-                v.pmean -= e.tp;
-                v.qmean -= e.q;
-                v.mqmean -= e.mapq;
-                v.nm -= e.nmv;
-                if (e.hiq)
-                    v.hicnt--;
-                else
-                    v.locnt--;
-                continue;
-            }
-            created = true;
-            if (e.kind == 2) {  // addCnt form
-                v.cnt++;
+    // A group (map, position, descriptor) never spans positions: position ranges are sorted on
+    // separate threads, then reduced in range order on this one (the maps' nodes are allocated
+    // by one thread, as the realigner and the release threads prefer).
+    const unsigned T = reduce_threads(n);
+    std::vector<size_t> all(n), idx, seg;
+    std::iota(all.begin(), all.end(), 0);
+    range_scatter(all, [&](size_t i) { return ev[i].pos; }, T, idx, seg);
+    all = std::vector<size_t>();
+    {
+        std::vector<std::thread> pool;
+        for (unsigned t = 1; t < T; t++)
+            pool.emplace_back([&, t] { std::sort(idx.begin() + (long)seg[t], idx.begin() + (long)seg[t + 1], cmp); });
+        std::sort(idx.begin() + (long)seg[0], idx.begin() + (long)seg[1], cmp);
+        for (auto& th : pool) th.join();
+    }
+    {
+        auto& nd = P.nd;
+        auto& ins = P.ins;
+        auto& del = P.delCount;
+        auto& insc = P.insCount;
+        auto& mnp = P.mnp;
+        const size_t hi = n;
+        size_t i = 0;
+        while (i < hi) {
+            const Event& h = ev[idx[i]];
+            std::string desc(arena + h.desc_off, h.desc_len);
+            int c = cls(h);
+            size_t j = i;
+            HVar v;
+            bool created = false;
+            int ndel = 0, nins = 0, nmnp = 0;
+            for (; j < hi; j++) {
+                const Event& e = ev[idx[j]];
+                if (cls(e) != c || e.pos != h.pos || e.desc_len != h.desc_len ||
+                    memcmp(arena + e.desc_off, arena + h.desc_off, h.desc_len) != 0)
+                    break;
+                if (e.kind == 3) {  // anchor subtraction against an existing entry
+                    if (!created) continue;
+                    v.cnt--;
+                    if (e.dir)
+                        v.rev--;
+                    else
+                        v.fwd--;
+                    v.pmean -= e.tp;
+                    v.qmean -= e.q;
+                    v.mqmean -= e.mapq;
+                    v.nm -= e.nmv;
+                    if (e.hiq)
+                        v.hicnt--;
+                    else
+                        v.locnt--;
+                    continue;
+                }
+                created = true;
+                if (e.kind == 2) {  // addCnt form
+                    v.cnt++;
+                    if (e.dir)
+                        v.rev++;
+                    else
+                        v.fwd++;
+                    v.pmean += e.rp;
+                    v.qmean += e.q;
+                    v.mqmean += e.mapq;
+                    v.nm += e.nmv;
+                    if (e.hiq)
+                        v.hicnt++;
+                    else
+                        v.locnt++;
+                    continue;
+                }
+                // match-run / deletion / insertion form (pp/pq running comparison)
                 if (e.dir)
                     v.rev++;
                 else
                     v.fwd++;
-                v.pmean += e.rp;
+                v.cnt++;
+                if (!v.pstd && v.pp != 0 && e.tp != v.pp) v.pstd = true;
+                if (!v.qstd && v.pq != 0 && e.q != v.pq) v.qstd = true;
+                v.pmean += e.tp;
                 v.qmean += e.q;
                 v.mqmean += e.mapq;
+                v.pp = e.tp;
+                v.pq = e.q;
                 v.nm += e.nmv;
                 if (e.hiq)
                     v.hicnt++;
                 else
                     v.locnt++;
-                continue;
+                ndel += e.cntflags & 1;
+                nins += (e.cntflags >> 1) & 1;
+                nmnp += (e.cntflags >> 2) & 1;
             }
-            // match-run / deletion / insertion form (pp/pq running comparison)
-            if (e.dir)
-                v.rev++;
-            else
-                v.fwd++;
-            v.cnt++;
-            if (!v.pstd && v.pp != 0 && e.tp != v.pp) v.pstd = true;
-            if (!v.qstd && v.pq != 0 && e.q != v.pq) v.qstd = true;
-            v.pmean += e.tp;
-            v.qmean += e.q;
-            v.mqmean += e.mapq;
-            v.pp = e.tp;
-            v.pq = e.q;
-            v.nm += e.nmv;
-            if (e.hiq)
-                v.hicnt++;
-            else
-                v.locnt++;
-            if (e.cntflags & 1) P.delCount[h.pos][desc]++;
-            if (e.cntflags & 2) P.insCount[h.pos][desc]++;
-            if (e.cntflags & 4) P.mnp[h.pos][desc]++;
+            // the counters grow once per counted record; the group's total is added at its end
+            if (ndel) del[h.pos][desc] += ndel;
+            if (nins) insc[h.pos][desc] += nins;
+            if (nmnp) mnp[h.pos][desc] += nmnp;
+            if (created) {
+                if (c == 1)
+                    ins[h.pos][desc] = v;
+                else
+                    nd[h.pos][desc] = v;
+            }
+            i = j;
         }
-        if (created) {
-            if (c == 1)
-                P.ins[h.pos][desc] = v;
-            else
-                P.nd[h.pos][desc] = v;
-        }
-        i = j;
     }
 }
 
 void reduce_clips(const ClipRec* cl, size_t n, const char* clip_arena, double goodq, HostPileup& P, int side) {
     // Records are reduced per position in read order.  Sort keys: (side, position, read ordinal),
     // packed in 64 bits (a read has at most one clip per side, so keys are distinct).
-    std::vector<std::pair<uint64_t, uint32_t>> keys;
-    keys.reserve(n);
+    using Key = std::pair<uint64_t, uint32_t>;
+    std::vector<Key> all;
+    all.reserve(n);
     for (size_t i = 0; i < n; i++)
         if (side < 0 || cl[i].side == side)
-            keys.push_back({((uint64_t)(cl[i].side == 5) << 63) | ((uint64_t)((uint32_t)cl[i].pos ^ 0x80000000u) << 31) |
-                                (uint64_t)(uint32_t)cl[i].ord,
-                            (uint32_t)i});
-    const size_t m = keys.size();
-    const unsigned T = m < 4096 ? 1u : std::min(8u, std::max(1u, std::thread::hardware_concurrency() / 4));
-    // sort: chunks in parallel, then pairwise merges
-    {
-        std::vector<size_t> cut(T + 1);
-        for (unsigned t = 0; t <= T; t++) cut[t] = m * t / T;
-        std::vector<std::thread> pool;
-        for (unsigned t = 1; t < T; t++)
-            pool.emplace_back([&, t] { std::sort(keys.begin() + (long)cut[t], keys.begin() + (long)cut[t + 1]); });
-        std::sort(keys.begin() + (long)cut[0], keys.begin() + (long)cut[1]);
-        for (auto& th : pool) th.join();
-        for (size_t w = 1; w < T; w *= 2) {
-            std::vector<std::thread> mp;
-            for (size_t a = 0; a + w < T; a += 2 * w) {
-                const size_t lo = cut[a], mid = cut[a + w], hi = cut[std::min<size_t>(a + 2 * w, T)];
-                mp.emplace_back([&, lo, mid, hi] {
-                    std::inplace_merge(keys.begin() + (long)lo, keys.begin() + (long)mid, keys.begin() + (long)hi);
-                });
-            }
-            for (auto& th : mp) th.join();
-        }
-    }
+            all.push_back({((uint64_t)(cl[i].side == 5) << 63) | ((uint64_t)((uint32_t)cl[i].pos ^ 0x80000000u) << 31) |
+                               (uint64_t)(uint32_t)cl[i].ord,
+                           (uint32_t)i});
+    const unsigned T = reduce_threads(all.size());
+    // position ranges, each sorted and accumulated on its own thread into maps of its own, whose
+    // nodes then move into the pileup's maps in order
+    std::vector<Key> keys;
+    std::vector<size_t> cut;
+    range_scatter(all, [](const Key& k) { return (int)((uint32_t)(k.first >> 31) ^ 0x80000000u); }, T, keys, cut);
+    all = std::vector<Key>();
     const double gq = (double)(float)goodq;
     auto add = [&](HVar& v, bool dir, int rp, double q, int mapq, int nm) {
         v.cnt++;
@@ -314,19 +367,14 @@ void reduce_clips(const ClipRec* cl, size_t n, const char* clip_arena, double go
         else
             v.locnt++;
     };
-    // accumulate position ranges on separate threads (no position spans two ranges), each into its
-    // own map, then move the nodes into the pileup's maps in order
-    auto same_pos = [&](size_t a, size_t b) { return (keys[a].first >> 31) == (keys[b].first >> 31); };
-    std::vector<size_t> cut(T + 1);
-    cut[0] = 0;
-    cut[T] = m;
-    for (unsigned t = 1; t < T; t++) {
-        size_t c = std::max(cut[t - 1], m * t / T);
-        while (c > cut[t - 1] && c < m && same_pos(c - 1, c)) c++;
-        cut[t] = c;
-    }
+    struct OrdKey {
+        int cnt, pos;
+        HSclip* sc;
+    };
     std::vector<std::map<int, HSclip>> part5(T), part3(T);
+    std::vector<std::vector<OrdKey>> ord5(T), ord3(T);
     auto run = [&](unsigned t) {
+        std::sort(keys.begin() + (long)cut[t], keys.begin() + (long)cut[t + 1]);
         for (size_t q = cut[t]; q < cut[t + 1]; q++) {
             const ClipRec& r = cl[keys[q].second];
             auto& mp = r.side == 5 ? part5[t] : part3[t];
@@ -352,12 +400,18 @@ void reduce_clips(const ClipRec* cl, size_t n, const char* clip_arena, double go
             add(sc.v, r.dir != 0, r.cel, r.sumq / (double)r.nhq, r.mapq, r.nm);
         }
         // the clusters are complete: their consensus, here on this range's thread rather than
-        // one by one in the realigner
-        for (auto* mp : {&part5[t], &part3[t]})
-            for (auto& e : *mp) {
+        // one by one in the realigner, and their order keys (the nodes keep their addresses when
+        // they move)
+        for (int s5 = 0; s5 < 2; s5++) {
+            auto& mp = s5 ? part5[t] : part3[t];
+            auto& ol = s5 ? ord5[t] : ord3[t];
+            ol.reserve(mp.size());
+            for (auto& e : mp) {
                 e.second.pre = clip_consensus(e.second, e.second.pre_used);
                 e.second.pre_valid = true;
+                ol.push_back({e.second.v.cnt, e.first, &e.second});
             }
+        }
     };
     {
         std::vector<std::thread> pool;
@@ -366,16 +420,30 @@ void reduce_clips(const ClipRec* cl, size_t n, const char* clip_arena, double go
         for (auto& th : pool) th.join();
     }
     for (unsigned t = 0; t < T; t++) {
-        for (auto* pr : {&part5[t], &part3[t]}) {
-            auto& dst = pr == &part5[t] ? P.sc5 : P.sc3;
-            while (!pr->empty()) dst.insert(dst.end(), pr->extract(pr->begin()));
-        }
+        move_nodes(part5[t], P.sc5);
+        move_nodes(part3[t], P.sc3);
     }
     for (int s5 = 0; s5 < 2; s5++) {
         if (side >= 0 && side != (s5 ? 5 : 3)) continue;
         auto& m = s5 ? P.sc5 : P.sc3;
         auto& ord = s5 ? P.sc5_order : P.sc3_order;
-        clip_order(m, ord);
+        // count descending, position ascending (clip_order's order), from the keys
+        size_t tot = 0;
+        for (auto& ol : s5 ? ord5 : ord3) tot += ol.size();
+        if (tot != m.size()) {  // the map held clusters before this call
+            clip_order(m, ord);
+        } else {
+            std::vector<OrdKey> ok;
+            ok.reserve(tot);
+            for (auto& ol : s5 ? ord5 : ord3) ok.insert(ok.end(), ol.begin(), ol.end());
+            std::sort(ok.begin(), ok.end(), [](const OrdKey& a, const OrdKey& b) {
+                if (a.cnt != b.cnt) return a.cnt > b.cnt;
+                return a.pos < b.pos;
+            });
+            ord.clear();
+            ord.reserve(tot);
+            for (auto& x : ok) ord.push_back({x.pos, x.sc});
+        }
         (s5 ? P.sc5_of : P.sc3_of) = &m;
     }
 }
