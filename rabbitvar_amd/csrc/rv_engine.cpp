@@ -90,7 +90,7 @@ Engine::~Engine() {
                      &b_out_arena, &b_score_ctr, &b_status, &b_keys, &b_keys2, &b_kpos, &b_kpos2, &b_sort_tmp,
                      &b_kcount, &b_args, &b_kpos3, &b_nopass, &b_rows2, &b_want, &b_dup, &b_drop,
                      &b_scr_cand, &b_scr_pos, &b_scr_cum, &b_scr_feat, &b_scr_score, &b_scr_params,
-                     &b_fisher_in, &b_fisher_out, &b_lgamma, &b_f35_pairs, &b_f35_arena, &b_f35_out, &b_pick,
+                     &b_fisher_in, &b_fisher_out, &b_lgamma, &b_f35_pairs, &b_f35_arena, &b_f35_out, &b_pick, &b_pre,
                      &b_so_ops, &b_so_tr, &b_so_ord[0], &b_so_ord[1], &b_so_k[0], &b_so_k[1], &b_so_k[2], &b_so_k[3],
                      &b_so_w[0], &b_so_w[1], &b_so_w[2], &b_so_w[3], &b_so_runs, &b_so_small, &b_so_tmp,
                      &b_so_proc, &b_so_rank, &b_so_rk[0], &b_so_rk[1], &b_so_rk[2], &b_so_rk[3]};

@@ -223,7 +223,7 @@ class Engine {
     DevBuf b_gather_ids, b_gather_out, b_cx_pos, b_cx_off, b_cx, b_cx_arena, b_cx_cov, b_cx_key, b_out, b_out_arena,
         b_score_ctr, b_status, b_keys, b_keys2, b_kpos, b_kpos2, b_sort_tmp, b_kcount, b_kpos3, b_args, b_nopass,
         b_want, b_dup, b_drop, b_scr_cand, b_scr_pos, b_scr_cum, b_scr_feat, b_scr_score, b_fisher_in, b_fisher_out,
-        b_lgamma, b_f35_pairs, b_f35_arena, b_f35_out, b_pick, b_so_ops, b_so_tr, b_so_ord[2], b_so_k[4], b_so_w[4],
+        b_lgamma, b_f35_pairs, b_f35_arena, b_f35_out, b_pick, b_pre, b_so_ops, b_so_tr, b_so_ord[2], b_so_k[4], b_so_w[4],
         b_so_runs, b_so_small, b_so_tmp, b_so_proc, b_so_rank, b_so_rk[4];
     WalkArgs* args_dev = nullptr;  // the last pileup's kernel arguments (its reads stay resident)
     int64_t args_n = 0;

@@ -84,6 +84,9 @@ hipError_t launch_max_displacement(const int32_t* keys, int n, int cap_log2, int
                                    void* temp, size_t& temp_bytes, hipStream_t s);
 
 hipError_t launch_score(const ScoreIn& S, hipStream_t s);
+// k_score_prepick into S.pick_list / S.pick_count (dense_n entries, then the count); S needs the
+// rows, the reference window and the configuration only
+hipError_t launch_score_prepick(const ScoreIn& S, hipStream_t s);
 hipError_t launch_fisher_table(const int* tab, int n, double* out, const double* lgt, int lgt_n, hipStream_t s);
 // the allocation-free variant's whole call (host tables in, host p-values out; rv_score.hip)
 constexpr int FISHER_NOALLOC_MAX_TABLES = 1 << 16;

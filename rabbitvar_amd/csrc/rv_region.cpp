@@ -187,6 +187,8 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     double prefetch_ms = 0;
     std::exception_ptr prefetch_err;
     std::thread prefetch_th;
+    const int32_t* pre_list = nullptr;  // k_score_prepick's list and count, when it was launched
+    const unsigned* pre_count = nullptr;
     // realignment screen: its MNV / deletion / insertion candidates are listed while the clips
     // are still being reduced
     const bool screening = e.screen_on || e.screen_collect;
@@ -225,6 +227,27 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
                 const auto f0 = std::chrono::steady_clock::now();
                 prefetch_ok = realign_prefetch(e.cfg, ref, contig.length, *hp, *rcp);
                 prefetch_ms = ms_since(f0);
+                // the scoring's row checks (the realigner changes no dense row on the device) run
+                // on the device while the realigner works on the host
+                if (prefetch_ok && hp->dense_n > 0) {
+                    ScoreIn q{};
+                    q.rows = e.rows;
+                    q.dense_lo = hp->dense_lo;
+                    q.dense_n = hp->dense_n;
+                    q.ref = e.ref_dev;
+                    q.ref_lo = ref.lo;
+                    q.ref_hi = ref.hi;
+                    q.cfg = e.cfg;
+                    const size_t list_n = ((size_t)hp->dense_n + 63) & ~(size_t)63;
+                    q.pick_list = (int32_t*)e.grow(e.b_pre, sizeof(int32_t) * (list_n + 64));
+                    if (q.pick_list) {
+                        q.pick_count = (unsigned*)(q.pick_list + list_n);
+                        if (launch_score_prepick(q, e.stream) == hipSuccess) {
+                            pre_list = q.pick_list;
+                            pre_count = q.pick_count;
+                        }
+                    }
+                }
             } catch (...) {
                 prefetch_err = std::current_exception();
             }
@@ -324,6 +347,8 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     in.rows = e.rows;
     in.dense_lo = pr.P.dense_lo;
     in.dense_n = pr.P.dense_n;
+    in.pre_list = pre_list;
+    in.pre_count = pre_count;
     in.n_cx = ncx;
     in.ref = e.ref_dev;
     in.ref_lo = ref.lo;

@@ -87,6 +87,12 @@ struct ScoreIn {
     // the dense kernel visits every dense position
     int32_t* pick_list;
     unsigned* pick_count;
+    // the dense positions that pass every check but the record-list one, listed by
+    // k_score_prepick while the realigner runs (those checks read only the pileup's rows and the
+    // reference); with it, k_score_filter drops the record-list positions from this list instead
+    // of k_score_pick reading every row again.  nullptr: k_score_pick
+    const int32_t* pre_list;
+    const unsigned* pre_count;
 };
 
 }  // namespace rv

@@ -29,6 +29,38 @@ __global__ void __launch_bounds__(256) k_score_pick(ScoreIn S) {
     if (need) S.pick_list[base + (unsigned)__popcll(m & ((1ull << lane) - 1ull))] = p;
 }
 
+// k_score_pick's row checks alone (no record list yet), into S.pick_list / S.pick_count
+__global__ void __launch_bounds__(256) k_score_prepick(ScoreIn S) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool need = false;
+    const int p = S.dense_lo + i;
+    if (i < S.dense_n) need = dense_row_needed(S, p);
+    const unsigned long long m = __ballot(need);
+    const int lane = (int)(threadIdx.x & 63u);
+    unsigned base = 0;
+    if (lane == 0 && m) base = atomicAdd(S.pick_count, (unsigned)__popcll(m));
+    base = __shfl(base, 0);
+    if (need) S.pick_list[base + (unsigned)__popcll(m & ((1ull << lane) - 1ull))] = p;
+}
+
+// k_score_pick from the early list: its positions the record list does not carry
+__global__ void __launch_bounds__(256) k_score_filter(ScoreIn S) {
+    const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned total = *S.pre_count;
+    bool need = false;
+    int p = 0;
+    if (i < total) {
+        p = S.pre_list[i];
+        need = cx_find(S, p) < 0;
+    }
+    const unsigned long long m = __ballot(need);
+    const int lane = (int)(threadIdx.x & 63u);
+    unsigned base = 0;
+    if (lane == 0 && m) base = atomicAdd(S.pick_count, (unsigned)__popcll(m));
+    base = __shfl(base, 0);
+    if (need) S.pick_list[base + (unsigned)__popcll(m & ((1ull << lane) - 1ull))] = p;
+}
+
 // n: the number of dense positions (the upper bound of the picked count).
 __global__ void __launch_bounds__(128) k_score_dense(ScoreIn S, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -67,13 +99,27 @@ hipError_t launch_score(const ScoreIn& S, hipStream_t s) {
             if (!S.pick_count) return hipErrorInvalidValue;
             // the count starts at zero with a fill kernel (no memset node if a caller captures this)
             hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, S.pick_count, 1);
-            hipLaunchKernelGGL(k_score_pick, dim3((S.dense_n + 255) / 256), dim3(256), 0, s, S);
+            if (S.pre_list) {
+                if (!S.pre_count) return hipErrorInvalidValue;
+                // the early list holds at most dense_n positions
+                hipLaunchKernelGGL(k_score_filter, dim3((S.dense_n + 255) / 256), dim3(256), 0, s, S);
+            } else {
+                hipLaunchKernelGGL(k_score_pick, dim3((S.dense_n + 255) / 256), dim3(256), 0, s, S);
+            }
         }
         hipLaunchKernelGGL(k_score_dense, dim3((S.dense_n + 127) / 128), dim3(128), 0, s, S, S.dense_n);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (S.n_cx > 0) hipLaunchKernelGGL(k_score_cx, dim3((S.n_cx + 63) / 64), dim3(64), 0, s, S);
+    return hipGetLastError();
+}
+
+hipError_t launch_score_prepick(const ScoreIn& S, hipStream_t s) {
+    if (S.dense_n <= 0) return hipSuccess;
+    if (!S.pick_list || !S.pick_count) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s, S.pick_count, 1);
+    hipLaunchKernelGGL(k_score_prepick, dim3((S.dense_n + 255) / 256), dim3(256), 0, s, S);
     return hipGetLastError();
 }
 

@@ -528,8 +528,8 @@ __device__ __forceinline__ bool score_needed(const ScoreIn& S, int p, bool dense
 
 // score_needed for a dense-kernel position, from the row's flags alone (no allele view): what the
 // pick pass evaluates for every dense position.
-__device__ __forceinline__ bool dense_needed(const ScoreIn& S, int p) {
-    if (cx_find(S, p) >= 0) return false;  // scored by the record-list kernel
+// dense_needed's checks of the row itself: they read only the pileup's rows and the reference
+__device__ __forceinline__ bool dense_row_needed(const ScoreIn& S, int p) {
     const int i = p - S.dense_lo;
     if (i < 0 || i >= S.dense_n) return false;
     const DenseRow& r = S.rows[i];
@@ -544,6 +544,11 @@ __device__ __forceinline__ bool dense_needed(const ScoreIn& S, int p) {
     const RefWindow rf{S.ref, S.ref_lo, S.ref_hi};
     if (n == 1 && rf.has(p) && "ACGT"[only] == rf.at(p)) return S.cfg.doPileup || S.cfg.hasBam2;
     return true;
+}
+
+__device__ __forceinline__ bool dense_needed(const ScoreIn& S, int p) {
+    if (cx_find(S, p) >= 0) return false;  // scored by the record-list kernel
+    return dense_row_needed(S, p);
 }
 
 template <int CAP, int NA>
