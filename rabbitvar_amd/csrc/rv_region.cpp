@@ -20,9 +20,17 @@ namespace rv {
 
 namespace {
 
+// The scored records as score_pass downloaded them into this region's pinned host memory.
+struct RecView {
+    const OutRec* p = nullptr;
+    size_t n = 0;
+    size_t size() const { return n; }
+    const OutRec& operator[](size_t i) const { return p[i]; }
+};
+
 // slot[i] = index of recs[i].position in `order` (distinct keys); left as it is when absent.  A
 // bitmap of the records' positions filters the scan, which is split over threads.
-void slot_ranks(const std::vector<int32_t>& order, const std::vector<OutRec>& recs, std::vector<int64_t>& slot,
+void slot_ranks(const std::vector<int32_t>& order, const RecView& recs, std::vector<int64_t>& slot,
                 unsigned threads) {
     const size_t n = recs.size();
     if (n == 0 || order.empty()) return;
@@ -357,10 +365,14 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     in.region_start = region.start;
     in.region_end = region.end;
     in.cfg = e.cfg;
+    // through pinned staging: from pageable memory each copy would be a synchronous, staged one
     auto up = [&](DevBuf& b, const void* src, size_t bytes) -> void* {
         void* d = e.grow(b, bytes);
         if (d && bytes) {
-            if (hipMemcpyAsync(d, src, bytes, hipMemcpyHostToDevice, e.stream) != hipSuccess) return nullptr;
+            void* h = e.pinned_alloc(bytes);
+            if (!h) return nullptr;
+            std::memcpy(h, src, bytes);
+            if (hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, e.stream) != hipSuccess) return nullptr;
         }
         return d;
     };
@@ -375,14 +387,20 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
     const size_t n_status = (size_t)pr.P.dense_n + (size_t)ncx;
     in.status = (uint8_t*)e.grow(e.b_status, n_status);
     if (!in.status) return e.fail(RV_ERR_OOM, "device buffer for position status failed");
-    std::vector<OutRec> recs;
-    std::vector<char> arena;
-    std::vector<uint8_t> status;
     unsigned long long hc[8];
     float score_ms = 0;
     e.stats.timeline_ms[3] = ms_since(w0);
-    s = score_pass(e, in, n_status, recs, arena, nullptr, hc, score_ms);  // the status stays on the device
+    // the records and their strings come back into pinned host memory (a copy into pageable
+    // memory would be staged); the status stays on the device
+    RecSpan rspan;
+    {
+        std::vector<OutRec> unused_recs;
+        std::vector<char> unused_arena;
+        s = score_pass(e, in, n_status, unused_recs, unused_arena, nullptr, hc, score_ms, &rspan);
+    }
     if (s != RV_OK) return s;
+    const RecView recs{rspan.p, rspan.n};
+    const char* const arena_p = rspan.arena;
     e.stats.timeline_ms[4] = ms_since(w0);
     // ---- text, in the printer's order: alignedVariants slot order, then the variants' order.  A
     // second thread works out that order while this one formats the lines in record order; the
@@ -501,8 +519,8 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
         for (size_t i = a; i < b; i++) {
             const OutRec& o = recs[i];
             line_at[i] = ((uint64_t)t << 40) | (uint64_t)txt.size();
-            format_line(txt, o, arena.data(), ctx);
-            const char* p = arena.data() + o.str_off;
+            format_line(txt, o, arena_p, ctx);
+            const char* p = arena_p + o.str_off;
             for (int k = 0; k < 6; k++) {
                 str_at[i][k] = (uint32_t)sa.size();
                 sa.append(p, o.len[k]);
@@ -510,7 +528,7 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
                 p += o.len[k];
             }
             str_at[i][6] = (uint32_t)sa.size();
-            sa += vartype_name(o, arena.data());
+            sa += vartype_name(o, arena_p);
             sa += '\0';
         }
     };
@@ -635,9 +653,8 @@ rv_status run_region(Engine& e, const rv_region& region, const rv_reads& reads, 
                     std::move(pr.P.sc5), std::move(pr.P.sc3), std::move(pr.P.delCount), std::move(pr.P.insCount),
                     std::move(pr.P.mnp), std::move(pr.P.rows));
     e.release_later(std::move(S), std::move(rcp), std::move(pr), std::move(cx), std::move(created),
-                    std::move(walk_sorted), std::move(nonins), std::move(recs), std::move(arena), std::move(status),
-                    std::move(proc), std::move(aligned), std::move(parts), std::move(line_at), std::move(str_at),
-                    std::move(perm));
+                    std::move(walk_sorted), std::move(nonins), std::move(proc), std::move(aligned), std::move(parts),
+                    std::move(line_at), std::move(str_at), std::move(perm));
     e.stats.timeline_ms[7] = ms_since(w0);
     return RV_OK;
 }
