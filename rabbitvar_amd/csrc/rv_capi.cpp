@@ -176,7 +176,10 @@ rv_status rv_debug_pileup(rv_engine* e, const rv_region* region, const rv_reads*
                           char** text, size_t* len) {
     if (!e || !region || !reads || !contig || !text) return RV_ERR_ARG;
     rv::PileupResult pr;
-    rv_status s = guarded(e, [&] { return e->e->pileup(*region, *reads, *contig, true, pr); });
+    rv_status s = guarded(e, [&] {
+        e->e->pinned_reset();  // the pileup's host copies live in the engine's pinned memory
+        return e->e->pileup(*region, *reads, *contig, true, pr);
+    });
     if (s != RV_OK) return s;
     std::string d;
     s = guarded(e, [&] {

@@ -341,14 +341,17 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
         if (!over && hc.overflow == 0) {
             if (hooks && hooks->after_walk) hooks->after_walk();
             // ---- copy the ordered records back and reduce them
-            std::vector<Event> evs(counts[0]);
-            std::vector<char> arena((size_t)aused);
-            std::vector<ClipRec> cls(counts[1]);
-            std::vector<char> carena((size_t)cused);
-            if (counts[0]) RV_HIP(hipMemcpyAsync(evs.data(), S.events, sizeof(Event) * counts[0], hipMemcpyDeviceToHost, stream));
-            if (aused) RV_HIP(hipMemcpyAsync(arena.data(), S.arena, (size_t)aused, hipMemcpyDeviceToHost, stream));
-            if (counts[1]) RV_HIP(hipMemcpyAsync(cls.data(), S.clips, sizeof(ClipRec) * counts[1], hipMemcpyDeviceToHost, stream));
-            if (cused) RV_HIP(hipMemcpyAsync(carena.data(), S.clip_arena, (size_t)cused, hipMemcpyDeviceToHost, stream));
+            // into this region's pinned host memory: copies into pageable memory are staged and
+            // synchronous
+            Event* evs = (Event*)pinned_alloc(sizeof(Event) * std::max(counts[0], 1u));
+            char* arena = (char*)pinned_alloc((size_t)aused + 1);
+            ClipRec* cls = (ClipRec*)pinned_alloc(sizeof(ClipRec) * std::max(counts[1], 1u));
+            char* carena = (char*)pinned_alloc((size_t)cused + 1);
+            if (!evs || !arena || !cls || !carena) return fail(RV_ERR_OOM, "pinned host memory for the walk's records failed");
+            if (counts[0]) RV_HIP(hipMemcpyAsync(evs, S.events, sizeof(Event) * counts[0], hipMemcpyDeviceToHost, stream));
+            if (aused) RV_HIP(hipMemcpyAsync(arena, S.arena, (size_t)aused, hipMemcpyDeviceToHost, stream));
+            if (counts[1]) RV_HIP(hipMemcpyAsync(cls, S.clips, sizeof(ClipRec) * counts[1], hipMemcpyDeviceToHost, stream));
+            if (cused) RV_HIP(hipMemcpyAsync(carena, S.clip_arena, (size_t)cused, hipMemcpyDeviceToHost, stream));
             if (download_rows) {
                 out.P.rows.resize((size_t)W.dense_n);
                 RV_HIP(hipMemcpyAsync(out.P.rows.data(), rows, sizeof(DenseRow) * (size_t)W.dense_n, hipMemcpyDeviceToHost,
@@ -371,7 +374,7 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
             auto clips = [&](int k) {
                 try {
                     const auto c0 = std::chrono::steady_clock::now();
-                    reduce_clips(cls.data(), cls.size(), carena.data(), cfg.goodq, out.P, k == 0 ? 5 : 3);
+                    reduce_clips(cls, counts[1], carena, cfg.goodq, out.P, k == 0 ? 5 : 3);
                     clip_ms[k] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0).count();
                 } catch (...) {
                     clip_err[k] = std::current_exception();
@@ -379,7 +382,7 @@ rv_status Engine::pileup(const rv_region& region, const rv_reads& rd, const rv_c
             };
             std::thread c5(clips, 0), c3(clips, 1);
             try {
-                reduce_events(evs.data(), evs.size(), arena.data(), cfg.goodq, out.P);
+                reduce_events(evs, counts[0], arena, cfg.goodq, out.P);
             } catch (...) {
                 c5.join();
                 c3.join();
