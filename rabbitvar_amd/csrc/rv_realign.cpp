@@ -115,17 +115,27 @@ char HostRef::get(int k) {
 }
 
 // ------------------------------------------------------------------ position store
+namespace {
+std::vector<RealignCand> sorted_candidates(const std::map<int, std::map<std::string, int>>& m);
+}
+
 RState::RState(HostPileup& P_, RowCache& rc) : P(P_), rows(rc) {
-    // the three position indexes are independent: the clip maps' on their own threads
+    // the three position indexes are independent: the clip maps' on their own threads, and
+    // adjustMNP's candidate order (it reads only the MNV map) on a third
     std::thread t5([this] { sc5_ix.build(P.sc5, 5); });
     std::thread t3([this] { sc3_ix.build(P.sc3, 5); });
+    std::thread tm([this] {
+        mnp_sorted = sorted_candidates(P.mnp);
+        mnp_sorted_ready = true;  // seen after the join below
+    });
     struct Join {
-        std::thread &a, &b;
+        std::thread &a, &b, &c;
         ~Join() {
             a.join();
             b.join();
+            c.join();
         }
-    } join{t5, t3};
+    } join{t5, t3, tm};
     nd_ix.build(P.nd, 5);
     for (auto& e : P.ins)
         for (auto& kv : e.second) {
@@ -537,11 +547,7 @@ void find35match(const std::string& seq5, const std::string& seq3, int& b5, int&
 
 namespace {
 
-struct Cand {
-    int position;
-    std::string desc;
-    int count;
-};
+using Cand = RealignCand;
 // CMP_tmp: count descending, position ascending, descriptor descending
 std::vector<Cand> sorted_candidates(const std::map<int, std::map<std::string, int>>& m) {
     std::vector<Cand> t;
@@ -865,7 +871,8 @@ struct Realigner {
     }
 
     void adjustMNP() {
-        std::vector<Cand> tmp = sorted_candidates(S.P.mnp);
+        std::vector<Cand> tmp = S.mnp_sorted_ready ? std::move(S.mnp_sorted) : sorted_candidates(S.P.mnp);
+        S.mnp_sorted_ready = false;
         st.candidates += (int64_t)tmp.size();
         MnpPre pre;
         mnp_pre(tmp, pre);

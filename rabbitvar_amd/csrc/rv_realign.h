@@ -176,6 +176,13 @@ class PosIndex {
     size_t mask_ = 0;
 };
 
+// A realignment candidate: position, descriptor and its count in the candidate map.
+struct RealignCand {
+    int position;
+    std::string desc;
+    int count;
+};
+
 struct RPos {
     bool loaded = false;
     bool key = false;   // nonInsertionVariants has the position
@@ -217,6 +224,10 @@ class RState {
     std::unordered_map<int, RPos> pos;
     std::map<int, std::map<std::string, HVar*>> ins;
     std::vector<KeyOp> key_log;  // nonInsertionVariants key insertions / erasures, in order
+    // adjustMNP's candidates in its order, sorted while the position indexes are built
+    // (mnp_sorted_ready until adjustMNP takes them)
+    std::vector<RealignCand> mnp_sorted;
+    bool mnp_sorted_ready = false;
 
     // Change tracking for the screen's labels: between track_begin() and track_end() every
     // position read through at() or track_ins() is snapshotted once; track_end() reports whether
